@@ -1,0 +1,735 @@
+/*
+ * FP64 CPU restatement of the MGBF localization operator: the oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests, the smoke check and the CPU-baseline
+ * leg of the benchmark load this code, as the checker; the product library
+ * never links or calls it.
+ *
+ * Every routine restates one reference routine (paths relative to the
+ * reference root, all in src.mgbf/), in the reference's own formulation on a
+ * single global domain:
+ *   - cholaspect1/2, getlinesum1/2 ............ jp_pbfil.f90:74-107, 149-218
+ *   - dinvmtf / dl1lmf (2x2 inverse, Cholesky) . jp_pmat.f90:147-183, 867-897
+ *   - vrbeta1 / vrbeta1T (1-D gather/scatter) .. jp_pbfil.f90:856-885, 981-1009
+ *   - vrbeta2 / vrbeta2T (2-D gather/scatter) .. jp_pbfil.f90:888-928, 1011-1050
+ *   - boco_2d (halo fill, half-sample mirror) .. mg_bocos.f90:225-262, 340-377
+ *   - bocoT_2d (adjoint fold) .................. mg_bocos.f90:929-957, 1033-1061
+ *   - adjoint / direct1 (R^T-side, P) .......... mg_generations.f90:1117-1187, 1190-1250
+ *   - def_offset_coef constants ................ mg_interpolate.f90:61-86
+ *   - upsending_ens / downsending_ens .......... mg_generations.f90:382-494
+ *   - weighting_ens ............................ mg_generations.f90:1012-1052
+ *   - sup_vrbeta1_ens / sup_vrbeta1T_ens ....... mg_filtering.f90:2042-2151
+ *   - filtering_rad2_ens composition ........... mg_filtering.f90:1000-1147
+ * On one domain the reference's MPI halo exchanges and up/down re-tiling
+ * (mg_bocos.f90 upsend_all/downsend_all) reduce to the global mirror
+ * boundary at the domain edges and to plain addressing.
+ */
+#include "mgbf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EPS 1.e-12
+
+/* interpolation constants, mg_interpolate.f90:76-79 */
+static const double PCOEF[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
+static const double QCOEF[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
+
+int mgbfo_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static double ipow(double x, int p) {
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= x;
+    return r;
+}
+
+/* ------------------------------------------------------------------------
+ * Padded field: i in [1-hx, n+hx], j in [1-hy, m+hy] (1-based like the
+ * reference), k fastest.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    double* d;
+    int K, n, m, hx, hy;
+} pf_t;
+
+static inline size_t pf_off(const pf_t* f, int k, int i, int j) {
+    return (size_t)k + (size_t)f->K * ((size_t)(i + f->hx - 1) +
+                                       (size_t)(f->n + 2 * f->hx) * (size_t)(j + f->hy - 1));
+}
+#define PF(f, k, i, j) ((f)->d[pf_off((f), (k), (i), (j))])
+#define PCOL(f, i, j) (&(f)->d[pf_off((f), 0, (i), (j))])
+
+static pf_t pf_alloc(int K, int n, int m, int hx, int hy) {
+    pf_t f;
+    f.K = K; f.n = n; f.m = m; f.hx = hx; f.hy = hy;
+    f.d = (double*)calloc((size_t)K * (n + 2 * hx) * (size_t)(m + 2 * hy), sizeof(double));
+    return f;
+}
+static void pf_free(pf_t* f) { free(f->d); f->d = NULL; }
+static void pf_zero(pf_t* f) {
+    memset(f->d, 0, sizeof(double) * (size_t)f->K * (f->n + 2 * f->hx) * (size_t)(f->m + 2 * f->hy));
+}
+static void pf_from_interior(pf_t* f, const double* in) {
+    const int K = f->K, n = f->n, m = f->m;
+    for (int j = 1; j <= m; ++j)
+        for (int i = 1; i <= n; ++i)
+            memcpy(PCOL(f, i, j), in + (size_t)K * ((size_t)(i - 1) + (size_t)n * (j - 1)),
+                   sizeof(double) * K);
+}
+static void pf_to_interior(const pf_t* f, double* out) {
+    const int K = f->K, n = f->n, m = f->m;
+    for (int j = 1; j <= m; ++j)
+        for (int i = 1; i <= n; ++i)
+            memcpy(out + (size_t)K * ((size_t)(i - 1) + (size_t)n * (j - 1)), PCOL(f, i, j),
+                   sizeof(double) * K);
+}
+
+/* ------------------------------------------------------------------------
+ * Coefficients
+ * ---------------------------------------------------------------------- */
+
+/* 2x2 symmetric aspect tensor -> Cholesky factor of its inverse (cholaspect2:
+   inv then l1lm, jp_pbfil.f90:90-107). el is column-major el[r + 2c]. */
+static void cholaspect2(const double a[4], double el[4]) {
+    /* inverse of a 2x2 matrix (dinvmtf reduces to the adjugate formula) */
+    double det = a[0] * a[3] - a[2] * a[1];
+    double inv[4];
+    inv[0] = a[3] / det;  inv[3] = a[0] / det;
+    inv[2] = -a[2] / det; inv[1] = -a[1] / det;
+    /* lower Cholesky factor, column by column (dl1lmf, jp_pmat.f90:867-897) */
+    double b00 = sqrt(inv[0]);
+    double b10 = inv[1] / b00;
+    double b11 = sqrt(inv[3] - b10 * b10);
+    el[0] = b00; el[1] = b10; el[2] = 0.0; el[3] = b11;
+}
+
+static void aspect2(const mgbfo_params* P, double a[4]) {
+    /* mg_intstate.f90:383-390 */
+    a[0] = P->ampl02 * (1. + P->p_del);
+    a[3] = P->ampl02 * (1. - P->p_del);
+    a[2] = P->ampl02 * P->p_eps;
+    a[1] = P->ampl02 * P->p_eps;
+}
+
+/* Visit the taps of the 2-D filter in the reference's loop order
+   (vrbeta2, jp_pbfil.f90:909-926). Returns count, or -1 if reach > halo. */
+static int taps2_raw(const mgbfo_params* P, int maxt, int* gxo, int* gyo, double* rrco) {
+    const double rmom2_2 = 1.0 / sqrt((double)(2 * P->p) + 4.0);
+    double a[4], el[4];
+    aspect2(P, a);
+    cholaspect2(a, el);
+    const double exx = el[0] * rmom2_2, eyy = el[3] * rmom2_2, eyx = el[1] * rmom2_2;
+    double y = 1.0 / eyy;
+    int gyl = (int)ceil(-y + EPS), gym = (int)floor(y - EPS);
+    if (gyl < -P->hy || gym > P->hy) return -1;
+    int nt = 0;
+    for (int gy = gyl; gy <= gym; ++gy) {
+        double yy = gy;
+        double xc = -yy * eyx;
+        double rrx = (yy * eyy) * (yy * eyy);
+        double x = sqrt(1.0 - rrx);
+        int gxl = (int)ceil((xc - x) / exx + EPS), gxm = (int)floor((xc + x) / exx - EPS);
+        if (gxl < -P->hx || gxm > P->hx) return -1;
+        for (int gx = gxl; gx <= gxm; ++gx) {
+            double xx = gx;
+            double t = xx * exx - xc;
+            double rr = rrx + t * t;
+            if (nt < maxt) {
+                gxo[nt] = gx;
+                gyo[nt] = gy;
+                rrco[nt] = 1.0 - rr;
+            }
+            ++nt;
+        }
+    }
+    return nt;
+}
+
+double mgbfo_ss2(const mgbfo_params* P) {
+    /* getlinesum2, jp_pbfil.f90:182-218 */
+    int cap = (2 * P->hx + 1) * (2 * P->hy + 1);
+    int* gx = malloc(sizeof(int) * cap);
+    int* gy = malloc(sizeof(int) * cap);
+    double* rrc = malloc(sizeof(double) * cap);
+    int nt = taps2_raw(P, cap, gx, gy, rrc);
+    double s = 0;
+    for (int t = 0; t < nt; ++t) s += ipow(rrc[t], P->p);
+    free(gx); free(gy); free(rrc);
+    return nt > 0 ? 1.0 / s : 0.0;
+}
+
+int mgbfo_taps2(const mgbfo_params* P, int maxt, int* gx, int* gy, double* w) {
+    int cap = (2 * P->hx + 1) * (2 * P->hy + 1);
+    int* tgx = malloc(sizeof(int) * cap);
+    int* tgy = malloc(sizeof(int) * cap);
+    double* rrc = malloc(sizeof(double) * cap);
+    int nt = taps2_raw(P, cap, tgx, tgy, rrc);
+    if (nt > 0) {
+        double ss = mgbfo_ss2(P);
+        for (int t = 0; t < nt && t < maxt; ++t) {
+            gx[t] = tgx[t];
+            gy[t] = tgy[t];
+            w[t] = ss * ipow(rrc[t], P->p);
+        }
+    }
+    free(tgx); free(tgy); free(rrc);
+    return nt;
+}
+
+static int taps1_raw(const mgbfo_params* P, int maxt, int* gxo, double* rrco) {
+    /* cholaspect1 + getlinesum1 / vrbeta1 loop, jp_pbfil.f90:74-88, 149-180, 872-884 */
+    const double rmom2_1 = 1.0 / sqrt((double)(2 * P->p) + 3.0);
+    double el = 1.0 / sqrt(P->ampl01);
+    double exx = el * rmom2_1;
+    double x = 1.0 / exx;
+    int gxl = (int)ceil(-x + EPS), gxm = (int)floor(x - EPS);
+    if (gxl < -P->hz || gxm > P->hz) return -1;
+    int nt = 0;
+    for (int gx = gxl; gx <= gxm; ++gx) {
+        double xx = gx;
+        double rr = (xx * exx) * (xx * exx);
+        if (nt < maxt) {
+            gxo[nt] = gx;
+            rrco[nt] = 1.0 - rr;
+        }
+        ++nt;
+    }
+    return nt;
+}
+
+double mgbfo_ss1(const mgbfo_params* P) {
+    int cap = 2 * P->hz + 1;
+    int* gx = malloc(sizeof(int) * cap);
+    double* rrc = malloc(sizeof(double) * cap);
+    int nt = taps1_raw(P, cap, gx, rrc);
+    double s = 0;
+    for (int t = 0; t < nt; ++t) s += ipow(rrc[t], P->p);
+    free(gx); free(rrc);
+    return nt > 0 ? 1.0 / s : 0.0;
+}
+
+int mgbfo_taps1(const mgbfo_params* P, int maxt, int* gx, double* w) {
+    int cap = 2 * P->hz + 1;
+    int* tgx = malloc(sizeof(int) * cap);
+    double* rrc = malloc(sizeof(double) * cap);
+    int nt = taps1_raw(P, cap, tgx, rrc);
+    if (nt > 0) {
+        double ss = mgbfo_ss1(P);
+        for (int t = 0; t < nt && t < maxt; ++t) {
+            gx[t] = tgx[t];
+            w[t] = ss * ipow(rrc[t], P->p);
+        }
+    }
+    free(tgx); free(rrc);
+    return nt;
+}
+
+/* ------------------------------------------------------------------------
+ * Halo fill and fold (boco_2d / bocoT_2d on the whole domain)
+ * ---------------------------------------------------------------------- */
+
+/* boco_2d: rows (N/S) over interior i first, then columns (E/W) over the
+   extended rows (mg_bocos.f90:225-262, 340-377). */
+static void boco_2d(pf_t* W, int nbx, int nby) {
+    const int K = W->K, n = W->n, m = W->m;
+    for (int j = 1; j <= nby; ++j)
+        for (int i = 1; i <= n; ++i)
+            memcpy(PCOL(W, i, -nby + j), PCOL(W, i, nby + 1 - j), sizeof(double) * K);
+    for (int j = 1; j <= nby; ++j)
+        for (int i = 1; i <= n; ++i)
+            memcpy(PCOL(W, i, m + j), PCOL(W, i, m + 1 - j), sizeof(double) * K);
+    for (int j = 1 - nby; j <= m + nby; ++j)
+        for (int i = 1; i <= nbx; ++i)
+            memcpy(PCOL(W, -nbx + i, j), PCOL(W, nbx + 1 - i, j), sizeof(double) * K);
+    for (int j = 1 - nby; j <= m + nby; ++j)
+        for (int i = 1; i <= nbx; ++i)
+            memcpy(PCOL(W, n + i, j), PCOL(W, n + 1 - i, j), sizeof(double) * K);
+}
+
+static inline void vadd(double* dst, const double* src, int K) {
+    for (int k = 0; k < K; ++k) dst[k] += src[k];
+}
+
+/* bocoT_2d: E/W fold over the extended rows first, then N/S over interior i
+   (mg_bocos.f90:929-957, 1033-1061). */
+static void bocoT_2d(pf_t* W, int nbx, int nby) {
+    const int K = W->K, n = W->n, m = W->m;
+    for (int j = 1 - nby; j <= m + nby; ++j)
+        for (int i = 1; i <= nbx; ++i) vadd(PCOL(W, i, j), PCOL(W, 1 - i, j), K);
+    for (int j = 1 - nby; j <= m + nby; ++j)
+        for (int i = 1; i <= nbx; ++i) vadd(PCOL(W, n - nbx + i, j), PCOL(W, n + 1 + nbx - i, j), K);
+    for (int j = 1; j <= nby; ++j)
+        for (int i = 1; i <= n; ++i) vadd(PCOL(W, i, j), PCOL(W, i, 1 - j), K);
+    for (int j = 1; j <= nby; ++j)
+        for (int i = 1; i <= n; ++i) vadd(PCOL(W, i, m - nby + j), PCOL(W, i, m + 1 + nby - j), K);
+}
+
+/* ------------------------------------------------------------------------
+ * Radial beta filters
+ * ---------------------------------------------------------------------- */
+
+typedef struct {
+    int nt;
+    int* gx;
+    int* gy;
+    double* w;
+} taps_t;
+
+static taps_t taps2_make(const mgbfo_params* P) {
+    taps_t T;
+    int cap = (2 * P->hx + 1) * (2 * P->hy + 1);
+    T.gx = malloc(sizeof(int) * cap);
+    T.gy = malloc(sizeof(int) * cap);
+    T.w = malloc(sizeof(double) * cap);
+    T.nt = mgbfo_taps2(P, cap, T.gx, T.gy, T.w);
+    if (T.nt < 0) abort(); /* the reference stops: "filter reach becomes too large" */
+    return T;
+}
+static taps_t taps1_make(const mgbfo_params* P) {
+    taps_t T;
+    int cap = 2 * P->hz + 1;
+    T.gx = malloc(sizeof(int) * cap);
+    T.gy = NULL;
+    T.w = malloc(sizeof(double) * cap);
+    T.nt = mgbfo_taps1(P, cap, T.gx, T.w);
+    if (T.nt < 0) abort();
+    return T;
+}
+static void taps_free(taps_t* T) { free(T->gx); free(T->gy); free(T->w); }
+
+/* vrbeta2: gather over the disc from the padded input; interior output, halo
+   of the result is zero (b=0; a=b). jp_pbfil.f90:888-928 */
+static void vrbeta2(const taps_t* T, pf_t* a) {
+    const int K = a->K, n = a->n, m = a->m;
+    pf_t b = pf_alloc(K, n, m, a->hx, a->hy);
+#pragma omp parallel for schedule(static)
+    for (int iy = 1; iy <= m; ++iy) {
+        double* tb = malloc(sizeof(double) * K);
+        for (int ix = 1; ix <= n; ++ix) {
+            memset(tb, 0, sizeof(double) * K);
+            for (int t = 0; t < T->nt; ++t) {
+                const double frow = T->w[t];
+                const double* src = PCOL(a, ix + T->gx[t], iy + T->gy[t]);
+                for (int k = 0; k < K; ++k) tb[k] += frow * src[k];
+            }
+            memcpy(PCOL(&b, ix, iy), tb, sizeof(double) * K);
+        }
+        free(tb);
+    }
+    free(a->d);
+    a->d = b.d;
+}
+
+/* vrbeta2T: scatter from each interior point into the padded result (halo
+   included). jp_pbfil.f90:1011-1050. Parallel over k: each k is independent. */
+static void vrbeta2T(const taps_t* T, pf_t* a) {
+    const int K = a->K, n = a->n, m = a->m;
+    pf_t b = pf_alloc(K, n, m, a->hx, a->hy);
+#pragma omp parallel
+    {
+        int nth = 1, tid = 0;
+#ifdef _OPENMP
+        nth = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        int k0 = (int)((long)K * tid / nth), k1 = (int)((long)K * (tid + 1) / nth);
+        for (int iy = 1; iy <= m; ++iy)
+            for (int ix = 1; ix <= n; ++ix) {
+                const double* ta = PCOL(a, ix, iy);
+                for (int t = 0; t < T->nt; ++t) {
+                    const double frow = T->w[t];
+                    double* dst = PCOL(&b, ix + T->gx[t], iy + T->gy[t]);
+                    for (int k = k0; k < k1; ++k) dst[k] += frow * ta[k];
+                }
+            }
+    }
+    free(a->d);
+    a->d = b.d;
+}
+
+/* ------------------------------------------------------------------------
+ * Inter-generation transfer: adjoint (P^T) and direct1 (P)
+ * Coarse arrays have a 2-point halo: indices -1 .. n/2+2.
+ * ---------------------------------------------------------------------- */
+
+/* adjoint: F(K,1:n,1:m) -> W(K,-1:nL+2,-1:mL+2), mg_generations.f90:1117-1187.
+   Scatter; parallel over k ranges. */
+static void adjoint_op(const pf_t* F, pf_t* W) {
+    const int K = F->K, n = F->n, m = F->m;
+    const int nL = n / 2, mL = m / 2;
+    /* W_AUX(K, 1:n, -1:mL+2) */
+    pf_t A = pf_alloc(K, n, mL, 0, 2);
+    pf_zero(W);
+#pragma omp parallel
+    {
+        int nth = 1, tid = 0;
+#ifdef _OPENMP
+        nth = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        int k0 = (int)((long)K * tid / nth), k1 = (int)((long)K * (tid + 1) / nth);
+        for (int j = m; j >= 2; j -= 2) {
+            int jL = j / 2;
+            for (int i = n; i >= 1; --i) {
+                const double* f = PCOL(F, i, j);
+                for (int t = 3; t >= 0; --t) {
+                    double* dst = PCOL(&A, i, jL - 1 + t);
+                    for (int k = k0; k < k1; ++k) dst[k] += PCOEF[t] * f[k];
+                }
+            }
+        }
+        for (int j = m - 1; j >= 1; j -= 2) {
+            int jL = j / 2;
+            for (int i = n; i >= 1; --i) {
+                const double* f = PCOL(F, i, j);
+                for (int t = 3; t >= 0; --t) {
+                    double* dst = PCOL(&A, i, jL - 1 + t);
+                    for (int k = k0; k < k1; ++k) dst[k] += QCOEF[t] * f[k];
+                }
+            }
+        }
+        for (int jL = mL + 2; jL >= -1; --jL) {
+            for (int i = n - 1; i >= 1; i -= 2) {
+                int iL = i / 2;
+                const double* a = PCOL(&A, i, jL);
+                for (int t = 3; t >= 0; --t) {
+                    double* dst = PCOL(W, iL - 1 + t, jL);
+                    for (int k = k0; k < k1; ++k) dst[k] += QCOEF[t] * a[k];
+                }
+            }
+            for (int i = n; i >= 2; i -= 2) {
+                int iL = i / 2;
+                const double* a = PCOL(&A, i, jL);
+                for (int t = 3; t >= 0; --t) {
+                    double* dst = PCOL(W, iL - 1 + t, jL);
+                    for (int k = k0; k < k1; ++k) dst[k] += PCOEF[t] * a[k];
+                }
+            }
+        }
+    }
+    (void)nL;
+    pf_free(&A);
+}
+
+/* direct1: W(K,-1:nL+2,-1:mL+2) -> F(K,1:n,1:m), mg_generations.f90:1190-1250.
+   Gather; F here is an interior-only output written into a padded field. */
+static void direct1_op(const pf_t* W, pf_t* F) {
+    const int K = F->K, n = F->n, m = F->m;
+    const int mL = m / 2;
+    pf_t A = pf_alloc(K, n, mL, 0, 2); /* W_AUX(K,1:n,-1:mL+2) */
+#pragma omp parallel for schedule(static)
+    for (int jL = -1; jL <= mL + 2; ++jL) {
+        for (int i = 1; i <= n; ++i) {
+            int iL = i / 2;
+            const double* c = (i & 1) ? QCOEF : PCOEF;
+            double* dst = PCOL(&A, i, jL);
+            const double* w0 = PCOL(W, iL - 1, jL);
+            const double* w1 = PCOL(W, iL, jL);
+            const double* w2 = PCOL(W, iL + 1, jL);
+            const double* w3 = PCOL(W, iL + 2, jL);
+            for (int k = 0; k < K; ++k) dst[k] = c[0] * w0[k] + c[1] * w1[k] + c[2] * w2[k] + c[3] * w3[k];
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= m; ++j) {
+        int jL = j / 2;
+        const double* c = (j & 1) ? QCOEF : PCOEF;
+        for (int i = 1; i <= n; ++i) {
+            double* dst = PCOL(F, i, j);
+            const double* a0 = PCOL(&A, i, jL - 1);
+            const double* a1 = PCOL(&A, i, jL);
+            const double* a2 = PCOL(&A, i, jL + 1);
+            const double* a3 = PCOL(&A, i, jL + 2);
+            for (int k = 0; k < K; ++k) dst[k] = c[0] * a0[k] + c[1] * a1[k] + c[2] * a2[k] + c[3] * a3[k];
+        }
+    }
+    pf_free(&A);
+}
+
+/* ------------------------------------------------------------------------
+ * Vertical filter (sup_vrbeta1_ens / sup_vrbeta1T_ens)
+ * Column (i,j): W(kc, L) = V((kc-1)*lm + L, i, j) for kc=1..nvcol, L=1..lm.
+ * ---------------------------------------------------------------------- */
+static void vertical_op(const mgbfo_params* P, pf_t* V, int adjoint) {
+    const int lm = P->lm, hz = P->hz, nv = P->nvcol;
+    const int n = V->n, m = V->m;
+    taps_t T = taps1_make(P);
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= m; ++j) {
+        double* Wc = malloc(sizeof(double) * (lm + 2 * hz));
+        double* Bc = malloc(sizeof(double) * (lm + 2 * hz));
+#define WZ(L) Wc[(L) + hz - 1]
+#define BZ(L) Bc[(L) + hz - 1]
+        for (int i = 1; i <= n; ++i) {
+            double* col = PCOL(V, i, j);
+            for (int kc = 0; kc < nv; ++kc) {
+                double* c = col + (size_t)kc * lm;
+                for (int L = 1; L <= lm; ++L) WZ(L) = c[L - 1];
+                for (int L = 1; L <= hz; ++L) {
+                    WZ(1 - L) = WZ(1 + L);   /* whole-sample mirror, mg_filtering.f90:2072-2075 */
+                    WZ(lm + L) = WZ(lm - L);
+                }
+                for (int L = 1 - hz; L <= lm + hz; ++L) BZ(L) = 0.0;
+                if (!adjoint) {
+                    /* vrbeta1: gather into the interior */
+                    for (int ix = 1; ix <= lm; ++ix) {
+                        double tb = 0;
+                        for (int t = 0; t < T.nt; ++t) tb += T.w[t] * WZ(ix + T.gx[t]);
+                        BZ(ix) = tb;
+                    }
+                } else {
+                    /* vrbeta1T: scatter from the interior, then fold the halo
+                       (mg_filtering.f90:2134-2137) */
+                    for (int ix = 1; ix <= lm; ++ix) {
+                        double ta = WZ(ix);
+                        for (int t = 0; t < T.nt; ++t) BZ(ix + T.gx[t]) += T.w[t] * ta;
+                    }
+                    for (int L = 1; L <= hz; ++L) {
+                        BZ(1 + L) += BZ(1 - L);
+                        BZ(lm - L) += BZ(lm + L);
+                    }
+                }
+                for (int L = 1; L <= lm; ++L) c[L - 1] = BZ(L);
+            }
+        }
+#undef WZ
+#undef BZ
+        free(Wc);
+        free(Bc);
+    }
+    taps_free(&T);
+}
+
+/* ------------------------------------------------------------------------
+ * Single-stage wrappers on interior arrays
+ * ---------------------------------------------------------------------- */
+
+void mgbfo_radial(const mgbfo_params* P, int K, int n, int m, const double* in, double* out,
+                  int adjoint) {
+    taps_t T = taps2_make(P);
+    pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
+    pf_from_interior(&a, in);
+    if (!adjoint) {
+        boco_2d(&a, P->hx, P->hy);
+        vrbeta2(&T, &a);
+    } else {
+        vrbeta2T(&T, &a);
+        bocoT_2d(&a, P->hx, P->hy);
+    }
+    pf_to_interior(&a, out);
+    pf_free(&a);
+    taps_free(&T);
+}
+
+void mgbfo_vertical(const mgbfo_params* P, int n, int m, const double* in, double* out,
+                    int adjoint) {
+    pf_t a = pf_alloc(P->K, n, m, 0, 0);
+    pf_from_interior(&a, in);
+    vertical_op(P, &a, adjoint);
+    pf_to_interior(&a, out);
+    pf_free(&a);
+}
+
+/* R = bocoT_2d(.,2,2) o adjoint (upsending_ens, mg_generations.f90:404-426) */
+static void restrict_pf(const pf_t* fine, pf_t* coarse_int /* (K,n/2,m/2,2,2) */) {
+    adjoint_op(fine, coarse_int);
+    bocoT_2d(coarse_int, 2, 2);
+}
+
+void mgbfo_restrict(int K, int n, int m, const double* fine, double* coarse) {
+    pf_t F = pf_alloc(K, n, m, 0, 0);
+    pf_from_interior(&F, fine);
+    pf_t W = pf_alloc(K, n / 2, m / 2, 2, 2);
+    restrict_pf(&F, &W);
+    pf_to_interior(&W, coarse);
+    pf_free(&F);
+    pf_free(&W);
+}
+
+/* P = direct1 o boco_2d(.,2,2) (downsending_ens, mg_generations.f90:461-491) */
+void mgbfo_prolong(int K, int n, int m, const double* coarse, double* fine) {
+    pf_t W = pf_alloc(K, n / 2, m / 2, 2, 2);
+    pf_from_interior(&W, coarse);
+    boco_2d(&W, 2, 2);
+    pf_t F = pf_alloc(K, n, m, 0, 0);
+    direct1_op(&W, &F);
+    pf_to_interior(&F, fine);
+    pf_free(&W);
+    pf_free(&F);
+}
+
+void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded) {
+    pf_t a = pf_alloc(K, n, m, h, h);
+    pf_from_interior(&a, in);
+    boco_2d(&a, h, h);
+    memcpy(padded, a.d, sizeof(double) * (size_t)K * (n + 2 * h) * (size_t)(m + 2 * h));
+    pf_free(&a);
+}
+
+void mgbfo_fold(int K, int n, int m, int h, const double* padded, double* out) {
+    pf_t a = pf_alloc(K, n, m, h, h);
+    memcpy(a.d, padded, sizeof(double) * (size_t)K * (n + 2 * h) * (size_t)(m + 2 * h));
+    bocoT_2d(&a, h, h);
+    pf_to_interior(&a, out);
+    pf_free(&a);
+}
+
+/* ------------------------------------------------------------------------
+ * The operator
+ * ---------------------------------------------------------------------- */
+
+size_t mgbfo_mg_size(const mgbfo_params* P) {
+    size_t s = 0;
+    for (int g = 0; g < P->gm; ++g) s += (size_t)P->K * (size_t)(P->nx >> g) * (size_t)(P->ny >> g);
+    return s;
+}
+
+/* weighting_ens (mg_generations.f90:1012-1052): interior points only */
+static void weight_pf(const mgbfo_params* P, int gi, pf_t* f, const double* wconst,
+                      const double* const* wfield) {
+    const int K = f->K, n = f->n, m = f->m;
+    const double* wf = (wfield && wfield[gi]) ? wfield[gi] : NULL;
+    const double wc = wconst ? wconst[gi] : 1.0;
+    for (int j = 1; j <= m; ++j)
+        for (int i = 1; i <= n; ++i) {
+            double* c = PCOL(f, i, j);
+            if (wf) {
+                const double* w = wf + (size_t)K * ((size_t)(i - 1) + (size_t)n * (j - 1));
+                for (int k = 0; k < K; ++k) c[k] = w[k] * c[k];
+            } else {
+                for (int k = 0; k < K; ++k) c[k] = wc * c[k];
+            }
+        }
+}
+
+/* Down-sweep shared by W A W^T and W^T: u[0] = Vf^T x, u[g] = R u[g-1]. */
+static void upsweep_inputs(const mgbfo_params* P, const double* x, pf_t* G) {
+    const int gm = P->gm;
+    pf_from_interior(&G[0], x);
+    if (P->l_vertical_filter) vertical_op(P, &G[0], 1);
+    for (int g = 1; g < gm; ++g) {
+        /* adjoint reads the interior of the finer generation only */
+        pf_t Fi = pf_alloc(P->K, G[g - 1].n, G[g - 1].m, 0, 0);
+        pf_t Wi = pf_alloc(P->K, G[g].n, G[g].m, 2, 2);
+        for (int j = 1; j <= Fi.m; ++j)
+            for (int i = 1; i <= Fi.n; ++i)
+                memcpy(PCOL(&Fi, i, j), PCOL(&G[g - 1], i, j), sizeof(double) * P->K);
+        restrict_pf(&Fi, &Wi);
+        for (int j = 1; j <= Wi.m; ++j)
+            for (int i = 1; i <= Wi.n; ++i)
+                memcpy(PCOL(&G[g], i, j), PCOL(&Wi, i, j), sizeof(double) * P->K);
+        pf_free(&Fi);
+        pf_free(&Wi);
+    }
+}
+
+/* Up-sweep: G[g-1] += P G[g] for g = gm-1 .. 1; then Vf on G[0]. */
+static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y) {
+    const int gm = P->gm;
+    for (int g = gm - 1; g >= 1; --g) {
+        pf_t Wi = pf_alloc(P->K, G[g].n, G[g].m, 2, 2);
+        for (int j = 1; j <= Wi.m; ++j)
+            for (int i = 1; i <= Wi.n; ++i)
+                memcpy(PCOL(&Wi, i, j), PCOL(&G[g], i, j), sizeof(double) * P->K);
+        boco_2d(&Wi, 2, 2);
+        pf_t Fp = pf_alloc(P->K, G[g - 1].n, G[g - 1].m, 0, 0);
+        direct1_op(&Wi, &Fp);
+        for (int j = 1; j <= Fp.m; ++j)
+            for (int i = 1; i <= Fp.n; ++i) vadd(PCOL(&G[g - 1], i, j), PCOL(&Fp, i, j), P->K);
+        pf_free(&Wi);
+        pf_free(&Fp);
+    }
+    if (P->l_vertical_filter) vertical_op(P, &G[0], 0);
+    pf_to_interior(&G[0], y);
+}
+
+static pf_t* gens_alloc(const mgbfo_params* P) {
+    pf_t* G = malloc(sizeof(pf_t) * P->gm);
+    for (int g = 0; g < P->gm; ++g) G[g] = pf_alloc(P->K, P->nx >> g, P->ny >> g, P->hx, P->hy);
+    return G;
+}
+static void gens_free(const mgbfo_params* P, pf_t* G) {
+    for (int g = 0; g < P->gm; ++g) pf_free(&G[g]);
+    free(G);
+}
+
+void mgbfo_wawt(const mgbfo_params* P, const double* wconst, const double* const* wfield,
+                const double* x, double* y) {
+    /* filtering_rad2_ens, mg_filtering.f90:1000-1147 */
+    const int gm = P->gm;
+    taps_t T = taps2_make(P);
+    pf_t* G = gens_alloc(P);
+    upsweep_inputs(P, x, G);
+    for (int g = 0; g < gm; ++g) {
+        if (g == 0 && !P->l_filt_g1) continue;
+        vrbeta2T(&T, &G[g]);
+        bocoT_2d(&G[g], P->hx, P->hy);
+        weight_pf(P, g, &G[g], wconst, wfield);
+        boco_2d(&G[g], P->hx, P->hy);
+        vrbeta2(&T, &G[g]);
+    }
+    downsweep_outputs(P, G, y);
+    gens_free(P, G);
+    taps_free(&T);
+}
+
+void mgbfo_wt(const mgbfo_params* P, const double* x, double* v) {
+    const int gm = P->gm;
+    taps_t T = taps2_make(P);
+    pf_t* G = gens_alloc(P);
+    upsweep_inputs(P, x, G);
+    size_t off = 0;
+    for (int g = 0; g < gm; ++g) {
+        if (!(g == 0 && !P->l_filt_g1)) {
+            vrbeta2T(&T, &G[g]);
+            bocoT_2d(&G[g], P->hx, P->hy);
+        }
+        pf_to_interior(&G[g], v + off);
+        off += (size_t)P->K * G[g].n * G[g].m;
+    }
+    gens_free(P, G);
+    taps_free(&T);
+}
+
+void mgbfo_w(const mgbfo_params* P, const double* v, double* y) {
+    const int gm = P->gm;
+    taps_t T = taps2_make(P);
+    pf_t* G = gens_alloc(P);
+    size_t off = 0;
+    for (int g = 0; g < gm; ++g) {
+        pf_from_interior(&G[g], v + off);
+        off += (size_t)P->K * G[g].n * G[g].m;
+        if (!(g == 0 && !P->l_filt_g1)) {
+            boco_2d(&G[g], P->hx, P->hy);
+            vrbeta2(&T, &G[g]);
+        }
+    }
+    downsweep_outputs(P, G, y);
+    gens_free(P, G);
+    taps_free(&T);
+}
+
+void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* const* wfield,
+                  double* v) {
+    size_t off = 0;
+    for (int g = 0; g < P->gm; ++g) {
+        const int n = P->nx >> g, m = P->ny >> g;
+        const size_t sz = (size_t)P->K * n * m;
+        if (!(g == 0 && !P->l_filt_g1)) {
+            const double* wf = (wfield && wfield[g]) ? wfield[g] : NULL;
+            const double wc = wconst ? wconst[g] : 1.0;
+            for (size_t e = 0; e < sz; ++e) v[off + e] *= wf ? wf[e] : wc;
+        }
+        off += sz;
+    }
+}

@@ -1,0 +1,85 @@
+/*
+ * FP64 CPU restatement of the MGBF localization operator (test infrastructure).
+ *
+ * This is the oracle: the checker that the HIP product path is compared with.
+ * It is never linked into the product library. It follows the reference's own
+ * formulation step by step (padded halo arrays, scatter-form adjoints, the
+ * explicit halo fill and fold order of the reference's boundary routines) on a
+ * single global domain. Field layout: k fastest, then x (i), then y (j), as
+ * the reference's VALL(k,i,j).
+ */
+#ifndef MGBF_ORACLE_H
+#define MGBF_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mgbfo_params {
+    int nx, ny;            /* global generation-1 grid */
+    int K;                 /* stacked k extent (km_all) */
+    int lm;                /* levels per vertical column */
+    int nvcol;             /* km3_all: vertical columns in the stack; the first nvcol*lm
+                              entries are vertically filtered, the rest pass through */
+    int hx, hy, hz;        /* halo widths = maximum filter reach */
+    int p;                 /* beta exponent (integer) */
+    int gm;                /* number of generations */
+    double ampl01, ampl02; /* aspect amplitudes: vertical (pasp01), horizontal (pasp02) */
+    double p_del, p_eps;   /* horizontal aspect perturbations (the reference fixes them at 0) */
+    int l_filt_g1;         /* filter and weight generation 1 */
+    int l_vertical_filter; /* apply the vertical 1-D filter */
+} mgbfo_params;
+
+/* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
+/* Returns the number of taps; fills up to maxt entries (gx, gy, w) of the
+   normalized 2-D tap table. Returns -1 if the reach exceeds (hx,hy). */
+int mgbfo_taps2(const mgbfo_params* P, int maxt, int* gx, int* gy, double* w);
+/* Same for the 1-D vertical filter (gx only). */
+int mgbfo_taps1(const mgbfo_params* P, int maxt, int* gx, double* w);
+/* Line-sum normalizers ss = 1/sum(rrc^p) (getlinesum1/2). */
+double mgbfo_ss1(const mgbfo_params* P);
+double mgbfo_ss2(const mgbfo_params* P);
+
+/* ---- single stages on interior-only arrays (K, n, m), k fastest ---- */
+/* Radial 2-D filter F (adjoint=0: halo fill + gather) or its adjoint
+   (adjoint=1: scatter + halo fold), on a generation of size n x m. */
+void mgbfo_radial(const mgbfo_params* P, int K, int n, int m,
+                  const double* in, double* out, int adjoint);
+/* Vertical filter Vf / Vf^T on an (K, n, m) field. */
+void mgbfo_vertical(const mgbfo_params* P, int n, int m,
+                    const double* in, double* out, int adjoint);
+/* Restriction R = Fold_2 o P^T: fine (K, n, m) -> coarse (K, n/2, m/2). */
+void mgbfo_restrict(int K, int n, int m, const double* fine, double* coarse);
+/* Prolongation P = interp o Pad_2: coarse (K, n/2, m/2) -> fine (K, n, m). */
+void mgbfo_prolong(int K, int n, int m, const double* coarse, double* fine);
+/* Halo fill (boco_2d, half-sample mirror) of an interior array into a padded
+   array (K, n+2h, m+2h); and its adjoint fold back to interior. */
+void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded);
+void mgbfo_fold(int K, int n, int m, int h, const double* padded, double* out);
+
+/* ---- the operator ----
+   Generation g (0-based gi = g-1) has size (nx>>gi, ny>>gi). The multigrid
+   control vector v is the concatenation of the gm generation fields.
+   wconst[gi] is the constant weight of generation g; wfield[gi], if wfield
+   and wfield[gi] are non-NULL, is a per-point weight field that replaces it. */
+size_t mgbfo_mg_size(const mgbfo_params* P);
+/* y = W A W^T x (filtering_rad2_ens). */
+void mgbfo_wawt(const mgbfo_params* P, const double* wconst, const double* const* wfield,
+                const double* x, double* y);
+/* v = W^T x: u_1 = Vf^T x, u_{g+1} = R u_g, v_g = F^T u_g (v_1 = u_1 if !l_filt_g1). */
+void mgbfo_wt(const mgbfo_params* P, const double* x, double* v);
+/* y = W v: s_gm = F v_gm, s_g = F v_g + P s_{g+1}, y = Vf s_1. */
+void mgbfo_w(const mgbfo_params* P, const double* v, double* y);
+/* Scales generation g of a multigrid vector by its weight (A). */
+void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* const* wfield,
+                  double* v);
+
+/* Number of OpenMP threads the oracle uses (1 without OpenMP). */
+int mgbfo_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

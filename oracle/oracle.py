@@ -1,0 +1,243 @@
+"""ctypes/numpy wrapper of the FP64 C oracle (test infrastructure only).
+
+The C code restates the reference algorithm routine by routine; see the
+header of mgbf_oracle.c for the reference file:line each routine follows.
+Arrays are numpy float64, shape (m, n, K) in C order, i.e. k fastest then x
+then y, the memory order of the reference's VALL(k, i, j).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libmgbf_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compiles the oracle with gcc (oracle/Makefile)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("K", ctypes.c_int),
+        ("lm", ctypes.c_int), ("nvcol", ctypes.c_int),
+        ("hx", ctypes.c_int), ("hy", ctypes.c_int), ("hz", ctypes.c_int),
+        ("p", ctypes.c_int), ("gm", ctypes.c_int),
+        ("ampl01", ctypes.c_double), ("ampl02", ctypes.c_double),
+        ("p_del", ctypes.c_double), ("p_eps", ctypes.c_double),
+        ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
+    ]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.POINTER(_Params)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.mgbfo_taps2.argtypes = [P, ctypes.c_int, ip, ip, dp]
+        L.mgbfo_taps1.argtypes = [P, ctypes.c_int, ip, dp]
+        L.mgbfo_ss1.argtypes = [P]; L.mgbfo_ss1.restype = ctypes.c_double
+        L.mgbfo_ss2.argtypes = [P]; L.mgbfo_ss2.restype = ctypes.c_double
+        L.mgbfo_radial.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
+        L.mgbfo_vertical.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
+        L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_pad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_fold.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_mg_size.argtypes = [P]; L.mgbfo_mg_size.restype = ctypes.c_size_t
+        L.mgbfo_wawt.argtypes = [P, dp, ctypes.c_void_p, dp, dp]
+        L.mgbfo_wt.argtypes = [P, dp, dp]
+        L.mgbfo_w.argtypes = [P, dp, dp]
+        L.mgbfo_weight.argtypes = [P, dp, ctypes.c_void_p, dp]
+        L.mgbfo_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class OracleParams:
+    """Problem description on one global domain (names follow mg_parameter.f90)."""
+    nx: int
+    ny: int
+    K: int
+    lm: int
+    nvcol: int
+    hx: int = 6
+    hy: int = 6
+    hz: int = 3
+    p: int = 2
+    gm: int = 4
+    ampl01: float = 16.0 / 7.0
+    ampl02: float = 49.0 / 8.0
+    p_del: float = 0.0
+    p_eps: float = 0.0
+    l_filt_g1: bool = True
+    l_vertical_filter: bool = True
+    weights: list = field(default_factory=lambda: [1.0, 0.5, 0.25, 0.125])
+
+    def c(self) -> _Params:
+        return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
+                       self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
+                       self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter))
+
+    def gen_shape(self, g: int):
+        """(m_g, n_g, K) of generation g (0-based)."""
+        return (self.ny >> g, self.nx >> g, self.K)
+
+    def wconst(self) -> np.ndarray:
+        w = list(self.weights)[: self.gm]
+        w = w + [w[-1]] * (self.gm - len(w))
+        return np.ascontiguousarray(w, dtype=np.float64)
+
+
+def taps2(P: OracleParams):
+    c = P.c()
+    cap = (2 * P.hx + 1) * (2 * P.hy + 1)
+    gx = (ctypes.c_int * cap)(); gy = (ctypes.c_int * cap)()
+    w = np.zeros(cap)
+    nt = lib().mgbfo_taps2(ctypes.byref(c), cap, gx, gy, _dp(w))
+    if nt < 0:
+        raise ValueError("filter reach exceeds the halo")
+    return np.array(gx[:nt]), np.array(gy[:nt]), w[:nt].copy()
+
+
+def taps1(P: OracleParams):
+    c = P.c()
+    cap = 2 * P.hz + 1
+    gx = (ctypes.c_int * cap)()
+    w = np.zeros(cap)
+    nt = lib().mgbfo_taps1(ctypes.byref(c), cap, gx, _dp(w))
+    if nt < 0:
+        raise ValueError("filter reach exceeds the halo")
+    return np.array(gx[:nt]), w[:nt].copy()
+
+
+def ss1(P: OracleParams) -> float:
+    c = P.c()
+    return lib().mgbfo_ss1(ctypes.byref(c))
+
+
+def ss2(P: OracleParams) -> float:
+    c = P.c()
+    return lib().mgbfo_ss2(ctypes.byref(c))
+
+
+def _f64(x) -> np.ndarray:
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def radial(P: OracleParams, x: np.ndarray, adjoint: bool = False) -> np.ndarray:
+    x = _f64(x); m, n, K = x.shape
+    y = np.empty_like(x); c = P.c()
+    lib().mgbfo_radial(ctypes.byref(c), K, n, m, _dp(x), _dp(y), int(adjoint))
+    return y
+
+
+def vertical(P: OracleParams, x: np.ndarray, adjoint: bool = False) -> np.ndarray:
+    x = _f64(x); m, n, K = x.shape
+    assert K == P.K
+    y = np.empty_like(x); c = P.c()
+    lib().mgbfo_vertical(ctypes.byref(c), n, m, _dp(x), _dp(y), int(adjoint))
+    return y
+
+
+def restrict(x: np.ndarray) -> np.ndarray:
+    x = _f64(x); m, n, K = x.shape
+    y = np.empty((m // 2, n // 2, K))
+    lib().mgbfo_restrict(K, n, m, _dp(x), _dp(y))
+    return y
+
+
+def prolong(xc: np.ndarray) -> np.ndarray:
+    xc = _f64(xc); mc, nc, K = xc.shape
+    y = np.empty((2 * mc, 2 * nc, K))
+    lib().mgbfo_prolong(K, 2 * nc, 2 * mc, _dp(xc), _dp(y))
+    return y
+
+
+def pad(x: np.ndarray, h: int) -> np.ndarray:
+    x = _f64(x); m, n, K = x.shape
+    y = np.empty((m + 2 * h, n + 2 * h, K))
+    lib().mgbfo_pad(K, n, m, h, _dp(x), _dp(y))
+    return y
+
+
+def fold(xp: np.ndarray, h: int) -> np.ndarray:
+    xp = _f64(xp); mp, np_, K = xp.shape
+    y = np.empty((mp - 2 * h, np_ - 2 * h, K))
+    lib().mgbfo_fold(K, np_ - 2 * h, mp - 2 * h, h, _dp(xp), _dp(y))
+    return y
+
+
+def _wfield_ptrs(P: OracleParams, wfield):
+    if wfield is None:
+        return None, []
+    keep = [None if w is None else _f64(w) for w in wfield]
+    arr = (ctypes.c_void_p * P.gm)(*[0 if w is None else w.ctypes.data for w in keep])
+    return arr, keep
+
+
+def mg_split(P: OracleParams, v: np.ndarray):
+    """Splits a flat multigrid vector into its generation fields."""
+    out, off = [], 0
+    for g in range(P.gm):
+        s = P.gen_shape(g)
+        sz = s[0] * s[1] * s[2]
+        out.append(v[off: off + sz].reshape(s))
+        off += sz
+    return out
+
+
+def mg_join(fields) -> np.ndarray:
+    return np.concatenate([_f64(f).ravel() for f in fields])
+
+
+def wawt(P: OracleParams, x: np.ndarray, wfield=None) -> np.ndarray:
+    x = _f64(x)
+    assert x.shape == P.gen_shape(0)
+    y = np.empty_like(x); c = P.c(); wc = P.wconst()
+    arr, _keep = _wfield_ptrs(P, wfield)
+    lib().mgbfo_wawt(ctypes.byref(c), _dp(wc), arr, _dp(x), _dp(y))
+    return y
+
+
+def wt(P: OracleParams, x: np.ndarray) -> np.ndarray:
+    x = _f64(x); c = P.c()
+    v = np.empty(lib().mgbfo_mg_size(ctypes.byref(c)))
+    lib().mgbfo_wt(ctypes.byref(c), _dp(x), _dp(v))
+    return v
+
+
+def w(P: OracleParams, v: np.ndarray) -> np.ndarray:
+    v = _f64(v); c = P.c()
+    y = np.empty(P.gen_shape(0))
+    lib().mgbfo_w(ctypes.byref(c), _dp(v), _dp(y))
+    return y
+
+
+def weight(P: OracleParams, v: np.ndarray, wfield=None) -> np.ndarray:
+    v = _f64(v).copy(); c = P.c(); wc = P.wconst()
+    arr, _keep = _wfield_ptrs(P, wfield)
+    lib().mgbfo_weight(ctypes.byref(c), _dp(wc), arr, _dp(v))
+    return v
+
+
+def num_threads() -> int:
+    return lib().mgbfo_num_threads()

@@ -1,0 +1,240 @@
+"""Pins of the FP64 oracle against what the mathematics fixes.
+
+The reference has no tests or fixtures and cannot be compiled here (every one
+of its modules has `use mpi`), so each oracle routine is pinned by checks that
+a plausible mistake would fail: closed-form lattice counts, exact rational tap
+weights, transposition (dot-product) tests between independently written
+forward and adjoint codes, conservation laws, reproduction of constants and
+linear functions, mirror symmetry of the whole operator, and a brute-force
+comparison with an independent dense-matrix model on tiny grids.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import dense_model as D
+
+
+def _P(**kw):
+    base = dict(nx=32, ny=32, K=6, lm=3, nvcol=2, hx=4, hy=4, hz=2, p=2, gm=3,
+                ampl01=Fraction(9, 7), ampl02=Fraction(25, 8))
+    base.update(kw)
+    base["ampl01"] = float(base["ampl01"])
+    base["ampl02"] = float(base["ampl02"])
+    return O.OracleParams(**base)
+
+
+def _rand(shape, seed):
+    return np.random.default_rng(seed).standard_normal(shape)
+
+
+@pytest.mark.parametrize("h,count2,count1", [(3, 45, 7), (4, 69, 9), (6, 145, 13),
+                                             (8, 249, 17), (10, 373, 21)])
+def test_tap_counts_match_lattice_count(h, count2, count1):
+    # reach = halo: ampl02 = (h+1)^2/(2p+4), ampl01 = (h+1)^2/(2p+3), p = 2
+    P = _P(hx=h, hy=h, hz=h, ampl02=Fraction((h + 1) ** 2, 8), ampl01=Fraction((h + 1) ** 2, 7))
+    gx, gy, w = O.taps2(P)
+    brute = sum(1 for a in range(-h, h + 1) for b in range(-h, h + 1) if a * a + b * b < (h + 1) ** 2)
+    assert len(gx) == brute == count2
+    g1, w1 = O.taps1(P)
+    assert len(g1) == 2 * h + 1 == count1
+    assert abs(w.sum() - 1) < 1e-14 and abs(w1.sum() - 1) < 1e-14
+
+
+@pytest.mark.parametrize("h", [3, 6, 8])
+def test_tap_weights_are_the_beta_function(h):
+    R2 = Fraction((h + 1) ** 2)
+    P = _P(hx=h, hy=h, hz=h, ampl02=R2 / 8, ampl01=R2 / 7)
+    gx, gy, w = O.taps2(P)
+    exact = {(a, b): wt for a, b, wt in D.disc_taps(h, R2, 2)}
+    for a, b, wt in zip(gx, gy, w):
+        assert abs(wt - exact[(a, b)]) < 1e-15
+    g1, w1 = O.taps1(P)
+    exact1 = dict(D.line_taps(h, R2, 2))
+    for a, wt in zip(g1, w1):
+        assert abs(wt - exact1[a]) < 1e-15
+    # the reference's loop order: gy outer (ascending), gx inner (ascending)
+    assert list(zip(gy, gx)) == sorted(zip(gy, gx))
+
+
+def test_aspect_tensor_is_second_moment():
+    # rmom2 = 1/sqrt(2p+4) scales the filter so that its second moment per axis
+    # approaches the aspect amplitude (continuous limit of the beta function).
+    P = _P(hx=10, hy=10, ampl02=Fraction(121, 8))
+    gx, gy, w = O.taps2(P)
+    m2 = float(np.sum(w * gx.astype(float) ** 2))
+    assert abs(m2 / P.ampl02 - 1) < 0.05
+    assert abs(np.sum(w * gx * gy)) < 1e-13
+
+
+def test_anisotropic_aspect_rotates_the_filter():
+    # an off-diagonal aspect term makes the taps symmetric only under (a,b)->(-a,-b)
+    P = _P(hx=6, hy=6, ampl02=Fraction(3, 1), p_eps=0.3)
+    gx, gy, w = O.taps2(P)
+    tab = {(a, b): v for a, b, v in zip(gx, gy, w)}
+    for (a, b), v in tab.items():
+        assert abs(tab[(-a, -b)] - v) < 1e-15
+    assert np.sum(w * gx * gy) > 0.1   # positive correlation along the diagonal
+
+
+def test_reach_beyond_halo_is_refused():
+    P = _P(hx=3, hy=3, ampl02=Fraction(25, 8))  # reach 5 > halo 3
+    with pytest.raises(ValueError):
+        O.taps2(P)
+
+
+def test_radial_preserves_constants_and_adjoint_conserves_mass():
+    P = _P()
+    x = np.full((20, 24, 3), 2.5)
+    assert np.allclose(O.radial(P, x), 2.5, atol=1e-13)
+    z = _rand((20, 24, 3), 1)
+    assert abs(O.radial(P, z, adjoint=True).sum() - z.sum()) < 1e-10
+
+
+def test_radial_adjoint_dot_product():
+    P = _P()
+    x, y = _rand((20, 24, 3), 2), _rand((20, 24, 3), 3)
+    lhs = np.vdot(O.radial(P, x), y)
+    rhs = np.vdot(x, O.radial(P, y, adjoint=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_radial_is_symmetric_with_half_sample_mirror():
+    # gather+pad and scatter+fold give the same matrix for a mirror-symmetric
+    # disc, so F^T = F; the GPU path relies on this.
+    P = _P()
+    x = _rand((20, 24, 3), 4)
+    assert np.max(np.abs(O.radial(P, x) - O.radial(P, x, adjoint=True))) < 1e-14
+
+
+def test_pad_fold_adjoint_and_mirror_values():
+    x = _rand((9, 7, 2), 5)
+    xp = O.pad(x, 3)
+    # half-sample mirror: X(1-t) = X(t), X(n+t) = X(n+1-t) in both directions
+    assert np.array_equal(xp[3:-3, 2, :], x[:, 0, :])
+    assert np.array_equal(xp[3:-3, 0, :], x[:, 2, :])
+    assert np.array_equal(xp[3:-3, -1, :], x[:, -3, :])
+    assert np.array_equal(xp[0, 0, :], x[2, 2, :])
+    z = _rand(xp.shape, 6)
+    assert abs(np.vdot(xp, z) - np.vdot(x, O.fold(z, 3))) < 1e-12
+
+
+def test_prolong_restrict_adjoint():
+    xc, yf = _rand((8, 10, 3), 7), _rand((16, 20, 3), 8)
+    lhs = np.vdot(O.prolong(xc), yf)
+    rhs = np.vdot(xc, O.restrict(yf))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_prolong_reproduces_constants_and_linear_functions():
+    assert np.allclose(O.prolong(np.ones((8, 8, 2))), 1.0, atol=1e-14)
+    # coarse point I sits at x = 2I - 1/2 in fine units (0-based fine i at i)
+    nc = 12
+    X = 2.0 * np.arange(nc) + 0.5
+    coarse = np.broadcast_to(X[None, :, None], (nc, nc, 1)).copy()
+    fine = O.prolong(coarse)
+    xi = np.arange(2 * nc, dtype=float)
+    # interior fine points (away from the mirrored 2-halo) reproduce x exactly
+    assert np.allclose(fine[:, 4:-4, 0], xi[None, 4:-4], atol=1e-12)
+
+
+def test_vertical_constants_and_adjoint():
+    P = _P(K=7, lm=3, nvcol=2, hz=2, ampl01=Fraction(9, 7))
+    x = np.full((4, 5, 7), 1.5)
+    y = O.vertical(P, x)
+    assert np.allclose(y, 1.5, atol=1e-14)
+    a, b = _rand((4, 5, 7), 9), _rand((4, 5, 7), 10)
+    lhs = np.vdot(O.vertical(P, a), b)
+    rhs = np.vdot(a, O.vertical(P, b, adjoint=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    # the entries beyond nvcol*lm pass through untouched
+    assert np.array_equal(O.vertical(P, a)[..., 6], a[..., 6])
+    # whole-sample mirror makes Vf non-symmetric
+    assert np.max(np.abs(O.vertical(P, a) - O.vertical(P, a, adjoint=True))) > 1e-3
+
+
+def _dense_for(P, wfield=None):
+    return D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm,
+                       Fraction(P.ampl02).limit_denominator(1000) * (2 * P.p + 4),
+                       Fraction(P.ampl01).limit_denominator(1000) * (2 * P.p + 3),
+                       list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, wfield)
+
+
+def _oracle_matrix(P, fn, N):
+    cols = []
+    for e in range(N):
+        x = np.zeros(N)
+        x[e] = 1.0
+        cols.append(fn(x))
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("filt_g1,vert", [(True, True), (False, True), (True, False)])
+def test_wawt_matches_dense_model(filt_g1, vert):
+    P = _P(nx=16, ny=12, K=4, lm=2, nvcol=2, hx=3, hy=3, hz=1, gm=3,
+           ampl02=Fraction(16, 8), ampl01=Fraction(4, 7), l_filt_g1=filt_g1,
+           l_vertical_filter=vert, weights=[1.0, 0.7, 0.3])
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    Bd = _dense_for(P).B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+    # symmetric positive semi-definite
+    assert np.max(np.abs(B - B.T)) < 1e-13
+    assert np.linalg.eigvalsh(0.5 * (B + B.T)).min() > -1e-12
+
+
+def test_wawt_with_per_point_weights_matches_dense_model():
+    P = _P(nx=16, ny=16, K=2, lm=2, nvcol=1, hx=3, hy=3, hz=1, gm=2,
+           ampl02=Fraction(16, 8), ampl01=Fraction(4, 7))
+    wf = [0.5 + np.random.default_rng(11 + g).random(P.gen_shape(g)) for g in range(P.gm)]
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape), wfield=wf).ravel(), N)
+    Bd = _dense_for(P, wf).B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+
+
+def test_w_and_wt_are_adjoint_and_compose_to_wawt():
+    P = _P()
+    x = _rand(P.gen_shape(0), 12)
+    size = sum(int(np.prod(P.gen_shape(g))) for g in range(P.gm))
+    v = _rand(size, 13)
+    lhs = np.vdot(O.w(P, v), x)
+    rhs = np.vdot(v, O.wt(P, x))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    y1 = O.wawt(P, x)
+    y2 = O.w(P, O.weight(P, O.wt(P, x)))
+    assert np.max(np.abs(y1 - y2)) < 1e-13 * np.max(np.abs(y1))
+
+
+def test_wt_matches_dense_model():
+    P = _P(nx=16, ny=16, K=2, lm=2, nvcol=1, hx=3, hy=3, hz=1, gm=3,
+           ampl02=Fraction(16, 8), ampl01=Fraction(4, 7))
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    Wt = _oracle_matrix(P, lambda x: O.wt(P, x.reshape(shape)), N)
+    Wtd = _dense_for(P).Wt()
+    assert np.max(np.abs(Wt - Wtd)) < 1e-14
+
+
+def test_wawt_commutes_with_domain_reflections():
+    # the offset grids are mirror symmetric about the domain centre, so B
+    # commutes with i -> n+1-i, j -> m+1-j and (square domain) i <-> j
+    P = _P(nx=32, ny=32, K=4, lm=2, nvcol=2)
+    x = _rand(P.gen_shape(0), 14)
+    y = O.wawt(P, x)
+    assert np.allclose(O.wawt(P, x[:, ::-1, :].copy()), y[:, ::-1, :], atol=1e-13)
+    assert np.allclose(O.wawt(P, x[::-1, :, :].copy()), y[::-1, :, :], atol=1e-13)
+    xt = np.ascontiguousarray(x.transpose(1, 0, 2))
+    assert np.allclose(O.wawt(P, xt), y.transpose(1, 0, 2), atol=1e-13)
+
+
+def test_wawt_is_linear():
+    P = _P()
+    a, b = _rand(P.gen_shape(0), 15), _rand(P.gen_shape(0), 16)
+    lhs = O.wawt(P, 2.0 * a - 3.0 * b)
+    rhs = 2.0 * O.wawt(P, a) - 3.0 * O.wawt(P, b)
+    assert np.max(np.abs(lhs - rhs)) < 1e-12
