@@ -1,0 +1,151 @@
+/*
+ * mgbf.h -- C ABI of the MI355X-native MGBF localization operator.
+ *
+ * The operator is B = W A W^T with W = Vf . D . F (vertical 1-D beta filter,
+ * multigrid prolongation chain, radial 2-D beta filter on every generation)
+ * and A the per-generation weights. B^{1/2} = W A^{1/2}.
+ *
+ * The reference exposes this path as a Fortran module call surface over
+ * module-global state (src.mgbf/RBETA_TEST.f90:34-126):
+ *   mg_initialize()                 src.mgbf/mg_entrymod.f90:36-219
+ *   anal_to_filt_all2()             src.mgbf/mg_transfer.f90:154-183
+ *   filtering_procedure(mgbf_proc)  src.mgbf/mg_filtering.f90:67-120
+ *   filt_to_anal_all2()             src.mgbf/mg_transfer.f90:186-203
+ *   mg_finalize()                   src.mgbf/mg_entrymod.f90:222-273
+ * The entry points below replace that lifecycle with an explicit context; the
+ * Fortran module mg_entrymod of this package binds them with ISO_C_BINDING.
+ *
+ * Conventions:
+ *   - every entry point returns MGBF_OK (0) or a negative status; the message
+ *     of the last failure on the calling thread is mgbf_last_error();
+ *   - fields are caller-owned device buffers of float (FP32), layout k fastest,
+ *     then x (i), then y (j): element (k,i,j) at k + K*(i + nx*j), K = km_all,
+ *     which is the memory order of the reference's WORKA/VALL(k,i,j);
+ *   - a multigrid (control) vector is the concatenation of the gm generation
+ *     fields, generation g of size K*(nx>>(g-1))*(ny>>(g-1));
+ *   - the context owns its device workspace; one context per host thread or
+ *     stream; calls on one context must not overlap;
+ *   - `stream` is a hipStream_t (NULL = the null stream).
+ */
+#ifndef MGBF_H
+#define MGBF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MGBF_OK 0
+#define MGBF_ERR_ARG (-1)      /* bad pointer or argument */
+#define MGBF_ERR_CONFIG (-2)   /* configuration rejected (the reference would `stop`) */
+#define MGBF_ERR_HIP (-3)      /* HIP runtime failure */
+#define MGBF_ERR_UNSUPPORTED (-4) /* option of the reference not built here */
+
+/*
+ * Every key of the namelist &PARAMETERS_MGBETA, with the reference's meaning
+ * (src.mgbf/mg_parameter.f90:190-208). Logicals are ints (0/1).
+ */
+typedef struct mgbf_config {
+    double mg_ampl01, mg_ampl02, mg_ampl03; /* aspect amplitudes: vertical, horizontal, 3-D */
+    double mg_weig1, mg_weig2, mg_weig3, mg_weig4, mg_weig5; /* generation weights (g>=5 -> weig5) */
+    int hx, hy, hz;  /* halo widths = maximum filter reach */
+    int p;           /* beta exponent (integer in the reference) */
+    int mgbf_line;   /* line (dibeta) filters: not built */
+    int mgbf_proc;   /* filtering procedure: 1 (rad2_bckg) and 6 (rad2_ens) are built */
+    int n_ens;       /* ensemble members, stacked in k */
+    int lm_a, lm;    /* analysis / filter vertical levels */
+    int km2, km3;    /* 2-D and 3-D variables */
+    int l_loc, l_loc_vertical, l_filt_g1, l_lin_vertical, l_lin_horizontal,
+        l_quad_horizontal, l_new_map, l_vertical_filter, ldelta, lquart, lhelm;
+    int gm_max;      /* requested number of generations */
+    int nm0, mm0;    /* global analysis grid */
+    int nxPE, nyPE;  /* reference PE grid: fixes the global grid and caps gm */
+    int im_filt, jm_filt; /* filter tile size per PE */
+} mgbf_config;
+
+/* Derived sizes (init_mg_parameter, src.mgbf/mg_parameter.f90:179-575). */
+typedef struct mgbf_dims {
+    int gm;          /* generations: min(def_maxgen(nxPE,nyPE), gm_max) */
+    int nx, ny;      /* global generation-1 grid: nxPE*im_filt, nyPE*jm_filt */
+    int km;          /* km2 + lm*km3 */
+    int km_all;      /* K: stacked k extent = km*n_ens */
+    int km3_all;     /* vertical columns in the stack */
+    int ntaps2;      /* taps of the radial 2-D filter */
+    int ntaps1;      /* taps of the vertical 1-D filter */
+    long long field_size; /* K*nx*ny */
+    long long mg_size;    /* elements of a multigrid vector */
+} mgbf_dims;
+
+typedef struct mgbf_ctx mgbf_ctx;
+
+/* operations of mgbf_apply */
+enum {
+    MGBF_OP_WAWT = 0,  /* y = W A W^T x   (filtering_rad2_ens / _bckg): field -> field */
+    MGBF_OP_WT = 1,    /* v = W^T x       : field -> multigrid vector */
+    MGBF_OP_W = 2,     /* y = W v         : multigrid vector -> field */
+    MGBF_OP_A = 3,     /* v = A v         : multigrid -> multigrid */
+    MGBF_OP_ASQRT = 4, /* v = A^{1/2} v   : multigrid -> multigrid */
+    MGBF_OP_BSQRT = 5, /* y = W A^{1/2} v : multigrid vector -> field */
+    MGBF_OP_BSQRTT = 6 /* v = A^{1/2} W^T x : field -> multigrid vector */
+};
+
+/* single stages of mgbf_apply_stage (gen is 1-based) */
+enum {
+    MGBF_STAGE_VF = 0,  /* Vf   on a generation-1 field (sup_vrbeta1_ens) */
+    MGBF_STAGE_VFT = 1, /* Vf^T on a generation-1 field (sup_vrbeta1T_ens) */
+    MGBF_STAGE_F = 2,   /* F  on generation gen: boco_2d + rbeta (vrbeta2) */
+    MGBF_STAGE_FT = 3,  /* F^T on generation gen: rbetaT (vrbeta2T) + bocoT_2d */
+    MGBF_STAGE_R = 4,   /* restriction gen -> gen+1: adjoint + bocoT_2d(.,2,2) */
+    MGBF_STAGE_P = 5,   /* prolongation gen+1 -> gen: boco_2d(.,2,2) + direct1 */
+    MGBF_STAGE_A = 6    /* weighting of generation gen (weighting_ens) */
+};
+
+/* Fills *cfg with the settings of a 4-generation ensemble run (mgbf_proc=6,
+   radial half-width 6, vertical half-width 3, p=2, reach equal to the halo). */
+int mgbf_config_default(mgbf_config* cfg);
+
+/* Validates cfg and derives the sizes, as init_mg_parameter does
+   (src.mgbf/mg_parameter.f90:179-575, def_maxgen :578-611); no device use. */
+int mgbf_derive(const mgbf_config* cfg, mgbf_dims* dims);
+
+/* mg_initialize (src.mgbf/mg_entrymod.f90:36-219) without MPI: validates,
+   builds the FP64 coefficients (def_offset_coef, def_mg_weights ->
+   cholaspect/getlinesum, src.mgbf/mg_intstate.f90:296-433), uploads the
+   FP32 tables and allocates the workspace on `device`. */
+int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** ctx);
+
+/* mg_finalize (src.mgbf/mg_entrymod.f90:222-273): frees the context. */
+int mgbf_finalize(mgbf_ctx* ctx);
+
+int mgbf_get_dims(const mgbf_ctx* ctx, mgbf_dims* dims);
+
+/* Replaces the weight of generation gen (1-based): a_diff_f (gen 1) or
+   a_diff_h (gen >= 2) of src.mgbf/mg_intstate.f90:333-351. `field` is a
+   caller-owned device array of K*n_g*m_g floats that must stay valid while
+   the context uses it, or NULL to use the constant `value`. */
+int mgbf_set_weights(mgbf_ctx* ctx, int gen, const float* field, double value);
+
+/* Applies an operator (MGBF_OP_*). For WAWT, `in` and `out` may alias. */
+int mgbf_apply(mgbf_ctx* ctx, int op, const float* in, float* out, void* stream);
+
+/* Applies one stage (MGBF_STAGE_*) on generation gen; in and out must not alias. */
+int mgbf_apply_stage(mgbf_ctx* ctx, int stage, int gen, const float* in, float* out,
+                     void* stream);
+
+/* FP64-accumulated inner product of two device float arrays of n elements;
+   *result is written on the host after the stream has finished the sum. */
+int mgbf_dot(mgbf_ctx* ctx, const float* a, const float* b, long long n, double* result,
+             void* stream);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* mgbf_last_error(void);
+
+/* Library version string. */
+const char* mgbf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGBF_H */

@@ -1,0 +1,458 @@
+// C ABI of the MGBF operator (declared in include/mgbf.h).
+//
+// The operator composition follows filtering_rad2_ens
+// (src.mgbf/mg_filtering.f90:1000-1147) and filtering_rad2_bckg (:560-680):
+//   u_1 = Vf^T x; u_{g+1} = R u_g            (sup_vrbeta1T_ens, upsending_ens)
+//   z_g = F A_g F^T u_g                       (rbetaT, bocoT_2d, weighting_ens, boco_2d, rbeta)
+//   s_gm = z_gm; s_g = z_g + P s_{g+1}        (downsending_ens)
+//   y = Vf s_1                                (sup_vrbeta1_ens)
+// On one device the reference's halo exchanges are the mirror boundary of
+// the global domain and its up/down re-tiling is addressing, so every
+// generation is one device array. F^T is applied as F: the radial filter is
+// symmetric under the half-sample mirror when the aspect tensor is diagonal.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mgbf.h"
+#include "mgbf_coef.hpp"
+#include "mgbf_launch.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(MGBF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                          \
+    do {                                             \
+        hipError_t e_ = (expr);                      \
+        if (e_ != hipSuccess) return hip_fail(e_, what); \
+    } while (0)
+
+// Device selection for the duration of one call.
+struct DeviceGuard {
+    int prev = -1;
+    bool changed = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) {
+            changed = hipSetDevice(dev) == hipSuccess;
+        }
+    }
+    ~DeviceGuard() {
+        if (changed) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+struct mgbf_ctx {
+    mgbf::Setup s;
+    int device = 0;
+    int K = 0, gm = 0;
+    int nfilt = 0;          // vertically filtered part of the stack (0: none)
+    bool filt_g1 = true;
+    mgbf::RadialCoef coef{};
+    int nx[mgbf::kMaxGen] = {}, ny[mgbf::kMaxGen] = {};
+    long long gsize[mgbf::kMaxGen] = {};
+    long long goff[mgbf::kMaxGen] = {};
+    void* d_tables = nullptr;
+    const float* d_vf = nullptr;
+    const float* d_vft = nullptr;
+    const int* d_rxs[mgbf::kMaxGen] = {};
+    const float* d_rxw[mgbf::kMaxGen] = {};
+    const int* d_rys[mgbf::kMaxGen] = {};
+    const float* d_ryw[mgbf::kMaxGen] = {};
+    const int* d_pxs[mgbf::kMaxGen] = {};
+    const float* d_pxw[mgbf::kMaxGen] = {};
+    const int* d_pys[mgbf::kMaxGen] = {};
+    const float* d_pyw[mgbf::kMaxGen] = {};
+    const float* wfield[mgbf::kMaxGen] = {};
+    float wconst[mgbf::kMaxGen] = {};
+    float* ws_u = nullptr;  // multigrid-sized
+    float* ws_t = nullptr;  // multigrid-sized
+    double* d_dot = nullptr;
+};
+
+namespace {
+
+int pick_tj(int K, int n, int m) {
+    const long long nkc = (K + 63) / 64;
+    const long long ntile_i = (n + 3) / 4;
+    int tj = m;
+    while (tj > 16 && nkc * ntile_i * ((m + tj - 1) / tj) < 16384) tj = (tj + 1) / 2;
+    return tj;
+}
+
+int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield, float wconst,
+           hipStream_t st) {
+    mgbf::RadialArgs a{};
+    a.in = in;
+    a.out = out;
+    a.wfield = wfield;
+    a.wconst = wconst;
+    a.K = c->K;
+    a.n = c->nx[g];
+    a.m = c->ny[g];
+    a.H = c->s.H;
+    a.TJ = pick_tj(c->K, a.n, a.m);
+    a.exact_disc = c->s.exact_disc;
+    a.coef = c->coef;
+    HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
+    return MGBF_OK;
+}
+
+int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t st) {
+    mgbf::TransferArgs a{fine, coarse, c->K, c->nx[g], c->ny[g],
+                         c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g]};
+    HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
+    return MGBF_OK;
+}
+
+int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, float* fine,
+              hipStream_t st) {
+    mgbf::TransferArgs a{coarse, fine, c->K, c->nx[g], c->ny[g],
+                         c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g]};
+    HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
+    return MGBF_OK;
+}
+
+int vertical(mgbf_ctx* c, const float* in, float* out, bool adjoint, hipStream_t st) {
+    if (c->nfilt == 0) {
+        if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
+        return MGBF_OK;
+    }
+    HIP_TRY(mgbf::launch_vertical(in, out, adjoint ? c->d_vft : c->d_vf, c->K, c->s.cfg.lm,
+                                  c->nfilt, c->s.cfg.hz, (long long)c->nx[0] * c->ny[0], st),
+            "vertical filter launch");
+    return MGBF_OK;
+}
+
+#define CALL(x)                      \
+    do {                             \
+        int st_ = (x);               \
+        if (st_ != MGBF_OK) return st_; \
+    } while (0)
+
+// u_1 = Vf^T x into U, then the restriction chain.
+int down_inputs(mgbf_ctx* c, const float* x, float* U, hipStream_t st) {
+    CALL(vertical(c, x, U, true, st));
+    for (int g = 0; g + 1 < c->gm; ++g)
+        CALL(restrict_g(c, g, U + c->goff[g], U + c->goff[g + 1], st));
+    return MGBF_OK;
+}
+
+// s_g = z_g + P s_{g+1} in place on Z, then y = Vf s_1.
+int up_outputs(mgbf_ctx* c, float* Z, float* y, hipStream_t st) {
+    for (int g = c->gm - 2; g >= 0; --g)
+        CALL(prolong_g(c, g, Z + c->goff[g + 1], Z + c->goff[g], Z + c->goff[g], st));
+    CALL(vertical(c, Z, y, false, st));
+    return MGBF_OK;
+}
+
+int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
+    float* U = c->ws_u;
+    float* T = c->ws_t;
+    CALL(down_inputs(c, x, U, st));
+    for (int g = 0; g < c->gm; ++g) {
+        if (g == 0 && !c->filt_g1) continue;
+        // F^T then A (weighting after the adjoint filter), then F
+        CALL(radial(c, g, U + c->goff[g], T, c->wfield[g], c->wconst[g], st));
+        CALL(radial(c, g, T, U + c->goff[g], nullptr, 1.0f, st));
+    }
+    return up_outputs(c, U, y, st);
+}
+
+int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
+    float* U = c->ws_u;
+    CALL(down_inputs(c, x, U, st));
+    for (int g = 0; g < c->gm; ++g) {
+        if (g == 0 && !c->filt_g1) {
+            HIP_TRY(mgbf::launch_copy(U, v, c->gsize[0], st), "copy launch");
+            continue;
+        }
+        CALL(radial(c, g, U + c->goff[g], v + c->goff[g], nullptr, 1.0f, st));
+        if (sqrt_weights)
+            HIP_TRY(mgbf::launch_scale(v + c->goff[g], v + c->goff[g], c->wfield[g], c->wconst[g], 1,
+                                       c->gsize[g], st),
+                    "scale launch");
+    }
+    return MGBF_OK;
+}
+
+int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
+    float* U = c->ws_u;
+    float* T = c->ws_t;
+    for (int g = 0; g < c->gm; ++g) {
+        const float* src = v + c->goff[g];
+        if (g == 0 && !c->filt_g1) {
+            HIP_TRY(mgbf::launch_copy(src, U, c->gsize[0], st), "copy launch");
+            continue;
+        }
+        if (sqrt_weights) {
+            HIP_TRY(mgbf::launch_scale(src, T + c->goff[g], c->wfield[g], c->wconst[g], 1,
+                                       c->gsize[g], st),
+                    "scale launch");
+            src = T + c->goff[g];
+        }
+        CALL(radial(c, g, src, U + c->goff[g], nullptr, 1.0f, st));
+    }
+    return up_outputs(c, U, y, st);
+}
+
+int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStream_t st) {
+    for (int g = 0; g < c->gm; ++g) {
+        if (g == 0 && !c->filt_g1) {
+            if (v != out)
+                HIP_TRY(mgbf::launch_copy(v, out, c->gsize[0], st), "copy launch");
+            continue;
+        }
+        HIP_TRY(mgbf::launch_scale(v + c->goff[g], out + c->goff[g], c->wfield[g], c->wconst[g],
+                                   use_sqrt ? 1 : 0, c->gsize[g], st),
+                "scale launch");
+    }
+    return MGBF_OK;
+}
+
+template <typename T>
+T* carve(char*& p, size_t count) {
+    T* r = reinterpret_cast<T*>(p);
+    size_t bytes = (sizeof(T) * count + 255) & ~size_t(255);
+    p += bytes;
+    return r;
+}
+
+size_t carve_size(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+}  // namespace
+
+extern "C" {
+
+const char* mgbf_version(void) { return "mgbf-mi355x 0.1.0"; }
+
+const char* mgbf_last_error(void) { return g_last_error.c_str(); }
+
+int mgbf_config_default(mgbf_config* cfg) {
+    if (!cfg) return fail(MGBF_ERR_ARG, "cfg is NULL");
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->hx = cfg->hy = 6;
+    cfg->hz = 3;
+    cfg->p = 2;
+    cfg->mg_ampl02 = 49.0 / 8.0;   // reach (h+1) = 7 = halo + 1
+    cfg->mg_ampl01 = 16.0 / 7.0;   // reach 4 = hz + 1
+    cfg->mg_ampl03 = 1.0;
+    cfg->mg_weig1 = 1.0;
+    cfg->mg_weig2 = 0.5;
+    cfg->mg_weig3 = 0.25;
+    cfg->mg_weig4 = 0.125;
+    cfg->mg_weig5 = 0.0625;
+    cfg->mgbf_proc = 6;
+    cfg->n_ens = 1;
+    cfg->lm_a = cfg->lm = 32;
+    cfg->km2 = 0;
+    cfg->km3 = 1;
+    cfg->l_filt_g1 = 1;
+    cfg->l_new_map = 1;
+    cfg->l_vertical_filter = 1;
+    cfg->gm_max = 4;
+    cfg->nxPE = cfg->nyPE = 8;
+    cfg->im_filt = cfg->jm_filt = 24;
+    cfg->nm0 = cfg->mm0 = 192;
+    return MGBF_OK;
+}
+
+int mgbf_derive(const mgbf_config* cfg, mgbf_dims* dims) {
+    if (!cfg || !dims) return fail(MGBF_ERR_ARG, "NULL argument");
+    std::string err;
+    mgbf::Setup s;
+    int st = mgbf::build_setup(*cfg, s, &err);
+    if (st != MGBF_OK) return fail(st, err);
+    *dims = s.dims;
+    return MGBF_OK;
+}
+
+int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
+    if (!cfg || !out) return fail(MGBF_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    auto* c = new mgbf_ctx();
+    std::string err;
+    int st = mgbf::build_setup(*cfg, c->s, &err);
+    if (st != MGBF_OK) {
+        delete c;
+        return fail(st, err);
+    }
+    const mgbf::Setup& s = c->s;
+    c->device = device;
+    c->K = s.dims.km_all;
+    c->gm = s.dims.gm;
+    c->filt_g1 = cfg->mgbf_proc == 1 ? true : cfg->l_filt_g1 != 0;
+    c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
+    long long off = 0;
+    for (int g = 0; g < c->gm; ++g) {
+        c->nx[g] = s.nx[g];
+        c->ny[g] = s.ny[g];
+        c->gsize[g] = (long long)c->K * s.nx[g] * s.ny[g];
+        c->goff[g] = off;
+        off += c->gsize[g];
+        c->wconst[g] = (float)s.wconst[g];
+    }
+    for (int b = 0; b <= mgbf::kMaxHaloDev; ++b)
+        for (int a = 0; a <= mgbf::kMaxHaloDev; ++a)
+            c->coef.w[b][a] = (b <= mgbf::kMaxHalo && a <= mgbf::kMaxHalo) ? (float)s.wsym[b][a] : 0.f;
+
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || device < 0 || device >= dev_count) {
+        delete c;
+        return fail(MGBF_ERR_HIP, "no HIP device with the requested index");
+    }
+    DeviceGuard guard(device);
+
+    // one device allocation for all coefficient tables
+    const size_t band = s.vf.size();
+    size_t bytes = 2 * carve_size(sizeof(float) * band) + carve_size(sizeof(double));
+    for (int g = 0; g + 1 < c->gm; ++g) {
+        const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
+        for (const auto* t : tabs)
+            bytes += carve_size(sizeof(int) * t->n_out) + carve_size(sizeof(float) * t->w.size());
+    }
+    std::vector<char> host(bytes, 0);
+    hipError_t e = hipMalloc(&c->d_tables, bytes);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "hipMalloc(tables)");
+    }
+    char* hp = host.data();
+    char* dp = static_cast<char*>(c->d_tables);
+    auto put_f = [&](const std::vector<double>& v) -> const float* {
+        float* h = carve<float>(hp, v.size());
+        float* d = carve<float>(dp, v.size());
+        for (size_t i = 0; i < v.size(); ++i) h[i] = (float)v[i];
+        return d;
+    };
+    auto put_i = [&](const std::vector<int>& v) -> const int* {
+        int* h = carve<int>(hp, v.size());
+        int* d = carve<int>(dp, v.size());
+        for (size_t i = 0; i < v.size(); ++i) h[i] = v[i];
+        return d;
+    };
+    c->d_vf = put_f(s.vf);
+    c->d_vft = put_f(s.vft);
+    c->d_dot = carve<double>(dp, 1);
+    carve<double>(hp, 1);
+    for (int g = 0; g + 1 < c->gm; ++g) {
+        c->d_rxs[g] = put_i(s.rx[g].start);
+        c->d_rxw[g] = put_f(s.rx[g].w);
+        c->d_rys[g] = put_i(s.ry[g].start);
+        c->d_ryw[g] = put_f(s.ry[g].w);
+        c->d_pxs[g] = put_i(s.px[g].start);
+        c->d_pxw[g] = put_f(s.px[g].w);
+        c->d_pys[g] = put_i(s.py[g].start);
+        c->d_pyw[g] = put_f(s.py[g].w);
+    }
+    e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
+    if (e == hipSuccess) e = hipMalloc(&c->ws_t, sizeof(float) * (size_t)s.dims.mg_size);
+    if (e != hipSuccess) {
+        int r = hip_fail(e, "device allocation");
+        mgbf_finalize(c);
+        return r;
+    }
+    *out = c;
+    return MGBF_OK;
+}
+
+int mgbf_finalize(mgbf_ctx* c) {
+    if (!c) return MGBF_OK;
+    DeviceGuard guard(c->device);
+    (void)hipDeviceSynchronize();
+    if (c->ws_u) (void)hipFree(c->ws_u);
+    if (c->ws_t) (void)hipFree(c->ws_t);
+    if (c->d_tables) (void)hipFree(c->d_tables);
+    delete c;
+    return MGBF_OK;
+}
+
+int mgbf_get_dims(const mgbf_ctx* c, mgbf_dims* dims) {
+    if (!c || !dims) return fail(MGBF_ERR_ARG, "NULL argument");
+    *dims = c->s.dims;
+    return MGBF_OK;
+}
+
+int mgbf_set_weights(mgbf_ctx* c, int gen, const float* field, double value) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    if (gen < 1 || gen > c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
+    c->wfield[gen - 1] = field;
+    if (!field) c->wconst[gen - 1] = (float)value;
+    return MGBF_OK;
+}
+
+int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
+    if (!c || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
+    DeviceGuard guard(c->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (op) {
+        case MGBF_OP_WAWT: return apply_wawt(c, in, out, st);
+        case MGBF_OP_WT: return apply_wt(c, in, out, false, st);
+        case MGBF_OP_W: return apply_w(c, in, out, false, st);
+        case MGBF_OP_A: return apply_weights(c, in, out, false, st);
+        case MGBF_OP_ASQRT: return apply_weights(c, in, out, true, st);
+        case MGBF_OP_BSQRT: return apply_w(c, in, out, true, st);
+        case MGBF_OP_BSQRTT: return apply_wt(c, in, out, true, st);
+        default: return fail(MGBF_ERR_ARG, "unknown operation");
+    }
+}
+
+int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* out,
+                     void* stream) {
+    if (!c || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
+    if (in == out) return fail(MGBF_ERR_ARG, "stages need distinct in and out");
+    DeviceGuard guard(c->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int g = gen - 1;
+    if (g < 0 || g >= c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
+    switch (stage) {
+        case MGBF_STAGE_VF:
+        case MGBF_STAGE_VFT:
+            if (g != 0) return fail(MGBF_ERR_ARG, "the vertical filter acts on generation 1");
+            return vertical(c, in, out, stage == MGBF_STAGE_VFT, st);
+        case MGBF_STAGE_F:
+        case MGBF_STAGE_FT:
+            return radial(c, g, in, out, nullptr, 1.0f, st);
+        case MGBF_STAGE_R:
+            if (g + 1 >= c->gm) return fail(MGBF_ERR_ARG, "no coarser generation");
+            return restrict_g(c, g, in, out, st);
+        case MGBF_STAGE_P:
+            if (g + 1 >= c->gm) return fail(MGBF_ERR_ARG, "no coarser generation");
+            return prolong_g(c, g, in, nullptr, out, st);
+        case MGBF_STAGE_A:
+            HIP_TRY(mgbf::launch_scale(in, out, c->wfield[g], c->wconst[g], 0, c->gsize[g], st),
+                    "scale launch");
+            return MGBF_OK;
+        default:
+            return fail(MGBF_ERR_ARG, "unknown stage");
+    }
+}
+
+int mgbf_dot(mgbf_ctx* c, const float* a, const float* b, long long n, double* result,
+             void* stream) {
+    if (!c || !a || !b || !result || n < 0) return fail(MGBF_ERR_ARG, "bad argument");
+    DeviceGuard guard(c->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    HIP_TRY(mgbf::launch_dot(a, b, n, c->d_dot, st), "dot launch");
+    HIP_TRY(hipMemcpyAsync(result, c->d_dot, sizeof(double), hipMemcpyDeviceToHost, st),
+            "dot copy");
+    HIP_TRY(hipStreamSynchronize(st), "dot sync");
+    return MGBF_OK;
+}
+
+}  // extern "C"

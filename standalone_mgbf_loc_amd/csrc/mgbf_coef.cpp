@@ -1,0 +1,362 @@
+// Host-side FP64 setup of the MGBF operator.
+//
+// Derived sizes follow init_mg_parameter / def_maxgen / def_ngens
+// (src.mgbf/mg_parameter.f90:179-635). Filter coefficients follow
+// def_mg_weights (src.mgbf/mg_intstate.f90:296-433): the aspect tensor is
+// turned into the Cholesky factor of its inverse (cholaspect2,
+// src.mgbf/jp_pbfil.f90:90-107), the taps are the lattice points the
+// reference's vrbeta2 loop visits (jp_pbfil.f90:909-926) and the line-sum
+// normalizer is getlinesum2 (jp_pbfil.f90:182-218). The transfer constants
+// are those of def_offset_coef (src.mgbf/mg_interpolate.f90:76-79) applied as
+// direct1 / adjoint do (src.mgbf/mg_generations.f90:1117-1250) with the
+// 2-point half-sample mirror of boco_2d / bocoT_2d.
+#include "mgbf_coef.hpp"
+
+#include <cmath>
+#include <cstdio>
+
+namespace mgbf {
+namespace {
+
+constexpr double kEps = 1.e-12;  // tap-boundary tolerance of jp_pbfil.f90
+
+const double kPCoef[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
+const double kQCoef[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
+
+double ipow(double x, int p) {
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= x;
+    return r;
+}
+
+std::string fmt(const char* f, int a = 0, int b = 0, int c = 0) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, f, a, b, c);
+    return buf;
+}
+
+// def_maxgen (mg_parameter.f90:578-611)
+int def_maxgen(int nxm, int nym) {
+    int npx = nxm, gx = 1;
+    for (;;) {
+        npx = (npx + 1) / 2;
+        gx += 1;
+        if (npx == 1) break;
+    }
+    int npy = nym, gy = 1;
+    for (;;) {
+        npy = (npy + 1) / 2;
+        gy += 1;
+        if (npy == 1) break;
+    }
+    return gx < gy ? gx : gy;
+}
+
+int mirror_half(int q, int n) {  // 0-based half-sample mirror
+    if (q < 0) return -1 - q;
+    if (q >= n) return 2 * n - 1 - q;
+    return q;
+}
+
+int mirror_whole(int q, int n) {  // 0-based whole-sample mirror
+    if (q < 0) return -q;
+    if (q >= n) return 2 * (n - 1) - q;
+    return q;
+}
+
+// Prolongation along one axis, fine nf <- coarse nf/2 (dense, FP64), with a
+// flag matrix of structural nonzeros.
+void interp_dense(int nf, std::vector<double>& P, std::vector<char>& nz) {
+    const int nc = nf / 2;
+    P.assign((size_t)nf * nc, 0.0);
+    nz.assign((size_t)nf * nc, 0);
+    for (int i1 = 1; i1 <= nf; ++i1) {
+        const int iL = i1 / 2;
+        const double* c = (i1 & 1) ? kQCoef : kPCoef;
+        for (int t = 0; t < 4; ++t) {
+            const int I1 = iL - 1 + t;                 // 1-based coarse, -1 .. nc+2
+            const int I0 = mirror_half(I1 - 1, nc);    // boco_2d(.,2,2) pad
+            P[(size_t)(i1 - 1) * nc + I0] += c[t];
+            nz[(size_t)(i1 - 1) * nc + I0] = 1;
+        }
+    }
+}
+
+// Packs the rows of a (rows x cols) banded matrix into a window table.
+bool pack_rows(int rows, int cols, const std::vector<double>& M, const std::vector<char>& nz,
+               bool transpose, int width, AxisTable& t) {
+    t.n_out = rows;
+    t.width = width;
+    t.start.assign(rows, 0);
+    t.w.assign((size_t)rows * width, 0.0);
+    for (int r = 0; r < rows; ++r) {
+        int lo = -1, hi = -1;
+        for (int c = 0; c < cols; ++c) {
+            const size_t e = transpose ? (size_t)c * rows + r : (size_t)r * cols + c;
+            if (nz[e]) {
+                if (lo < 0) lo = c;
+                hi = c;
+            }
+        }
+        if (lo < 0) {
+            lo = 0;
+            hi = -1;
+        }
+        if (hi - lo + 1 > width) return false;
+        // keep the window inside [0, cols)
+        int s = lo;
+        if (s + width > cols) s = cols - width;
+        if (s < 0) s = 0;
+        t.start[r] = s;
+        for (int c = lo; c <= hi; ++c) {
+            const size_t e = transpose ? (size_t)c * rows + r : (size_t)r * cols + c;
+            t.w[(size_t)r * width + (c - s)] = M[e];
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
+    taps.clear();
+    const int p = cfg.p;
+    const double rmom2_2 = 1.0 / std::sqrt(2.0 * p + 4.0);
+    // aspect tensor pasp2 with p_del = p_eps = 0 (mg_intstate.f90:316-319, 383-390)
+    const double a00 = cfg.mg_ampl02, a11 = cfg.mg_ampl02, a01 = 0.0;
+    // inverse, then its lower Cholesky factor
+    const double det = a00 * a11 - a01 * a01;
+    const double i00 = a11 / det, i11 = a00 / det, i10 = -a01 / det;
+    const double b00 = std::sqrt(i00);
+    const double b10 = i10 / b00;
+    const double b11 = std::sqrt(i11 - b10 * b10);
+    const double exx = b00 * rmom2_2, eyy = b11 * rmom2_2, eyx = b10 * rmom2_2;
+    const double y = 1.0 / eyy;
+    const int gyl = (int)std::ceil(-y + kEps), gym = (int)std::floor(y - kEps);
+    if (gyl < -cfg.hy || gym > cfg.hy) return false;
+    std::vector<double> rrc;
+    for (int gy = gyl; gy <= gym; ++gy) {
+        const double yy = gy;
+        const double xc = -yy * eyx;
+        const double rrx = (yy * eyy) * (yy * eyy);
+        const double x = std::sqrt(1.0 - rrx);
+        const int gxl = (int)std::ceil((xc - x) / exx + kEps);
+        const int gxm = (int)std::floor((xc + x) / exx - kEps);
+        if (gxl < -cfg.hx || gxm > cfg.hx) return false;
+        for (int gx = gxl; gx <= gxm; ++gx) {
+            const double t = gx * exx - xc;
+            taps.push_back({gx, gy, 0.0});
+            rrc.push_back(1.0 - (rrx + t * t));
+        }
+    }
+    double s = 0;
+    for (double r : rrc) s += ipow(r, p);
+    const double ss = 1.0 / s;
+    for (size_t t = 0; t < taps.size(); ++t) taps[t].w = ss * ipow(rrc[t], p);
+    return true;
+}
+
+bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w) {
+    gx.clear();
+    w.clear();
+    const int p = cfg.p;
+    const double rmom2_1 = 1.0 / std::sqrt(2.0 * p + 3.0);
+    const double el = 1.0 / std::sqrt(cfg.mg_ampl01);     // cholaspect1
+    const double exx = el * rmom2_1;
+    const double x = 1.0 / exx;
+    const int gxl = (int)std::ceil(-x + kEps), gxm = (int)std::floor(x - kEps);
+    if (gxl < -cfg.hz || gxm > cfg.hz) return false;
+    double s = 0;
+    for (int g = gxl; g <= gxm; ++g) {
+        const double rr = (g * exx) * (g * exx);
+        gx.push_back(g);
+        w.push_back(ipow(1.0 - rr, p));
+        s += w.back();
+    }
+    for (double& v : w) v /= s;
+    return true;
+}
+
+int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* err) {
+    d = mgbf_dims{};
+    if (c.nxPE < 1 || c.nyPE < 1 || c.im_filt < 1 || c.jm_filt < 1 || c.lm < 1 || c.n_ens < 1 ||
+        c.km3 < 0 || c.km2 < 0 || c.gm_max < 1 || c.p < 0 || c.hx < 0 || c.hy < 0 || c.hz < 0) {
+        *err = "invalid namelist value (sizes must be positive)";
+        return MGBF_ERR_CONFIG;
+    }
+    if (c.mgbf_proc != 1 && c.mgbf_proc != 6) {
+        *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 6 filtering_rad2_ens)",
+                   c.mgbf_proc);
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    if (c.mgbf_line) {
+        *err = "mgbf_line=.true. (dibeta line filters) is not built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    if (c.l_loc) {
+        *err = "l_loc=.true. (localization generations) is not built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    if (c.lquart) {
+        *err = "lquart=.true. (quadratic inter-generation transfer) is not built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    // km, km_all (mg_parameter.f90:293-297)
+    d.km = c.km2 + c.lm * c.km3;
+    if (c.mgbf_proc == 1) {
+        if (c.n_ens != 1) {
+            *err = "mgbf_proc=1 (background) needs n_ens=1";
+            return MGBF_ERR_CONFIG;
+        }
+        d.km_all = d.km;
+        d.km3_all = c.km3;
+    } else {
+        d.km_all = d.km * c.n_ens;
+        d.km3_all = c.km3 * c.n_ens;
+        if (c.km2 != 0 && c.l_vertical_filter) {
+            // sup_vrbeta1_ens strides the stack by lm only (mg_filtering.f90:2066-2069)
+            *err = "mgbf_proc=6 with km2>0 and l_vertical_filter: the reference's ensemble "
+                   "vertical filter assumes km2=0";
+            return MGBF_ERR_CONFIG;
+        }
+    }
+    if (d.km_all < 1) {
+        *err = "km_all must be positive";
+        return MGBF_ERR_CONFIG;
+    }
+    // analysis grid divisibility (mg_parameter.f90:447-455)
+    const int nm = c.nm0 / c.nxPE, mm = c.mm0 / c.nyPE;
+    if (nm * c.nxPE != c.nm0) {
+        *err = "nm0 is not divisible by nxm";
+        return MGBF_ERR_CONFIG;
+    }
+    if (mm * c.nyPE != c.mm0) {
+        *err = "mm0 is not divisible by nym";
+        return MGBF_ERR_CONFIG;
+    }
+    if (nm != c.im_filt || mm != c.jm_filt || c.lm_a != c.lm) {
+        *err = "analysis grid differs from the filter grid (nm0/nxPE != im_filt, mm0/nyPE != "
+               "jm_filt or lm_a != lm): the analysis-filter interpolation is not built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    // generations (mg_parameter.f90:322-328)
+    int gm = def_maxgen(c.nxPE, c.nyPE);
+    if (gm > c.gm_max) gm = c.gm_max;
+    if (gm > kMaxGen) {
+        *err = fmt("gm=%d exceeds the built maximum %d", gm, kMaxGen);
+        return MGBF_ERR_CONFIG;
+    }
+    d.gm = gm;
+    d.nx = c.nxPE * c.im_filt;
+    d.ny = c.nyPE * c.jm_filt;
+    long long fs = 0, ms = 0;
+    const int hmax = c.hx > c.hy ? c.hx : c.hy;
+    for (int g = 0; g < gm; ++g) {
+        const int n = d.nx >> g, m = d.ny >> g;
+        if (g + 1 < gm && ((n & 1) || (m & 1))) {
+            *err = fmt("generation %d grid %dx%d is not even: cannot restrict", g + 1, n, m);
+            return MGBF_ERR_CONFIG;
+        }
+        if (n < hmax || m < hmax || n < 2 || m < 2) {
+            *err = fmt("generation %d grid %dx%d is smaller than the halo", g + 1, n, m);
+            return MGBF_ERR_CONFIG;
+        }
+        nx[g] = n;
+        ny[g] = m;
+        const long long sz = (long long)d.km_all * n * m;
+        if (g == 0) fs = sz;
+        ms += sz;
+    }
+    d.field_size = fs;
+    d.mg_size = ms;
+    if (c.l_vertical_filter && c.lm <= c.hz) {
+        *err = "lm must exceed hz for the vertical mirror";
+        return MGBF_ERR_CONFIG;
+    }
+    return MGBF_OK;
+}
+
+int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
+    s = Setup{};
+    s.cfg = cfg;
+    int st = derive(cfg, s.dims, s.nx, s.ny, err);
+    if (st != MGBF_OK) return st;
+    if (!radial_taps(cfg, s.taps2)) {
+        *err = "In getlinesum2; filter reach becomes too large for hx/hy";
+        return MGBF_ERR_CONFIG;
+    }
+    if (!vertical_taps(cfg, s.taps1_gx, s.taps1_w)) {
+        *err = "In getlinesum1; filter reach fx becomes too large for hz";
+        return MGBF_ERR_CONFIG;
+    }
+    s.dims.ntaps2 = (int)s.taps2.size();
+    s.dims.ntaps1 = (int)s.taps1_gx.size();
+    s.H = cfg.hx > cfg.hy ? cfg.hx : cfg.hy;
+    if (s.H > kMaxHalo || s.H < 1) {
+        *err = fmt("radial half-width %d outside the built range 1..%d", s.H, kMaxHalo);
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    // fold the taps into w(|a|,|b|); the filter must be even in a and in b
+    bool seen[kMaxHalo + 1][kMaxHalo + 1] = {};
+    for (const Tap2& t : s.taps2) {
+        const int a = t.gx < 0 ? -t.gx : t.gx, b = t.gy < 0 ? -t.gy : t.gy;
+        if (seen[b][a]) {
+            if (std::fabs(s.wsym[b][a] - t.w) > 1e-15 * std::fabs(t.w)) {
+                *err = "radial taps are not mirror symmetric";
+                return MGBF_ERR_UNSUPPORTED;
+            }
+        }
+        seen[b][a] = true;
+        s.wsym[b][a] = t.w;
+    }
+    // exact lattice disc a^2 + b^2 < (H+1)^2 ?
+    {
+        const int R2 = (s.H + 1) * (s.H + 1);
+        bool exact = cfg.hx == cfg.hy;
+        for (int b = 0; b <= s.H && exact; ++b)
+            for (int a = 0; a <= s.H; ++a)
+                if (seen[b][a] != (a * a + b * b < R2)) exact = false;
+        s.exact_disc = exact;
+    }
+    // vertical band matrices (sup_vrbeta1_ens / sup_vrbeta1T_ens)
+    {
+        const int lm = cfg.lm, hz = cfg.hz, W = 2 * hz + 1;
+        std::vector<double> V((size_t)lm * lm, 0.0);
+        for (int L = 0; L < lm; ++L)
+            for (size_t t = 0; t < s.taps1_gx.size(); ++t) {
+                const int src = mirror_whole(L + s.taps1_gx[t], lm);
+                V[(size_t)L * lm + src] += s.taps1_w[t];
+            }
+        s.vf.assign((size_t)lm * W, 0.0);
+        s.vft.assign((size_t)lm * W, 0.0);
+        for (int L = 0; L < lm; ++L)
+            for (int t = -hz; t <= hz; ++t) {
+                const int c = L + t;
+                if (c < 0 || c >= lm) continue;
+                s.vf[(size_t)L * W + (t + hz)] = V[(size_t)L * lm + c];
+                s.vft[(size_t)L * W + (t + hz)] = V[(size_t)c * lm + L];
+            }
+    }
+    // transfer tables for each generation pair g -> g+1
+    for (int g = 0; g + 1 < s.dims.gm; ++g) {
+        std::vector<double> P;
+        std::vector<char> nz;
+        for (int axis = 0; axis < 2; ++axis) {
+            const int nf = axis == 0 ? s.nx[g] : s.ny[g];
+            interp_dense(nf, P, nz);
+            AxisTable& pt = axis == 0 ? s.px[g] : s.py[g];
+            AxisTable& rt = axis == 0 ? s.rx[g] : s.ry[g];
+            if (!pack_rows(nf, nf / 2, P, nz, false, kPWidth, pt) ||
+                !pack_rows(nf / 2, nf, P, nz, true, kRWidth, rt)) {
+                *err = "transfer stencil wider than its table";
+                return MGBF_ERR_CONFIG;
+            }
+        }
+    }
+    // generation weights (mg_intstate.f90:333-351): g >= 5 share mg_weig5
+    const double wv[5] = {cfg.mg_weig1, cfg.mg_weig2, cfg.mg_weig3, cfg.mg_weig4, cfg.mg_weig5};
+    for (int g = 0; g < s.dims.gm; ++g) s.wconst[g] = wv[g < 4 ? g : 4];
+    return MGBF_OK;
+}
+
+}  // namespace mgbf

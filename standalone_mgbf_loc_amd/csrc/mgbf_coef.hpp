@@ -1,0 +1,62 @@
+// Host-side FP64 setup of the MGBF operator: derived sizes, filter taps,
+// vertical band matrices and inter-generation transfer tables.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "mgbf.h"
+
+namespace mgbf {
+
+constexpr int kMaxGen = 12;
+constexpr int kMaxHalo = 10;      // largest radial half-width with a built kernel
+constexpr int kRWidth = 8;        // restriction window per coarse index
+constexpr int kPWidth = 4;        // prolongation window per fine index
+
+// One tap of a beta filter (offsets from the output point, FP64 weight).
+struct Tap2 {
+    int gx, gy;
+    double w;
+};
+
+// 1-D transfer table along one axis of one generation pair: for every output
+// index a window start and kRWidth (restriction) or kPWidth (prolongation)
+// weights; entries beyond the structural nonzeros are 0.
+struct AxisTable {
+    int n_out = 0, width = 0;
+    std::vector<int> start;
+    std::vector<double> w;   // n_out * width
+};
+
+struct Setup {
+    mgbf_config cfg{};
+    mgbf_dims dims{};
+    int nx[kMaxGen] = {}, ny[kMaxGen] = {};
+    std::vector<Tap2> taps2;           // reference loop order
+    std::vector<int> taps1_gx;         // vertical taps
+    std::vector<double> taps1_w;
+    int H = 0;                          // max(hx, hy) used by the radial kernels
+    double wsym[kMaxHalo + 1][kMaxHalo + 1] = {};  // w(|a|,|b|) as [b][a]
+    bool exact_disc = false;            // taps = lattice disc a^2+b^2 < (H+1)^2
+    std::vector<double> vf, vft;       // lm x (2hz+1) band of Vf and Vf^T
+    AxisTable rx[kMaxGen], ry[kMaxGen];   // restriction g -> g+1 (coarse rows)
+    AxisTable px[kMaxGen], py[kMaxGen];   // prolongation g+1 -> g (fine rows)
+    double wconst[kMaxGen] = {};
+};
+
+// init_mg_parameter-style validation and derived sizes. Returns MGBF_OK or
+// an error status with a message in *err.
+int derive(const mgbf_config& cfg, mgbf_dims& dims, int* nx, int* ny, std::string* err);
+
+// Full FP64 setup (def_offset_coef + def_mg_weights). Returns MGBF_OK or an
+// error status with a message in *err.
+int build_setup(const mgbf_config& cfg, Setup& s, std::string* err);
+
+// Radial taps of the isotropic / diagonal-aspect beta filter (cholaspect2 +
+// getlinesum2 + the vrbeta2 tap loop). Returns false if the reach exceeds
+// the halo (the reference stops there).
+bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps);
+bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w);
+
+}  // namespace mgbf

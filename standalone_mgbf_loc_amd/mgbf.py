@@ -1,0 +1,334 @@
+"""ctypes binding of libmgbf (the C ABI in include/mgbf.h).
+
+Mirrors the reference's operator surface for the localization path:
+`Config` carries every key of the namelist &PARAMETERS_MGBETA
+(src.mgbf/mg_parameter.f90:190-208); `MGBF(config)` is mg_initialize
+(src.mgbf/mg_entrymod.f90:36-219) and `MGBF.close()` is mg_finalize
+(:222-273); `MGBF.filtering_procedure(x)` applies W A W^T as
+filtering_procedure(mgbf_proc) does (src.mgbf/mg_filtering.f90:67-120).
+Fields are torch tensors on the HIP device, float32, shape (ny, nx, K) in C
+order (k fastest), the memory order of the reference's VALL(k, i, j).
+
+There is no CPU fallback: if libmgbf.so is missing or no HIP device is
+present, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import Optional
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgbf.so")
+
+MGBF_OK = 0
+OP_WAWT, OP_WT, OP_W, OP_A, OP_ASQRT, OP_BSQRT, OP_BSQRTT = range(7)
+STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_R, STAGE_P, STAGE_A = range(7)
+
+EXPORTED_SYMBOLS = (
+    "mgbf_config_default", "mgbf_derive", "mgbf_init", "mgbf_finalize", "mgbf_get_dims",
+    "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
+    "mgbf_version",
+)
+
+
+class MGBFError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"mgbf status {status}: {msg}")
+        self.status = status
+
+
+_DOUBLE_KEYS = ("mg_ampl01", "mg_ampl02", "mg_ampl03",
+                "mg_weig1", "mg_weig2", "mg_weig3", "mg_weig4", "mg_weig5")
+_INT_KEYS = ("hx", "hy", "hz", "p", "mgbf_line", "mgbf_proc", "n_ens", "lm_a", "lm", "km2", "km3",
+             "l_loc", "l_loc_vertical", "l_filt_g1", "l_lin_vertical", "l_lin_horizontal",
+             "l_quad_horizontal", "l_new_map", "l_vertical_filter", "ldelta", "lquart", "lhelm",
+             "gm_max", "nm0", "mm0", "nxPE", "nyPE", "im_filt", "jm_filt")
+
+
+class _CConfig(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_double) for k in _DOUBLE_KEYS] + [(k, ctypes.c_int) for k in _INT_KEYS]
+
+
+class _CDims(ctypes.Structure):
+    _fields_ = [("gm", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("km", ctypes.c_int), ("km_all", ctypes.c_int), ("km3_all", ctypes.c_int),
+                ("ntaps2", ctypes.c_int), ("ntaps1", ctypes.c_int),
+                ("field_size", ctypes.c_longlong), ("mg_size", ctypes.c_longlong)]
+
+
+@dataclasses.dataclass
+class Config:
+    """Every key of &PARAMETERS_MGBETA (defaults: a 4-generation ensemble run)."""
+    mg_ampl01: float = 16.0 / 7.0
+    mg_ampl02: float = 49.0 / 8.0
+    mg_ampl03: float = 1.0
+    mg_weig1: float = 1.0
+    mg_weig2: float = 0.5
+    mg_weig3: float = 0.25
+    mg_weig4: float = 0.125
+    mg_weig5: float = 0.0625
+    hx: int = 6
+    hy: int = 6
+    hz: int = 3
+    p: int = 2
+    mgbf_line: bool = False
+    mgbf_proc: int = 6
+    n_ens: int = 1
+    lm_a: int = 32
+    lm: int = 32
+    km2: int = 0
+    km3: int = 1
+    l_loc: bool = False
+    l_loc_vertical: bool = False
+    l_filt_g1: bool = True
+    l_lin_vertical: bool = False
+    l_lin_horizontal: bool = False
+    l_quad_horizontal: bool = False
+    l_new_map: bool = True
+    l_vertical_filter: bool = True
+    ldelta: bool = False
+    lquart: bool = False
+    lhelm: bool = False
+    gm_max: int = 4
+    nm0: int = 192
+    mm0: int = 192
+    nxPE: int = 8
+    nyPE: int = 8
+    im_filt: int = 24
+    jm_filt: int = 24
+
+    @classmethod
+    def for_grid(cls, nx: int, ny: int, lm: int, n_ens: int = 1, gm: int = 4, h: int = 6,
+                 hz: int = 3, p: int = 2, weights=None, **kw) -> "Config":
+        """A configuration whose global grid is nx x ny with gm generations.
+
+        The reference derives the grid from nxPE*im_filt and caps gm by the PE
+        grid (def_maxgen); this picks the smallest power-of-two PE grid that
+        allows gm generations, and sets the aspect amplitudes so that the
+        filter reach equals the halo (mg_ampl02 = (h+1)^2/(2p+4),
+        mg_ampl01 = (hz+1)^2/(2p+3)).
+        """
+        pe = 1
+        while _def_maxgen(pe, pe) < gm:
+            pe *= 2
+        if nx % pe or ny % pe:
+            raise ValueError(f"grid {nx}x{ny} not divisible by the {pe}x{pe} PE grid needed for gm={gm}")
+        w = list(weights) if weights is not None else [2.0 ** -g for g in range(5)]
+        w = (w + [w[-1]] * 5)[:5]
+        c = cls(mg_ampl01=(hz + 1) ** 2 / (2 * p + 3), mg_ampl02=(h + 1) ** 2 / (2 * p + 4),
+                mg_weig1=w[0], mg_weig2=w[1], mg_weig3=w[2], mg_weig4=w[3], mg_weig5=w[4],
+                hx=h, hy=h, hz=hz, p=p, n_ens=n_ens, lm_a=lm, lm=lm, gm_max=gm,
+                nm0=nx, mm0=ny, nxPE=pe, nyPE=pe, im_filt=nx // pe, jm_filt=ny // pe)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def to_c(self) -> _CConfig:
+        c = _CConfig()
+        for k in _DOUBLE_KEYS:
+            setattr(c, k, float(getattr(self, k)))
+        for k in _INT_KEYS:
+            setattr(c, k, int(getattr(self, k)))
+        return c
+
+    def weights(self):
+        return [self.mg_weig1, self.mg_weig2, self.mg_weig3, self.mg_weig4, self.mg_weig5]
+
+
+def _def_maxgen(nxm: int, nym: int) -> int:
+    """def_maxgen (src.mgbf/mg_parameter.f90:578-611)."""
+    def one(n):
+        g = 1
+        while True:
+            n = (n + 1) // 2
+            g += 1
+            if n == 1:
+                return g
+    return min(one(nxm), one(nym))
+
+
+_lib = None
+
+
+def load_library(path: Optional[str] = None):
+    """Loads libmgbf.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is missing: build it with `python -m standalone_mgbf_loc_amd.build`")
+    L = ctypes.CDLL(p)
+    cfgp = ctypes.POINTER(_CConfig)
+    L.mgbf_config_default.argtypes = [cfgp]
+    L.mgbf_derive.argtypes = [cfgp, ctypes.POINTER(_CDims)]
+    L.mgbf_init.argtypes = [cfgp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.mgbf_finalize.argtypes = [ctypes.c_void_p]
+    L.mgbf_get_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(_CDims)]
+    L.mgbf_set_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
+    L.mgbf_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p]
+    L.mgbf_apply_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    L.mgbf_dot.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                           ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    L.mgbf_last_error.restype = ctypes.c_char_p
+    L.mgbf_version.restype = ctypes.c_char_p
+    if path is None:
+        _lib = L
+    return L
+
+
+def _check(status: int):
+    if status != MGBF_OK:
+        raise MGBFError(status, load_library().mgbf_last_error().decode())
+
+
+@dataclasses.dataclass
+class Dims:
+    gm: int
+    nx: int
+    ny: int
+    km: int
+    km_all: int
+    km3_all: int
+    ntaps2: int
+    ntaps1: int
+    field_size: int
+    mg_size: int
+
+    def gen_shape(self, g: int):
+        """(ny_g, nx_g, K) of generation g (1-based)."""
+        return (self.ny >> (g - 1), self.nx >> (g - 1), self.km_all)
+
+
+def derive(cfg: Config) -> Dims:
+    """Validated derived sizes of a configuration (no device use)."""
+    L = load_library()
+    d = _CDims()
+    c = cfg.to_c()
+    _check(L.mgbf_derive(ctypes.byref(c), ctypes.byref(d)))
+    return Dims(*[getattr(d, f) for f, _ in _CDims._fields_])
+
+
+def _ptr(t) -> int:
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if t.device.type != "cuda" or t.dtype != torch.float32 or not t.is_contiguous():
+        raise ValueError("fields must be contiguous float32 tensors on the HIP device")
+    return t.data_ptr()
+
+
+def _stream(stream) -> int:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+class MGBF:
+    """One operator context (mg_initialize ... mg_finalize) on one device."""
+
+    def __init__(self, cfg: Config, device: int = 0):
+        self.cfg = cfg
+        self.device = device
+        self._L = load_library()
+        c = cfg.to_c()
+        h = ctypes.c_void_p()
+        _check(self._L.mgbf_init(ctypes.byref(c), int(device), ctypes.byref(h)))
+        self._h = h
+        d = _CDims()
+        _check(self._L.mgbf_get_dims(self._h, ctypes.byref(d)))
+        self.dims = Dims(*[getattr(d, f) for f, _ in _CDims._fields_])
+        self._wkeep = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mgbf_finalize(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- shapes ---
+    def field_shape(self):
+        return self.dims.gen_shape(1)
+
+    def mg_numel(self) -> int:
+        return self.dims.mg_size
+
+    def split_mg(self, v):
+        """Views of the generation fields of a flat multigrid vector."""
+        out, off = [], 0
+        for g in range(1, self.dims.gm + 1):
+            s = self.dims.gen_shape(g)
+            n = s[0] * s[1] * s[2]
+            out.append(v[off:off + n].view(s))
+            off += n
+        return out
+
+    # --- weights ---
+    def set_weights(self, gen: int, field=None, value: Optional[float] = None):
+        """Per-point weight field (a_diff) or constant for generation gen (1-based)."""
+        if field is not None:
+            if tuple(field.shape) != self.dims.gen_shape(gen):
+                raise ValueError("weight field shape mismatch")
+            self._wkeep[gen] = field
+            _check(self._L.mgbf_set_weights(self._h, gen, _ptr(field), 0.0))
+        else:
+            self._wkeep.pop(gen, None)
+            v = value if value is not None else self.cfg.weights()[min(gen, 5) - 1]
+            _check(self._L.mgbf_set_weights(self._h, gen, None, float(v)))
+
+    # --- operators ---
+    def apply(self, op: int, x, out=None, stream=None):
+        import torch
+        if op in (OP_WAWT, OP_W, OP_BSQRT):
+            oshape = self.field_shape()
+        else:
+            oshape = (self.dims.mg_size,)
+        if out is None:
+            out = torch.empty(oshape, dtype=torch.float32, device=x.device)
+        _check(self._L.mgbf_apply(self._h, op, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def filtering_procedure(self, x, out=None, stream=None):
+        """y = W A W^T x (filtering_rad2_ens / filtering_rad2_bckg)."""
+        return self.apply(OP_WAWT, x, out, stream)
+
+    def stage(self, stage: int, gen: int, x, out=None, stream=None):
+        import torch
+        if stage == STAGE_R:
+            oshape = self.dims.gen_shape(gen + 1)
+        elif stage == STAGE_P:
+            oshape = self.dims.gen_shape(gen)
+        else:
+            oshape = self.dims.gen_shape(gen)
+        if out is None:
+            out = torch.empty(oshape, dtype=torch.float32, device=x.device)
+        _check(self._L.mgbf_apply_stage(self._h, stage, gen, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def dot(self, a, b, stream=None) -> float:
+        """FP64-accumulated <a, b> on the device."""
+        if a.numel() != b.numel():
+            raise ValueError("size mismatch")
+        r = ctypes.c_double()
+        _check(self._L.mgbf_dot(self._h, _ptr(a), _ptr(b), a.numel(), ctypes.byref(r),
+                                _stream(stream)))
+        return r.value
+
+
+def version() -> str:
+    return load_library().mgbf_version().decode()

@@ -1,0 +1,102 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/mgbf.h declares, and derives/validates configurations the way
+init_mg_parameter does (src.mgbf/mg_parameter.f90:179-635)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from standalone_mgbf_loc_amd import mgbf as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "mgbf.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(mgbf_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = M.load_library()
+    names = _declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(M.EXPORTED_SYMBOLS)
+
+
+def test_version_and_default_config():
+    L = M.load_library()
+    assert L.mgbf_version().decode().startswith("mgbf")
+    c = M._CConfig()
+    assert L.mgbf_config_default(ctypes.byref(c)) == 0
+    assert c.mgbf_proc == 6 and c.hx == 6 and c.hz == 3 and c.p == 2
+
+
+def test_derive_config1_needs_8x8_pes_for_4_generations():
+    # 192x192x32 with 4 generations; 2x2 PEs would cap gm at 2 (def_maxgen)
+    cfg = M.Config.for_grid(192, 192, 32, gm=4)
+    d = M.derive(cfg)
+    assert (d.gm, d.nx, d.ny, d.km_all) == (4, 192, 192, 32)
+    assert d.ntaps2 == 145 and d.ntaps1 == 7
+    assert d.mg_size == 32 * (192 * 192 + 96 * 96 + 48 * 48 + 24 * 24)
+    cfg2 = M.Config.for_grid(192, 192, 32, gm=4)
+    cfg2.nxPE = cfg2.nyPE = 2
+    cfg2.im_filt = cfg2.jm_filt = 96
+    assert M.derive(cfg2).gm == 2
+
+
+@pytest.mark.parametrize("pe,gm", [(1, 2), (2, 2), (4, 3), (5, 4), (8, 4), (9, 5), (16, 5), (17, 6), (32, 6)])
+def test_def_maxgen_table(pe, gm):
+    assert M._def_maxgen(pe, pe) == gm
+
+
+def test_ensemble_stacks_members_in_k():
+    cfg = M.Config.for_grid(768, 768, 64, n_ens=80, gm=4)
+    d = M.derive(cfg)
+    assert d.km_all == 80 * 64 and d.km3_all == 80
+    assert d.field_size == 80 * 64 * 768 * 768
+
+
+def _err(cfg):
+    with pytest.raises(M.MGBFError) as e:
+        M.derive(cfg)
+    return e.value
+
+
+def test_reach_beyond_halo_is_refused():
+    cfg = M.Config.for_grid(192, 192, 32)
+    cfg.mg_ampl02 = 64 / 8.0  # reach 8 > hx 6
+    assert "too large" in str(_err(cfg))
+
+
+def test_nm0_divisibility_is_checked():
+    cfg = M.Config.for_grid(192, 192, 32)
+    cfg.nm0 = 190
+    assert "not divisible" in str(_err(cfg))
+
+
+@pytest.mark.parametrize("key,val", [("mgbf_line", True), ("l_loc", True), ("lquart", True),
+                                     ("mgbf_proc", 8)])
+def test_options_not_built_are_refused(key, val):
+    cfg = M.Config.for_grid(192, 192, 32)
+    setattr(cfg, key, val)
+    assert _err(cfg).status == -4
+
+
+def test_ensemble_vertical_filter_with_2d_variables_is_refused():
+    cfg = M.Config.for_grid(192, 192, 32, n_ens=2)
+    cfg.km2 = 4
+    assert _err(cfg).status == -2
+    cfg.mgbf_proc = 1
+    cfg.n_ens = 1
+    assert M.derive(cfg).km_all == 36
+
+
+def test_init_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(M.MGBFError):
+        M.MGBF(M.Config.for_grid(192, 192, 32))

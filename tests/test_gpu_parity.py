@@ -1,0 +1,247 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerance (FP32 product vs FP64 oracle): normwise relative error
+max|y - y64| / max|y64| <= 1e-5 for whole operators, 2e-6 for single stages;
+adjoint dot-product mismatch |<Wv,x> - <v,W^T x>| <= 1e-6 * |<Wv,x>|.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O
+from standalone_mgbf_loc_amd import mgbf as M
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+TOL_OP = 1e-5
+TOL_STAGE = 2e-6
+
+
+def relerr(y, y64):
+    y = y.detach().cpu().double().numpy() if isinstance(y, torch.Tensor) else y
+    return float(np.max(np.abs(y - y64)) / max(np.max(np.abs(y64)), 1e-300))
+
+
+def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
+    bckg = cfg.mgbf_proc == 1
+    return O.OracleParams(
+        nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx, hy=cfg.hy,
+        hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+        l_filt_g1=True if bckg else bool(cfg.l_filt_g1),
+        l_vertical_filter=bool(cfg.l_vertical_filter),
+        weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
+
+
+def field(shape, seed):
+    x = np.random.default_rng(seed).standard_normal(shape).astype(np.float32)
+    return x
+
+
+def to_dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(DEV)
+
+
+def sync():
+    torch.cuda.synchronize()
+
+
+# --------------------------------------------------------------------------
+# single stages
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("h,K,n,gen", [(6, 64, 96, 1), (6, 40, 96, 3), (3, 130, 48, 2),
+                                       (8, 64, 128, 1), (10, 24, 160, 1), (4, 70, 64, 1)])
+def test_radial_stage_matches_oracle(h, K, n, gen):
+    cfg = M.Config.for_grid(n, n, K, gm=3, h=h, hz=min(3, K - 1))
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(gen), 100 + gen)
+        y = op.stage(M.STAGE_F, gen, to_dev(x))
+        yt = op.stage(M.STAGE_FT, gen, to_dev(x))
+        sync()
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.radial(P, x64)) <= TOL_STAGE
+        assert relerr(yt, O.radial(P, x64, adjoint=True)) <= TOL_STAGE
+
+
+def test_radial_stage_on_a_rectangle_with_halo_sized_generation():
+    # the coarsest generation (12 x 6) is as narrow as the halo allows
+    cfg = M.Config(mg_ampl01=9 / 7, mg_ampl02=49 / 8, hx=6, hy=6, hz=2, n_ens=2, lm_a=4, lm=4,
+                   gm_max=4, nm0=96, mm0=48, nxPE=8, nyPE=8, im_filt=12, jm_filt=6)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.gm == 4 and d.gen_shape(4) == (6, 12, 8)
+        P = oparams(cfg, d)
+        for gen in (1, 4):
+            x = field(d.gen_shape(gen), 7 + gen)
+            y = op.stage(M.STAGE_F, gen, to_dev(x))
+            sync()
+            assert relerr(y, O.radial(P, x.astype(np.float64))) <= TOL_STAGE
+
+
+@pytest.mark.parametrize("lm,n_ens,hz", [(32, 2, 3), (64, 1, 3), (16, 5, 4), (7, 3, 2)])
+def test_vertical_stage_matches_oracle(lm, n_ens, hz):
+    cfg = M.Config.for_grid(48, 40, lm, n_ens=n_ens, gm=2, h=4, hz=hz)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 11)
+        y = op.stage(M.STAGE_VF, 1, to_dev(x))
+        yt = op.stage(M.STAGE_VFT, 1, to_dev(x))
+        sync()
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.vertical(P, x64)) <= TOL_STAGE
+        assert relerr(yt, O.vertical(P, x64, adjoint=True)) <= TOL_STAGE
+
+
+def test_vertical_stage_passes_2d_variables_through():
+    # background procedure with km2 = 3 two-dimensional variables after the 3-D ones
+    cfg = M.Config.for_grid(32, 32, 8, gm=2, h=3, hz=2, km2=3, km3=2, mgbf_proc=1)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.km_all == 19
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 12)
+        y = op.stage(M.STAGE_VF, 1, to_dev(x))
+        sync()
+        ref = O.vertical(P, x.astype(np.float64))
+        assert relerr(y, ref) <= TOL_STAGE
+        assert np.array_equal(y.cpu().numpy()[..., 16:], x[..., 16:])
+
+
+@pytest.mark.parametrize("K,n,m", [(64, 96, 96), (33, 64, 48), (128, 32, 16)])
+def test_transfer_stages_match_oracle(K, n, m):
+    cfg = M.Config(mg_ampl01=16 / 7, mg_ampl02=16 / 8, hx=3, hy=3, hz=3, lm_a=K, lm=K,
+                   gm_max=3, nm0=n, mm0=m, nxPE=4, nyPE=4, im_filt=n // 4, jm_filt=m // 4)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        for gen in (1, 2):
+            xf = field(d.gen_shape(gen), 20 + gen)
+            xc = field(d.gen_shape(gen + 1), 30 + gen)
+            r = op.stage(M.STAGE_R, gen, to_dev(xf))
+            p = op.stage(M.STAGE_P, gen, to_dev(xc))
+            sync()
+            assert relerr(r, O.restrict(xf.astype(np.float64))) <= TOL_STAGE
+            assert relerr(p, O.prolong(xc.astype(np.float64))) <= TOL_STAGE
+
+
+# --------------------------------------------------------------------------
+# whole operator
+# --------------------------------------------------------------------------
+
+def impulses_plus_noise(shape, seed_noise, seed_imp, count=16):
+    x = field(shape, seed_noise)
+    rng = np.random.default_rng(seed_imp)
+    m, n, K = shape
+    for _ in range(count):
+        x[rng.integers(m), rng.integers(n), rng.integers(K)] += 1.0
+    return x
+
+
+def test_wawt_192x192x32_four_generations_matches_oracle():
+    # 192x192 grid, 32 levels, 4 generations, radial half-width 6, vertical 3,
+    # p=2, weights [1, .5, .25, .125]; N(0,1) plus 16 unit impulses
+    cfg = M.Config.for_grid(192, 192, 32, gm=4, h=6, hz=3, weights=[1, .5, .25, .125])
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = impulses_plus_noise(d.gen_shape(1), 1, 2)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        ref = O.wawt(oparams(cfg, d), x.astype(np.float64))
+        assert relerr(y, ref) <= TOL_OP
+
+
+@pytest.mark.parametrize("n_ens,lm,gm,h,filt_g1", [(3, 16, 3, 6, True), (2, 20, 4, 4, False),
+                                                  (1, 64, 2, 8, True)])
+def test_wawt_ensemble_matches_oracle(n_ens, lm, gm, h, filt_g1):
+    cfg = M.Config.for_grid(96, 64, lm, n_ens=n_ens, gm=gm, h=h, hz=3,
+                            weights=[1.0, 0.7, 0.4, 0.2], l_filt_g1=filt_g1)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = field(d.gen_shape(1), 5 + n_ens)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+
+
+def test_wawt_background_procedure_with_2d_variables():
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=2, km2=3, km3=2, mgbf_proc=1)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = field(d.gen_shape(1), 40)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+
+
+def test_wawt_per_point_weights_match_oracle():
+    cfg = M.Config.for_grid(64, 64, 16, n_ens=2, gm=3, h=6, hz=3)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        rng = np.random.default_rng(9)
+        wf = [(0.5 + rng.random(d.gen_shape(g))).astype(np.float32) for g in range(1, d.gm + 1)]
+        keep = [to_dev(w) for w in wf]
+        for g, w in enumerate(keep, start=1):
+            op.set_weights(g, field=w)
+        x = field(d.gen_shape(1), 41)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        ref = O.wawt(oparams(cfg, d), x.astype(np.float64), wfield=[w.astype(np.float64) for w in wf])
+        assert relerr(y, ref) <= TOL_OP
+
+
+def test_wawt_in_place_and_linear_and_symmetric():
+    cfg = M.Config.for_grid(128, 128, 32, gm=4, h=6, hz=3)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        a = to_dev(field(d.gen_shape(1), 50))
+        b = to_dev(field(d.gen_shape(1), 51))
+        Ba = op.filtering_procedure(a)
+        Bb = op.filtering_procedure(b)
+        c = (2 * a - 3 * b).contiguous()
+        Bc = op.filtering_procedure(c, out=c)      # in place
+        sync()
+        lin = (Bc - (2 * Ba - 3 * Bb)).abs().max().item() / Bc.abs().max().item()
+        assert lin < 1e-5
+        lhs, rhs = op.dot(a, Bb), op.dot(Ba, b)
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), 1e-30) + 1e-9
+
+
+def test_w_and_wt_match_oracle_and_pass_the_dot_product_test():
+    cfg = M.Config.for_grid(96, 96, 24, n_ens=2, gm=3, h=6, hz=3)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 60)
+        v = np.random.default_rng(61).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs = op.dot(w, xd)
+        rhs = op.dot(vd, wt)
+        assert abs(lhs - rhs) <= 1e-6 * abs(lhs)
+        # B = (W A^1/2)(W A^1/2)^T reproduces W A W^T
+        s = op.apply(M.OP_BSQRTT, xd)
+        y2 = op.apply(M.OP_BSQRT, s)
+        y1 = op.filtering_procedure(xd)
+        sync()
+        assert (y1 - y2).abs().max().item() <= 1e-5 * y1.abs().max().item()
+
+
+def test_dot_is_fp64_accumulated():
+    cfg = M.Config.for_grid(64, 64, 8, gm=2, h=3, hz=2)
+    with M.MGBF(cfg) as op:
+        n = 3_000_001
+        a = torch.full((n,), 1.0 + 2.0 ** -20, device=DEV)
+        b = torch.ones(n, device=DEV)
+        r = op.dot(a, b)
+        assert abs(r - n * (1.0 + 2.0 ** -20)) < 1e-6
