@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the MGBF localization operator W A W^T on one MI355X.
+
+Workload: batched ensemble localization, FP32, 80 members x 768 x 768 x 64
+levels, 4 generations, radial half-width 6, vertical half-width 3, p = 2,
+generation weights [1, 1/2, 1/4, 1/8]; member m is an independent seeded
+N(0,1) field (seed 100+m). One step = one application of W A W^T to the whole
+ensemble (members stacked in k, as the reference stacks them in VALL), with
+inputs resident in HBM. The plain form (one member at a time) is timed beside
+it. The value is gridpoint.members per second of the batched form.
+
+Also reported: the maximum relative error of member 0 against an FP64 run
+(max|y-y64|/max|y64|), the roofline of the dominant kernel (the radial
+filter) from device-event timers around its launches, a device copy
+bandwidth, and the FP64 CPU restatement of the algorithm timed on member 0
+on this host's cores.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "gridpoint·members/sec for W·Wᵀ apply (batched vs plain GPU) at rel err ≤1e-5"
+
+
+def bytes_per_unit(gm: int) -> float:
+    """Two-sweep minimum HBM traffic of W A W^T per generation-1 unit (FP32)."""
+    S = sum(4.0 ** -(g - 1) for g in range(1, gm + 1))
+    return 4.0 * (4 * S + 2 * (S - 1))
+
+
+def make_members(torch, M, n, lm, seed0=100):
+    x = torch.empty((n, n, M * lm), dtype=torch.float32, device="cuda")
+    v = x.view(n, n, M, lm)
+    g = torch.Generator(device="cuda")
+    for m in range(M):
+        g.manual_seed(seed0 + m)
+        v[:, :, m, :] = torch.randn((n, n, lm), generator=g, device="cuda", dtype=torch.float32)
+    return x
+
+
+def stream_copy_gbps(torch, nbytes=4 << 30, reps=5):
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / reps
+    del a, b
+    return 2 * nbytes / t / 1e9
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--members", type=int, default=80)
+    ap.add_argument("--n", type=int, default=768)
+    ap.add_argument("--levels", type=int, default=64)
+    ap.add_argument("--gm", type=int, default=4)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(the operator shards by member with no exchange; see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from standalone_mgbf_loc_amd import mgbf as MG
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+
+    M, n, lm, gm = args.members, args.n, args.levels, args.gm
+    weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
+    cfg = MG.Config.for_grid(n, n, lm, n_ens=M, gm=gm, h=6, hz=3, p=2, weights=weights)
+    units = M * n * n * lm
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    log(f"[bench] workload {M} x {n}x{n}x{lm}, gm={gm}; generating members")
+    x = make_members(torch, M, n, lm)
+    y = torch.empty_like(x)
+    op = MG.MGBF(cfg)
+    assert op.dims.gm == gm
+
+    # ---- batched: one application over all members ----
+    for _ in range(args.warmup):
+        op.filtering_procedure(x, out=y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        op.filtering_procedure(x, out=y)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = units / (ms_step / 1e3)
+    log(f"[bench] batched: {ms_step:.3f} ms/step, {value:.4e} units/s")
+
+    # ---- per-phase device timers (one extra step) ----
+    op.timers(True)
+    op.filtering_procedure(x, out=y)
+    phases = op.read_timers()
+    op.timers(False)
+    torch.cuda.synchronize()
+    r_ms = phases["hfiltT"][0] + phases["hfilt"][0]
+    r_n = phases["hfiltT"][1] + phases["hfilt"][1]
+    r_bytes = phases["hfiltT"][2] + phases["hfilt"][2]
+    achieved = r_bytes / (r_ms / 1e3) / 1e9
+    phase_out = {k: {"ms": round(v[0], 4), "launches": v[1],
+                     "GBps": round(v[2] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 else None}
+                 for k, v in phases.items()}
+    log(f"[bench] phases: {phase_out}")
+
+    # ---- plain: one member at a time ----
+    plain = None
+    if not args.no_plain:
+        cfg1 = MG.Config.for_grid(n, n, lm, n_ens=1, gm=gm, h=6, hz=3, p=2, weights=weights)
+        op1 = MG.MGBF(cfg1)
+        xp = x.view(n, n, M, lm).permute(2, 0, 1, 3).contiguous()
+        yp = torch.empty_like(xp)
+        for m in range(min(M, 2)):
+            op1.filtering_procedure(xp[m], out=yp[m])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            for m in range(M):
+                op1.filtering_procedure(xp[m], out=yp[m])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        p_ms = (t1 - t0) * 1e3 / args.plain_steps
+        diff = (yp.permute(1, 2, 0, 3).reshape(n, n, M * lm) - y).abs().max().item()
+        plain = {"value": units / (p_ms / 1e3), "ms_per_step": round(p_ms, 3),
+                 "max_abs_diff_vs_batched": diff}
+        log(f"[bench] plain: {p_ms:.3f} ms/step ({plain['value']:.4e} units/s), diff {diff:.3e}")
+        op1.close()
+        del xp, yp
+
+    gbps_copy = stream_copy_gbps(torch)
+    log(f"[bench] device copy: {gbps_copy:.1f} GB/s")
+
+    # ---- FP64 CPU leg on member 0: error check and CPU baseline ----
+    relerr = None
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        P = O.OracleParams(nx=n, ny=n, K=lm, lm=lm, nvcol=1, hx=6, hy=6, hz=3, p=2, gm=gm,
+                           ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02, weights=weights[:gm])
+        x0 = x.view(n, n, M, lm)[:, :, 0, :].double().cpu().numpy()
+        y0 = y.view(n, n, M, lm)[:, :, 0, :].double().cpu().numpy()
+        t0 = time.perf_counter()
+        ref = O.wawt(P, x0)
+        tc = time.perf_counter() - t0
+        import numpy as np
+        relerr = float(np.max(np.abs(y0 - ref)) / np.max(np.abs(ref)))
+        cpu = {"value": n * n * lm / tc, "unit": "gridpoint·members/s", "cores": O.num_threads(),
+               "kind": "port",
+               "sample": f"member 0 of the workload ({n}x{n}x{lm}, gm={gm}) by the FP64 C "
+                         f"restatement with OpenMP, {tc:.2f} s"}
+        log(f"[bench] cpu: {tc:.2f} s, rel err {relerr:.3e}")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "gridpoint·members/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: member m ~ N(0,1), seed 100+m",
+        "config": {"workload": f"batched ensemble localization W A W^T: {M} members x {n}x{n}x{lm}, "
+                               f"{gm} generations, radial half-width 6, vertical half-width 3, p=2, "
+                               f"weights [1,1/2,1/4,1/8]",
+                   "members": M, "nx": n, "ny": n, "levels": lm, "gm": gm},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "radial_kernel (F and F^T on every generation)",
+                     "launches": r_n, "avg_launch_ms": round(r_ms / max(r_n, 1), 4),
+                     "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1)},
+        "cpu_baseline": cpu,
+        "max_rel_err_vs_fp64": relerr,
+        "plain": plain,
+        "batched_over_plain": round(value / plain["value"], 3) if plain else None,
+        "operator_hbm_frac": round(bytes_per_unit(gm) * value / 1e9 / HBM_PEAK_GBPS, 4),
+        "device_copy_GBps": round(gbps_copy, 1),
+        "phases": phase_out,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

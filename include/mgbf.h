@@ -139,6 +139,27 @@ int mgbf_apply_stage(mgbf_ctx* ctx, int stage, int gen, const float* in, float* 
 int mgbf_dot(mgbf_ctx* ctx, const float* a, const float* b, long long n, double* result,
              void* stream);
 
+/* Per-phase device timers, the phases of the reference's mg_timers
+   (src.mgbf/mg_timers.f90:17-60, filtering_rad2_ens btim/etim brackets):
+   hipEvents recorded on the caller's stream around every launch of a phase,
+   with the algorithmic bytes (reads + writes) of each launch. The weighting
+   (weight_tim) is fused into the adjoint radial filter (hfiltT). */
+enum {
+    MGBF_T_VFILTT = 0, /* Vf^T */
+    MGBF_T_UPSEND = 1, /* restriction chain */
+    MGBF_T_HFILTT = 2, /* F^T with the weighting A */
+    MGBF_T_HFILT = 3,  /* F */
+    MGBF_T_DNSEND = 4, /* prolongation and accumulation */
+    MGBF_T_VFILT = 5,  /* Vf */
+    MGBF_NTIMERS = 6
+};
+/* Turns the timers on (1) or off (0) and clears the accumulators. */
+int mgbf_timers_enable(mgbf_ctx* ctx, int enable);
+/* Waits for the recorded events and returns, per phase, the accumulated
+   milliseconds, number of launches and algorithmic bytes (arrays of
+   MGBF_NTIMERS entries; any may be NULL). */
+int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* bytes);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* mgbf_last_error(void);
 

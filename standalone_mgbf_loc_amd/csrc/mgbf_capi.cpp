@@ -82,9 +82,77 @@ struct mgbf_ctx {
     float* ws_u = nullptr;  // multigrid-sized
     float* ws_t = nullptr;  // multigrid-sized
     double* d_dot = nullptr;
+    // per-phase timers
+    struct Rec {
+        int phase;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool timing = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double t_ms[MGBF_NTIMERS] = {};
+    double t_bytes[MGBF_NTIMERS] = {};
+    long long t_n[MGBF_NTIMERS] = {};
 };
 
 namespace {
+
+hipEvent_t take_event(mgbf_ctx* c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the launches of one timer phase with events when timing is on.
+struct Phase {
+    mgbf_ctx* c;
+    int ph;
+    hipStream_t st;
+    double bytes;
+    hipEvent_t a = nullptr;
+    Phase(mgbf_ctx* c_, int ph_, hipStream_t st_, double bytes_)
+        : c(c_), ph(ph_), st(st_), bytes(bytes_) {
+        if (c->timing) {
+            a = take_event(c);
+            if (a && hipEventRecord(a, st) != hipSuccess) {
+                c->pool.push_back(a);
+                a = nullptr;
+            }
+        }
+    }
+    ~Phase() {
+        if (!a) return;
+        hipEvent_t b = take_event(c);
+        if (b && hipEventRecord(b, st) == hipSuccess) {
+            c->recs.push_back({ph, a, b, bytes});
+        } else {
+            c->pool.push_back(a);
+            if (b) c->pool.push_back(b);
+        }
+    }
+};
+
+int drain_timers(mgbf_ctx* c) {
+    int status = MGBF_OK;
+    for (auto& r : c->recs) {
+        float ms = 0.f;
+        if (hipEventSynchronize(r.b) != hipSuccess || hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess)
+            status = fail(MGBF_ERR_HIP, "timer event failure");
+        c->t_ms[r.phase] += ms;
+        c->t_bytes[r.phase] += r.bytes;
+        c->t_n[r.phase] += 1;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    return status;
+}
 
 int pick_tj(int K, int n, int m) {
     const long long nkc = (K + 63) / 64;
@@ -95,7 +163,8 @@ int pick_tj(int K, int n, int m) {
 }
 
 int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield, float wconst,
-           hipStream_t st) {
+           hipStream_t st, int phase = MGBF_T_HFILT) {
+    Phase ph(c, phase, st, 4.0 * (double)c->gsize[g] * (wfield ? 3 : 2));
     mgbf::RadialArgs a{};
     a.in = in;
     a.out = out;
@@ -113,6 +182,7 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
 }
 
 int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t st) {
+    Phase ph(c, MGBF_T_UPSEND, st, 4.0 * (double)(c->gsize[g] + c->gsize[g + 1]));
     mgbf::TransferArgs a{fine, coarse, c->K, c->nx[g], c->ny[g],
                          c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g]};
     HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
@@ -121,6 +191,8 @@ int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t
 
 int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, float* fine,
               hipStream_t st) {
+    Phase ph(c, MGBF_T_DNSEND, st,
+             4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)));
     mgbf::TransferArgs a{coarse, fine, c->K, c->nx[g], c->ny[g],
                          c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g]};
     HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
@@ -128,6 +200,7 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
 }
 
 int vertical(mgbf_ctx* c, const float* in, float* out, bool adjoint, hipStream_t st) {
+    Phase ph(c, adjoint ? MGBF_T_VFILTT : MGBF_T_VFILT, st, 8.0 * (double)c->gsize[0]);
     if (c->nfilt == 0) {
         if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
         return MGBF_OK;
@@ -167,8 +240,8 @@ int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     for (int g = 0; g < c->gm; ++g) {
         if (g == 0 && !c->filt_g1) continue;
         // F^T then A (weighting after the adjoint filter), then F
-        CALL(radial(c, g, U + c->goff[g], T, c->wfield[g], c->wconst[g], st));
-        CALL(radial(c, g, T, U + c->goff[g], nullptr, 1.0f, st));
+        CALL(radial(c, g, U + c->goff[g], T, c->wfield[g], c->wconst[g], st, MGBF_T_HFILTT));
+        CALL(radial(c, g, T, U + c->goff[g], nullptr, 1.0f, st, MGBF_T_HFILT));
     }
     return up_outputs(c, U, y, st);
 }
@@ -181,7 +254,7 @@ int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream
             HIP_TRY(mgbf::launch_copy(U, v, c->gsize[0], st), "copy launch");
             continue;
         }
-        CALL(radial(c, g, U + c->goff[g], v + c->goff[g], nullptr, 1.0f, st));
+        CALL(radial(c, g, U + c->goff[g], v + c->goff[g], nullptr, 1.0f, st, MGBF_T_HFILTT));
         if (sqrt_weights)
             HIP_TRY(mgbf::launch_scale(v + c->goff[g], v + c->goff[g], c->wfield[g], c->wconst[g], 1,
                                        c->gsize[g], st),
@@ -375,6 +448,8 @@ int mgbf_finalize(mgbf_ctx* c) {
     if (!c) return MGBF_OK;
     DeviceGuard guard(c->device);
     (void)hipDeviceSynchronize();
+    (void)drain_timers(c);
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
     if (c->ws_u) (void)hipFree(c->ws_u);
     if (c->ws_t) (void)hipFree(c->ws_t);
     if (c->d_tables) (void)hipFree(c->d_tables);
@@ -453,6 +528,31 @@ int mgbf_dot(mgbf_ctx* c, const float* a, const float* b, long long n, double* r
             "dot copy");
     HIP_TRY(hipStreamSynchronize(st), "dot sync");
     return MGBF_OK;
+}
+
+int mgbf_timers_enable(mgbf_ctx* c, int enable) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    DeviceGuard guard(c->device);
+    int st = drain_timers(c);
+    c->timing = enable != 0;
+    for (int i = 0; i < MGBF_NTIMERS; ++i) {
+        c->t_ms[i] = 0;
+        c->t_bytes[i] = 0;
+        c->t_n[i] = 0;
+    }
+    return st;
+}
+
+int mgbf_timers_read(mgbf_ctx* c, double* ms, long long* launches, double* bytes) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    DeviceGuard guard(c->device);
+    int st = drain_timers(c);
+    for (int i = 0; i < MGBF_NTIMERS; ++i) {
+        if (ms) ms[i] = c->t_ms[i];
+        if (launches) launches[i] = c->t_n[i];
+        if (bytes) bytes[i] = c->t_bytes[i];
+    }
+    return st;
 }
 
 }  // extern "C"

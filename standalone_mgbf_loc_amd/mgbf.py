@@ -29,8 +29,9 @@ STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_R, STAGE_P, STAGE_A = range(7)
 EXPORTED_SYMBOLS = (
     "mgbf_config_default", "mgbf_derive", "mgbf_init", "mgbf_finalize", "mgbf_get_dims",
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
-    "mgbf_version",
+    "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read",
 )
+TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
 
 class MGBFError(RuntimeError):
@@ -174,6 +175,9 @@ def load_library(path: Optional[str] = None):
                                    ctypes.c_void_p, ctypes.c_void_p]
     L.mgbf_dot.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
                            ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    L.mgbf_timers_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.mgbf_timers_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
     L.mgbf_last_error.restype = ctypes.c_char_p
     L.mgbf_version.restype = ctypes.c_char_p
     if path is None:
@@ -319,6 +323,19 @@ class MGBF:
             out = torch.empty(oshape, dtype=torch.float32, device=x.device)
         _check(self._L.mgbf_apply_stage(self._h, stage, gen, _ptr(x), _ptr(out), _stream(stream)))
         return out
+
+    def timers(self, enable: bool = True):
+        """Turns the per-phase device timers on or off (and clears them)."""
+        _check(self._L.mgbf_timers_enable(self._h, int(enable)))
+
+    def read_timers(self) -> dict:
+        """{phase: (ms, launches, algorithmic bytes)} accumulated since enabling."""
+        n = len(TIMER_NAMES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        by = (ctypes.c_double * n)()
+        _check(self._L.mgbf_timers_read(self._h, ms, cnt, by))
+        return {name: (ms[i], cnt[i], by[i]) for i, name in enumerate(TIMER_NAMES)}
 
     def dot(self, a, b, stream=None) -> float:
         """FP64-accumulated <a, b> on the device."""
