@@ -20,6 +20,8 @@ LIB = os.path.join(LIBDIR, "libmgbf.so")
 INCLUDE = os.path.join(ROOT, "include")
 
 SOURCES = ["mgbf_coef.cpp", "mgbf_capi.cpp", "mgbf_kernels.hip"]
+RADIAL_TU = "mgbf_radial_inst.hip"
+RADIAL_H = range(1, 11)
 ARCH = os.environ.get("MGBF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -32,7 +34,7 @@ def hipcc() -> str:
 
 def flags() -> list:
     return [
-        "-O3", "-std=c++17", "-fPIC", "-shared",
+        "-O3", "-std=c++17", "-fPIC",
         f"--offload-arch={ARCH}",
         # packed FP32 (v_pk_*_f32) code generation is disabled on purpose
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -41,16 +43,42 @@ def flags() -> list:
     ]
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+def _compile(job):
+    src, obj, extra, verbose = job
+    cmd = [hipcc()] + flags() + extra + ["-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
+        return " ".join(cmd) + "\n" + res.stdout + res.stderr
+    return None
+
+
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    """Compiles every translation unit (in parallel) and links libmgbf.so."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    work = []
+    for s in SOURCES:
+        work.append((os.path.join(CSRC, s), os.path.join(objdir, s + ".o"), [], verbose))
+    for h in RADIAL_H:
+        work.append((os.path.join(CSRC, RADIAL_TU), os.path.join(objdir, f"radial_h{h}.o"),
+                     [f"-DMGBF_RADIAL_H={h}"], verbose))
+    jobs = jobs or min(len(work), max(1, (os.cpu_count() or 4)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        errs = [e for e in ex.map(_compile, work) if e]
+    if errs:
+        sys.stderr.write("\n".join(errs))
         raise RuntimeError("hipcc failed building libmgbf.so")
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}"] + [w[1] for w in work] + ["-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + res.stdout + res.stderr)
+        raise RuntimeError("linking libmgbf.so failed")
     os.replace(tmp, LIB)
     return LIB
 

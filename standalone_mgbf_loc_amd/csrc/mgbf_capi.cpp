@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -67,8 +68,11 @@ struct mgbf_ctx {
     long long gsize[mgbf::kMaxGen] = {};
     long long goff[mgbf::kMaxGen] = {};
     void* d_tables = nullptr;
-    const float* d_vf = nullptr;
-    const float* d_vft = nullptr;
+    // vertical bands: 0 = Vf, 1 = Vf^T, 2 = Vf Vf^T
+    const float* d_band[3] = {};
+    mgbf::VBandW band_int[3] = {};
+    int band_hb[3] = {}, band_ilo[3] = {}, band_ihi[3] = {};
+    bool commute = true;     // weights constant along the stack: B = Vf Vf^T H
     const int* d_rxs[mgbf::kMaxGen] = {};
     const float* d_rxw[mgbf::kMaxGen] = {};
     const int* d_rys[mgbf::kMaxGen] = {};
@@ -162,14 +166,35 @@ int pick_tj(int K, int n, int m) {
     return tj;
 }
 
+enum { BAND_VF = 0, BAND_VFT = 1, BAND_VVT = 2 };
+
+// Can the vertical band `which` be fused into the last radial pass of
+// generation 1? Each wave must hold whole columns, and the variant built.
+bool can_fuse_band(const mgbf_ctx* c, int which) {
+    const int hb = c->band_hb[which];
+    const int lm = c->s.cfg.lm;
+    return c->nfilt > 0 && c->s.exact_disc && c->gm > 1 && (64 % lm) == 0 && (c->K % 64) == 0 &&
+           (hb == 3 || hb == 4 || hb == 6 || hb == 8);
+}
+
+// out = scale * F(in) [+ P coarse] [then vertical band `band`] on generation g
 int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield, float wconst,
-           hipStream_t st, int phase = MGBF_T_HFILT) {
-    Phase ph(c, phase, st, 4.0 * (double)c->gsize[g] * (wfield ? 3 : 2));
+           hipStream_t st, int phase = MGBF_T_HFILT, const float* coarse = nullptr,
+           int band = -1) {
+    Phase ph(c, phase, st,
+             4.0 * ((double)c->gsize[g] * (wfield ? 3 : 2) + (coarse ? (double)c->gsize[g + 1] : 0.0)));
     mgbf::RadialArgs a{};
     a.in = in;
     a.out = out;
     a.wfield = wfield;
     a.wconst = wconst;
+    a.coarse = coarse;
+    if (band >= 0) {
+        a.band = c->d_band[band];
+        a.vhb = c->band_hb[band];
+    }
+    a.lm = c->s.cfg.lm;
+    a.nfilt = c->nfilt;
     a.K = c->K;
     a.n = c->nx[g];
     a.m = c->ny[g];
@@ -199,15 +224,25 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
     return MGBF_OK;
 }
 
-int vertical(mgbf_ctx* c, const float* in, float* out, bool adjoint, hipStream_t st) {
-    Phase ph(c, adjoint ? MGBF_T_VFILTT : MGBF_T_VFILT, st, 8.0 * (double)c->gsize[0]);
+int vertical(mgbf_ctx* c, const float* in, float* out, int which, hipStream_t st) {
+    Phase ph(c, which == BAND_VFT ? MGBF_T_VFILTT : MGBF_T_VFILT, st, 8.0 * (double)c->gsize[0]);
     if (c->nfilt == 0) {
         if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
         return MGBF_OK;
     }
-    HIP_TRY(mgbf::launch_vertical(in, out, adjoint ? c->d_vft : c->d_vf, c->K, c->s.cfg.lm,
-                                  c->nfilt, c->s.cfg.hz, (long long)c->nx[0] * c->ny[0], st),
-            "vertical filter launch");
+    mgbf::VBandArgs a{};
+    a.in = in;
+    a.out = out;
+    a.band = c->d_band[which];
+    a.wint = c->band_int[which];
+    a.ilo = c->band_ilo[which];
+    a.ihi = c->band_ihi[which];
+    a.HB = c->band_hb[which];
+    a.K = c->K;
+    a.lm = c->s.cfg.lm;
+    a.nfilt = c->nfilt;
+    a.npts = (long long)c->nx[0] * c->ny[0];
+    HIP_TRY(mgbf::launch_vband(a, st), "vertical filter launch");
     return MGBF_OK;
 }
 
@@ -217,38 +252,77 @@ int vertical(mgbf_ctx* c, const float* in, float* out, bool adjoint, hipStream_t
         if (st_ != MGBF_OK) return st_; \
     } while (0)
 
-// u_1 = Vf^T x into U, then the restriction chain.
-int down_inputs(mgbf_ctx* c, const float* x, float* U, hipStream_t st) {
-    CALL(vertical(c, x, U, true, st));
+// Restriction chain u_{g+1} = R u_g from u_1 (in U0 or a caller array).
+int restrict_chain(mgbf_ctx* c, const float* u1, float* U, hipStream_t st) {
     for (int g = 0; g + 1 < c->gm; ++g)
-        CALL(restrict_g(c, g, U + c->goff[g], U + c->goff[g + 1], st));
+        CALL(restrict_g(c, g, g == 0 ? u1 : U + c->goff[g], U + c->goff[g + 1], st));
     return MGBF_OK;
 }
 
-// s_g = z_g + P s_{g+1} in place on Z, then y = Vf s_1.
-int up_outputs(mgbf_ctx* c, float* Z, float* y, hipStream_t st) {
-    for (int g = c->gm - 2; g >= 0; --g)
-        CALL(prolong_g(c, g, Z + c->goff[g + 1], Z + c->goff[g], Z + c->goff[g], st));
-    CALL(vertical(c, Z, y, false, st));
+// Up-sweep from the coarsest generation: for g = gm..1 (1-based)
+//   s_g = F (A_g F u_g) + P s_{g+1}      (twice: W A W^T)
+//   s_g = F (A_g^{1/2}? v_g) + P s_{g+1}  (once: W, W A^{1/2})
+// Generation 1 is read from in1, generation g >= 2 from inb + goff[g]; s_g is
+// written into U. The last pass writes y = band(s_1) directly when the
+// vertical band fuses (or there is none), else s_1 goes to U0 and a separate
+// vertical pass follows.
+int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band, bool twice,
+            bool sqrt_w, hipStream_t st) {
+    float* U = c->ws_u;
+    float* T = c->ws_t;
+    const bool fuse = band >= 0 && can_fuse_band(c, band);
+    for (int g = c->gm - 1; g >= 0; --g) {
+        const float* src = g == 0 ? in1 : inb + c->goff[g];
+        const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
+        const bool direct = g == 0 && (fuse || band < 0);
+        float* dst = direct ? y : U + c->goff[g];
+        const int fb = (g == 0 && fuse) ? band : -1;
+        if (g == 0 && !c->filt_g1) {
+            // generation 1 unfiltered: s_1 = u_1 + P s_2
+            float* s1 = band >= 0 ? U : y;
+            if (coarse) CALL(prolong_g(c, 0, coarse, src, s1, st));
+            else if (src != s1) HIP_TRY(mgbf::launch_copy(src, s1, c->gsize[0], st), "copy launch");
+            return band >= 0 ? vertical(c, U, y, band, st) : MGBF_OK;
+        }
+        if (twice) {
+            CALL(radial(c, g, src, T, c->wfield[g], c->wconst[g], st, MGBF_T_HFILTT));
+            CALL(radial(c, g, T, dst, nullptr, 1.0f, st, MGBF_T_HFILT, coarse, fb));
+        } else {
+            if (sqrt_w) {
+                HIP_TRY(mgbf::launch_scale(src, T, c->wfield[g], c->wconst[g], 1, c->gsize[g], st),
+                        "scale launch");
+                src = T;
+            }
+            CALL(radial(c, g, src, dst, nullptr, 1.0f, st, MGBF_T_HFILT, coarse, fb));
+        }
+    }
+    if (band >= 0 && !fuse) return vertical(c, U, y, band, st);
     return MGBF_OK;
+}
+
+// W A W^T with weights that are the same along the stack: the horizontal
+// multigrid operator H acts identically on every k and so commutes with the
+// vertical filter, B = Vf H Vf^T = (Vf Vf^T) H; one banded vertical pass,
+// fused into the last radial pass when possible.
+int apply_wawt_commuted(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
+    CALL(restrict_chain(c, x, c->ws_u, st));
+    return upsweep(c, x, c->ws_u, y, c->nfilt ? BAND_VVT : -1, true, false, st);
 }
 
 int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
+    bool fields = false;
+    for (int g = 0; g < c->gm; ++g) fields = fields || c->wfield[g] != nullptr;
+    if (!fields && c->commute) return apply_wawt_commuted(c, x, y, st);
     float* U = c->ws_u;
-    float* T = c->ws_t;
-    CALL(down_inputs(c, x, U, st));
-    for (int g = 0; g < c->gm; ++g) {
-        if (g == 0 && !c->filt_g1) continue;
-        // F^T then A (weighting after the adjoint filter), then F
-        CALL(radial(c, g, U + c->goff[g], T, c->wfield[g], c->wconst[g], st, MGBF_T_HFILTT));
-        CALL(radial(c, g, T, U + c->goff[g], nullptr, 1.0f, st, MGBF_T_HFILT));
-    }
-    return up_outputs(c, U, y, st);
+    CALL(vertical(c, x, U, BAND_VFT, st));
+    CALL(restrict_chain(c, U, U, st));
+    return upsweep(c, U, U, y, c->nfilt ? BAND_VF : -1, true, false, st);
 }
 
 int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
     float* U = c->ws_u;
-    CALL(down_inputs(c, x, U, st));
+    CALL(vertical(c, x, U, BAND_VFT, st));
+    CALL(restrict_chain(c, U, U, st));
     for (int g = 0; g < c->gm; ++g) {
         if (g == 0 && !c->filt_g1) {
             HIP_TRY(mgbf::launch_copy(U, v, c->gsize[0], st), "copy launch");
@@ -263,24 +337,10 @@ int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream
     return MGBF_OK;
 }
 
+// y = W v: the control vector's generations are filtered once and summed up
+// the hierarchy, then Vf.
 int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
-    float* U = c->ws_u;
-    float* T = c->ws_t;
-    for (int g = 0; g < c->gm; ++g) {
-        const float* src = v + c->goff[g];
-        if (g == 0 && !c->filt_g1) {
-            HIP_TRY(mgbf::launch_copy(src, U, c->gsize[0], st), "copy launch");
-            continue;
-        }
-        if (sqrt_weights) {
-            HIP_TRY(mgbf::launch_scale(src, T + c->goff[g], c->wfield[g], c->wconst[g], 1,
-                                       c->gsize[g], st),
-                    "scale launch");
-            src = T + c->goff[g];
-        }
-        CALL(radial(c, g, src, U + c->goff[g], nullptr, 1.0f, st));
-    }
-    return up_outputs(c, U, y, st);
+    return upsweep(c, v, v, y, c->nfilt ? BAND_VF : -1, false, sqrt_weights, st);
 }
 
 int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStream_t st) {
@@ -370,6 +430,12 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->gm = s.dims.gm;
     c->filt_g1 = cfg->mgbf_proc == 1 ? true : cfg->l_filt_g1 != 0;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
+    {
+        // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for
+        // weights that are constant along the stack
+        const char* ev = std::getenv("MGBF_EXACT_ORDER");
+        c->commute = !(ev && ev[0] == '1');
+    }
     long long off = 0;
     for (int g = 0; g < c->gm; ++g) {
         c->nx[g] = s.nx[g];
@@ -390,9 +456,39 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     }
     DeviceGuard guard(device);
 
+    // vertical band descriptors and their interior rows
+    const std::vector<double>* bands[3] = {&s.vf, &s.vft, &s.vvt};
+    const int lmv = cfg->lm;
+    for (int b = 0; b < 3; ++b) {
+        const int hb = b == BAND_VVT ? 2 * cfg->hz : cfg->hz;
+        const int nb = 2 * hb + 1;
+        if (hb > mgbf::kMaxVBand || hb < 1) {
+            if (c->nfilt > 0) {
+                delete c;
+                return fail(MGBF_ERR_UNSUPPORTED, "vertical half-width outside 1..10");
+            }
+            continue;
+        }
+        c->band_hb[b] = hb;
+        // rows equal to the middle row form the interior range
+        const int mid = lmv / 2;
+        auto same = [&](int L) {
+            for (int t = 0; t < nb; ++t)
+                if ((float)(*bands[b])[(size_t)L * nb + t] != (float)(*bands[b])[(size_t)mid * nb + t])
+                    return false;
+            return true;
+        };
+        int lo = mid, hi = mid + 1;
+        while (lo > 0 && same(lo - 1)) --lo;
+        while (hi < lmv && same(hi)) ++hi;
+        c->band_ilo[b] = lo;
+        c->band_ihi[b] = hi;
+        for (int t = 0; t < nb; ++t) c->band_int[b].w[t] = (float)(*bands[b])[(size_t)mid * nb + t];
+    }
+
     // one device allocation for all coefficient tables
-    const size_t band = s.vf.size();
-    size_t bytes = 2 * carve_size(sizeof(float) * band) + carve_size(sizeof(double));
+    size_t bytes = carve_size(sizeof(double));
+    for (int b = 0; b < 3; ++b) bytes += carve_size(sizeof(float) * bands[b]->size());
     for (int g = 0; g + 1 < c->gm; ++g) {
         const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
         for (const auto* t : tabs)
@@ -418,8 +514,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (size_t i = 0; i < v.size(); ++i) h[i] = v[i];
         return d;
     };
-    c->d_vf = put_f(s.vf);
-    c->d_vft = put_f(s.vft);
+    for (int b = 0; b < 3; ++b) c->d_band[b] = put_f(*bands[b]);
     c->d_dot = carve<double>(dp, 1);
     carve<double>(hp, 1);
     for (int g = 0; g + 1 < c->gm; ++g) {
@@ -499,7 +594,7 @@ int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* ou
         case MGBF_STAGE_VF:
         case MGBF_STAGE_VFT:
             if (g != 0) return fail(MGBF_ERR_ARG, "the vertical filter acts on generation 1");
-            return vertical(c, in, out, stage == MGBF_STAGE_VFT, st);
+            return vertical(c, in, out, stage == MGBF_STAGE_VFT ? BAND_VFT : BAND_VF, st);
         case MGBF_STAGE_F:
         case MGBF_STAGE_FT:
             return radial(c, g, in, out, nullptr, 1.0f, st);

@@ -329,6 +329,20 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
             }
         s.vf.assign((size_t)lm * W, 0.0);
         s.vft.assign((size_t)lm * W, 0.0);
+        // Vf Vf^T (band 2hz): the vertical part of W A W^T when the weights
+        // are the same along the stack, where it commutes with the
+        // horizontal multigrid operator
+        {
+            const int W2 = 4 * hz + 1;
+            s.vvt.assign((size_t)lm * W2, 0.0);
+            for (int L = 0; L < lm; ++L)
+                for (int c = L - 2 * hz; c <= L + 2 * hz; ++c) {
+                    if (c < 0 || c >= lm) continue;
+                    double acc = 0;
+                    for (int q = 0; q < lm; ++q) acc += V[(size_t)L * lm + q] * V[(size_t)c * lm + q];
+                    s.vvt[(size_t)L * W2 + (c - L + 2 * hz)] = acc;
+                }
+        }
         for (int L = 0; L < lm; ++L)
             for (int t = -hz; t <= hz; ++t) {
                 const int c = L + t;

@@ -40,6 +40,7 @@ struct Setup {
     double wsym[kMaxHalo + 1][kMaxHalo + 1] = {};  // w(|a|,|b|) as [b][a]
     bool exact_disc = false;            // taps = lattice disc a^2+b^2 < (H+1)^2
     std::vector<double> vf, vft;       // lm x (2hz+1) band of Vf and Vf^T
+    std::vector<double> vvt;           // lm x (4hz+1) band of Vf Vf^T
     AxisTable rx[kMaxGen], ry[kMaxGen];   // restriction g -> g+1 (coarse rows)
     AxisTable px[kMaxGen], py[kMaxGen];   // prolongation g+1 -> g (fine rows)
     double wconst[kMaxGen] = {};

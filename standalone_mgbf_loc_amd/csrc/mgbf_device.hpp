@@ -1,0 +1,34 @@
+// Device helpers shared by the HIP kernels of the MGBF operator.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mgbf {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+// Half-sample mirror (X(1-t)=X(t), X(n+t)=X(n+1-t), 1-based), clamped into
+// the domain: a clamped index only feeds outputs outside the domain.
+__device__ __forceinline__ int mirror_half(int q, int n) {
+    q = q < 0 ? -1 - q : q;
+    q = q >= n ? 2 * n - 1 - q : q;
+    return q < 0 ? 0 : (q >= n ? n - 1 : q);
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
+    constexpr unsigned kXcd = 8;
+    if (nblocks % kXcd) return bid;
+    return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
+}
+
+// Interpolation constants of the offset grids (def_offset_coef): fine index
+// 2u (even, 0-based) reads coarse u-2..u+1 with kQc, fine 2u+1 reads coarse
+// u-1..u+2 with kPc.
+constexpr float kPc[4] = {-9.f / 128.f, 111.f / 128.f, 29.f / 128.f, -3.f / 128.f};
+constexpr float kQc[4] = {-3.f / 128.f, 29.f / 128.f, 111.f / 128.f, -9.f / 128.f};
+
+}  // namespace mgbf

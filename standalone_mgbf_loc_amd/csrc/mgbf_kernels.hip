@@ -1,4 +1,6 @@
-// HIP kernels of the MGBF operator for CDNA4 (gfx950).
+// HIP kernels of the MGBF operator for CDNA4 (gfx950): vertical band,
+// restriction, prolongation, scaling and reductions; the radial filter kernel
+// is in mgbf_radial.hpp (one translation unit per half-width).
 //
 // Field layout: k fastest, then x (i), then y (j); element (k,i,j) at
 // k + K*(i + n*j). One wavefront (64 lanes) owns 64 consecutive k; every
@@ -12,294 +14,259 @@
 
 #include <cstdint>
 
+#include "mgbf_device.hpp"
 #include "mgbf_launch.hpp"
 
 namespace mgbf {
 namespace {
 
-constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
-constexpr int kBlock = kWave * kWavesPerBlock;
-
-// Half-sample mirror (X(1-t)=X(t), X(n+t)=X(n+1-t), 1-based), clamped into
-// the domain: a clamped index only feeds outputs outside the domain.
-__device__ __forceinline__ int mirror_half(int q, int n) {
-    q = q < 0 ? -1 - q : q;
-    q = q >= n ? 2 * n - 1 - q : q;
-    return q < 0 ? 0 : (q >= n ? n - 1 : q);
-}
-
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
-// contiguous range of logical blocks so neighbouring tiles share its L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
-    constexpr unsigned kXcd = 8;
-    if (nblocks % kXcd) return bid;
-    return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
-}
-
-template <int H>
-__host__ __device__ constexpr int disc_extent(int b) {
-    int a = H;
-    while (a >= 0 && a * a + b * b >= (H + 1) * (H + 1)) --a;
-    return a;
-}
-
 // ---------------------------------------------------------------------------
-// Radial 2-D beta filter F (gather form, half-sample mirror halo):
-//   out(k,i,j) = scale(k,i,j) * sum_{a,b} w(|a|,|b|) in(k, mx(i+a), my(j+b))
-// scale = wfield (per point) or wconst. One wave computes TI x TJ outputs for
-// its 64 k: input rows stream through registers; for every input row the
-// x-pair sums P_a = v(i+a)+v(i-a) give the per-row partial sums
-// Z_b = sum_a w(a,b) P_a, which are added to the output rows j = r-b and r+b
-// held in a ring of 2H+1 accumulator rows.
+// Banded vertical operator along the stack (Vf, Vf^T or the product Vf Vf^T):
+// out(k) = sum_{t=-HB..HB} c(L,t) in(k+t) for k < nfilt, L = k mod lm, with the
+// whole-sample mirror folded into the band; entries k >= nfilt pass through.
+// Rows ilo <= L < ihi (clear of the mirror folds) share the uniform interior
+// weights; the rows near the ends of a column read their row of the table. A block stages a
+// contiguous span of up to kVSpan elements (one chunk of a point's column, or
+// several whole points) plus the halo in LDS, so HBM is streamed contiguously.
 // ---------------------------------------------------------------------------
-template <int H, int TI, bool DISC>
+constexpr int kVPer = 8;                    // elements per thread
+constexpr int kVSpan = kBlock * kVPer;      // elements per block
+
+template <int HB>
 __global__ void __launch_bounds__(kBlock)
-radial_kernel(const float* __restrict__ in, float* __restrict__ out,
-              const float* __restrict__ wfield, float wconst,
-              int K, int n, int m, int TJ, int ntile_i, int ntile_j, int nitems,
-              RadialCoef W) {
-    constexpr int NR = 2 * H + 1;
-    constexpr int NC = TI + 2 * H;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    if (item >= nitems) return;
-    const int ti = item % ntile_i;
-    const int rest = item / ntile_i;
-    const int tj = rest % ntile_j;
-    const int kc = rest / ntile_j;
-    const int k = kc * kWave + lane;
-    const bool kvalid = k < K;
-    const int kk = kvalid ? k : K - 1;
-    const int i0 = ti * TI, j0 = tj * TJ;
-    const size_t rowstride = (size_t)n * K;
-
-    int coff[NC];
+vband_kernel(const float* __restrict__ in, float* __restrict__ out,
+             const float* __restrict__ band, VBandW wint, int ilo, int ihi, int K, int lm,
+             int nfilt, long long N, int chunks_per_point, int pspan) {
+    constexpr int NB = 2 * HB + 1;
+    __shared__ float s[kVSpan + 2 * HB];
+    long long e0;
+    int kbase, span;
+    if (chunks_per_point > 0) {            // K > kVSpan: one chunk of one point
+        const long long pt = blockIdx.x / chunks_per_point;
+        kbase = (int)(blockIdx.x % chunks_per_point) * kVSpan;
+        e0 = pt * K + kbase;
+        span = K - kbase < kVSpan ? K - kbase : kVSpan;
+    } else {                               // whole points, pspan = ppb * K elements
+        kbase = 0;
+        e0 = (long long)blockIdx.x * pspan;
+        span = (int)(N - e0 < pspan ? N - e0 : pspan);
+    }
+    const int tid = threadIdx.x;
+    float tmp[kVPer];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) coff[c] = mirror_half(i0 - H + c, n) * K;
-
-    float acc[NR][TI];
+    for (int r = 0; r < kVPer; ++r) {
+        const int i = tid + r * kBlock;
+        const long long e = e0 - HB + i;
+        tmp[r] = (i < span + 2 * HB && e >= 0 && e < N) ? in[e] : 0.f;
+    }
+    float tail = 0.f;
+    if (tid < 2 * HB) {
+        const int i = kVSpan + tid;
+        const long long e = e0 - HB + i;
+        tail = (i < span + 2 * HB && e >= 0 && e < N) ? in[e] : 0.f;
+    }
 #pragma unroll
-    for (int s = 0; s < NR; ++s)
+    for (int r = 0; r < kVPer; ++r) s[tid + r * kBlock] = tmp[r];
+    if (tid < 2 * HB) s[kVSpan + tid] = tail;
+    __syncthreads();
 #pragma unroll
-        for (int t = 0; t < TI; ++t) acc[s][t] = 0.f;
-
-    const int tj_eff = TJ < m - j0 ? TJ : m - j0;
-    const int nsteps = tj_eff + 2 * H;
-    const float* base_in = in + kk;
-    float* base_out = out + kk;
-    const float* base_w = wfield ? wfield + kk : nullptr;
-
-    for (int rb = 0; rb < nsteps; rb += NR) {
+    for (int r = 0; r < kVPer; ++r) {
+        const int l = tid + r * kBlock;
+        if (l < span) {
+            const int k = chunks_per_point > 0 ? kbase + l : l % K;
+            const float* row = s + l + HB;
+            float v;
+            if (k < nfilt) {
+                const int L = k % lm;
+                v = 0.f;
+                if (L >= ilo && L < ihi) {
 #pragma unroll
-        for (int s = 0; s < NR; ++s) {
-            const int rr = rb + s;
-            if (rr < nsteps) {
-                const int r = mirror_half(j0 - H + rr, m);
-                const float* row = base_in + (size_t)r * rowstride;
-                float v[NC];
+                    for (int t = 0; t < NB; ++t) v = __builtin_fmaf(wint.w[t], row[t - HB], v);
+                } else {
+                    const float* c = band + L * NB;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) v[c] = row[coff[c]];
-#pragma unroll
-                for (int t = 0; t < TI; ++t) {
-                    float P[H + 1];
-                    P[0] = v[t + H];
-#pragma unroll
-                    for (int a = 1; a <= H; ++a) P[a] = v[t + H + a] + v[t + H - a];
-#pragma unroll
-                    for (int b = 0; b <= H; ++b) {
-                        const int A = DISC ? disc_extent<H>(b) : H;
-                        float z = W.w[b][0] * P[0];
-#pragma unroll
-                        for (int a = 1; a <= H; ++a)
-                            if (a <= A) z = __builtin_fmaf(W.w[b][a], P[a], z);
-                        acc[(s + b) % NR][t] += z;
-                        if (b > 0) acc[(s + NR - b) % NR][t] += z;
-                    }
+                    for (int t = 0; t < NB; ++t) v = __builtin_fmaf(c[t], row[t - HB], v);
                 }
-                // output row j = j0 + rr - 2H is complete
-                const int jrel = rr - 2 * H;
-                constexpr int slot_off = NR - H;   // (s - H) mod NR
-                if (jrel >= 0) {
-                    const int j = j0 + jrel;
-                    if (jrel < tj_eff && kvalid) {
-#pragma unroll
-                        for (int t = 0; t < TI; ++t) {
-                            const int i = i0 + t;
-                            if (i < n) {
-                                const size_t e = (size_t)j * rowstride + (size_t)i * K;
-                                const float sc = base_w ? base_w[e] : wconst;
-                                base_out[e] = sc * acc[(s + slot_off) % NR][t];
-                            }
-                        }
-                    }
-                }
-#pragma unroll
-                for (int t = 0; t < TI; ++t) acc[(s + slot_off) % NR][t] = 0.f;
+            } else {
+                v = row[0];
             }
+            out[e0 + l] = v;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Vertical 1-D beta filter on a generation-1 field: for k < nfilt (the
-// vertical columns of the stack, lm levels each) out(k) = sum_t c(L,t)
-// in(k - L + L + t), L = k mod lm, with the whole-sample mirror folded into
-// the band table c (lm x (2hz+1)); entries k >= nfilt pass through.
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
-vertical_kernel(const float* __restrict__ in, float* __restrict__ out,
-                const float* __restrict__ coef, int K, int lm, int nfilt, int hz,
-                long long npts) {
-    const int k = blockIdx.y * kWave + (threadIdx.x & (kWave - 1));
-    const long long p = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (k >= K || p >= npts) return;
-    const float* col = in + p * K;
-    float r;
-    if (k < nfilt) {
-        const int L = k % lm;
-        const int c0 = k - L;
-        const float* cf = coef + L * (2 * hz + 1);
-        r = 0.f;
-        for (int t = -hz; t <= hz; ++t) {
-            int q = L + t;
-            q = q < 0 ? 0 : (q >= lm ? lm - 1 : q);   // zero-weight taps stay in range
-            r = __builtin_fmaf(cf[t + hz], col[c0 + q], r);
-        }
-    } else {
-        r = col[k];
-    }
-    out[p * K + k] = r;
-}
-
-// ---------------------------------------------------------------------------
-// Restriction R (fine n x m -> coarse n/2 x m/2) from window tables: one lane
-// computes TJ2 coarse rows of one coarse column; every fine row of the strip
-// is x-restricted once and added to the coarse rows whose window holds it.
-// ---------------------------------------------------------------------------
-template <int TJ2>
-__global__ void __launch_bounds__(kBlock)
-restrict_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
-                int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
-                const int* __restrict__ rys, const float* __restrict__ ryw, int nstrip,
-                int nitems) {
-    constexpr int WR = kRWidthDev;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
-    if (item >= nitems) return;
-    const int nc = nf / 2, mc = mf / 2;
-    const int I = item % nc;
-    const int rest = item / nc;
-    const int js = rest % nstrip;
-    const int kc = rest / nstrip;
-    const int k = kc * kWave + lane;
-    const bool kvalid = k < K;
-    const int kk = kvalid ? k : K - 1;
-    const int J0 = js * TJ2;
-    const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
-    const int xs = rxs[I];
-    float xw[WR];
-    int xo[WR];
-#pragma unroll
-    for (int t = 0; t < WR; ++t) {
-        xw[t] = rxw[I * WR + t];
-        const int ii = xs + t < nf ? xs + t : nf - 1;
-        xo[t] = ii * K;
-    }
-    const int jlo = rys[J0];
-    int jhi = rys[J1 - 1] + WR;
-    if (jhi > mf) jhi = mf;
-    float acc[TJ2];
-#pragma unroll
-    for (int q = 0; q < TJ2; ++q) acc[q] = 0.f;
-    const size_t rowstride = (size_t)nf * K;
-    for (int jj = jlo; jj < jhi; ++jj) {
-        const float* row = fine + kk + (size_t)jj * rowstride;
-        float xr = 0.f;
-#pragma unroll
-        for (int t = 0; t < WR; ++t) xr = __builtin_fmaf(xw[t], row[xo[t]], xr);
-#pragma unroll
-        for (int q = 0; q < TJ2; ++q) {
-            const int J = J0 + q;
-            if (J < J1) {
-                const int d = jj - rys[J];
-                if (d >= 0 && d < WR) acc[q] = __builtin_fmaf(ryw[J * WR + d], xr, acc[q]);
-            }
-        }
-    }
-    if (!kvalid) return;
-#pragma unroll
-    for (int q = 0; q < TJ2; ++q) {
-        const int J = J0 + q;
-        if (J < J1) coarse[kk + (size_t)K * ((size_t)I + (size_t)nc * J)] = acc[q];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Prolongation P (coarse n/2 x m/2 -> fine n x m), optionally accumulated:
-// out = (addend ? addend : 0) + P coarse. One lane computes TJ fine rows of
-// one fine column.
+// Prolongation P (coarse -> fine), optionally accumulated: the coarse field is
+// read through the 2-point half-sample mirror (boco_2d(.,2,2)) and
+// interpolated with the 4-point constants, x then y (direct1). One lane
+// computes TJ fine rows of one fine column for its k; the TJ/2+4 x-interpolated
+// coarse rows it needs stay in registers.
 // ---------------------------------------------------------------------------
 template <int TJ>
 __global__ void __launch_bounds__(kBlock)
-prolong_kernel(const float* __restrict__ coarse, const float* addend, float* out, int K,
-               int nf, int mf, const int* __restrict__ pxs, const float* __restrict__ pxw,
-               const int* __restrict__ pys, const float* __restrict__ pyw, int nstrip,
-               int nitems) {
-    constexpr int WP = kPWidthDev;
+prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* out, int K, int nf,
+                int mf, int nkc, int nstrip, int nitems) {
+    constexpr int NR = TJ / 2 + 4;
     const int lane = threadIdx.x & (kWave - 1);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (item >= nitems) return;
-    const int nc = nf / 2;
-    const int i = item % nf;
-    const int rest = item / nf;
-    const int js = rest % nstrip;
-    const int kc = rest / nstrip;
+    const int kc = item % nkc;
+    const int rest = item / nkc;
+    const int i = rest % nf;
+    const int js = rest / nf;
     const int k = kc * kWave + lane;
     const bool kvalid = k < K;
     const int kk = kvalid ? k : K - 1;
-    const int j0 = js * TJ;
-    const int j1 = j0 + TJ < mf ? j0 + TJ : mf;
-    const int xs = pxs[i];
-    float xw[WP];
-    int xo[WP];
+    const int nc = nf / 2, mc = mf / 2;
+    const int u = i >> 1;
+    const bool odd = (i & 1) != 0;
+    const int cs = odd ? u - 1 : u - 2;
+    float cw[4];
+    int xo[4];
 #pragma unroll
-    for (int u = 0; u < WP; ++u) {
-        xw[u] = pxw[i * WP + u];
-        const int ii = xs + u < nc ? xs + u : nc - 1;
-        xo[u] = ii * K;
+    for (int t = 0; t < 4; ++t) {
+        cw[t] = odd ? kPc[t] : kQc[t];
+        xo[t] = mirror_half(cs + t, nc) * K;
     }
-    const int mc = mf / 2;
-    const int clo = pys[j0];
-    int chi = pys[j1 - 1] + WP;
-    if (chi > mc) chi = mc;
-    float acc[TJ];
-#pragma unroll
-    for (int q = 0; q < TJ; ++q) acc[q] = 0.f;
+    const int j0 = js * TJ;
+    const int u0 = j0 >> 1;
     const size_t crow = (size_t)nc * K;
-    for (int jc = clo; jc < chi; ++jc) {
-        const float* row = coarse + kk + (size_t)jc * crow;
-        float xv = 0.f;
+    float xv[NR];
 #pragma unroll
-        for (int u = 0; u < WP; ++u) xv = __builtin_fmaf(xw[u], row[xo[u]], xv);
+    for (int t = 0; t < NR; ++t) {
+        const int r = mirror_half(u0 - 2 + t, mc);
+        const float* row = coarse + kk + (size_t)r * crow;
+        float a = 0.f;
 #pragma unroll
-        for (int q = 0; q < TJ; ++q) {
-            const int j = j0 + q;
-            if (j < j1) {
-                const int d = jc - pys[j];
-                if (d >= 0 && d < WP) acc[q] = __builtin_fmaf(pyw[j * WP + d], xv, acc[q]);
-            }
-        }
+        for (int c = 0; c < 4; ++c) a = __builtin_fmaf(cw[c], row[xo[c]], a);
+        xv[t] = a;
     }
     if (!kvalid) return;
     const size_t frow = (size_t)nf * K;
 #pragma unroll
     for (int q = 0; q < TJ; ++q) {
         const int j = j0 + q;
-        if (j < j1) {
+        if (j < mf) {
+            float a;
+            if ((q & 1) == 0)
+                a = kQc[0] * xv[q / 2] + kQc[1] * xv[q / 2 + 1] + kQc[2] * xv[q / 2 + 2] +
+                    kQc[3] * xv[q / 2 + 3];
+            else
+                a = kPc[0] * xv[(q + 1) / 2] + kPc[1] * xv[(q + 1) / 2 + 1] +
+                    kPc[2] * xv[(q + 1) / 2 + 2] + kPc[3] * xv[(q + 1) / 2 + 3];
             const size_t e = kk + (size_t)j * frow + (size_t)i * K;
-            out[e] = (addend ? addend[e] : 0.f) + acc[q];
+            out[e] = (addend ? addend[e] : 0.f) + a;
         }
+    }
+}
+
+// restriction weights of fine rows 2J-3 .. 2J+4 onto interior coarse row J
+__constant__ float kW8[8] = {-3.f / 128.f, -9.f / 128.f, 29.f / 128.f, 111.f / 128.f,
+                             111.f / 128.f, 29.f / 128.f, -9.f / 128.f, -3.f / 128.f};
+
+// ---------------------------------------------------------------------------
+// Restriction R = Fold_2 o P^T (fine -> coarse). One lane computes a tile of
+// TI2 x TJ2 coarse points for its k. Tiles clear of the fold rows/columns
+// (the two at each edge) use the 8-point interior stencil: fine rows stream
+// through once, each is x-restricted for the TI2 columns and added to the
+// coarse rows whose window holds it. Tiles at an edge use the window tables,
+// which carry the folded mirror contributions.
+// ---------------------------------------------------------------------------
+template <int TI2, int TJ2>
+__global__ void __launch_bounds__(kBlock)
+restrict2_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
+                 int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
+                 const int* __restrict__ rys, const float* __restrict__ ryw, int nkc,
+                 int ntile_i, int nstrip, int nitems) {
+    constexpr int WR = kRWidthDev;
+    constexpr int NCF = 2 * TI2 + 6;
+    constexpr int NRF = 2 * TJ2 + 6;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int kc = item % nkc;
+    const int rest = item / nkc;
+    const int ti = rest % ntile_i;
+    const int js = rest / ntile_i;
+    const int k = kc * kWave + lane;
+    const bool kvalid = k < K;
+    const int kk = kvalid ? k : K - 1;
+    const int nc = nf / 2, mc = mf / 2;
+    const int I0 = ti * TI2, J0 = js * TJ2;
+    const size_t frow = (size_t)nf * K;
+    const bool interior = I0 >= 2 && I0 + TI2 <= nc - 2 && J0 >= 2 && J0 + TJ2 <= mc - 2;
+    if (interior) {
+        float acc[TJ2][TI2];
+#pragma unroll
+        for (int q = 0; q < TJ2; ++q)
+#pragma unroll
+            for (int c = 0; c < TI2; ++c) acc[q][c] = 0.f;
+        const float* base = fine + kk + (size_t)(2 * I0 - 3) * K + (size_t)(2 * J0 - 3) * frow;
+#pragma unroll
+        for (int t = 0; t < NRF; ++t) {
+            const float* row = base + (size_t)t * frow;
+            float v[NCF];
+#pragma unroll
+            for (int c = 0; c < NCF; ++c) v[c] = row[(size_t)c * K];
+#pragma unroll
+            for (int c = 0; c < TI2; ++c) {
+                float xr = 0.f;
+#pragma unroll
+                for (int s8 = 0; s8 < 8; ++s8) xr = __builtin_fmaf(kW8[s8], v[2 * c + s8], xr);
+#pragma unroll
+                for (int q = 0; q < TJ2; ++q)
+                    if (t - 2 * q >= 0 && t - 2 * q < 8)
+                        acc[q][c] = __builtin_fmaf(kW8[t - 2 * q], xr, acc[q][c]);
+            }
+        }
+        if (!kvalid) return;
+#pragma unroll
+        for (int q = 0; q < TJ2; ++q)
+#pragma unroll
+            for (int c = 0; c < TI2; ++c)
+                coarse[kk + (size_t)K * ((size_t)(I0 + c) + (size_t)nc * (J0 + q))] = acc[q][c];
+        return;
+    }
+    // edge tile: window tables, one coarse column at a time
+    const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
+    const int jlo = rys[J0];
+    int jhi = rys[J1 - 1] + WR;
+    if (jhi > mf) jhi = mf;
+    for (int c = 0; c < TI2; ++c) {
+        const int I = I0 + c;
+        if (I >= nc) break;
+        const int xs = rxs[I];
+        float xw[WR];
+        int xo[WR];
+#pragma unroll
+        for (int t = 0; t < WR; ++t) {
+            xw[t] = rxw[I * WR + t];
+            const int ii = xs + t < nf ? xs + t : nf - 1;
+            xo[t] = ii * K;
+        }
+        float acc[TJ2];
+#pragma unroll
+        for (int q = 0; q < TJ2; ++q) acc[q] = 0.f;
+        for (int jj = jlo; jj < jhi; ++jj) {
+            const float* row = fine + kk + (size_t)jj * frow;
+            float xr = 0.f;
+#pragma unroll
+            for (int t = 0; t < WR; ++t) xr = __builtin_fmaf(xw[t], row[xo[t]], xr);
+#pragma unroll
+            for (int q = 0; q < TJ2; ++q) {
+                const int J = J0 + q;
+                if (J < J1) {
+                    const int d = jj - rys[J];
+                    if (d >= 0 && d < WR) acc[q] = __builtin_fmaf(ryw[J * WR + d], xr, acc[q]);
+                }
+            }
+        }
+        if (kvalid)
+#pragma unroll
+            for (int q = 0; q < TJ2; ++q) {
+                const int J = J0 + q;
+                if (J < J1) coarse[kk + (size_t)K * ((size_t)I + (size_t)nc * J)] = acc[q];
+            }
     }
 }
 
@@ -349,60 +316,35 @@ unsigned grid_for(long long count) {
     return (unsigned)g;
 }
 
-template <int H, int TI>
-hipError_t launch_radial_h(const RadialArgs& a, hipStream_t st) {
-    const int ntile_i = (a.n + TI - 1) / TI;
-    const int ntile_j = (a.m + a.TJ - 1) / a.TJ;
-    const int nkc = (a.K + kWave - 1) / kWave;
-    const long long nitems = (long long)ntile_i * ntile_j * nkc;
-    const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    if (a.exact_disc)
-        hipLaunchKernelGGL((radial_kernel<H, TI, true>), dim3(nblocks), dim3(kBlock), 0, st, a.in,
-                           a.out, a.wfield, a.wconst, a.K, a.n, a.m, a.TJ, ntile_i, ntile_j,
-                           (int)nitems, a.coef);
-    else
-        hipLaunchKernelGGL((radial_kernel<H, TI, false>), dim3(nblocks), dim3(kBlock), 0, st, a.in,
-                           a.out, a.wfield, a.wconst, a.K, a.n, a.m, a.TJ, ntile_i, ntile_j,
-                           (int)nitems, a.coef);
-    return hipGetLastError();
-}
-
 }  // namespace
 
 hipError_t launch_radial(const RadialArgs& a, hipStream_t st) {
     switch (a.H) {
-        case 1: return launch_radial_h<1, 4>(a, st);
-        case 2: return launch_radial_h<2, 4>(a, st);
-        case 3: return launch_radial_h<3, 4>(a, st);
-        case 4: return launch_radial_h<4, 4>(a, st);
-        case 5: return launch_radial_h<5, 4>(a, st);
-        case 6: return launch_radial_h<6, 4>(a, st);
-        case 7: return launch_radial_h<7, 4>(a, st);
-        case 8: return launch_radial_h<8, 4>(a, st);
-        case 9: return launch_radial_h<9, 4>(a, st);
-        case 10: return launch_radial_h<10, 4>(a, st);
+        case 1: return launch_radial_H1(a, st);
+        case 2: return launch_radial_H2(a, st);
+        case 3: return launch_radial_H3(a, st);
+        case 4: return launch_radial_H4(a, st);
+        case 5: return launch_radial_H5(a, st);
+        case 6: return launch_radial_H6(a, st);
+        case 7: return launch_radial_H7(a, st);
+        case 8: return launch_radial_H8(a, st);
+        case 9: return launch_radial_H9(a, st);
+        case 10: return launch_radial_H10(a, st);
         default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_vertical(const float* in, float* out, const float* coef, int K, int lm,
-                           int nfilt, int hz, long long npts, hipStream_t st) {
-    dim3 grid((unsigned)((npts + kWavesPerBlock - 1) / kWavesPerBlock),
-              (unsigned)((K + kWave - 1) / kWave));
-    hipLaunchKernelGGL(vertical_kernel, grid, dim3(kBlock), 0, st, in, out, coef, K, lm, nfilt,
-                       hz, npts);
-    return hipGetLastError();
-}
-
 hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
-    constexpr int TJ2 = 8;
+    constexpr int TI2 = 4, TJ2 = 8;
     const int nc = a.nf / 2, mc = a.mf / 2;
+    const int ntile_i = (nc + TI2 - 1) / TI2;
     const int nstrip = (mc + TJ2 - 1) / TJ2;
     const int nkc = (a.K + kWave - 1) / kWave;
-    const long long nitems = (long long)nc * nstrip * nkc;
+    const long long nitems = (long long)ntile_i * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(restrict_kernel<TJ2>, dim3(nblocks), dim3(kBlock), 0, st, a.src, a.dst,
-                       a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nstrip, (int)nitems);
+    hipLaunchKernelGGL((restrict2_kernel<TI2, TJ2>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
+                       a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nkc, ntile_i, nstrip,
+                       (int)nitems);
     return hipGetLastError();
 }
 
@@ -412,9 +354,38 @@ hipError_t launch_prolong(const TransferArgs& a, const float* addend, hipStream_
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)a.nf * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(prolong_kernel<TJ>, dim3(nblocks), dim3(kBlock), 0, st, a.src, addend,
-                       a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nstrip, (int)nitems);
+    hipLaunchKernelGGL(prolong2_kernel<TJ>, dim3(nblocks), dim3(kBlock), 0, st, a.src, addend,
+                       a.dst, a.K, a.nf, a.mf, nkc, nstrip, (int)nitems);
     return hipGetLastError();
+}
+
+template <int HB>
+hipError_t launch_vband_h(const VBandArgs& a, hipStream_t st) {
+    const long long N = (long long)a.K * a.npts;
+    int cpp = 0, pspan = 0;
+    long long nblocks;
+    if (a.K > kVSpan) {
+        cpp = (a.K + kVSpan - 1) / kVSpan;
+        nblocks = a.npts * cpp;
+    } else {
+        const int ppb = kVSpan / a.K;
+        pspan = ppb * a.K;
+        nblocks = (N + pspan - 1) / pspan;
+    }
+    hipLaunchKernelGGL(vband_kernel<HB>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a.in,
+                       a.out, a.band, a.wint, a.ilo, a.ihi, a.K, a.lm, a.nfilt, N, cpp, pspan);
+    return hipGetLastError();
+}
+
+hipError_t launch_vband(const VBandArgs& a, hipStream_t st) {
+    switch (a.HB) {
+#define MGBF_VB(h) case h: return launch_vband_h<h>(a, st);
+        MGBF_VB(1) MGBF_VB(2) MGBF_VB(3) MGBF_VB(4) MGBF_VB(5) MGBF_VB(6) MGBF_VB(7)
+        MGBF_VB(8) MGBF_VB(9) MGBF_VB(10) MGBF_VB(11) MGBF_VB(12) MGBF_VB(13) MGBF_VB(14)
+        MGBF_VB(15) MGBF_VB(16) MGBF_VB(17) MGBF_VB(18) MGBF_VB(19) MGBF_VB(20)
+#undef MGBF_VB
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_scale(const float* in, float* out, const float* wfield, float wconst,

@@ -19,12 +19,35 @@ struct RadialArgs {
     float* out;
     const float* wfield;  // per-point scale, or nullptr
     float wconst;         // constant scale when wfield is nullptr
+    const float* coarse;  // next coarser generation to prolong and add, or nullptr
+    const float* band;    // vertical band (lm x (2 vhb + 1)) applied at the end, or nullptr
+    int vhb;              // its half-width
+    int lm, nfilt;        // column length and vertically filtered part of the stack
     int K, n, m;
     int H;                // max(hx, hy)
     int TJ;               // output rows per wavefront
     bool exact_disc;
     RadialCoef coef;
 };
+
+// kernel argument record of the radial kernel
+struct RadialParams {
+    const float* in;
+    float* out;
+    const float* wfield;
+    const float* coarse;
+    const float* band;
+    float wconst;
+    int K, n, m, TJ, lm, nfilt;
+    int ntile_i, ntile_j, nitems;
+    RadialCoef W;
+};
+
+#define MGBF_DECLARE_RADIAL(h) hipError_t launch_radial_H##h(const RadialArgs& a, hipStream_t st);
+MGBF_DECLARE_RADIAL(1) MGBF_DECLARE_RADIAL(2) MGBF_DECLARE_RADIAL(3) MGBF_DECLARE_RADIAL(4)
+MGBF_DECLARE_RADIAL(5) MGBF_DECLARE_RADIAL(6) MGBF_DECLARE_RADIAL(7) MGBF_DECLARE_RADIAL(8)
+MGBF_DECLARE_RADIAL(9) MGBF_DECLARE_RADIAL(10)
+#undef MGBF_DECLARE_RADIAL
 
 struct TransferArgs {
     const float* src;
@@ -36,9 +59,25 @@ struct TransferArgs {
     const float* yw;
 };
 
+constexpr int kMaxVBand = 20;   // half-band of Vf Vf^T for hz <= 10
+
+struct VBandW {
+    float w[2 * kMaxVBand + 1];
+};
+
+struct VBandArgs {
+    const float* in;
+    float* out;
+    const float* band;   // lm x (2HB+1) rows
+    VBandW wint;         // interior row
+    int ilo, ihi;        // rows using wint
+    int HB;
+    int K, lm, nfilt;
+    long long npts;
+};
+
 hipError_t launch_radial(const RadialArgs& a, hipStream_t st);
-hipError_t launch_vertical(const float* in, float* out, const float* coef, int K, int lm,
-                           int nfilt, int hz, long long npts, hipStream_t st);
+hipError_t launch_vband(const VBandArgs& a, hipStream_t st);
 hipError_t launch_restrict(const TransferArgs& a, hipStream_t st);
 hipError_t launch_prolong(const TransferArgs& a, const float* addend, hipStream_t st);
 hipError_t launch_scale(const float* in, float* out, const float* wfield, float wconst,

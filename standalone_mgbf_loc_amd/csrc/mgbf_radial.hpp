@@ -1,0 +1,244 @@
+// Radial 2-D beta filter kernel (template), instantiated per half-width H in
+// mgbf_radial_inst.hip.
+#pragma once
+
+#include "mgbf_device.hpp"
+#include "mgbf_launch.hpp"
+
+namespace mgbf {
+
+template <int H>
+__host__ __device__ constexpr int disc_extent(int b) {
+    int a = H;
+    while (a >= 0 && a * a + b * b >= (H + 1) * (H + 1)) --a;
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// Radial 2-D beta filter F (gather form, half-sample mirror halo) with the
+// epilogue of its generation:
+//   s(k,i,j) = scale(k,i,j) * sum_{a,b} w(|a|,|b|) in(k, mx(i+a), my(j+b))
+//              [+ (P coarse)(k,i,j)]                       (PRO)
+//   out(k,i,j) = sum_d c(L,d) s(k+d,i,j) over the column    (VHB > 0)
+// scale = wfield (per point) or wconst. One wave computes TI x TJ outputs for
+// its 64 k: input rows stream through registers; for every input row the
+// x-pair sums P_a = v(i+a)+v(i-a) give the per-row partial sums
+// Z_b = sum_a w(a,b) P_a, which are added to the output rows j = r-b and r+b
+// held in a ring of 2H+1 accumulator rows.
+// PRO adds the prolongation of the next coarser generation (direct1 on the
+// 2-point mirrored coarse field): a window of 4 x-interpolated coarse rows
+// slides down with the output rows, one new coarse row every other row.
+// VHB applies a vertical band of half-width VHB across the lanes of the wave
+// (each wave then holds whole columns: lm divides 64 and K % 64 == 0).
+// ---------------------------------------------------------------------------
+template <int H, int TI, bool DISC, bool PRO, int VHB>
+__global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
+    constexpr int NR = 2 * H + 1;
+    constexpr int NC = TI + 2 * H;
+    constexpr int NCC = TI / 2 + 4;   // coarse columns of the tile (TI even)
+    const int lane = threadIdx.x & (kWave - 1);
+    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    if (item >= p.nitems) return;
+    const int K = p.K, n = p.n, m = p.m;
+    const int ti = item % p.ntile_i;
+    const int rest = item / p.ntile_i;
+    const int tj = rest % p.ntile_j;
+    const int kc = rest / p.ntile_j;
+    const int k = kc * kWave + lane;
+    const bool kvalid = k < K;
+    const int kk = kvalid ? k : K - 1;
+    const int i0 = ti * TI, j0 = tj * p.TJ;
+    const size_t rowstride = (size_t)n * K;
+
+    int coff[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) coff[c] = mirror_half(i0 - H + c, n) * K;
+
+    float acc[NR][TI];
+#pragma unroll
+    for (int s = 0; s < NR; ++s)
+#pragma unroll
+        for (int t = 0; t < TI; ++t) acc[s][t] = 0.f;
+
+    // prolongation window
+    float XV[4][TI];
+    int ccol[NCC];
+    int cbase = 0;
+    const int nc = n / 2, mc = m / 2;
+    const size_t crow = (size_t)nc * K;
+    auto load_xrow = [&](int r, float* dst) {
+        const float* rowp = p.coarse + kk + (size_t)mirror_half(r, mc) * crow;
+        float cv[NCC];
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) cv[c] = rowp[ccol[c]];
+#pragma unroll
+        for (int t = 0; t < TI; ++t) {
+            if ((t & 1) == 0)
+                dst[t] = kQc[0] * cv[t / 2] + kQc[1] * cv[t / 2 + 1] + kQc[2] * cv[t / 2 + 2] +
+                         kQc[3] * cv[t / 2 + 3];
+            else
+                dst[t] = kPc[0] * cv[(t + 1) / 2] + kPc[1] * cv[(t + 1) / 2 + 1] +
+                         kPc[2] * cv[(t + 1) / 2 + 2] + kPc[3] * cv[(t + 1) / 2 + 3];
+        }
+    };
+    if constexpr (PRO) {
+        const int v0 = i0 / 2;
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) ccol[c] = mirror_half(v0 - 2 + c, nc) * K;
+        cbase = ((j0 + 1) >> 1) - 2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) load_xrow(cbase + u, XV[u]);
+    }
+
+    // vertical band coefficients of this lane's level (identity past nfilt)
+    constexpr int NVB = VHB > 0 ? 2 * VHB + 1 : 1;
+    float vc[NVB];
+    if constexpr (VHB > 0) {
+        if (k < p.nfilt) {
+            const float* row = p.band + (k % p.lm) * NVB;
+#pragma unroll
+            for (int d = 0; d < NVB; ++d) vc[d] = row[d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < NVB; ++d) vc[d] = d == VHB ? 1.f : 0.f;
+        }
+    }
+
+    const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
+    const int nsteps = tj_eff + 2 * H;
+    const float* base_in = p.in + kk;
+    float* base_out = p.out + kk;
+    const float* base_w = p.wfield ? p.wfield + kk : nullptr;
+
+    for (int rb = 0; rb < nsteps; rb += NR) {
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            const int rr = rb + s;
+            if (rr < nsteps) {
+                const int r = mirror_half(j0 - H + rr, m);
+                const float* row = base_in + (size_t)r * rowstride;
+                float v[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) v[c] = row[coff[c]];
+#pragma unroll
+                for (int t = 0; t < TI; ++t) {
+                    float P[H + 1];
+                    P[0] = v[t + H];
+#pragma unroll
+                    for (int a = 1; a <= H; ++a) P[a] = v[t + H + a] + v[t + H - a];
+#pragma unroll
+                    for (int b = 0; b <= H; ++b) {
+                        const int A = DISC ? disc_extent<H>(b) : H;
+                        float z = p.W.w[b][0] * P[0];
+#pragma unroll
+                        for (int a = 1; a <= H; ++a)
+                            if (a <= A) z = __builtin_fmaf(p.W.w[b][a], P[a], z);
+                        acc[(s + b) % NR][t] += z;
+                        if (b > 0) acc[(s + NR - b) % NR][t] += z;
+                    }
+                }
+                // output row j = j0 + rr - 2H is complete
+                const int jrel = rr - 2 * H;
+                constexpr int slot_off = NR - H;   // (s - H) mod NR
+                if (jrel >= 0 && jrel < tj_eff) {
+                    const int j = j0 + jrel;
+                    float o[TI];
+#pragma unroll
+                    for (int t = 0; t < TI; ++t) {
+                        const size_t e = (size_t)j * rowstride + (size_t)(i0 + t < n ? i0 + t : n - 1) * K;
+                        const float sc = base_w ? base_w[e] : p.wconst;
+                        o[t] = sc * acc[(s + slot_off) % NR][t];
+                    }
+                    if constexpr (PRO) {
+                        const int nb = ((j + 1) >> 1) - 2;
+                        if (nb != cbase) {
+#pragma unroll
+                            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                                for (int t = 0; t < TI; ++t) XV[u][t] = XV[u + 1][t];
+                            load_xrow(nb + 3, XV[3]);
+                            cbase = nb;
+                        }
+                        const bool oj = (j & 1) != 0;
+                        const float y0 = oj ? kPc[0] : kQc[0], y1 = oj ? kPc[1] : kQc[1];
+                        const float y2 = oj ? kPc[2] : kQc[2], y3 = oj ? kPc[3] : kQc[3];
+#pragma unroll
+                        for (int t = 0; t < TI; ++t)
+                            o[t] += y0 * XV[0][t] + y1 * XV[1][t] + y2 * XV[2][t] + y3 * XV[3][t];
+                    }
+                    if constexpr (VHB > 0) {
+#pragma unroll
+                        for (int t = 0; t < TI; ++t) {
+                            float rsum = 0.f;
+#pragma unroll
+                            for (int d = 0; d < NVB; ++d) {
+                                int src = lane + d - VHB;
+                                src = src < 0 ? 0 : (src > kWave - 1 ? kWave - 1 : src);
+                                rsum = __builtin_fmaf(vc[d], __shfl(o[t], src, kWave), rsum);
+                            }
+                            o[t] = rsum;
+                        }
+                    }
+                    if (kvalid) {
+#pragma unroll
+                        for (int t = 0; t < TI; ++t)
+                            if (i0 + t < n) base_out[(size_t)j * rowstride + (size_t)(i0 + t) * K] = o[t];
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TI; ++t) acc[(s + slot_off) % NR][t] = 0.f;
+            }
+        }
+    }
+}
+
+template <int H, int TI, bool DISC, bool PRO, int VHB>
+hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
+    RadialParams p{};
+    p.in = a.in;
+    p.out = a.out;
+    p.wfield = a.wfield;
+    p.wconst = a.wconst;
+    p.coarse = a.coarse;
+    p.band = a.band;
+    p.K = a.K;
+    p.n = a.n;
+    p.m = a.m;
+    p.TJ = a.TJ;
+    p.lm = a.lm;
+    p.nfilt = a.nfilt;
+    p.W = a.coef;
+    p.ntile_i = (a.n + TI - 1) / TI;
+    p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
+    const int nkc = (a.K + kWave - 1) / kWave;
+    const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
+    p.nitems = (int)nitems;
+    const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
+    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    return hipGetLastError();
+}
+
+// Dispatch over the built variants for one H.
+template <int H>
+hipError_t launch_radial_h(const RadialArgs& a, hipStream_t st) {
+    constexpr int TI = 4;
+    const bool pro = a.coarse != nullptr;
+    const int vhb = a.band ? a.vhb : 0;
+    if (a.exact_disc) {
+        if (vhb == 0) return pro ? launch_radial_variant<H, TI, true, true, 0>(a, st)
+                                 : launch_radial_variant<H, TI, true, false, 0>(a, st);
+        if (!pro) return hipErrorInvalidValue;
+        switch (vhb) {
+            case 3: return launch_radial_variant<H, TI, true, true, 3>(a, st);
+            case 4: return launch_radial_variant<H, TI, true, true, 4>(a, st);
+            case 6: return launch_radial_variant<H, TI, true, true, 6>(a, st);
+            case 8: return launch_radial_variant<H, TI, true, true, 8>(a, st);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    if (vhb != 0) return hipErrorInvalidValue;
+    return pro ? launch_radial_variant<H, TI, false, true, 0>(a, st)
+               : launch_radial_variant<H, TI, false, false, 0>(a, st);
+}
+
+}  // namespace mgbf
