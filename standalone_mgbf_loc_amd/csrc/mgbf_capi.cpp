@@ -73,6 +73,7 @@ struct mgbf_ctx {
     mgbf::VBandW band_int[3] = {};
     int band_hb[3] = {}, band_ilo[3] = {}, band_ihi[3] = {};
     bool commute = true;     // weights constant along the stack: B = Vf Vf^T H
+    int fuse = 2;            // epilogue fusion of the up-sweep: 0 none, 1 +P, 2 +P and band
     const int* d_rxs[mgbf::kMaxGen] = {};
     const float* d_rxw[mgbf::kMaxGen] = {};
     const int* d_rys[mgbf::kMaxGen] = {};
@@ -270,10 +271,11 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band,
             bool sqrt_w, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    const bool fuse = band >= 0 && can_fuse_band(c, band);
+    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band);
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
+        const float* pro = c->fuse >= 1 ? coarse : nullptr;
         const bool direct = g == 0 && (fuse || band < 0);
         float* dst = direct ? y : U + c->goff[g];
         const int fb = (g == 0 && fuse) ? band : -1;
@@ -286,15 +288,16 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band,
         }
         if (twice) {
             CALL(radial(c, g, src, T, c->wfield[g], c->wconst[g], st, MGBF_T_HFILTT));
-            CALL(radial(c, g, T, dst, nullptr, 1.0f, st, MGBF_T_HFILT, coarse, fb));
+            CALL(radial(c, g, T, dst, nullptr, 1.0f, st, MGBF_T_HFILT, pro, fb));
         } else {
             if (sqrt_w) {
                 HIP_TRY(mgbf::launch_scale(src, T, c->wfield[g], c->wconst[g], 1, c->gsize[g], st),
                         "scale launch");
                 src = T;
             }
-            CALL(radial(c, g, src, dst, nullptr, 1.0f, st, MGBF_T_HFILT, coarse, fb));
+            CALL(radial(c, g, src, dst, nullptr, 1.0f, st, MGBF_T_HFILT, pro, fb));
         }
+        if (coarse && !pro) CALL(prolong_g(c, g, coarse, dst, dst, st));
     }
     if (band >= 0 && !fuse) return vertical(c, U, y, band, st);
     return MGBF_OK;
@@ -435,6 +438,10 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         // weights that are constant along the stack
         const char* ev = std::getenv("MGBF_EXACT_ORDER");
         c->commute = !(ev && ev[0] == '1');
+        // MGBF_FUSE=0|1|2 selects how much of the up-sweep rides in the
+        // epilogue of the last radial pass (all levels compute the same)
+        const char* fv = std::getenv("MGBF_FUSE");
+        if (fv && fv[0] >= '0' && fv[0] <= '2') c->fuse = fv[0] - '0';
     }
     long long off = 0;
     for (int g = 0; g < c->gm; ++g) {

@@ -31,4 +31,9 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
 constexpr float kPc[4] = {-9.f / 128.f, 111.f / 128.f, 29.f / 128.f, -3.f / 128.f};
 constexpr float kQc[4] = {-3.f / 128.f, 29.f / 128.f, 111.f / 128.f, -9.f / 128.f};
 
+// Restriction weights of fine rows 2J-3 .. 2J+4 onto an interior coarse row J
+// (the transpose of the two interpolation stencils above).
+constexpr float kW8[8] = {-3.f / 128.f, -9.f / 128.f, 29.f / 128.f, 111.f / 128.f,
+                          111.f / 128.f, 29.f / 128.f, -9.f / 128.f, -3.f / 128.f};
+
 }  // namespace mgbf

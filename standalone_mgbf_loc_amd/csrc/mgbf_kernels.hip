@@ -32,7 +32,7 @@ namespace {
 constexpr int kVPer = 8;                    // elements per thread
 constexpr int kVSpan = kBlock * kVPer;      // elements per block
 
-template <int HB>
+template <int HB, bool FIXL>
 __global__ void __launch_bounds__(kBlock)
 vband_kernel(const float* __restrict__ in, float* __restrict__ out,
              const float* __restrict__ band, VBandW wint, int ilo, int ihi, int K, int lm,
@@ -69,6 +69,26 @@ vband_kernel(const float* __restrict__ in, float* __restrict__ out,
     for (int r = 0; r < kVPer; ++r) s[tid + r * kBlock] = tmp[r];
     if (tid < 2 * HB) s[kVSpan + tid] = tail;
     __syncthreads();
+    if constexpr (FIXL) {
+        // every column is filtered and lm divides the block: the level of
+        // each element of this thread is tid mod lm, its band row is fixed
+        float vc[NB];
+        const float* c = band + (tid % lm) * NB;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) vc[t] = c[t];
+#pragma unroll
+        for (int r = 0; r < kVPer; ++r) {
+            const int l = tid + r * kBlock;
+            if (l < span) {
+                const float* row = s + l + HB;
+                float v = 0.f;
+#pragma unroll
+                for (int t = 0; t < NB; ++t) v = __builtin_fmaf(vc[t], row[t - HB], v);
+                out[e0 + l] = v;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < kVPer; ++r) {
         const int l = tid + r * kBlock;
@@ -160,27 +180,58 @@ prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* ou
     }
 }
 
-// restriction weights of fine rows 2J-3 .. 2J+4 onto interior coarse row J
-__constant__ float kW8[8] = {-3.f / 128.f, -9.f / 128.f, 29.f / 128.f, 111.f / 128.f,
-                             111.f / 128.f, 29.f / 128.f, -9.f / 128.f, -3.f / 128.f};
 
 // ---------------------------------------------------------------------------
-// Restriction R = Fold_2 o P^T (fine -> coarse). One lane computes a tile of
-// TI2 x TJ2 coarse points for its k. Tiles clear of the fold rows/columns
-// (the two at each edge) use the 8-point interior stencil: fine rows stream
-// through once, each is x-restricted for the TI2 columns and added to the
-// coarse rows whose window holds it. Tiles at an edge use the window tables,
-// which carry the folded mirror contributions.
+// Restriction R = Fold_2 o P^T (fine -> coarse). One lane computes TI2 coarse
+// columns x TJ2 coarse rows for its k, coarse row by coarse row. An interior
+// coarse row J is the 8-point stencil kW8 over the x-restricted fine rows
+// 2J-3 .. 2J+4, which a sliding window keeps in registers: the next row
+// needs two new fine rows. Rows and columns within two of an edge carry the
+// folded mirror contributions and use the window tables.
 // ---------------------------------------------------------------------------
+// Edge tile of the restriction: window tables in x and y, one coarse point
+// at a time (only the tiles within two columns of an edge come here).
+template <int TI2>
+__device__ void restrict_edge_tile(const float* __restrict__ fine, float* __restrict__ coarse,
+                                   int kk, bool kvalid, int K, int nf, int mf, int I0, int J0,
+                                   int J1, const int* __restrict__ rxs,
+                                   const float* __restrict__ rxw, const int* __restrict__ rys,
+                                   const float* __restrict__ ryw) {
+    constexpr int WR = kRWidthDev;
+    const int nc = nf / 2;
+    const size_t frow = (size_t)nf * K;
+    for (int c = 0; c < TI2; ++c) {
+        const int I = I0 + c;
+        if (I >= nc) break;
+        const int xs = rxs[I];
+        for (int J = J0; J < J1; ++J) {
+            const int ys = rys[J];
+            float acc = 0.f;
+            for (int ty = 0; ty < WR; ++ty) {
+                const int jj = ys + ty < mf ? ys + ty : mf - 1;
+                const float wy = ys + ty < mf ? ryw[J * WR + ty] : 0.f;
+                const float* row = fine + kk + (size_t)jj * frow;
+                float a = 0.f;
+                for (int tx = 0; tx < WR; ++tx) {
+                    const int ii = xs + tx < nf ? xs + tx : nf - 1;
+                    const float wx = xs + tx < nf ? rxw[I * WR + tx] : 0.f;
+                    a = __builtin_fmaf(wx, row[(size_t)ii * K], a);
+                }
+                acc = __builtin_fmaf(wy, a, acc);
+            }
+            if (kvalid) coarse[kk + (size_t)K * ((size_t)I + (size_t)nc * J)] = acc;
+        }
+    }
+}
+
 template <int TI2, int TJ2>
 __global__ void __launch_bounds__(kBlock)
-restrict2_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
+restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
                  int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
                  const int* __restrict__ rys, const float* __restrict__ ryw, int nkc,
                  int ntile_i, int nstrip, int nitems) {
     constexpr int WR = kRWidthDev;
     constexpr int NCF = 2 * TI2 + 6;
-    constexpr int NRF = 2 * TJ2 + 6;
     const int lane = threadIdx.x & (kWave - 1);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (item >= nitems) return;
@@ -193,80 +244,72 @@ restrict2_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
     const int kk = kvalid ? k : K - 1;
     const int nc = nf / 2, mc = mf / 2;
     const int I0 = ti * TI2, J0 = js * TJ2;
-    const size_t frow = (size_t)nf * K;
-    const bool interior = I0 >= 2 && I0 + TI2 <= nc - 2 && J0 >= 2 && J0 + TJ2 <= mc - 2;
-    if (interior) {
-        float acc[TJ2][TI2];
-#pragma unroll
-        for (int q = 0; q < TJ2; ++q)
-#pragma unroll
-            for (int c = 0; c < TI2; ++c) acc[q][c] = 0.f;
-        const float* base = fine + kk + (size_t)(2 * I0 - 3) * K + (size_t)(2 * J0 - 3) * frow;
-#pragma unroll
-        for (int t = 0; t < NRF; ++t) {
-            const float* row = base + (size_t)t * frow;
-            float v[NCF];
-#pragma unroll
-            for (int c = 0; c < NCF; ++c) v[c] = row[(size_t)c * K];
-#pragma unroll
-            for (int c = 0; c < TI2; ++c) {
-                float xr = 0.f;
-#pragma unroll
-                for (int s8 = 0; s8 < 8; ++s8) xr = __builtin_fmaf(kW8[s8], v[2 * c + s8], xr);
-#pragma unroll
-                for (int q = 0; q < TJ2; ++q)
-                    if (t - 2 * q >= 0 && t - 2 * q < 8)
-                        acc[q][c] = __builtin_fmaf(kW8[t - 2 * q], xr, acc[q][c]);
-            }
-        }
-        if (!kvalid) return;
-#pragma unroll
-        for (int q = 0; q < TJ2; ++q)
-#pragma unroll
-            for (int c = 0; c < TI2; ++c)
-                coarse[kk + (size_t)K * ((size_t)(I0 + c) + (size_t)nc * (J0 + q))] = acc[q][c];
+    const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
+    if (!(I0 >= 2 && I0 + TI2 <= nc - 2)) {
+        restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw);
         return;
     }
-    // edge tile: window tables, one coarse column at a time
-    const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
-    const int jlo = rys[J0];
-    int jhi = rys[J1 - 1] + WR;
-    if (jhi > mf) jhi = mf;
-    for (int c = 0; c < TI2; ++c) {
-        const int I = I0 + c;
-        if (I >= nc) break;
-        const int xs = rxs[I];
-        float xw[WR];
-        int xo[WR];
+    const size_t frow = (size_t)nf * K;
+    // interior columns: fine columns 2*I0-3 .. 2*I0+2*TI2+2
+    const float* base = fine + kk + (size_t)(2 * I0 - 3) * K;
+    auto xrow = [&](int fr, float* out) {
+        const float* row = base + (size_t)fr * frow;
+        float v[NCF];
 #pragma unroll
-        for (int t = 0; t < WR; ++t) {
-            xw[t] = rxw[I * WR + t];
-            const int ii = xs + t < nf ? xs + t : nf - 1;
-            xo[t] = ii * K;
+        for (int c = 0; c < NCF; ++c) v[c] = row[(size_t)c * K];
+#pragma unroll
+        for (int c = 0; c < TI2; ++c) {
+            float a = 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[2 * c + t], a);
+            out[c] = a;
         }
-        float acc[TJ2];
+    };
+    float win[8][TI2];   // x-restricted fine rows 2J-3 .. 2J+4
+    int wbase = -1000;   // fine row held in win[0]
+    for (int J = J0; J < J1; ++J) {
+        float acc[TI2];
+        if (J >= 2 && J <= mc - 3) {
+            const int fb = 2 * J - 3;
+            if (fb == wbase + 2) {
 #pragma unroll
-        for (int q = 0; q < TJ2; ++q) acc[q] = 0.f;
-        for (int jj = jlo; jj < jhi; ++jj) {
-            const float* row = fine + kk + (size_t)jj * frow;
-            float xr = 0.f;
+                for (int t = 0; t < 6; ++t)
 #pragma unroll
-            for (int t = 0; t < WR; ++t) xr = __builtin_fmaf(xw[t], row[xo[t]], xr);
+                    for (int c = 0; c < TI2; ++c) win[t][c] = win[t + 2][c];
+                xrow(fb + 6, win[6]);
+                xrow(fb + 7, win[7]);
+            } else {
 #pragma unroll
-            for (int q = 0; q < TJ2; ++q) {
-                const int J = J0 + q;
-                if (J < J1) {
-                    const int d = jj - rys[J];
-                    if (d >= 0 && d < WR) acc[q] = __builtin_fmaf(ryw[J * WR + d], xr, acc[q]);
-                }
+                for (int t = 0; t < 8; ++t) xrow(fb + t, win[t]);
             }
+            wbase = fb;
+#pragma unroll
+            for (int c = 0; c < TI2; ++c) {
+                float a = 0.f;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], win[t][c], a);
+                acc[c] = a;
+            }
+        } else {
+            // fold rows: the y window table of coarse row J
+            const int ys = rys[J];
+#pragma unroll
+            for (int c = 0; c < TI2; ++c) acc[c] = 0.f;
+            for (int t = 0; t < WR; ++t) {
+                const int jj = ys + t < mf ? ys + t : mf - 1;
+                const float wy = ys + t < mf ? ryw[J * WR + t] : 0.f;
+                float xv[TI2];
+                xrow(jj, xv);
+#pragma unroll
+                for (int c = 0; c < TI2; ++c) acc[c] = __builtin_fmaf(wy, xv[c], acc[c]);
+            }
+            wbase = -1000;
         }
-        if (kvalid)
+        if (kvalid) {
 #pragma unroll
-            for (int q = 0; q < TJ2; ++q) {
-                const int J = J0 + q;
-                if (J < J1) coarse[kk + (size_t)K * ((size_t)I + (size_t)nc * J)] = acc[q];
-            }
+            for (int c = 0; c < TI2; ++c)
+                coarse[kk + (size_t)K * ((size_t)(I0 + c) + (size_t)nc * J)] = acc[c];
+        }
     }
 }
 
@@ -335,14 +378,14 @@ hipError_t launch_radial(const RadialArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
-    constexpr int TI2 = 4, TJ2 = 8;
+    constexpr int TI2 = 4, TJ2 = 16;
     const int nc = a.nf / 2, mc = a.mf / 2;
     const int ntile_i = (nc + TI2 - 1) / TI2;
     const int nstrip = (mc + TJ2 - 1) / TJ2;
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)ntile_i * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((restrict2_kernel<TI2, TJ2>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
+    hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
                        a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nkc, ntile_i, nstrip,
                        (int)nitems);
     return hipGetLastError();
@@ -361,6 +404,7 @@ hipError_t launch_prolong(const TransferArgs& a, const float* addend, hipStream_
 
 template <int HB>
 hipError_t launch_vband_h(const VBandArgs& a, hipStream_t st) {
+    const bool fixl = a.nfilt == a.K && kBlock % a.lm == 0 && a.K % a.lm == 0;
     const long long N = (long long)a.K * a.npts;
     int cpp = 0, pspan = 0;
     long long nblocks;
@@ -372,8 +416,14 @@ hipError_t launch_vband_h(const VBandArgs& a, hipStream_t st) {
         pspan = ppb * a.K;
         nblocks = (N + pspan - 1) / pspan;
     }
-    hipLaunchKernelGGL(vband_kernel<HB>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a.in,
-                       a.out, a.band, a.wint, a.ilo, a.ihi, a.K, a.lm, a.nfilt, N, cpp, pspan);
+    if (fixl)
+        hipLaunchKernelGGL((vband_kernel<HB, true>), dim3((unsigned)nblocks), dim3(kBlock), 0, st,
+                           a.in, a.out, a.band, a.wint, a.ilo, a.ihi, a.K, a.lm, a.nfilt, N, cpp,
+                           pspan);
+    else
+        hipLaunchKernelGGL((vband_kernel<HB, false>), dim3((unsigned)nblocks), dim3(kBlock), 0, st,
+                           a.in, a.out, a.band, a.wint, a.ilo, a.ihi, a.K, a.lm, a.nfilt, N, cpp,
+                           pspan);
     return hipGetLastError();
 }
 
