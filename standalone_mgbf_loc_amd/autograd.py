@@ -1,0 +1,85 @@
+"""torch.autograd over the MGBF operator B_w = W A_w W^T.
+
+`localize(op, z, weights)` returns B_w z, where A_w scales generation g of
+the multigrid vector by weights[g] (a scalar tensor, or a per-point field of
+the generation's shape: the reference's a_diff_f / a_diff_h,
+src.mgbf/mg_intstate.f90:333-351). Gradients:
+
+    dJ/dz   = B_w g                          (B_w is symmetric)
+    dJ/dw_g = < (W^T g)_g , (W^T z)_g >      scalar weight, FP64 accumulation
+    dJ/dw_g = (W^T g)_g * (W^T z)_g          per-point weight field
+
+with g = dJ/dy. The per-generation states W^T z are recomputed in the
+backward pass, not stored. The adjoint halves are the hand-written kernels of
+the library (W^T: restriction chain + radial filter; W: radial filter +
+prolongation chain + vertical filter).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import mgbf as M
+
+
+def _set_weights(op: M.MGBF, weights):
+    for g, w in enumerate(weights, start=1):
+        if w.dim() == 0 or w.numel() == 1:
+            op.set_weights(g, value=float(w.detach().reshape(()).item()))
+        else:
+            op.set_weights(g, field=w.detach().to(torch.float32).contiguous())
+
+
+class _Localize(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, z, op, *weights):
+        if len(weights) != op.dims.gm:
+            raise ValueError(f"need {op.dims.gm} generation weights")
+        _set_weights(op, weights)
+        zc = z.detach().contiguous()
+        y = op.apply(M.OP_WAWT, zc)
+        ctx.op = op
+        ctx.save_for_backward(zc, *weights)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        op = ctx.op
+        z, *weights = ctx.saved_tensors
+        _set_weights(op, weights)
+        gy = gy.contiguous()
+        gz = op.apply(M.OP_WAWT, gy) if ctx.needs_input_grad[0] else None
+        gw = [None] * len(weights)
+        if any(ctx.needs_input_grad[2:]):
+            # recompute the per-generation states of both vectors
+            vg = op.split_mg(op.apply(M.OP_WT, gy))
+            vz = op.split_mg(op.apply(M.OP_WT, z))
+            for g, w in enumerate(weights):
+                if not ctx.needs_input_grad[2 + g]:
+                    continue
+                if g == 0 and not _filters_g1(op):
+                    gw[g] = torch.zeros_like(w)
+                elif w.dim() == 0 or w.numel() == 1:
+                    val = op.dot(vg[g], vz[g])
+                    gw[g] = torch.tensor(val, dtype=w.dtype, device=w.device).reshape(w.shape)
+                else:
+                    gw[g] = (vg[g] * vz[g]).to(w.dtype)
+        return (gz, None, *gw)
+
+
+def _filters_g1(op: M.MGBF) -> bool:
+    return op.cfg.mgbf_proc == 1 or bool(op.cfg.l_filt_g1)
+
+
+def localize(op: M.MGBF, z: torch.Tensor, weights) -> torch.Tensor:
+    """y = W A_w W^T z with differentiable z and generation weights."""
+    return _Localize.apply(z, op, *weights)
+
+
+def loss_and_grads(op: M.MGBF, a, x, y, weights):
+    """J = 1/2 ||W A_w W^T (a * x) - y||^2 and its gradients w.r.t. a and w."""
+    a = a.detach().requires_grad_(True)
+    ws = [w.detach().requires_grad_(True) for w in weights]
+    r = localize(op, a * x, ws) - y
+    J = 0.5 * (r.double() ** 2).sum()
+    J.backward()
+    return J.item(), a.grad, [w.grad for w in ws]

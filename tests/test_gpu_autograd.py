@@ -1,0 +1,141 @@
+"""Differentiation through the MGBF operator (torch.autograd over the C ABI).
+
+Loss J = 1/2 ||W A_w W^T (a * x) - y||^2 with learnable generation weights w
+and amplitude field a. J is quadratic in w and in a, so central differences
+of the FP64 oracle (h = 1e-3) are exact up to rounding: the GPU gradients
+must match them to 1e-4 relative. At the full size (16 members x 512x512x64,
+5 generations) the check is homogeneity: B_w z is linear in w and in a, so
+sum_g w_g dJ/dw_g = <r, B_w z> = sum_i a_i dJ/da_i.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O
+from standalone_mgbf_loc_amd import mgbf as M
+from standalone_mgbf_loc_amd.autograd import localize, loss_and_grads
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+
+
+def smooth(a, passes=2):
+    """3x3 box smoothing with the half-sample mirror, on (m, n, K) arrays."""
+    for _ in range(passes):
+        p = np.pad(a, ((1, 1), (1, 1), (0, 0)), mode="symmetric")
+        a = sum(p[1 + dj: p.shape[0] - 1 + dj, 1 + di: p.shape[1] - 1 + di]
+                for dj in (-1, 0, 1) for di in (-1, 0, 1)) / 9.0
+    return a
+
+
+def inputs(shape, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(shape).astype(np.float32)
+    y = rng.standard_normal(shape).astype(np.float32)
+    a = (1.0 + 0.2 * smooth(rng.standard_normal(shape))).astype(np.float32)
+    return x, y, a
+
+
+def oracle_J(P, a, x, y, w):
+    P.weights = list(w)
+    r = O.wawt(P, a.astype(np.float64) * x.astype(np.float64)) - y.astype(np.float64)
+    return 0.5 * float(np.sum(r * r))
+
+
+@pytest.mark.parametrize("filt_g1", [True, False])
+def test_gradients_match_fp64_central_differences(filt_g1):
+    cfg = M.Config.for_grid(48, 48, 8, n_ens=2, gm=3, h=4, hz=2, l_filt_g1=filt_g1)
+    w0 = [0.9, 0.6, 0.3]
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x, y, a = inputs(d.gen_shape(1), 3)
+        ws = [torch.tensor(v, dtype=torch.float64, device=DEV) for v in w0]
+        J, ga, gw = loss_and_grads(op, torch.from_numpy(a).to(DEV), torch.from_numpy(x).to(DEV),
+                                   torch.from_numpy(y).to(DEV), ws)
+        P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
+                           hz=2, p=2, gm=3, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                           l_filt_g1=filt_g1)
+        J64 = oracle_J(P, a, x, y, w0)
+        assert abs(J - J64) <= 1e-5 * abs(J64)
+        h = 1e-3
+        for g in range(3):
+            wp, wm = list(w0), list(w0)
+            wp[g] += h
+            wm[g] -= h
+            fd = (oracle_J(P, a, x, y, wp) - oracle_J(P, a, x, y, wm)) / (2 * h)
+            got = gw[g].item()
+            if g == 0 and not filt_g1:
+                assert got == 0.0 and abs(fd) < 1e-9 * abs(J64)
+            else:
+                assert abs(got - fd) <= 1e-4 * abs(fd), (g, got, fd)
+        rng = np.random.default_rng(7)
+        gan = ga.cpu().double().numpy()
+        for _ in range(4):
+            idx = tuple(int(rng.integers(s)) for s in a.shape)
+            ap, am = a.astype(np.float64).copy(), a.astype(np.float64).copy()
+            ap[idx] += h
+            am[idx] -= h
+            fd = (oracle_J(P, ap, x, y, w0) - oracle_J(P, am, x, y, w0)) / (2 * h)
+            assert abs(gan[idx] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(gan).max()), (idx, gan[idx], fd)
+
+
+def test_per_point_weight_field_gradient():
+    cfg = M.Config.for_grid(32, 32, 4, n_ens=2, gm=2, h=3, hz=1)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x, y, _ = inputs(d.gen_shape(1), 5)
+        rng = np.random.default_rng(6)
+        f = [(0.5 + rng.random(d.gen_shape(g))).astype(np.float64) for g in (1, 2)]
+        ws = [torch.from_numpy(v).to(DEV).requires_grad_(True) for v in f]
+        xd, yd = torch.from_numpy(x).to(DEV), torch.from_numpy(y).to(DEV)
+        r = localize(op, xd, ws) - yd
+        (0.5 * (r.double() ** 2).sum()).backward()
+        P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=3, hy=3,
+                           hz=1, p=2, gm=2, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02)
+
+        def J(fields):
+            rr = O.wawt(P, x.astype(np.float64), wfield=fields) - y
+            return 0.5 * float(np.sum(rr * rr))
+
+        h = 1e-3
+        for g in (0, 1):
+            for _ in range(3):
+                idx = tuple(int(rng.integers(s)) for s in f[g].shape)
+                fp = [v.copy() for v in f]
+                fm = [v.copy() for v in f]
+                fp[g][idx] += h
+                fm[g][idx] -= h
+                fd = (J(fp) - J(fm)) / (2 * h)
+                got = ws[g].grad.cpu().numpy()[idx]
+                scale = np.abs(ws[g].grad.cpu().numpy()).max()
+                assert abs(got - fd) <= 1e-4 * max(abs(fd), 1e-2 * scale), (g, idx, got, fd)
+
+
+def test_full_size_gradients_are_homogeneous():
+    # 16 members x 512x512x64, 5 generations, learnable weights [1, .5, .25, .125, .0625]
+    cfg = M.Config.for_grid(512, 512, 64, n_ens=16, gm=5, h=6, hz=3)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        gen = torch.Generator(device="cuda").manual_seed(11)
+        x = torch.randn(d.gen_shape(1), generator=gen, device=DEV)
+        y = torch.randn(d.gen_shape(1), generator=gen, device=DEV)
+        a = 1.0 + 0.2 * torch.randn(d.gen_shape(1), generator=gen, device=DEV)
+        w0 = [1.0, 0.5, 0.25, 0.125, 0.0625]
+        ws = [torch.tensor(v, dtype=torch.float64, device=DEV) for v in w0]
+        J, ga, gw = loss_and_grads(op, a, x, y, ws)
+        z = (a * x).contiguous()
+        for g, v in enumerate(w0, start=1):
+            op.set_weights(g, value=v)
+        Bz = op.filtering_procedure(z)
+        r = (Bz - y).contiguous()
+        rBz = op.dot(r, Bz)
+        lhs_w = sum(v * g.item() for v, g in zip(w0, gw))
+        lhs_a = op.dot(a.contiguous(), ga.to(torch.float32).contiguous())
+        assert abs(lhs_w - rBz) <= 1e-5 * abs(rBz)
+        assert abs(lhs_a - rBz) <= 1e-4 * abs(rBz)
+        del x, y, a, z, Bz, r, ga
