@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--gm", type=int, default=4)
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default radial tiles")
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
     args = ap.parse_args()
 
@@ -101,6 +102,11 @@ def main():
     y = torch.empty_like(x)
     op = MG.MGBF(cfg)
     assert op.dims.gm == gm
+    tiles = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tiles = op.autotune()
+        log(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiles}")
 
     # ---- batched: one application over all members ----
     for _ in range(args.warmup):
@@ -135,6 +141,8 @@ def main():
     if not args.no_plain:
         cfg1 = MG.Config.for_grid(n, n, lm, n_ens=1, gm=gm, h=6, hz=3, p=2, weights=weights)
         op1 = MG.MGBF(cfg1)
+        if not args.no_tune:
+            op1.autotune()
         xp = x.view(n, n, M, lm).permute(2, 0, 1, 3).contiguous()
         yp = torch.empty_like(xp)
         for m in range(min(M, 2)):
@@ -206,6 +214,7 @@ def main():
         "operator_hbm_frac": round(bytes_per_unit(gm) * value / 1e9 / HBM_PEAK_GBPS, 4),
         "device_copy_GBps": round(gbps_copy, 1),
         "phases": phase_out,
+        "tiles": {f"g{g}k{k}": list(v) for (g, k), v in tiles.items()} if tiles else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -160,6 +160,16 @@ int mgbf_timers_enable(mgbf_ctx* ctx, int enable);
    MGBF_NTIMERS entries; any may be NULL). */
 int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* bytes);
 
+/* Times the radial filter on this context's shapes for every candidate tile
+   (output columns per lane x output rows per wavefront), for each generation
+   and each pass kind (0 plain, 1 with the prolongation, 2 with the
+   prolongation and the vertical band), and keeps the fastest. Uses the
+   context's workspace; synchronizes `stream`. No counterpart in the
+   reference, whose loops have no tiles. */
+int mgbf_autotune(mgbf_ctx* ctx, void* stream);
+/* The tile in use for generation gen (1-based) and pass kind. */
+int mgbf_get_tile(const mgbf_ctx* ctx, int gen, int kind, int* ti, int* tj);
+
 /* Message of the last failure on this thread ("" if none). */
 const char* mgbf_last_error(void);
 

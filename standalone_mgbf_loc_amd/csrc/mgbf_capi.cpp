@@ -74,6 +74,9 @@ struct mgbf_ctx {
     int band_hb[3] = {}, band_ilo[3] = {}, band_ihi[3] = {};
     bool commute = true;     // weights constant along the stack: B = Vf Vf^T H
     int fuse = 2;            // epilogue fusion of the up-sweep: 0 none, 1 +P, 2 +P and band
+    // radial tiles per generation and pass kind (0 plain, 1 +P, 2 +P and band)
+    int tile_ti[mgbf::kMaxGen][3] = {};
+    int tile_tj[mgbf::kMaxGen][3] = {};
     const int* d_rxs[mgbf::kMaxGen] = {};
     const float* d_rxw[mgbf::kMaxGen] = {};
     const int* d_rys[mgbf::kMaxGen] = {};
@@ -159,9 +162,10 @@ int drain_timers(mgbf_ctx* c) {
     return status;
 }
 
-int pick_tj(int K, int n, int m) {
+// Default rows per wave: the tallest tile that still gives 16384 waves.
+int pick_tj(int K, int n, int m, int ti) {
     const long long nkc = (K + 63) / 64;
-    const long long ntile_i = (n + 3) / 4;
+    const long long ntile_i = (n + ti - 1) / ti;
     int tj = m;
     while (tj > 16 && nkc * ntile_i * ((m + tj - 1) / tj) < 16384) tj = (tj + 1) / 2;
     return tj;
@@ -200,7 +204,9 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
     a.n = c->nx[g];
     a.m = c->ny[g];
     a.H = c->s.H;
-    a.TJ = pick_tj(c->K, a.n, a.m);
+    const int kind = band >= 0 ? 2 : (coarse ? 1 : 0);
+    a.TI = c->tile_ti[g][kind];
+    a.TJ = c->tile_tj[g][kind];
     a.exact_disc = c->s.exact_disc;
     a.coef = c->coef;
     HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
@@ -451,6 +457,10 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->goff[g] = off;
         off += c->gsize[g];
         c->wconst[g] = (float)s.wconst[g];
+        for (int kind = 0; kind < 3; ++kind) {
+            c->tile_ti[g][kind] = 4;
+            c->tile_tj[g][kind] = pick_tj(c->K, s.nx[g], s.ny[g], 4);
+        }
     }
     for (int b = 0; b <= mgbf::kMaxHaloDev; ++b)
         for (int a = 0; a <= mgbf::kMaxHaloDev; ++a)
@@ -629,6 +639,83 @@ int mgbf_dot(mgbf_ctx* c, const float* a, const float* b, long long n, double* r
     HIP_TRY(hipMemcpyAsync(result, c->d_dot, sizeof(double), hipMemcpyDeviceToHost, st),
             "dot copy");
     HIP_TRY(hipStreamSynchronize(st), "dot sync");
+    return MGBF_OK;
+}
+
+int mgbf_autotune(mgbf_ctx* c, void* stream) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    DeviceGuard guard(c->device);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t mgb = sizeof(float) * (size_t)c->s.dims.mg_size;
+    HIP_TRY(hipMemsetAsync(c->ws_u, 0, mgb, st), "autotune memset");
+    HIP_TRY(hipMemsetAsync(c->ws_t, 0, mgb, st), "autotune memset");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "event");
+    HIP_TRY(hipEventCreate(&e1), "event");
+    const bool timing = c->timing;
+    c->timing = false;
+    int status = MGBF_OK;
+    const int vband = c->commute ? BAND_VVT : BAND_VF;
+    for (int g = 0; g < c->gm && status == MGBF_OK; ++g) {
+        const int n = c->nx[g], m = c->ny[g];
+        for (int kind = 0; kind < 3 && status == MGBF_OK; ++kind) {
+            if (kind >= 1 && g + 1 >= c->gm) continue;
+            if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
+            const float* coarse = kind >= 1 ? c->ws_u + c->goff[g + 1] : nullptr;
+            const int band = kind == 2 ? vband : -1;
+            float best = 1e30f;
+            int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind];
+            const int tis[2] = {2, 4};
+            const int tjs[11] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, m};
+            for (int ti : tis) {
+                for (int q = 0; q < 11; ++q) {
+                    const int tj = tjs[q];
+                    bool dup = false;
+                    for (int q2 = 0; q2 < q; ++q2) dup = dup || tjs[q2] == tj;
+                    if (tj > m || dup) continue;
+                    const long long waves = (long long)((c->K + 63) / 64) * ((n + ti - 1) / ti) *
+                                            ((m + tj - 1) / tj);
+                    if (waves < 2048 && tj != 16) continue;
+                    c->tile_ti[g][kind] = ti;
+                    c->tile_tj[g][kind] = tj;
+                    float ms = 0.f;
+                    int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st,
+                                   MGBF_T_HFILT, coarse, band);
+                    if (r == MGBF_OK && hipEventRecord(e0, st) == hipSuccess) {
+                        for (int rep = 0; rep < 3 && r == MGBF_OK; ++rep)
+                            r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st,
+                                       MGBF_T_HFILT, coarse, band);
+                        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                            r = fail(MGBF_ERR_HIP, "autotune timing");
+                    }
+                    if (r != MGBF_OK) {
+                        status = r;
+                        break;
+                    }
+                    if (ms < best) {
+                        best = ms;
+                        bti = ti;
+                        btj = tj;
+                    }
+                }
+                if (status != MGBF_OK) break;
+            }
+            c->tile_ti[g][kind] = bti;
+            c->tile_tj[g][kind] = btj;
+        }
+    }
+    c->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return status;
+}
+
+int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
+    if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 2)
+        return fail(MGBF_ERR_ARG, "bad argument");
+    *ti = c->tile_ti[gen - 1][kind];
+    *tj = c->tile_tj[gen - 1][kind];
     return MGBF_OK;
 }
 

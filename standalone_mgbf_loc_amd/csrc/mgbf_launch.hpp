@@ -25,6 +25,7 @@ struct RadialArgs {
     int lm, nfilt;        // column length and vertically filtered part of the stack
     int K, n, m;
     int H;                // max(hx, hy)
+    int TI;               // output columns per lane (2 or 4)
     int TJ;               // output rows per wavefront
     bool exact_disc;
     RadialCoef coef;

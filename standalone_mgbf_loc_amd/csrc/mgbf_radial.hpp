@@ -218,10 +218,9 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-// Dispatch over the built variants for one H.
-template <int H>
-hipError_t launch_radial_h(const RadialArgs& a, hipStream_t st) {
-    constexpr int TI = 4;
+// Dispatch over the built variants for one H and tile width TI.
+template <int H, int TI>
+hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     const int vhb = a.band ? a.vhb : 0;
     if (a.exact_disc) {
@@ -239,6 +238,15 @@ hipError_t launch_radial_h(const RadialArgs& a, hipStream_t st) {
     if (vhb != 0) return hipErrorInvalidValue;
     return pro ? launch_radial_variant<H, TI, false, true, 0>(a, st)
                : launch_radial_variant<H, TI, false, false, 0>(a, st);
+}
+
+template <int H>
+hipError_t launch_radial_h(const RadialArgs& a, hipStream_t st) {
+    switch (a.TI) {
+        case 2: return launch_radial_ti<H, 2>(a, st);
+        case 4: return launch_radial_ti<H, 4>(a, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace mgbf

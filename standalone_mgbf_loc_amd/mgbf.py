@@ -29,7 +29,7 @@ STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_R, STAGE_P, STAGE_A = range(7)
 EXPORTED_SYMBOLS = (
     "mgbf_config_default", "mgbf_derive", "mgbf_init", "mgbf_finalize", "mgbf_get_dims",
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
-    "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read",
+    "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
 )
 TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
@@ -178,6 +178,9 @@ def load_library(path: Optional[str] = None):
     L.mgbf_timers_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.mgbf_timers_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
+    L.mgbf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.mgbf_get_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.mgbf_last_error.restype = ctypes.c_char_p
     L.mgbf_version.restype = ctypes.c_char_p
     if path is None:
@@ -322,6 +325,21 @@ class MGBF:
         if out is None:
             out = torch.empty(oshape, dtype=torch.float32, device=x.device)
         _check(self._L.mgbf_apply_stage(self._h, stage, gen, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def autotune(self, stream=None) -> dict:
+        """Times the candidate radial tiles on this context's shapes and keeps
+        the fastest; returns {(gen, kind): (TI, TJ)}."""
+        _check(self._L.mgbf_autotune(self._h, _stream(stream)))
+        return self.tiles()
+
+    def tiles(self) -> dict:
+        out = {}
+        ti, tj = ctypes.c_int(), ctypes.c_int()
+        for g in range(1, self.dims.gm + 1):
+            for kind in range(3):
+                _check(self._L.mgbf_get_tile(self._h, g, kind, ctypes.byref(ti), ctypes.byref(tj)))
+                out[(g, kind)] = (ti.value, tj.value)
         return out
 
     def timers(self, enable: bool = True):

@@ -245,3 +245,16 @@ def test_dot_is_fp64_accumulated():
         b = torch.ones(n, device=DEV)
         r = op.dot(a, b)
         assert abs(r - n * (1.0 + 2.0 ** -20)) < 1e-6
+
+
+def test_autotuned_tiles_give_bit_identical_results():
+    # tiles change how work is split over wavefronts, not the order of any sum
+    cfg = M.Config.for_grid(128, 96, 32, n_ens=2, gm=3, h=6, hz=3)
+    with M.MGBF(cfg) as op:
+        x = to_dev(field(op.dims.gen_shape(1), 70))
+        y0 = op.filtering_procedure(x)
+        tiles = op.autotune()
+        y1 = op.filtering_procedure(x)
+        sync()
+        assert all(ti in (2, 4) and tj >= 1 for ti, tj in tiles.values())
+        assert torch.equal(y0, y1)
