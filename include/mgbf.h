@@ -167,7 +167,10 @@ int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* byt
    context's workspace; synchronizes `stream`. No counterpart in the
    reference, whose loops have no tiles. */
 int mgbf_autotune(mgbf_ctx* ctx, void* stream);
-/* The tile in use for generation gen (1-based) and pass kind. */
+/* The tile in use for generation gen (1-based) and pass kind: *ti is the
+   output columns per lane of the vector kernel (2 or 4), or 16 when the pass
+   runs on the matrix cores (16 output columns per block of 4 wavefronts);
+   *tj is the output rows per wavefront or block. */
 int mgbf_get_tile(const mgbf_ctx* ctx, int gen, int kind, int* ti, int* tj);
 
 /* Message of the last failure on this thread ("" if none). */

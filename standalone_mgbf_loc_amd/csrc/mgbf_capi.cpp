@@ -12,6 +12,7 @@
 // symmetric under the half-sample mirror when the aspect tensor is diagonal.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -77,6 +78,13 @@ struct mgbf_ctx {
     // radial tiles per generation and pass kind (0 plain, 1 +P, 2 +P and band)
     int tile_ti[mgbf::kMaxGen][3] = {};
     int tile_tj[mgbf::kMaxGen][3] = {};
+    int tile_mfma[mgbf::kMaxGen][3] = {};   // 1: matrix-core engine for this pass
+    // matrix-core engine: per-k max |u_g| from the restriction, weight fragments
+    float* d_kmax = nullptr;                 // gm x K
+    unsigned kmax_valid = 0;                 // generations whose max is current
+    const void* d_afrag = nullptr;
+    int wexp = 0;
+    bool mfma_ok = false;                    // radial half-width fits the MFMA window
     const int* d_rxs[mgbf::kMaxGen] = {};
     const float* d_rxw[mgbf::kMaxGen] = {};
     const int* d_rys[mgbf::kMaxGen] = {};
@@ -207,6 +215,12 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
     const int kind = band >= 0 ? 2 : (coarse ? 1 : 0);
     a.TI = c->tile_ti[g][kind];
     a.TJ = c->tile_tj[g][kind];
+    if (kind == 0 && c->tile_mfma[g][0] && c->mfma_ok && ((c->kmax_valid >> g) & 1u)) {
+        a.mfma = true;
+        a.kmax_in = c->d_kmax + (size_t)g * c->K;
+        a.afrag = c->d_afrag;
+        a.wexp = c->wexp;
+    }
     a.exact_disc = c->s.exact_disc;
     a.coef = c->coef;
     HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
@@ -215,8 +229,11 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
 
 int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t st) {
     Phase ph(c, MGBF_T_UPSEND, st, 4.0 * (double)(c->gsize[g] + c->gsize[g + 1]));
-    mgbf::TransferArgs a{fine, coarse, c->K, c->nx[g], c->ny[g],
+    // the restriction reads all of generation g: it records max |u_g| per k
+    float* kmax = c->d_kmax ? c->d_kmax + (size_t)g * c->K : nullptr;
+    mgbf::TransferArgs a{fine, coarse, kmax, c->K, c->nx[g], c->ny[g],
                          c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g]};
+    if (kmax) c->kmax_valid |= 1u << g;
     HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
     return MGBF_OK;
 }
@@ -225,7 +242,7 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
               hipStream_t st) {
     Phase ph(c, MGBF_T_DNSEND, st,
              4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)));
-    mgbf::TransferArgs a{coarse, fine, c->K, c->nx[g], c->ny[g],
+    mgbf::TransferArgs a{coarse, fine, nullptr, c->K, c->nx[g], c->ny[g],
                          c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g]};
     HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
     return MGBF_OK;
@@ -261,6 +278,9 @@ int vertical(mgbf_ctx* c, const float* in, float* out, int which, hipStream_t st
 
 // Restriction chain u_{g+1} = R u_g from u_1 (in U0 or a caller array).
 int restrict_chain(mgbf_ctx* c, const float* u1, float* U, hipStream_t st) {
+    c->kmax_valid = 0;
+    if (c->d_kmax && c->gm > 1)
+        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)(c->gm - 1) * c->K, st), "fill launch");
     for (int g = 0; g + 1 < c->gm; ++g)
         CALL(restrict_g(c, g, g == 0 ? u1 : U + c->goff[g], U + c->goff[g + 1], st));
     return MGBF_OK;
@@ -349,6 +369,7 @@ int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream
 // y = W v: the control vector's generations are filtered once and summed up
 // the hierarchy, then Vf.
 int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
+    c->kmax_valid = 0;
     return upsweep(c, v, v, y, c->nfilt ? BAND_VF : -1, false, sqrt_weights, st);
 }
 
@@ -448,6 +469,10 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         // epilogue of the last radial pass (all levels compute the same)
         const char* fv = std::getenv("MGBF_FUSE");
         if (fv && fv[0] >= '0' && fv[0] <= '2') c->fuse = fv[0] - '0';
+        // MGBF_MFMA=1 puts every eligible first pass on the matrix cores
+        const char* mv = std::getenv("MGBF_MFMA");
+        if (mv && mv[0] == '1')
+            for (int g = 0; g < mgbf::kMaxGen; ++g) c->tile_mfma[g][0] = 1;
     }
     long long off = 0;
     for (int g = 0; g < c->gm; ++g) {
@@ -460,6 +485,8 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (int kind = 0; kind < 3; ++kind) {
             c->tile_ti[g][kind] = 4;
             c->tile_tj[g][kind] = pick_tj(c->K, s.nx[g], s.ny[g], 4);
+            if (kind == 0 && c->tile_mfma[g][0])
+                c->tile_tj[g][0] = pick_tj(c->K, s.nx[g], s.ny[g], 16);
         }
     }
     for (int b = 0; b <= mgbf::kMaxHaloDev; ++b)
@@ -503,8 +530,32 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (int t = 0; t < nb; ++t) c->band_int[b].w[t] = (float)(*bands[b])[(size_t)mid * nb + t];
     }
 
+    // FP16 hi/lo lane fragments of the weight bands T_b (b = 0..H) for the
+    // matrix-core engine, A operand of v_mfma_f32_16x16x32_f16: lane l holds
+    // row l%16 (output column) and window columns (l/16)*8 .. +7
+    std::vector<_Float16> afrag;
+    if (s.H <= 8) {
+        double maxw = 0;
+        for (int b = 0; b <= s.H; ++b)
+            for (int a = 0; a <= s.H; ++a) maxw = std::max(maxw, s.wsym[b][a]);
+        c->wexp = maxw > 0 ? 10 - std::ilogb(maxw) : 0;
+        afrag.assign((size_t)(s.H + 1) * 2 * 64 * 8, (_Float16)0.f);
+        for (int b = 0; b <= s.H; ++b)
+            for (int l = 0; l < 64; ++l)
+                for (int q = 0; q < 8; ++q) {
+                    const int a = (l / 16) * 8 + q - s.H - (l % 16);
+                    const int aa = a < 0 ? -a : a;
+                    const float wv = aa <= s.H ? (float)std::ldexp(s.wsym[b][aa], c->wexp) : 0.f;
+                    const _Float16 hi = (_Float16)wv;
+                    const _Float16 lo = (_Float16)(wv - (float)hi);
+                    afrag[((size_t)(b * 2 + 0) * 64 + l) * 8 + q] = hi;
+                    afrag[((size_t)(b * 2 + 1) * 64 + l) * 8 + q] = lo;
+                }
+        c->mfma_ok = true;
+    }
+
     // one device allocation for all coefficient tables
-    size_t bytes = carve_size(sizeof(double));
+    size_t bytes = carve_size(sizeof(double)) + carve_size(sizeof(_Float16) * afrag.size());
     for (int b = 0; b < 3; ++b) bytes += carve_size(sizeof(float) * bands[b]->size());
     for (int g = 0; g + 1 < c->gm; ++g) {
         const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
@@ -532,6 +583,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         return d;
     };
     for (int b = 0; b < 3; ++b) c->d_band[b] = put_f(*bands[b]);
+    {
+        _Float16* h = carve<_Float16>(hp, afrag.size());
+        c->d_afrag = carve<_Float16>(dp, afrag.size());
+        for (size_t i = 0; i < afrag.size(); ++i) h[i] = afrag[i];
+    }
     c->d_dot = carve<double>(dp, 1);
     carve<double>(hp, 1);
     for (int g = 0; g + 1 < c->gm; ++g) {
@@ -547,6 +603,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->ws_t, sizeof(float) * (size_t)s.dims.mg_size);
+    if (e == hipSuccess) e = hipMalloc(&c->d_kmax, sizeof(float) * (size_t)c->gm * c->K);
     if (e != hipSuccess) {
         int r = hip_fail(e, "device allocation");
         mgbf_finalize(c);
@@ -565,6 +622,7 @@ int mgbf_finalize(mgbf_ctx* c) {
     if (c->ws_u) (void)hipFree(c->ws_u);
     if (c->ws_t) (void)hipFree(c->ws_t);
     if (c->d_tables) (void)hipFree(c->d_tables);
+    if (c->d_kmax) (void)hipFree(c->d_kmax);
     delete c;
     return MGBF_OK;
 }
@@ -656,6 +714,22 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     c->timing = false;
     int status = MGBF_OK;
     const int vband = c->commute ? BAND_VVT : BAND_VF;
+    // the restriction chain records the per-k maxima the matrix-core engine needs
+    if (status == MGBF_OK) status = restrict_chain(c, c->ws_u, c->ws_u, st);
+    auto time_pass = [&](int g, const float* coarse, int band, float* ms) -> int {
+        int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT,
+                       coarse, band);
+        if (r != MGBF_OK) return r;
+        if (hipEventRecord(e0, st) != hipSuccess) return fail(MGBF_ERR_HIP, "autotune timing");
+        for (int rep = 0; rep < 3 && r == MGBF_OK; ++rep)
+            r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT, coarse,
+                       band);
+        if (r != MGBF_OK) return r;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            return fail(MGBF_ERR_HIP, "autotune timing");
+        return MGBF_OK;
+    };
     for (int g = 0; g < c->gm && status == MGBF_OK; ++g) {
         const int n = c->nx[g], m = c->ny[g];
         for (int kind = 0; kind < 3 && status == MGBF_OK; ++kind) {
@@ -664,45 +738,37 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             const float* coarse = kind >= 1 ? c->ws_u + c->goff[g + 1] : nullptr;
             const int band = kind == 2 ? vband : -1;
             float best = 1e30f;
-            int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind];
-            const int tis[2] = {2, 4};
+            int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind], bmf = c->tile_mfma[g][kind];
+            const bool try_mfma = kind == 0 && c->mfma_ok && ((c->kmax_valid >> g) & 1u);
             const int tjs[11] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, m};
-            for (int ti : tis) {
-                for (int q = 0; q < 11; ++q) {
+            // engines: TI = 2, TI = 4 (vector), 16 (matrix cores)
+            const int engines[3] = {2, 4, 16};
+            for (int en : engines) {
+                if (en == 16 && !try_mfma) continue;
+                for (int q = 0; q < 11 && status == MGBF_OK; ++q) {
                     const int tj = tjs[q];
                     bool dup = false;
                     for (int q2 = 0; q2 < q; ++q2) dup = dup || tjs[q2] == tj;
                     if (tj > m || dup) continue;
-                    const long long waves = (long long)((c->K + 63) / 64) * ((n + ti - 1) / ti) *
-                                            ((m + tj - 1) / tj);
+                    const long long waves = (long long)((c->K + 63) / 64) * ((n + en - 1) / en) *
+                                            ((m + tj - 1) / tj) * (en == 16 ? 4 : 1);
                     if (waves < 2048 && tj != 16) continue;
-                    c->tile_ti[g][kind] = ti;
+                    c->tile_ti[g][kind] = en == 16 ? 4 : en;
                     c->tile_tj[g][kind] = tj;
+                    c->tile_mfma[g][kind] = en == 16 ? 1 : 0;
                     float ms = 0.f;
-                    int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st,
-                                   MGBF_T_HFILT, coarse, band);
-                    if (r == MGBF_OK && hipEventRecord(e0, st) == hipSuccess) {
-                        for (int rep = 0; rep < 3 && r == MGBF_OK; ++rep)
-                            r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st,
-                                       MGBF_T_HFILT, coarse, band);
-                        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-                            r = fail(MGBF_ERR_HIP, "autotune timing");
-                    }
-                    if (r != MGBF_OK) {
-                        status = r;
-                        break;
-                    }
-                    if (ms < best) {
+                    status = time_pass(g, coarse, band, &ms);
+                    if (status == MGBF_OK && ms < best) {
                         best = ms;
-                        bti = ti;
+                        bti = c->tile_ti[g][kind];
                         btj = tj;
+                        bmf = c->tile_mfma[g][kind];
                     }
                 }
-                if (status != MGBF_OK) break;
             }
             c->tile_ti[g][kind] = bti;
             c->tile_tj[g][kind] = btj;
+            c->tile_mfma[g][kind] = bmf;
         }
     }
     c->timing = timing;
@@ -714,7 +780,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
 int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
     if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 2)
         return fail(MGBF_ERR_ARG, "bad argument");
-    *ti = c->tile_ti[gen - 1][kind];
+    *ti = c->tile_mfma[gen - 1][kind] ? 16 : c->tile_ti[gen - 1][kind];
     *tj = c->tile_tj[gen - 1][kind];
     return MGBF_OK;
 }

@@ -196,7 +196,7 @@ __device__ void restrict_edge_tile(const float* __restrict__ fine, float* __rest
                                    int kk, bool kvalid, int K, int nf, int mf, int I0, int J0,
                                    int J1, const int* __restrict__ rxs,
                                    const float* __restrict__ rxw, const int* __restrict__ rys,
-                                   const float* __restrict__ ryw) {
+                                   const float* __restrict__ ryw, float& lmax) {
     constexpr int WR = kRWidthDev;
     const int nc = nf / 2;
     const size_t frow = (size_t)nf * K;
@@ -215,7 +215,9 @@ __device__ void restrict_edge_tile(const float* __restrict__ fine, float* __rest
                 for (int tx = 0; tx < WR; ++tx) {
                     const int ii = xs + tx < nf ? xs + tx : nf - 1;
                     const float wx = xs + tx < nf ? rxw[I * WR + tx] : 0.f;
-                    a = __builtin_fmaf(wx, row[(size_t)ii * K], a);
+                    const float fv = row[(size_t)ii * K];
+                    lmax = fmaxf(lmax, fabsf(fv));
+                    a = __builtin_fmaf(wx, fv, a);
                 }
                 acc = __builtin_fmaf(wy, a, acc);
             }
@@ -228,8 +230,8 @@ template <int TI2, int TJ2>
 __global__ void __launch_bounds__(kBlock)
 restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
                  int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
-                 const int* __restrict__ rys, const float* __restrict__ ryw, int nkc,
-                 int ntile_i, int nstrip, int nitems) {
+                 const int* __restrict__ rys, const float* __restrict__ ryw, float* kmax,
+                 int nkc, int ntile_i, int nstrip, int nitems) {
     constexpr int WR = kRWidthDev;
     constexpr int NCF = 2 * TI2 + 6;
     const int lane = threadIdx.x & (kWave - 1);
@@ -245,8 +247,11 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
     const int nc = nf / 2, mc = mf / 2;
     const int I0 = ti * TI2, J0 = js * TJ2;
     const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
+    float lmax = 0.f;   // max |fine| read by this lane (its k)
     if (!(I0 >= 2 && I0 + TI2 <= nc - 2)) {
-        restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw);
+        restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw,
+                                lmax);
+        if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));
         return;
     }
     const size_t frow = (size_t)nf * K;
@@ -256,7 +261,10 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
         const float* row = base + (size_t)fr * frow;
         float v[NCF];
 #pragma unroll
-        for (int c = 0; c < NCF; ++c) v[c] = row[(size_t)c * K];
+        for (int c = 0; c < NCF; ++c) {
+            v[c] = row[(size_t)c * K];
+            lmax = fmaxf(lmax, fabsf(v[c]));
+        }
 #pragma unroll
         for (int c = 0; c < TI2; ++c) {
             float a = 0.f;
@@ -311,6 +319,7 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
                 coarse[kk + (size_t)K * ((size_t)(I0 + c) + (size_t)nc * J)] = acc[c];
         }
     }
+    if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));
 }
 
 // out = in * (wfield ? f(wfield) : f(wconst)), f = identity or sqrt
@@ -352,6 +361,12 @@ dot_kernel(const float* __restrict__ a, const float* __restrict__ b, long long c
 
 __global__ void zero_double_kernel(double* p) { *p = 0.0; }
 
+__global__ void __launch_bounds__(kBlock) fill_kernel(float* p, float v, long long count) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += stride)
+        p[e] = v;
+}
+
 unsigned grid_for(long long count) {
     long long g = (count + kBlock - 1) / kBlock;
     if (g > 65536) g = 65536;
@@ -386,8 +401,8 @@ hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
     const long long nitems = (long long)ntile_i * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
     hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
-                       a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nkc, ntile_i, nstrip,
-                       (int)nitems);
+                       a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc, ntile_i,
+                       nstrip, (int)nitems);
     return hipGetLastError();
 }
 
@@ -447,6 +462,11 @@ hipError_t launch_scale(const float* in, float* out, const float* wfield, float 
 
 hipError_t launch_copy(const float* in, float* out, long long count, hipStream_t st) {
     hipLaunchKernelGGL(copy_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, in, out, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, float v, long long count, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 

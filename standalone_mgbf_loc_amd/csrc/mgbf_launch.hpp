@@ -26,6 +26,12 @@ struct RadialArgs {
     int K, n, m;
     int H;                // max(hx, hy)
     int TI;               // output columns per lane (2 or 4)
+    // matrix-core engine (first pass of a generation)
+    bool mfma;
+    const float* kmax_in;   // per-k max |in| over the field (device, K floats)
+    float* kmax_out;        // per-k max |out| accumulated here, or nullptr
+    const void* afrag;      // FP16 hi/lo lane fragments of the weight bands
+    int wexp;               // power-of-two exponent applied to the weights
     int TJ;               // output rows per wavefront
     bool exact_disc;
     RadialCoef coef;
@@ -44,6 +50,20 @@ struct RadialParams {
     RadialCoef W;
 };
 
+// kernel argument record of the matrix-core radial kernel
+struct RadialMfmaParams {
+    const float* in;
+    float* out;
+    const float* wfield;
+    const float* kmax_in;
+    float* kmax_out;
+    const void* afrag;
+    float wconst;
+    int wexp;
+    int K, n, m, TJ;
+    int ntile_i, ntile_j;
+};
+
 #define MGBF_DECLARE_RADIAL(h) hipError_t launch_radial_H##h(const RadialArgs& a, hipStream_t st);
 MGBF_DECLARE_RADIAL(1) MGBF_DECLARE_RADIAL(2) MGBF_DECLARE_RADIAL(3) MGBF_DECLARE_RADIAL(4)
 MGBF_DECLARE_RADIAL(5) MGBF_DECLARE_RADIAL(6) MGBF_DECLARE_RADIAL(7) MGBF_DECLARE_RADIAL(8)
@@ -53,6 +73,7 @@ MGBF_DECLARE_RADIAL(9) MGBF_DECLARE_RADIAL(10)
 struct TransferArgs {
     const float* src;
     float* dst;
+    float* kmax;          // per-k max |src| accumulated here, or nullptr
     int K, nf, mf;        // fine generation size
     const int* xs;        // window starts along x
     const float* xw;      // window weights along x
@@ -84,6 +105,7 @@ hipError_t launch_prolong(const TransferArgs& a, const float* addend, hipStream_
 hipError_t launch_scale(const float* in, float* out, const float* wfield, float wconst,
                         int use_sqrt, long long count, hipStream_t st);
 hipError_t launch_copy(const float* in, float* out, long long count, hipStream_t st);
+hipError_t launch_fill(float* p, float v, long long count, hipStream_t st);
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st);
 

@@ -256,5 +256,27 @@ def test_autotuned_tiles_give_bit_identical_results():
         tiles = op.autotune()
         y1 = op.filtering_procedure(x)
         sync()
-        assert all(ti in (2, 4) and tj >= 1 for ti, tj in tiles.values())
+        assert all(ti in (2, 4, 16) and tj >= 1 for ti, tj in tiles.values())
         assert torch.equal(y0, y1)
+
+
+@pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 40, 80, 48, 3, 1e-6),
+                                              (8, 130, 64, 64, 2, 1e4), (3, 24, 48, 40, 2, 1.0),
+                                              (6, 128, 192, 96, 4, 1.0)])
+def test_matrix_core_engine_matches_oracle(monkeypatch, h, K, n, m, gm, scale):
+    # the first radial pass of every restricted generation on v_mfma_f32_16x16x32_f16
+    cfg = M.Config.for_grid(n, m, K, gm=gm, h=h, hz=min(3, K - 1))
+    x = (field((m, n, K), 90 + h) * scale).astype(np.float32)
+    with M.MGBF(cfg) as op:
+        y_vec = op.filtering_procedure(to_dev(x))
+        sync()
+    monkeypatch.setenv("MGBF_MFMA", "1")
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        y = op.filtering_procedure(to_dev(x))
+        v = op.apply(M.OP_WT, to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert not torch.equal(y, y_vec)   # the matrix-core path ran
