@@ -247,17 +247,23 @@ def test_dot_is_fp64_accumulated():
         assert abs(r - n * (1.0 + 2.0 ** -20)) < 1e-6
 
 
-def test_autotuned_tiles_give_bit_identical_results():
-    # tiles change how work is split over wavefronts, not the order of any sum
+def test_autotuned_tiles_keep_parity():
+    # vector-engine tiles change how work is split over wavefronts, not the
+    # order of any sum (bit-identical); the matrix-core engine rounds
+    # differently and is held to the oracle bar
     cfg = M.Config.for_grid(128, 96, 32, n_ens=2, gm=3, h=6, hz=3)
     with M.MGBF(cfg) as op:
-        x = to_dev(field(op.dims.gen_shape(1), 70))
+        d = op.dims
+        xh = field(d.gen_shape(1), 70)
+        x = to_dev(xh)
         y0 = op.filtering_procedure(x)
         tiles = op.autotune()
         y1 = op.filtering_procedure(x)
         sync()
         assert all(ti in (2, 4, 16) and tj >= 1 for ti, tj in tiles.values())
-        assert torch.equal(y0, y1)
+        if all(ti != 16 for ti, _ in tiles.values()):
+            assert torch.equal(y0, y1)
+        assert relerr(y1, O.wawt(oparams(cfg, d), xh.astype(np.float64))) <= TOL_OP
 
 
 @pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 40, 80, 48, 3, 1e-6),
