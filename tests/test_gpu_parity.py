@@ -248,9 +248,9 @@ def test_dot_is_fp64_accumulated():
 
 
 def test_autotuned_tiles_keep_parity():
-    # vector-engine tiles change how work is split over wavefronts, not the
-    # order of any sum (bit-identical); the matrix-core engine rounds
-    # differently and is held to the oracle bar
+    # tiles change how work is split over wavefronts; each instantiation may
+    # contract multiply-adds differently, so results agree to rounding, and
+    # every engine is held to the oracle bar
     cfg = M.Config.for_grid(128, 96, 32, n_ens=2, gm=3, h=6, hz=3)
     with M.MGBF(cfg) as op:
         d = op.dims
@@ -261,8 +261,8 @@ def test_autotuned_tiles_keep_parity():
         y1 = op.filtering_procedure(x)
         sync()
         assert all(ti in (2, 4, 16) and tj >= 1 for ti, tj in tiles.values())
-        if all(ti != 16 for ti, _ in tiles.values()):
-            assert torch.equal(y0, y1)
+        assert (y1 - y0).abs().max().item() <= 1e-6 * y0.abs().max().item() or \
+            any(ti == 16 for ti, _ in tiles.values())
         assert relerr(y1, O.wawt(oparams(cfg, d), xh.astype(np.float64))) <= TOL_OP
 
 
