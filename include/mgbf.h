@@ -139,6 +139,20 @@ int mgbf_apply_stage(mgbf_ctx* ctx, int stage, int gen, const float* in, float* 
 int mgbf_dot(mgbf_ctx* ctx, const float* a, const float* b, long long n, double* result,
              void* stream);
 
+/* A group of contexts (regional domains of uneven size, each with its own
+   grid, generations and filter) applied as one hipGraph. The reference has
+   no counterpart: it runs one domain per launch of the program. */
+typedef struct mgbf_group mgbf_group;
+/* Creates a group over n distinct contexts of one device (the contexts must
+   outlive the group and not be used elsewhere while it runs). */
+int mgbf_group_create(mgbf_ctx* const* ctxs, int n, mgbf_group** grp);
+/* Applies `op` (MGBF_OP_*) to every context i, in[i] -> out[i], as one graph
+   launch on `stream`. The graph is captured on the first call with a given
+   op and set of buffers and replayed while they stay the same. */
+int mgbf_group_apply(mgbf_group* grp, int op, const float* const* in, float* const* out,
+                     void* stream);
+int mgbf_group_destroy(mgbf_group* grp);
+
 /* Per-phase device timers, the phases of the reference's mg_timers
    (src.mgbf/mg_timers.f90:17-60, filtering_rad2_ens btim/etim brackets):
    hipEvents recorded on the caller's stream around every launch of a phase,

@@ -785,6 +785,134 @@ int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
     return MGBF_OK;
 }
 
+}  // extern "C"
+
+// A group applies one operation over several contexts (regional domains of
+// uneven size) as one hipGraph: captured once per set of buffers, with one
+// branch stream per domain so independent domains overlap, then replayed.
+struct mgbf_group {
+    std::vector<mgbf_ctx*> ctx;
+    int device = 0;
+    std::vector<hipStream_t> branch;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int key_op = -1;
+    std::vector<const float*> key_in;
+    std::vector<float*> key_out;
+};
+
+namespace {
+
+void group_drop_graph(mgbf_group* g) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->exec = nullptr;
+    g->graph = nullptr;
+    g->key_op = -1;
+}
+
+int group_capture(mgbf_group* g, int op, const float* const* in, float* const* out) {
+    group_drop_graph(g);
+    const int n = (int)g->ctx.size();
+    std::vector<hipEvent_t> ev(n + 1, nullptr);
+    for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "group event");
+    int status = MGBF_OK;
+    hipError_t e = hipStreamBeginCapture(g->cap, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) status = hip_fail(e, "graph capture begin");
+    if (status == MGBF_OK) {
+        (void)hipEventRecord(ev[n], g->cap);   // fork
+        for (int i = 0; i < n && status == MGBF_OK; ++i) {
+            hipStream_t bs = g->branch[i];
+            (void)hipStreamWaitEvent(bs, ev[n], 0);
+            mgbf_ctx* c = g->ctx[i];
+            const bool timing = c->timing;
+            c->timing = false;
+            status = mgbf_apply(c, op, in[i], out[i], bs);
+            c->timing = timing;
+            (void)hipEventRecord(ev[i], bs);
+            (void)hipStreamWaitEvent(g->cap, ev[i], 0);   // join
+        }
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(g->cap, &graph);
+        if (status == MGBF_OK && e != hipSuccess) status = hip_fail(e, "graph capture end");
+        if (status == MGBF_OK) {
+            g->graph = graph;
+            e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) status = hip_fail(e, "graph instantiate");
+        } else if (graph) {
+            (void)hipGraphDestroy(graph);
+        }
+    }
+    for (auto& ee : ev) (void)hipEventDestroy(ee);
+    if (status != MGBF_OK) {
+        group_drop_graph(g);
+        return status;
+    }
+    g->key_op = op;
+    g->key_in.assign(in, in + n);
+    g->key_out.assign(out, out + n);
+    return MGBF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgbf_group_create(mgbf_ctx* const* ctxs, int n, mgbf_group** out) {
+    if (!ctxs || !out || n < 1) return fail(MGBF_ERR_ARG, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        if (!ctxs[i]) return fail(MGBF_ERR_ARG, "NULL context in group");
+        for (int j = 0; j < i; ++j)
+            if (ctxs[j] == ctxs[i]) return fail(MGBF_ERR_ARG, "a context appears twice in the group");
+        if (ctxs[i]->device != ctxs[0]->device) return fail(MGBF_ERR_ARG, "group spans devices");
+    }
+    auto* g = new mgbf_group();
+    g->ctx.assign(ctxs, ctxs + n);
+    g->device = ctxs[0]->device;
+    DeviceGuard guard(g->device);
+    hipError_t e = hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking);
+    g->branch.assign(n, nullptr);
+    for (int i = 0; i < n && e == hipSuccess; ++i)
+        e = hipStreamCreateWithFlags(&g->branch[i], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        mgbf_group_destroy(g);
+        return hip_fail(e, "group streams");
+    }
+    *out = g;
+    return MGBF_OK;
+}
+
+int mgbf_group_apply(mgbf_group* g, int op, const float* const* in, float* const* out,
+                     void* stream) {
+    if (!g || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
+    const int n = (int)g->ctx.size();
+    for (int i = 0; i < n; ++i)
+        if (!in[i] || !out[i]) return fail(MGBF_ERR_ARG, "NULL buffer in group apply");
+    DeviceGuard guard(g->device);
+    const bool same = g->exec && g->key_op == op &&
+                      std::equal(g->key_in.begin(), g->key_in.end(), in) &&
+                      std::equal(g->key_out.begin(), g->key_out.end(), out);
+    if (!same) {
+        int st = group_capture(g, op, in, out);
+        if (st != MGBF_OK) return st;
+    }
+    HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "graph launch");
+    return MGBF_OK;
+}
+
+int mgbf_group_destroy(mgbf_group* g) {
+    if (!g) return MGBF_OK;
+    DeviceGuard guard(g->device);
+    group_drop_graph(g);
+    for (hipStream_t s : g->branch)
+        if (s) (void)hipStreamDestroy(s);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
+    delete g;
+    return MGBF_OK;
+}
+
 int mgbf_timers_enable(mgbf_ctx* c, int enable) {
     if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
     DeviceGuard guard(c->device);

@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "mgbf_config_default", "mgbf_derive", "mgbf_init", "mgbf_finalize", "mgbf_get_dims",
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
     "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
+    "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
 )
 TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
@@ -181,6 +182,11 @@ def load_library(path: Optional[str] = None):
     L.mgbf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.mgbf_get_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.mgbf_group_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_void_p)]
+    L.mgbf_group_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]
+    L.mgbf_group_destroy.argtypes = [ctypes.c_void_p]
     L.mgbf_last_error.restype = ctypes.c_char_p
     L.mgbf_version.restype = ctypes.c_char_p
     if path is None:
@@ -363,6 +369,45 @@ class MGBF:
         _check(self._L.mgbf_dot(self._h, _ptr(a), _ptr(b), a.numel(), ctypes.byref(r),
                                 _stream(stream)))
         return r.value
+
+
+class Group:
+    """Several contexts (regional domains of uneven size) applied as one
+    hipGraph, captured on first use with a given set of buffers."""
+
+    def __init__(self, ops):
+        self.ops = list(ops)
+        self._L = load_library()
+        arr = (ctypes.c_void_p * len(self.ops))(*[o._h.value for o in self.ops])
+        h = ctypes.c_void_p()
+        _check(self._L.mgbf_group_create(arr, len(self.ops), ctypes.byref(h)))
+        self._h = h
+
+    def apply(self, op: int, xs, outs, stream=None):
+        n = len(self.ops)
+        if len(xs) != n or len(outs) != n:
+            raise ValueError("one input and one output per domain")
+        ins = (ctypes.c_void_p * n)(*[_ptr(x) for x in xs])
+        ous = (ctypes.c_void_p * n)(*[_ptr(y) for y in outs])
+        _check(self._L.mgbf_group_apply(self._h, op, ins, ous, _stream(stream)))
+        return outs
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mgbf_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def version() -> str:

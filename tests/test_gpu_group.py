@@ -1,0 +1,76 @@
+"""Uneven batch: regional domains of different sizes, generations and filter
+widths applied as one hipGraph (mgbf_group_*), against the same contexts
+called one by one and against the FP64 oracle. The graph is replayed
+several times with eager work in between and every replay is checked."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O
+from standalone_mgbf_loc_amd import mgbf as M
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+
+# (nx, ny, levels, generations, radial half-width) of each domain
+DOMAINS = [(96, 96, 16, 4, 6), (128, 64, 32, 3, 8), (48, 48, 8, 2, 4), (160, 96, 24, 4, 10)]
+
+
+def _cfg(nx, ny, lm, gm, h):
+    return M.Config.for_grid(nx, ny, lm, gm=gm, h=h, hz=3, weights=[1.0, 0.6, 0.3, 0.15])
+
+
+def _oracle(cfg, d, x):
+    P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx,
+                       hy=cfg.hy, hz=cfg.hz, p=2, gm=d.gm, ampl01=cfg.mg_ampl01,
+                       ampl02=cfg.mg_ampl02, weights=[1.0, 0.6, 0.3, 0.15][: d.gm])
+    return O.wawt(P, x.astype(np.float64))
+
+
+def test_group_graph_matches_eager_and_oracle():
+    cfgs = [_cfg(*dm) for dm in DOMAINS]
+    ops = [M.MGBF(c) for c in cfgs]
+    try:
+        xs_h = [np.random.default_rng(300 + i).standard_normal(op.field_shape()).astype(np.float32)
+                for i, op in enumerate(ops)]
+        xs = [torch.from_numpy(x).to(DEV) for x in xs_h]
+        eager = [op.filtering_procedure(x) for op, x in zip(ops, xs)]
+        torch.cuda.synchronize()
+        for cfg, op, x, y in zip(cfgs, ops, xs_h, eager):
+            ref = _oracle(cfg, op.dims, x)
+            assert np.max(np.abs(y.cpu().double().numpy() - ref)) / np.max(np.abs(ref)) <= 1e-5
+        outs = [torch.empty_like(x) for x in xs]
+        with M.Group(ops) as grp:
+            for rep in range(3):
+                for o in outs:
+                    o.fill_(float("nan"))
+                grp.apply(M.OP_WAWT, xs, outs)
+                torch.cuda.synchronize()
+                for y, o in zip(eager, outs):
+                    assert torch.equal(y, o), rep
+                # eager work on the same contexts between replays
+                ops[rep % len(ops)].filtering_procedure(xs[rep % len(ops)])
+                torch.cuda.synchronize()
+            # new buffers force a new capture
+            outs2 = [torch.empty_like(x) for x in xs]
+            grp.apply(M.OP_WAWT, xs, outs2)
+            torch.cuda.synchronize()
+            for y, o in zip(eager, outs2):
+                assert torch.equal(y, o)
+    finally:
+        for op in ops:
+            op.close()
+
+
+def test_group_rejects_a_context_twice():
+    op = M.MGBF(_cfg(*DOMAINS[2]))
+    try:
+        with pytest.raises(M.MGBFError):
+            M.Group([op, op])
+    finally:
+        op.close()
