@@ -286,3 +286,23 @@ def test_matrix_core_engine_matches_oracle(monkeypatch, h, K, n, m, gm, scale):
         assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
         assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
         assert not torch.equal(y, y_vec)   # the matrix-core path ran
+
+
+def box3(a):
+    """3x3 box average with the half-sample mirror at the edges, (m, n, K)."""
+    p = np.pad(a, ((1, 1), (1, 1), (0, 0)), mode="symmetric")
+    return sum(p[1 + dj: p.shape[0] - 1 + dj, 1 + di: p.shape[1] - 1 + di]
+               for dj in (-1, 0, 1) for di in (-1, 0, 1)) / 9.0
+
+
+def test_single_field_768x768x64_five_generations_full_size():
+    # one field 768x768x64, 5 generations, radial half-width 8, vertical 4,
+    # p=2, weights 2^-g (g = 0..4); seeded N(0,1) smoothed by a 3x3 box
+    cfg = M.Config.for_grid(768, 768, 64, gm=5, h=8, hz=4, weights=[2.0 ** -g for g in range(5)])
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.gm == 5 and d.ntaps2 == 249 and d.ntaps1 == 9
+        x = box3(np.random.default_rng(3).standard_normal(d.gen_shape(1))).astype(np.float32)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
