@@ -1,0 +1,60 @@
+"""Golden fixtures (tests/golden/, written by make_golden.py from the FP64
+oracle): the oracle must keep reproducing them (CPU), and the HIP path must
+match them within the FP32 bars (GPU)."""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+
+
+def _load(name):
+    c = json.load(open(os.path.join(HERE, name + ".json")))
+    d = np.load(os.path.join(HERE, name + ".npz"))
+    return c, {k: d[k] for k in d.files}
+
+
+def test_fixtures_present():
+    assert len(CASES) == 3
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_reproduces_golden(name):
+    c, g = _load(name)
+    P = O.OracleParams(**{k: v for k, v in c.items() if k != "n_ens"})
+    x = g["x"]
+    for key, val in (("wawt", O.wawt(P, x)), ("wt", O.wt(P, x)), ("radial", O.radial(P, x)),
+                     ("vertical", O.vertical(P, x)), ("vertical_t", O.vertical(P, x, adjoint=True)),
+                     ("restrict", O.restrict(x)), ("prolong", O.prolong(g["x_coarse"]))):
+        assert np.max(np.abs(val - g[key])) <= 1e-12 * np.max(np.abs(g[key])), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", CASES)
+def test_hip_path_matches_golden(name):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from standalone_mgbf_loc_amd import mgbf as M
+    c, g = _load(name)
+    n_ens, lm = c["n_ens"], c["lm"]
+    cfg = M.Config.for_grid(c["nx"], c["ny"], lm, n_ens=n_ens, gm=c["gm"], h=c["hx"], hz=c["hz"],
+                            weights=c["weights"], l_filt_g1=c["l_filt_g1"])
+    with M.MGBF(cfg) as op:
+        x = torch.from_numpy(g["x"].astype(np.float32)).to("cuda:0")
+        xc = torch.from_numpy(g["x_coarse"].astype(np.float32)).to("cuda:0")
+        got = {"wawt": op.filtering_procedure(x), "wt": op.apply(M.OP_WT, x),
+               "radial": op.stage(M.STAGE_F, 1, x), "vertical": op.stage(M.STAGE_VF, 1, x),
+               "vertical_t": op.stage(M.STAGE_VFT, 1, x), "restrict": op.stage(M.STAGE_R, 1, x),
+               "prolong": op.stage(M.STAGE_P, 1, xc)}
+        torch.cuda.synchronize()
+        for key, y in got.items():
+            ref = g[key]
+            err = np.max(np.abs(y.cpu().double().numpy().reshape(ref.shape) - ref)) / np.max(np.abs(ref))
+            assert err <= 1e-5, (key, err)
