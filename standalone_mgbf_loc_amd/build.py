@@ -38,6 +38,9 @@ def flags() -> list:
         f"--offload-arch={ARCH}",
         # packed FP32 (v_pk_*_f32) code generation is disabled on purpose
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # matrix-core results stay in VGPRs: the ring adds then read them
+        # directly instead of copying out of AGPRs first
+        "-mllvm", "-amdgpu-mfma-vgpr-form",
         "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
         f"-I{INCLUDE}", f"-I{CSRC}",
     ]

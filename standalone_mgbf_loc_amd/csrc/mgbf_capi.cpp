@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -714,6 +715,9 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     c->timing = false;
     int status = MGBF_OK;
     const int vband = c->commute ? BAND_VVT : BAND_VF;
+    // MGBF_AUTOTUNE_LOG=1 prints every candidate's time (3 passes) to stderr
+    const char* logv = std::getenv("MGBF_AUTOTUNE_LOG");
+    const bool log = logv && logv[0] == '1';
     // the restriction chain records the per-k maxima the matrix-core engine needs
     if (status == MGBF_OK) status = restrict_chain(c, c->ws_u, c->ws_u, st);
     auto time_pass = [&](int g, const float* coarse, int band, float* ms) -> int {
@@ -758,6 +762,9 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
                     c->tile_mfma[g][kind] = en == 16 ? 1 : 0;
                     float ms = 0.f;
                     status = time_pass(g, coarse, band, &ms);
+                    if (log && status == MGBF_OK)
+                        std::fprintf(stderr, "[autotune] gen %d kind %d engine %d TJ %d: %.4f ms\n",
+                                     g + 1, kind, en, tj, ms / 3);
                     if (status == MGBF_OK && ms < best) {
                         best = ms;
                         bti = c->tile_ti[g][kind];

@@ -261,6 +261,11 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
             *err = fmt("generation %d grid %dx%d is smaller than the halo", g + 1, n, m);
             return MGBF_ERR_CONFIG;
         }
+        if ((long long)d.km_all * (n + 16) * 4 >= (1ll << 31)) {
+            // the kernels address within one grid row with 32-bit byte offsets
+            *err = fmt("generation %d row of %d x %d values exceeds 2 GiB", g + 1, n, d.km_all);
+            return MGBF_ERR_CONFIG;
+        }
         nx[g] = n;
         ny[g] = m;
         const long long sz = (long long)d.km_all * n * m;
@@ -309,13 +314,18 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         seen[b][a] = true;
         s.wsym[b][a] = t.w;
     }
-    // exact lattice disc a^2 + b^2 < (H+1)^2 ?
+    // exact lattice disc a^2 + b^2 < (H+1)^2 with weights even under a <-> b
+    // (an isotropic aspect tensor)? The disc kernels then skip the lattice
+    // points outside the disc and read w(a,b) and w(b,a) from one register.
     {
         const int R2 = (s.H + 1) * (s.H + 1);
         bool exact = cfg.hx == cfg.hy;
         for (int b = 0; b <= s.H && exact; ++b)
-            for (int a = 0; a <= s.H; ++a)
+            for (int a = 0; a <= s.H; ++a) {
                 if (seen[b][a] != (a * a + b * b < R2)) exact = false;
+                if (std::fabs(s.wsym[b][a] - s.wsym[a][b]) > 1e-15 * std::fabs(s.wsym[b][a]))
+                    exact = false;
+            }
         s.exact_disc = exact;
     }
     // vertical band matrices (sup_vrbeta1_ens / sup_vrbeta1T_ens)

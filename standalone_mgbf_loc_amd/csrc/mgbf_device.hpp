@@ -25,6 +25,23 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
     return (bid % kXcd) * (nblocks / kXcd) + bid / kXcd;
 }
 
+// Buffer resource over 4 GiB from `base`: raw dword accesses address
+// base + voffset (per lane) + soffset (wave-uniform), both in bytes, so a row
+// access needs no per-lane 64-bit address arithmetic. Callers keep every
+// offset below 2^31 (one generation row is limited to that size).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)0xffffffff,
+                                             0x00020000);
+}
+
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+}
+
 // Interpolation constants of the offset grids (def_offset_coef): fine index
 // 2u (even, 0-based) reads coarse u-2..u+1 with kQc, fine 2u+1 reads coarse
 // u-1..u+2 with kPc.

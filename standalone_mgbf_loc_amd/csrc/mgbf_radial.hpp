@@ -37,7 +37,8 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     constexpr int NC = TI + 2 * H;
     constexpr int NCC = TI / 2 + 4;   // coarse columns of the tile (TI even)
     const int lane = threadIdx.x & (kWave - 1);
-    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
     if (item >= p.nitems) return;
     const int K = p.K, n = p.n, m = p.m;
     const int ti = item % p.ntile_i;
@@ -50,9 +51,11 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     const int i0 = ti * TI, j0 = tj * p.TJ;
     const size_t rowstride = (size_t)n * K;
 
-    int coff[NC];
+    // byte offsets: per lane k, per column wave-uniform (buffer addressing)
+    const int vk = kk * 4;
+    int soff[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) coff[c] = mirror_half(i0 - H + c, n) * K;
+    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - H + c, n) * K * 4;
 
     float acc[NR][TI];
 #pragma unroll
@@ -60,17 +63,20 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 #pragma unroll
         for (int t = 0; t < TI; ++t) acc[s][t] = 0.f;
 
-    // prolongation window
+    // prolongation window: 4 x-interpolated coarse rows, and the raw values
+    // of the row that enters next, loaded one window step ahead
     float XV[4][TI];
-    int ccol[NCC];
+    float cvn[NCC];
+    int csoff[NCC];
     int cbase = 0;
     const int nc = n / 2, mc = m / 2;
     const size_t crow = (size_t)nc * K;
-    auto load_xrow = [&](int r, float* dst) {
-        const float* rowp = p.coarse + kk + (size_t)mirror_half(r, mc) * crow;
-        float cv[NCC];
+    auto load_craw = [&](int r, float* dst) {
+        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(r, mc) * crow);
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) cv[c] = rowp[ccol[c]];
+        for (int c = 0; c < NCC; ++c) dst[c] = buf_ld(rc, vk, csoff[c]);
+    };
+    auto xinterp = [&](const float* cv, float* dst) {
 #pragma unroll
         for (int t = 0; t < TI; ++t) {
             if ((t & 1) == 0)
@@ -84,10 +90,14 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     if constexpr (PRO) {
         const int v0 = i0 / 2;
 #pragma unroll
-        for (int c = 0; c < NCC; ++c) ccol[c] = mirror_half(v0 - 2 + c, nc) * K;
+        for (int c = 0; c < NCC; ++c) csoff[c] = mirror_half(v0 - 2 + c, nc) * K * 4;
         cbase = ((j0 + 1) >> 1) - 2;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) load_xrow(cbase + u, XV[u]);
+        for (int u = 0; u < 4; ++u) {
+            load_craw(cbase + u, cvn);
+            xinterp(cvn, XV[u]);
+        }
+        load_craw(cbase + 4, cvn);
     }
 
     // vertical band coefficients of this lane's level (identity past nfilt)
@@ -106,9 +116,6 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
     const int nsteps = tj_eff + 2 * H;
-    const float* base_in = p.in + kk;
-    float* base_out = p.out + kk;
-    const float* base_w = p.wfield ? p.wfield + kk : nullptr;
 
     for (int rb = 0; rb < nsteps; rb += NR) {
 #pragma unroll
@@ -116,10 +123,10 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
             const int rr = rb + s;
             if (rr < nsteps) {
                 const int r = mirror_half(j0 - H + rr, m);
-                const float* row = base_in + (size_t)r * rowstride;
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)r * rowstride);
                 float v[NC];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) v[c] = row[coff[c]];
+                for (int c = 0; c < NC; ++c) v[c] = buf_ld(rs, vk, soff[c]);
 #pragma unroll
                 for (int t = 0; t < TI; ++t) {
                     float P[H + 1];
@@ -129,10 +136,13 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 #pragma unroll
                     for (int b = 0; b <= H; ++b) {
                         const int A = DISC ? disc_extent<H>(b) : H;
-                        float z = p.W.w[b][0] * P[0];
+                        // on the disc w(a,b) = w(b,a): one kernel argument each
+                        float z = p.W.w[DISC ? 0 : b][DISC ? b : 0] * P[0];
 #pragma unroll
                         for (int a = 1; a <= H; ++a)
-                            if (a <= A) z = __builtin_fmaf(p.W.w[b][a], P[a], z);
+                            if (a <= A)
+                                z = __builtin_fmaf(p.W.w[DISC && a < b ? a : b][DISC && a < b ? b : a],
+                                                   P[a], z);
                         acc[(s + b) % NR][t] += z;
                         if (b > 0) acc[(s + NR - b) % NR][t] += z;
                     }
@@ -142,21 +152,28 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                 constexpr int slot_off = NR - H;   // (s - H) mod NR
                 if (jrel >= 0 && jrel < tj_eff) {
                     const int j = j0 + jrel;
+                    const size_t rowoff = (size_t)j * rowstride + (size_t)i0 * K;
                     float o[TI];
+                    if (p.wfield) {
+                        const __amdgpu_buffer_rsrc_t rw = buf_rsrc(p.wfield + rowoff);
 #pragma unroll
-                    for (int t = 0; t < TI; ++t) {
-                        const size_t e = (size_t)j * rowstride + (size_t)(i0 + t < n ? i0 + t : n - 1) * K;
-                        const float sc = base_w ? base_w[e] : p.wconst;
-                        o[t] = sc * acc[(s + slot_off) % NR][t];
+                        for (int t = 0; t < TI; ++t) {
+                            const int tc = i0 + t < n ? t : n - 1 - i0;
+                            o[t] = buf_ld(rw, vk, tc * K * 4) * acc[(s + slot_off) % NR][t];
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < TI; ++t) o[t] = p.wconst * acc[(s + slot_off) % NR][t];
                     }
                     if constexpr (PRO) {
                         const int nb = ((j + 1) >> 1) - 2;
-                        if (nb != cbase) {
+                        if (nb != cbase) {   // the window moves down one coarse row
 #pragma unroll
                             for (int u = 0; u < 3; ++u)
 #pragma unroll
                                 for (int t = 0; t < TI; ++t) XV[u][t] = XV[u + 1][t];
-                            load_xrow(nb + 3, XV[3]);
+                            xinterp(cvn, XV[3]);
+                            load_craw(nb + 4, cvn);
                             cbase = nb;
                         }
                         const bool oj = (j & 1) != 0;
@@ -180,9 +197,10 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                         }
                     }
                     if (kvalid) {
+                        const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + rowoff);
 #pragma unroll
                         for (int t = 0; t < TI; ++t)
-                            if (i0 + t < n) base_out[(size_t)j * rowstride + (size_t)(i0 + t) * K] = o[t];
+                            if (i0 + t < n) buf_st(o[t], ro, vk, t * K * 4);
                     }
                 }
 #pragma unroll

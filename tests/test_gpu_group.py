@@ -74,3 +74,70 @@ def test_group_rejects_a_context_twice():
             M.Group([op, op])
     finally:
         op.close()
+
+
+# 24 regional domains drawn from (n, levels, generations): the full sizes of
+# the uneven-batch workload, radial half-width drawn from {6, 8, 10}
+UNEVEN_SIZES = [(384, 32, 4), (512, 64, 5), (768, 64, 5), (1024, 127, 6)]
+
+
+def _uneven_domains(seed=24, count=24):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n, lm, gm = UNEVEN_SIZES[rng.integers(len(UNEVEN_SIZES))]
+        out.append((n, lm, gm, int(rng.choice([6, 8, 10]))))
+    return out
+
+
+def test_uneven_batch_full_size_with_amplitude_fields():
+    """24 domains of 384^2x32 .. 1024^2x127 in one graph; generation 1 carries a
+    per-point amplitude field 0.5 + U(0,1). Every replay equals the eager calls
+    bit for bit; the first domain of the two smallest sizes is checked against
+    the FP64 oracle; the largest domain is checked for symmetry <x,By> = <Bx,y>."""
+    doms = _uneven_domains()
+    assert {d[0] for d in doms} == {384, 512, 768, 1024}
+    weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
+    cfgs = [M.Config.for_grid(n, n, lm, gm=gm, h=h, hz=3, weights=weights) for n, lm, gm, h in doms]
+    ops = [M.MGBF(c) for c in cfgs]
+    try:
+        gen = torch.Generator(device="cuda")
+        xs, amps = [], []
+        for i, op in enumerate(ops):
+            gen.manual_seed(500 + i)
+            xs.append(torch.randn(op.field_shape(), generator=gen, device=DEV))
+            amps.append(0.5 + torch.rand(op.field_shape(), generator=gen, device=DEV))
+            op.set_weights(1, field=amps[-1])
+        eager = [op.filtering_procedure(x) for op, x in zip(ops, xs)]
+        torch.cuda.synchronize()
+        outs = [torch.empty_like(x) for x in xs]
+        with M.Group(ops) as grp:
+            for rep in range(2):
+                for o in outs:
+                    o.fill_(float("nan"))
+                grp.apply(M.OP_WAWT, xs, outs)
+                torch.cuda.synchronize()
+                for i, (y, o) in enumerate(zip(eager, outs)):
+                    assert torch.equal(y, o), (rep, i, doms[i])
+                ops[rep].filtering_procedure(xs[rep])
+                torch.cuda.synchronize()
+        for size in (384, 512):
+            i = next(i for i, d in enumerate(doms) if d[0] == size)
+            cfg, op = cfgs[i], ops[i]
+            d = op.dims
+            P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx,
+                               hy=cfg.hy, hz=cfg.hz, p=2, gm=d.gm, ampl01=cfg.mg_ampl01,
+                               ampl02=cfg.mg_ampl02, weights=weights[: d.gm])
+            wf = [amps[i].cpu().double().numpy()] + [None] * (d.gm - 1)
+            ref = O.wawt(P, xs[i].cpu().double().numpy(), wfield=wf)
+            err = np.max(np.abs(eager[i].cpu().double().numpy() - ref)) / np.max(np.abs(ref))
+            assert err <= 1e-5, (doms[i], err)
+        i = next(i for i, d in enumerate(doms) if d[0] == 1024)
+        op = ops[i]
+        y2 = torch.randn(op.field_shape(), generator=gen, device=DEV)
+        By2 = op.filtering_procedure(y2)
+        lhs, rhs = op.dot(xs[i], By2), op.dot(eager[i], y2)
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+    finally:
+        for op in ops:
+            op.close()
