@@ -42,6 +42,18 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int vo
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
 }
 
+// Whole-wave single-lane shifts (DPP wave_shl:1 / wave_shr:1): lane l reads
+// lane l+1 (from above) or lane l-1 (from below); the lane with no source
+// reads 0.
+__device__ __forceinline__ float lane_from_above(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_from_below(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x138, 0xf, 0xf, false));
+}
+
 // Interpolation constants of the offset grids (def_offset_coef): fine index
 // 2u (even, 0-based) reads coarse u-2..u+1 with kQc, fine 2u+1 reads coarse
 // u-1..u+2 with kPc.

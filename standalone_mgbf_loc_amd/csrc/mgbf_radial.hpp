@@ -184,14 +184,21 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             o[t] += y0 * XV[0][t] + y1 * XV[1][t] + y2 * XV[2][t] + y3 * XV[3][t];
                     }
                     if constexpr (VHB > 0) {
+                        // the band over the lanes: level k+d sits in lane
+                        // lane+d, reached by d single-lane wave shifts (DPP);
+                        // lanes shifted in from outside the wave read 0 and
+                        // meet a zero band coefficient, as do lanes of the
+                        // neighbouring column
 #pragma unroll
                         for (int t = 0; t < TI; ++t) {
-                            float rsum = 0.f;
+                            float up = o[t], dn = o[t];
+                            float rsum = vc[VHB] * o[t];
 #pragma unroll
-                            for (int d = 0; d < NVB; ++d) {
-                                int src = lane + d - VHB;
-                                src = src < 0 ? 0 : (src > kWave - 1 ? kWave - 1 : src);
-                                rsum = __builtin_fmaf(vc[d], __shfl(o[t], src, kWave), rsum);
+                            for (int d = 1; d <= VHB; ++d) {
+                                up = lane_from_above(up);
+                                dn = lane_from_below(dn);
+                                rsum = __builtin_fmaf(vc[VHB + d], up, rsum);
+                                rsum = __builtin_fmaf(vc[VHB - d], dn, rsum);
                             }
                             o[t] = rsum;
                         }

@@ -170,6 +170,24 @@ def test_wawt_ensemble_matches_oracle(n_ens, lm, gm, h, filt_g1):
         assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
 
 
+@pytest.mark.parametrize("lm,n_ens,hz,exact", [(32, 2, 3, False), (16, 4, 2, False), (64, 2, 4, False),
+                                            (32, 2, 3, True), (64, 1, 4, True), (8, 8, 3, True)])
+def test_vertical_band_fused_into_the_last_radial_pass(monkeypatch, lm, n_ens, hz, exact):
+    # whole columns per wavefront (lm divides 64, K % 64 == 0): the vertical
+    # band (Vf Vf^T, half-width 2hz, or Vf alone in the reference's order)
+    # rides in the epilogue of the last radial pass, across the lanes
+    if exact:
+        monkeypatch.setenv("MGBF_EXACT_ORDER", "1")
+    cfg = M.Config.for_grid(64, 64, lm, n_ens=n_ens, gm=3, h=4, hz=hz, weights=[1.0, 0.6, 0.3])
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.km_all % 64 == 0 and 64 % lm == 0
+        x = field(d.gen_shape(1), 50 + lm + hz)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+
+
 def test_wawt_background_procedure_with_2d_variables():
     cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=2, km2=3, km3=2, mgbf_proc=1)
     with M.MGBF(cfg) as op:
