@@ -35,6 +35,26 @@ def bytes_per_unit(gm: int) -> float:
     return 4.0 * (4 * S + 2 * (S - 1))
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "radial_80x768x768x64_hbm_traffic.md")
+
+
+def profiled_traffic(M, n, lm, gm):
+    """HBM bytes per radial launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE on this workload, calibrated against a copy kernel; recorded in
+    the profile summary as `traffic_bytes_per_launch: <workload> <bytes>`."""
+    key = f"{M}x{n}x{n}x{lm}_gm{gm}"
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == key:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def make_members(torch, M, n, lm, seed0=100):
     x = torch.empty((n, n, M * lm), dtype=torch.float32, device="cuda")
     v = x.view(n, n, M, lm)
@@ -73,6 +93,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default radial tiles")
+    ap.add_argument("--tiles", default=None,
+                    help='pin radial tiles instead of autotuning, JSON as printed under "tiles", '
+                         'e.g. \'{"g1k0": [4, 384], ...}\'')
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
     args = ap.parse_args()
 
@@ -103,7 +126,13 @@ def main():
     op = MG.MGBF(cfg)
     assert op.dims.gm == gm
     tiles = None
-    if not args.no_tune:
+    if args.tiles:
+        pinned = {(int(k[1:k.index("k")]), int(k[k.index("k") + 1:])): tuple(v)
+                  for k, v in json.loads(args.tiles).items()}
+        op.set_tiles(pinned)
+        tiles = op.tiles()
+        log(f"[bench] pinned tiles: {tiles}")
+    elif not args.no_tune:
         t0 = time.perf_counter()
         tiles = op.autotune()
         log(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiles}")
@@ -112,26 +141,24 @@ def main():
     for _ in range(args.warmup):
         op.filtering_procedure(x, out=y)
     torch.cuda.synchronize()
+    # the library's per-phase device timers (HIP events on the launch stream
+    # around every kernel of a phase) run through the timed steps
+    op.timers(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         op.filtering_procedure(x, out=y)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    phases = op.read_timers()
+    op.timers(False)
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = units / (ms_step / 1e3)
     log(f"[bench] batched: {ms_step:.3f} ms/step, {value:.4e} units/s")
-
-    # ---- per-phase device timers (one extra step) ----
-    op.timers(True)
-    op.filtering_procedure(x, out=y)
-    phases = op.read_timers()
-    op.timers(False)
-    torch.cuda.synchronize()
     r_ms = phases["hfiltT"][0] + phases["hfilt"][0]
     r_n = phases["hfiltT"][1] + phases["hfilt"][1]
     r_bytes = phases["hfiltT"][2] + phases["hfilt"][2]
     achieved = r_bytes / (r_ms / 1e3) / 1e9
-    phase_out = {k: {"ms": round(v[0], 4), "launches": v[1],
+    phase_out = {k: {"ms": round(v[0] / args.steps, 4), "launches": v[1] // args.steps,
                      "GBps": round(v[2] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 else None}
                  for k, v in phases.items()}
     log(f"[bench] phases: {phase_out}")
@@ -203,7 +230,8 @@ def main():
                                f"weights [1,1/2,1/4,1/8]",
                    "members": M, "nx": n, "ny": n, "levels": lm, "gm": gm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": profiled_traffic(M, n, lm, gm),
                      "kernel": "radial_kernel (F and F^T on every generation)",
                      "launches": r_n, "avg_launch_ms": round(r_ms / max(r_n, 1), 4),
                      "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1)},

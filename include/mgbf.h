@@ -186,6 +186,12 @@ int mgbf_autotune(mgbf_ctx* ctx, void* stream);
    runs on the matrix cores (16 output columns per block of 4 wavefronts);
    *tj is the output rows per wavefront or block. */
 int mgbf_get_tile(const mgbf_ctx* ctx, int gen, int kind, int* ti, int* tj);
+/* Pins the tile of generation gen (1-based) and pass kind, in the terms of
+   mgbf_get_tile (ti = 2 or 4, or 16 for the matrix cores, which only the
+   plain pass kind 0 offers, for half-widths up to 8; tj >= 1), so that tiles
+   found by mgbf_autotune in one process can be reused in another without
+   timing again. No counterpart in the reference. */
+int mgbf_set_tile(mgbf_ctx* ctx, int gen, int kind, int ti, int tj);
 
 /* Message of the last failure on this thread ("" if none). */
 const char* mgbf_last_error(void);

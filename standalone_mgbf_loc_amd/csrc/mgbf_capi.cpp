@@ -792,6 +792,19 @@ int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
     return MGBF_OK;
 }
 
+int mgbf_set_tile(mgbf_ctx* c, int gen, int kind, int ti, int tj) {
+    if (!c || gen < 1 || gen > c->gm || kind < 0 || kind > 2)
+        return fail(MGBF_ERR_ARG, "bad argument");
+    if (!(ti == 2 || ti == 4 || ti == 16) || tj < 1)
+        return fail(MGBF_ERR_ARG, "tile must be ti in {2, 4, 16} and tj >= 1");
+    if (ti == 16 && (kind != 0 || !c->mfma_ok))
+        return fail(MGBF_ERR_UNSUPPORTED, "the matrix-core engine serves pass kind 0 with half-width <= 8");
+    c->tile_ti[gen - 1][kind] = ti == 16 ? 4 : ti;
+    c->tile_tj[gen - 1][kind] = tj;
+    c->tile_mfma[gen - 1][kind] = ti == 16 ? 1 : 0;
+    return MGBF_OK;
+}
+
 }  // extern "C"
 
 // A group applies one operation over several contexts (regional domains of

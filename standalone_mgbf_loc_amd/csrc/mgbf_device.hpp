@@ -42,6 +42,8 @@ __device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, int vo
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
 }
 
+__device__ __forceinline__ unsigned umax(unsigned a, unsigned b) { return a > b ? a : b; }
+
 // Whole-wave single-lane shifts (DPP wave_shl:1 / wave_shr:1): lane l reads
 // lane l+1 (from above) or lane l-1 (from below); the lane with no source
 // reads 0.

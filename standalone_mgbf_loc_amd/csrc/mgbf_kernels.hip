@@ -235,8 +235,12 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
     constexpr int WR = kRWidthDev;
     constexpr int NCF = 2 * TI2 + 6;
     const int lane = threadIdx.x & (kWave - 1);
-    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
+    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (item >= nitems) return;
+    // k chunks fastest: consecutive waves read the whole k extent of a column
+    // (K x 4 contiguous bytes) together, which keeps DRAM pages open; the
+    // halo columns this re-reads mostly hit the caches
     const int kc = item % nkc;
     const int rest = item / nkc;
     const int ti = rest % ntile_i;
@@ -255,14 +259,16 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
         return;
     }
     const size_t frow = (size_t)nf * K;
-    // interior columns: fine columns 2*I0-3 .. 2*I0+2*TI2+2
-    const float* base = fine + kk + (size_t)(2 * I0 - 3) * K;
+    // interior columns: fine columns 2*I0-3 .. 2*I0+2*TI2+2, addressed as the
+    // row's buffer + lane k + wave-uniform column offset
+    const float* base = fine + (size_t)(2 * I0 - 3) * K;
+    const int vk = kk * 4;
     auto xrow = [&](int fr, float* out) {
-        const float* row = base + (size_t)fr * frow;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(base + (size_t)fr * frow);
         float v[NCF];
 #pragma unroll
         for (int c = 0; c < NCF; ++c) {
-            v[c] = row[(size_t)c * K];
+            v[c] = buf_ld(rs, vk, c * K * 4);
             lmax = fmaxf(lmax, fabsf(v[c]));
         }
 #pragma unroll
@@ -314,9 +320,9 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
             wbase = -1000;
         }
         if (kvalid) {
+            const __amdgpu_buffer_rsrc_t ro = buf_rsrc(coarse + (size_t)K * ((size_t)I0 + (size_t)nc * J));
 #pragma unroll
-            for (int c = 0; c < TI2; ++c)
-                coarse[kk + (size_t)K * ((size_t)(I0 + c) + (size_t)nc * J)] = acc[c];
+            for (int c = 0; c < TI2; ++c) buf_st(acc[c], ro, vk, c * K * 4);
         }
     }
     if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));

@@ -34,8 +34,6 @@ __device__ __forceinline__ int mfma_exp(float m) {
     return e < -100 ? -100 : (e > 100 ? 100 : e);
 }
 
-__device__ __forceinline__ unsigned umax(unsigned a, unsigned b) { return a > b ? a : b; }
-
 template <int H>
 __global__ void __launch_bounds__(kBlock) radial_mfma_kernel(RadialMfmaParams p) {
     static_assert(kMfmaCols + 2 * H <= kMfmaWin, "window exceeds one MFMA K step");

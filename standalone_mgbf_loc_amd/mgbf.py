@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     "mgbf_config_default", "mgbf_derive", "mgbf_init", "mgbf_finalize", "mgbf_get_dims",
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
     "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
-    "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
+    "mgbf_set_tile", "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
 )
 TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
@@ -182,6 +182,8 @@ def load_library(path: Optional[str] = None):
     L.mgbf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.mgbf_get_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.mgbf_set_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int]
     L.mgbf_group_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_void_p)]
     L.mgbf_group_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
@@ -347,6 +349,12 @@ class MGBF:
                 _check(self._L.mgbf_get_tile(self._h, g, kind, ctypes.byref(ti), ctypes.byref(tj)))
                 out[(g, kind)] = (ti.value, tj.value)
         return out
+
+    def set_tiles(self, tiles: dict):
+        """Pins tiles {(gen, kind): (TI, TJ)} as returned by autotune()/tiles()
+        (TI = 16: the matrix cores)."""
+        for (g, kind), (ti, tj) in tiles.items():
+            _check(self._L.mgbf_set_tile(self._h, int(g), int(kind), int(ti), int(tj)))
 
     def timers(self, enable: bool = True):
         """Turns the per-phase device timers on or off (and clears them)."""

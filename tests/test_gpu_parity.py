@@ -282,6 +282,13 @@ def test_autotuned_tiles_keep_parity():
         assert (y1 - y0).abs().max().item() <= 1e-6 * y0.abs().max().item() or \
             any(ti == 16 for ti, _ in tiles.values())
         assert relerr(y1, O.wawt(oparams(cfg, d), xh.astype(np.float64))) <= TOL_OP
+        # pinned in a second context, the same tiles give the same bits
+        with M.MGBF(cfg) as op2:
+            op2.set_tiles(tiles)
+            assert op2.tiles() == tiles
+            y2 = op2.filtering_procedure(x)
+            sync()
+            assert torch.equal(y1, y2)
 
 
 @pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 40, 80, 48, 3, 1e-6),
