@@ -177,7 +177,8 @@ int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* byt
 /* Times the radial filter on this context's shapes for every candidate tile
    (output columns per lane x output rows per wavefront), for each generation
    and each pass kind (0 plain, 1 with the prolongation, 2 with the
-   prolongation and the vertical band), and keeps the fastest. Uses the
+   prolongation and the vertical band, 3 plain with the restriction to the
+   next generation), and keeps the fastest. Uses the
    context's workspace; synchronizes `stream`. No counterpart in the
    reference, whose loops have no tiles. */
 int mgbf_autotune(mgbf_ctx* ctx, void* stream);

@@ -75,11 +75,19 @@ struct mgbf_ctx {
     mgbf::VBandW band_int[3] = {};
     int band_hb[3] = {}, band_ilo[3] = {}, band_ihi[3] = {};
     bool commute = true;     // weights constant along the stack: B = Vf Vf^T H
-    int fuse = 2;            // epilogue fusion of the up-sweep: 0 none, 1 +P, 2 +P and band
-    // radial tiles per generation and pass kind (0 plain, 1 +P, 2 +P and band)
-    int tile_ti[mgbf::kMaxGen][3] = {};
-    int tile_tj[mgbf::kMaxGen][3] = {};
-    int tile_mfma[mgbf::kMaxGen][3] = {};   // 1: matrix-core engine for this pass
+    // fusion into the radial passes: 0 none, 1 prolongation, 2 prolongation and
+    // band (up-sweep epilogues), 3 also the restriction (down-sweep)
+    int fuse = 3;
+    // radial tiles per generation and pass kind (0 plain, 1 +P, 2 +P and band,
+    // 3 plain with the restriction to the next generation)
+    int tile_ti[mgbf::kMaxGen][4] = {};
+    int tile_tj[mgbf::kMaxGen][4] = {};
+    int tile_mfma[mgbf::kMaxGen][4] = {};   // 1: matrix-core engine for this pass
+    // fused restriction: per-fine-row y tables, and whether generation g's
+    // restriction has the structure the fused kernel assumes
+    const float4* d_rytab[mgbf::kMaxGen] = {};
+    const int2* d_rymeta[mgbf::kMaxGen] = {};
+    bool res_ok[mgbf::kMaxGen] = {};
     // matrix-core engine: per-k max |u_g| from the restriction, weight fragments
     float* d_kmax = nullptr;                 // gm x K
     unsigned kmax_valid = 0;                 // generations whose max is current
@@ -182,6 +190,90 @@ int pick_tj(int K, int n, int m, int ti) {
 
 enum { BAND_VF = 0, BAND_VFT = 1, BAND_VVT = 2 };
 
+// Restriction weights of the 2 coarse indices next to an edge over the first
+// 8 fine indices (the host twin of the kernels' compile-time table).
+void edge_restrict_weights(double w[2][8]) {
+    const double kp[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
+    const double kq[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
+    for (int I = 0; I < 2; ++I)
+        for (int t = 0; t < 8; ++t) w[I][t] = 0.0;
+    for (int i = 0; i < 8; ++i) {
+        const bool odd = (i & 1) != 0;
+        const int u = i / 2;
+        for (int t = 0; t < 4; ++t) {
+            int I = odd ? u - 1 + t : u - 2 + t;
+            if (I < 0) I = -1 - I;
+            if (I < 2) w[I][i] += odd ? kp[t] : kq[t];
+        }
+    }
+}
+
+// Does the restriction of a generation with nf x mf fine points have the
+// shape the fused radial kernel assumes? Along x: interior coarse columns
+// read fine 2I-3 .. 2I+4 with the 8-point stencil, the 2 columns at each edge
+// read the 8 fine columns next to it with the edge weights; nf % 4 == 0.
+// Along y the per-fine-row table (lowest open coarse row Jb, rows closing at
+// v, weights into rows Jb .. Jb+5, padded to 8) is built here; every fine row
+// must feed only those 6 rows (4 in the interior, 6 next to an edge, where
+// the two edge rows share a window), and windows must close in row order.
+bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int nf, int mf,
+                      std::vector<float>& tab, std::vector<int>& meta) {
+    if (nf % 4 || nf < 8 || mf < 8 || rx.width != 8 || ry.width != 8) return false;
+    const double w8[8] = {-3. / 128., -9. / 128., 29. / 128., 111. / 128.,
+                          111. / 128., 29. / 128., -9. / 128., -3. / 128.};
+    double we[2][8];
+    edge_restrict_weights(we);
+    const int nc = nf / 2, mc = mf / 2;
+    auto close = [](double a, double b) { return std::fabs(a - b) <= 1e-12; };
+    for (int I = 0; I < nc; ++I) {
+        for (int t = 0; t < 8; ++t) {
+            const double w = rx.w[(size_t)I * 8 + t];
+            bool ok;
+            if (I < 2) ok = rx.start[I] == 0 && close(w, we[I][t]);
+            else if (I >= nc - 2) ok = rx.start[I] == nf - 8 && close(w, we[nc - 1 - I][7 - t]);
+            else ok = rx.start[I] == 2 * I - 3 && close(w, w8[t]);
+            if (!ok) return false;
+        }
+    }
+    for (int J = 1; J < mc; ++J)
+        if (ry.start[J] < ry.start[J - 1]) return false;
+    tab.assign((size_t)mf * 8, 0.f);
+    meta.assign((size_t)mf * 2, 0);
+    for (int v = 0; v < mf; ++v) {
+        int jb = 0, ne = 0;
+        for (int J = 0; J < mc; ++J) {
+            const int e = ry.start[J] + 7;
+            if (e < v) ++jb;
+            if (e == v) ++ne;
+        }
+        for (int J = 0; J < mc; ++J) {
+            if (v < ry.start[J] || v > ry.start[J] + 7) continue;
+            if (J < jb || J > jb + 5) return false;
+            tab[(size_t)v * 8 + (J - jb)] = (float)ry.w[(size_t)J * 8 + (v - ry.start[J])];
+        }
+        if (ne > 6) return false;
+        meta[(size_t)v * 2] = jb;
+        meta[(size_t)v * 2 + 1] = ne;
+    }
+    return true;
+}
+
+// Can the restriction of generation g ride in its first radial pass with
+// output rows per wave TJ? The wave of output rows j0 .. j0+TJ-1 loads fine
+// rows j0-H .. j0+TJ-1+H and owns coarse rows j0/2 .. (j0+TJ)/2-1, whose
+// 8-row windows must lie inside what it loads.
+bool fused_ok(const mgbf_ctx* c, int g, int tj) {
+    if (g + 1 >= c->gm || !c->res_ok[g] || c->s.H < 4 || tj < 2 || (tj & 1)) return false;
+    const int m = c->ny[g], H = c->s.H;
+    const std::vector<int>& st = c->s.ry[g].start;
+    for (int j0 = 0; j0 < m; j0 += tj) {
+        const int te = tj < m - j0 ? tj : m - j0;
+        for (int J = j0 / 2; J < (j0 + te) / 2; ++J)
+            if (st[J] < j0 - H || st[J] + 7 > j0 + te - 1 + H) return false;
+    }
+    return true;
+}
+
 // Can the vertical band `which` be fused into the last radial pass of
 // generation 1? Each wave must hold whole columns, and the variant built.
 bool can_fuse_band(const mgbf_ctx* c, int which) {
@@ -194,9 +286,10 @@ bool can_fuse_band(const mgbf_ctx* c, int which) {
 // out = scale * F(in) [+ P coarse] [then vertical band `band`] on generation g
 int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield, float wconst,
            hipStream_t st, int phase = MGBF_T_HFILT, const float* coarse = nullptr,
-           int band = -1) {
+           int band = -1, float* rout = nullptr) {
     Phase ph(c, phase, st,
-             4.0 * ((double)c->gsize[g] * (wfield ? 3 : 2) + (coarse ? (double)c->gsize[g + 1] : 0.0)));
+             4.0 * ((double)c->gsize[g] * (wfield ? 3 : 2) +
+                    (coarse || rout ? (double)c->gsize[g + 1] : 0.0)));
     mgbf::RadialArgs a{};
     a.in = in;
     a.out = out;
@@ -213,9 +306,15 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
     a.n = c->nx[g];
     a.m = c->ny[g];
     a.H = c->s.H;
-    const int kind = band >= 0 ? 2 : (coarse ? 1 : 0);
+    const int kind = rout ? 3 : (band >= 0 ? 2 : (coarse ? 1 : 0));
     a.TI = c->tile_ti[g][kind];
     a.TJ = c->tile_tj[g][kind];
+    if (rout) {
+        a.rout = rout;
+        a.rytab = c->d_rytab[g];
+        a.rymeta = c->d_rymeta[g];
+        a.rkmax = c->d_kmax ? c->d_kmax + (size_t)(g + 1) * c->K : nullptr;
+    }
     if (kind == 0 && c->tile_mfma[g][0] && c->mfma_ok && ((c->kmax_valid >> g) & 1u)) {
         a.mfma = true;
         a.kmax_in = c->d_kmax + (size_t)g * c->K;
@@ -287,14 +386,41 @@ int restrict_chain(mgbf_ctx* c, const float* u1, float* U, hipStream_t st) {
     return MGBF_OK;
 }
 
+// Down-sweep: the first radial pass of every filtered generation,
+// T_g = A_g F u_g (A_g = 1 unless `weighted`), written at T + goff[g], and
+// the restriction u_{g+1} = R u_g into U + goff[g+1]. The restriction rides
+// in the radial pass where the tiles allow (fusion level 3), else it runs on
+// its own first. u_1 is read from u1.
+int down_sweep(mgbf_ctx* c, const float* u1, float* U, float* T, bool weighted, hipStream_t st) {
+    c->kmax_valid = 0;
+    if (c->d_kmax)
+        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)c->gm * c->K, st), "fill launch");
+    for (int g = 0; g < c->gm; ++g) {
+        const float* ug = g == 0 ? u1 : U + c->goff[g];
+        const bool last = g + 1 >= c->gm;
+        const bool filt = g > 0 || c->filt_g1;
+        float* next = last ? nullptr : U + c->goff[g + 1];
+        const float* wf = weighted ? c->wfield[g] : nullptr;
+        const float wc = weighted ? c->wconst[g] : 1.0f;
+        if (filt && !last && c->fuse >= 3 && fused_ok(c, g, c->tile_tj[g][3])) {
+            CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT, nullptr, -1, next));
+            if (c->d_kmax) c->kmax_valid |= 1u << (g + 1);
+            continue;
+        }
+        if (!last) CALL(restrict_g(c, g, ug, next, st));
+        if (filt) CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT));
+    }
+    return MGBF_OK;
+}
+
 // Up-sweep from the coarsest generation: for g = gm..1 (1-based)
-//   s_g = F (A_g F u_g) + P s_{g+1}      (twice: W A W^T)
-//   s_g = F (A_g^{1/2}? v_g) + P s_{g+1}  (once: W, W A^{1/2})
-// Generation 1 is read from in1, generation g >= 2 from inb + goff[g]; s_g is
-// written into U. The last pass writes y = band(s_1) directly when the
-// vertical band fuses (or there is none), else s_1 goes to U0 and a separate
-// vertical pass follows.
-int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band, bool twice,
+//   s_g = F in_g + P s_{g+1},  in_1 = in1, in_g = inb + goff[g] (g >= 2)
+// (in_g = T_g after a down-sweep for W A W^T; the control vector's v_g for W,
+// scaled by A_g^{1/2} first when sqrt_w). s_g is written into U. The last
+// pass writes y = band(s_1) directly when the vertical band fuses (or there
+// is none), else s_1 goes to U0 and a separate vertical pass follows. An
+// unfiltered generation 1 gives s_1 = u1 + P s_2.
+int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, float* y, int band,
             bool sqrt_w, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
@@ -307,23 +433,17 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band,
         float* dst = direct ? y : U + c->goff[g];
         const int fb = (g == 0 && fuse) ? band : -1;
         if (g == 0 && !c->filt_g1) {
-            // generation 1 unfiltered: s_1 = u_1 + P s_2
             float* s1 = band >= 0 ? U : y;
-            if (coarse) CALL(prolong_g(c, 0, coarse, src, s1, st));
-            else if (src != s1) HIP_TRY(mgbf::launch_copy(src, s1, c->gsize[0], st), "copy launch");
+            if (coarse) CALL(prolong_g(c, 0, coarse, u1, s1, st));
+            else if (u1 != s1) HIP_TRY(mgbf::launch_copy(u1, s1, c->gsize[0], st), "copy launch");
             return band >= 0 ? vertical(c, U, y, band, st) : MGBF_OK;
         }
-        if (twice) {
-            CALL(radial(c, g, src, T, c->wfield[g], c->wconst[g], st, MGBF_T_HFILTT));
-            CALL(radial(c, g, T, dst, nullptr, 1.0f, st, MGBF_T_HFILT, pro, fb));
-        } else {
-            if (sqrt_w) {
-                HIP_TRY(mgbf::launch_scale(src, T, c->wfield[g], c->wconst[g], 1, c->gsize[g], st),
-                        "scale launch");
-                src = T;
-            }
-            CALL(radial(c, g, src, dst, nullptr, 1.0f, st, MGBF_T_HFILT, pro, fb));
+        if (sqrt_w) {
+            HIP_TRY(mgbf::launch_scale(src, T, c->wfield[g], c->wconst[g], 1, c->gsize[g], st),
+                    "scale launch");
+            src = T;
         }
+        CALL(radial(c, g, src, dst, nullptr, 1.0f, st, MGBF_T_HFILT, pro, fb));
         if (coarse && !pro) CALL(prolong_g(c, g, coarse, dst, dst, st));
     }
     if (band >= 0 && !fuse) return vertical(c, U, y, band, st);
@@ -335,8 +455,8 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, float* y, int band,
 // vertical filter, B = Vf H Vf^T = (Vf Vf^T) H; one banded vertical pass,
 // fused into the last radial pass when possible.
 int apply_wawt_commuted(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
-    CALL(restrict_chain(c, x, c->ws_u, st));
-    return upsweep(c, x, c->ws_u, y, c->nfilt ? BAND_VVT : -1, true, false, st);
+    CALL(down_sweep(c, x, c->ws_u, c->ws_t, true, st));
+    return upsweep(c, c->ws_t, c->ws_t, x, y, c->nfilt ? BAND_VVT : -1, false, st);
 }
 
 int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
@@ -345,20 +465,19 @@ int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     if (!fields && c->commute) return apply_wawt_commuted(c, x, y, st);
     float* U = c->ws_u;
     CALL(vertical(c, x, U, BAND_VFT, st));
-    CALL(restrict_chain(c, U, U, st));
-    return upsweep(c, U, U, y, c->nfilt ? BAND_VF : -1, true, false, st);
+    CALL(down_sweep(c, U, U, c->ws_t, true, st));
+    return upsweep(c, c->ws_t, c->ws_t, U, y, c->nfilt ? BAND_VF : -1, false, st);
 }
 
 int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
     float* U = c->ws_u;
     CALL(vertical(c, x, U, BAND_VFT, st));
-    CALL(restrict_chain(c, U, U, st));
+    CALL(down_sweep(c, U, U, v, false, st));
     for (int g = 0; g < c->gm; ++g) {
         if (g == 0 && !c->filt_g1) {
             HIP_TRY(mgbf::launch_copy(U, v, c->gsize[0], st), "copy launch");
             continue;
         }
-        CALL(radial(c, g, U + c->goff[g], v + c->goff[g], nullptr, 1.0f, st, MGBF_T_HFILTT));
         if (sqrt_weights)
             HIP_TRY(mgbf::launch_scale(v + c->goff[g], v + c->goff[g], c->wfield[g], c->wconst[g], 1,
                                        c->gsize[g], st),
@@ -371,7 +490,7 @@ int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream
 // the hierarchy, then Vf.
 int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
     c->kmax_valid = 0;
-    return upsweep(c, v, v, y, c->nfilt ? BAND_VF : -1, false, sqrt_weights, st);
+    return upsweep(c, v, v, v, y, c->nfilt ? BAND_VF : -1, sqrt_weights, st);
 }
 
 int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStream_t st) {
@@ -469,7 +588,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         // MGBF_FUSE=0|1|2 selects how much of the up-sweep rides in the
         // epilogue of the last radial pass (all levels compute the same)
         const char* fv = std::getenv("MGBF_FUSE");
-        if (fv && fv[0] >= '0' && fv[0] <= '2') c->fuse = fv[0] - '0';
+        if (fv && fv[0] >= '0' && fv[0] <= '3') c->fuse = fv[0] - '0';
         // MGBF_MFMA=1 puts every eligible first pass on the matrix cores
         const char* mv = std::getenv("MGBF_MFMA");
         if (mv && mv[0] == '1')
@@ -483,12 +602,15 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->goff[g] = off;
         off += c->gsize[g];
         c->wconst[g] = (float)s.wconst[g];
-        for (int kind = 0; kind < 3; ++kind) {
+        for (int kind = 0; kind < 4; ++kind) {
             c->tile_ti[g][kind] = 4;
             c->tile_tj[g][kind] = pick_tj(c->K, s.nx[g], s.ny[g], 4);
             if (kind == 0 && c->tile_mfma[g][0])
                 c->tile_tj[g][0] = pick_tj(c->K, s.nx[g], s.ny[g], 16);
         }
+        // the fused restriction needs an even number of rows per wave
+        int& tj3 = c->tile_tj[g][3];
+        if (tj3 & 1) tj3 = tj3 + 1 <= s.ny[g] ? tj3 + 1 : tj3 - 1;
     }
     for (int b = 0; b <= mgbf::kMaxHaloDev; ++b)
         for (int a = 0; a <= mgbf::kMaxHaloDev; ++a)
@@ -558,10 +680,14 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     // one device allocation for all coefficient tables
     size_t bytes = carve_size(sizeof(double)) + carve_size(sizeof(_Float16) * afrag.size());
     for (int b = 0; b < 3; ++b) bytes += carve_size(sizeof(float) * bands[b]->size());
+    std::vector<float> rytab[mgbf::kMaxGen];
+    std::vector<int> rymeta[mgbf::kMaxGen];
     for (int g = 0; g + 1 < c->gm; ++g) {
         const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
         for (const auto* t : tabs)
             bytes += carve_size(sizeof(int) * t->n_out) + carve_size(sizeof(float) * t->w.size());
+        c->res_ok[g] = build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], rytab[g], rymeta[g]);
+        bytes += carve_size(sizeof(float) * rytab[g].size()) + carve_size(sizeof(int) * rymeta[g].size());
     }
     std::vector<char> host(bytes, 0);
     hipError_t e = hipMalloc(&c->d_tables, bytes);
@@ -600,6 +726,16 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->d_pxw[g] = put_f(s.px[g].w);
         c->d_pys[g] = put_i(s.py[g].start);
         c->d_pyw[g] = put_f(s.py[g].w);
+        if (c->res_ok[g]) {
+            float* hf = carve<float>(hp, rytab[g].size());
+            float* df = carve<float>(dp, rytab[g].size());
+            for (size_t i = 0; i < rytab[g].size(); ++i) hf[i] = rytab[g][i];
+            int* hi = carve<int>(hp, rymeta[g].size());
+            int* di = carve<int>(dp, rymeta[g].size());
+            for (size_t i = 0; i < rymeta[g].size(); ++i) hi[i] = rymeta[g][i];
+            c->d_rytab[g] = reinterpret_cast<const float4*>(df);
+            c->d_rymeta[g] = reinterpret_cast<const int2*>(di);
+        }
     }
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
@@ -720,14 +856,16 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     const bool log = logv && logv[0] == '1';
     // the restriction chain records the per-k maxima the matrix-core engine needs
     if (status == MGBF_OK) status = restrict_chain(c, c->ws_u, c->ws_u, st);
-    auto time_pass = [&](int g, const float* coarse, int band, float* ms) -> int {
+    // kind 3 writes its restriction over the (zero) next generation of U,
+    // which the restriction of zeros leaves unchanged
+    auto time_pass = [&](int g, const float* coarse, int band, float* rout, float* ms) -> int {
         int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT,
-                       coarse, band);
+                       coarse, band, rout);
         if (r != MGBF_OK) return r;
         if (hipEventRecord(e0, st) != hipSuccess) return fail(MGBF_ERR_HIP, "autotune timing");
         for (int rep = 0; rep < 3 && r == MGBF_OK; ++rep)
             r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT, coarse,
-                       band);
+                       band, rout);
         if (r != MGBF_OK) return r;
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(ms, e0, e1) != hipSuccess)
@@ -736,10 +874,12 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     };
     for (int g = 0; g < c->gm && status == MGBF_OK; ++g) {
         const int n = c->nx[g], m = c->ny[g];
-        for (int kind = 0; kind < 3 && status == MGBF_OK; ++kind) {
+        for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
             if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
-            const float* coarse = kind >= 1 ? c->ws_u + c->goff[g + 1] : nullptr;
+            if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 4)) continue;
+            const float* coarse = kind == 1 || kind == 2 ? c->ws_u + c->goff[g + 1] : nullptr;
+            float* rout = kind == 3 ? c->ws_u + c->goff[g + 1] : nullptr;
             const int band = kind == 2 ? vband : -1;
             float best = 1e30f;
             int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind], bmf = c->tile_mfma[g][kind];
@@ -749,11 +889,13 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             const int engines[3] = {2, 4, 16};
             for (int en : engines) {
                 if (en == 16 && !try_mfma) continue;
+                if (kind == 3 && en != 4) continue;
                 for (int q = 0; q < 11 && status == MGBF_OK; ++q) {
                     const int tj = tjs[q];
                     bool dup = false;
                     for (int q2 = 0; q2 < q; ++q2) dup = dup || tjs[q2] == tj;
                     if (tj > m || dup) continue;
+                    if (kind == 3 && !fused_ok(c, g, tj)) continue;
                     const long long waves = (long long)((c->K + 63) / 64) * ((n + en - 1) / en) *
                                             ((m + tj - 1) / tj) * (en == 16 ? 4 : 1);
                     if (waves < 2048 && tj != 16) continue;
@@ -761,7 +903,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
                     c->tile_tj[g][kind] = tj;
                     c->tile_mfma[g][kind] = en == 16 ? 1 : 0;
                     float ms = 0.f;
-                    status = time_pass(g, coarse, band, &ms);
+                    status = time_pass(g, coarse, band, rout, &ms);
                     if (log && status == MGBF_OK)
                         std::fprintf(stderr, "[autotune] gen %d kind %d engine %d TJ %d: %.4f ms\n",
                                      g + 1, kind, en, tj, ms / 3);
@@ -785,7 +927,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
 }
 
 int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
-    if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 2)
+    if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 3)
         return fail(MGBF_ERR_ARG, "bad argument");
     *ti = c->tile_mfma[gen - 1][kind] ? 16 : c->tile_ti[gen - 1][kind];
     *tj = c->tile_tj[gen - 1][kind];
@@ -793,10 +935,12 @@ int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
 }
 
 int mgbf_set_tile(mgbf_ctx* c, int gen, int kind, int ti, int tj) {
-    if (!c || gen < 1 || gen > c->gm || kind < 0 || kind > 2)
+    if (!c || gen < 1 || gen > c->gm || kind < 0 || kind > 3)
         return fail(MGBF_ERR_ARG, "bad argument");
     if (!(ti == 2 || ti == 4 || ti == 16) || tj < 1)
         return fail(MGBF_ERR_ARG, "tile must be ti in {2, 4, 16} and tj >= 1");
+    if (kind == 3 && (ti != 4 || (tj & 1)))
+        return fail(MGBF_ERR_ARG, "the pass with the restriction takes ti = 4 and an even tj");
     if (ti == 16 && (kind != 0 || !c->mfma_ok))
         return fail(MGBF_ERR_UNSUPPORTED, "the matrix-core engine serves pass kind 0 with half-width <= 8");
     c->tile_ti[gen - 1][kind] = ti == 16 ? 4 : ti;

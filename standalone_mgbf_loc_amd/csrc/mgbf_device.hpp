@@ -67,4 +67,27 @@ constexpr float kQc[4] = {-3.f / 128.f, 29.f / 128.f, 111.f / 128.f, -9.f / 128.
 constexpr float kW8[8] = {-3.f / 128.f, -9.f / 128.f, 29.f / 128.f, 111.f / 128.f,
                           111.f / 128.f, 29.f / 128.f, -9.f / 128.f, -3.f / 128.f};
 
+// Restriction weights of the two coarse indices next to a domain edge over
+// the first 8 fine indices: R = P^T where fine 2u reads coarse u-2..u+1 with
+// kQc and fine 2u+1 reads coarse u-1..u+2 with kPc, the coarse indices
+// mirrored (half-sample) into the domain. The far edge is the mirror image:
+// R[nc-1-I][nf-1-t] = R[I][t].
+struct EdgeRW {
+    float w[2][8];
+};
+constexpr EdgeRW make_edge_rw() {
+    EdgeRW e{};
+    for (int i = 0; i < 8; ++i) {
+        const bool odd = (i & 1) != 0;
+        const int u = i / 2;
+        for (int t = 0; t < 4; ++t) {
+            int I = odd ? u - 1 + t : u - 2 + t;
+            if (I < 0) I = -1 - I;
+            if (I < 2) e.w[I][i] += odd ? kPc[t] : kQc[t];
+        }
+    }
+    return e;
+}
+constexpr EdgeRW kEdgeRW = make_edge_rw();
+
 }  // namespace mgbf

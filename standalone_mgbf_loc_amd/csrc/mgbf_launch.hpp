@@ -33,6 +33,11 @@ struct RadialArgs {
     const void* afrag;      // FP16 hi/lo lane fragments of the weight bands
     int wexp;               // power-of-two exponent applied to the weights
     int TJ;               // output rows per wavefront
+    // restriction to the next generation fused into the pass (TI = 4, H >= 4)
+    float* rout;            // coarse output, or nullptr
+    const float4* rytab;    // per fine row: weights into the 6 open coarse rows (2 x float4)
+    const int2* rymeta;     // per fine row: lowest open coarse row, rows completed
+    float* rkmax;           // per-k max |coarse output|, or nullptr
     bool exact_disc;
     RadialCoef coef;
 };
@@ -47,6 +52,11 @@ struct RadialParams {
     float wconst;
     int K, n, m, TJ, lm, nfilt;
     int ntile_i, ntile_j, nitems;
+    // restriction to the next generation fused into the pass (RES)
+    float* rout;              // coarse output (n/2 x m/2 x K), or nullptr
+    const float4* rytab;      // per fine row: weights into the 6 open coarse rows (2 x float4)
+    const int2* rymeta;       // per fine row: lowest open coarse row, rows completed
+    float* rkmax;             // per-k max |coarse output| accumulated here, or nullptr
     RadialCoef W;
 };
 

@@ -30,9 +30,19 @@ __host__ __device__ constexpr int disc_extent(int b) {
 // slides down with the output rows, one new coarse row every other row.
 // VHB applies a vertical band of half-width VHB across the lanes of the wave
 // (each wave then holds whole columns: lm divides 64 and K % 64 == 0).
+// RES restricts the input field to the next generation on the way (R = P^T
+// with the folded edges): every loaded fine row is restricted in x over the
+// wave's 2 coarse columns (interior columns 2I-3 .. 2I+4 with kW8, the 2
+// columns at each domain edge with kEdgeRW) and added into a shift register
+// of the coarse rows whose 8-row windows are open (4 in the interior, up to
+// 6 next to an edge), with weights from a per-fine-row table; a coarse row
+// is stored when its window closes.
+// The wave owns coarse rows j0/2 .. (j0+TJ)/2 - 1, whose windows lie inside
+// the rows it loads (checked on the host).
 // ---------------------------------------------------------------------------
-template <int H, int TI, bool DISC, bool PRO, int VHB>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES>
 __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
+    static_assert(!RES || (TI == 4 && H >= 4 && !PRO && VHB == 0), "restriction epilogue shape");
     constexpr int NR = 2 * H + 1;
     constexpr int NC = TI + 2 * H;
     constexpr int NCC = TI / 2 + 4;   // coarse columns of the tile (TI even)
@@ -117,6 +127,21 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
     const int nsteps = tj_eff + 2 * H;
 
+    // fused restriction state
+    constexpr int NRC = RES ? TI / 2 : 1;   // coarse columns of the wave
+    constexpr int NQ = 6;                    // open coarse rows at most
+    float racc[NQ][NRC];
+    float rmax = 0.f;
+    int xcls = 0;                            // 0 interior, 1 left edge, 2 right edge
+    const int Jlo = j0 / 2, Jhi = (j0 + tj_eff) / 2, nc2 = n / 2;
+    if constexpr (RES) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int c = 0; c < NRC; ++c) racc[q][c] = 0.f;
+        xcls = i0 == 0 ? 1 : (i0 + TI >= n ? 2 : 0);
+    }
+
     for (int rb = 0; rb < nsteps; rb += NR) {
 #pragma unroll
         for (int s = 0; s < NR; ++s) {
@@ -127,6 +152,110 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                 float v[NC];
 #pragma unroll
                 for (int c = 0; c < NC; ++c) v[c] = buf_ld(rs, vk, soff[c]);
+                if constexpr (RES) {
+                    const int vr = j0 - H + rr;   // fine row, before mirroring
+                    if (vr >= 0 && vr < m) {
+                        float xr[NRC];
+                        if (xcls == 0) {          // fine columns 2I-3 .. 2I+4
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[H - 3 + 2 * c + t], a);
+                                xr[c] = a;
+                            }
+                        } else if (xcls == 1) {   // fine columns 0 .. 7
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], v[H + t], a);
+                                xr[c] = a;
+                            }
+                        } else {                  // fine columns n-8 .. n-1
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t)
+                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], v[H - 4 + t], a);
+                                xr[c] = a;
+                            }
+                        }
+                        if (vr >= 8 && vr <= m - 9) {
+                            // interior fine row: coarse rows jb .. jb+3 are open
+                            // (windows 2J-3 .. 2J+4); an even row reads taps
+                            // 7,5,3,1 of their windows and closes row jb, an
+                            // odd row reads taps 6,4,2,0
+                            const bool odd = (vr & 1) != 0;
+                            const float yw[4] = {odd ? kW8[6] : kW8[7], odd ? kW8[4] : kW8[5],
+                                                 odd ? kW8[2] : kW8[3], odd ? kW8[0] : kW8[1]};
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c)
+                                    racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
+                            if (!odd) {
+                                const int J = (vr - 3) >> 1;
+                                if (J >= Jlo && J < Jhi && kvalid) {
+                                    const __amdgpu_buffer_rsrc_t ro =
+                                        buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
+#pragma unroll
+                                    for (int c = 0; c < NRC; ++c) {
+                                        if (i0 / 2 + c < nc2) {
+                                            buf_st(racc[0][c], ro, vk, c * K * 4);
+                                            rmax = fmaxf(rmax, fabsf(racc[0][c]));
+                                        }
+                                    }
+                                }
+                                // slots 4 and 5 stay empty between the edges
+#pragma unroll
+                                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                                    for (int c = 0; c < NRC; ++c) racc[q][c] = racc[q + 1][c];
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c) racc[3][c] = 0.f;
+                            }
+                        } else {
+                            // rows next to an edge: weights and closing rows
+                            // from the table
+                            const float4 ya = p.rytab[2 * vr], yb = p.rytab[2 * vr + 1];
+                            const float yw[NQ] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y};
+                            const int2 meta = p.rymeta[vr];
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c)
+                                    racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
+                            // coarse rows meta.x .. meta.x + meta.y - 1 close here
+#pragma unroll
+                            for (int e = 0; e < NQ; ++e) {
+                                const int J = meta.x + e;
+                                if (e < meta.y && J >= Jlo && J < Jhi && kvalid) {
+                                    const __amdgpu_buffer_rsrc_t ro =
+                                        buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
+#pragma unroll
+                                    for (int c = 0; c < NRC; ++c) {
+                                        if (i0 / 2 + c < nc2) {
+                                            buf_st(racc[e][c], ro, vk, c * K * 4);
+                                            rmax = fmaxf(rmax, fabsf(racc[e][c]));
+                                        }
+                                    }
+                                }
+                            }
+#pragma unroll
+                            for (int sft = 1; sft <= NQ; ++sft) {
+                                if (meta.y == sft) {
+#pragma unroll
+                                    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                                        for (int c = 0; c < NRC; ++c)
+                                            racc[q][c] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][c] : 0.f;
+                                }
+                            }
+                        }
+                    }
+                }
 #pragma unroll
                 for (int t = 0; t < TI; ++t) {
                     float P[H + 1];
@@ -215,9 +344,13 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
             }
         }
     }
+    if constexpr (RES) {
+        if (p.rkmax && kvalid)
+            atomicMax(reinterpret_cast<unsigned int*>(p.rkmax) + k, __float_as_uint(rmax));
+    }
 }
 
-template <int H, int TI, bool DISC, bool PRO, int VHB>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false>
 hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     RadialParams p{};
     p.in = a.in;
@@ -233,13 +366,17 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     p.lm = a.lm;
     p.nfilt = a.nfilt;
     p.W = a.coef;
+    p.rout = a.rout;
+    p.rytab = a.rytab;
+    p.rymeta = a.rymeta;
+    p.rkmax = a.rkmax;
     p.ntile_i = (a.n + TI - 1) / TI;
     p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB, RES>), dim3(nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
@@ -248,6 +385,15 @@ template <int H, int TI>
 hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     const int vhb = a.band ? a.vhb : 0;
+    if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 4
+        if constexpr (TI == 4 && H >= 4) {
+            if (pro || vhb) return hipErrorInvalidValue;
+            return a.exact_disc ? launch_radial_variant<H, 4, true, false, 0, true>(a, st)
+                                : launch_radial_variant<H, 4, false, false, 0, true>(a, st);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     if (a.exact_disc) {
         if (vhb == 0) return pro ? launch_radial_variant<H, TI, true, true, 0>(a, st)
                                  : launch_radial_variant<H, TI, true, false, 0>(a, st);

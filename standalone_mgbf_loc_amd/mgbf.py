@@ -345,7 +345,7 @@ class MGBF:
         out = {}
         ti, tj = ctypes.c_int(), ctypes.c_int()
         for g in range(1, self.dims.gm + 1):
-            for kind in range(3):
+            for kind in range(4):
                 _check(self._L.mgbf_get_tile(self._h, g, kind, ctypes.byref(ti), ctypes.byref(tj)))
                 out[(g, kind)] = (ti.value, tj.value)
         return out
