@@ -107,7 +107,9 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
             load_craw(cbase + u, cvn);
             xinterp(cvn, XV[u]);
         }
-        load_craw(cbase + 4, cvn);
+        // prefetch the next coarse row, except where the band epilogue
+        // already presses on the registers
+        if constexpr (VHB == 0) load_craw(cbase + 4, cvn);
     }
 
     // vertical band coefficients of this lane's level (identity past nfilt)
@@ -301,8 +303,13 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             for (int u = 0; u < 3; ++u)
 #pragma unroll
                                 for (int t = 0; t < TI; ++t) XV[u][t] = XV[u + 1][t];
-                            xinterp(cvn, XV[3]);
-                            load_craw(nb + 4, cvn);
+                            if constexpr (VHB == 0) {
+                                xinterp(cvn, XV[3]);
+                                load_craw(nb + 4, cvn);
+                            } else {
+                                load_craw(nb + 3, cvn);
+                                xinterp(cvn, XV[3]);
+                            }
                             cbase = nb;
                         }
                         const bool oj = (j & 1) != 0;
