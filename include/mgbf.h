@@ -76,6 +76,9 @@ typedef struct mgbf_dims {
     int ntaps1;      /* taps of the vertical 1-D filter */
     long long field_size; /* K*nx*ny */
     long long mg_size;    /* elements of a multigrid vector */
+    int na, ma;           /* global analysis grid: nm0 x mm0 */
+    int km_a_all;         /* stacked k extent on the analysis grid: n_ens*(km2 + km3*lm_a) */
+    long long anal_size;  /* km_a_all*na*ma (WORKA) */
 } mgbf_dims;
 
 typedef struct mgbf_ctx mgbf_ctx;
@@ -88,7 +91,13 @@ enum {
     MGBF_OP_A = 3,     /* v = A v         : multigrid -> multigrid */
     MGBF_OP_ASQRT = 4, /* v = A^{1/2} v   : multigrid -> multigrid */
     MGBF_OP_BSQRT = 5, /* y = W A^{1/2} v : multigrid vector -> field */
-    MGBF_OP_BSQRTT = 6 /* v = A^{1/2} W^T x : field -> multigrid vector */
+    MGBF_OP_BSQRTT = 6, /* v = A^{1/2} W^T x : field -> multigrid vector */
+    MGBF_OP_A2F = 7,   /* anal_to_filt_all2 (mg_transfer.f90:154-176): analysis field
+                          (km_a_all x na x ma) -> generation-1 field; the transpose of F2A */
+    MGBF_OP_F2A = 8    /* filt_to_anal_all2 (mg_transfer.f90:179-193): generation-1 field ->
+                          analysis field: 4-point horizontal interpolation (lsqr_direct_offset,
+                          mg_interpolate.f90:634-681) then lm -> lm_a = 2 lm - 1 levels
+                          (l_vertical_direct_spec2, :1006-1044); a copy when the grids agree */
 };
 
 /* single stages of mgbf_apply_stage (gen is 1-based) */

@@ -733,3 +733,177 @@ void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* con
         off += sz;
     }
 }
+
+/* ------------------------------------------------------------------------
+ * analysis <-> filter transfer
+ * ---------------------------------------------------------------------- */
+
+/* lsqr_mg_coef (mg_interpolate.f90:89-191) on one PE tile: analysis points at
+   xa(n) = dxa/2 + dxa (n-1), filter points at xf(i) = dxf/2 + dxf (i-1) over a
+   tile of length nloc_a (mg_parameter.f90:518-541); iref(n) = (first i in
+   1-ib .. im+ib-1 with xa(n) < xf(i)) - 2, ib = 6; the 4 weights blend the
+   quadratic interpolants through points iref..iref+2 and iref+1..iref+3
+   linearly between points iref+1 and iref+2. Returns -1 if no i is found. */
+static int anal_coef(int nloc_a, int nloc_f, int* iref, double* cx)
+{
+    const double len = (double)nloc_a;
+    const double dxa = len / nloc_a, dxf = len / nloc_f;
+    for (int n = 1; n <= nloc_a; ++n) {
+        const double x = 0.5 * dxa + dxa * (n - 1);
+        int i0 = 0, found = 0;
+        for (int i = 1 - 6; i <= nloc_f + 6 - 1; ++i)
+            if (x < 0.5 * dxf + dxf * (i - 1)) { i0 = i - 2; found = 1; break; }
+        if (!found) return -1;
+        double z[4];
+        for (int t = 0; t < 4; ++t) z[t] = 0.5 * dxf + dxf * (i0 + t - 1);
+        /* left quadratic through z0,z1,z2 and right one through z1,z2,z3 */
+        const double l0 = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
+        const double l1 = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
+        const double l2 = (x - z[0]) * (x - z[1]) / ((z[2] - z[0]) * (z[2] - z[1]));
+        const double r0 = (x - z[2]) * (x - z[3]) / ((z[1] - z[2]) * (z[1] - z[3]));
+        const double r1 = (x - z[1]) * (x - z[3]) / ((z[2] - z[1]) * (z[2] - z[3]));
+        const double r2 = (x - z[1]) * (x - z[2]) / ((z[3] - z[1]) * (z[3] - z[2]));
+        const double wl = (z[2] - x) / (z[2] - z[1]), wr = (x - z[1]) / (z[2] - z[1]);
+        iref[n - 1] = i0;
+        cx[4 * (n - 1) + 0] = wl * l0;
+        cx[4 * (n - 1) + 1] = wl * l1 + wr * r0;
+        cx[4 * (n - 1) + 2] = wl * l2 + wr * r1;
+        cx[4 * (n - 1) + 3] = wr * r2;
+    }
+    return 0;
+}
+
+#define XH 3   /* transfer halo (ibm = jbm = 3 in anal_to_filt / filt_to_anal) */
+
+void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double* vf, double* wa)
+{
+    const int K = P->K, nx = P->nx, ny = P->ny;
+    const int im = nx / A->npx, jm = ny / A->npy, nm = A->na / A->npx, mm = A->ma / A->npy;
+    const int lm = P->lm, la = A->lm_a, nv = P->nvcol;
+    const int K2 = K - nv * lm, Ka = nv * la + K2;
+    /* the reference's spec2 maps run over the nv columns only (km2 = 0) */
+    if (la != lm && K2 != 0) abort();
+    int* iref = (int*)malloc(sizeof(int) * nm);
+    int* jref = (int*)malloc(sizeof(int) * mm);
+    double* cx = (double*)malloc(sizeof(double) * 4 * nm);
+    double* cy = (double*)malloc(sizeof(double) * 4 * mm);
+    if (anal_coef(nm, im, iref, cx) || anal_coef(mm, jm, jref, cy)) abort();
+    /* boco_2d: the whole-domain field with a half-sample mirror halo of 3 */
+    const int px = nx + 2 * XH, py = ny + 2 * XH;
+    double* V = (double*)malloc(sizeof(double) * (size_t)K * px * py);
+    mgbfo_pad(K, nx, ny, XH, vf, V);
+    /* lsqr_direct_offset on every PE tile, at K levels of the filter */
+    double* W = (double*)malloc(sizeof(double) * (size_t)K * A->na * A->ma);
+    double* VX = (double*)malloc(sizeof(double) * (size_t)K * nm * (jm + 2 * XH));
+    for (int tj = 0; tj < A->npy; ++tj)
+        for (int ti = 0; ti < A->npx; ++ti) {
+            for (int j = 1 - XH; j <= jm + XH; ++j)
+                for (int n = 1; n <= nm; ++n) {
+                    double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
+                    for (int k = 0; k < K; ++k) vx[k] = 0.0;
+                    for (int t = 0; t < 4; ++t) {
+                        const int gi = ti * im + iref[n - 1] + t;   /* 1-based */
+                        const int gj = tj * jm + j;
+                        const double* v = V + (size_t)K * ((size_t)(gj - 1 + XH) * px + (gi - 1 + XH));
+                        for (int k = 0; k < K; ++k) vx[k] += cx[4 * (n - 1) + t] * v[k];
+                    }
+                }
+            for (int m = 1; m <= mm; ++m)
+                for (int n = 1; n <= nm; ++n) {
+                    const int gn = ti * nm + n, gm = tj * mm + m;
+                    double* w = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
+                    for (int k = 0; k < K; ++k) w[k] = 0.0;
+                    for (int t = 0; t < 4; ++t) {
+                        const int j = jref[m - 1] + t;
+                        const double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
+                        for (int k = 0; k < K; ++k) w[k] += cy[4 * (m - 1) + t] * vx[k];
+                    }
+                }
+        }
+    /* l_vertical_direct_spec2 over the nv columns (or a copy when lm_a = lm) */
+    const long long npts = (long long)A->na * A->ma;
+    for (long long q = 0; q < npts; ++q) {
+        const double* f = W + (size_t)K * q;
+        double* o = wa + (size_t)Ka * q;
+        if (la == lm) {
+            for (int k = 0; k < K; ++k) o[k] = f[k];
+            continue;
+        }
+        for (int e = 0; e < nv; ++e) {
+            int k = 1;
+            for (int n = 1; n <= la; n += 2, ++k) o[e * la + n - 1] = f[e * lm + k - 1];
+            k = 1;
+            for (int n = 2; n <= la - 1; n += 2, ++k)
+                o[e * la + n - 1] = 0.5 * (f[e * lm + k - 1] + f[e * lm + k]);
+        }
+        for (int k = 0; k < K2; ++k) o[nv * la + k] = f[nv * lm + k];
+    }
+    free(VX); free(W); free(V); free(cy); free(cx); free(jref); free(iref);
+}
+
+void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double* wa, double* vf)
+{
+    const int K = P->K, nx = P->nx, ny = P->ny;
+    const int im = nx / A->npx, jm = ny / A->npy, nm = A->na / A->npx, mm = A->ma / A->npy;
+    const int lm = P->lm, la = A->lm_a, nv = P->nvcol;
+    const int K2 = K - nv * lm, Ka = nv * la + K2;
+    /* the reference's spec2 maps run over the nv columns only (km2 = 0) */
+    if (la != lm && K2 != 0) abort();
+    int* iref = (int*)malloc(sizeof(int) * nm);
+    int* jref = (int*)malloc(sizeof(int) * mm);
+    double* cx = (double*)malloc(sizeof(double) * 4 * nm);
+    double* cy = (double*)malloc(sizeof(double) * 4 * mm);
+    if (anal_coef(nm, im, iref, cx) || anal_coef(mm, jm, jref, cy)) abort();
+    /* l_vertical_adjoint_spec2 (a copy when lm_a = lm) */
+    const long long npts = (long long)A->na * A->ma;
+    double* W = (double*)calloc((size_t)K * npts, sizeof(double));
+    for (long long q = 0; q < npts; ++q) {
+        const double* w = wa + (size_t)Ka * q;
+        double* f = W + (size_t)K * q;
+        if (la == lm) {
+            for (int k = 0; k < K; ++k) f[k] = w[k];
+            continue;
+        }
+        for (int e = 0; e < nv; ++e) {
+            int k = 1;
+            for (int n = 2; n <= la - 1; n += 2, ++k) {
+                f[e * lm + k - 1] += 0.5 * w[e * la + n - 1];
+                f[e * lm + k] += 0.5 * w[e * la + n - 1];
+            }
+            k = 1;
+            for (int n = 1; n <= la; n += 2, ++k) f[e * lm + k - 1] += w[e * la + n - 1];
+        }
+        for (int k = 0; k < K2; ++k) f[nv * lm + k] = w[nv * la + k];
+    }
+    /* lsqr_adjoint_offset on every PE tile into the padded field */
+    const int px = nx + 2 * XH, py = ny + 2 * XH;
+    double* V = (double*)calloc((size_t)K * px * py, sizeof(double));
+    double* VX = (double*)malloc(sizeof(double) * (size_t)K * nm * (jm + 2 * XH));
+    for (int tj = 0; tj < A->npy; ++tj)
+        for (int ti = 0; ti < A->npx; ++ti) {
+            for (size_t e = 0; e < (size_t)K * nm * (jm + 2 * XH); ++e) VX[e] = 0.0;
+            for (int m = 1; m <= mm; ++m)
+                for (int n = 1; n <= nm; ++n) {
+                    const int gn = ti * nm + n, gm = tj * mm + m;
+                    const double* w = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
+                    for (int t = 0; t < 4; ++t) {
+                        const int j = jref[m - 1] + t;
+                        double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
+                        for (int k = 0; k < K; ++k) vx[k] += w[k] * cy[4 * (m - 1) + t];
+                    }
+                }
+            for (int n = 1; n <= nm; ++n)
+                for (int j = 1 - XH; j <= jm + XH; ++j) {
+                    const double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
+                    for (int t = 0; t < 4; ++t) {
+                        const int gi = ti * im + iref[n - 1] + t;
+                        const int gj = tj * jm + j;
+                        double* v = V + (size_t)K * ((size_t)(gj - 1 + XH) * px + (gi - 1 + XH));
+                        for (int k = 0; k < K; ++k) v[k] += vx[k] * cx[4 * (n - 1) + t];
+                    }
+                }
+        }
+    /* bocoT_2d: fold the halo back with the half-sample mirror */
+    mgbfo_fold(K, nx, ny, XH, V, vf);
+    free(VX); free(V); free(W); free(cy); free(cx); free(jref); free(iref);
+}

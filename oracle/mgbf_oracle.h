@@ -79,6 +79,24 @@ void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* con
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int mgbfo_num_threads(void);
 
+/* ---- analysis <-> filter transfer (mg_transfer.f90:154-203) ---- */
+/* The analysis grid: na x ma global points (nm0 x mm0) split over an
+   npx x npy PE grid like the filter grid (nx = npx*im, ny = npy*jm), and
+   lm_a levels per column (lm_a = lm or 2 lm - 1). */
+typedef struct mgbfo_anal {
+    int na, ma;
+    int npx, npy;
+    int lm_a;
+} mgbfo_anal;
+/* filt_to_anal_all2: boco_2d(.,3,3) halo, lsqr_direct_offset per PE tile
+   (mg_interpolate.f90:634-681, coefficients of lsqr_mg_coef :89-191), then
+   l_vertical_direct_spec2 (:1006-1044). vf: K x nx x ny; wa: Ka x na x ma
+   with Ka = nvcol*lm_a + (K - nvcol*lm). */
+void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double* vf, double* wa);
+/* anal_to_filt_all2: l_vertical_adjoint_spec2 (:963-1003), lsqr_adjoint_offset
+   per PE tile (:684-747) into the halo-padded field, then bocoT_2d(.,3,3). */
+void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double* wa, double* vf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,8 +62,40 @@ def lib():
         L.mgbfo_w.argtypes = [P, dp, dp]
         L.mgbfo_weight.argtypes = [P, dp, ctypes.c_void_p, dp]
         L.mgbfo_num_threads.restype = ctypes.c_int
+        L.mgbfo_filt_to_anal.argtypes = [P, ctypes.POINTER(_Anal), dp, dp]
+        L.mgbfo_anal_to_filt.argtypes = [P, ctypes.POINTER(_Anal), dp, dp]
         _lib = L
     return _lib
+
+
+class _Anal(ctypes.Structure):
+    _fields_ = [("na", ctypes.c_int), ("ma", ctypes.c_int), ("npx", ctypes.c_int),
+                ("npy", ctypes.c_int), ("lm_a", ctypes.c_int)]
+
+
+def _anal_ka(P, lm_a: int) -> int:
+    return P.nvcol * lm_a + (P.K - P.nvcol * P.lm)
+
+
+def filt_to_anal(P, na: int, ma: int, npx: int, npy: int, lm_a: int, vf: np.ndarray) -> np.ndarray:
+    """filt_to_anal_all2: generation-1 field (ny, nx, K) -> analysis field
+    (ma, na, Ka) over an npx x npy PE grid."""
+    vf = _f64(vf)
+    assert vf.shape == P.gen_shape(0)
+    A = _Anal(na, ma, npx, npy, lm_a)
+    wa = np.empty((ma, na, _anal_ka(P, lm_a)))
+    lib().mgbfo_filt_to_anal(ctypes.byref(P.c()), ctypes.byref(A), _dp(vf), _dp(wa))
+    return wa
+
+
+def anal_to_filt(P, na: int, ma: int, npx: int, npy: int, lm_a: int, wa: np.ndarray) -> np.ndarray:
+    """anal_to_filt_all2: the transpose of filt_to_anal."""
+    wa = _f64(wa)
+    assert wa.shape == (ma, na, _anal_ka(P, lm_a))
+    A = _Anal(na, ma, npx, npy, lm_a)
+    vf = np.empty(P.gen_shape(0))
+    lib().mgbfo_anal_to_filt(ctypes.byref(P.c()), ctypes.byref(A), _dp(wa), _dp(vf))
+    return vf
 
 
 def _dp(a: np.ndarray):

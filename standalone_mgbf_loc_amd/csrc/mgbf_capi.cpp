@@ -104,6 +104,16 @@ struct mgbf_ctx {
     const float* d_pyw[mgbf::kMaxGen] = {};
     const float* wfield[mgbf::kMaxGen] = {};
     float wconst[mgbf::kMaxGen] = {};
+    // analysis <-> filter transfer: separable window tables (direct: filter ->
+    // analysis, adjoint: its transpose) and the vertical k windows
+    mgbf::XferArgs xf_dir{}, xf_adj{};
+    const int* d_kvd_s = nullptr;
+    const float* d_kvd_w = nullptr;
+    int kvd_w = 0;
+    const int* d_kva_s = nullptr;
+    const float* d_kva_w = nullptr;
+    int kva_w = 0;
+    float* ws_a = nullptr;  // analysis grid x K, when the vertical transfer is not the identity
     float* ws_u = nullptr;  // multigrid-sized
     float* ws_t = nullptr;  // multigrid-sized
     double* d_dot = nullptr;
@@ -493,6 +503,49 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
     return upsweep(c, v, v, v, y, c->nfilt ? BAND_VF : -1, sqrt_weights, st);
 }
 
+// filt_to_anal_all2: analysis field = T_v T_h (generation-1 field), the
+// horizontal 4-point interpolation first, then lm -> lm_a levels.
+int apply_f2a(mgbf_ctx* c, const float* in, float* out, hipStream_t st) {
+    const mgbf_dims& d = c->s.dims;
+    if (c->s.anal_identity) {
+        if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
+        return MGBF_OK;
+    }
+    if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
+    float* mid = c->d_kvd_s ? c->ws_a : out;
+    mgbf::XferArgs a = c->xf_dir;
+    a.in = in;
+    a.out = mid;
+    HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    if (c->d_kvd_s)
+        HIP_TRY(mgbf::launch_kwin(mid, out, c->K, d.km_a_all, (long long)d.na * d.ma, c->d_kvd_s,
+                                  c->d_kvd_w, c->kvd_w, st),
+                "vertical transfer launch");
+    return MGBF_OK;
+}
+
+// anal_to_filt_all2: the transpose of filt_to_anal_all2.
+int apply_a2f(mgbf_ctx* c, const float* in, float* out, hipStream_t st) {
+    const mgbf_dims& d = c->s.dims;
+    if (c->s.anal_identity) {
+        if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
+        return MGBF_OK;
+    }
+    if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
+    const float* src = in;
+    if (c->d_kva_s) {
+        HIP_TRY(mgbf::launch_kwin(in, c->ws_a, d.km_a_all, c->K, (long long)d.na * d.ma, c->d_kva_s,
+                                  c->d_kva_w, c->kva_w, st),
+                "vertical transfer launch");
+        src = c->ws_a;
+    }
+    mgbf::XferArgs a = c->xf_adj;
+    a.in = src;
+    a.out = out;
+    HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    return MGBF_OK;
+}
+
 int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStream_t st) {
     for (int g = 0; g < c->gm; ++g) {
         if (g == 0 && !c->filt_g1) {
@@ -689,6 +742,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->res_ok[g] = build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], rytab[g], rymeta[g]);
         bytes += carve_size(sizeof(float) * rytab[g].size()) + carve_size(sizeof(int) * rymeta[g].size());
     }
+    {
+        const mgbf::AxisTable* tabs[6] = {&s.ax_d, &s.ay_d, &s.ax_a, &s.ay_a, &s.kv_d, &s.kv_a};
+        for (const auto* t : tabs)
+            bytes += carve_size(sizeof(int) * t->start.size()) + carve_size(sizeof(float) * t->w.size());
+    }
     std::vector<char> host(bytes, 0);
     hipError_t e = hipMalloc(&c->d_tables, bytes);
     if (e != hipSuccess) {
@@ -737,10 +795,29 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
             c->d_rymeta[g] = reinterpret_cast<const int2*>(di);
         }
     }
+    if (!s.anal_identity) {
+        const mgbf_dims& d = s.dims;
+        c->xf_dir = mgbf::XferArgs{nullptr, nullptr, c->K, d.nx, d.ny, d.na, d.ma,
+                                   put_i(s.ax_d.start), put_f(s.ax_d.w), s.ax_d.width,
+                                   put_i(s.ay_d.start), put_f(s.ay_d.w), s.ay_d.width};
+        c->xf_adj = mgbf::XferArgs{nullptr, nullptr, c->K, d.na, d.ma, d.nx, d.ny,
+                                   put_i(s.ax_a.start), put_f(s.ax_a.w), s.ax_a.width,
+                                   put_i(s.ay_a.start), put_f(s.ay_a.w), s.ay_a.width};
+        if (s.kv_d.n_out > 0) {
+            c->d_kvd_s = put_i(s.kv_d.start);
+            c->d_kvd_w = put_f(s.kv_d.w);
+            c->kvd_w = s.kv_d.width;
+            c->d_kva_s = put_i(s.kv_a.start);
+            c->d_kva_w = put_f(s.kv_a.w);
+            c->kva_w = s.kv_a.width;
+        }
+    }
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->ws_t, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->d_kmax, sizeof(float) * (size_t)c->gm * c->K);
+    if (e == hipSuccess && c->d_kvd_s)
+        e = hipMalloc(&c->ws_a, sizeof(float) * (size_t)c->K * s.dims.na * s.dims.ma);
     if (e != hipSuccess) {
         int r = hip_fail(e, "device allocation");
         mgbf_finalize(c);
@@ -760,6 +837,7 @@ int mgbf_finalize(mgbf_ctx* c) {
     if (c->ws_t) (void)hipFree(c->ws_t);
     if (c->d_tables) (void)hipFree(c->d_tables);
     if (c->d_kmax) (void)hipFree(c->d_kmax);
+    if (c->ws_a) (void)hipFree(c->ws_a);
     delete c;
     return MGBF_OK;
 }
@@ -790,6 +868,8 @@ int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
         case MGBF_OP_ASQRT: return apply_weights(c, in, out, true, st);
         case MGBF_OP_BSQRT: return apply_w(c, in, out, true, st);
         case MGBF_OP_BSQRTT: return apply_wt(c, in, out, true, st);
+        case MGBF_OP_A2F: return apply_a2f(c, in, out, st);
+        case MGBF_OP_F2A: return apply_f2a(c, in, out, st);
         default: return fail(MGBF_ERR_ARG, "unknown operation");
     }
 }

@@ -116,7 +116,120 @@ bool pack_rows(int rows, int cols, const std::vector<double>& M, const std::vect
     return true;
 }
 
+// Widest structural row (or column, transposed) span of a banded matrix.
+int max_span(int rows, int cols, const std::vector<char>& nz, bool transpose) {
+    int wmax = 0;
+    for (int r = 0; r < rows; ++r) {
+        int lo = -1, hi = -1;
+        for (int c = 0; c < cols; ++c) {
+            const size_t e = transpose ? (size_t)c * rows + r : (size_t)r * cols + c;
+            if (nz[e]) {
+                if (lo < 0) lo = c;
+                hi = c;
+            }
+        }
+        if (lo >= 0 && hi - lo + 1 > wmax) wmax = hi - lo + 1;
+    }
+    return wmax;
+}
+
+// Vertical transfer over the stacked k (l_vertical_direct_spec2 /
+// l_vertical_adjoint_spec2, mg_interpolate.f90:963-1044): ncol columns of
+// lm filter levels and lm_a = 2 lm - 1 analysis levels; analysis level n
+// (1-based) is filter level (n+1)/2 for odd n and the mean of levels n/2 and
+// n/2+1 for even n. Tables of the direct map (rows: analysis k) and of its
+// transpose (rows: filter k), windows kept inside each column.
+void vertical_transfer_tables(int ncol, int lm, AxisTable& dir, AxisTable& adj) {
+    const int la = 2 * lm - 1;
+    std::vector<double> V((size_t)la * lm, 0.0);   // one column, la x lm
+    std::vector<char> nz((size_t)la * lm, 0);
+    for (int n = 1; n <= la; ++n) {
+        if (n & 1) {
+            V[(size_t)(n - 1) * lm + (n + 1) / 2 - 1] = 1.0;
+            nz[(size_t)(n - 1) * lm + (n + 1) / 2 - 1] = 1;
+        } else {
+            for (int l = n / 2; l <= n / 2 + 1; ++l) {
+                V[(size_t)(n - 1) * lm + l - 1] = 0.5;
+                nz[(size_t)(n - 1) * lm + l - 1] = 1;
+            }
+        }
+    }
+    AxisTable d1, a1;
+    pack_rows(la, lm, V, nz, false, 2, d1);
+    pack_rows(lm, la, V, nz, true, 3, a1);
+    auto tile = [&](const AxisTable& one, int nrow, int ncolk, AxisTable& t) {
+        t.n_out = ncol * nrow;
+        t.width = one.width;
+        t.start.resize(t.n_out);
+        t.w.resize((size_t)t.n_out * t.width);
+        for (int c = 0; c < ncol; ++c)
+            for (int r = 0; r < nrow; ++r) {
+                t.start[(size_t)c * nrow + r] = c * ncolk + one.start[r];
+                for (int u = 0; u < t.width; ++u)
+                    t.w[((size_t)c * nrow + r) * t.width + u] = one.w[(size_t)r * t.width + u];
+            }
+    };
+    tile(d1, la, lm, dir);
+    tile(a1, lm, la, adj);
+}
+
 }  // namespace
+
+bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, std::vector<double>& T) {
+    // Per PE tile the reference places the analysis points at cell centres
+    // xa(n) = dxa/2 + dxa (n-1) and the filter points at xf(i) = dxf/2 +
+    // dxf (i-1), with the tile length equal to nloc_a (mg_parameter.f90:
+    // 518-541), so dxa = 1 and dxf = nloc_a/nloc_f. Analysis point n takes
+    // the filter points iref..iref+3, iref = (first i with xa(n) < xf(i)) - 2
+    // (searched over 1-ib .. im+ib-1, ib = 6), with the 4-point weights of
+    // lsqr_mg_coef (mg_interpolate.f90:156-191): the quadratic Lagrange
+    // interpolants through (x1,x2,x3) and (x2,x3,x4), blended linearly
+    // between x2 and x3. Filter indices outside the global domain take the
+    // half-sample mirror (boco_2d); inside, tiles join up seamlessly.
+    const int na = npe * nloc_a, nf = npe * nloc_f;
+    T.assign((size_t)na * nf, 0.0);
+    const double len = 1.0 * nloc_a;
+    const double dxa = len / nloc_a, dxf = len / nloc_f;
+    const double xa0 = 0.5 * dxa, xf0 = 0.5 * dxf;
+    const int ib = 6;
+    for (int n = 1; n <= nloc_a; ++n) {
+        const double x = xa0 + dxa * (n - 1);
+        int iref = 0;
+        bool found = false;
+        for (int i = 1 - ib; i <= nloc_f + ib - 1; ++i)
+            if (x < xf0 + dxf * (i - 1)) {
+                iref = i - 2;
+                found = true;
+                break;
+            }
+        if (!found) return false;
+        double xs[4];
+        for (int t = 0; t < 4; ++t) xs[t] = xf0 + dxf * (iref + t - 1);
+        // quadratic Lagrange basis on 3 nodes at x
+        auto lag3 = [&](const double* z, double* l) {
+            l[0] = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
+            l[1] = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
+            l[2] = (x - z[0]) * (x - z[1]) / ((z[2] - z[0]) * (z[2] - z[1]));
+        };
+        double ql[3], qr[3];
+        lag3(xs, ql);
+        lag3(xs + 1, qr);
+        const double bl = (xs[2] - x) / (xs[2] - xs[1]);   // weight of the left interpolant
+        const double br = (x - xs[1]) / (xs[2] - xs[1]);
+        const double c[4] = {bl * ql[0], bl * ql[1] + br * qr[0], bl * ql[2] + br * qr[1], br * qr[2]};
+        for (int p = 0; p < npe; ++p) {
+            const int row = p * nloc_a + n - 1;
+            for (int t = 0; t < 4; ++t) {
+                int I = p * nloc_f + iref + t;   // 1-based global filter index
+                if (I < 1) I = 1 - I;
+                if (I > nf) I = 2 * nf + 1 - I;
+                if (I < 1 || I > nf) return false;
+                T[(size_t)row * nf + (I - 1)] += c[t];
+            }
+        }
+    }
+    return true;
+}
 
 bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     taps.clear();
@@ -234,10 +347,26 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "mm0 is not divisible by nym";
         return MGBF_ERR_CONFIG;
     }
-    if (nm != c.im_filt || mm != c.jm_filt || c.lm_a != c.lm) {
-        *err = "analysis grid differs from the filter grid (nm0/nxPE != im_filt, mm0/nyPE != "
-               "jm_filt or lm_a != lm): the analysis-filter interpolation is not built";
+    // analysis <-> filter transfer (mg_transfer.f90:154-203, lsqr_mg_coef
+    // mg_interpolate.f90:89-298, l_vertical_*_spec2 :963-1044)
+    if (nm < c.im_filt || mm < c.jm_filt) {
+        *err = "filter grid finer than the analysis grid (im_filt > nm0/nxPE or jm_filt > mm0/nyPE): "
+               "the transfer assumes im <= nm and jm <= mm";
         return MGBF_ERR_UNSUPPORTED;
+    }
+    if ((nm != c.im_filt || mm != c.jm_filt) && (c.l_lin_horizontal || c.l_quad_horizontal)) {
+        *err = "l_lin_horizontal / l_quad_horizontal: only the 4-point (lsqr) horizontal transfer is built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    if (c.lm_a != c.lm && c.lm_a != 2 * c.lm - 1) {
+        *err = fmt("lm_a=%d: the vertical transfer maps lm_a = 2 lm - 1 = %d levels (or lm_a = lm)",
+                   c.lm_a, 2 * c.lm - 1);
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    if (c.lm_a != c.lm && (c.km2 > 0 || c.lm < 2)) {
+        // l_vertical_*_spec2 run over km3*n_ens columns of lm_a levels only
+        *err = "lm_a > lm needs km2 = 0 and lm >= 2 (the vertical transfer covers the 3-D variables)";
+        return MGBF_ERR_CONFIG;
     }
     // generations (mg_parameter.f90:322-328)
     int gm = def_maxgen(c.nxPE, c.nyPE);
@@ -274,6 +403,10 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     }
     d.field_size = fs;
     d.mg_size = ms;
+    d.na = c.nm0;
+    d.ma = c.mm0;
+    d.km_a_all = (c.mgbf_proc == 1 ? 1 : c.n_ens) * (c.km2 + c.km3 * c.lm_a);
+    d.anal_size = (long long)d.km_a_all * d.na * d.ma;
     if (c.l_vertical_filter && c.lm <= c.hz) {
         *err = "lm must exceed hz for the vertical mirror";
         return MGBF_ERR_CONFIG;
@@ -380,6 +513,32 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     // generation weights (mg_intstate.f90:333-351): g >= 5 share mg_weig5
     const double wv[5] = {cfg.mg_weig1, cfg.mg_weig2, cfg.mg_weig3, cfg.mg_weig4, cfg.mg_weig5};
     for (int g = 0; g < s.dims.gm; ++g) s.wconst[g] = wv[g < 4 ? g : 4];
+    // analysis <-> filter transfer tables
+    const int nm = cfg.nm0 / cfg.nxPE, mm = cfg.mm0 / cfg.nyPE;
+    s.anal_identity = nm == cfg.im_filt && mm == cfg.jm_filt && cfg.lm_a == cfg.lm;
+    if (!s.anal_identity) {
+        const int axes[2][3] = {{cfg.nxPE, nm, cfg.im_filt}, {cfg.nyPE, mm, cfg.jm_filt}};
+        AxisTable* dirs[2] = {&s.ax_d, &s.ay_d};
+        AxisTable* adjs[2] = {&s.ax_a, &s.ay_a};
+        for (int a = 0; a < 2; ++a) {
+            const int na = axes[a][0] * axes[a][1], nf = axes[a][0] * axes[a][2];
+            std::vector<double> T;
+            if (!anal_axis_matrix(axes[a][0], axes[a][1], axes[a][2], T)) {
+                *err = "analysis-filter interpolation: stencil outside the 3-point halo";
+                return MGBF_ERR_CONFIG;
+            }
+            std::vector<char> nz(T.size());
+            for (size_t e = 0; e < T.size(); ++e) nz[e] = T[e] != 0.0;
+            const int wd = max_span(na, nf, nz, false), wa = max_span(nf, na, nz, true);
+            if (wd > kMaxXferWidth || wa > kMaxXferWidth || !pack_rows(na, nf, T, nz, false, wd, *dirs[a]) ||
+                !pack_rows(nf, na, T, nz, true, wa, *adjs[a])) {
+                *err = fmt("analysis-filter interpolation wider than %d points", kMaxXferWidth);
+                return MGBF_ERR_UNSUPPORTED;
+            }
+        }
+        if (cfg.lm_a != cfg.lm)
+            vertical_transfer_tables(s.dims.km3_all, cfg.lm, s.kv_d, s.kv_a);
+    }
     return MGBF_OK;
 }
 

@@ -13,6 +13,7 @@ constexpr int kMaxGen = 12;
 constexpr int kMaxHalo = 10;      // largest radial half-width with a built kernel
 constexpr int kRWidth = 8;        // restriction window per coarse index
 constexpr int kPWidth = 4;        // prolongation window per fine index
+constexpr int kMaxXferWidth = 16; // analysis-filter interpolation window
 
 // One tap of a beta filter (offsets from the output point, FP64 weight).
 struct Tap2 {
@@ -44,7 +45,16 @@ struct Setup {
     AxisTable rx[kMaxGen], ry[kMaxGen];   // restriction g -> g+1 (coarse rows)
     AxisTable px[kMaxGen], py[kMaxGen];   // prolongation g+1 -> g (fine rows)
     double wconst[kMaxGen] = {};
+    // analysis <-> filter transfer (anal_to_filt_all2 / filt_to_anal_all2):
+    // direct = filter -> analysis, adjoint = its transpose
+    bool anal_identity = true;         // nm = im, mm = jm and lm_a = lm
+    AxisTable ax_d, ax_a, ay_d, ay_a;  // horizontal, per axis
+    AxisTable kv_d, kv_a;              // vertical over the stacked k (lm_a = 2 lm - 1)
 };
+
+// Dense 1-D analysis <- filter interpolation matrix of one axis over the
+// whole domain (na = npe*nloc_a rows, nf = npe*nloc_f columns, row-major).
+bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, std::vector<double>& T);
 
 // init_mg_parameter-style validation and derived sizes. Returns MGBF_OK or
 // an error status with a message in *err.

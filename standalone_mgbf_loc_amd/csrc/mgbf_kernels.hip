@@ -471,6 +471,61 @@ hipError_t launch_copy(const float* in, float* out, long long count, hipStream_t
     return hipGetLastError();
 }
 
+// One wave per output point and 64 consecutive k: the window weights are
+// wave-uniform (scalar loads), every tap a 256-byte coalesced read.
+__global__ void __launch_bounds__(kBlock) xfer2d_kernel(XferArgs a, long long nitems) {
+    const long long item = (long long)blockIdx.x * kWavesPerBlock +
+                           __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nkc = (a.K + kWave - 1) / kWave;
+    const int kc = (int)(item % nkc);
+    const long long pt = item / nkc;
+    const int i = (int)(pt % a.nout_x), j = (int)(pt / a.nout_x);
+    const int k = kc * kWave + lane;
+    if (k >= a.K) return;
+    const int x0 = a.xs[i], y0 = a.ys[j];
+    float acc = 0.f;
+    for (int ty = 0; ty < a.wy; ++ty) {
+        const float wyv = a.yw[(size_t)j * a.wy + ty];
+        const float* row = a.in + k + (size_t)a.K * ((size_t)a.nin_x * (y0 + ty) + x0);
+        float r = 0.f;
+        for (int tx = 0; tx < a.wx; ++tx) r = __builtin_fmaf(a.xw[(size_t)i * a.wx + tx], row[(size_t)tx * a.K], r);
+        acc = __builtin_fmaf(wyv, r, acc);
+    }
+    a.out[k + (size_t)a.K * ((size_t)a.nout_x * j + i)] = acc;
+}
+
+__global__ void __launch_bounds__(kBlock)
+kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int kout, long long count,
+            const int* __restrict__ ks, const float* __restrict__ kw, int w) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += stride) {
+        const int ko = (int)(e % kout);
+        const long long p = e / kout;
+        const float* col = in + p * kin + ks[ko];
+        float acc = 0.f;
+        for (int u = 0; u < w; ++u) acc = __builtin_fmaf(kw[(size_t)ko * w + u], col[u], acc);
+        out[e] = acc;
+    }
+}
+
+hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st) {
+    const long long nkc = (a.K + kWave - 1) / kWave;
+    const long long nitems = nkc * a.nout_x * a.nout_y;
+    const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(xfer2d_kernel, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,
+                       const float* kw, int w, hipStream_t st) {
+    const long long count = (long long)kout * npts;
+    hipLaunchKernelGGL(kwin_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, in, out, kin, kout,
+                       count, ks, kw, w);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(float* p, float v, long long count, hipStream_t st) {
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();

@@ -116,6 +116,26 @@ hipError_t launch_scale(const float* in, float* out, const float* wfield, float 
                         int use_sqrt, long long count, hipStream_t st);
 hipError_t launch_copy(const float* in, float* out, long long count, hipStream_t st);
 hipError_t launch_fill(float* p, float v, long long count, hipStream_t st);
+
+// Separable window transfer between two grids of K-long columns:
+// out(k,i,j) = sum_ty yw[j][ty] sum_tx xw[i][tx] in(k, xs[i]+tx, ys[j]+ty)
+struct XferArgs {
+    const float* in;
+    float* out;
+    int K;
+    int nin_x, nin_y, nout_x, nout_y;
+    const int* xs;
+    const float* xw;
+    int wx;
+    const int* ys;
+    const float* yw;
+    int wy;
+};
+hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st);
+// Window transfer along the stacked k of every point:
+// out(ko,p) = sum_u kw[ko][u] in(ks[ko]+u, p)
+hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,
+                       const float* kw, int w, hipStream_t st);
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st);
 

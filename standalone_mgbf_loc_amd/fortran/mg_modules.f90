@@ -99,8 +99,9 @@ implicit none
 real(r_kind), allocatable, target :: WORKA(:,:,:)
 type(c_ptr) :: mgbf_ctx = c_null_ptr      ! libmgbf context
 type(c_ptr) :: d_vall = c_null_ptr        ! device generation-1 field (FP32)
-real(c_float), allocatable, target :: stage4(:,:,:)   ! FP32 host staging
-integer(c_size_t) :: vall_bytes = 0
+type(c_ptr) :: d_worka = c_null_ptr       ! device analysis field (FP32)
+real(c_float), allocatable, target :: stage4(:,:,:)   ! FP32 host staging of WORKA
+integer(c_size_t) :: vall_bytes = 0, worka_bytes = 0
 end module mg_intstate
 
 !=======================================================================
@@ -151,10 +152,12 @@ subroutine mg_initialize(nml_file, device)
   call mgbf_check(mgbf_get_dims(mgbf_ctx, dims), 'mg_initialize')
   gm = dims%gm
   allocate(WORKA(km_a_all, nm0, mm0))
-  allocate(stage4(km_all, im00, jm00))
+  allocate(stage4(km_a_all, nm0, mm0))
   WORKA = 0.
   vall_bytes = int(dims%field_size, c_size_t)*4_c_size_t
+  worka_bytes = int(dims%anal_size, c_size_t)*4_c_size_t
   if (hipMalloc(d_vall, vall_bytes) /= 0) stop 'mg_initialize: hipMalloc failed'
+  if (hipMalloc(d_worka, worka_bytes) /= 0) stop 'mg_initialize: hipMalloc failed'
   if (ldelta) then
     ! unit impulse of input_spec1_2d(..,'md') on the PE (nxPE/2, nyPE/2)
     ! into stack level 3*lm_a+lm_a/2 (src.mgbf/mg_entrymod.f90:198-213)
@@ -170,6 +173,8 @@ subroutine mg_finalize
   integer(c_int) :: st
   if (c_associated(d_vall)) st = hipFree(d_vall)
   d_vall = c_null_ptr
+  if (c_associated(d_worka)) st = hipFree(d_worka)
+  d_worka = c_null_ptr
   if (c_associated(mgbf_ctx)) st = mgbf_finalize(mgbf_ctx)
   mgbf_ctx = c_null_ptr
   if (allocated(WORKA)) deallocate(WORKA)
@@ -179,9 +184,11 @@ end subroutine mg_finalize
 end module mg_entrymod
 
 !=======================================================================
-! Analysis <-> filter grid (src.mgbf/mg_transfer.f90:154-203). The
-! configurations built here have nm = im and lm_a = lm, where the
-! reference's transfer is the identity; it is an FP32 upload/download.
+! Analysis <-> filter grid (src.mgbf/mg_transfer.f90:154-203): WORKA goes
+! to the device through an FP32 staging array, and the library maps it to
+! the generation-1 field (anal_to_filt_all2: the transpose of the 4-point
+! horizontal and 2:1 vertical interpolation) and back (filt_to_anal_all2).
+! With nm = im, mm = jm and lm_a = lm both maps are copies.
 !=======================================================================
 module mg_transfer
 use iso_c_binding
@@ -196,12 +203,14 @@ contains
 
 subroutine anal_to_filt_all2
   stage4 = real(WORKA, c_float)
-  if (hipMemcpy(d_vall, c_loc(stage4), vall_bytes, hipMemcpyHostToDevice) /= 0) &
+  if (hipMemcpy(d_worka, c_loc(stage4), worka_bytes, hipMemcpyHostToDevice) /= 0) &
     stop 'anal_to_filt: hipMemcpy failed'
+  call mgbf_check(mgbf_apply(mgbf_ctx, MGBF_OP_A2F, d_worka, d_vall, c_null_ptr), 'anal_to_filt')
 end subroutine anal_to_filt_all2
 
 subroutine filt_to_anal_all2
-  if (hipMemcpy(c_loc(stage4), d_vall, vall_bytes, hipMemcpyDeviceToHost) /= 0) &
+  call mgbf_check(mgbf_apply(mgbf_ctx, MGBF_OP_F2A, d_vall, d_worka, c_null_ptr), 'filt_to_anal')
+  if (hipMemcpy(c_loc(stage4), d_worka, worka_bytes, hipMemcpyDeviceToHost) /= 0) &
     stop 'filt_to_anal: hipMemcpy failed'
   WORKA = real(stage4, r_kind)
 end subroutine filt_to_anal_all2

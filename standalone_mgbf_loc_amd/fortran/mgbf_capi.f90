@@ -9,7 +9,8 @@ implicit none
 integer(c_int), parameter :: MGBF_OK = 0
 integer(c_int), parameter :: MGBF_OP_WAWT = 0, MGBF_OP_WT = 1, MGBF_OP_W = 2, &
                              MGBF_OP_A = 3, MGBF_OP_ASQRT = 4,              &
-                             MGBF_OP_BSQRT = 5, MGBF_OP_BSQRTT = 6
+                             MGBF_OP_BSQRT = 5, MGBF_OP_BSQRTT = 6,         &
+                             MGBF_OP_A2F = 7, MGBF_OP_F2A = 8
 integer(c_int), parameter :: MGBF_NTIMERS = 6
 integer(c_int), parameter :: hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2
 
@@ -32,6 +33,8 @@ end type mgbf_config_t
 type, bind(C) :: mgbf_dims_t
   integer(c_int) :: gm, nx, ny, km, km_all, km3_all, ntaps2, ntaps1
   integer(c_long_long) :: field_size, mg_size
+  integer(c_int) :: na, ma, km_a_all
+  integer(c_long_long) :: anal_size
 end type mgbf_dims_t
 
 interface

@@ -23,7 +23,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgbf.so")
 
 MGBF_OK = 0
-OP_WAWT, OP_WT, OP_W, OP_A, OP_ASQRT, OP_BSQRT, OP_BSQRTT = range(7)
+OP_WAWT, OP_WT, OP_W, OP_A, OP_ASQRT, OP_BSQRT, OP_BSQRTT, OP_A2F, OP_F2A = range(9)
 STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_R, STAGE_P, STAGE_A = range(7)
 
 EXPORTED_SYMBOLS = (
@@ -57,7 +57,9 @@ class _CDims(ctypes.Structure):
     _fields_ = [("gm", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
                 ("km", ctypes.c_int), ("km_all", ctypes.c_int), ("km3_all", ctypes.c_int),
                 ("ntaps2", ctypes.c_int), ("ntaps1", ctypes.c_int),
-                ("field_size", ctypes.c_longlong), ("mg_size", ctypes.c_longlong)]
+                ("field_size", ctypes.c_longlong), ("mg_size", ctypes.c_longlong),
+                ("na", ctypes.c_int), ("ma", ctypes.c_int), ("km_a_all", ctypes.c_int),
+                ("anal_size", ctypes.c_longlong)]
 
 
 @dataclasses.dataclass
@@ -213,10 +215,18 @@ class Dims:
     ntaps1: int
     field_size: int
     mg_size: int
+    na: int
+    ma: int
+    km_a_all: int
+    anal_size: int
 
     def gen_shape(self, g: int):
         """(ny_g, nx_g, K) of generation g (1-based)."""
         return (self.ny >> (g - 1), self.nx >> (g - 1), self.km_all)
+
+    def anal_shape(self):
+        """(mm0, nm0, km_a_all): the analysis field WORKA in C order."""
+        return (self.ma, self.na, self.km_a_all)
 
 
 def derive(cfg: Config) -> Dims:
@@ -309,8 +319,10 @@ class MGBF:
     # --- operators ---
     def apply(self, op: int, x, out=None, stream=None):
         import torch
-        if op in (OP_WAWT, OP_W, OP_BSQRT):
+        if op in (OP_WAWT, OP_W, OP_BSQRT, OP_A2F):
             oshape = self.field_shape()
+        elif op == OP_F2A:
+            oshape = self.dims.anal_shape()
         else:
             oshape = (self.dims.mg_size,)
         if out is None:
@@ -321,6 +333,23 @@ class MGBF:
     def filtering_procedure(self, x, out=None, stream=None):
         """y = W A W^T x (filtering_rad2_ens / filtering_rad2_bckg)."""
         return self.apply(OP_WAWT, x, out, stream)
+
+    def anal_to_filt(self, w, out=None, stream=None):
+        """Analysis field (mm0, nm0, km_a_all) -> generation-1 field
+        (anal_to_filt_all2, src.mgbf/mg_transfer.f90:154-176)."""
+        return self.apply(OP_A2F, w, out, stream)
+
+    def filt_to_anal(self, v, out=None, stream=None):
+        """Generation-1 field -> analysis field (filt_to_anal_all2,
+        src.mgbf/mg_transfer.f90:179-193)."""
+        return self.apply(OP_F2A, v, out, stream)
+
+    def localize_analysis(self, w, stream=None):
+        """The driver's sequence on an analysis field: anal_to_filt_all2,
+        filtering_procedure, filt_to_anal_all2 (RBETA_TEST.f90:34-126)."""
+        v = self.anal_to_filt(w, stream=stream)
+        self.apply(OP_WAWT, v, v, stream)
+        return self.filt_to_anal(v, stream=stream)
 
     def stage(self, stage: int, gen: int, x, out=None, stream=None):
         import torch

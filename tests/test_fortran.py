@@ -77,3 +77,30 @@ def test_fortran_driver_matches_oracle(tmp_path):
     # the driver rounds the input to FP32 at the device boundary
     ref = O.wawt(P, x.astype(np.float32).astype(np.float64))
     assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5
+
+
+@pytest.mark.gpu
+def test_fortran_driver_on_a_finer_analysis_grid_matches_oracle(tmp_path):
+    # nm0 = 1.5 nxPE im_filt and lm_a = 2 lm - 1: anal_to_filt_all2 and
+    # filt_to_anal_all2 interpolate (mg_transfer.f90:154-203)
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle import oracle as O
+    _need(DRIVER)
+    cfg = M.Config.for_grid(64, 48, 8, n_ens=2, gm=3, h=4, hz=2, weights=[1.0, 0.5, 0.25],
+                            nm0=96, mm0=72, lm_a=15)
+    d = M.derive(cfg)
+    write_namelist(cfg, str(tmp_path / "mgbeta.nml"))
+    x = np.random.default_rng(6).standard_normal(d.anal_shape())
+    x.astype("<f8").tofile(tmp_path / "in.bin")
+    out = subprocess.run([DRIVER, "mgbeta.nml", "in.bin", "out.bin"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(d.anal_shape())
+    P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
+                       hz=2, p=2, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                       weights=[1.0, 0.5, 0.25])
+    args = (P, 96, 72, cfg.nxPE, cfg.nyPE, 15)
+    ref = O.filt_to_anal(*args, O.wawt(P, O.anal_to_filt(*args, x.astype(np.float32).astype(np.float64))))
+    assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5

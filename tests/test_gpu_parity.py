@@ -188,6 +188,23 @@ def test_vertical_band_fused_into_the_last_radial_pass(monkeypatch, lm, n_ens, h
         assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
 
 
+@pytest.mark.parametrize("fuse", ["0", "1", "2", "3"])
+def test_every_fusion_level_matches_oracle(monkeypatch, fuse):
+    # how much rides in the radial passes (nothing; the prolongation; and the
+    # vertical band; and the restriction) changes launches, not results
+    monkeypatch.setenv("MGBF_FUSE", fuse)
+    cfg = M.Config.for_grid(128, 96, 32, n_ens=2, gm=3, h=6, hz=3, weights=[1.0, 0.6, 0.3])
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = field(d.gen_shape(1), 61)
+        P = oparams(cfg, d)
+        y = op.filtering_procedure(to_dev(x))
+        v = op.apply(M.OP_WT, to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
+
+
 def test_wawt_background_procedure_with_2d_variables():
     cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=2, km2=3, km3=2, mgbf_proc=1)
     with M.MGBF(cfg) as op:

@@ -1,0 +1,90 @@
+"""Analysis <-> filter transfer on the GPU (anal_to_filt_all2 /
+filt_to_anal_all2, src.mgbf/mg_transfer.f90:154-203) against the FP64
+oracle: the 4-point horizontal interpolation between a coarser filter grid
+and the analysis grid, the 2:1 vertical map (lm_a = 2 lm - 1), their
+transposes, and the driver's whole sequence on an analysis field."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O
+from standalone_mgbf_loc_amd import mgbf as M
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("no HIP device", allow_module_level=True)
+
+DEV = torch.device("cuda:0")
+TOL = 1e-5
+
+
+def relerr(y, ref):
+    y = y.detach().cpu().double().numpy().reshape(ref.shape)
+    return float(np.max(np.abs(y - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def setup(nx, ny, lm, n_ens, na, ma, lm_a, gm=3, h=4):
+    cfg = M.Config.for_grid(nx, ny, lm, n_ens=n_ens, gm=gm, h=h, hz=2, weights=[1.0, 0.5, 0.25],
+                            nm0=na, mm0=ma, lm_a=lm_a)
+    op = M.MGBF(cfg)
+    d = op.dims
+    P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=lm, nvcol=d.km3_all, hx=h, hy=h, hz=2, p=2,
+                       gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02, weights=[1.0, 0.5, 0.25])
+    return cfg, op, P
+
+
+# (filter nx, ny, lm, members, analysis na, ma, lm_a): ratios 1.5 and 2, with
+# and without the vertical map, one and two PE tiles per axis
+CASES = [(32, 24, 4, 2, 48, 36, 7), (32, 32, 6, 1, 64, 64, 6), (48, 32, 3, 3, 72, 48, 5),
+         (32, 24, 5, 1, 32, 24, 9)]
+
+
+@pytest.mark.parametrize("nx,ny,lm,n_ens,na,ma,lm_a", CASES)
+def test_transfer_matches_oracle_and_is_its_own_transpose(nx, ny, lm, n_ens, na, ma, lm_a):
+    cfg, op, P = setup(nx, ny, lm, n_ens, na, ma, lm_a)
+    try:
+        d = op.dims
+        assert (d.na, d.ma, d.km_a_all) == (na, ma, n_ens * lm_a)
+        rng = np.random.default_rng(nx + na + lm_a)
+        vf = rng.standard_normal(op.field_shape()).astype(np.float32)
+        wa = rng.standard_normal(d.anal_shape()).astype(np.float32)
+        a = op.filt_to_anal(torch.from_numpy(vf).to(DEV))
+        b = op.anal_to_filt(torch.from_numpy(wa).to(DEV))
+        torch.cuda.synchronize()
+        args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
+        assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64))) <= TOL
+        assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64))) <= TOL
+        wd = torch.from_numpy(wa).to(DEV)
+        lhs = op.dot(a, wd)
+        rhs = op.dot(torch.from_numpy(vf).to(DEV), b)
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs))
+    finally:
+        op.close()
+
+
+def test_driver_sequence_on_an_analysis_field():
+    # anal_to_filt_all2 -> filtering_procedure -> filt_to_anal_all2
+    # (RBETA_TEST.f90:34-126) on a 1.5x finer analysis grid with 2 lm - 1 levels
+    cfg, op, P = setup(32, 24, 4, 2, 48, 36, 7)
+    try:
+        d = op.dims
+        w = np.random.default_rng(21).standard_normal(d.anal_shape()).astype(np.float32)
+        y = op.localize_analysis(torch.from_numpy(w).to(DEV))
+        torch.cuda.synchronize()
+        args = (P, 48, 36, cfg.nxPE, cfg.nyPE, 7)
+        ref = O.filt_to_anal(*args, O.wawt(P, O.anal_to_filt(*args, w.astype(np.float64))))
+        assert relerr(y, ref) <= TOL
+    finally:
+        op.close()
+
+
+def test_matching_grids_make_the_transfer_a_copy():
+    cfg, op, P = setup(32, 32, 4, 2, 32, 32, 4)
+    try:
+        x = torch.randn(op.field_shape(), device=DEV)
+        assert torch.equal(op.filt_to_anal(x), x)
+        assert torch.equal(op.anal_to_filt(x), x)
+    finally:
+        op.close()

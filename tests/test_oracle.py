@@ -238,3 +238,67 @@ def test_wawt_is_linear():
     lhs = O.wawt(P, 2.0 * a - 3.0 * b)
     rhs = 2.0 * O.wawt(P, a) - 3.0 * O.wawt(P, b)
     assert np.max(np.abs(lhs - rhs)) < 1e-12
+
+
+# ---- analysis <-> filter transfer (anal_to_filt_all2 / filt_to_anal_all2) ----
+
+def _tparams(nx, ny, lm, nvcol, K=None):
+    return O.OracleParams(nx=nx, ny=ny, K=K if K is not None else lm * nvcol, lm=lm, nvcol=nvcol,
+                          hx=3, hy=3, hz=1, p=2, gm=2, ampl01=4 / 7, ampl02=2.0)
+
+
+def test_transfer_adjoint_is_the_transpose():
+    # <T v, w> = <v, T^T w> with the vertical 2:1 map and 2 PE tiles per axis
+    P = _tparams(32, 24, 3, 2)
+    rng = np.random.default_rng(11)
+    vf = rng.standard_normal(P.gen_shape(0))
+    wa = rng.standard_normal((36, 48, 2 * 5))
+    a = O.filt_to_anal(P, 48, 36, 2, 2, 5, vf)
+    b = O.anal_to_filt(P, 48, 36, 2, 2, 5, wa)
+    assert abs(np.sum(a * wa) - np.sum(vf * b)) <= 1e-12 * abs(np.sum(a * wa))
+
+
+def test_transfer_reproduces_quadratics_inside_and_constants_everywhere():
+    # the 4-point weights blend two quadratic interpolants: exact for any
+    # quadratic away from the mirrored edges; the half-sample mirror keeps a
+    # constant constant up to the edges
+    P = _tparams(32, 24, 1, 0, K=1)
+    xf = (np.arange(32) + 0.5) * 1.5
+    yf = (np.arange(24) + 0.5) * 1.5
+    f = lambda x, y: 1 + 0.3 * x - 0.02 * x * x + 0.1 * y + 0.004 * x * y - 0.01 * y * y
+    XF, YF = np.meshgrid(xf, yf)
+    a = O.filt_to_anal(P, 48, 36, 2, 2, 1, f(XF, YF)[:, :, None])[:, :, 0]
+    XA, YA = np.meshgrid(np.arange(48) + 0.5, np.arange(36) + 0.5)
+    assert np.max(np.abs(a - f(XA, YA))[4:-4, 4:-4]) < 1e-12
+    assert np.max(np.abs(a - f(XA, YA))) > 1e-3          # the edges are mirrored, not extrapolated
+    c = O.filt_to_anal(P, 48, 36, 2, 2, 1, np.full(P.gen_shape(0), 2.5))
+    assert np.max(np.abs(c - 2.5)) < 1e-13
+
+
+def test_transfer_is_the_identity_on_matching_grids_and_levels():
+    P = _tparams(24, 16, 4, 2)
+    x = np.random.default_rng(12).standard_normal(P.gen_shape(0))
+    assert np.array_equal(O.filt_to_anal(P, 24, 16, 2, 2, 4, x), x)
+    assert np.array_equal(O.anal_to_filt(P, 24, 16, 2, 2, 4, x), x)
+
+
+def test_vertical_transfer_is_the_two_to_one_linear_map():
+    # lm -> 2 lm - 1 levels: odd analysis levels copy, even ones average
+    # (l_vertical_direct_spec2)
+    P = _tparams(16, 16, 4, 3)
+    f = np.random.default_rng(13).standard_normal(P.gen_shape(0))
+    a = O.filt_to_anal(P, 16, 16, 1, 1, 7, f)
+    for e in range(3):
+        col, out = f[:, :, 4 * e: 4 * e + 4], a[:, :, 7 * e: 7 * e + 7]
+        assert np.allclose(out[:, :, 0::2], col, atol=1e-15)
+        assert np.allclose(out[:, :, 1::2], 0.5 * (col[:, :, :-1] + col[:, :, 1:]), atol=1e-15)
+
+
+def test_transfer_tiles_join_up_like_one_domain():
+    # the per-tile interpolation with halos from the neighbours is the same
+    # operator as on one tile covering the domain
+    P = _tparams(32, 24, 1, 0, K=3)
+    x = np.random.default_rng(14).standard_normal(P.gen_shape(0))
+    one = O.filt_to_anal(P, 48, 36, 1, 1, 1, x)
+    tiled = O.filt_to_anal(P, 48, 36, 4, 2, 1, x)
+    assert np.max(np.abs(one - tiled)) < 1e-12
