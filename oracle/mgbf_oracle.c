@@ -234,6 +234,31 @@ int mgbfo_taps1(const mgbfo_params* P, int maxt, int* gx, double* w) {
     return nt;
 }
 
+/* 1-D taps of the fast filter: getlinesum1 / vrbeta1 with el = ampl02 as
+   given (paspx, paspy are not passed through cholaspect,
+   mg_intstate.f90:370-375, 422-431). Returns count, or -1 past the halo. */
+static int taps_fast_raw(const mgbfo_params* P, int halo, int maxt, int* gxo, double* wo) {
+    const double rmom2_1 = 1.0 / sqrt((double)(2 * P->p) + 3.0);
+    const double exx = P->ampl02 * rmom2_1;
+    const double x = 1.0 / exx;
+    int gxl = (int)ceil(-x + EPS), gxm = (int)floor(x - EPS);
+    if (gxl < -halo || gxm > halo) return -1;
+    int nt = 0;
+    double s = 0.0;
+    for (int gx = gxl; gx <= gxm; ++gx, ++nt) {
+        double rr = ((double)gx * exx) * ((double)gx * exx);
+        double v = ipow(1.0 - rr, P->p);
+        s += v;
+        if (nt < maxt) { gxo[nt] = gx; wo[nt] = v; }
+    }
+    for (int t = 0; t < nt && t < maxt; ++t) wo[t] /= s;
+    return nt;
+}
+
+int mgbfo_taps_fast(const mgbfo_params* P, int maxt, int* gx, double* w) {
+    return taps_fast_raw(P, P->hx, maxt, gx, w);
+}
+
 /* ------------------------------------------------------------------------
  * Halo fill and fold (boco_2d / bocoT_2d on the whole domain)
  * ---------------------------------------------------------------------- */
@@ -514,8 +539,138 @@ static void vertical_op(const mgbfo_params* P, pf_t* V, int adjoint) {
  * Single-stage wrappers on interior arrays
  * ---------------------------------------------------------------------- */
 
+/* ------------------------------------------------------------------------
+ * Fast (separable) horizontal filter: filtering_fast_ens / _bckg,
+ * mg_filtering.f90:1316-1506, 825-997
+ * ---------------------------------------------------------------------- */
+
+typedef struct {
+    int nx, ny;                  /* taps along x and y */
+    int gx[64], gy[64];
+    double wx[64], wy[64];
+} fast_t;
+
+static fast_t fast_make(const mgbfo_params* P) {
+    fast_t F;
+    F.nx = taps_fast_raw(P, P->hx, 64, F.gx, F.wx);
+    F.ny = taps_fast_raw(P, P->hy, 64, F.gy, F.wy);
+    if (F.nx < 0 || F.ny < 0 || F.nx > 64 || F.ny > 64) abort();   /* the reference stops */
+    return F;
+}
+
+/* F: bocoy, rbeta along y over the interior columns, bocox, rbeta along x
+   over the interior rows (gathers; the halo of each result is zero).
+   F^T: rbetaT along x (scatter into the x halo), bocoTx, rbetaT along y,
+   bocoTy. bocox / bocoy are the x and y halves of boco_2d, the same
+   half-sample mirror. */
+static void fast_filter(const fast_t* F, pf_t* a, int adjoint) {
+    const int K = a->K, n = a->n, m = a->m, hx = a->hx, hy = a->hy;
+    if (!adjoint) {
+        for (int j = 1; j <= hy; ++j)
+            for (int i = 1; i <= n; ++i) {
+                memcpy(PCOL(a, i, 1 - j), PCOL(a, i, j), sizeof(double) * K);
+                memcpy(PCOL(a, i, m + j), PCOL(a, i, m + 1 - j), sizeof(double) * K);
+            }
+        pf_t b = pf_alloc(K, n, m, hx, hy);
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= n; ++i) {
+                double* d = PCOL(&b, i, j);
+                for (int t = 0; t < F->ny; ++t) {
+                    const double* src = PCOL(a, i, j + F->gy[t]);
+                    for (int k = 0; k < K; ++k) d[k] += F->wy[t] * src[k];
+                }
+            }
+        free(a->d);
+        a->d = b.d;
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= hx; ++i) {
+                memcpy(PCOL(a, 1 - i, j), PCOL(a, i, j), sizeof(double) * K);
+                memcpy(PCOL(a, n + i, j), PCOL(a, n + 1 - i, j), sizeof(double) * K);
+            }
+        pf_t c = pf_alloc(K, n, m, hx, hy);
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= n; ++i) {
+                double* d = PCOL(&c, i, j);
+                for (int t = 0; t < F->nx; ++t) {
+                    const double* src = PCOL(a, i + F->gx[t], j);
+                    for (int k = 0; k < K; ++k) d[k] += F->wx[t] * src[k];
+                }
+            }
+        free(a->d);
+        a->d = c.d;
+    } else {
+        pf_t b = pf_alloc(K, n, m, hx, hy);
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= n; ++i) {
+                const double* src = PCOL(a, i, j);
+                for (int t = 0; t < F->nx; ++t) {
+                    double* d = PCOL(&b, i + F->gx[t], j);
+                    for (int k = 0; k < K; ++k) d[k] += F->wx[t] * src[k];
+                }
+            }
+        free(a->d);
+        a->d = b.d;
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= hx; ++i) {
+                vadd(PCOL(a, i, j), PCOL(a, 1 - i, j), K);
+                vadd(PCOL(a, n + 1 - i, j), PCOL(a, n + i, j), K);
+            }
+        pf_t c = pf_alloc(K, n, m, hx, hy);
+#pragma omp parallel for schedule(static)
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= m; ++j) {
+                const double* src = PCOL(a, i, j);
+                for (int t = 0; t < F->ny; ++t) {
+                    double* d = PCOL(&c, i, j + F->gy[t]);
+                    for (int k = 0; k < K; ++k) d[k] += F->wy[t] * src[k];
+                }
+            }
+        free(a->d);
+        a->d = c.d;
+        for (int j = 1; j <= hy; ++j)
+            for (int i = 1; i <= n; ++i) {
+                vadd(PCOL(a, i, j), PCOL(a, i, 1 - j), K);
+                vadd(PCOL(a, i, m + 1 - j), PCOL(a, i, m + j), K);
+            }
+    }
+}
+
+/* The horizontal filter of the procedure: the radial 2-D beta filter with
+   boco_2d / bocoT_2d, or the fast separable one. */
+static void hfilter(const mgbfo_params* P, const taps_t* T, const fast_t* F, pf_t* a, int adjoint) {
+    if (P->fast) {
+        fast_filter(F, a, adjoint);
+    } else if (!adjoint) {
+        boco_2d(a, P->hx, P->hy);
+        vrbeta2(T, a);
+    } else {
+        vrbeta2T(T, a);
+        bocoT_2d(a, P->hx, P->hy);
+    }
+}
+
+static taps_t htaps_make(const mgbfo_params* P) {
+    if (P->fast) {
+        taps_t T = {0, NULL, NULL, NULL};
+        return T;
+    }
+    return taps2_make(P);
+}
+
 void mgbfo_radial(const mgbfo_params* P, int K, int n, int m, const double* in, double* out,
                   int adjoint) {
+    if (P->fast) {
+        fast_t F = fast_make(P);
+        pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
+        pf_from_interior(&a, in);
+        fast_filter(&F, &a, adjoint);
+        pf_to_interior(&a, out);
+        pf_free(&a);
+        return;
+    }
     taps_t T = taps2_make(P);
     pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
     pf_from_interior(&a, in);
@@ -667,16 +822,16 @@ void mgbfo_wawt(const mgbfo_params* P, const double* wconst, const double* const
                 const double* x, double* y) {
     /* filtering_rad2_ens, mg_filtering.f90:1000-1147 */
     const int gm = P->gm;
-    taps_t T = taps2_make(P);
+    taps_t T = htaps_make(P);
+    fast_t F;
+    if (P->fast) F = fast_make(P);
     pf_t* G = gens_alloc(P);
     upsweep_inputs(P, x, G);
     for (int g = 0; g < gm; ++g) {
         if (g == 0 && !P->l_filt_g1) continue;
-        vrbeta2T(&T, &G[g]);
-        bocoT_2d(&G[g], P->hx, P->hy);
+        hfilter(P, &T, &F, &G[g], 1);
         weight_pf(P, g, &G[g], wconst, wfield);
-        boco_2d(&G[g], P->hx, P->hy);
-        vrbeta2(&T, &G[g]);
+        hfilter(P, &T, &F, &G[g], 0);
     }
     downsweep_outputs(P, G, y);
     gens_free(P, G);
@@ -685,15 +840,14 @@ void mgbfo_wawt(const mgbfo_params* P, const double* wconst, const double* const
 
 void mgbfo_wt(const mgbfo_params* P, const double* x, double* v) {
     const int gm = P->gm;
-    taps_t T = taps2_make(P);
+    taps_t T = htaps_make(P);
+    fast_t F;
+    if (P->fast) F = fast_make(P);
     pf_t* G = gens_alloc(P);
     upsweep_inputs(P, x, G);
     size_t off = 0;
     for (int g = 0; g < gm; ++g) {
-        if (!(g == 0 && !P->l_filt_g1)) {
-            vrbeta2T(&T, &G[g]);
-            bocoT_2d(&G[g], P->hx, P->hy);
-        }
+        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &T, &F, &G[g], 1);
         pf_to_interior(&G[g], v + off);
         off += (size_t)P->K * G[g].n * G[g].m;
     }
@@ -703,16 +857,15 @@ void mgbfo_wt(const mgbfo_params* P, const double* x, double* v) {
 
 void mgbfo_w(const mgbfo_params* P, const double* v, double* y) {
     const int gm = P->gm;
-    taps_t T = taps2_make(P);
+    taps_t T = htaps_make(P);
+    fast_t F;
+    if (P->fast) F = fast_make(P);
     pf_t* G = gens_alloc(P);
     size_t off = 0;
     for (int g = 0; g < gm; ++g) {
         pf_from_interior(&G[g], v + off);
         off += (size_t)P->K * G[g].n * G[g].m;
-        if (!(g == 0 && !P->l_filt_g1)) {
-            boco_2d(&G[g], P->hx, P->hy);
-            vrbeta2(&T, &G[g]);
-        }
+        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &T, &F, &G[g], 0);
     }
     downsweep_outputs(P, G, y);
     gens_free(P, G);

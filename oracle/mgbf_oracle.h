@@ -30,6 +30,8 @@ typedef struct mgbfo_params {
     double p_del, p_eps;   /* horizontal aspect perturbations (the reference fixes them at 0) */
     int l_filt_g1;         /* filter and weight generation 1 */
     int l_vertical_filter; /* apply the vertical 1-D filter */
+    int fast;              /* filtering_fast_* (procedures 3, 8): the horizontal filter is the
+                              product of 1-D beta filters along x and y */
 } mgbfo_params;
 
 /* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
@@ -38,6 +40,9 @@ typedef struct mgbfo_params {
 int mgbfo_taps2(const mgbfo_params* P, int maxt, int* gx, int* gy, double* w);
 /* Same for the 1-D vertical filter (gx only). */
 int mgbfo_taps1(const mgbfo_params* P, int maxt, int* gx, double* w);
+/* The 1-D factor of the fast filter along x (halo hx): paspx = ampl02 enters
+   getlinesum1 / vrbeta1 without cholaspect (mg_intstate.f90:370-375, 427). */
+int mgbfo_taps_fast(const mgbfo_params* P, int maxt, int* gx, double* w);
 /* Line-sum normalizers ss = 1/sum(rrc^p) (getlinesum1/2). */
 double mgbfo_ss1(const mgbfo_params* P);
 double mgbfo_ss2(const mgbfo_params* P);

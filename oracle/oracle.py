@@ -35,6 +35,7 @@ class _Params(ctypes.Structure):
         ("ampl01", ctypes.c_double), ("ampl02", ctypes.c_double),
         ("p_del", ctypes.c_double), ("p_eps", ctypes.c_double),
         ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
+        ("fast", ctypes.c_int),
     ]
 
 
@@ -48,6 +49,7 @@ def lib():
         ip = ctypes.POINTER(ctypes.c_int)
         L.mgbfo_taps2.argtypes = [P, ctypes.c_int, ip, ip, dp]
         L.mgbfo_taps1.argtypes = [P, ctypes.c_int, ip, dp]
+        L.mgbfo_taps_fast.argtypes = [P, ctypes.c_int, ip, dp]
         L.mgbfo_ss1.argtypes = [P]; L.mgbfo_ss1.restype = ctypes.c_double
         L.mgbfo_ss2.argtypes = [P]; L.mgbfo_ss2.restype = ctypes.c_double
         L.mgbfo_radial.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
@@ -123,11 +125,12 @@ class OracleParams:
     l_filt_g1: bool = True
     l_vertical_filter: bool = True
     weights: list = field(default_factory=lambda: [1.0, 0.5, 0.25, 0.125])
+    fast: bool = False     # filtering_fast_* (procedures 3, 8)
 
     def c(self) -> _Params:
         return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
                        self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
-                       self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter))
+                       self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast))
 
     def gen_shape(self, g: int):
         """(m_g, n_g, K) of generation g (0-based)."""
@@ -156,6 +159,18 @@ def taps1(P: OracleParams):
     gx = (ctypes.c_int * cap)()
     w = np.zeros(cap)
     nt = lib().mgbfo_taps1(ctypes.byref(c), cap, gx, _dp(w))
+    if nt < 0:
+        raise ValueError("filter reach exceeds the halo")
+    return np.array(gx[:nt]), w[:nt].copy()
+
+
+def taps_fast(P: OracleParams):
+    """1-D factor (offsets, weights) of the fast filter along x."""
+    c = P.c()
+    cap = 2 * P.hx + 1
+    gx = (ctypes.c_int * cap)()
+    w = np.zeros(cap)
+    nt = lib().mgbfo_taps_fast(ctypes.byref(c), cap, gx, _dp(w))
     if nt < 0:
         raise ValueError("filter reach exceeds the halo")
     return np.array(gx[:nt]), w[:nt].copy()

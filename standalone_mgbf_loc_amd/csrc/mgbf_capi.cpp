@@ -631,7 +631,8 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->device = device;
     c->K = s.dims.km_all;
     c->gm = s.dims.gm;
-    c->filt_g1 = cfg->mgbf_proc == 1 ? true : cfg->l_filt_g1 != 0;
+    // the background procedures filter generation 1 whatever l_filt_g1 says
+    c->filt_g1 = cfg->mgbf_proc == 1 || cfg->mgbf_proc == 3 ? true : cfg->l_filt_g1 != 0;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     {
         // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for

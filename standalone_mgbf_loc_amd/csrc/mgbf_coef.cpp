@@ -12,6 +12,7 @@
 // 2-point half-sample mirror of boco_2d / bocoT_2d.
 #include "mgbf_coef.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 
@@ -269,16 +270,18 @@ bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     return true;
 }
 
-bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w) {
+// 1-D beta taps of getlinesum1 / vrbeta1 (jp_pbfil.f90:149-177, 856-885)
+// for the factor el of the aspect: offsets gx with |gx el'| < 1, el' = el /
+// sqrt(2p+3), weights (1 - (gx el')^2)^p normalized to sum 1. False if the
+// reach exceeds the halo.
+bool beta1_taps(double el, int halo, int p, std::vector<int>& gx, std::vector<double>& w) {
     gx.clear();
     w.clear();
-    const int p = cfg.p;
     const double rmom2_1 = 1.0 / std::sqrt(2.0 * p + 3.0);
-    const double el = 1.0 / std::sqrt(cfg.mg_ampl01);     // cholaspect1
     const double exx = el * rmom2_1;
     const double x = 1.0 / exx;
     const int gxl = (int)std::ceil(-x + kEps), gxm = (int)std::floor(x - kEps);
-    if (gxl < -cfg.hz || gxm > cfg.hz) return false;
+    if (gxl < -halo || gxm > halo) return false;
     double s = 0;
     for (int g = gxl; g <= gxm; ++g) {
         const double rr = (g * exx) * (g * exx);
@@ -290,6 +293,24 @@ bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<dou
     return true;
 }
 
+bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w) {
+    return beta1_taps(1.0 / std::sqrt(cfg.mg_ampl01), cfg.hz, cfg.p, gx, w);   // cholaspect1
+}
+
+bool fast_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
+    // the fast filter's x and y factors keep the aspect amplitude as the
+    // factor itself: paspx = paspy = mg_ampl02 reach getlinesum1 and vrbeta1
+    // without cholaspect (mg_intstate.f90:370-375, 422-431)
+    std::vector<int> gx, gy;
+    std::vector<double> wx, wy;
+    if (!beta1_taps(cfg.mg_ampl02, cfg.hx, cfg.p, gx, wx) || !beta1_taps(cfg.mg_ampl02, cfg.hy, cfg.p, gy, wy))
+        return false;
+    taps.clear();
+    for (size_t b = 0; b < gy.size(); ++b)
+        for (size_t a = 0; a < gx.size(); ++a) taps.push_back({gx[a], gy[b], wx[a] * wy[b]});
+    return true;
+}
+
 int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* err) {
     d = mgbf_dims{};
     if (c.nxPE < 1 || c.nyPE < 1 || c.im_filt < 1 || c.jm_filt < 1 || c.lm < 1 || c.n_ens < 1 ||
@@ -297,8 +318,9 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "invalid namelist value (sizes must be positive)";
         return MGBF_ERR_CONFIG;
     }
-    if (c.mgbf_proc != 1 && c.mgbf_proc != 6) {
-        *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 6 filtering_rad2_ens)",
+    if (c.mgbf_proc != 1 && c.mgbf_proc != 3 && c.mgbf_proc != 6 && c.mgbf_proc != 8) {
+        *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 3 filtering_fast_bckg, "
+                   "6 filtering_rad2_ens, 8 filtering_fast_ens)",
                    c.mgbf_proc);
         return MGBF_ERR_UNSUPPORTED;
     }
@@ -316,9 +338,10 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     }
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
-    if (c.mgbf_proc == 1) {
+    const bool bckg = c.mgbf_proc == 1 || c.mgbf_proc == 3;
+    if (bckg) {
         if (c.n_ens != 1) {
-            *err = "mgbf_proc=1 (background) needs n_ens=1";
+            *err = fmt("mgbf_proc=%d (background) needs n_ens=1", c.mgbf_proc);
             return MGBF_ERR_CONFIG;
         }
         d.km_all = d.km;
@@ -328,7 +351,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         d.km3_all = c.km3 * c.n_ens;
         if (c.km2 != 0 && c.l_vertical_filter) {
             // sup_vrbeta1_ens strides the stack by lm only (mg_filtering.f90:2066-2069)
-            *err = "mgbf_proc=6 with km2>0 and l_vertical_filter: the reference's ensemble "
+            *err = "ensemble procedures (6, 8) with km2>0 and l_vertical_filter: the reference's ensemble "
                    "vertical filter assumes km2=0";
             return MGBF_ERR_CONFIG;
         }
@@ -405,7 +428,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     d.mg_size = ms;
     d.na = c.nm0;
     d.ma = c.mm0;
-    d.km_a_all = (c.mgbf_proc == 1 ? 1 : c.n_ens) * (c.km2 + c.km3 * c.lm_a);
+    d.km_a_all = (bckg ? 1 : c.n_ens) * (c.km2 + c.km3 * c.lm_a);
     d.anal_size = (long long)d.km_a_all * d.na * d.ma;
     if (c.l_vertical_filter && c.lm <= c.hz) {
         *err = "lm must exceed hz for the vertical mirror";
@@ -419,7 +442,15 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     s.cfg = cfg;
     int st = derive(cfg, s.dims, s.nx, s.ny, err);
     if (st != MGBF_OK) return st;
-    if (!radial_taps(cfg, s.taps2)) {
+    const bool fast = cfg.mgbf_proc == 3 || cfg.mgbf_proc == 8;
+    if (fast) {
+        // filtering_fast_*: the horizontal filter is the product of 1-D beta
+        // filters along x and y (rbeta over each row and column)
+        if (!fast_taps(cfg, s.taps2)) {
+            *err = "In getlinesum1; filter reach fx becomes too large for hx";
+            return MGBF_ERR_CONFIG;
+        }
+    } else if (!radial_taps(cfg, s.taps2)) {
         *err = "In getlinesum2; filter reach becomes too large for hx/hy";
         return MGBF_ERR_CONFIG;
     }
@@ -430,6 +461,13 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     s.dims.ntaps2 = (int)s.taps2.size();
     s.dims.ntaps1 = (int)s.taps1_gx.size();
     s.H = cfg.hx > cfg.hy ? cfg.hx : cfg.hy;
+    if (fast) {
+        // the kernels need only the taps' extent, which a raw factor can keep
+        // far inside the halo
+        int ext = 1;
+        for (const Tap2& t : s.taps2) ext = std::max(ext, std::max(std::abs(t.gx), std::abs(t.gy)));
+        s.H = ext;
+    }
     if (s.H > kMaxHalo || s.H < 1) {
         *err = fmt("radial half-width %d outside the built range 1..%d", s.H, kMaxHalo);
         return MGBF_ERR_UNSUPPORTED;

@@ -100,9 +100,15 @@ class DenseMGBF:
     """Dense matrices of every stage for one tiny problem."""
 
     def __init__(self, nx, ny, K, lm, nvcol, hx, hz, p, gm, R2, R2v,
-                 weights, l_filt_g1=True, l_vertical_filter=True, wfield=None):
+                 weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False):
         self.nx, self.ny, self.K, self.gm = nx, ny, K, gm
-        taps = disc_taps(hx, Fraction(R2), p)
+        if fast:
+            # separable: the outer product of two 1-D beta filters whose
+            # squared reach is R2 (offsets a with a^2 < R2)
+            lt = line_taps(hx, Fraction(R2), p)
+            taps = [(a, b, wa * wb) for b, wb in lt for a, wa in lt]
+        else:
+            taps = disc_taps(hx, Fraction(R2), p)
         ltaps = line_taps(hz, Fraction(R2v), p)
         IK = np.eye(K)
         # vertical: block diagonal over the nvcol columns, identity on the rest

@@ -78,7 +78,7 @@ def test_nm0_divisibility_is_checked():
 
 
 @pytest.mark.parametrize("key,val", [("mgbf_line", True), ("l_loc", True), ("lquart", True),
-                                     ("mgbf_proc", 8)])
+                                     ("mgbf_proc", 2), ("mgbf_proc", 4), ("mgbf_proc", 7)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)
     setattr(cfg, key, val)

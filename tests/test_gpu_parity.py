@@ -28,12 +28,12 @@ def relerr(y, y64):
 
 
 def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
-    bckg = cfg.mgbf_proc == 1
+    bckg = cfg.mgbf_proc in (1, 3)
     return O.OracleParams(
         nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx, hy=cfg.hy,
         hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
         l_filt_g1=True if bckg else bool(cfg.l_filt_g1),
-        l_vertical_filter=bool(cfg.l_vertical_filter),
+        l_vertical_filter=bool(cfg.l_vertical_filter), fast=cfg.mgbf_proc in (3, 8),
         weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
 
 
@@ -203,6 +203,44 @@ def test_every_fusion_level_matches_oracle(monkeypatch, fuse):
         sync()
         assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
         assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
+
+
+@pytest.mark.parametrize("case", ["run_namelist", "wide", "filt_g1_off"])
+def test_fast_ensemble_procedure_matches_oracle(case):
+    # filtering_fast_ens (mgbf_proc = 8): the horizontal filter is the product
+    # of 1-D beta filters with el = mg_ampl02 as given. "run_namelist" takes
+    # the reference's RUN/mgbf.nml values (mg_ampl01 = mg_ampl02 = 2,
+    # p = 1.3 read as 1, hx = 6, hz = 10): a 3-point filter per axis
+    kw = dict(n_ens=3, gm=3, h=6, hz=10, p=1, mgbf_proc=8, weights=[1.0, 0.5, 0.25])
+    if case == "filt_g1_off":
+        kw["l_filt_g1"] = False
+    cfg = M.Config.for_grid(96, 64, 16, **kw)
+    cfg.mg_ampl01 = 2.0
+    cfg.mg_ampl02 = 0.45 if case == "wide" else 2.0
+    if case == "wide":
+        cfg.p = 2     # reach sqrt(7)/0.45 = 5.9: 11 taps per axis
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = field(d.gen_shape(1), 71)
+        y = op.filtering_procedure(to_dev(x))
+        v = op.apply(M.OP_WT, to_dev(x))
+        sync()
+        P = oparams(cfg, d)
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
+
+
+def test_fast_background_procedure_with_2d_variables():
+    # filtering_fast_bckg (mgbf_proc = 3): one field, 2-D variables after the
+    # 3-D ones, generation 1 always filtered
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=2, km2=3, km3=2, mgbf_proc=3)
+    cfg.mg_ampl02 = 0.8
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x = field(d.gen_shape(1), 72)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
 
 
 def test_wawt_background_procedure_with_2d_variables():

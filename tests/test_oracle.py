@@ -302,3 +302,58 @@ def test_transfer_tiles_join_up_like_one_domain():
     one = O.filt_to_anal(P, 48, 36, 1, 1, 1, x)
     tiled = O.filt_to_anal(P, 48, 36, 4, 2, 1, x)
     assert np.max(np.abs(one - tiled)) < 1e-12
+
+
+# ---- the fast (separable) filter of procedures 3 and 8 ----
+
+def test_fast_taps_take_the_aspect_amplitude_as_the_factor():
+    # filtering_fast_*: el = mg_ampl02 itself (no cholaspect), so with
+    # mg_ampl02 = 2 and p = 1 the reach is sqrt(2p+3)/2 = 1.118: 3 taps with
+    # weights (1 - 4 a^2/5)^1 / (7/5) = (1/7, 5/7, 1/7)
+    P = O.OracleParams(nx=16, ny=16, K=1, lm=1, nvcol=0, hx=6, hy=6, hz=3, p=1, gm=2,
+                       ampl01=2.0, ampl02=2.0, fast=True)
+    gx, w = O.taps_fast(P)
+    assert list(gx) == [-1, 0, 1]
+    assert np.allclose(w, [1 / 7, 5 / 7, 1 / 7], atol=1e-15)
+    P.ampl02 = 0.25   # reach 8.9 > hx: the reference stops
+    with pytest.raises(ValueError):
+        O.taps_fast(P)
+
+
+def test_fast_filter_is_separable_and_self_adjoint():
+    # on a rank-1 field f(x) g(y) the filter is the product of the 1-D
+    # filters with the half-sample mirror; F (bocoy, y gather, bocox, x gather)
+    # and F^T (x scatter, bocoTx, y scatter, bocoTy) are the same matrix
+    P = O.OracleParams(nx=20, ny=14, K=2, lm=1, nvcol=0, hx=4, hy=4, hz=1, p=2, gm=2,
+                       ampl01=4 / 7, ampl02=0.6, fast=True)
+    gx, w = O.taps_fast(P)
+    assert gx.max() == 4
+    rng = np.random.default_rng(15)
+    fx, gy = rng.standard_normal(20), rng.standard_normal(14)
+
+    def filt1(v):
+        h = int(gx.max())
+        vp = np.pad(v, h, mode="symmetric")
+        return np.array([sum(wt * vp[i + h + a] for a, wt in zip(gx, w)) for i in range(len(v))])
+
+    x = np.outer(gy, fx)[:, :, None] * np.array([1.0, -2.0])
+    y = O.radial(P, x)
+    assert np.allclose(y, np.outer(filt1(gy), filt1(fx))[:, :, None] * np.array([1.0, -2.0]), atol=1e-13)
+    z = rng.standard_normal((14, 20, 2))
+    assert np.allclose(O.radial(P, z), O.radial(P, z, adjoint=True), atol=1e-13)
+
+
+@pytest.mark.parametrize("filt_g1", [True, False])
+def test_fast_wawt_matches_dense_model(filt_g1):
+    P = _P(nx=16, ny=12, K=4, lm=2, nvcol=2, hx=3, hy=3, hz=1, gm=3,
+           ampl02=Fraction(1, 1), ampl01=Fraction(4, 7), l_filt_g1=filt_g1,
+           weights=[1.0, 0.7, 0.3], fast=True)
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    Dm = D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm,
+                     Fraction(2 * P.p + 3) / Fraction(P.ampl02).limit_denominator(1000) ** 2,
+                     Fraction(P.ampl01).limit_denominator(1000) * (2 * P.p + 3),
+                     list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, None, fast=True)
+    assert np.max(np.abs(B - Dm.B())) < 1e-13
+
