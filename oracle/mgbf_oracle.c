@@ -16,6 +16,8 @@
  *   - bocoT_2d (adjoint fold) .................. mg_bocos.f90:929-957, 1033-1061
  *   - adjoint / direct1 (R^T-side, P) .......... mg_generations.f90:1117-1187, 1190-1250
  *   - def_offset_coef constants ................ mg_interpolate.f90:61-86
+ *   - adjoint2 / direct2 (lquart) .............. mg_generations.f90:1253-1374
+ *   - upsending2_ens / downsending2_ens ........ mg_generations.f90:497-609
  *   - upsending_ens / downsending_ens .......... mg_generations.f90:382-494
  *   - weighting_ens ............................ mg_generations.f90:1012-1052
  *   - sup_vrbeta1_ens / sup_vrbeta1T_ens ....... mg_filtering.f90:2042-2151
@@ -38,6 +40,9 @@
 /* interpolation constants, mg_interpolate.f90:76-79 */
 static const double PCOEF[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
 static const double QCOEF[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
+/* quadratic constants a_coef, b_coef, mg_interpolate.f90:81-84 */
+static const double ACOEF[3] = {5. / 32., 30. / 32., -3. / 32.};
+static const double BCOEF[3] = {-3. / 32., 30. / 32., 5. / 32.};
 
 int mgbfo_num_threads(void) {
 #ifdef _OPENMP
@@ -482,6 +487,91 @@ static void direct1_op(const pf_t* W, pf_t* F) {
 }
 
 /* ------------------------------------------------------------------------
+ * Quadratic inter-generation transfer (lquart): adjoint2 and direct2.
+ * Coarse arrays have a 1-point halo: indices 0 .. n/2+1. Fine index i
+ * (1-based) reads coarse iL-1 .. iL+1 with iL = (i+1)/2 and a_coef for odd
+ * i, iL = i/2 and b_coef for even i; y as x.
+ * ---------------------------------------------------------------------- */
+
+/* coarse base index and constants of fine index i (1-based) */
+static inline int quad_base(int i, const double** c) {
+    *c = (i & 1) ? ACOEF : BCOEF;
+    return ((i & 1) ? (i + 1) / 2 : i / 2) - 1;
+}
+
+/* adjoint2: F(K,1:n,1:m) -> W(K,0:nL+1,0:mL+1), mg_generations.f90:1253-1315.
+   Scatter: first along y into W_AUX(K,1:n,0:mL+1), then along x. */
+static void adjoint2_op(const pf_t* F, pf_t* W) {
+    const int K = F->K, n = F->n, m = F->m;
+    const int mL = m / 2;
+    pf_t A = pf_alloc(K, n, mL, 0, 1);
+    pf_zero(W);
+#pragma omp parallel
+    {
+        int nth = 1, tid = 0;
+#ifdef _OPENMP
+        nth = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        int k0 = (int)((long)K * tid / nth), k1 = (int)((long)K * (tid + 1) / nth);
+        for (int j = m; j >= 1; --j) {
+            const double* c;
+            const int b = quad_base(j, &c);
+            for (int i = n; i >= 1; --i) {
+                const double* f = PCOL(F, i, j);
+                for (int t = 2; t >= 0; --t) {
+                    double* dst = PCOL(&A, i, b + t);
+                    for (int k = k0; k < k1; ++k) dst[k] += c[t] * f[k];
+                }
+            }
+        }
+        for (int jL = mL + 1; jL >= 0; --jL)
+            for (int i = n; i >= 1; --i) {
+                const double* c;
+                const int b = quad_base(i, &c);
+                const double* a = PCOL(&A, i, jL);
+                for (int t = 2; t >= 0; --t) {
+                    double* dst = PCOL(W, b + t, jL);
+                    for (int k = k0; k < k1; ++k) dst[k] += c[t] * a[k];
+                }
+            }
+    }
+    pf_free(&A);
+}
+
+/* direct2: W(K,0:nL+1,0:mL+1) -> F(K,1:n,1:m), mg_generations.f90:1318-1374.
+   Gather: along x into W_AUX(K,1:n,0:mL+1), then along y. */
+static void direct2_op(const pf_t* W, pf_t* F) {
+    const int K = F->K, n = F->n, m = F->m;
+    const int mL = m / 2;
+    pf_t A = pf_alloc(K, n, mL, 0, 1);
+#pragma omp parallel for schedule(static)
+    for (int jL = 0; jL <= mL + 1; ++jL)
+        for (int i = 1; i <= n; ++i) {
+            const double* c;
+            const int b = quad_base(i, &c);
+            double* dst = PCOL(&A, i, jL);
+            const double* w0 = PCOL(W, b, jL);
+            const double* w1 = PCOL(W, b + 1, jL);
+            const double* w2 = PCOL(W, b + 2, jL);
+            for (int k = 0; k < K; ++k) dst[k] = c[0] * w0[k] + c[1] * w1[k] + c[2] * w2[k];
+        }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= m; ++j) {
+        const double* c;
+        const int b = quad_base(j, &c);
+        for (int i = 1; i <= n; ++i) {
+            double* dst = PCOL(F, i, j);
+            const double* a0 = PCOL(&A, i, b);
+            const double* a1 = PCOL(&A, i, b + 1);
+            const double* a2 = PCOL(&A, i, b + 2);
+            for (int k = 0; k < K; ++k) dst[k] = c[0] * a0[k] + c[1] * a1[k] + c[2] * a2[k];
+        }
+    }
+    pf_free(&A);
+}
+
+/* ------------------------------------------------------------------------
  * Vertical filter (sup_vrbeta1_ens / sup_vrbeta1T_ens)
  * Column (i,j): W(kc, L) = V((kc-1)*lm + L, i, j) for kc=1..nvcol, L=1..lm.
  * ---------------------------------------------------------------------- */
@@ -695,32 +785,63 @@ void mgbfo_vertical(const mgbfo_params* P, int n, int m, const double* in, doubl
     pf_free(&a);
 }
 
-/* R = bocoT_2d(.,2,2) o adjoint (upsending_ens, mg_generations.f90:404-426) */
-static void restrict_pf(const pf_t* fine, pf_t* coarse_int /* (K,n/2,m/2,2,2) */) {
-    adjoint_op(fine, coarse_int);
-    bocoT_2d(coarse_int, 2, 2);
+/* R = bocoT_2d(.,2,2) o adjoint (upsending_ens, mg_generations.f90:404-426),
+   or with quad R2 = bocoT_2d(.,1,1) o adjoint2 (upsending2_ens,
+   mg_generations.f90:497-547). */
+static void restrict_pf(const pf_t* fine, pf_t* coarse_int /* (K,n/2,m/2,2,2) */, int quad) {
+    if (quad) {
+        adjoint2_op(fine, coarse_int);
+        bocoT_2d(coarse_int, 1, 1);
+    } else {
+        adjoint_op(fine, coarse_int);
+        bocoT_2d(coarse_int, 2, 2);
+    }
 }
 
-void mgbfo_restrict(int K, int n, int m, const double* fine, double* coarse) {
+/* P = direct1 o boco_2d(.,2,2) (downsending_ens, mg_generations.f90:461-491),
+   or with quad P2 = direct2 o boco_2d(.,1,1) (downsending2_ens,
+   mg_generations.f90:550-609). coarse has a halo of 2, fine none. */
+static void prolong_pf(pf_t* coarse, pf_t* fine, int quad) {
+    if (quad) {
+        boco_2d(coarse, 1, 1);
+        direct2_op(coarse, fine);
+    } else {
+        boco_2d(coarse, 2, 2);
+        direct1_op(coarse, fine);
+    }
+}
+
+static void restrict_any(int K, int n, int m, const double* fine, double* coarse, int quad) {
     pf_t F = pf_alloc(K, n, m, 0, 0);
     pf_from_interior(&F, fine);
     pf_t W = pf_alloc(K, n / 2, m / 2, 2, 2);
-    restrict_pf(&F, &W);
+    restrict_pf(&F, &W, quad);
     pf_to_interior(&W, coarse);
     pf_free(&F);
     pf_free(&W);
 }
 
-/* P = direct1 o boco_2d(.,2,2) (downsending_ens, mg_generations.f90:461-491) */
-void mgbfo_prolong(int K, int n, int m, const double* coarse, double* fine) {
+static void prolong_any(int K, int n, int m, const double* coarse, double* fine, int quad) {
     pf_t W = pf_alloc(K, n / 2, m / 2, 2, 2);
     pf_from_interior(&W, coarse);
-    boco_2d(&W, 2, 2);
     pf_t F = pf_alloc(K, n, m, 0, 0);
-    direct1_op(&W, &F);
+    prolong_pf(&W, &F, quad);
     pf_to_interior(&F, fine);
     pf_free(&W);
     pf_free(&F);
+}
+
+void mgbfo_restrict(int K, int n, int m, const double* fine, double* coarse) {
+    restrict_any(K, n, m, fine, coarse, 0);
+}
+void mgbfo_prolong(int K, int n, int m, const double* coarse, double* fine) {
+    prolong_any(K, n, m, coarse, fine, 0);
+}
+void mgbfo_restrict2(int K, int n, int m, const double* fine, double* coarse) {
+    restrict_any(K, n, m, fine, coarse, 1);
+}
+void mgbfo_prolong2(int K, int n, int m, const double* coarse, double* fine) {
+    prolong_any(K, n, m, coarse, fine, 1);
 }
 
 void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded) {
@@ -779,7 +900,7 @@ static void upsweep_inputs(const mgbfo_params* P, const double* x, pf_t* G) {
         for (int j = 1; j <= Fi.m; ++j)
             for (int i = 1; i <= Fi.n; ++i)
                 memcpy(PCOL(&Fi, i, j), PCOL(&G[g - 1], i, j), sizeof(double) * P->K);
-        restrict_pf(&Fi, &Wi);
+        restrict_pf(&Fi, &Wi, P->lquart);
         for (int j = 1; j <= Wi.m; ++j)
             for (int i = 1; i <= Wi.n; ++i)
                 memcpy(PCOL(&G[g], i, j), PCOL(&Wi, i, j), sizeof(double) * P->K);
@@ -796,9 +917,8 @@ static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y) {
         for (int j = 1; j <= Wi.m; ++j)
             for (int i = 1; i <= Wi.n; ++i)
                 memcpy(PCOL(&Wi, i, j), PCOL(&G[g], i, j), sizeof(double) * P->K);
-        boco_2d(&Wi, 2, 2);
         pf_t Fp = pf_alloc(P->K, G[g - 1].n, G[g - 1].m, 0, 0);
-        direct1_op(&Wi, &Fp);
+        prolong_pf(&Wi, &Fp, P->lquart);
         for (int j = 1; j <= Fp.m; ++j)
             for (int i = 1; i <= Fp.n; ++i) vadd(PCOL(&G[g - 1], i, j), PCOL(&Fp, i, j), P->K);
         pf_free(&Wi);

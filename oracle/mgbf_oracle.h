@@ -32,6 +32,8 @@ typedef struct mgbfo_params {
     int l_vertical_filter; /* apply the vertical 1-D filter */
     int fast;              /* filtering_fast_* (procedures 3, 8): the horizontal filter is the
                               product of 1-D beta filters along x and y */
+    int lquart;            /* quadratic inter-generation transfer (adjoint2 / direct2 with a
+                              1-point halo) instead of the 4-point one */
 } mgbfo_params;
 
 /* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
@@ -59,6 +61,9 @@ void mgbfo_vertical(const mgbfo_params* P, int n, int m,
 void mgbfo_restrict(int K, int n, int m, const double* fine, double* coarse);
 /* Prolongation P = interp o Pad_2: coarse (K, n/2, m/2) -> fine (K, n, m). */
 void mgbfo_prolong(int K, int n, int m, const double* coarse, double* fine);
+/* The quadratic pair (lquart): R2 = Fold_1 o adjoint2, P2 = direct2 o Pad_1. */
+void mgbfo_restrict2(int K, int n, int m, const double* fine, double* coarse);
+void mgbfo_prolong2(int K, int n, int m, const double* coarse, double* fine);
 /* Halo fill (boco_2d, half-sample mirror) of an interior array into a padded
    array (K, n+2h, m+2h); and its adjoint fold back to interior. */
 void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded);

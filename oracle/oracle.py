@@ -35,7 +35,7 @@ class _Params(ctypes.Structure):
         ("ampl01", ctypes.c_double), ("ampl02", ctypes.c_double),
         ("p_del", ctypes.c_double), ("p_eps", ctypes.c_double),
         ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
-        ("fast", ctypes.c_int),
+        ("fast", ctypes.c_int), ("lquart", ctypes.c_int),
     ]
 
 
@@ -56,6 +56,8 @@ def lib():
         L.mgbfo_vertical.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
         L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_restrict2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_prolong2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_pad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_fold.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_mg_size.argtypes = [P]; L.mgbfo_mg_size.restype = ctypes.c_size_t
@@ -126,11 +128,13 @@ class OracleParams:
     l_vertical_filter: bool = True
     weights: list = field(default_factory=lambda: [1.0, 0.5, 0.25, 0.125])
     fast: bool = False     # filtering_fast_* (procedures 3, 8)
+    lquart: bool = False   # quadratic inter-generation transfer (adjoint2 / direct2)
 
     def c(self) -> _Params:
         return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
                        self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
-                       self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast))
+                       self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast),
+                       int(self.lquart))
 
     def gen_shape(self, g: int):
         """(m_g, n_g, K) of generation g (0-based)."""
@@ -205,17 +209,19 @@ def vertical(P: OracleParams, x: np.ndarray, adjoint: bool = False) -> np.ndarra
     return y
 
 
-def restrict(x: np.ndarray) -> np.ndarray:
+def restrict(x: np.ndarray, quad: bool = False) -> np.ndarray:
+    """R (adjoint + bocoT_2d), or with quad R2 (adjoint2, lquart)."""
     x = _f64(x); m, n, K = x.shape
     y = np.empty((m // 2, n // 2, K))
-    lib().mgbfo_restrict(K, n, m, _dp(x), _dp(y))
+    (lib().mgbfo_restrict2 if quad else lib().mgbfo_restrict)(K, n, m, _dp(x), _dp(y))
     return y
 
 
-def prolong(xc: np.ndarray) -> np.ndarray:
+def prolong(xc: np.ndarray, quad: bool = False) -> np.ndarray:
+    """P (boco_2d + direct1), or with quad P2 (direct2, lquart)."""
     xc = _f64(xc); mc, nc, K = xc.shape
     y = np.empty((2 * mc, 2 * nc, K))
-    lib().mgbfo_prolong(K, 2 * nc, 2 * mc, _dp(xc), _dp(y))
+    (lib().mgbfo_prolong2 if quad else lib().mgbfo_prolong)(K, 2 * nc, 2 * mc, _dp(xc), _dp(y))
     return y
 
 

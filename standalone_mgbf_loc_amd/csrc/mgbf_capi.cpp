@@ -65,6 +65,7 @@ struct mgbf_ctx {
     int K = 0, gm = 0;
     int nfilt = 0;          // vertically filtered part of the stack (0: none)
     bool filt_g1 = true;
+    bool quad = false;      // lquart: quadratic inter-generation transfer
     mgbf::RadialCoef coef{};
     int nx[mgbf::kMaxGen] = {}, ny[mgbf::kMaxGen] = {};
     long long gsize[mgbf::kMaxGen] = {};
@@ -342,7 +343,7 @@ int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t
     // the restriction reads all of generation g: it records max |u_g| per k
     float* kmax = c->d_kmax ? c->d_kmax + (size_t)g * c->K : nullptr;
     mgbf::TransferArgs a{fine, coarse, kmax, c->K, c->nx[g], c->ny[g],
-                         c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g]};
+                         c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g], c->quad};
     if (kmax) c->kmax_valid |= 1u << g;
     HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
     return MGBF_OK;
@@ -353,7 +354,7 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
     Phase ph(c, MGBF_T_DNSEND, st,
              4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)));
     mgbf::TransferArgs a{coarse, fine, nullptr, c->K, c->nx[g], c->ny[g],
-                         c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g]};
+                         c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g], c->quad};
     HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
     return MGBF_OK;
 }
@@ -434,11 +435,13 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, fl
             bool sqrt_w, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band);
+    // the radial kernels fuse the 4-point prolongation only: with the
+    // quadratic one it runs on its own, and the band after it
+    const bool fuse = band >= 0 && c->fuse >= 2 && !c->quad && can_fuse_band(c, band);
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
-        const float* pro = c->fuse >= 1 ? coarse : nullptr;
+        const float* pro = c->fuse >= 1 && !c->quad ? coarse : nullptr;
         const bool direct = g == 0 && (fuse || band < 0);
         float* dst = direct ? y : U + c->goff[g];
         const int fb = (g == 0 && fuse) ? band : -1;
@@ -634,6 +637,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     // the background procedures filter generation 1 whatever l_filt_g1 says
     c->filt_g1 = cfg->mgbf_proc == 1 || cfg->mgbf_proc == 3 ? true : cfg->l_filt_g1 != 0;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
+    c->quad = cfg->lquart != 0;
     {
         // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for
         // weights that are constant along the stack
@@ -740,7 +744,9 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
         for (const auto* t : tabs)
             bytes += carve_size(sizeof(int) * t->n_out) + carve_size(sizeof(float) * t->w.size());
-        c->res_ok[g] = build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], rytab[g], rymeta[g]);
+        // the fused restriction carries the 4-point stencil only
+        c->res_ok[g] = !c->quad &&
+                       build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], rytab[g], rymeta[g]);
         bytes += carve_size(sizeof(float) * rytab[g].size()) + carve_size(sizeof(int) * rymeta[g].size());
     }
     {
@@ -957,6 +963,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
         const int n = c->nx[g], m = c->ny[g];
         for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
+            if ((kind == 1 || kind == 2) && c->quad) continue;   // never run with lquart
             if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
             if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 4)) continue;
             const float* coarse = kind == 1 || kind == 2 ? c->ws_u + c->goff[g + 1] : nullptr;

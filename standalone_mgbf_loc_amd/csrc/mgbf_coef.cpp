@@ -7,9 +7,10 @@
 // src.mgbf/jp_pbfil.f90:90-107), the taps are the lattice points the
 // reference's vrbeta2 loop visits (jp_pbfil.f90:909-926) and the line-sum
 // normalizer is getlinesum2 (jp_pbfil.f90:182-218). The transfer constants
-// are those of def_offset_coef (src.mgbf/mg_interpolate.f90:76-79) applied as
+// are those of def_offset_coef (src.mgbf/mg_interpolate.f90:76-84) applied as
 // direct1 / adjoint do (src.mgbf/mg_generations.f90:1117-1250) with the
-// 2-point half-sample mirror of boco_2d / bocoT_2d.
+// 2-point half-sample mirror of boco_2d / bocoT_2d, or with lquart as
+// direct2 / adjoint2 do (mg_generations.f90:1253-1374) with a 1-point one.
 #include "mgbf_coef.hpp"
 
 #include <algorithm>
@@ -23,6 +24,8 @@ constexpr double kEps = 1.e-12;  // tap-boundary tolerance of jp_pbfil.f90
 
 const double kPCoef[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
 const double kQCoef[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
+const double kACoef[3] = {5. / 32., 30. / 32., -3. / 32.};   // a_coef
+const double kBCoef[3] = {-3. / 32., 30. / 32., 5. / 32.};   // b_coef
 
 double ipow(double x, int p) {
     double r = 1.0;
@@ -66,17 +69,22 @@ int mirror_whole(int q, int n) {  // 0-based whole-sample mirror
 }
 
 // Prolongation along one axis, fine nf <- coarse nf/2 (dense, FP64), with a
-// flag matrix of structural nonzeros.
-void interp_dense(int nf, std::vector<double>& P, std::vector<char>& nz) {
+// flag matrix of structural nonzeros. 4-point (direct1): fine i1 (1-based)
+// reads coarse iL-1 .. iL+2, iL = i1/2, padded by boco_2d(.,2,2). Quadratic
+// (direct2): odd i1 reads coarse iL-1 .. iL+1 with a_coef, iL = (i1+1)/2,
+// even i1 the same with b_coef, iL = i1/2, padded by boco_2d(.,1,1).
+void interp_dense(int nf, bool quad, std::vector<double>& P, std::vector<char>& nz) {
     const int nc = nf / 2;
     P.assign((size_t)nf * nc, 0.0);
     nz.assign((size_t)nf * nc, 0);
     for (int i1 = 1; i1 <= nf; ++i1) {
-        const int iL = i1 / 2;
-        const double* c = (i1 & 1) ? kQCoef : kPCoef;
-        for (int t = 0; t < 4; ++t) {
-            const int I1 = iL - 1 + t;                 // 1-based coarse, -1 .. nc+2
-            const int I0 = mirror_half(I1 - 1, nc);    // boco_2d(.,2,2) pad
+        const bool odd = (i1 & 1) != 0;
+        const int np = quad ? 3 : 4;
+        const int lo = quad ? (odd ? (i1 + 1) / 2 : i1 / 2) - 1 : i1 / 2 - 1;
+        const double* c = quad ? (odd ? kACoef : kBCoef) : (odd ? kQCoef : kPCoef);
+        for (int t = 0; t < np; ++t) {
+            const int I1 = lo + t;                     // 1-based coarse
+            const int I0 = mirror_half(I1 - 1, nc);    // boco_2d pad
             P[(size_t)(i1 - 1) * nc + I0] += c[t];
             nz[(size_t)(i1 - 1) * nc + I0] = 1;
         }
@@ -332,10 +340,6 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "l_loc=.true. (localization generations) is not built";
         return MGBF_ERR_UNSUPPORTED;
     }
-    if (c.lquart) {
-        *err = "lquart=.true. (quadratic inter-generation transfer) is not built";
-        return MGBF_ERR_UNSUPPORTED;
-    }
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
     const bool bckg = c.mgbf_proc == 1 || c.mgbf_proc == 3;
@@ -538,7 +542,7 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         std::vector<char> nz;
         for (int axis = 0; axis < 2; ++axis) {
             const int nf = axis == 0 ? s.nx[g] : s.ny[g];
-            interp_dense(nf, P, nz);
+            interp_dense(nf, cfg.lquart != 0, P, nz);
             AxisTable& pt = axis == 0 ? s.px[g] : s.py[g];
             AxisTable& rt = axis == 0 ? s.rx[g] : s.ry[g];
             if (!pack_rows(nf, nf / 2, P, nz, false, kPWidth, pt) ||

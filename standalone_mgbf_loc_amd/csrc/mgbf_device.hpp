@@ -90,4 +90,35 @@ constexpr EdgeRW make_edge_rw() {
 }
 constexpr EdgeRW kEdgeRW = make_edge_rw();
 
+// Inter-generation transfer along one axis, as a compile-time stencil.
+// Fine index i (0-based, u = i >> 1) reads coarse u + OFF[i & 1] + t,
+// t < NP, with weight C[i & 1][t], the coarse indices mirrored (half-sample)
+// into the domain. The interior restriction onto coarse I reads fine
+// 2I + ROFF + t, t < NR, with weight R[t]; coarse indices within EDGE of a
+// domain edge also receive mirrored contributions.
+//   Stencil<false>: the 4-point offset constants (direct1 / adjoint)
+//   Stencil<true>:  the quadratic a_coef / b_coef (direct2 / adjoint2)
+template <bool Q>
+struct Stencil;
+
+template <>
+struct Stencil<false> {
+    static constexpr int NP = 4, NR = 8, ROFF = -3, EDGE = 2;
+    static constexpr int OFF[2] = {-2, -1};
+    static constexpr float C[2][4] = {{kQc[0], kQc[1], kQc[2], kQc[3]},
+                                      {kPc[0], kPc[1], kPc[2], kPc[3]}};
+    static constexpr float R[8] = {kW8[0], kW8[1], kW8[2], kW8[3],
+                                   kW8[4], kW8[5], kW8[6], kW8[7]};
+};
+
+template <>
+struct Stencil<true> {
+    static constexpr int NP = 3, NR = 6, ROFF = -2, EDGE = 1;
+    static constexpr int OFF[2] = {-1, -1};
+    static constexpr float C[2][3] = {{5.f / 32.f, 30.f / 32.f, -3.f / 32.f},
+                                      {-3.f / 32.f, 30.f / 32.f, 5.f / 32.f}};
+    static constexpr float R[6] = {-3.f / 32.f, 5.f / 32.f, 30.f / 32.f,
+                                   30.f / 32.f, 5.f / 32.f, -3.f / 32.f};
+};
+
 }  // namespace mgbf

@@ -117,16 +117,19 @@ vband_kernel(const float* __restrict__ in, float* __restrict__ out,
 
 // ---------------------------------------------------------------------------
 // Prolongation P (coarse -> fine), optionally accumulated: the coarse field is
-// read through the 2-point half-sample mirror (boco_2d(.,2,2)) and
-// interpolated with the 4-point constants, x then y (direct1). One lane
-// computes TJ fine rows of one fine column for its k; the TJ/2+4 x-interpolated
-// coarse rows it needs stay in registers.
+// read through the half-sample mirror (boco_2d) and interpolated with the
+// stencil S, x then y (direct1, or direct2 for the quadratic stencil). One
+// lane computes TJ fine rows of one fine column for its k; the NR
+// x-interpolated coarse rows they need stay in registers.
 // ---------------------------------------------------------------------------
-template <int TJ>
+template <int TJ, bool Q>
 __global__ void __launch_bounds__(kBlock)
 prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* out, int K, int nf,
                 int mf, int nkc, int nstrip, int nitems) {
-    constexpr int NR = TJ / 2 + 4;
+    using S = Stencil<Q>;
+    constexpr int NP = S::NP;
+    constexpr int CO = S::OFF[0] < S::OFF[1] ? S::OFF[0] : S::OFF[1];  // lowest coarse offset
+    constexpr int NR = TJ / 2 + NP - 1 + (S::OFF[1] - CO);
     const int lane = threadIdx.x & (kWave - 1);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + (threadIdx.x >> 6);
     if (item >= nitems) return;
@@ -139,26 +142,26 @@ prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* ou
     const int kk = kvalid ? k : K - 1;
     const int nc = nf / 2, mc = mf / 2;
     const int u = i >> 1;
-    const bool odd = (i & 1) != 0;
-    const int cs = odd ? u - 1 : u - 2;
-    float cw[4];
-    int xo[4];
+    const int par = i & 1;
+    const int cs = u + (par ? S::OFF[1] : S::OFF[0]);
+    float cw[NP];
+    int xo[NP];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        cw[t] = odd ? kPc[t] : kQc[t];
+    for (int t = 0; t < NP; ++t) {
+        cw[t] = par ? S::C[1][t] : S::C[0][t];
         xo[t] = mirror_half(cs + t, nc) * K;
     }
     const int j0 = js * TJ;
     const int u0 = j0 >> 1;
     const size_t crow = (size_t)nc * K;
-    float xv[NR];
+    float xv[NR];   // coarse rows u0 + CO .. u0 + CO + NR - 1, interpolated along x
 #pragma unroll
     for (int t = 0; t < NR; ++t) {
-        const int r = mirror_half(u0 - 2 + t, mc);
+        const int r = mirror_half(u0 + CO + t, mc);
         const float* row = coarse + kk + (size_t)r * crow;
         float a = 0.f;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) a = __builtin_fmaf(cw[c], row[xo[c]], a);
+        for (int c = 0; c < NP; ++c) a = __builtin_fmaf(cw[c], row[xo[c]], a);
         xv[t] = a;
     }
     if (!kvalid) return;
@@ -167,13 +170,11 @@ prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* ou
     for (int q = 0; q < TJ; ++q) {
         const int j = j0 + q;
         if (j < mf) {
-            float a;
-            if ((q & 1) == 0)
-                a = kQc[0] * xv[q / 2] + kQc[1] * xv[q / 2 + 1] + kQc[2] * xv[q / 2 + 2] +
-                    kQc[3] * xv[q / 2 + 3];
-            else
-                a = kPc[0] * xv[(q + 1) / 2] + kPc[1] * xv[(q + 1) / 2 + 1] +
-                    kPc[2] * xv[(q + 1) / 2 + 2] + kPc[3] * xv[(q + 1) / 2 + 3];
+            const int pq = q & 1;
+            const int b = (q >> 1) + (pq ? S::OFF[1] : S::OFF[0]) - CO;
+            float a = 0.f;
+#pragma unroll
+            for (int t = 0; t < NP; ++t) a = __builtin_fmaf(pq ? S::C[1][t] : S::C[0][t], xv[b + t], a);
             const size_t e = kk + (size_t)j * frow + (size_t)i * K;
             out[e] = (addend ? addend[e] : 0.f) + a;
         }
@@ -182,12 +183,13 @@ prolong2_kernel(const float* __restrict__ coarse, const float* addend, float* ou
 
 
 // ---------------------------------------------------------------------------
-// Restriction R = Fold_2 o P^T (fine -> coarse). One lane computes TI2 coarse
+// Restriction R = Fold o P^T (fine -> coarse). One lane computes TI2 coarse
 // columns x TJ2 coarse rows for its k, coarse row by coarse row. An interior
-// coarse row J is the 8-point stencil kW8 over the x-restricted fine rows
-// 2J-3 .. 2J+4, which a sliding window keeps in registers: the next row
-// needs two new fine rows. Rows and columns within two of an edge carry the
-// folded mirror contributions and use the window tables.
+// coarse row J is the NR-point stencil S::R over the x-restricted fine rows
+// 2J+ROFF .. 2J+ROFF+NR-1 (2J-3 .. 2J+4 for the 4-point stencil, 2J-2 ..
+// 2J+3 for the quadratic one), which a sliding window keeps in registers: the
+// next row needs two new fine rows. Rows and columns within S::EDGE of an
+// edge carry the folded mirror contributions and use the window tables.
 // ---------------------------------------------------------------------------
 // Edge tile of the restriction: window tables in x and y, one coarse point
 // at a time (only the tiles within two columns of an edge come here).
@@ -226,14 +228,16 @@ __device__ void restrict_edge_tile(const float* __restrict__ fine, float* __rest
     }
 }
 
-template <int TI2, int TJ2>
+template <int TI2, int TJ2, bool Q>
 __global__ void __launch_bounds__(kBlock)
 restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
                  int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
                  const int* __restrict__ rys, const float* __restrict__ ryw, float* kmax,
                  int nkc, int ntile_i, int nstrip, int nitems) {
     constexpr int WR = kRWidthDev;
-    constexpr int NCF = 2 * TI2 + 6;
+    using S = Stencil<Q>;
+    constexpr int NRW = S::NR, E = S::EDGE;
+    constexpr int NCF = 2 * TI2 + NRW - 2;
     const int lane = threadIdx.x & (kWave - 1);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
                      __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -252,16 +256,16 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
     const int I0 = ti * TI2, J0 = js * TJ2;
     const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
     float lmax = 0.f;   // max |fine| read by this lane (its k)
-    if (!(I0 >= 2 && I0 + TI2 <= nc - 2)) {
+    if (!(I0 >= E && I0 + TI2 <= nc - E)) {
         restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw,
                                 lmax);
         if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));
         return;
     }
     const size_t frow = (size_t)nf * K;
-    // interior columns: fine columns 2*I0-3 .. 2*I0+2*TI2+2, addressed as the
-    // row's buffer + lane k + wave-uniform column offset
-    const float* base = fine + (size_t)(2 * I0 - 3) * K;
+    // interior columns: fine columns 2*I0+ROFF .. 2*I0+ROFF+NCF-1, addressed
+    // as the row's buffer + lane k + wave-uniform column offset
+    const float* base = fine + (size_t)(2 * I0 + S::ROFF) * K;
     const int vk = kk * 4;
     auto xrow = [&](int fr, float* out) {
         const __amdgpu_buffer_rsrc_t rs = buf_rsrc(base + (size_t)fr * frow);
@@ -275,33 +279,33 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
         for (int c = 0; c < TI2; ++c) {
             float a = 0.f;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[2 * c + t], a);
+            for (int t = 0; t < NRW; ++t) a = __builtin_fmaf(S::R[t], v[2 * c + t], a);
             out[c] = a;
         }
     };
-    float win[8][TI2];   // x-restricted fine rows 2J-3 .. 2J+4
-    int wbase = -1000;   // fine row held in win[0]
+    float win[NRW][TI2];   // x-restricted fine rows 2J+ROFF .. 2J+ROFF+NRW-1
+    int wbase = -1000;     // fine row held in win[0]
     for (int J = J0; J < J1; ++J) {
         float acc[TI2];
-        if (J >= 2 && J <= mc - 3) {
-            const int fb = 2 * J - 3;
+        if (J >= E && J <= mc - 1 - E) {
+            const int fb = 2 * J + S::ROFF;
             if (fb == wbase + 2) {
 #pragma unroll
-                for (int t = 0; t < 6; ++t)
+                for (int t = 0; t < NRW - 2; ++t)
 #pragma unroll
                     for (int c = 0; c < TI2; ++c) win[t][c] = win[t + 2][c];
-                xrow(fb + 6, win[6]);
-                xrow(fb + 7, win[7]);
+                xrow(fb + NRW - 2, win[NRW - 2]);
+                xrow(fb + NRW - 1, win[NRW - 1]);
             } else {
 #pragma unroll
-                for (int t = 0; t < 8; ++t) xrow(fb + t, win[t]);
+                for (int t = 0; t < NRW; ++t) xrow(fb + t, win[t]);
             }
             wbase = fb;
 #pragma unroll
             for (int c = 0; c < TI2; ++c) {
                 float a = 0.f;
 #pragma unroll
-                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], win[t][c], a);
+                for (int t = 0; t < NRW; ++t) a = __builtin_fmaf(S::R[t], win[t][c], a);
                 acc[c] = a;
             }
         } else {
@@ -406,9 +410,14 @@ hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)ntile_i * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
-                       a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc, ntile_i,
-                       nstrip, (int)nitems);
+    if (a.quad)
+        hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2, true>), dim3(nblocks), dim3(kBlock), 0, st,
+                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc,
+                           ntile_i, nstrip, (int)nitems);
+    else
+        hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2, false>), dim3(nblocks), dim3(kBlock), 0, st,
+                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc,
+                           ntile_i, nstrip, (int)nitems);
     return hipGetLastError();
 }
 
@@ -418,8 +427,12 @@ hipError_t launch_prolong(const TransferArgs& a, const float* addend, hipStream_
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)a.nf * nstrip * nkc;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL(prolong2_kernel<TJ>, dim3(nblocks), dim3(kBlock), 0, st, a.src, addend,
-                       a.dst, a.K, a.nf, a.mf, nkc, nstrip, (int)nitems);
+    if (a.quad)
+        hipLaunchKernelGGL((prolong2_kernel<TJ, true>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
+                           addend, a.dst, a.K, a.nf, a.mf, nkc, nstrip, (int)nitems);
+    else
+        hipLaunchKernelGGL((prolong2_kernel<TJ, false>), dim3(nblocks), dim3(kBlock), 0, st, a.src,
+                           addend, a.dst, a.K, a.nf, a.mf, nkc, nstrip, (int)nitems);
     return hipGetLastError();
 }
 

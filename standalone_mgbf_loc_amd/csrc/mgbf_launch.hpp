@@ -89,6 +89,7 @@ struct TransferArgs {
     const float* xw;      // window weights along x
     const int* ys;
     const float* yw;
+    bool quad;            // quadratic stencil (direct2 / adjoint2) instead of the 4-point one
 };
 
 constexpr int kMaxVBand = 20;   // half-band of Vf Vf^T for hz <= 10

@@ -8,7 +8,9 @@ the mathematical definitions rather than from the reference's loop structure:
   - half-sample mirror for horizontal halos (mg_bocos.f90:225-262, 340-377),
     whole-sample mirror for the vertical column (mg_filtering.f90:2072-2075);
   - interpolation P from the constants of mg_interpolate.f90:76-79 applied on
-    the offset grids as in mg_generations.f90:1190-1250;
+    the offset grids as in mg_generations.f90:1190-1250; with quad (lquart)
+    the 3-point Lagrange interpolation through the coarse points nearest to
+    each fine point, its weights computed from the point positions;
   - adjoints are plain matrix transposes.
 Flat index of a (m, n, K) C-order field: (j*n + i)*K + k.
 """
@@ -96,11 +98,40 @@ def interp_matrix_1d(nf: int) -> np.ndarray:
     return P
 
 
+def lagrange_weights(nodes, x):
+    """Lagrange interpolation weights of the nodes at the point x."""
+    w = []
+    for a in nodes:
+        v = 1.0
+        for b in nodes:
+            if b != a:
+                v *= (x - b) / (a - b)
+        w.append(v)
+    return w
+
+
+def quad_interp_matrix_1d(nf: int) -> np.ndarray:
+    """Quadratic P along one axis: coarse point I sits at 2I + 1/2 in fine
+    units (0-based fine i at i). Fine i is interpolated through the coarse
+    point u = i // 2 nearest to it and its two neighbours; coarse indices
+    outside the domain are the half-sample mirror of those inside."""
+    nc = nf // 2
+    P = np.zeros((nf, nc))
+    for i in range(nf):
+        u = i // 2
+        nodes = [u - 1, u, u + 1]
+        w = lagrange_weights([2.0 * c + 0.5 for c in nodes], float(i))
+        for c, wc in zip(nodes, w):
+            P[i, mirror_half(c, nc)] += wc
+    return P
+
+
 class DenseMGBF:
     """Dense matrices of every stage for one tiny problem."""
 
     def __init__(self, nx, ny, K, lm, nvcol, hx, hz, p, gm, R2, R2v,
-                 weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False):
+                 weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False,
+                 quad=False):
         self.nx, self.ny, self.K, self.gm = nx, ny, K, gm
         if fast:
             # separable: the outer product of two 1-D beta filters whose
@@ -127,8 +158,9 @@ class DenseMGBF:
                 self.A.append(weights[g] * np.eye(n * m * K))
             if g > 0:
                 # P: generation g -> generation g-1
-                Px = interp_matrix_1d(nx >> (g - 1))
-                Py = interp_matrix_1d(ny >> (g - 1))
+                interp = quad_interp_matrix_1d if quad else interp_matrix_1d
+                Px = interp(nx >> (g - 1))
+                Py = interp(ny >> (g - 1))
                 self.P.append(np.kron(np.kron(Py, Px), IK))
         if not l_filt_g1:
             N1 = nx * ny * K

@@ -77,7 +77,7 @@ def test_nm0_divisibility_is_checked():
     assert "not divisible" in str(_err(cfg))
 
 
-@pytest.mark.parametrize("key,val", [("mgbf_line", True), ("l_loc", True), ("lquart", True),
+@pytest.mark.parametrize("key,val", [("mgbf_line", True), ("l_loc", True),
                                      ("mgbf_proc", 2), ("mgbf_proc", 4), ("mgbf_proc", 7)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)

@@ -55,13 +55,15 @@ def test_python_namelist_roundtrip_and_integer_p(tmp_path):
 
 
 @pytest.mark.gpu
-def test_fortran_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("lquart", [False, True])
+def test_fortran_driver_matches_oracle(tmp_path, lquart):
     torch = pytest.importorskip("torch")
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
     from oracle import oracle as O
     _need(DRIVER)
-    cfg = M.Config.for_grid(96, 96, 16, n_ens=2, gm=3, h=6, hz=3, weights=[1.0, 0.5, 0.25])
+    cfg = M.Config.for_grid(96, 96, 16, n_ens=2, gm=3, h=6, hz=3, weights=[1.0, 0.5, 0.25],
+                            lquart=lquart)
     d = M.derive(cfg)
     write_namelist(cfg, str(tmp_path / "mgbeta.nml"))
     # WORKA(km_a_all, nm0, mm0) in Fortran order == C array (mm0, nm0, km_a_all)
@@ -73,7 +75,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
     y = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(d.gen_shape(1))
     P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=6, hy=6,
                        hz=3, p=2, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
-                       weights=[1.0, 0.5, 0.25])
+                       weights=[1.0, 0.5, 0.25], lquart=lquart)
     # the driver rounds the input to FP32 at the device boundary
     ref = O.wawt(P, x.astype(np.float32).astype(np.float64))
     assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5

@@ -2,7 +2,9 @@
 
 Tolerance (FP32 product vs FP64 oracle): normwise relative error
 max|y - y64| / max|y64| <= 1e-5 for whole operators, 2e-6 for single stages;
-adjoint dot-product mismatch |<Wv,x> - <v,W^T x>| <= 1e-6 * |<Wv,x>|.
+adjoint dot-product mismatch |<Wv,x> - <v,W^T x>| <= 1e-6 * |<Wv,x>|, or
+<= 1e-6 * |Wv| |x| (the Cauchy-Schwarz scale) where random x and v make
+<Wv,x> cancel down to a small fraction of that scale.
 """
 import numpy as np
 import pytest
@@ -34,6 +36,7 @@ def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
         hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
         l_filt_g1=True if bckg else bool(cfg.l_filt_g1),
         l_vertical_filter=bool(cfg.l_vertical_filter), fast=cfg.mgbf_proc in (3, 8),
+        lquart=bool(cfg.lquart),
         weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
 
 
@@ -129,6 +132,25 @@ def test_transfer_stages_match_oracle(K, n, m):
             sync()
             assert relerr(r, O.restrict(xf.astype(np.float64))) <= TOL_STAGE
             assert relerr(p, O.prolong(xc.astype(np.float64))) <= TOL_STAGE
+
+
+@pytest.mark.parametrize("K,n,m", [(64, 96, 96), (33, 64, 48), (128, 32, 16), (7, 32, 16)])
+def test_quadratic_transfer_stages_match_oracle(K, n, m):
+    # lquart: direct2 / adjoint2 with the 1-point mirror halo; on the
+    # smallest grids every restriction column tile touches an edge
+    cfg = M.Config(mg_ampl01=16 / 7, mg_ampl02=16 / 8, hx=3, hy=3, hz=3, lm_a=K, lm=K,
+                   gm_max=3, nm0=n, mm0=m, nxPE=4, nyPE=4, im_filt=n // 4, jm_filt=m // 4,
+                   lquart=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        for gen in (1, 2):
+            xf = field(d.gen_shape(gen), 120 + gen)
+            xc = field(d.gen_shape(gen + 1), 130 + gen)
+            r = op.stage(M.STAGE_R, gen, to_dev(xf))
+            p = op.stage(M.STAGE_P, gen, to_dev(xc))
+            sync()
+            assert relerr(r, O.restrict(xf.astype(np.float64), quad=True)) <= TOL_STAGE
+            assert relerr(p, O.prolong(xc.astype(np.float64), quad=True)) <= TOL_STAGE
 
 
 # --------------------------------------------------------------------------
@@ -308,6 +330,41 @@ def test_w_and_wt_match_oracle_and_pass_the_dot_product_test():
         y1 = op.filtering_procedure(xd)
         sync()
         assert (y1 - y2).abs().max().item() <= 1e-5 * y1.abs().max().item()
+
+
+@pytest.mark.parametrize("case", ["ens", "ens_filt_g1_off", "bckg", "fast", "fuse0"])
+def test_lquart_procedures_match_oracle(monkeypatch, case):
+    # upsending2_ens / downsending2_ens: the quadratic transfer between
+    # generations in every procedure; W and W^T stay adjoint
+    if case == "fuse0":
+        monkeypatch.setenv("MGBF_FUSE", "0")
+    kw = dict(n_ens=2, gm=4, h=6, hz=3, weights=[1.0, 0.6, 0.3, 0.15], lquart=True)
+    if case == "ens_filt_g1_off":
+        kw["l_filt_g1"] = False
+    if case == "bckg":
+        kw.update(n_ens=1, km2=3, km3=2, mgbf_proc=1)
+    if case == "fast":
+        kw.update(mgbf_proc=8, p=1)
+    cfg = M.Config.for_grid(128, 96, 16, **kw)
+    if case == "fast":
+        cfg.mg_ampl02 = 1.0
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 140)
+        v = np.random.default_rng(141).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs = op.dot(w, xd)
+        rhs = op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
 
 
 def test_dot_is_fp64_accumulated():

@@ -159,7 +159,8 @@ def _dense_for(P, wfield=None):
     return D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm,
                        Fraction(P.ampl02).limit_denominator(1000) * (2 * P.p + 4),
                        Fraction(P.ampl01).limit_denominator(1000) * (2 * P.p + 3),
-                       list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, wfield)
+                       list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, wfield,
+                       quad=P.lquart)
 
 
 def _oracle_matrix(P, fn, N):
@@ -357,3 +358,75 @@ def test_fast_wawt_matches_dense_model(filt_g1):
                      list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, None, fast=True)
     assert np.max(np.abs(B - Dm.B())) < 1e-13
 
+
+
+# --------------------------------------------------------------------------
+# quadratic inter-generation transfer (lquart: adjoint2 / direct2)
+# --------------------------------------------------------------------------
+
+def test_quadratic_prolong_restrict_adjoint():
+    xc, yf = _rand((6, 10, 3), 81), _rand((12, 20, 3), 82)
+    lhs = np.vdot(O.prolong(xc, quad=True), yf)
+    rhs = np.vdot(xc, O.restrict(yf, quad=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_quadratic_prolong_reproduces_quadratics_inside_and_constants_everywhere():
+    # coarse point I sits at 2I + 1/2 in fine units: a 3-point interpolation
+    # is exact for every polynomial of degree 2 away from the 1-point halo
+    nc = 10
+    X = 2.0 * np.arange(nc) + 0.5
+
+    def q(x, y):
+        return 0.3 * x * x - 1.1 * x * y + 0.7 * y * y + 2.0 * x - y + 5.0
+
+    coarse = q(X[None, :], X[:, None])[:, :, None]
+    fine = O.prolong(coarse, quad=True)
+    xi = np.arange(2 * nc, dtype=float)
+    assert np.allclose(fine[2:-2, 2:-2, 0], q(xi[None, 2:-2], xi[2:-2, None]), atol=1e-11)
+    # but not for cubics (the stencil has 3 points, not 4)
+    cub = (X[None, :] ** 3 + 0 * X[:, None])[:, :, None]
+    assert np.abs(O.prolong(cub, quad=True)[2:-2, 2:-2, 0] - xi[None, 2:-2] ** 3).max() > 1e-3
+    # the mirror halo keeps constants exact up to the edges
+    assert np.allclose(O.prolong(np.ones((5, 7, 2)), quad=True), 1.0, atol=1e-14)
+
+
+def test_quadratic_prolong_edges_are_the_half_sample_mirror():
+    # the half-sample mirror of a coarse field c is the even extension
+    # [c reversed, c, c reversed]: prolonging that on a 3x wider domain and
+    # keeping its middle third (interior there) gives the edge rows here
+    nc, mc = 6, 4
+    c = _rand((mc, nc, 2), 83)
+    ext = np.concatenate([c[:, ::-1], c, c[:, ::-1]], axis=1)
+    ext = np.concatenate([ext[::-1], ext, ext[::-1]], axis=0)
+    big = O.prolong(ext, quad=True)
+    mid = big[2 * mc:4 * mc, 2 * nc:4 * nc]
+    assert np.allclose(O.prolong(c, quad=True), mid, atol=1e-13)
+
+
+def test_quadratic_transfer_differs_from_the_four_point_one():
+    x = _rand((16, 16, 2), 84)
+    assert np.abs(O.restrict(x, quad=True) - O.restrict(x)).max() > 1e-3
+
+
+@pytest.mark.parametrize("filt_g1", [True, False])
+def test_lquart_wawt_matches_dense_model(filt_g1):
+    P = _P(nx=16, ny=12, K=4, lm=2, nvcol=2, hx=3, hy=3, hz=1, gm=3,
+           ampl02=Fraction(16, 8), ampl01=Fraction(4, 7), l_filt_g1=filt_g1,
+           weights=[1.0, 0.7, 0.3], lquart=True)
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    Bd = _dense_for(P).B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+    assert np.max(np.abs(B - B.T)) < 1e-13
+
+
+def test_lquart_w_and_wt_are_adjoint():
+    P = _P(nx=16, ny=16, K=3, lm=3, nvcol=1, gm=3, lquart=True)
+    x = _rand(P.gen_shape(0), 85)
+    vt = O.wt(P, x)
+    v = _rand(vt.shape, 86)
+    lhs = np.vdot(O.w(P, v), x)
+    rhs = np.vdot(v, vt)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
