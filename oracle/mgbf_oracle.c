@@ -17,6 +17,11 @@
  *   - adjoint / direct1 (R^T-side, P) .......... mg_generations.f90:1117-1187, 1190-1250
  *   - def_offset_coef constants ................ mg_interpolate.f90:61-86
  *   - adjoint2 / direct2 (lquart) .............. mg_generations.f90:1253-1374
+ *   - inimomtab, hstform, blinfil .............. jp_pbfil3.f90:378-457, 1399-1430, 1463-1501
+ *   - triad / tritform ......................... jp_pbfil3.f90:496-532, 573-624
+ *   - vdibeta2 / vdibeta2t ..................... jp_pbfil3.f90:1801-1837, 2345-2382
+ *   - init_mg_line ............................. mg_intstate.f90:436-480
+ *   - filtering_lin2_ens / _bckg composition ... mg_filtering.f90:1150-1313, 683-822
  *   - upsending2_ens / downsending2_ens ........ mg_generations.f90:497-609
  *   - upsending_ens / downsending_ens .......... mg_generations.f90:382-494
  *   - weighting_ens ............................ mg_generations.f90:1012-1052
@@ -728,52 +733,292 @@ static void fast_filter(const fast_t* F, pf_t* a, int adjoint) {
     }
 }
 
+/* ------------------------------------------------------------------------
+ * Triad line filters (procedures 2 and 7, filtering_lin2_bckg / _ens).
+ * init_mg_line resolves the aspect 3-vector of each point onto three
+ * lattice lines (triad), turns each line's spread**2 into a real half-span
+ * (hstform), and each line is filtered by the "dibeta" mixture of the beta
+ * filters of the two neighbouring integer half-spans (blinfil), gathered
+ * along the line (vdibeta2) or scattered (vdibeta2t).
+ * ---------------------------------------------------------------------- */
+#define LINE_NH 6     /* nh, mg_parameter.f90:562 */
+#define LINE_NFIL 8   /* nfil = nh + 2, mg_parameter.f90:563 */
+
+/* inimomtab (jp_pbfil3.f90:378-457): for the beta exponent p and integer
+   half-spans h = 1..nh, bnorm[h] = 1 / (0th moment) and bsprds[h] =
+   (2nd moment) / (0th moment), from the continuum moments of (1-(x/h)^2)^p
+   plus their Euler-Maclaurin corrections (polynomials n0pk, n2pk in h^2).
+   Arrays are indexed 1..nh. Returns 1 for p or nh out of range. */
+int mgbfo_momtab(int p, int nh, double* bnorm, double* bsprds) {
+    static const double n0pk[6][3] = {
+        {-1, 0, 0}, {-1, 0, 0}, {-5, 14, 0}, {-63, 240, 0},
+        {-1575, 6930, -2640}, {-68409, 327600, -216216}};
+    static const double n2pk[6][4] = {
+        {1, -5, 0, 0}, {5, -21, 0, 0}, {63, -285, 126, 0}, {1575, -7623, 5280, 0},
+        {68409, -348075, 306306, -34320}, {4729725, -24969285, 25552800, -5405400}};
+    if (p < 1 || p > 6 || nh < 2 || nh > 1000) return 1;
+    double ffac[17];            /* ffac[i + 1] = i!! for i = -1 .. 15 */
+    ffac[0] = 1.0;
+    ffac[1] = 1.0;
+    for (int i = 1; i <= 15; ++i) ffac[i + 1] = i * ffac[i - 1];
+#define FFAC(i) ffac[(i) + 1]
+    const int p2 = 2 * p, mk0 = (p - 1) / 2, mk2 = mk0 + 1;
+    for (int h = 1; h <= nh; ++h) {
+        const double x = h, xx = x * x;
+        const double m0 = 2.0 * FFAC(p2) * x / FFAC(p2 + 1);
+        const double m2 = 2.0 * FFAC(p2) * (x * x * x) / FFAC(p2 + 3);
+        double hm0 = 0.0, hm2 = 0.0;
+        for (int k = 0; k <= mk0; ++k) hm0 += n0pk[p - 1][k] * ipow(xx, k);
+        for (int k = 0; k <= mk2; ++k) hm2 += n2pk[p - 1][k] * ipow(xx, k);
+        const double cm0 = m0 + hm0 / (FFAC(p2 + 1) * ipow(x, p2 - 1));
+        const double cm2 = m2 + hm2 / (FFAC(p2 + 3) * ipow(x, p2 - 1));
+        bnorm[h] = 1.0 / cm0;
+        bsprds[h] = cm2 / cm0;
+    }
+#undef FFAC
+    return 0;
+}
+
+/* hstform (jp_pbfil3.f90:1399-1430): spread**2 -> real half-span hs by
+   linear interpolation in the table of spreads**2. Returns 1 when hs is
+   negative or beyond the table. */
+int mgbfo_hstform(int p, double* hs) {
+    double bnorm[LINE_NH + 1], bsprds[LINE_NH + 1];
+    if (mgbfo_momtab(p, LINE_NH, bnorm, bsprds)) return 1;
+    const int ff = *hs < 0.0;
+    for (int h = 2; h <= LINE_NH; ++h)
+        if (bsprds[h] >= *hs) {
+            *hs = h - (bsprds[h] - *hs) / (bsprds[h] - bsprds[h - 1]);
+            return ff;
+        }
+    return 1;
+}
+
+/* blinfil (jp_pbfil3.f90:1463-1501): half of the symmetric dibeta filter of
+   real half-span hspan: fil[0..h], h = int(hspan), the mixture of the
+   normalized beta filters of half-spans h+1 and h with weights hspan-h and
+   h+1-hspan. Returns 1 when h < 1 or h+1 exceeds nh or nfil. */
+int mgbfo_blinfil(int p, double hspan, int* h, double* fil) {
+    double bnorm[LINE_NH + 1], bsprds[LINE_NH + 1];
+    if (mgbfo_momtab(p, LINE_NH, bnorm, bsprds)) return 1;
+    const int hh = (int)hspan, hp = hh + 1;
+    *h = hh;
+    if (hh < 1 || hp > LINE_NH || hp > LINE_NFIL) return 1;
+    const double whp = (hspan - hh) * bnorm[hp];
+    const double wh = (hp - hspan) * bnorm[hh];
+    for (int i = 0; i <= hh; ++i) {
+        double z = (double)i / hp;
+        z = z * z;
+        fil[i] = whp * ipow(1.0 - z, p);
+    }
+    for (int i = 0; i < hh; ++i) {
+        double z = (double)i / hh;
+        z = z * z;
+        fil[i] += wh * ipow(1.0 - z, p);
+    }
+    return 0;
+}
+
+/* triad (jp_pbfil3.f90:573-624): resolve the aspect 3-vector
+   (A11, A22, A12) onto three integer line generators (ltri[2c], ltri[2c+1])
+   with weights wtri[c] >= 0, starting from the lines (1,0), (0,1), (-1,-1)
+   and flipping the most negative one until none is. lui (columns c) maps
+   the aspect to the weights. Returns 1 when 200 iterations do not end it. */
+int mgbfo_triad(const double aspect[3], int ltri[6], double wtri[3]) {
+    int L[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+    int lui[3][3] = {{1, 0, -1}, {0, 1, -1}, {0, 0, 1}};   /* lui[c][r] */
+    for (int c = 0; c < 3; ++c)
+        wtri[c] = aspect[0] * lui[c][0] + aspect[1] * lui[c][1] + aspect[2] * lui[c][2];
+    int it;
+    for (it = 1; it <= 200; ++it) {
+        int k = 0;
+        for (int c = 1; c < 3; ++c)
+            if (wtri[c] < wtri[k]) k = c;
+        const double dw = 2.0 * wtri[k];
+        if (dw >= -1.e-14) break;
+        const int l = (k + 1) % 3, m = (l + 1) % 3;
+        int dl[3];
+        for (int r = 0; r < 3; ++r) dl[r] = 2 * lui[k][r];
+        L[l][0] = -L[l][0];
+        L[l][1] = -L[l][1];
+        L[k][0] = -L[l][0] - L[m][0];
+        L[k][1] = -L[l][1] - L[m][1];
+        for (int c = 0; c < 3; ++c) {
+            const int sg = c == k ? -1 : 1;
+            for (int r = 0; r < 3; ++r) lui[c][r] += dl[r] * sg;
+            wtri[c] += sg * dw;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        ltri[2 * c] = L[c][0];
+        ltri[2 * c + 1] = L[c][1];
+    }
+    return it > 200;
+}
+
+/* The three lines of the domain: init_mg_line on the aspect pasp2 after
+   def_mg_weights has replaced it by the Cholesky factor of its inverse
+   (cholaspect), which is what reaches t22_to_3 and tritform. Without the
+   tables (mgbf_line off) every generator is 0 and each line is the
+   identity. The tables do not vary over the domain (p_del = p_eps fixed). */
+typedef struct {
+    int dx[3], dy[3];
+    int h[3];                 /* discrete half-span, or -1: the line is the identity */
+    double fil[3][LINE_NFIL + 1];
+} line_t;
+
+static line_t line_make(const mgbfo_params* P) {
+    line_t Lf;
+    memset(&Lf, 0, sizeof Lf);
+    for (int c = 0; c < 3; ++c) Lf.h[c] = -1;
+    if (!P->line_tables) return Lf;
+    double a[4], el[4];
+    aspect2(P, a);
+    cholaspect2(a, el);
+    const double v[3] = {el[0], el[3], el[2]};   /* t22_to_3: (1,1), (2,2), (1,2) */
+    int ltri[6];
+    double hs[3];
+    if (mgbfo_triad(v, ltri, hs)) abort();       /* tritform failure: tables unset */
+    for (int c = 0; c < 3; ++c)
+        if (mgbfo_hstform(P->p, &hs[c])) abort();
+    for (int c = 0; c < 3; ++c) {
+        Lf.dx[c] = ltri[2 * c];
+        Lf.dy[c] = ltri[2 * c + 1];
+        if (Lf.dx[c] == 0 && Lf.dy[c] == 0) continue;
+        int h;
+        /* blinfil failing returns from vdibeta2 with the field untouched */
+        if (mgbfo_blinfil(P->p, hs[c], &h, Lf.fil[c])) continue;
+        const int ax = abs(Lf.dx[c]) * h, ay = abs(Lf.dy[c]) * h;
+        if (ax > P->hx || ay > P->hy) abort();   /* reads beyond the halo */
+        Lf.h[c] = h;
+    }
+    return Lf;
+}
+
+/* vdibeta2 / vdibeta2t on one line c of a padded field: the result replaces
+   the field, halo included (the reference's a = b). */
+static void dibeta_line(const line_t* Lf, int c, pf_t* a, int adjoint) {
+    const int K = a->K, n = a->n, m = a->m;
+    const int dx = Lf->dx[c], dy = Lf->dy[c], h = Lf->h[c];
+    const int ident = dx == 0 && dy == 0;
+    if (!ident && h < 0) return;
+    pf_t b = pf_alloc(K, n, m, a->hx, a->hy);
+    if (!adjoint) {
+#pragma omp parallel for schedule(static)
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= n; ++i) {
+                double* d = PCOL(&b, i, j);
+                const double* s0 = PCOL(a, i, j);
+                if (ident) {
+                    memcpy(d, s0, sizeof(double) * K);
+                    continue;
+                }
+                for (int k = 0; k < K; ++k) d[k] = Lf->fil[c][0] * s0[k];
+                for (int t = 1; t <= h; ++t) {
+                    const double f = Lf->fil[c][t];
+                    const double* sp = PCOL(a, i + dx * t, j + dy * t);
+                    const double* sm = PCOL(a, i - dx * t, j - dy * t);
+                    for (int k = 0; k < K; ++k) d[k] += f * (sp[k] + sm[k]);
+                }
+            }
+    } else {
+        for (int j = 1; j <= m; ++j)
+            for (int i = 1; i <= n; ++i) {
+                const double* s0 = PCOL(a, i, j);
+                if (ident) {
+                    vadd(PCOL(&b, i, j), s0, K);
+                    continue;
+                }
+                double* d0 = PCOL(&b, i, j);
+                for (int k = 0; k < K; ++k) d0[k] += Lf->fil[c][0] * s0[k];
+                for (int t = 1; t <= h; ++t) {
+                    const double f = Lf->fil[c][t];
+                    double* dp = PCOL(&b, i + dx * t, j + dy * t);
+                    double* dm = PCOL(&b, i - dx * t, j - dy * t);
+                    for (int k = 0; k < K; ++k) {
+                        const double v = f * s0[k];
+                        dp[k] += v;
+                        dm[k] += v;
+                    }
+                }
+            }
+    }
+    free(a->d);
+    a->d = b.d;
+}
+
+/* The line filter of filtering_lin2_*: forward boco_2d + dibeta for lines
+   1, 2, 3; adjoint dibetat + bocoT_2d for lines 3, 2, 1. */
+static void line_filter(const mgbfo_params* P, const line_t* Lf, pf_t* a, int adjoint) {
+    if (!adjoint) {
+        for (int c = 0; c < 3; ++c) {
+            boco_2d(a, P->hx, P->hy);
+            dibeta_line(Lf, c, a, 0);
+        }
+    } else {
+        for (int c = 2; c >= 0; --c) {
+            dibeta_line(Lf, c, a, 1);
+            bocoT_2d(a, P->hx, P->hy);
+        }
+    }
+}
+
+/* One line of the domain's line filter, for the pins: generator and the
+   half filter fil[0..h]; h = -1 for an identity line. */
+int mgbfo_line(const mgbfo_params* P, int c, int* dx, int* dy, int* h, double* fil) {
+    if (c < 0 || c > 2) return 1;
+    line_t Lf = line_make(P);
+    *dx = Lf.dx[c];
+    *dy = Lf.dy[c];
+    *h = Lf.h[c];
+    for (int t = 0; t <= (Lf.h[c] > 0 ? Lf.h[c] : 0); ++t) fil[t] = Lf.fil[c][t];
+    return 0;
+}
+
 /* The horizontal filter of the procedure: the radial 2-D beta filter with
-   boco_2d / bocoT_2d, or the fast separable one. */
-static void hfilter(const mgbfo_params* P, const taps_t* T, const fast_t* F, pf_t* a, int adjoint) {
-    if (P->fast) {
-        fast_filter(F, a, adjoint);
+   boco_2d / bocoT_2d, the fast separable one, or the triad line filters. */
+typedef struct {
+    taps_t T;
+    fast_t F;
+    line_t L;
+} hfilt_t;
+
+static hfilt_t hfilt_make(const mgbfo_params* P) {
+    hfilt_t H;
+    memset(&H, 0, sizeof H);
+    if (P->line) H.L = line_make(P);
+    else if (P->fast) H.F = fast_make(P);
+    else H.T = taps2_make(P);
+    return H;
+}
+
+static void hfilt_free(hfilt_t* H) {
+    if (H->T.nt > 0) taps_free(&H->T);
+}
+
+static void hfilter(const mgbfo_params* P, const hfilt_t* H, pf_t* a, int adjoint) {
+    if (P->line) {
+        line_filter(P, &H->L, a, adjoint);
+    } else if (P->fast) {
+        fast_filter(&H->F, a, adjoint);
     } else if (!adjoint) {
         boco_2d(a, P->hx, P->hy);
-        vrbeta2(T, a);
+        vrbeta2(&H->T, a);
     } else {
-        vrbeta2T(T, a);
+        vrbeta2T(&H->T, a);
         bocoT_2d(a, P->hx, P->hy);
     }
 }
 
-static taps_t htaps_make(const mgbfo_params* P) {
-    if (P->fast) {
-        taps_t T = {0, NULL, NULL, NULL};
-        return T;
-    }
-    return taps2_make(P);
-}
-
 void mgbfo_radial(const mgbfo_params* P, int K, int n, int m, const double* in, double* out,
                   int adjoint) {
-    if (P->fast) {
-        fast_t F = fast_make(P);
-        pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
-        pf_from_interior(&a, in);
-        fast_filter(&F, &a, adjoint);
-        pf_to_interior(&a, out);
-        pf_free(&a);
-        return;
-    }
-    taps_t T = taps2_make(P);
+    hfilt_t H = hfilt_make(P);
     pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
     pf_from_interior(&a, in);
-    if (!adjoint) {
-        boco_2d(&a, P->hx, P->hy);
-        vrbeta2(&T, &a);
-    } else {
-        vrbeta2T(&T, &a);
-        bocoT_2d(&a, P->hx, P->hy);
-    }
+    hfilter(P, &H, &a, adjoint);
     pf_to_interior(&a, out);
     pf_free(&a);
-    taps_free(&T);
+    hfilt_free(&H);
 }
 
 void mgbfo_vertical(const mgbfo_params* P, int n, int m, const double* in, double* out,
@@ -909,8 +1154,9 @@ static void upsweep_inputs(const mgbfo_params* P, const double* x, pf_t* G) {
     }
 }
 
-/* Up-sweep: G[g-1] += P G[g] for g = gm-1 .. 1; then Vf on G[0]. */
-static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y) {
+/* Up-sweep: G[g-1] += P G[g] for g = gm-1 .. 1; then Vf on G[0] (Vf^T
+   with final_vft, as filtering_lin2_ens ends, mg_filtering.f90:1303-1305). */
+static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y, int final_vft) {
     const int gm = P->gm;
     for (int g = gm - 1; g >= 1; --g) {
         pf_t Wi = pf_alloc(P->K, G[g].n, G[g].m, 2, 2);
@@ -924,7 +1170,7 @@ static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y) {
         pf_free(&Wi);
         pf_free(&Fp);
     }
-    if (P->l_vertical_filter) vertical_op(P, &G[0], 0);
+    if (P->l_vertical_filter) vertical_op(P, &G[0], final_vft);
     pf_to_interior(&G[0], y);
 }
 
@@ -940,56 +1186,51 @@ static void gens_free(const mgbfo_params* P, pf_t* G) {
 
 void mgbfo_wawt(const mgbfo_params* P, const double* wconst, const double* const* wfield,
                 const double* x, double* y) {
-    /* filtering_rad2_ens, mg_filtering.f90:1000-1147 */
+    /* filtering_rad2_ens, mg_filtering.f90:1000-1147 (and the _bckg, fast
+       and lin2 procedures, which share the skeleton) */
     const int gm = P->gm;
-    taps_t T = htaps_make(P);
-    fast_t F;
-    if (P->fast) F = fast_make(P);
+    hfilt_t H = hfilt_make(P);
     pf_t* G = gens_alloc(P);
     upsweep_inputs(P, x, G);
     for (int g = 0; g < gm; ++g) {
         if (g == 0 && !P->l_filt_g1) continue;
-        hfilter(P, &T, &F, &G[g], 1);
+        hfilter(P, &H, &G[g], 1);
         weight_pf(P, g, &G[g], wconst, wfield);
-        hfilter(P, &T, &F, &G[g], 0);
+        hfilter(P, &H, &G[g], 0);
     }
-    downsweep_outputs(P, G, y);
+    downsweep_outputs(P, G, y, P->final_vft);
     gens_free(P, G);
-    taps_free(&T);
+    hfilt_free(&H);
 }
 
 void mgbfo_wt(const mgbfo_params* P, const double* x, double* v) {
     const int gm = P->gm;
-    taps_t T = htaps_make(P);
-    fast_t F;
-    if (P->fast) F = fast_make(P);
+    hfilt_t H = hfilt_make(P);
     pf_t* G = gens_alloc(P);
     upsweep_inputs(P, x, G);
     size_t off = 0;
     for (int g = 0; g < gm; ++g) {
-        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &T, &F, &G[g], 1);
+        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &H, &G[g], 1);
         pf_to_interior(&G[g], v + off);
         off += (size_t)P->K * G[g].n * G[g].m;
     }
     gens_free(P, G);
-    taps_free(&T);
+    hfilt_free(&H);
 }
 
 void mgbfo_w(const mgbfo_params* P, const double* v, double* y) {
     const int gm = P->gm;
-    taps_t T = htaps_make(P);
-    fast_t F;
-    if (P->fast) F = fast_make(P);
+    hfilt_t H = hfilt_make(P);
     pf_t* G = gens_alloc(P);
     size_t off = 0;
     for (int g = 0; g < gm; ++g) {
         pf_from_interior(&G[g], v + off);
         off += (size_t)P->K * G[g].n * G[g].m;
-        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &T, &F, &G[g], 0);
+        if (!(g == 0 && !P->l_filt_g1)) hfilter(P, &H, &G[g], 0);
     }
-    downsweep_outputs(P, G, y);
+    downsweep_outputs(P, G, y, 0);
     gens_free(P, G);
-    taps_free(&T);
+    hfilt_free(&H);
 }
 
 void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* const* wfield,

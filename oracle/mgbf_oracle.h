@@ -34,6 +34,11 @@ typedef struct mgbfo_params {
                               product of 1-D beta filters along x and y */
     int lquart;            /* quadratic inter-generation transfer (adjoint2 / direct2 with a
                               1-point halo) instead of the 4-point one */
+    int line;              /* filtering_lin2_* (procedures 2, 7): the horizontal filter is
+                              the triad of dibeta line filters */
+    int line_tables;       /* mgbf_line: init_mg_line has set the line tables (without them
+                              every line generator is 0 and each line is the identity) */
+    int final_vft;         /* W A W^T ends with Vf^T instead of Vf (filtering_lin2_ens) */
 } mgbfo_params;
 
 /* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
@@ -68,6 +73,18 @@ void mgbfo_prolong2(int K, int n, int m, const double* coarse, double* fine);
    array (K, n+2h, m+2h); and its adjoint fold back to interior. */
 void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded);
 void mgbfo_fold(int K, int n, int m, int h, const double* padded, double* out);
+
+/* ---- triad line filters (jp_pbfil3.f90) ---- */
+/* inimomtab: bnorm[1..nh], bsprds[1..nh] for exponent p. Nonzero on bad p/nh. */
+int mgbfo_momtab(int p, int nh, double* bnorm, double* bsprds);
+/* hstform with the nh = 6 table of exponent p: spread**2 -> half-span. */
+int mgbfo_hstform(int p, double* hs);
+/* blinfil: half dibeta filter fil[0..*h] of half-span hspan (nfil = 8). */
+int mgbfo_blinfil(int p, double hspan, int* h, double* fil);
+/* triad: aspect (A11, A22, A12) -> line generators ltri[2c..2c+1], weights wtri[c]. */
+int mgbfo_triad(const double aspect[3], int ltri[6], double wtri[3]);
+/* Line c (0..2) of the domain's line filter: generator, half-span (-1: identity), fil[0..h]. */
+int mgbfo_line(const mgbfo_params* P, int c, int* dx, int* dy, int* h, double* fil);
 
 /* ---- the operator ----
    Generation g (0-based gi = g-1) has size (nx>>gi, ny>>gi). The multigrid

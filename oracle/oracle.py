@@ -35,7 +35,8 @@ class _Params(ctypes.Structure):
         ("ampl01", ctypes.c_double), ("ampl02", ctypes.c_double),
         ("p_del", ctypes.c_double), ("p_eps", ctypes.c_double),
         ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
-        ("fast", ctypes.c_int), ("lquart", ctypes.c_int),
+        ("fast", ctypes.c_int), ("lquart", ctypes.c_int), ("line", ctypes.c_int),
+        ("line_tables", ctypes.c_int), ("final_vft", ctypes.c_int),
     ]
 
 
@@ -56,6 +57,11 @@ def lib():
         L.mgbfo_vertical.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
         L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_momtab.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_hstform.argtypes = [ctypes.c_int, dp]
+        L.mgbfo_blinfil.argtypes = [ctypes.c_int, ctypes.c_double, ip, dp]
+        L.mgbfo_triad.argtypes = [dp, ip, dp]
+        L.mgbfo_line.argtypes = [P, ctypes.c_int, ip, ip, ip, dp]
         L.mgbfo_restrict2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong2.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_pad.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
@@ -129,12 +135,15 @@ class OracleParams:
     weights: list = field(default_factory=lambda: [1.0, 0.5, 0.25, 0.125])
     fast: bool = False     # filtering_fast_* (procedures 3, 8)
     lquart: bool = False   # quadratic inter-generation transfer (adjoint2 / direct2)
+    line: bool = False     # filtering_lin2_* (procedures 2, 7): triad line filters
+    line_tables: bool = True   # mgbf_line: the line tables are set (else identity lines)
+    final_vft: bool = False    # W A W^T ends with Vf^T (filtering_lin2_ens)
 
     def c(self) -> _Params:
         return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
                        self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
                        self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast),
-                       int(self.lquart))
+                       int(self.lquart), int(self.line), int(self.line_tables), int(self.final_vft))
 
     def gen_shape(self, g: int):
         """(m_g, n_g, K) of generation g (0-based)."""
@@ -178,6 +187,48 @@ def taps_fast(P: OracleParams):
     if nt < 0:
         raise ValueError("filter reach exceeds the halo")
     return np.array(gx[:nt]), w[:nt].copy()
+
+
+def momtab(p: int, nh: int):
+    """inimomtab: (bnorm, bsprds) for half-spans 1..nh (index 0 unused)."""
+    bn = np.zeros(nh + 1); bs = np.zeros(nh + 1)
+    if lib().mgbfo_momtab(p, nh, _dp(bn), _dp(bs)):
+        raise ValueError("p or nh out of range")
+    return bn, bs
+
+
+def hstform(p: int, spread2: float):
+    """hstform: (half-span, failure flag) for a spread**2."""
+    v = np.array([float(spread2)])
+    ff = lib().mgbfo_hstform(p, _dp(v))
+    return float(v[0]), bool(ff)
+
+
+def blinfil(p: int, hspan: float):
+    """blinfil: (h, fil[0..h]) of the dibeta filter, or None on failure."""
+    h = ctypes.c_int(0)
+    fil = np.zeros(16)
+    if lib().mgbfo_blinfil(p, float(hspan), ctypes.byref(h), _dp(fil)):
+        return None
+    return h.value, fil[: h.value + 1].copy()
+
+
+def triad(aspect):
+    """triad: (generators (3, 2), weights (3,), failure flag)."""
+    a = _f64(aspect)
+    lt = (ctypes.c_int * 6)()
+    w = np.zeros(3)
+    ff = lib().mgbfo_triad(_dp(a), lt, _dp(w))
+    return np.array(lt[:]).reshape(3, 2), w, bool(ff)
+
+
+def line(P: OracleParams, c: int):
+    """Line c of the domain's line filter: ((dx, dy), h, fil[0..h]); h = -1 for identity."""
+    cp = P.c()
+    dx, dy, h = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    fil = np.zeros(16)
+    lib().mgbfo_line(ctypes.byref(cp), c, ctypes.byref(dx), ctypes.byref(dy), ctypes.byref(h), _dp(fil))
+    return (dx.value, dy.value), h.value, fil[: max(h.value, 0) + 1].copy()
 
 
 def ss1(P: OracleParams) -> float:

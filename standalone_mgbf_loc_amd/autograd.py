@@ -32,6 +32,10 @@ def _set_weights(op: M.MGBF, weights):
 class _Localize(torch.autograd.Function):
     @staticmethod
     def forward(ctx, z, op, *weights):
+        if op.cfg.mgbf_proc == 7 and op.cfg.l_vertical_filter:
+            # filtering_lin2_ens ends with Vf^T (mg_filtering.f90:1303-1305)
+            raise ValueError("mgbf_proc=7 ends with the adjoint vertical filter: its W A W^T is "
+                             "not symmetric, and the gradients here assume it is")
         if len(weights) != op.dims.gm:
             raise ValueError(f"need {op.dims.gm} generation weights")
         _set_weights(op, weights)
@@ -67,7 +71,7 @@ class _Localize(torch.autograd.Function):
 
 
 def _filters_g1(op: M.MGBF) -> bool:
-    return op.cfg.mgbf_proc in (1, 3) or bool(op.cfg.l_filt_g1)
+    return op.cfg.mgbf_proc in (1, 2, 3) or bool(op.cfg.l_filt_g1)
 
 
 def localize(op: M.MGBF, z: torch.Tensor, weights) -> torch.Tensor:

@@ -66,6 +66,7 @@ struct mgbf_ctx {
     int nfilt = 0;          // vertically filtered part of the stack (0: none)
     bool filt_g1 = true;
     bool quad = false;      // lquart: quadratic inter-generation transfer
+    int vf_last = 0;        // vertical band that ends W A W^T in the reference's order
     mgbf::RadialCoef coef{};
     int nx[mgbf::kMaxGen] = {}, ny[mgbf::kMaxGen] = {};
     long long gsize[mgbf::kMaxGen] = {};
@@ -479,7 +480,7 @@ int apply_wawt(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     float* U = c->ws_u;
     CALL(vertical(c, x, U, BAND_VFT, st));
     CALL(down_sweep(c, U, U, c->ws_t, true, st));
-    return upsweep(c, c->ws_t, c->ws_t, U, y, c->nfilt ? BAND_VF : -1, false, st);
+    return upsweep(c, c->ws_t, c->ws_t, U, y, c->nfilt ? c->vf_last : -1, false, st);
 }
 
 int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
@@ -635,7 +636,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->K = s.dims.km_all;
     c->gm = s.dims.gm;
     // the background procedures filter generation 1 whatever l_filt_g1 says
-    c->filt_g1 = cfg->mgbf_proc == 1 || cfg->mgbf_proc == 3 ? true : cfg->l_filt_g1 != 0;
+    c->filt_g1 = cfg->mgbf_proc == 1 || cfg->mgbf_proc == 2 || cfg->mgbf_proc == 3
+                     ? true
+                     : cfg->l_filt_g1 != 0;
+    // filtering_lin2_ens ends with sup_vrbeta1T_ens (mg_filtering.f90:1303-1305)
+    c->vf_last = cfg->mgbf_proc == 7 ? BAND_VFT : BAND_VF;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     c->quad = cfg->lquart != 0;
     {
@@ -937,7 +942,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     const bool timing = c->timing;
     c->timing = false;
     int status = MGBF_OK;
-    const int vband = c->commute ? BAND_VVT : BAND_VF;
+    const int vband = c->commute ? BAND_VVT : c->vf_last;
     // MGBF_AUTOTUNE_LOG=1 prints every candidate's time (3 passes) to stderr
     const char* logv = std::getenv("MGBF_AUTOTUNE_LOG");
     const bool log = logv && logv[0] == '1';

@@ -305,6 +305,149 @@ bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<dou
     return beta1_taps(1.0 / std::sqrt(cfg.mg_ampl01), cfg.hz, cfg.p, gx, w);   // cholaspect1
 }
 
+// Triad line filters of filtering_lin2_* (jp_pbfil3.f90). init_mg_line
+// (mg_intstate.f90:436-480) runs after def_mg_weights has replaced pasp2 by
+// the Cholesky factor of its inverse, so the 3-vector that reaches tritform
+// is (L11, L22, L12) = (1/sqrt(a), 1/sqrt(a), 0), a = mg_ampl02 (p_del =
+// p_eps = 0). tritform resolves it onto three lattice lines (triad) and
+// turns each line's spread**2 into a real half-span (hstform); each line is
+// then filtered by the mixture of the normalized beta filters of the two
+// neighbouring integer half-spans (blinfil) along its generator.
+namespace {
+
+constexpr int kLineNH = 6;     // nh (mg_parameter.f90:562)
+constexpr int kLineNFil = 8;   // nfil = nh + 2
+
+// Normalizer and spread**2 of the beta filter (1-(i/h)^2)^p, |i| < h, for
+// h = 1..nh: the discrete sums that inimomtab evaluates in closed form.
+void line_moments(int p, double* bnorm, double* bsprds) {
+    for (int h = 1; h <= kLineNH; ++h) {
+        double s0 = 0.0, s2 = 0.0;
+        for (int i = -h + 1; i <= h - 1; ++i) {
+            const double z = (double)i / h;
+            const double w = ipow(1.0 - z * z, p);
+            s0 += w;
+            s2 += (double)i * i * w;
+        }
+        bnorm[h] = 1.0 / s0;
+        bsprds[h] = s2 / s0;
+    }
+}
+
+// triad (jp_pbfil3.f90:573-624): aspect (A11, A22, A12) -> generators
+// l[c] and weights w[c] >= 0, from the superbase (1,0), (0,1), (-1,-1),
+// flipping the line of the most negative weight until none is negative.
+bool line_triad(const double a[3], int l[3][2], double w[3]) {
+    int L[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+    int lui[3][3] = {{1, 0, -1}, {0, 1, -1}, {0, 0, 1}};   // lui[c][r]
+    for (int c = 0; c < 3; ++c) w[c] = a[0] * lui[c][0] + a[1] * lui[c][1] + a[2] * lui[c][2];
+    bool done = false;
+    for (int it = 0; it < 200 && !done; ++it) {
+        int k = 0;
+        for (int c = 1; c < 3; ++c)
+            if (w[c] < w[k]) k = c;
+        const double dw = 2.0 * w[k];
+        if (dw >= -1.e-14) {
+            done = true;
+            break;
+        }
+        const int lc = (k + 1) % 3, mc = (lc + 1) % 3;
+        int dl[3];
+        for (int r = 0; r < 3; ++r) dl[r] = 2 * lui[k][r];
+        for (int e = 0; e < 2; ++e) {
+            L[lc][e] = -L[lc][e];
+            L[k][e] = -L[lc][e] - L[mc][e];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const int sg = c == k ? -1 : 1;
+            for (int r = 0; r < 3; ++r) lui[c][r] += dl[r] * sg;
+            w[c] += sg * dw;
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        l[c][0] = L[c][0];
+        l[c][1] = L[c][1];
+    }
+    return done;
+}
+
+}  // namespace
+
+int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err) {
+    taps.clear();
+    if (!cfg.mgbf_line) {
+        // without init_mg_line every generator is 0: each line is the identity
+        taps.push_back({0, 0, 1.0});
+        return MGBF_OK;
+    }
+    if (cfg.p < 1 || cfg.p > 6) {
+        *err = " In inimomtab; prescribed exponent p out of bounds";
+        return MGBF_ERR_CONFIG;
+    }
+    double bnorm[kLineNH + 1], bsprds[kLineNH + 1];
+    line_moments(cfg.p, bnorm, bsprds);
+    const double el = 1.0 / std::sqrt(cfg.mg_ampl02);
+    const double asp[3] = {el, el, 0.0};
+    int l[3][2];
+    double hs[3];
+    if (!line_triad(asp, l, hs)) {
+        *err = " In tritform; triad failed; check aspect tensor";
+        return MGBF_ERR_CONFIG;
+    }
+    // 1-D half filters along x and y (identity unless a line says otherwise)
+    std::vector<double> fx{1.0}, fy{1.0};
+    bool have_x = false, have_y = false;
+    for (int c = 0; c < 3; ++c) {
+        // hstform (jp_pbfil3.f90:1399-1430)
+        const bool neg = hs[c] < 0.0;
+        bool found = false;
+        for (int h = 2; h <= kLineNH && !found; ++h)
+            if (bsprds[h] >= hs[c]) {
+                hs[c] = h - (bsprds[h] - hs[c]) / (bsprds[h] - bsprds[h - 1]);
+                found = true;
+            }
+        if (neg || !found) {
+            *err = fmt(" In tritform; hstform failed at i=%d", c + 1);
+            return MGBF_ERR_CONFIG;
+        }
+        if (l[c][0] == 0 && l[c][1] == 0) continue;
+        // blinfil (jp_pbfil3.f90:1463-1501); failing, vdibeta2 leaves the field
+        const int h = (int)hs[c], hp = h + 1;
+        if (h < 1 || hp > kLineNH || hp > kLineNFil) continue;
+        std::vector<double> fil(h + 1, 0.0);
+        const double whp = (hs[c] - h) * bnorm[hp], wh = (hp - hs[c]) * bnorm[h];
+        for (int i = 0; i <= h; ++i) {
+            const double z = (double)i / hp;
+            fil[i] = whp * ipow(1.0 - z * z, cfg.p);
+        }
+        for (int i = 0; i < h; ++i) {
+            const double z = (double)i / h;
+            fil[i] += wh * ipow(1.0 - z * z, cfg.p);
+        }
+        bool ident = fil[0] == 1.0;
+        for (int i = 1; i <= h; ++i) ident = ident && fil[i] == 0.0;
+        if (ident) continue;
+        const bool xline = l[c][1] == 0 && std::abs(l[c][0]) == 1;
+        const bool yline = l[c][0] == 0 && std::abs(l[c][1]) == 1;
+        if ((xline && have_x) || (yline && have_y) || (!xline && !yline)) {
+            *err = "line filters off the grid axes are not built";
+            return MGBF_ERR_UNSUPPORTED;
+        }
+        if ((xline && h > cfg.hx) || (yline && h > cfg.hy)) {
+            *err = "line filter reach exceeds the halo";
+            return MGBF_ERR_CONFIG;
+        }
+        (xline ? fx : fy) = fil;
+        (xline ? have_x : have_y) = true;
+    }
+    // the lines apply one after the other with the half-sample mirror in
+    // between: on the grid axes that is the product filter fx(a) fy(b)
+    const int hxl = (int)fx.size() - 1, hyl = (int)fy.size() - 1;
+    for (int b = -hyl; b <= hyl; ++b)
+        for (int a = -hxl; a <= hxl; ++a) taps.push_back({a, b, fx[std::abs(a)] * fy[std::abs(b)]});
+    return MGBF_OK;
+}
+
 bool fast_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     // the fast filter's x and y factors keep the aspect amplitude as the
     // factor itself: paspx = paspy = mg_ampl02 reach getlinesum1 and vrbeta1
@@ -326,14 +469,11 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "invalid namelist value (sizes must be positive)";
         return MGBF_ERR_CONFIG;
     }
-    if (c.mgbf_proc != 1 && c.mgbf_proc != 3 && c.mgbf_proc != 6 && c.mgbf_proc != 8) {
-        *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 3 filtering_fast_bckg, "
-                   "6 filtering_rad2_ens, 8 filtering_fast_ens)",
+    if (c.mgbf_proc < 1 || c.mgbf_proc > 8 || c.mgbf_proc == 4 || c.mgbf_proc == 5) {
+        *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 2 filtering_lin2_bckg, "
+                   "3 filtering_fast_bckg, 6 filtering_rad2_ens, 7 filtering_lin2_ens, "
+                   "8 filtering_fast_ens)",
                    c.mgbf_proc);
-        return MGBF_ERR_UNSUPPORTED;
-    }
-    if (c.mgbf_line) {
-        *err = "mgbf_line=.true. (dibeta line filters) is not built";
         return MGBF_ERR_UNSUPPORTED;
     }
     if (c.l_loc) {
@@ -342,7 +482,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     }
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
-    const bool bckg = c.mgbf_proc == 1 || c.mgbf_proc == 3;
+    const bool bckg = c.mgbf_proc == 1 || c.mgbf_proc == 2 || c.mgbf_proc == 3;
     if (bckg) {
         if (c.n_ens != 1) {
             *err = fmt("mgbf_proc=%d (background) needs n_ens=1", c.mgbf_proc);
@@ -355,8 +495,8 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         d.km3_all = c.km3 * c.n_ens;
         if (c.km2 != 0 && c.l_vertical_filter) {
             // sup_vrbeta1_ens strides the stack by lm only (mg_filtering.f90:2066-2069)
-            *err = "ensemble procedures (6, 8) with km2>0 and l_vertical_filter: the reference's ensemble "
-                   "vertical filter assumes km2=0";
+            *err = "ensemble procedures (6, 7, 8) with km2>0 and l_vertical_filter: the reference's "
+                   "ensemble vertical filter assumes km2=0";
             return MGBF_ERR_CONFIG;
         }
     }
@@ -447,7 +587,12 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     int st = derive(cfg, s.dims, s.nx, s.ny, err);
     if (st != MGBF_OK) return st;
     const bool fast = cfg.mgbf_proc == 3 || cfg.mgbf_proc == 8;
-    if (fast) {
+    const bool line = cfg.mgbf_proc == 2 || cfg.mgbf_proc == 7;
+    if (line) {
+        // filtering_lin2_*: the triad of dibeta line filters
+        st = line_taps(cfg, s.taps2, err);
+        if (st != MGBF_OK) return st;
+    } else if (fast) {
         // filtering_fast_*: the horizontal filter is the product of 1-D beta
         // filters along x and y (rbeta over each row and column)
         if (!fast_taps(cfg, s.taps2)) {
@@ -465,7 +610,7 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     s.dims.ntaps2 = (int)s.taps2.size();
     s.dims.ntaps1 = (int)s.taps1_gx.size();
     s.H = cfg.hx > cfg.hy ? cfg.hx : cfg.hy;
-    if (fast) {
+    if (fast || line) {
         // the kernels need only the taps' extent, which a raw factor can keep
         // far inside the halo
         int ext = 1;
@@ -516,15 +661,19 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         s.vft.assign((size_t)lm * W, 0.0);
         // Vf Vf^T (band 2hz): the vertical part of W A W^T when the weights
         // are the same along the stack, where it commutes with the
-        // horizontal multigrid operator
+        // horizontal multigrid operator. filtering_lin2_ens ends with
+        // sup_vrbeta1T_ens where the others end with Vf
+        // (mg_filtering.f90:1303-1305): its product is Vf^T Vf^T.
         {
+            const bool tt = cfg.mgbf_proc == 7;
             const int W2 = 4 * hz + 1;
             s.vvt.assign((size_t)lm * W2, 0.0);
             for (int L = 0; L < lm; ++L)
                 for (int c = L - 2 * hz; c <= L + 2 * hz; ++c) {
                     if (c < 0 || c >= lm) continue;
                     double acc = 0;
-                    for (int q = 0; q < lm; ++q) acc += V[(size_t)L * lm + q] * V[(size_t)c * lm + q];
+                    for (int q = 0; q < lm; ++q)
+                        acc += (tt ? V[(size_t)q * lm + L] : V[(size_t)L * lm + q]) * V[(size_t)c * lm + q];
                     s.vvt[(size_t)L * W2 + (c - L + 2 * hz)] = acc;
                 }
         }

@@ -69,5 +69,8 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err);
 // the halo (the reference stops there).
 bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps);
 bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w);
+// 2-D taps of the triad line filters of procedures 2 and 7 (mgbf_line), or
+// the identity without line tables. Returns MGBF_OK or an error status.
+int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err);
 
 }  // namespace mgbf

@@ -131,9 +131,12 @@ class DenseMGBF:
 
     def __init__(self, nx, ny, K, lm, nvcol, hx, hz, p, gm, R2, R2v,
                  weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False,
-                 quad=False):
+                 quad=False, taps2d=None, final_vft=False):
         self.nx, self.ny, self.K, self.gm = nx, ny, K, gm
-        if fast:
+        self.final_vft = final_vft
+        if taps2d is not None:
+            taps = list(taps2d)
+        elif fast:
             # separable: the outer product of two 1-D beta filters whose
             # squared reach is R2 (offsets a with a^2 < R2)
             lt = line_taps(hx, Fraction(R2), p)
@@ -180,7 +183,7 @@ class DenseMGBF:
         for g in range(self.gm):
             C = self.chain(g)
             S = S + C @ self.F[g] @ self.A[g] @ self.F[g].T @ C.T
-        return self.Vf @ S @ self.Vf.T
+        return (self.Vf.T if self.final_vft else self.Vf) @ S @ self.Vf.T
 
     def Wt(self) -> np.ndarray:
         """Stacked W^T: rows are the generations of the multigrid vector."""

@@ -77,8 +77,8 @@ def test_nm0_divisibility_is_checked():
     assert "not divisible" in str(_err(cfg))
 
 
-@pytest.mark.parametrize("key,val", [("mgbf_line", True), ("l_loc", True),
-                                     ("mgbf_proc", 2), ("mgbf_proc", 4), ("mgbf_proc", 7)])
+@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 4), ("mgbf_proc", 5),
+                                     ("mgbf_proc", 9)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)
     setattr(cfg, key, val)
@@ -100,3 +100,28 @@ def test_init_without_device_fails_cleanly():
         pytest.skip("a device is present")
     with pytest.raises(M.MGBFError):
         M.MGBF(M.Config.for_grid(192, 192, 32))
+
+
+@pytest.mark.parametrize("proc", [2, 7])
+@pytest.mark.parametrize("ampl02,p", [(0.25, 2), (2.0, 2), (0.6, 1), (0.1, 3)])
+def test_line_procedure_taps_follow_the_oracle_lines(proc, ampl02, p):
+    # the library's line filter is the product of the x and y lines of the
+    # oracle's init_mg_line restatement (the third line idle); without
+    # mgbf_line it is the identity
+    from oracle import oracle as O
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=6, hz=2, mgbf_proc=proc, p=p)
+    cfg.mg_ampl02 = ampl02
+    assert M.derive(cfg).ntaps2 == 1
+    cfg.mgbf_line = True
+    P = O.OracleParams(nx=64, ny=64, K=8, lm=8, nvcol=1, hx=6, hy=6, hz=2, p=p, ampl02=ampl02,
+                       line=True)
+    hx = O.line(P, 0)[1]
+    hy = O.line(P, 1)[1]
+    assert O.line(P, 0)[0] == (1, 0) and O.line(P, 1)[0] == (0, 1)
+    assert M.derive(cfg).ntaps2 == (2 * max(hx, 0) + 1) * (2 * max(hy, 0) + 1)
+
+
+def test_line_procedure_needs_exponent_in_the_moment_table():
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=6, hz=2, mgbf_proc=7, p=7, mgbf_line=True)
+    cfg.mg_ampl02 = 0.5
+    assert "inimomtab" in str(_err(cfg))

@@ -36,7 +36,8 @@ def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
         hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
         l_filt_g1=True if bckg else bool(cfg.l_filt_g1),
         l_vertical_filter=bool(cfg.l_vertical_filter), fast=cfg.mgbf_proc in (3, 8),
-        lquart=bool(cfg.lquart),
+        lquart=bool(cfg.lquart), line=cfg.mgbf_proc in (2, 7), line_tables=bool(cfg.mgbf_line),
+        final_vft=cfg.mgbf_proc == 7,
         weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
 
 
@@ -353,6 +354,47 @@ def test_lquart_procedures_match_oracle(monkeypatch, case):
         P = oparams(cfg, d)
         x = field(d.gen_shape(1), 140)
         v = np.random.default_rng(141).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs = op.dot(w, xd)
+        rhs = op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("case", ["ens", "ens_exact_order", "ens_no_tables", "ens_filt_g1_off",
+                                  "ens_lquart", "bckg", "bckg_p1"])
+def test_line_procedures_match_oracle(monkeypatch, case):
+    # filtering_lin2_ens / _bckg (mgbf_proc = 7 / 2): the triad of dibeta
+    # line filters, spread**2 = 1/sqrt(mg_ampl02) along x and y; the
+    # ensemble procedure ends with Vf^T as the reference's does. Without
+    # mgbf_line the lines are the identity
+    if case == "ens_exact_order":
+        monkeypatch.setenv("MGBF_EXACT_ORDER", "1")
+    kw = dict(n_ens=2, gm=3, h=6, hz=3, mgbf_proc=7, mgbf_line=True, weights=[1.0, 0.5, 0.25])
+    if case == "ens_no_tables":
+        kw["mgbf_line"] = False
+    if case == "ens_filt_g1_off":
+        kw["l_filt_g1"] = False
+    if case == "ens_lquart":
+        kw["lquart"] = True
+    if case.startswith("bckg"):
+        kw.update(n_ens=1, km2=3, km3=2, mgbf_proc=2)
+    if case == "bckg_p1":
+        kw["p"] = 1
+    cfg = M.Config.for_grid(96, 64, 16, **kw)
+    cfg.mg_ampl02 = 0.25
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 150)
+        v = np.random.default_rng(151).standard_normal(d.mg_size).astype(np.float32)
         xd, vd = to_dev(x), to_dev(v)
         y = op.filtering_procedure(xd)
         wt = op.apply(M.OP_WT, xd)

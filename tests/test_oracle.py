@@ -430,3 +430,119 @@ def test_lquart_w_and_wt_are_adjoint():
     lhs = np.vdot(O.w(P, v), x)
     rhs = np.vdot(v, vt)
     assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+# --------------------------------------------------------------------------
+# triad line filters (filtering_lin2_*, procedures 2 and 7)
+# --------------------------------------------------------------------------
+
+def _beta(p, h):
+    """Normalized beta filter (1-(i/h)^2)^p, |i| < h, as {offset: weight}."""
+    i = np.arange(-h + 1, h)
+    w = (1 - (i / h) ** 2) ** p
+    return dict(zip(i.tolist(), (w / w.sum()).tolist()))
+
+
+def _spread2(f):
+    return sum(a * a * w for a, w in f.items())
+
+
+def _dibeta_from_spread(p, s2):
+    """The unit-sum filter with spread**2 s2 that mixes the normalized beta
+    filters of two consecutive integer half-spans (found by search)."""
+    for h in range(1, 7):
+        lo, hi = _beta(p, h), _beta(p, h + 1)
+        s_lo, s_hi = _spread2(lo), _spread2(hi)
+        if s_lo <= s2 <= s_hi:
+            lam = (s2 - s_lo) / (s_hi - s_lo)
+            return {a: (1 - lam) * lo.get(a, 0.0) + lam * hi.get(a, 0.0) for a in hi}
+    raise ValueError("spread out of range")
+
+
+@pytest.mark.parametrize("p", range(1, 7))
+def test_moment_tables_are_the_discrete_beta_moments(p):
+    # inimomtab's continuum moments plus Euler-Maclaurin corrections are the
+    # exact discrete moments of the beta filter
+    bn, bs = O.momtab(p, 6)
+    for h in range(1, 7):
+        f = _beta(p, h)
+        i = np.arange(-h + 1, h)
+        w = (1 - (i / h) ** 2) ** p
+        assert abs(bn[h] * w.sum() - 1) < 1e-14
+        assert abs(bs[h] - _spread2(f)) < 1e-13 * max(1.0, bs[h])
+
+
+@pytest.mark.parametrize("p,s2", [(2, 0.3), (2, 0.7), (2, 2.0), (1, 1.0), (3, 2.5), (6, 0.1), (4, 1.9)])
+def test_dibeta_is_the_beta_mixture_with_the_prescribed_spread(p, s2):
+    hs, ff = O.hstform(p, s2)
+    assert not ff
+    h, fil = O.blinfil(p, hs)
+    full = {a: fil[abs(a)] for a in range(-h, h + 1)}
+    assert abs(sum(full.values()) - 1) < 1e-14
+    assert abs(_spread2(full) - s2) < 1e-13 * max(1.0, s2)
+    ref = _dibeta_from_spread(p, s2)
+    for a in set(full) | set(ref):
+        assert abs(full.get(a, 0.0) - ref.get(a, 0.0)) < 1e-14
+
+
+@pytest.mark.parametrize("a", [(1.0, 1.0, 0.0), (2.0, 1.0, 0.9), (1.0, 3.0, -1.2),
+                               (5.0, 0.3, 1.1), (0.4, 0.4, -0.39), (0.2, 7.0, 1.1)])
+def test_triad_resolves_the_aspect_onto_a_superbase(a):
+    L, w, ff = O.triad(a)
+    assert not ff
+    assert (w >= -1e-14).all()
+    A = sum(w[c] * np.outer(L[c], L[c]) for c in range(3))
+    assert np.allclose([A[0, 0], A[1, 1], A[0, 1]], a, atol=1e-12)
+    assert (L.sum(axis=0) == 0).all()
+    assert abs(L[0, 0] * L[1, 1] - L[0, 1] * L[1, 0]) == 1
+
+
+def test_domain_lines_take_the_cholesky_factor_as_their_aspect():
+    # init_mg_line runs after cholaspect: the spread**2 of the x and y lines
+    # is 1/sqrt(mg_ampl02), and the third line of the superbase is idle
+    P = _P(line=True, ampl02=Fraction(1, 4), hx=4, hy=4)
+    lines = [O.line(P, c) for c in range(3)]
+    assert [ln[0] for ln in lines[:2]] == [(1, 0), (0, 1)]
+    for g, h, fil in lines[:2]:
+        full = {a: fil[abs(a)] for a in range(-h, h + 1)}
+        assert abs(_spread2(full) - 2.0) < 1e-13
+    g, h, fil = lines[2]
+    assert h == -1 or (fil[0] == 1.0 and not fil[1:].any())
+    # without the line tables every line is the identity
+    P0 = _P(line=True, line_tables=False)
+    x = _rand(P0.gen_shape(0), 91)
+    assert np.array_equal(O.radial(P0, x), x)
+
+
+def test_diagonal_line_gather_and_scatter_are_adjoint():
+    # an off-diagonal aspect puts a line on a lattice diagonal; pad + gather
+    # and scatter + fold are transposes of each other
+    P = _P(line=True, ampl02=Fraction(1, 4), p_eps=0.6, hx=6, hy=6)
+    gens = {O.line(P, c)[0] for c in range(3)}
+    assert any(dx != 0 and dy != 0 for dx, dy in gens)
+    x, y = _rand((12, 14, 2), 92), _rand((12, 14, 2), 93)
+    lhs = np.vdot(O.radial(P, x), y)
+    rhs = np.vdot(x, O.radial(P, y, adjoint=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+@pytest.mark.parametrize("final_vft,filt_g1,tables", [(False, True, True), (True, True, True),
+                                                      (True, False, True), (False, True, False)])
+def test_line_procedures_match_dense_model(final_vft, filt_g1, tables):
+    P = _P(nx=16, ny=12, K=4, lm=2, nvcol=2, hx=4, hy=4, hz=1, gm=3,
+           ampl02=Fraction(1, 4), ampl01=Fraction(4, 7), l_filt_g1=filt_g1,
+           weights=[1.0, 0.7, 0.3], line=True, line_tables=tables, final_vft=final_vft)
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    if tables:
+        f = _dibeta_from_spread(P.p, 2.0)           # spread**2 = 1/sqrt(1/4)
+        taps = [(a, b, f[a] * f[b]) for b in f for a in f]
+    else:
+        taps = [(0, 0, 1.0)]
+    Dm = D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm, None,
+                     Fraction(P.ampl01).limit_denominator(1000) * (2 * P.p + 3),
+                     list(P.wconst()), P.l_filt_g1, P.l_vertical_filter, None,
+                     taps2d=taps, final_vft=final_vft)
+    Bd = Dm.B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
