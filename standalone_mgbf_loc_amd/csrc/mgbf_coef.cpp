@@ -448,6 +448,38 @@ int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err)
     return MGBF_OK;
 }
 
+// getlinesum3's reach test (jp_pbfil.f90:220-262) for pasp3 = mg_ampl03 I
+// (p_del = p_sig = p_rho = 0): the lattice ball |g| < sqrt(mg_ampl03 (2p+5))
+// visited z, then y, then x, with the message of the first halo exceeded.
+bool ball_reach_ok(const mgbf_config& cfg, std::string* err) {
+    const double e = std::sqrt(1.0 / cfg.mg_ampl03) / std::sqrt(2.0 * cfg.p + 5.0);
+    const double z = 1.0 / e;
+    const int gzl = (int)std::ceil(-z + kEps), gzm = (int)std::floor(z - kEps);
+    if (gzl < -cfg.hz || gzm > cfg.hz) {
+        *err = "In getlinesum3; filter reach becomes too large for hz";
+        return false;
+    }
+    for (int gz = gzl; gz <= gzm; ++gz) {
+        const double rry = (gz * e) * (gz * e);
+        const double y = std::sqrt(1.0 - rry);
+        const int gyl = (int)std::ceil(-y / e + kEps), gym = (int)std::floor(y / e - kEps);
+        if (gyl < -cfg.hy || gym > cfg.hy) {
+            *err = "In getlinesum3; filter reach becomes too large for hy";
+            return false;
+        }
+        for (int gy = gyl; gy <= gym; ++gy) {
+            const double rrx = rry + (gy * e) * (gy * e);
+            const double x = std::sqrt(std::max(0.0, 1.0 - rrx));
+            const int gxl = (int)std::ceil(-x / e + kEps), gxm = (int)std::floor(x / e - kEps);
+            if (gxl < -cfg.hx || gxm > cfg.hx) {
+                *err = "In getlinesum3; filter reach becomes too large for hx";
+                return false;
+            }
+        }
+    }
+    return true;
+}
+
 bool fast_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     // the fast filter's x and y factors keep the aspect amplitude as the
     // factor itself: paspx = paspy = mg_ampl02 reach getlinesum1 and vrbeta1
@@ -586,6 +618,26 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     s.cfg = cfg;
     int st = derive(cfg, s.dims, s.nx, s.ny, err);
     if (st != MGBF_OK) return st;
+    // def_mg_weights normalizes every filter of the namelist whatever the
+    // procedure (mg_intstate.f90:422-431), and getlinesum stops on the first
+    // reach beyond its halo (jp_pbfil.f90:149-262): paspx and paspy (the
+    // fast factors, mg_ampl02 as given) along x and y, pasp1 along z, pasp2
+    // in the plane, pasp3 in the volume
+    {
+        std::vector<int> g;
+        std::vector<double> w;
+        std::vector<Tap2> t2;
+        if (!beta1_taps(cfg.mg_ampl02, cfg.hx, cfg.p, g, w) ||
+            !beta1_taps(cfg.mg_ampl02, cfg.hy, cfg.p, g, w) || !vertical_taps(cfg, g, w)) {
+            *err = "In getlinesum1; filter reach fx becomes too large for hx";
+            return MGBF_ERR_CONFIG;
+        }
+        if (!radial_taps(cfg, t2)) {
+            *err = "In getlinesum2; filter reach becomes too large for hx/hy";
+            return MGBF_ERR_CONFIG;
+        }
+        if (!ball_reach_ok(cfg, err)) return MGBF_ERR_CONFIG;
+    }
     const bool fast = cfg.mgbf_proc == 3 || cfg.mgbf_proc == 8;
     const bool line = cfg.mgbf_proc == 2 || cfg.mgbf_proc == 7;
     if (line) {

@@ -112,7 +112,8 @@ class Config:
         grid (def_maxgen); this picks the smallest power-of-two PE grid that
         allows gm generations, and sets the aspect amplitudes so that the
         filter reach equals the halo (mg_ampl02 = (h+1)^2/(2p+4),
-        mg_ampl01 = (hz+1)^2/(2p+3)).
+        mg_ampl01 = (hz+1)^2/(2p+3), and for the 3-D filter every
+        configuration normalizes, mg_ampl03 = (min(h,hz)+1)^2/(2p+5)).
         """
         pe = 1
         while _def_maxgen(pe, pe) < gm:
@@ -122,6 +123,7 @@ class Config:
         w = list(weights) if weights is not None else [2.0 ** -g for g in range(5)]
         w = (w + [w[-1]] * 5)[:5]
         c = cls(mg_ampl01=(hz + 1) ** 2 / (2 * p + 3), mg_ampl02=(h + 1) ** 2 / (2 * p + 4),
+                mg_ampl03=(min(h, hz) + 1) ** 2 / (2 * p + 5),
                 mg_weig1=w[0], mg_weig2=w[1], mg_weig3=w[2], mg_weig4=w[3], mg_weig5=w[4],
                 hx=h, hy=h, hz=hz, p=p, n_ens=n_ens, lm_a=lm, lm=lm, gm_max=gm,
                 nm0=nx, mm0=ny, nxPE=pe, nyPE=pe, im_filt=nx // pe, jm_filt=ny // pe)

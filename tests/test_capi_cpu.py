@@ -103,7 +103,7 @@ def test_init_without_device_fails_cleanly():
 
 
 @pytest.mark.parametrize("proc", [2, 7])
-@pytest.mark.parametrize("ampl02,p", [(0.25, 2), (2.0, 2), (0.6, 1), (0.1, 3)])
+@pytest.mark.parametrize("ampl02,p", [(0.5, 2), (2.0, 2), (0.6, 1), (1.0, 3)])
 def test_line_procedure_taps_follow_the_oracle_lines(proc, ampl02, p):
     # the library's line filter is the product of the x and y lines of the
     # oracle's init_mg_line restatement (the third line idle); without
@@ -122,6 +122,20 @@ def test_line_procedure_taps_follow_the_oracle_lines(proc, ampl02, p):
 
 
 def test_line_procedure_needs_exponent_in_the_moment_table():
-    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=6, hz=2, mgbf_proc=7, p=7, mgbf_line=True)
-    cfg.mg_ampl02 = 0.5
+    # every filter fits its halo at p = 7; only the moment table refuses it
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=6, hz=3, mgbf_proc=7, p=7, mgbf_line=True,
+                            mg_ampl01=0.5, mg_ampl02=1.0, mg_ampl03=0.5)
     assert "inimomtab" in str(_err(cfg))
+    cfg.mgbf_line = False
+    assert M.derive(cfg).ntaps2 == 1
+
+
+@pytest.mark.parametrize("key,val,msg", [("mg_ampl02", 0.3, "getlinesum1"), ("mg_ampl03", 3.0, "getlinesum3"),
+                                         ("hz", 1, "getlinesum")])
+def test_every_filter_of_the_namelist_must_fit_its_halo(key, val, msg):
+    # def_mg_weights normalizes the fast factors, the vertical, the 2-D and
+    # the 3-D filters whatever the procedure; each stops beyond its halo
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=6, hz=3)
+    assert M.derive(cfg).gm == 3
+    setattr(cfg, key, val)
+    assert msg in str(_err(cfg))

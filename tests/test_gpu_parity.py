@@ -242,6 +242,7 @@ def test_fast_ensemble_procedure_matches_oracle(case):
     cfg.mg_ampl02 = 0.45 if case == "wide" else 2.0
     if case == "wide":
         cfg.p = 2     # reach sqrt(7)/0.45 = 5.9: 11 taps per axis
+        cfg.mg_ampl03 = 49 / 9     # keeps the 3-D normalization inside the halo at p = 2
     with M.MGBF(cfg) as op:
         d = op.dims
         x = field(d.gen_shape(1), 71)
@@ -389,7 +390,7 @@ def test_line_procedures_match_oracle(monkeypatch, case):
     if case == "bckg_p1":
         kw["p"] = 1
     cfg = M.Config.for_grid(96, 64, 16, **kw)
-    cfg.mg_ampl02 = 0.25
+    cfg.mg_ampl02 = 0.5    # the smallest the fast factors' normalization allows at hx = 6
     with M.MGBF(cfg) as op:
         d = op.dims
         P = oparams(cfg, d)
