@@ -79,6 +79,7 @@ typedef struct mgbf_dims {
     int na, ma;           /* global analysis grid: nm0 x mm0 */
     int km_a_all;         /* stacked k extent on the analysis grid: n_ens*(km2 + km3*lm_a) */
     long long anal_size;  /* km_a_all*na*ma (WORKA) */
+    int ntaps3;           /* taps of the 3-D filter (procedure 4), else 0 */
 } mgbf_dims;
 
 typedef struct mgbf_ctx mgbf_ctx;

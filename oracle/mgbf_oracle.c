@@ -22,6 +22,10 @@
  *   - vdibeta2 / vdibeta2t ..................... jp_pbfil3.f90:1801-1837, 2345-2382
  *   - init_mg_line ............................. mg_intstate.f90:436-480
  *   - filtering_lin2_ens / _bckg composition ... mg_filtering.f90:1150-1313, 683-822
+ *   - cholaspect3, getlinesum3 ................. jp_pbfil.f90:108-128, 220-262
+ *   - vrbeta3 / vrbeta3T ....................... jp_pbfil.f90:930-977, 1053-1100
+ *   - sup_vrbeta3 / sup_vrbeta3T ............... mg_filtering.f90:1923-2039
+ *   - filtering_rad3 composition ............... mg_filtering.f90:123-257
  *   - upsending2_ens / downsending2_ens ........ mg_generations.f90:497-609
  *   - upsending_ens / downsending_ens .......... mg_generations.f90:382-494
  *   - weighting_ens ............................ mg_generations.f90:1012-1052
@@ -734,6 +738,202 @@ static void fast_filter(const fast_t* F, pf_t* a, int adjoint) {
 }
 
 /* ------------------------------------------------------------------------
+ * 3-D radial filter (procedure 4, filtering_rad3): the 3-D variables of the
+ * stack are filtered over (i, j, L) by the beta filter of the 3x3 aspect
+ * pasp3 (mg_intstate.f90:395-410; p_sig = p_rho = 0), the 2-D variables by
+ * the 2-D filter of pasp2. L has the whole-sample mirror of sup_vrbeta3.
+ * ---------------------------------------------------------------------- */
+
+/* cholaspect3 (jp_pbfil.f90:108-128): inverse, then the lower Cholesky
+   factor (dl1lmf), of a symmetric 3x3 matrix a (row-major). */
+static void cholaspect3(const double a[9], double el[9]) {
+    double inv[9];
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8],
+                 c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    inv[0] = c00 / det;
+    inv[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    inv[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    inv[3] = c01 / det;
+    inv[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    inv[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    inv[6] = c02 / det;
+    inv[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    inv[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+    for (int e = 0; e < 9; ++e) el[e] = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        double sj = inv[j * 3 + j];
+        for (int q = 0; q < j; ++q) sj -= el[j * 3 + q] * el[j * 3 + q];
+        el[j * 3 + j] = sqrt(sj);
+        for (int i = j + 1; i < 3; ++i) {
+            double si = inv[i * 3 + j];
+            for (int q = 0; q < j; ++q) si -= el[i * 3 + q] * el[j * 3 + q];
+            el[i * 3 + j] = si / el[j * 3 + j];
+        }
+    }
+}
+
+/* Taps of the 3-D filter in vrbeta3's loop order (gz, gy, gx;
+   jp_pbfil.f90:955-971). Returns the count, or -1 beyond a halo. */
+static int taps3_raw(const mgbfo_params* P, int maxt, int* gxo, int* gyo, int* gzo, double* rrco) {
+    const double rmom2_3 = 1.0 / sqrt((double)(2 * P->p) + 5.0);
+    const double a3 = P->ampl03;
+    const double a[9] = {a3 * (1. + P->p_del), a3 * P->p_eps, 0.0,
+                         a3 * P->p_eps, a3, 0.0,
+                         0.0, 0.0, a3 * (1. - P->p_del)};
+    double el[9];
+    cholaspect3(a, el);
+    const double exx = el[0] * rmom2_3, eyy = el[4] * rmom2_3, ezz = el[8] * rmom2_3;
+    const double eyx = el[3] * rmom2_3, ezx = el[6] * rmom2_3, ezy = el[7] * rmom2_3;
+    const double z0 = 1.0 / ezz;
+    const int gzl = (int)ceil(-z0 + EPS), gzm = (int)floor(z0 - EPS);
+    if (gzl < -P->hz || gzm > P->hz) return -1;
+    int nt = 0;
+    for (int gz = gzl; gz <= gzm; ++gz) {
+        const double z = gz, yc = -z * ezy;
+        const double rry = (z * ezz) * (z * ezz), y0 = sqrt(1.0 - rry);
+        const int gyl = (int)ceil((yc - y0) / eyy + EPS), gym = (int)floor((yc + y0) / eyy - EPS);
+        if (gyl < -P->hy || gym > P->hy) return -1;
+        for (int gy = gyl; gy <= gym; ++gy) {
+            const double y = gy, xc = -y * eyx - z * ezx;
+            const double rrx = rry + (y * eyy - yc) * (y * eyy - yc), x0 = sqrt(1.0 - rrx);
+            const int gxl = (int)ceil((xc - x0) / exx + EPS), gxm = (int)floor((xc + x0) / exx - EPS);
+            if (gxl < -P->hx || gxm > P->hx) return -1;
+            for (int gx = gxl; gx <= gxm; ++gx) {
+                const double x = gx;
+                const double rr = rrx + (x * exx - xc) * (x * exx - xc);
+                if (nt < maxt) {
+                    gxo[nt] = gx;
+                    gyo[nt] = gy;
+                    gzo[nt] = gz;
+                    rrco[nt] = 1.0 - rr;
+                }
+                ++nt;
+            }
+        }
+    }
+    return nt;
+}
+
+typedef struct {
+    int nt;
+    int *gx, *gy, *gz;
+    double* w;
+} taps3_t;
+
+/* taps with weights ss3 rrc^p (getlinesum3 then vrbeta3) */
+static taps3_t taps3_make(const mgbfo_params* P) {
+    const int cap = (2 * P->hx + 1) * (2 * P->hy + 1) * (2 * P->hz + 1);
+    taps3_t T;
+    T.gx = malloc(sizeof(int) * cap);
+    T.gy = malloc(sizeof(int) * cap);
+    T.gz = malloc(sizeof(int) * cap);
+    T.w = malloc(sizeof(double) * cap);
+    T.nt = taps3_raw(P, cap, T.gx, T.gy, T.gz, T.w);
+    if (T.nt < 0) abort();   /* "In getlinesum3; filter reach becomes too large" */
+    double s = 0;
+    for (int t = 0; t < T.nt; ++t) s += ipow(T.w[t], P->p);
+    const double ss = 1.0 / s;
+    for (int t = 0; t < T.nt; ++t) T.w[t] = ss * ipow(T.w[t], P->p);
+    return T;
+}
+
+static void taps3_free(taps3_t* T) {
+    free(T->gx); free(T->gy); free(T->gz); free(T->w);
+}
+
+int mgbfo_taps3(const mgbfo_params* P, int maxt, int* gx, int* gy, int* gz, double* w) {
+    taps3_t T = taps3_make(P);
+    for (int t = 0; t < T.nt && t < maxt; ++t) {
+        gx[t] = T.gx[t];
+        gy[t] = T.gy[t];
+        gz[t] = T.gz[t];
+        w[t] = T.w[t];
+    }
+    const int nt = T.nt;
+    taps3_free(&T);
+    return nt;
+}
+
+/* The filter of filtering_rad3 on one generation of the stack (padded
+   field a with halos hx, hy). Forward: boco_2d, then the 3-D variables
+   through sup_vrbeta3 (levels padded by the whole-sample mirror, vrbeta3
+   gathers into the interior) and the 2-D variables through vrbeta2.
+   Adjoint: vrbeta3T scatters from the interior into the padded volume, the
+   level halo folds back (sup_vrbeta3T), the 2-D variables scatter
+   (vrbeta2T), then bocoT_2d. */
+static void rad3_filter(const mgbfo_params* P, const taps_t* T2, const taps3_t* T3, pf_t* a,
+                        int adjoint) {
+    const int K = a->K, n = a->n, m = a->m, hx = a->hx, hy = a->hy, hz = P->hz;
+    const int lm = P->lm, nv = P->nvcol, K3 = nv * lm, K2 = K - K3;
+    const int nxp = n + 2 * hx, nyp = m + 2 * hy, nlp = lm + 2 * hz;
+    /* one 3-D variable as W(i, j, L), padded in all three */
+#define W3(w, i, j, L) (w)[((size_t)((L) + hz - 1) * nyp + (size_t)((j) + hy - 1)) * nxp + (size_t)((i) + hx - 1)]
+    if (!adjoint) boco_2d(a, hx, hy);
+    pf_t b = pf_alloc(K, n, m, hx, hy);
+#pragma omp parallel for schedule(dynamic)
+    for (int v = 0; v < nv; ++v) {
+        double* W = calloc((size_t)nxp * nyp * nlp, sizeof(double));
+        double* B = calloc((size_t)nxp * nyp * nlp, sizeof(double));
+        for (int L = 1; L <= lm; ++L)
+            for (int j = 1 - hy; j <= m + hy; ++j)
+                for (int i = 1 - hx; i <= n + hx; ++i) W3(W, i, j, L) = PF(a, v * lm + L - 1, i, j);
+        if (!adjoint) {
+            for (int L = 1; L <= hz; ++L)
+                for (int j = 1 - hy; j <= m + hy; ++j)
+                    for (int i = 1 - hx; i <= n + hx; ++i) {
+                        W3(W, i, j, 1 - L) = W3(W, i, j, 1 + L);
+                        W3(W, i, j, lm + L) = W3(W, i, j, lm - L);
+                    }
+            for (int L = 1; L <= lm; ++L)
+                for (int j = 1; j <= m; ++j)
+                    for (int i = 1; i <= n; ++i) {
+                        double tb = 0;
+                        for (int t = 0; t < T3->nt; ++t)
+                            tb += T3->w[t] * W3(W, i + T3->gx[t], j + T3->gy[t], L + T3->gz[t]);
+                        PF(&b, v * lm + L - 1, i, j) = tb;
+                    }
+        } else {
+            for (int L = 1; L <= lm; ++L)
+                for (int j = 1; j <= m; ++j)
+                    for (int i = 1; i <= n; ++i) {
+                        const double ta = W3(W, i, j, L);
+                        for (int t = 0; t < T3->nt; ++t)
+                            W3(B, i + T3->gx[t], j + T3->gy[t], L + T3->gz[t]) += T3->w[t] * ta;
+                    }
+            for (int L = 1; L <= hz; ++L)
+                for (int j = 1 - hy; j <= m + hy; ++j)
+                    for (int i = 1 - hx; i <= n + hx; ++i) {
+                        W3(B, i, j, 1 + L) += W3(B, i, j, 1 - L);
+                        W3(B, i, j, lm - L) += W3(B, i, j, lm + L);
+                    }
+            for (int L = 1; L <= lm; ++L)
+                for (int j = 1 - hy; j <= m + hy; ++j)
+                    for (int i = 1 - hx; i <= n + hx; ++i) PF(&b, v * lm + L - 1, i, j) = W3(B, i, j, L);
+        }
+        free(W);
+        free(B);
+    }
+#undef W3
+    if (K2 > 0) {
+        /* the 2-D variables: the stack's tail as its own padded field */
+        pf_t a2 = pf_alloc(K2, n, m, hx, hy);
+        for (int j = 1 - hy; j <= m + hy; ++j)
+            for (int i = 1 - hx; i <= n + hx; ++i)
+                memcpy(PCOL(&a2, i, j), PCOL(a, i, j) + K3, sizeof(double) * K2);
+        if (!adjoint) vrbeta2(T2, &a2);
+        else vrbeta2T(T2, &a2);
+        for (int j = 1 - hy; j <= m + hy; ++j)
+            for (int i = 1 - hx; i <= n + hx; ++i)
+                memcpy(PCOL(&b, i, j) + K3, PCOL(&a2, i, j), sizeof(double) * K2);
+        pf_free(&a2);
+    }
+    free(a->d);
+    a->d = b.d;
+    if (adjoint) bocoT_2d(a, hx, hy);
+}
+
+/* ------------------------------------------------------------------------
  * Triad line filters (procedures 2 and 7, filtering_lin2_bckg / _ens).
  * init_mg_line resolves the aspect 3-vector of each point onto three
  * lattice lines (triad), turns each line's spread**2 into a real half-span
@@ -981,6 +1181,7 @@ typedef struct {
     taps_t T;
     fast_t F;
     line_t L;
+    taps3_t T3;
 } hfilt_t;
 
 static hfilt_t hfilt_make(const mgbfo_params* P) {
@@ -989,15 +1190,19 @@ static hfilt_t hfilt_make(const mgbfo_params* P) {
     if (P->line) H.L = line_make(P);
     else if (P->fast) H.F = fast_make(P);
     else H.T = taps2_make(P);
+    if (P->rad3) H.T3 = taps3_make(P);
     return H;
 }
 
 static void hfilt_free(hfilt_t* H) {
     if (H->T.nt > 0) taps_free(&H->T);
+    if (H->T3.gx) taps3_free(&H->T3);
 }
 
 static void hfilter(const mgbfo_params* P, const hfilt_t* H, pf_t* a, int adjoint) {
-    if (P->line) {
+    if (P->rad3) {
+        rad3_filter(P, &H->T, &H->T3, a, adjoint);
+    } else if (P->line) {
         line_filter(P, &H->L, a, adjoint);
     } else if (P->fast) {
         fast_filter(&H->F, a, adjoint);
@@ -1137,7 +1342,7 @@ static void weight_pf(const mgbfo_params* P, int gi, pf_t* f, const double* wcon
 static void upsweep_inputs(const mgbfo_params* P, const double* x, pf_t* G) {
     const int gm = P->gm;
     pf_from_interior(&G[0], x);
-    if (P->l_vertical_filter) vertical_op(P, &G[0], 1);
+    if (P->l_vertical_filter && !P->rad3) vertical_op(P, &G[0], 1);
     for (int g = 1; g < gm; ++g) {
         /* adjoint reads the interior of the finer generation only */
         pf_t Fi = pf_alloc(P->K, G[g - 1].n, G[g - 1].m, 0, 0);
@@ -1170,7 +1375,7 @@ static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y, int fin
         pf_free(&Wi);
         pf_free(&Fp);
     }
-    if (P->l_vertical_filter) vertical_op(P, &G[0], final_vft);
+    if (P->l_vertical_filter && !P->rad3) vertical_op(P, &G[0], final_vft);
     pf_to_interior(&G[0], y);
 }
 

@@ -39,6 +39,10 @@ typedef struct mgbfo_params {
     int line_tables;       /* mgbf_line: init_mg_line has set the line tables (without them
                               every line generator is 0 and each line is the identity) */
     int final_vft;         /* W A W^T ends with Vf^T instead of Vf (filtering_lin2_ens) */
+    int rad3;              /* filtering_rad3 (procedure 4): the first nvcol*lm entries of the
+                              stack are nvcol 3-D variables filtered over (i, j, L), the rest
+                              2-D variables; no separate vertical filter */
+    double ampl03;         /* 3-D aspect amplitude (pasp03 = mg_ampl03) */
 } mgbfo_params;
 
 /* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
@@ -73,6 +77,10 @@ void mgbfo_prolong2(int K, int n, int m, const double* coarse, double* fine);
    array (K, n+2h, m+2h); and its adjoint fold back to interior. */
 void mgbfo_pad(int K, int n, int m, int h, const double* in, double* padded);
 void mgbfo_fold(int K, int n, int m, int h, const double* padded, double* out);
+
+/* Taps of the 3-D filter (gx, gy, gz, w) in vrbeta3's order; returns the count
+   (fills up to maxt). Aborts beyond a halo. */
+int mgbfo_taps3(const mgbfo_params* P, int maxt, int* gx, int* gy, int* gz, double* w);
 
 /* ---- triad line filters (jp_pbfil3.f90) ---- */
 /* inimomtab: bnorm[1..nh], bsprds[1..nh] for exponent p. Nonzero on bad p/nh. */

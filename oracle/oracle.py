@@ -37,6 +37,7 @@ class _Params(ctypes.Structure):
         ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
         ("fast", ctypes.c_int), ("lquart", ctypes.c_int), ("line", ctypes.c_int),
         ("line_tables", ctypes.c_int), ("final_vft", ctypes.c_int),
+        ("rad3", ctypes.c_int), ("ampl03", ctypes.c_double),
     ]
 
 
@@ -57,6 +58,7 @@ def lib():
         L.mgbfo_vertical.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
         L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.mgbfo_taps3.argtypes = [P, ctypes.c_int, ip, ip, ip, dp]
         L.mgbfo_momtab.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_hstform.argtypes = [ctypes.c_int, dp]
         L.mgbfo_blinfil.argtypes = [ctypes.c_int, ctypes.c_double, ip, dp]
@@ -138,12 +140,15 @@ class OracleParams:
     line: bool = False     # filtering_lin2_* (procedures 2, 7): triad line filters
     line_tables: bool = True   # mgbf_line: the line tables are set (else identity lines)
     final_vft: bool = False    # W A W^T ends with Vf^T (filtering_lin2_ens)
+    rad3: bool = False     # filtering_rad3 (procedure 4): 3-D radial filter of the 3-D variables
+    ampl03: float = 1.0    # 3-D aspect amplitude (pasp03)
 
     def c(self) -> _Params:
         return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
                        self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
                        self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast),
-                       int(self.lquart), int(self.line), int(self.line_tables), int(self.final_vft))
+                       int(self.lquart), int(self.line), int(self.line_tables), int(self.final_vft),
+                       int(self.rad3), float(self.ampl03))
 
     def gen_shape(self, g: int):
         """(m_g, n_g, K) of generation g (0-based)."""
@@ -175,6 +180,16 @@ def taps1(P: OracleParams):
     if nt < 0:
         raise ValueError("filter reach exceeds the halo")
     return np.array(gx[:nt]), w[:nt].copy()
+
+
+def taps3(P: OracleParams):
+    """(gx, gy, gz, w) of the 3-D filter of procedure 4 in vrbeta3's order."""
+    c = P.c()
+    cap = (2 * P.hx + 1) * (2 * P.hy + 1) * (2 * P.hz + 1)
+    gx = (ctypes.c_int * cap)(); gy = (ctypes.c_int * cap)(); gz = (ctypes.c_int * cap)()
+    w = np.zeros(cap)
+    nt = lib().mgbfo_taps3(ctypes.byref(c), cap, gx, gy, gz, _dp(w))
+    return np.array(gx[:nt]), np.array(gy[:nt]), np.array(gz[:nt]), w[:nt].copy()
 
 
 def taps_fast(P: OracleParams):

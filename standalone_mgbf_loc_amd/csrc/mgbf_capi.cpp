@@ -74,6 +74,13 @@ struct mgbf_ctx {
     void* d_tables = nullptr;
     // vertical bands: 0 = Vf, 1 = Vf^T, 2 = Vf Vf^T
     const float* d_band[3] = {};
+    // procedure 4 (filtering_rad3): tap lists of the 3-D filter and of the
+    // 2-D filter of its 2-D variables
+    bool rad3 = false;
+    const int4* d_t3 = nullptr;
+    const float* d_w3 = nullptr;
+    const int4* d_t2 = nullptr;
+    const float* d_w2 = nullptr;
     mgbf::VBandW band_int[3] = {};
     int band_hb[3] = {}, band_ilo[3] = {}, band_ihi[3] = {};
     bool commute = true;     // weights constant along the stack: B = Vf Vf^T H
@@ -507,6 +514,83 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
     return upsweep(c, v, v, v, y, c->nfilt ? BAND_VF : -1, sqrt_weights, st);
 }
 
+// ---------------------------------------------------------------------------
+// Procedure 4, filtering_rad3 (mg_filtering.f90:123-257): every generation
+// filtered by F (3-D variables over (i, j, L), 2-D variables in the plane),
+// no separate vertical filter:  y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
+// ---------------------------------------------------------------------------
+int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st) {
+    Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st, 8.0 * (double)c->gsize[g]);
+    mgbf::BallArgs a{};
+    a.in = in;
+    a.out = out;
+    a.t3 = c->d_t3;
+    a.w3 = c->d_w3;
+    a.n3 = (int)c->s.taps3.size();
+    a.t2 = c->d_t2;
+    a.w2 = c->d_w2;
+    a.n2 = (int)c->s.taps2.size();
+    a.K = c->K;
+    a.n = c->nx[g];
+    a.m = c->ny[g];
+    a.lm = c->s.cfg.lm;
+    a.K3 = c->s.dims.km3_all * c->s.cfg.lm;
+    a.transpose = transpose ? 1 : 0;
+    HIP_TRY(mgbf::launch_ball(a, st), "3-D filter launch");
+    return MGBF_OK;
+}
+
+// s_g = in_g + P s_{g+1} from the coarsest generation up, in place in U
+// (generation 1 into y)
+int rad3_upsweep(mgbf_ctx* c, float* U, float* y, hipStream_t st) {
+    for (int g = c->gm - 2; g >= 0; --g)
+        CALL(prolong_g(c, g, U + c->goff[g + 1], U + c->goff[g], g == 0 ? y : U + c->goff[g], st));
+    if (c->gm == 1) HIP_TRY(mgbf::launch_copy(U, y, c->gsize[0], st), "copy launch");
+    return MGBF_OK;
+}
+
+int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
+    float* U = c->ws_u;
+    float* T = c->ws_t;
+    CALL(restrict_chain(c, x, U, st));
+    for (int g = 0; g < c->gm; ++g) {
+        CALL(ball(c, g, g == 0 ? x : U + c->goff[g], T + c->goff[g], true, st));
+        HIP_TRY(mgbf::launch_scale(T + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 0,
+                                   c->gsize[g], st),
+                "scale launch");
+        CALL(ball(c, g, T + c->goff[g], U + c->goff[g], false, st));
+    }
+    return rad3_upsweep(c, U, y, st);
+}
+
+int apply_wt_rad3(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
+    CALL(restrict_chain(c, x, c->ws_u, st));
+    for (int g = 0; g < c->gm; ++g) {
+        CALL(ball(c, g, g == 0 ? x : c->ws_u + c->goff[g], v + c->goff[g], true, st));
+        if (sqrt_weights)
+            HIP_TRY(mgbf::launch_scale(v + c->goff[g], v + c->goff[g], c->wfield[g], c->wconst[g], 1,
+                                       c->gsize[g], st),
+                    "scale launch");
+    }
+    return MGBF_OK;
+}
+
+int apply_w_rad3(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
+    float* U = c->ws_u;
+    float* T = c->ws_t;
+    for (int g = 0; g < c->gm; ++g) {
+        const float* src = v + c->goff[g];
+        if (sqrt_weights) {
+            HIP_TRY(mgbf::launch_scale(src, T + c->goff[g], c->wfield[g], c->wconst[g], 1, c->gsize[g],
+                                       st),
+                    "scale launch");
+            src = T + c->goff[g];
+        }
+        CALL(ball(c, g, src, U + c->goff[g], false, st));
+    }
+    return rad3_upsweep(c, U, y, st);
+}
+
 // filt_to_anal_all2: analysis field = T_v T_h (generation-1 field), the
 // horizontal 4-point interpolation first, then lm -> lm_a levels.
 int apply_f2a(mgbf_ctx* c, const float* in, float* out, hipStream_t st) {
@@ -643,6 +727,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->vf_last = cfg->mgbf_proc == 7 ? BAND_VFT : BAND_VF;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     c->quad = cfg->lquart != 0;
+    c->rad3 = cfg->mgbf_proc == 4;
     {
         // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for
         // weights that are constant along the stack
@@ -759,6 +844,10 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (const auto* t : tabs)
             bytes += carve_size(sizeof(int) * t->start.size()) + carve_size(sizeof(float) * t->w.size());
     }
+    if (c->rad3) {
+        bytes += carve_size(sizeof(int4) * s.taps3.size()) + carve_size(sizeof(float) * s.taps3.size());
+        bytes += carve_size(sizeof(int4) * s.taps2.size()) + carve_size(sizeof(float) * s.taps2.size());
+    }
     std::vector<char> host(bytes, 0);
     hipError_t e = hipMalloc(&c->d_tables, bytes);
     if (e != hipSuccess) {
@@ -787,6 +876,23 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     }
     c->d_dot = carve<double>(dp, 1);
     carve<double>(hp, 1);
+    if (c->rad3) {
+        auto put_taps = [&](size_t nt, auto tap, const int4** dt, const float** dw) {
+            int4* ht = carve<int4>(hp, nt);
+            *dt = carve<int4>(dp, nt);
+            float* hw = carve<float>(hp, nt);
+            *dw = carve<float>(dp, nt);
+            for (size_t t = 0; t < nt; ++t) tap(t, ht[t], hw[t]);
+        };
+        put_taps(s.taps3.size(), [&](size_t t, int4& g, float& w) {
+            g = make_int4(s.taps3[t].gx, s.taps3[t].gy, s.taps3[t].gz, 0);
+            w = (float)s.taps3[t].w;
+        }, &c->d_t3, &c->d_w3);
+        put_taps(s.taps2.size(), [&](size_t t, int4& g, float& w) {
+            g = make_int4(s.taps2[t].gx, s.taps2[t].gy, 0, 0);
+            w = (float)s.taps2[t].w;
+        }, &c->d_t2, &c->d_w2);
+    }
     for (int g = 0; g + 1 < c->gm; ++g) {
         c->d_rxs[g] = put_i(s.rx[g].start);
         c->d_rxw[g] = put_f(s.rx[g].w);
@@ -872,6 +978,16 @@ int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
     if (!c || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
     DeviceGuard guard(c->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (c->rad3) {
+        switch (op) {
+            case MGBF_OP_WAWT: return apply_wawt_rad3(c, in, out, st);
+            case MGBF_OP_WT: return apply_wt_rad3(c, in, out, false, st);
+            case MGBF_OP_W: return apply_w_rad3(c, in, out, false, st);
+            case MGBF_OP_BSQRT: return apply_w_rad3(c, in, out, true, st);
+            case MGBF_OP_BSQRTT: return apply_wt_rad3(c, in, out, true, st);
+            default: break;
+        }
+    }
     switch (op) {
         case MGBF_OP_WAWT: return apply_wawt(c, in, out, st);
         case MGBF_OP_WT: return apply_wt(c, in, out, false, st);
@@ -901,6 +1017,7 @@ int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* ou
             return vertical(c, in, out, stage == MGBF_STAGE_VFT ? BAND_VFT : BAND_VF, st);
         case MGBF_STAGE_F:
         case MGBF_STAGE_FT:
+            if (c->rad3) return ball(c, g, in, out, stage == MGBF_STAGE_FT, st);
             return radial(c, g, in, out, nullptr, 1.0f, st);
         case MGBF_STAGE_R:
             if (g + 1 >= c->gm) return fail(MGBF_ERR_ARG, "no coarser generation");
@@ -931,6 +1048,7 @@ int mgbf_dot(mgbf_ctx* c, const float* a, const float* b, long long n, double* r
 
 int mgbf_autotune(mgbf_ctx* c, void* stream) {
     if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    if (c->rad3) return MGBF_OK;   // procedure 4 runs no radial kernel: nothing to tune
     DeviceGuard guard(c->device);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const size_t mgb = sizeof(float) * (size_t)c->s.dims.mg_size;

@@ -480,6 +480,34 @@ bool ball_reach_ok(const mgbf_config& cfg, std::string* err) {
     return true;
 }
 
+// Taps of the 3-D filter of procedure 4 (getlinesum3 + the vrbeta3 loop,
+// jp_pbfil.f90:220-262, 930-977) for pasp3 = mg_ampl03 I: the lattice ball
+// |g| < sqrt(mg_ampl03 (2p+5)) visited z, y, x, weights ss3 (1-|g e|^2)^p.
+void ball_taps(const mgbf_config& cfg, std::vector<Tap3>& taps) {
+    taps.clear();
+    const double e = std::sqrt(1.0 / cfg.mg_ampl03) / std::sqrt(2.0 * cfg.p + 5.0);
+    const double z = 1.0 / e;
+    const int gzl = (int)std::ceil(-z + kEps), gzm = (int)std::floor(z - kEps);
+    std::vector<double> rrc;
+    for (int gz = gzl; gz <= gzm; ++gz) {
+        const double rry = (gz * e) * (gz * e);
+        const double y = std::sqrt(1.0 - rry);
+        const int gyl = (int)std::ceil(-y / e + kEps), gym = (int)std::floor(y / e - kEps);
+        for (int gy = gyl; gy <= gym; ++gy) {
+            const double rrx = rry + (gy * e) * (gy * e);
+            const double x = std::sqrt(std::max(0.0, 1.0 - rrx));
+            const int gxl = (int)std::ceil(-x / e + kEps), gxm = (int)std::floor(x / e - kEps);
+            for (int gx = gxl; gx <= gxm; ++gx) {
+                taps.push_back({gx, gy, gz, 0.0});
+                rrc.push_back(1.0 - (rrx + (gx * e) * (gx * e)));
+            }
+        }
+    }
+    double s = 0;
+    for (double r : rrc) s += ipow(r, cfg.p);
+    for (size_t t = 0; t < taps.size(); ++t) taps[t].w = ipow(rrc[t], cfg.p) / s;
+}
+
 bool fast_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     // the fast filter's x and y factors keep the aspect amplitude as the
     // factor itself: paspx = paspy = mg_ampl02 reach getlinesum1 and vrbeta1
@@ -501,10 +529,10 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "invalid namelist value (sizes must be positive)";
         return MGBF_ERR_CONFIG;
     }
-    if (c.mgbf_proc < 1 || c.mgbf_proc > 8 || c.mgbf_proc == 4 || c.mgbf_proc == 5) {
+    if (c.mgbf_proc < 1 || c.mgbf_proc > 8 || c.mgbf_proc == 5) {
         *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 2 filtering_lin2_bckg, "
-                   "3 filtering_fast_bckg, 6 filtering_rad2_ens, 7 filtering_lin2_ens, "
-                   "8 filtering_fast_ens)",
+                   "3 filtering_fast_bckg, 4 filtering_rad3, 6 filtering_rad2_ens, "
+                   "7 filtering_lin2_ens, 8 filtering_fast_ens)",
                    c.mgbf_proc);
         return MGBF_ERR_UNSUPPORTED;
     }
@@ -514,7 +542,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     }
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
-    const bool bckg = c.mgbf_proc == 1 || c.mgbf_proc == 2 || c.mgbf_proc == 3;
+    const bool bckg = c.mgbf_proc >= 1 && c.mgbf_proc <= 4;
     if (bckg) {
         if (c.n_ens != 1) {
             *err = fmt("mgbf_proc=%d (background) needs n_ens=1", c.mgbf_proc);
@@ -606,7 +634,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     d.ma = c.mm0;
     d.km_a_all = (bckg ? 1 : c.n_ens) * (c.km2 + c.km3 * c.lm_a);
     d.anal_size = (long long)d.km_a_all * d.na * d.ma;
-    if (c.l_vertical_filter && c.lm <= c.hz) {
+    if ((c.l_vertical_filter || (c.mgbf_proc == 4 && c.km3 > 0)) && c.lm <= c.hz) {
         *err = "lm must exceed hz for the vertical mirror";
         return MGBF_ERR_CONFIG;
     }
@@ -659,7 +687,9 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         *err = "In getlinesum1; filter reach fx becomes too large for hz";
         return MGBF_ERR_CONFIG;
     }
+    if (cfg.mgbf_proc == 4) ball_taps(cfg, s.taps3);   // filtering_rad3 (reach checked above)
     s.dims.ntaps2 = (int)s.taps2.size();
+    s.dims.ntaps3 = (int)s.taps3.size();
     s.dims.ntaps1 = (int)s.taps1_gx.size();
     s.H = cfg.hx > cfg.hy ? cfg.hx : cfg.hy;
     if (fast || line) {

@@ -16,6 +16,11 @@ constexpr int kPWidth = 4;        // prolongation window per fine index
 constexpr int kMaxXferWidth = 16; // analysis-filter interpolation window
 
 // One tap of a beta filter (offsets from the output point, FP64 weight).
+struct Tap3 {
+    int gx, gy, gz;
+    double w;
+};
+
 struct Tap2 {
     int gx, gy;
     double w;
@@ -35,6 +40,7 @@ struct Setup {
     mgbf_dims dims{};
     int nx[kMaxGen] = {}, ny[kMaxGen] = {};
     std::vector<Tap2> taps2;           // reference loop order
+    std::vector<Tap3> taps3;           // 3-D filter of procedure 4 (vrbeta3 loop order)
     std::vector<int> taps1_gx;         // vertical taps
     std::vector<double> taps1_w;
     int H = 0;                          // max(hx, hy) used by the radial kernels

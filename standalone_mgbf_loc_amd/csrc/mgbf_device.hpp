@@ -17,6 +17,14 @@ __device__ __forceinline__ int mirror_half(int q, int n) {
     return q < 0 ? 0 : (q >= n ? n - 1 : q);
 }
 
+// Whole-sample mirror (X(1-t)=X(1+t), X(n+t)=X(n-t), 1-based), clamped
+// into the domain.
+__device__ __forceinline__ int mirror_whole(int q, int n) {
+    q = q < 0 ? -q : q;
+    q = q >= n ? 2 * (n - 1) - q : q;
+    return q < 0 ? 0 : (q >= n ? n - 1 : q);
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous range of logical blocks so neighbouring tiles share its L2.
 __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {

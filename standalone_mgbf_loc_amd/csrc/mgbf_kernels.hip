@@ -523,6 +523,60 @@ kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int 
     }
 }
 
+// 3-D radial filter of procedure 4 (vrbeta3 / vrbeta3T with sup_vrbeta3's
+// level mirror) and the 2-D filter of its 2-D variables, as a gather: one
+// lane per stacked k, one wave per 64 k of a grid point, taps wave-uniform.
+// The transpose gathers every level L whose mirrored shift L + c lands on
+// the output level (at most one direct and one mirrored preimage).
+template <bool T>
+__global__ void __launch_bounds__(kBlock) ball_kernel(BallArgs a, int nkc, long long nitems) {
+    const long long item = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int kc = (int)(item % nkc);
+    const long long rest = item / nkc;
+    const int i = (int)(rest % a.n), j = (int)(rest / a.n);
+    const int k = kc * kWave + (threadIdx.x & (kWave - 1));
+    if (k >= a.K) return;
+    const int lm = a.lm;
+    float acc = 0.f;
+    if (k < a.K3) {
+        const int v0 = (k / lm) * lm, L = k - v0;
+        for (int t = 0; t < a.n3; ++t) {
+            const int4 g = a.t3[t];
+            const int ii = mirror_half(i + g.x, a.n), jj = mirror_half(j + g.y, a.m);
+            const float* col = a.in + (size_t)a.K * ((size_t)ii + (size_t)a.n * jj) + v0;
+            float s;
+            if (!T) {
+                s = col[mirror_whole(L + g.z, lm)];
+            } else {
+                const int L1 = L - g.z, L2 = -L - g.z, L3 = 2 * (lm - 1) - L - g.z;
+                s = (L1 >= 0 && L1 < lm) ? col[L1] : 0.f;
+                if (L > 0 && L2 >= 0 && L2 < lm) s += col[L2];
+                if (L < lm - 1 && L3 >= 0 && L3 < lm) s += col[L3];
+            }
+            acc = __builtin_fmaf(a.w3[t], s, acc);
+        }
+    } else {
+        for (int t = 0; t < a.n2; ++t) {
+            const int4 g = a.t2[t];
+            const int ii = mirror_half(i + g.x, a.n), jj = mirror_half(j + g.y, a.m);
+            acc = __builtin_fmaf(a.w2[t], a.in[k + (size_t)a.K * ((size_t)ii + (size_t)a.n * jj)], acc);
+        }
+    }
+    a.out[k + (size_t)a.K * ((size_t)i + (size_t)a.n * j)] = acc;
+}
+
+hipError_t launch_ball(const BallArgs& a, hipStream_t st) {
+    const int nkc = (a.K + kWave - 1) / kWave;
+    const long long nitems = (long long)nkc * a.n * a.m;
+    const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (a.transpose)
+        hipLaunchKernelGGL(ball_kernel<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nkc, nitems);
+    else
+        hipLaunchKernelGGL(ball_kernel<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nkc, nitems);
+    return hipGetLastError();
+}
+
 hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st) {
     const long long nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = nkc * a.nout_x * a.nout_y;

@@ -140,4 +140,23 @@ hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st);
 
+// The filter of procedure 4 (filtering_rad3) on one generation: the first
+// K3 = nv*lm entries of the stack are nv columns of lm levels filtered over
+// (i, j, L) by the 3-D taps, the rest by the 2-D taps; half-sample mirror
+// in i and j, whole-sample mirror in L. transpose: the adjoint (the 2-D and
+// horizontal parts are symmetric; the level mirror is transposed).
+struct BallArgs {
+    const float* in;
+    float* out;
+    const int4* t3;      // 3-D taps (a, b, c, 0)
+    const float* w3;
+    int n3;
+    const int4* t2;      // 2-D taps (a, b, 0, 0) of the 2-D variables
+    const float* w2;
+    int n2;
+    int K, n, m, lm, K3;
+    int transpose;
+};
+hipError_t launch_ball(const BallArgs& a, hipStream_t st);
+
 }  // namespace mgbf

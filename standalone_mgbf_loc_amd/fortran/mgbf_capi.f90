@@ -35,6 +35,7 @@ type, bind(C) :: mgbf_dims_t
   integer(c_long_long) :: field_size, mg_size
   integer(c_int) :: na, ma, km_a_all
   integer(c_long_long) :: anal_size
+  integer(c_int) :: ntaps3
 end type mgbf_dims_t
 
 interface

@@ -59,7 +59,7 @@ class _CDims(ctypes.Structure):
                 ("ntaps2", ctypes.c_int), ("ntaps1", ctypes.c_int),
                 ("field_size", ctypes.c_longlong), ("mg_size", ctypes.c_longlong),
                 ("na", ctypes.c_int), ("ma", ctypes.c_int), ("km_a_all", ctypes.c_int),
-                ("anal_size", ctypes.c_longlong)]
+                ("anal_size", ctypes.c_longlong), ("ntaps3", ctypes.c_int)]
 
 
 @dataclasses.dataclass
@@ -221,6 +221,7 @@ class Dims:
     ma: int
     km_a_all: int
     anal_size: int
+    ntaps3: int
 
     def gen_shape(self, g: int):
         """(ny_g, nx_g, K) of generation g (1-based)."""

@@ -75,6 +75,44 @@ def radial_matrix(n: int, m: int, taps) -> np.ndarray:
     return F
 
 
+def ball_taps(h: int, hz: int, R2: Fraction, p: int):
+    """Taps of the isotropic 3-D beta filter: lattice points strictly inside
+    the ball a^2 + b^2 + c^2 < R2, weight (1 - r^2/R2)^p, unit sum."""
+    taps = []
+    for c in range(-hz, hz + 1):
+        for b in range(-h, h + 1):
+            for a in range(-h, h + 1):
+                r2 = Fraction(a * a + b * b + c * c)
+                if r2 < R2:
+                    taps.append((a, b, c, (1 - r2 / R2) ** p))
+    s = sum(t[3] for t in taps)
+    return [(a, b, c, float(w / s)) for a, b, c, w in taps]
+
+
+def rad3_matrix(n: int, m: int, K: int, lm: int, nv: int, taps3, taps2) -> np.ndarray:
+    """The filter of procedure 4 on an (m, n, K) generation: the first nv*lm
+    entries of k are nv columns of lm levels filtered over (i, j, L) with the
+    half-sample mirror in i and j and the whole-sample mirror in L; the rest
+    of k by the 2-D filter."""
+    N = n * m * K
+    F = np.zeros((N, N))
+    for j in range(m):
+        for i in range(n):
+            for k in range(K):
+                row = (j * n + i) * K + k
+                if k < nv * lm:
+                    v, L = divmod(k, lm)
+                    for a, b, c, w in taps3:
+                        ii, jj = mirror_half(i + a, n), mirror_half(j + b, m)
+                        kk = v * lm + mirror_whole(L + c, lm)
+                        F[row, (jj * n + ii) * K + kk] += w
+                else:
+                    for a, b, w in taps2:
+                        ii, jj = mirror_half(i + a, n), mirror_half(j + b, m)
+                        F[row, (jj * n + ii) * K + k] += w
+    return F
+
+
 def vertical_matrix(lm: int, taps) -> np.ndarray:
     V = np.zeros((lm, lm))
     for L in range(lm):
@@ -131,7 +169,9 @@ class DenseMGBF:
 
     def __init__(self, nx, ny, K, lm, nvcol, hx, hz, p, gm, R2, R2v,
                  weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False,
-                 quad=False, taps2d=None, final_vft=False):
+                 quad=False, taps2d=None, final_vft=False, rad3=None):
+        """rad3 = (R2 of the 3-D ball, hz): procedure 4, whose filter couples
+        the levels of each 3-D variable and which has no vertical filter."""
         self.nx, self.ny, self.K, self.gm = nx, ny, K, gm
         self.final_vft = final_vft
         if taps2d is not None:
@@ -152,9 +192,15 @@ class DenseMGBF:
             Vcol[c * lm:(c + 1) * lm, c * lm:(c + 1) * lm] = V1
         self.Vf = np.kron(np.eye(nx * ny), Vcol) if l_vertical_filter else np.eye(nx * ny * K)
         self.F, self.P, self.A = [], [], []
+        if rad3 is not None:
+            self.Vf = np.eye(nx * ny * K)
+            t3 = ball_taps(hx, rad3[1], Fraction(rad3[0]), p)
         for g in range(gm):
             n, m = nx >> g, ny >> g
-            self.F.append(np.kron(radial_matrix(n, m, taps), IK))
+            if rad3 is not None:
+                self.F.append(rad3_matrix(n, m, K, lm, nvcol, t3, taps))
+            else:
+                self.F.append(np.kron(radial_matrix(n, m, taps), IK))
             if wfield is not None and wfield[g] is not None:
                 self.A.append(np.diag(np.asarray(wfield[g], dtype=np.float64).ravel()))
             else:

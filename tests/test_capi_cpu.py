@@ -77,8 +77,7 @@ def test_nm0_divisibility_is_checked():
     assert "not divisible" in str(_err(cfg))
 
 
-@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 4), ("mgbf_proc", 5),
-                                     ("mgbf_proc", 9)])
+@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 5), ("mgbf_proc", 9)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)
     setattr(cfg, key, val)
@@ -139,3 +138,20 @@ def test_every_filter_of_the_namelist_must_fit_its_halo(key, val, msg):
     assert M.derive(cfg).gm == 3
     setattr(cfg, key, val)
     assert msg in str(_err(cfg))
+
+
+@pytest.mark.parametrize("ampl03,p,hz", [(1.0, 2, 2), (25 / 9, 2, 4), (4 / 7, 1, 2)])
+def test_rad3_procedure_taps_follow_the_oracle(ampl03, p, hz):
+    from oracle import oracle as O
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=hz, mgbf_proc=4, km3=2, p=p)
+    cfg.mg_ampl03 = ampl03
+    P = O.OracleParams(nx=64, ny=64, K=16, lm=8, nvcol=2, hx=4, hy=4, hz=hz, p=p, rad3=True,
+                       ampl03=ampl03)
+    gx, gy, gz, w = O.taps3(P)
+    d = M.derive(cfg)
+    assert d.ntaps3 == len(w) and d.km_all == 16
+
+
+def test_rad3_procedure_needs_more_levels_than_hz():
+    cfg = M.Config.for_grid(64, 64, 2, gm=3, h=4, hz=2, mgbf_proc=4, km3=2, l_vertical_filter=False)
+    assert "lm must exceed hz" in str(_err(cfg))

@@ -30,14 +30,14 @@ def relerr(y, y64):
 
 
 def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
-    bckg = cfg.mgbf_proc in (1, 3)
+    bckg = cfg.mgbf_proc in (1, 2, 3, 4)
     return O.OracleParams(
         nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx, hy=cfg.hy,
         hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
         l_filt_g1=True if bckg else bool(cfg.l_filt_g1),
         l_vertical_filter=bool(cfg.l_vertical_filter), fast=cfg.mgbf_proc in (3, 8),
         lquart=bool(cfg.lquart), line=cfg.mgbf_proc in (2, 7), line_tables=bool(cfg.mgbf_line),
-        final_vft=cfg.mgbf_proc == 7,
+        final_vft=cfg.mgbf_proc == 7, rad3=cfg.mgbf_proc == 4, ampl03=cfg.mg_ampl03,
         weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
 
 
@@ -401,6 +401,45 @@ def test_line_procedures_match_oracle(monkeypatch, case):
         wt = op.apply(M.OP_WT, xd)
         w = op.apply(M.OP_W, vd)
         sync()
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs = op.dot(w, xd)
+        rhs = op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("case", ["3d_only", "with_2d", "lquart", "wide"])
+def test_rad3_procedure_matches_oracle(case):
+    # filtering_rad3 (mgbf_proc = 4): the 3-D variables filtered over
+    # (i, j, L) by the beta ball of mg_ampl03 (whole-sample mirror in L), the
+    # 2-D variables by the 2-D filter; every generation, no vertical filter
+    kw = dict(gm=3, h=4, hz=2, mgbf_proc=4, km3=3, weights=[1.0, 0.5, 0.25])
+    if case == "with_2d":
+        kw["km2"] = 5
+    if case == "lquart":
+        kw.update(km2=2, lquart=True)
+    cfg = M.Config.for_grid(64, 48, 12, **kw)
+    if case == "wide":
+        cfg.hz = 4
+        cfg.mg_ampl03 = 25 / 9     # |g| < 5: 4 points of reach in i, j and L (485 taps)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        assert d.ntaps3 == len(O.taps3(P)[3])
+        x = field(d.gen_shape(1), 160)
+        v = np.random.default_rng(161).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        f = op.stage(M.STAGE_F, 1, xd)
+        ft = op.stage(M.STAGE_FT, 2, to_dev(field(d.gen_shape(2), 162)))
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        assert relerr(f, O.radial(P, x.astype(np.float64))) <= TOL_STAGE
+        x2 = field(d.gen_shape(2), 162).astype(np.float64)
+        assert relerr(ft, O.radial(P, x2, adjoint=True)) <= TOL_STAGE
         assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
         assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
         assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP

@@ -546,3 +546,53 @@ def test_line_procedures_match_dense_model(final_vft, filt_g1, tables):
                      taps2d=taps, final_vft=final_vft)
     Bd = Dm.B()
     assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+
+
+# --------------------------------------------------------------------------
+# 3-D radial filter (filtering_rad3, procedure 4)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("ampl03,p,count", [(1.0, 2, 93), (Fraction(16, 9), 2, 251), (Fraction(4, 7), 1, 27)])
+def test_ball_tap_counts_and_weights(ampl03, p, count):
+    # lattice points strictly inside the ball |g|^2 < ampl03 (2p+5), weights
+    # (1 - |g|^2/R^2)^p with unit sum, visited z, y, x
+    P = _P(rad3=True, ampl03=float(ampl03), p=p, hx=4, hy=4, hz=4)
+    gx, gy, gz, w = O.taps3(P)
+    assert len(w) == count
+    R2 = Fraction(ampl03) * (2 * p + 5)
+    ref = [(1 - Fraction(a * a + b * b + c * c) / R2) ** p for a, b, c in zip(gx, gy, gz)]
+    assert np.allclose(w, np.array([float(r) for r in ref]) / float(sum(ref)), rtol=1e-13)
+    assert list(gz) == sorted(gz)
+
+
+def test_rad3_filter_adjoint_constants_and_variable_split():
+    P = _P(nx=16, ny=12, K=8, lm=3, nvcol=2, hx=3, hy=3, hz=2, gm=2, rad3=True, ampl03=1.0,
+           ampl02=Fraction(16, 8))
+    x, y = _rand((12, 16, 8), 95), _rand((12, 16, 8), 96)
+    lhs = np.vdot(O.radial(P, x), y)
+    rhs = np.vdot(x, O.radial(P, y, adjoint=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    # constants: preserved by F (the mirrors repeat interior values), not by F^T
+    assert np.allclose(O.radial(P, np.ones((12, 16, 8))), 1.0, atol=1e-14)
+    # the 2-D variables (k >= nvcol*lm) see only the 2-D filter
+    z = np.zeros((12, 16, 8)); z[5, 7, 7] = 1.0
+    out = O.radial(P, z)
+    assert np.abs(out[..., :6]).max() == 0.0
+    P2 = _P(nx=16, ny=12, K=1, lm=3, nvcol=0, hx=3, hy=3, hz=2, ampl02=Fraction(16, 8))
+    assert np.allclose(out[..., 7:], O.radial(P2, z[..., 7:]), atol=1e-15)
+
+
+@pytest.mark.parametrize("lquart", [False, True])
+def test_rad3_wawt_matches_dense_model(lquart):
+    P = _P(nx=16, ny=12, K=5, lm=3, nvcol=1, hx=3, hy=3, hz=2, gm=2, rad3=True, ampl03=1.0,
+           ampl02=Fraction(16, 8), weights=[1.0, 0.6], lquart=lquart, l_vertical_filter=True)
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    Dm = D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm,
+                     Fraction(P.ampl02).limit_denominator(1000) * (2 * P.p + 4),
+                     Fraction(4, 7), list(P.wconst()), True, True, None, quad=lquart,
+                     rad3=(Fraction(1) * (2 * P.p + 5), P.hz))
+    Bd = Dm.B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+    assert np.max(np.abs(B - B.T)) < 1e-13 * np.max(np.abs(B))
