@@ -4,7 +4,10 @@
 
 Each fixture holds seeded inputs, the configuration and the FP64 outputs of
 W A W^T, W^T and the single stages at small sizes. They are regression
-vectors: the oracle itself is pinned by tests/test_oracle.py.
+vectors: the oracle itself is pinned by tests/test_oracle.py. The json file
+holds the oracle's parameters, the namelist values the library is built
+from ("cfg", on top of Config.for_grid) and, for the stages, whether the
+generation transfer is the quadratic one ("quad").
 """
 import json
 import os
@@ -16,34 +19,57 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle import oracle as O  # noqa: E402
 
+# name: grid, stack and filter of the case; the seed is the case's position
 CASES = {
-    # name: (nx, ny, n_ens, lm, gm, h, hz, weights, l_filt_g1)
-    "ens_32x32_gm3_h3": (32, 32, 2, 2, 3, 3, 1, [1.0, 0.6, 0.3], True),
-    "ens_24x24_gm2_h6_nofilt1": (24, 24, 2, 2, 2, 6, 1, [1.0, 0.5], False),
-    "single_32x24_gm3_h4": (32, 24, 1, 3, 3, 4, 2, [1.0, 0.5, 0.25], True),
+    "ens_32x32_gm3_h3": dict(nx=32, ny=32, n_ens=2, lm=2, gm=3, h=3, hz=1, weights=[1.0, 0.6, 0.3]),
+    "ens_24x24_gm2_h6_nofilt1": dict(nx=24, ny=24, n_ens=2, lm=2, gm=2, h=6, hz=1, weights=[1.0, 0.5],
+                                     l_filt_g1=False),
+    "single_32x24_gm3_h4": dict(nx=32, ny=24, n_ens=1, lm=3, gm=3, h=4, hz=2, weights=[1.0, 0.5, 0.25]),
+    # filtering_rad2_ens with the quadratic transfer (lquart)
+    "lquart_ens_32x32_gm3_h3": dict(nx=32, ny=32, n_ens=2, lm=4, gm=3, h=3, hz=1,
+                                    weights=[1.0, 0.6, 0.3], lquart=True),
+    # filtering_lin2_ens (mgbf_proc 7, mgbf_line): triad line filters, Vf^T at the end
+    "line_ens_32x24_gm3_h6": dict(nx=32, ny=24, n_ens=2, lm=4, gm=3, h=6, hz=2,
+                                  weights=[1.0, 0.5, 0.25], proc=7, ampl02=0.5),
+    # filtering_lin2_bckg (mgbf_proc 2) with 2-D variables
+    "line_bckg_32x32_gm3_km2": dict(nx=32, ny=32, n_ens=1, lm=3, km3=2, km2=2, gm=3, h=4, hz=2,
+                                    weights=[1.0, 0.5, 0.25], proc=2, ampl02=0.75),
+    # filtering_rad3 (mgbf_proc 4): 3-D variables over (i, j, L), 2-D variables
+    "rad3_32x24_gm3_km2": dict(nx=32, ny=24, n_ens=1, lm=5, km3=2, km2=3, gm=3, h=4, hz=2,
+                               weights=[1.0, 0.5, 0.25], proc=4),
 }
 
 
-def config(nx, ny, n_ens, lm, gm, h, hz, weights, filt_g1):
-    return dict(nx=nx, ny=ny, K=n_ens * lm, lm=lm, nvcol=n_ens, hx=h, hy=h, hz=hz, p=2, gm=gm,
-                ampl01=(hz + 1) ** 2 / 7.0, ampl02=(h + 1) ** 2 / 8.0, weights=weights,
-                l_filt_g1=filt_g1, n_ens=n_ens)
+def config(nx, ny, n_ens, lm, gm, h, hz, weights, l_filt_g1=True, lquart=False, proc=6,
+           km3=1, km2=0, ampl02=None):
+    ampl01 = (hz + 1) ** 2 / 7.0
+    a2 = (h + 1) ** 2 / 8.0 if ampl02 is None else ampl02
+    a3 = (min(h, hz) + 1) ** 2 / 9.0
+    bckg = proc in (1, 2, 3, 4)
+    c = dict(nx=nx, ny=ny, K=n_ens * (km2 + lm * km3), lm=lm, nvcol=n_ens * km3, hx=h, hy=h, hz=hz,
+             p=2, gm=gm, ampl01=ampl01, ampl02=a2, weights=weights,
+             l_filt_g1=True if bckg else l_filt_g1, n_ens=n_ens, lquart=lquart,
+             line=proc in (2, 7), line_tables=True, final_vft=proc == 7, rad3=proc == 4, ampl03=a3)
+    cfg = dict(mgbf_proc=proc, lquart=lquart, mgbf_line=proc in (2, 7), km2=km2, km3=km3,
+               l_filt_g1=l_filt_g1, mg_ampl02=a2)
+    return c, cfg
 
 
 def main():
     for seed, (name, args) in enumerate(CASES.items()):
-        c = config(*args)
+        c, cfg = config(**args)
         P = O.OracleParams(**{k: v for k, v in c.items() if k != "n_ens"})
         rng = np.random.default_rng(1000 + seed)
         x = rng.standard_normal(P.gen_shape(0)).astype(np.float32).astype(np.float64)
         x1 = rng.standard_normal(P.gen_shape(1)).astype(np.float32).astype(np.float64)
+        q = bool(c["lquart"])
         out = dict(x=x, x_coarse=x1, wawt=O.wawt(P, x), wt=O.wt(P, x),
                    radial=O.radial(P, x), vertical=O.vertical(P, x),
                    vertical_t=O.vertical(P, x, adjoint=True),
-                   restrict=O.restrict(x), prolong=O.prolong(x1))
+                   restrict=O.restrict(x, quad=q), prolong=O.prolong(x1, quad=q))
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
         with open(os.path.join(HERE, name + ".json"), "w") as f:
-            json.dump(c, f, indent=1)
+            json.dump(dict(c, cfg=cfg), f, indent=1)
         print(name, {k: v.shape for k, v in out.items()})
 
 

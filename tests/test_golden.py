@@ -20,18 +20,25 @@ def _load(name):
     return c, {k: d[k] for k in d.files}
 
 
+def _oparams(c):
+    return O.OracleParams(**{k: v for k, v in c.items() if k not in ("n_ens", "cfg")})
+
+
 def test_fixtures_present():
-    assert len(CASES) == 3
+    # radial (ensemble, no generation-1 filter, single field), lquart, line
+    # procedures 7 and 2, and the 3-D procedure 4
+    assert len(CASES) == 7
 
 
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_reproduces_golden(name):
     c, g = _load(name)
-    P = O.OracleParams(**{k: v for k, v in c.items() if k != "n_ens"})
+    P = _oparams(c)
     x = g["x"]
+    q = bool(c.get("lquart", False))
     for key, val in (("wawt", O.wawt(P, x)), ("wt", O.wt(P, x)), ("radial", O.radial(P, x)),
                      ("vertical", O.vertical(P, x)), ("vertical_t", O.vertical(P, x, adjoint=True)),
-                     ("restrict", O.restrict(x)), ("prolong", O.prolong(g["x_coarse"]))):
+                     ("restrict", O.restrict(x, quad=q)), ("prolong", O.prolong(g["x_coarse"], quad=q))):
         assert np.max(np.abs(val - g[key])) <= 1e-12 * np.max(np.abs(g[key])), key
 
 
@@ -44,8 +51,9 @@ def test_hip_path_matches_golden(name):
     from standalone_mgbf_loc_amd import mgbf as M
     c, g = _load(name)
     n_ens, lm = c["n_ens"], c["lm"]
+    over = dict(c.get("cfg", {"l_filt_g1": c["l_filt_g1"]}))
     cfg = M.Config.for_grid(c["nx"], c["ny"], lm, n_ens=n_ens, gm=c["gm"], h=c["hx"], hz=c["hz"],
-                            weights=c["weights"], l_filt_g1=c["l_filt_g1"])
+                            weights=c["weights"], **over)
     with M.MGBF(cfg) as op:
         x = torch.from_numpy(g["x"].astype(np.float32)).to("cuda:0")
         xc = torch.from_numpy(g["x_coarse"].astype(np.float32)).to("cuda:0")
