@@ -55,13 +55,14 @@ __device__ __forceinline__ unsigned umax(unsigned a, unsigned b) { return a > b 
 // Whole-wave single-lane shifts (DPP wave_shl:1 / wave_shr:1): lane l reads
 // lane l+1 (from above) or lane l-1 (from below); the lane with no source
 // reads 0.
+// bound_ctrl: the lane without a source lane writes 0 (no old value to set up)
 __device__ __forceinline__ float lane_from_above(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 0x130, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf,
+                                                              0xf, true));
 }
 __device__ __forceinline__ float lane_from_below(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
-                                                                 0x138, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xf,
+                                                              0xf, true));
 }
 
 // Interpolation constants of the offset grids (def_offset_coef): fine index
