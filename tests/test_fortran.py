@@ -106,3 +106,37 @@ def test_fortran_driver_on_a_finer_analysis_grid_matches_oracle(tmp_path):
     args = (P, 96, 72, cfg.nxPE, cfg.nyPE, 15)
     ref = O.filt_to_anal(*args, O.wawt(P, O.anal_to_filt(*args, x.astype(np.float32).astype(np.float64))))
     assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("proc", [4, 7])
+def test_fortran_driver_runs_the_3d_and_line_procedures(tmp_path, proc):
+    # filtering_procedure(4) (filtering_rad3) and (7) (filtering_lin2_ens
+    # with mgbf_line) through the Fortran module surface
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle import oracle as O
+    _need(DRIVER)
+    kw = dict(gm=3, h=4, hz=2, weights=[1.0, 0.5, 0.25], mgbf_proc=proc)
+    if proc == 4:
+        kw.update(km3=2, km2=2)
+    else:
+        kw.update(n_ens=2, mgbf_line=True)
+    cfg = M.Config.for_grid(64, 64, 8, **kw)
+    if proc == 7:
+        cfg.mg_ampl02 = 0.7
+    d = M.derive(cfg)
+    write_namelist(cfg, str(tmp_path / "mgbeta.nml"))
+    x = np.random.default_rng(6 + proc).standard_normal(d.gen_shape(1))
+    x.astype("<f8").tofile(tmp_path / "in.bin")
+    out = subprocess.run([DRIVER, "mgbeta.nml", "in.bin", "out.bin"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(d.gen_shape(1))
+    P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
+                       hz=2, p=2, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                       weights=[1.0, 0.5, 0.25], line=proc == 7, final_vft=proc == 7,
+                       rad3=proc == 4, ampl03=cfg.mg_ampl03)
+    ref = O.wawt(P, x.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5

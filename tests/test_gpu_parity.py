@@ -449,6 +449,39 @@ def test_rad3_procedure_matches_oracle(case):
         assert abs(lhs - rhs) <= 1e-6 * scale
 
 
+@pytest.mark.parametrize("proc,gm", [(4, 1), (4, 2), (7, 1), (2, 2), (8, 1)])
+def test_procedures_with_one_or_two_generations_and_ragged_stack(proc, gm):
+    # the smallest hierarchies (gm = 1: no transfer at all) and a stack that
+    # fills no whole wavefront (K = 3 + 5*7 = 38 for the background
+    # procedures, 2*7 = 14 for the ensembles), with per-point weights on
+    # every generation
+    kw = dict(gm=gm, h=4, hz=2, mgbf_proc=proc, km3=5, weights=[1.0, 0.5])
+    if proc in (7, 8):
+        kw.update(n_ens=2, km3=1)
+    else:
+        kw["km2"] = 3
+    if proc in (2, 7):
+        kw["mgbf_line"] = True
+    cfg = M.Config.for_grid(48, 32, 7, **kw)
+    if proc in (2, 7):
+        cfg.mg_ampl02 = 0.7
+    if proc == 8:
+        cfg.mg_ampl02 = 1.0
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.gm == gm
+        rng = np.random.default_rng(170 + proc)
+        wf = [(0.5 + rng.random(d.gen_shape(g))).astype(np.float32) for g in range(1, gm + 1)]
+        keep = [to_dev(w) for w in wf]
+        for g, w in enumerate(keep, start=1):
+            op.set_weights(g, field=w)
+        x = field(d.gen_shape(1), 171)
+        y = op.filtering_procedure(to_dev(x))
+        sync()
+        ref = O.wawt(oparams(cfg, d), x.astype(np.float64), wfield=[w.astype(np.float64) for w in wf])
+        assert relerr(y, ref) <= TOL_OP
+
+
 def test_dot_is_fp64_accumulated():
     cfg = M.Config.for_grid(64, 64, 8, gm=2, h=3, hz=2)
     with M.MGBF(cfg) as op:
