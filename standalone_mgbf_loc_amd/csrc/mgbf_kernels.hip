@@ -570,6 +570,7 @@ hipError_t launch_ball(const BallArgs& a, hipStream_t st) {
     const int nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = (long long)nkc * a.n * a.m;
     const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (nblocks >= (1ll << 31)) return hipErrorInvalidValue;
     if (a.transpose)
         hipLaunchKernelGGL(ball_kernel<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nkc, nitems);
     else

@@ -84,6 +84,56 @@ def test_gradients_match_fp64_central_differences(filt_g1):
             assert abs(gan[idx] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(gan).max()), (idx, gan[idx], fd)
 
 
+@pytest.mark.parametrize("case", ["rad3", "fast", "lquart", "line_bckg"])
+def test_weight_gradients_of_the_other_procedures(case):
+    # procedures 4 (3-D radial), 8 (fast), 6 with lquart and 2 (line): the
+    # weight gradients <(W^T g)_g, (W^T z)_g> against FP64 central differences
+    kw = dict(gm=3, h=4, hz=2)
+    if case == "rad3":
+        kw.update(mgbf_proc=4, km3=2, km2=1)
+    elif case == "fast":
+        kw.update(mgbf_proc=8, n_ens=2)
+    elif case == "lquart":
+        kw.update(n_ens=2, lquart=True)
+    else:
+        kw.update(mgbf_proc=2, km3=2, km2=1, mgbf_line=True)
+    cfg = M.Config.for_grid(48, 48, 6, **kw)
+    if case == "fast":
+        cfg.mg_ampl02 = 1.0
+    if case == "line_bckg":
+        cfg.mg_ampl02 = 0.7
+    w0 = [0.9, 0.6, 0.3]
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        x, y, a = inputs(d.gen_shape(1), 11)
+        ws = [torch.tensor(v, dtype=torch.float64, device=DEV) for v in w0]
+        J, ga, gw = loss_and_grads(op, torch.from_numpy(a).to(DEV), torch.from_numpy(x).to(DEV),
+                                   torch.from_numpy(y).to(DEV), ws)
+        P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
+                           hz=2, p=2, gm=3, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                           fast=case == "fast", lquart=case == "lquart", line=case == "line_bckg",
+                           rad3=case == "rad3", ampl03=cfg.mg_ampl03)
+        J64 = oracle_J(P, a, x, y, w0)
+        assert abs(J - J64) <= 1e-5 * abs(J64)
+        h = 1e-3
+        for g in range(3):
+            wp, wm = list(w0), list(w0)
+            wp[g] += h
+            wm[g] -= h
+            fd = (oracle_J(P, a, x, y, wp) - oracle_J(P, a, x, y, wm)) / (2 * h)
+            assert abs(gw[g].item() - fd) <= 1e-4 * abs(fd), (g, gw[g].item(), fd)
+
+
+def test_ensemble_line_procedure_is_refused_by_autograd():
+    # filtering_lin2_ens ends with Vf^T: its W A W^T is not symmetric
+    cfg = M.Config.for_grid(32, 32, 4, n_ens=2, gm=2, h=3, hz=1, mgbf_proc=7)
+    with M.MGBF(cfg) as op:
+        z = torch.zeros(op.dims.gen_shape(1), device=DEV)
+        w = [torch.tensor(1.0, dtype=torch.float64, device=DEV, requires_grad=True)] * 2
+        with pytest.raises(ValueError, match="not symmetric"):
+            localize(op, z, w)
+
+
 def test_per_point_weight_field_gradient():
     cfg = M.Config.for_grid(32, 32, 4, n_ens=2, gm=2, h=3, hz=1)
     with M.MGBF(cfg) as op:
