@@ -87,7 +87,7 @@ typedef struct mgbf_ctx mgbf_ctx;
 /* operations of mgbf_apply */
 enum {
     MGBF_OP_WAWT = 0,  /* y = W A W^T x   (filtering_procedure(mgbf_proc), mg_filtering.f90:67-120:
-                          procedures 1-4, 6-8): field -> field */
+                          procedures 1-8): field -> field */
     MGBF_OP_WT = 1,    /* v = W^T x       : field -> multigrid vector */
     MGBF_OP_W = 2,     /* y = W v         : multigrid vector -> field */
     MGBF_OP_A = 3,     /* v = A v         : multigrid -> multigrid */
@@ -107,9 +107,9 @@ enum {
     MGBF_STAGE_VF = 0,  /* Vf   on a generation-1 field (sup_vrbeta1_ens) */
     MGBF_STAGE_VFT = 1, /* Vf^T on a generation-1 field (sup_vrbeta1T_ens) */
     MGBF_STAGE_F = 2,   /* F  on generation gen: boco_2d + rbeta (vrbeta2; procedure 4:
-                           sup_vrbeta3 for the 3-D variables) */
-    MGBF_STAGE_FT = 3,  /* F^T on generation gen: rbetaT (vrbeta2T) + bocoT_2d (procedure 4:
-                           sup_vrbeta3T) */
+                           sup_vrbeta3, procedure 5: the hexad lines, for the 3-D variables) */
+    MGBF_STAGE_FT = 3,  /* F^T on generation gen: rbetaT (vrbeta2T) + bocoT_2d (procedures 4
+                           and 5: their transposes) */
     MGBF_STAGE_R = 4,   /* restriction gen -> gen+1: adjoint + bocoT_2d(.,2,2)
                            (lquart: adjoint2 + bocoT_2d(.,1,1)) */
     MGBF_STAGE_P = 5,   /* prolongation gen+1 -> gen: boco_2d(.,2,2) + direct1

@@ -26,6 +26,9 @@
  *   - vrbeta3 / vrbeta3T ....................... jp_pbfil.f90:930-977, 1053-1100
  *   - sup_vrbeta3 / sup_vrbeta3T ............... mg_filtering.f90:1923-2039
  *   - filtering_rad3 composition ............... mg_filtering.f90:123-257
+ *   - hexad / hextform ......................... jp_pbfil3.f90:702-750, 798-904
+ *   - vdibetax3 / vdibetax3t ................... jp_pbfil3.f90:1928-1977, 2475-2525
+ *   - filtering_lin3 composition ............... mg_filtering.f90:260-557
  *   - upsending2_ens / downsending2_ens ........ mg_generations.f90:497-609
  *   - upsending_ens / downsending_ens .......... mg_generations.f90:382-494
  *   - weighting_ens ............................ mg_generations.f90:1012-1052
@@ -1163,6 +1166,273 @@ static void line_filter(const mgbfo_params* P, const line_t* Lf, pf_t* a, int ad
     }
 }
 
+/* ------------------------------------------------------------------------
+ * Hexad line filters of the 3-D variables (procedure 5, filtering_lin3).
+ * init_mg_line resolves the 6-vector of pasp3 (after cholaspect3) onto six
+ * lattice lines with "colors" 1..7 (hexad, hextform); filtering_lin3 runs
+ * the color of each call, 1..7 forward and 7..1 adjoint, each line a dibeta
+ * filter (vdibetax3 / vdibetax3t). The level halo is filled once by the
+ * whole-sample mirror before the forward loop; boco_3d / bocoT_3d refresh
+ * and fold only the x/y halos of the interior levels.
+ * ---------------------------------------------------------------------- */
+
+/* hexad (jp_pbfil3.f90:798-904): aspect (A11, A22, A33, A23, A31, A12) ->
+   seven generators lhex7[3c..3c+2] (colors c+1 = 1..7, one of them null)
+   and weights whex7[c]. Returns 1 when 200 iterations do not end it. */
+int mgbfo_hexad(const double aspect[6], int lhex7[21], double whex7[7]) {
+    static const int deflhex[7][3] = {{0, 0, 0}, {1, -1, 0}, {0, 1, -1}, {0, 0, 1},
+                                      {-1, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+    static const int deflui[7][6] = {{0, 0, 0, 0, 0, 0},  {0, 0, 0, 0, 0, -1}, {0, 0, 0, -1, 0, 0},
+                                     {0, 0, 1, 1, 1, 0},  {0, 0, 0, 0, -1, 0}, {0, 1, 0, 1, 0, 1},
+                                     {1, 0, 0, 0, 1, 1}};
+    /* Mmats(:,:,d), d = 1..6, column-major: mm[d-1][col][row] */
+    static const int mm[6][3][3] = {{{1, 1, -1}, {1, 0, 0}, {1, 0, -1}},
+                                    {{-1, 1, 0}, {-1, 1, 1}, {0, 1, 0}},
+                                    {{0, -1, 1}, {1, -1, 0}, {1, 0, 0}},
+                                    {{0, 0, 1}, {0, -1, 1}, {1, -1, 1}},
+                                    {{-1, 0, 1}, {0, 0, 1}, {-1, 1, 0}},
+                                    {{0, 1, 0}, {1, 0, -1}, {0, 1, -1}}};
+    static const int ttriad[6] = {5, 3, 3, 6, 5, 6};
+    static const int sstriad[13] = {-1, -1, 1, -1, 1, 1, 1, -1, -1, 1, -1, 1, 1};  /* -6..6 */
+    static const int line3[3] = {1, 2, 4};
+    static const int jcol[7] = {7, 4, 6, 3, 5, 2, 1};
+    int lhex[7][3], lui[7][6], kset[3][3];
+    double whex[7];
+    memcpy(lhex, deflhex, sizeof lhex);
+    memcpy(lui, deflui, sizeof lui);
+    int hcol = 0;
+    for (int c = 0; c < 7; ++c) {
+        whex[c] = 0.0;
+        for (int r = 0; r < 6; ++r) whex[c] += aspect[r] * lui[c][r];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int e = 0; e < 3; ++e) kset[i][e] = lhex[((hcol - line3[i]) % 7 + 7) % 7][e];
+    int it, kcol = 0;
+    for (it = 1; it <= 200; ++it) {
+        kcol = 0;
+        for (int c = 1; c < 7; ++c)
+            if (whex[c] < whex[kcol]) kcol = c;
+        const double dw = whex[kcol];
+        if (dw >= -1.e-14) break;
+        const int dcol = ((kcol - hcol) % 7 + 7) % 7;
+        hcol = kcol;
+        const int L = (hcol + ttriad[dcol - 1]) % 7;
+        int knew[3][3];
+        for (int j = 0; j < 3; ++j)
+            for (int e = 0; e < 3; ++e) {
+                int acc = 0;
+                for (int i = 0; i < 3; ++i) acc += kset[i][e] * mm[dcol - 1][j][i];
+                knew[j][e] = acc;
+            }
+        memcpy(kset, knew, sizeof kset);
+        int dl[6];
+        for (int r = 0; r < 6; ++r) dl[r] = lui[hcol][r];
+        for (int c = 0; c < 7; ++c) {
+            const int sg = sstriad[c - L + 6];
+            for (int r = 0; r < 6; ++r) lui[c][r] += dl[r] * sg;
+            whex[c] += sg * dw;
+        }
+    }
+    if (it > 200) return 1;
+    for (int i = 0; i < 3; ++i) {
+        const int ip = (i + 1) % 3;
+        for (int e = 0; e < 3; ++e) {
+            lhex[((hcol - line3[i]) % 7 + 7) % 7][e] = kset[i][e];
+            lhex[(hcol + line3[i]) % 7][e] = kset[i][e] - kset[ip][e];
+        }
+    }
+    for (int e = 0; e < 3; ++e) lhex[kcol][e] = 0;
+    for (int i = 0; i < 7; ++i) {
+        const int j = jcol[i] - 1;
+        for (int e = 0; e < 3; ++e) lhex7[3 * j + e] = lhex[i][e];
+        whex7[j] = whex[i];
+    }
+    return 0;
+}
+
+/* The six lines of the domain's 3-D variables (hextform on the Cholesky
+   factor of pasp3's inverse, as init_mg_line sees it): qcol[j] is the color
+   of line j, generator g[j], half filter fil[j][0..h[j]]. */
+typedef struct {
+    int n;                 /* lines (6) */
+    int qcol[8];           /* qcol[1..n]; qcol[0] = qcol[7] = 0 */
+    int g[7][3];
+    int h[7];
+    double fil[7][LINE_NFIL + 1];
+} hex_t;
+
+static hex_t hex_make(const mgbfo_params* P) {
+    hex_t X;
+    memset(&X, 0, sizeof X);
+    if (!P->line_tables) return X;   /* no lines: every color copies */
+    const double a3 = P->ampl03;
+    const double a[9] = {a3 * (1. + P->p_del), a3 * P->p_eps, 0.0,
+                         a3 * P->p_eps, a3, 0.0,
+                         0.0, 0.0, a3 * (1. - P->p_del)};
+    double el[9];
+    cholaspect3(a, el);
+    /* t33_to_6 of the lower factor: (1,1), (2,2), (3,3), (2,3), (3,1), (1,2);
+       the upper entries (2,3) and (1,2) are 0 */
+    const double asp[6] = {el[0], el[4], el[8], 0.0, el[6], 0.0};
+    int lh7[21];
+    double wh7[7];
+    if (mgbfo_hexad(asp, lh7, wh7)) abort();
+    int j = 1;
+    double hs[7];
+    for (int c = 1; c <= 7; ++c) {
+        const int* g = lh7 + 3 * (c - 1);
+        if (abs(g[0]) + abs(g[1]) + abs(g[2]) == 0) continue;
+        if (j > 6) abort();    /* inconsistent hexad generator set */
+        X.qcol[j] = c;
+        X.g[j][0] = g[0];
+        X.g[j][1] = g[1];
+        X.g[j][2] = g[2];
+        hs[j] = wh7[c - 1];
+        ++j;
+    }
+    if (j != 7) abort();
+    X.n = 6;
+    for (int l = 1; l <= 6; ++l) {
+        if (mgbfo_hstform(P->p, &hs[l])) abort();
+        int h;
+        /* a failing blinfil leaves the call after moving one point's line
+           counter: not a state to restate */
+        if (mgbfo_blinfil(P->p, hs[l], &h, X.fil[l])) abort();
+        X.h[l] = h;
+        if (abs(X.g[l][0]) * h > P->hx || abs(X.g[l][1]) * h > P->hy || abs(X.g[l][2]) * h > P->hz)
+            abort();           /* reads beyond the halos */
+    }
+    return X;
+}
+
+int mgbfo_hexlines(const mgbfo_params* P, int* qcol, int* g, int* h, double* fil) {
+    hex_t X = hex_make(P);
+    for (int l = 1; l <= 6; ++l) {
+        qcol[l - 1] = X.qcol[l];
+        for (int e = 0; e < 3; ++e) g[3 * (l - 1) + e] = X.g[l][e];
+        h[l - 1] = X.h[l];
+        for (int t = 0; t <= LINE_NFIL; ++t) fil[(l - 1) * (LINE_NFIL + 1) + t] = X.fil[l][t];
+    }
+    return X.n;
+}
+
+/* The filter of filtering_lin3 on one generation of the stack: the 2-D
+   variables through the triad lines of pasp2 (dibeta / dibetat with
+   boco_2d / bocoT_2d), the 3-D variables through the hexad lines over
+   (i, j, L). */
+static void lin3_filter(const mgbfo_params* P, const line_t* L2, const hex_t* X, pf_t* a,
+                        int adjoint) {
+    const int K = a->K, n = a->n, m = a->m, hx = a->hx, hy = a->hy, hz = P->hz;
+    const int lm = P->lm, nv = P->nvcol, K3 = nv * lm, K2 = K - K3;
+    const int nxp = n + 2 * hx, nyp = m + 2 * hy, nlp = lm + 2 * hz;
+    const size_t vol = (size_t)nxp * nyp * nlp;
+#define W3(w, i, j, L) (w)[((size_t)((L) + hz - 1) * nyp + (size_t)((j) + hy - 1)) * nxp + (size_t)((i) + hx - 1)]
+    pf_t b = pf_alloc(K, n, m, hx, hy);
+    memcpy(b.d, a->d, sizeof(double) * (size_t)K * nxp * nyp);
+#pragma omp parallel for schedule(dynamic)
+    for (int v = 0; v < nv; ++v) {
+        double* W = calloc(vol, sizeof(double));
+        double* B = malloc(vol * sizeof(double));
+        for (int L = 1; L <= lm; ++L)
+            for (int j = 1 - hy; j <= m + hy; ++j)
+                for (int i = 1 - hx; i <= n + hx; ++i) W3(W, i, j, L) = PF(a, v * lm + L - 1, i, j);
+        /* boco_3d / bocoT_3d: the x/y halos of the interior levels */
+        pf_t lev = {NULL, 1, n, m, hx, hy};
+        double* levbuf = malloc(sizeof(double) * (size_t)nxp * nyp);
+        lev.d = levbuf;
+        int jc = 0;
+        if (!adjoint) {
+            for (int L = 1; L <= hz; ++L)
+                for (int j = 1 - hy; j <= m + hy; ++j)
+                    for (int i = 1 - hx; i <= n + hx; ++i) {
+                        W3(W, i, j, 1 - L) = W3(W, i, j, 1 + L);
+                        W3(W, i, j, lm + L) = W3(W, i, j, lm - L);
+                    }
+            for (int icol = 1; icol <= 7; ++icol) {
+                for (int L = 1; L <= lm; ++L) {
+                    memcpy(levbuf, &W3(W, 1 - hx, 1 - hy, L), sizeof(double) * (size_t)nxp * nyp);
+                    boco_2d(&lev, hx, hy);
+                    memcpy(&W3(W, 1 - hx, 1 - hy, L), levbuf, sizeof(double) * (size_t)nxp * nyp);
+                }
+                if (icol == 1) jc = 1;
+                const int hit = jc <= 6 && X->qcol[jc] == icol;
+                memset(B, 0, vol * sizeof(double));
+                if (!hit) {
+                    for (int L = 1; L <= lm; ++L)
+                        for (int j = 1; j <= m; ++j)
+                            for (int i = 1; i <= n; ++i) W3(B, i, j, L) = W3(W, i, j, L);
+                } else {
+                    const int l = jc++;
+                    const int* g = X->g[l];
+                    for (int L = 1; L <= lm; ++L)
+                        for (int j = 1; j <= m; ++j)
+                            for (int i = 1; i <= n; ++i) {
+                                double t = X->fil[l][0] * W3(W, i, j, L);
+                                for (int q = 1; q <= X->h[l]; ++q)
+                                    t += X->fil[l][q] * (W3(W, i + g[0] * q, j + g[1] * q, L + g[2] * q) +
+                                                         W3(W, i - g[0] * q, j - g[1] * q, L - g[2] * q));
+                                W3(B, i, j, L) = t;
+                            }
+                }
+                memcpy(W, B, vol * sizeof(double));
+            }
+        } else {
+            for (int icol = 7; icol >= 1; --icol) {
+                /* the level halo is filled here too; the scatter from the
+                   interior below replaces the whole array */
+                if (icol == 7) jc = 6;
+                const int hit = jc >= 1 && X->qcol[jc] == icol;
+                memset(B, 0, vol * sizeof(double));
+                if (!hit) {
+                    for (int L = 1; L <= lm; ++L)
+                        for (int j = 1; j <= m; ++j)
+                            for (int i = 1; i <= n; ++i) W3(B, i, j, L) += W3(W, i, j, L);
+                } else {
+                    const int l = jc--;
+                    const int* g = X->g[l];
+                    for (int L = 1; L <= lm; ++L)
+                        for (int j = 1; j <= m; ++j)
+                            for (int i = 1; i <= n; ++i) {
+                                const double at = W3(W, i, j, L);
+                                W3(B, i, j, L) += X->fil[l][0] * at;
+                                for (int q = 1; q <= X->h[l]; ++q) {
+                                    const double f = X->fil[l][q] * at;
+                                    W3(B, i + g[0] * q, j + g[1] * q, L + g[2] * q) += f;
+                                    W3(B, i - g[0] * q, j - g[1] * q, L - g[2] * q) += f;
+                                }
+                            }
+                }
+                memcpy(W, B, vol * sizeof(double));
+                for (int L = 1; L <= lm; ++L) {
+                    memcpy(levbuf, &W3(W, 1 - hx, 1 - hy, L), sizeof(double) * (size_t)nxp * nyp);
+                    bocoT_2d(&lev, hx, hy);
+                    memcpy(&W3(W, 1 - hx, 1 - hy, L), levbuf, sizeof(double) * (size_t)nxp * nyp);
+                }
+            }
+        }
+        for (int L = 1; L <= lm; ++L)
+            for (int j = 1 - hy; j <= m + hy; ++j)
+                for (int i = 1 - hx; i <= n + hx; ++i) PF(&b, v * lm + L - 1, i, j) = W3(W, i, j, L);
+        free(levbuf);
+        free(W);
+        free(B);
+    }
+#undef W3
+    if (K2 > 0) {
+        pf_t a2 = pf_alloc(K2, n, m, hx, hy);
+        for (int j = 1 - hy; j <= m + hy; ++j)
+            for (int i = 1 - hx; i <= n + hx; ++i)
+                memcpy(PCOL(&a2, i, j), PCOL(a, i, j) + K3, sizeof(double) * K2);
+        line_filter(P, L2, &a2, adjoint);
+        for (int j = 1 - hy; j <= m + hy; ++j)
+            for (int i = 1 - hx; i <= n + hx; ++i)
+                memcpy(PCOL(&b, i, j) + K3, PCOL(&a2, i, j), sizeof(double) * K2);
+        pf_free(&a2);
+    }
+    free(a->d);
+    a->d = b.d;
+}
+
 /* One line of the domain's line filter, for the pins: generator and the
    half filter fil[0..h]; h = -1 for an identity line. */
 int mgbfo_line(const mgbfo_params* P, int c, int* dx, int* dy, int* h, double* fil) {
@@ -1182,15 +1452,17 @@ typedef struct {
     fast_t F;
     line_t L;
     taps3_t T3;
+    hex_t X;
 } hfilt_t;
 
 static hfilt_t hfilt_make(const mgbfo_params* P) {
     hfilt_t H;
     memset(&H, 0, sizeof H);
-    if (P->line) H.L = line_make(P);
+    if (P->line || P->lin3) H.L = line_make(P);
     else if (P->fast) H.F = fast_make(P);
     else H.T = taps2_make(P);
     if (P->rad3) H.T3 = taps3_make(P);
+    if (P->lin3) H.X = hex_make(P);
     return H;
 }
 
@@ -1202,6 +1474,8 @@ static void hfilt_free(hfilt_t* H) {
 static void hfilter(const mgbfo_params* P, const hfilt_t* H, pf_t* a, int adjoint) {
     if (P->rad3) {
         rad3_filter(P, &H->T, &H->T3, a, adjoint);
+    } else if (P->lin3) {
+        lin3_filter(P, &H->L, &H->X, a, adjoint);
     } else if (P->line) {
         line_filter(P, &H->L, a, adjoint);
     } else if (P->fast) {
@@ -1342,7 +1616,7 @@ static void weight_pf(const mgbfo_params* P, int gi, pf_t* f, const double* wcon
 static void upsweep_inputs(const mgbfo_params* P, const double* x, pf_t* G) {
     const int gm = P->gm;
     pf_from_interior(&G[0], x);
-    if (P->l_vertical_filter && !P->rad3) vertical_op(P, &G[0], 1);
+    if (P->l_vertical_filter && !P->rad3 && !P->lin3) vertical_op(P, &G[0], 1);
     for (int g = 1; g < gm; ++g) {
         /* adjoint reads the interior of the finer generation only */
         pf_t Fi = pf_alloc(P->K, G[g - 1].n, G[g - 1].m, 0, 0);
@@ -1375,7 +1649,7 @@ static void downsweep_outputs(const mgbfo_params* P, pf_t* G, double* y, int fin
         pf_free(&Wi);
         pf_free(&Fp);
     }
-    if (P->l_vertical_filter && !P->rad3) vertical_op(P, &G[0], final_vft);
+    if (P->l_vertical_filter && !P->rad3 && !P->lin3) vertical_op(P, &G[0], final_vft);
     pf_to_interior(&G[0], y);
 }
 

@@ -43,6 +43,8 @@ typedef struct mgbfo_params {
                               stack are nvcol 3-D variables filtered over (i, j, L), the rest
                               2-D variables; no separate vertical filter */
     double ampl03;         /* 3-D aspect amplitude (pasp03 = mg_ampl03) */
+    int lin3;              /* filtering_lin3 (procedure 5): the 3-D variables through the hexad
+                              line filters over (i, j, L), the 2-D ones through the triad lines */
 } mgbfo_params;
 
 /* ---- coefficient generation (jp_pbfil.f90 cholaspect/getlinesum) ---- */
@@ -81,6 +83,13 @@ void mgbfo_fold(int K, int n, int m, int h, const double* padded, double* out);
 /* Taps of the 3-D filter (gx, gy, gz, w) in vrbeta3's order; returns the count
    (fills up to maxt). Aborts beyond a halo. */
 int mgbfo_taps3(const mgbfo_params* P, int maxt, int* gx, int* gy, int* gz, double* w);
+
+/* hexad: aspect (A11, A22, A33, A23, A31, A12) -> generators lhex7[3c..3c+2] and
+   weights whex7[c] of colors c+1 = 1..7. Nonzero when it does not converge. */
+int mgbfo_hexad(const double aspect[6], int lhex7[21], double whex7[7]);
+/* The six hexad lines of the domain's 3-D variables: colors qcol[6], generators
+   g[18], half-spans h[6], half filters fil[6][9]. Returns the number of lines. */
+int mgbfo_hexlines(const mgbfo_params* P, int* qcol, int* g, int* h, double* fil);
 
 /* ---- triad line filters (jp_pbfil3.f90) ---- */
 /* inimomtab: bnorm[1..nh], bsprds[1..nh] for exponent p. Nonzero on bad p/nh. */

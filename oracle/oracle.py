@@ -37,7 +37,7 @@ class _Params(ctypes.Structure):
         ("l_filt_g1", ctypes.c_int), ("l_vertical_filter", ctypes.c_int),
         ("fast", ctypes.c_int), ("lquart", ctypes.c_int), ("line", ctypes.c_int),
         ("line_tables", ctypes.c_int), ("final_vft", ctypes.c_int),
-        ("rad3", ctypes.c_int), ("ampl03", ctypes.c_double),
+        ("rad3", ctypes.c_int), ("ampl03", ctypes.c_double), ("lin3", ctypes.c_int),
     ]
 
 
@@ -59,6 +59,8 @@ def lib():
         L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_taps3.argtypes = [P, ctypes.c_int, ip, ip, ip, dp]
+        L.mgbfo_hexad.argtypes = [dp, ip, dp]
+        L.mgbfo_hexlines.argtypes = [P, ip, ip, ip, dp]
         L.mgbfo_momtab.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_hstform.argtypes = [ctypes.c_int, dp]
         L.mgbfo_blinfil.argtypes = [ctypes.c_int, ctypes.c_double, ip, dp]
@@ -142,13 +144,14 @@ class OracleParams:
     final_vft: bool = False    # W A W^T ends with Vf^T (filtering_lin2_ens)
     rad3: bool = False     # filtering_rad3 (procedure 4): 3-D radial filter of the 3-D variables
     ampl03: float = 1.0    # 3-D aspect amplitude (pasp03)
+    lin3: bool = False     # filtering_lin3 (procedure 5): hexad line filters of the 3-D variables
 
     def c(self) -> _Params:
         return _Params(self.nx, self.ny, self.K, self.lm, self.nvcol, self.hx, self.hy,
                        self.hz, self.p, self.gm, self.ampl01, self.ampl02, self.p_del,
                        self.p_eps, int(self.l_filt_g1), int(self.l_vertical_filter), int(self.fast),
                        int(self.lquart), int(self.line), int(self.line_tables), int(self.final_vft),
-                       int(self.rad3), float(self.ampl03))
+                       int(self.rad3), float(self.ampl03), int(self.lin3))
 
     def gen_shape(self, g: int):
         """(m_g, n_g, K) of generation g (0-based)."""
@@ -190,6 +193,25 @@ def taps3(P: OracleParams):
     w = np.zeros(cap)
     nt = lib().mgbfo_taps3(ctypes.byref(c), cap, gx, gy, gz, _dp(w))
     return np.array(gx[:nt]), np.array(gy[:nt]), np.array(gz[:nt]), w[:nt].copy()
+
+
+def hexad(aspect):
+    """hexad: (generators (7, 3) by color 1..7, weights (7,), failure flag)."""
+    a = _f64(aspect)
+    lh = (ctypes.c_int * 21)()
+    w = np.zeros(7)
+    ff = lib().mgbfo_hexad(_dp(a), lh, _dp(w))
+    return np.array(lh[:]).reshape(7, 3), w, bool(ff)
+
+
+def hexlines(P: OracleParams):
+    """The six hexad lines of the 3-D variables: [(color, (gx, gy, gz), h, fil[0..h])]."""
+    c = P.c()
+    q = (ctypes.c_int * 6)(); g = (ctypes.c_int * 18)(); h = (ctypes.c_int * 6)()
+    fil = np.zeros(6 * 9)
+    n = lib().mgbfo_hexlines(ctypes.byref(c), q, g, h, _dp(fil))
+    return [(q[l], tuple(g[3 * l: 3 * l + 3]), h[l], fil[9 * l: 9 * l + h[l] + 1].copy())
+            for l in range(n)]
 
 
 def taps_fast(P: OracleParams):

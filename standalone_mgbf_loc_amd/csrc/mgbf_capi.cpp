@@ -515,9 +515,10 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// Procedure 4, filtering_rad3 (mg_filtering.f90:123-257): every generation
-// filtered by F (3-D variables over (i, j, L), 2-D variables in the plane),
-// no separate vertical filter:  y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
+// Procedures 4 and 5, filtering_rad3 / filtering_lin3 (mg_filtering.f90:
+// 123-557): every generation filtered by F (3-D variables over (i, j, L),
+// 2-D variables in the plane), no separate vertical filter:
+//   y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
 // ---------------------------------------------------------------------------
 int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st) {
     Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st, 8.0 * (double)c->gsize[g]);
@@ -536,6 +537,7 @@ int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStr
     a.lm = c->s.cfg.lm;
     a.K3 = c->s.dims.km3_all * c->s.cfg.lm;
     a.transpose = transpose ? 1 : 0;
+    a.zero_levels = c->s.zero_levels ? 1 : 0;
     HIP_TRY(mgbf::launch_ball(a, st), "3-D filter launch");
     return MGBF_OK;
 }
@@ -727,7 +729,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->vf_last = cfg->mgbf_proc == 7 ? BAND_VFT : BAND_VF;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     c->quad = cfg->lquart != 0;
-    c->rad3 = cfg->mgbf_proc == 4;
+    c->rad3 = cfg->mgbf_proc == 4 || cfg->mgbf_proc == 5;
     {
         // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for
         // weights that are constant along the stack

@@ -371,7 +371,197 @@ bool line_triad(const double a[3], int l[3][2], double w[3]) {
     return done;
 }
 
+// hstform (jp_pbfil3.f90:1399-1430): spread**2 -> real half-span by linear
+// interpolation in the table of spreads**2; false when negative or beyond it.
+bool line_hstform(const double* bsprds, double& hs) {
+    const bool neg = hs < 0.0;
+    for (int h = 2; h <= kLineNH; ++h)
+        if (bsprds[h] >= hs) {
+            hs = h - (bsprds[h] - hs) / (bsprds[h] - bsprds[h - 1]);
+            return !neg;
+        }
+    return false;
+}
+
+// blinfil (jp_pbfil3.f90:1463-1501): the half dibeta filter fil[0..h] of
+// half-span hs, the mixture of the normalized betas of half-spans h+1 and h,
+// h = int(hs); empty when h < 1 or h+1 leaves the table.
+std::vector<double> line_blinfil(const double* bnorm, int p, double hs) {
+    const int h = (int)hs, hp = h + 1;
+    if (h < 1 || hp > kLineNH || hp > kLineNFil) return {};
+    std::vector<double> fil(h + 1, 0.0);
+    const double whp = (hs - h) * bnorm[hp], wh = (hp - hs) * bnorm[h];
+    for (int i = 0; i <= h; ++i) {
+        const double z = (double)i / hp;
+        fil[i] = whp * ipow(1.0 - z * z, p);
+    }
+    for (int i = 0; i < h; ++i) {
+        const double z = (double)i / h;
+        fil[i] += wh * ipow(1.0 - z * z, p);
+    }
+    return fil;
+}
+
+bool is_identity(const std::vector<double>& fil) {
+    bool id = !fil.empty() && fil[0] == 1.0;
+    for (size_t i = 1; i < fil.size(); ++i) id = id && fil[i] == 0.0;
+    return id;
+}
+
+// hexad (jp_pbfil3.f90:798-904): aspect (A11, A22, A33, A23, A31, A12) ->
+// generators l7[c] and weights w7[c] of colors c+1 = 1..7 (one null), from
+// the default hexad, iterating its K-set of three generators with the
+// Fano-plane tables until no weight is negative.
+bool line_hexad(const double a[6], int l7[7][3], double w7[7]) {
+    static const int deflhex[7][3] = {{0, 0, 0}, {1, -1, 0}, {0, 1, -1}, {0, 0, 1},
+                                      {-1, 0, 1}, {0, 1, 0}, {1, 0, 0}};
+    static const int deflui[7][6] = {{0, 0, 0, 0, 0, 0},  {0, 0, 0, 0, 0, -1}, {0, 0, 0, -1, 0, 0},
+                                     {0, 0, 1, 1, 1, 0},  {0, 0, 0, 0, -1, 0}, {0, 1, 0, 1, 0, 1},
+                                     {1, 0, 0, 0, 1, 1}};
+    static const int mm[6][3][3] = {{{1, 1, -1}, {1, 0, 0}, {1, 0, -1}},
+                                    {{-1, 1, 0}, {-1, 1, 1}, {0, 1, 0}},
+                                    {{0, -1, 1}, {1, -1, 0}, {1, 0, 0}},
+                                    {{0, 0, 1}, {0, -1, 1}, {1, -1, 1}},
+                                    {{-1, 0, 1}, {0, 0, 1}, {-1, 1, 0}},
+                                    {{0, 1, 0}, {1, 0, -1}, {0, 1, -1}}};   // [d][column][row]
+    static const int ttriad[6] = {5, 3, 3, 6, 5, 6};
+    static const int sstriad[13] = {-1, -1, 1, -1, 1, 1, 1, -1, -1, 1, -1, 1, 1};   // -6..6
+    static const int line3[3] = {1, 2, 4};
+    static const int jcol[7] = {7, 4, 6, 3, 5, 2, 1};
+    auto mod7 = [](int q) { return ((q % 7) + 7) % 7; };
+    int lhex[7][3], lui[7][6], kset[3][3];
+    double w[7];
+    std::copy(&deflhex[0][0], &deflhex[0][0] + 21, &lhex[0][0]);
+    std::copy(&deflui[0][0], &deflui[0][0] + 42, &lui[0][0]);
+    int hcol = 0, kcol = 0;
+    for (int c = 0; c < 7; ++c) {
+        w[c] = 0.0;
+        for (int r = 0; r < 6; ++r) w[c] += a[r] * lui[c][r];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int e = 0; e < 3; ++e) kset[i][e] = lhex[mod7(hcol - line3[i])][e];
+    bool done = false;
+    for (int it = 0; it < 200 && !done; ++it) {
+        kcol = 0;
+        for (int c = 1; c < 7; ++c)
+            if (w[c] < w[kcol]) kcol = c;
+        const double dw = w[kcol];
+        if (dw >= -1.e-14) {
+            done = true;
+            break;
+        }
+        const int dcol = mod7(kcol - hcol);
+        hcol = kcol;
+        const int L = (hcol + ttriad[dcol - 1]) % 7;
+        int kn[3][3];
+        for (int j = 0; j < 3; ++j)
+            for (int e = 0; e < 3; ++e) {
+                int acc = 0;
+                for (int i = 0; i < 3; ++i) acc += kset[i][e] * mm[dcol - 1][j][i];
+                kn[j][e] = acc;
+            }
+        std::copy(&kn[0][0], &kn[0][0] + 9, &kset[0][0]);
+        int dl[6];
+        for (int r = 0; r < 6; ++r) dl[r] = lui[hcol][r];
+        for (int c = 0; c < 7; ++c) {
+            const int sg = sstriad[c - L + 6];
+            for (int r = 0; r < 6; ++r) lui[c][r] += dl[r] * sg;
+            w[c] += sg * dw;
+        }
+    }
+    if (!done) return false;
+    for (int i = 0; i < 3; ++i)
+        for (int e = 0; e < 3; ++e) {
+            lhex[mod7(hcol - line3[i])][e] = kset[i][e];
+            lhex[(hcol + line3[i]) % 7][e] = kset[i][e] - kset[(i + 1) % 3][e];
+        }
+    for (int e = 0; e < 3; ++e) lhex[kcol][e] = 0;
+    for (int i = 0; i < 7; ++i) {
+        for (int e = 0; e < 3; ++e) l7[jcol[i] - 1][e] = lhex[i][e];
+        w7[jcol[i] - 1] = w[i];
+    }
+    return true;
+}
+
 }  // namespace
+
+int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err) {
+    taps.clear();
+    *zero_levels = true;
+    if (!cfg.mgbf_line) {
+        // without init_mg_line no color has a line: every call copies
+        taps.push_back({0, 0, 0, 1.0});
+        return MGBF_OK;
+    }
+    if (cfg.p < 1 || cfg.p > 6) {
+        *err = " In inimomtab; prescribed exponent p out of bounds";
+        return MGBF_ERR_CONFIG;
+    }
+    double bnorm[kLineNH + 1], bsprds[kLineNH + 1];
+    line_moments(cfg.p, bnorm, bsprds);
+    // the Cholesky factor of pasp3 = mg_ampl03 I (p_del = p_eps = 0) as
+    // t33_to_6 hands it to hextform: (s, s, s, 0, 0, 0)
+    const double el = 1.0 / std::sqrt(cfg.mg_ampl03);
+    const double asp[6] = {el, el, el, 0.0, 0.0, 0.0};
+    int l7[7][3];
+    double w7[7];
+    if (!line_hexad(asp, l7, w7)) {
+        *err = " In hextform; hexad, failed; check aspect tensor";
+        return MGBF_ERR_CONFIG;
+    }
+    // hextform: the six non-null lines in color order, each as a dibeta filter
+    std::vector<double> f[3] = {{1.0}, {1.0}, {1.0}};
+    int axis_color[3] = {0, 0, 0};
+    int nline = 0;
+    for (int c = 1; c <= 7; ++c) {
+        const int* g = l7[c - 1];
+        if (std::abs(g[0]) + std::abs(g[1]) + std::abs(g[2]) == 0) continue;
+        ++nline;
+        double hs = w7[c - 1];
+        if (!line_hstform(bsprds, hs)) {
+            *err = fmt(" In hextform; hstform failed at i=%d", nline);
+            return MGBF_ERR_CONFIG;
+        }
+        std::vector<double> fil = line_blinfil(bnorm, cfg.p, hs);
+        if (fil.empty()) {
+            *err = "hexad line half-span outside the moment table";
+            return MGBF_ERR_CONFIG;
+        }
+        if (is_identity(fil)) continue;
+        int ax = -1;
+        for (int e = 0; e < 3; ++e)
+            if (std::abs(g[e]) == 1 && g[(e + 1) % 3] == 0 && g[(e + 2) % 3] == 0) ax = e;
+        if (ax < 0 || axis_color[ax]) {
+            *err = "line filters off the grid axes are not built";
+            return MGBF_ERR_UNSUPPORTED;
+        }
+        const int halo[3] = {cfg.hx, cfg.hy, cfg.hz};
+        if ((int)fil.size() - 1 > halo[ax]) {
+            *err = "line filter reach exceeds the halo";
+            return MGBF_ERR_CONFIG;
+        }
+        f[ax] = fil;
+        axis_color[ax] = c;
+    }
+    if (nline != 6) {
+        *err = " In hextform; inconsistent hexad generator set found";
+        return MGBF_ERR_CONFIG;
+    }
+    // filtering_lin3 mirrors the level halo once, before the color loop;
+    // every call ends with a = b, which zeroes the halos: only a line of
+    // color 1 reads the mirror, the others read zeros beyond the column ends
+    // (and the adjoint folds no level halo)
+    if (axis_color[2] == 1) {
+        *err = "a level line of the first color is not built";
+        return MGBF_ERR_UNSUPPORTED;
+    }
+    const int hx = (int)f[0].size() - 1, hy = (int)f[1].size() - 1, hz = (int)f[2].size() - 1;
+    for (int c = -hz; c <= hz; ++c)
+        for (int b = -hy; b <= hy; ++b)
+            for (int a = -hx; a <= hx; ++a)
+                taps.push_back({a, b, c, f[0][std::abs(a)] * f[1][std::abs(b)] * f[2][std::abs(c)]});
+    return MGBF_OK;
+}
 
 int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err) {
     taps.clear();
@@ -398,35 +588,15 @@ int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err)
     std::vector<double> fx{1.0}, fy{1.0};
     bool have_x = false, have_y = false;
     for (int c = 0; c < 3; ++c) {
-        // hstform (jp_pbfil3.f90:1399-1430)
-        const bool neg = hs[c] < 0.0;
-        bool found = false;
-        for (int h = 2; h <= kLineNH && !found; ++h)
-            if (bsprds[h] >= hs[c]) {
-                hs[c] = h - (bsprds[h] - hs[c]) / (bsprds[h] - bsprds[h - 1]);
-                found = true;
-            }
-        if (neg || !found) {
+        if (!line_hstform(bsprds, hs[c])) {
             *err = fmt(" In tritform; hstform failed at i=%d", c + 1);
             return MGBF_ERR_CONFIG;
         }
         if (l[c][0] == 0 && l[c][1] == 0) continue;
-        // blinfil (jp_pbfil3.f90:1463-1501); failing, vdibeta2 leaves the field
-        const int h = (int)hs[c], hp = h + 1;
-        if (h < 1 || hp > kLineNH || hp > kLineNFil) continue;
-        std::vector<double> fil(h + 1, 0.0);
-        const double whp = (hs[c] - h) * bnorm[hp], wh = (hp - hs[c]) * bnorm[h];
-        for (int i = 0; i <= h; ++i) {
-            const double z = (double)i / hp;
-            fil[i] = whp * ipow(1.0 - z * z, cfg.p);
-        }
-        for (int i = 0; i < h; ++i) {
-            const double z = (double)i / h;
-            fil[i] += wh * ipow(1.0 - z * z, cfg.p);
-        }
-        bool ident = fil[0] == 1.0;
-        for (int i = 1; i <= h; ++i) ident = ident && fil[i] == 0.0;
-        if (ident) continue;
+        // a failing blinfil makes vdibeta2 leave the field as it is
+        const std::vector<double> fil = line_blinfil(bnorm, cfg.p, hs[c]);
+        if (fil.empty() || is_identity(fil)) continue;
+        const int h = (int)fil.size() - 1;
         const bool xline = l[c][1] == 0 && std::abs(l[c][0]) == 1;
         const bool yline = l[c][0] == 0 && std::abs(l[c][1]) == 1;
         if ((xline && have_x) || (yline && have_y) || (!xline && !yline)) {
@@ -529,9 +699,9 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "invalid namelist value (sizes must be positive)";
         return MGBF_ERR_CONFIG;
     }
-    if (c.mgbf_proc < 1 || c.mgbf_proc > 8 || c.mgbf_proc == 5) {
+    if (c.mgbf_proc < 1 || c.mgbf_proc > 8) {
         *err = fmt("mgbf_proc=%d is not built (built: 1 filtering_rad2_bckg, 2 filtering_lin2_bckg, "
-                   "3 filtering_fast_bckg, 4 filtering_rad3, 6 filtering_rad2_ens, "
+                   "3 filtering_fast_bckg, 4 filtering_rad3, 5 filtering_lin3, 6 filtering_rad2_ens, "
                    "7 filtering_lin2_ens, 8 filtering_fast_ens)",
                    c.mgbf_proc);
         return MGBF_ERR_UNSUPPORTED;
@@ -542,7 +712,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     }
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
-    const bool bckg = c.mgbf_proc >= 1 && c.mgbf_proc <= 4;
+    const bool bckg = c.mgbf_proc >= 1 && c.mgbf_proc <= 5;
     if (bckg) {
         if (c.n_ens != 1) {
             *err = fmt("mgbf_proc=%d (background) needs n_ens=1", c.mgbf_proc);
@@ -634,7 +804,7 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
     d.ma = c.mm0;
     d.km_a_all = (bckg ? 1 : c.n_ens) * (c.km2 + c.km3 * c.lm_a);
     d.anal_size = (long long)d.km_a_all * d.na * d.ma;
-    if ((c.l_vertical_filter || (c.mgbf_proc == 4 && c.km3 > 0)) && c.lm <= c.hz) {
+    if ((c.l_vertical_filter || ((c.mgbf_proc == 4 || c.mgbf_proc == 5) && c.km3 > 0)) && c.lm <= c.hz) {
         *err = "lm must exceed hz for the vertical mirror";
         return MGBF_ERR_CONFIG;
     }
@@ -667,7 +837,7 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         if (!ball_reach_ok(cfg, err)) return MGBF_ERR_CONFIG;
     }
     const bool fast = cfg.mgbf_proc == 3 || cfg.mgbf_proc == 8;
-    const bool line = cfg.mgbf_proc == 2 || cfg.mgbf_proc == 7;
+    const bool line = cfg.mgbf_proc == 2 || cfg.mgbf_proc == 5 || cfg.mgbf_proc == 7;
     if (line) {
         // filtering_lin2_*: the triad of dibeta line filters
         st = line_taps(cfg, s.taps2, err);
@@ -688,6 +858,10 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         return MGBF_ERR_CONFIG;
     }
     if (cfg.mgbf_proc == 4) ball_taps(cfg, s.taps3);   // filtering_rad3 (reach checked above)
+    if (cfg.mgbf_proc == 5) {                           // filtering_lin3
+        st = hex_taps(cfg, s.taps3, &s.zero_levels, err);
+        if (st != MGBF_OK) return st;
+    }
     s.dims.ntaps2 = (int)s.taps2.size();
     s.dims.ntaps3 = (int)s.taps3.size();
     s.dims.ntaps1 = (int)s.taps1_gx.size();

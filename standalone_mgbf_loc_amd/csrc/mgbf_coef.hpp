@@ -40,7 +40,8 @@ struct Setup {
     mgbf_dims dims{};
     int nx[kMaxGen] = {}, ny[kMaxGen] = {};
     std::vector<Tap2> taps2;           // reference loop order
-    std::vector<Tap3> taps3;           // 3-D filter of procedure 4 (vrbeta3 loop order)
+    std::vector<Tap3> taps3;           // 3-D filter of procedures 4 (vrbeta3 loop order) and 5
+    bool zero_levels = false;          // the 3-D filter reads zeros beyond the column ends (5)
     std::vector<int> taps1_gx;         // vertical taps
     std::vector<double> taps1_w;
     int H = 0;                          // max(hx, hy) used by the radial kernels
@@ -78,5 +79,8 @@ bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<dou
 // 2-D taps of the triad line filters of procedures 2 and 7 (mgbf_line), or
 // the identity without line tables. Returns MGBF_OK or an error status.
 int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err);
+// 3-D taps of the hexad line filters of procedure 5 (mgbf_line): the x, y and
+// level lines as a product; *zero_levels: zeros beyond the column ends.
+int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err);
 
 }  // namespace mgbf

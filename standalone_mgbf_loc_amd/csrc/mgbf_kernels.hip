@@ -523,8 +523,9 @@ kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int 
     }
 }
 
-// 3-D radial filter of procedure 4 (vrbeta3 / vrbeta3T with sup_vrbeta3's
-// level mirror) and the 2-D filter of its 2-D variables, as a gather: one
+// 3-D filter of procedures 4 and 5 (vrbeta3 / vrbeta3T with sup_vrbeta3's
+// level mirror; the hexad lines with zeros beyond the column ends) and the
+// 2-D filter of the 2-D variables, as a gather: one
 // lane per stacked k, one wave per 64 k of a grid point, taps wave-uniform.
 // The transpose gathers every level L whose mirrored shift L + c lands on
 // the output level (at most one direct and one mirrored preimage).
@@ -547,12 +548,15 @@ __global__ void __launch_bounds__(kBlock) ball_kernel(BallArgs a, int nkc, long 
             const float* col = a.in + (size_t)a.K * ((size_t)ii + (size_t)a.n * jj) + v0;
             float s;
             if (!T) {
-                s = col[mirror_whole(L + g.z, lm)];
+                const int Ls = L + g.z;
+                s = (a.zero_levels && (Ls < 0 || Ls >= lm)) ? 0.f : col[mirror_whole(Ls, lm)];
             } else {
                 const int L1 = L - g.z, L2 = -L - g.z, L3 = 2 * (lm - 1) - L - g.z;
                 s = (L1 >= 0 && L1 < lm) ? col[L1] : 0.f;
-                if (L > 0 && L2 >= 0 && L2 < lm) s += col[L2];
-                if (L < lm - 1 && L3 >= 0 && L3 < lm) s += col[L3];
+                if (!a.zero_levels) {
+                    if (L > 0 && L2 >= 0 && L2 < lm) s += col[L2];
+                    if (L < lm - 1 && L3 >= 0 && L3 < lm) s += col[L3];
+                }
             }
             acc = __builtin_fmaf(a.w3[t], s, acc);
         }

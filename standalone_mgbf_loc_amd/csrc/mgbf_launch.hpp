@@ -140,7 +140,7 @@ hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st);
 
-// The filter of procedure 4 (filtering_rad3) on one generation: the first
+// The filter of procedures 4 and 5 (filtering_rad3 / _lin3) on one generation: the first
 // K3 = nv*lm entries of the stack are nv columns of lm levels filtered over
 // (i, j, L) by the 3-D taps, the rest by the 2-D taps; half-sample mirror
 // in i and j, whole-sample mirror in L. transpose: the adjoint (the 2-D and
@@ -156,6 +156,7 @@ struct BallArgs {
     int n2;
     int K, n, m, lm, K3;
     int transpose;
+    int zero_levels;     // zeros beyond the column ends instead of the level mirror
 };
 hipError_t launch_ball(const BallArgs& a, hipStream_t st);
 

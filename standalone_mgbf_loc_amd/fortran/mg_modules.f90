@@ -228,7 +228,7 @@ end module mg_transfer
 !=======================================================================
 ! filtering_procedure (src.mgbf/mg_filtering.f90:67-120): W A W^T of the
 ! procedure the context was built for (1: rad2_bckg, 2: lin2_bckg,
-! 3: fast_bckg, 4: rad3, 6: rad2_ens, 7: lin2_ens, 8: fast_ens)
+! 3: fast_bckg, 4: rad3, 5: lin3, 6: rad2_ens, 7: lin2_ens, 8: fast_ens)
 !=======================================================================
 module mg_filtering
 use iso_c_binding

@@ -89,11 +89,13 @@ def ball_taps(h: int, hz: int, R2: Fraction, p: int):
     return [(a, b, c, float(w / s)) for a, b, c, w in taps]
 
 
-def rad3_matrix(n: int, m: int, K: int, lm: int, nv: int, taps3, taps2) -> np.ndarray:
+def rad3_matrix(n: int, m: int, K: int, lm: int, nv: int, taps3, taps2,
+                zero_levels: bool = False) -> np.ndarray:
     """The filter of procedure 4 on an (m, n, K) generation: the first nv*lm
     entries of k are nv columns of lm levels filtered over (i, j, L) with the
-    half-sample mirror in i and j and the whole-sample mirror in L; the rest
-    of k by the 2-D filter."""
+    half-sample mirror in i and j and the whole-sample mirror in L (or, with
+    zero_levels, zeros beyond the column ends); the rest of k by the 2-D
+    filter."""
     N = n * m * K
     F = np.zeros((N, N))
     for j in range(m):
@@ -104,6 +106,8 @@ def rad3_matrix(n: int, m: int, K: int, lm: int, nv: int, taps3, taps2) -> np.nd
                     v, L = divmod(k, lm)
                     for a, b, c, w in taps3:
                         ii, jj = mirror_half(i + a, n), mirror_half(j + b, m)
+                        if zero_levels and not 0 <= L + c < lm:
+                            continue
                         kk = v * lm + mirror_whole(L + c, lm)
                         F[row, (jj * n + ii) * K + kk] += w
                 else:
@@ -169,9 +173,11 @@ class DenseMGBF:
 
     def __init__(self, nx, ny, K, lm, nvcol, hx, hz, p, gm, R2, R2v,
                  weights, l_filt_g1=True, l_vertical_filter=True, wfield=None, fast=False,
-                 quad=False, taps2d=None, final_vft=False, rad3=None):
+                 quad=False, taps2d=None, final_vft=False, rad3=None, taps3d=None):
         """rad3 = (R2 of the 3-D ball, hz): procedure 4, whose filter couples
-        the levels of each 3-D variable and which has no vertical filter."""
+        the levels of each 3-D variable and which has no vertical filter.
+        taps3d: procedure 5 instead, the given (a, b, c, w) taps over (i, j, L)
+        with zeros beyond the column ends."""
         self.nx, self.ny, self.K, self.gm = nx, ny, K, gm
         self.final_vft = final_vft
         if taps2d is not None:
@@ -192,13 +198,13 @@ class DenseMGBF:
             Vcol[c * lm:(c + 1) * lm, c * lm:(c + 1) * lm] = V1
         self.Vf = np.kron(np.eye(nx * ny), Vcol) if l_vertical_filter else np.eye(nx * ny * K)
         self.F, self.P, self.A = [], [], []
-        if rad3 is not None:
+        if rad3 is not None or taps3d is not None:
             self.Vf = np.eye(nx * ny * K)
-            t3 = ball_taps(hx, rad3[1], Fraction(rad3[0]), p)
+            t3 = list(taps3d) if taps3d is not None else ball_taps(hx, rad3[1], Fraction(rad3[0]), p)
         for g in range(gm):
             n, m = nx >> g, ny >> g
-            if rad3 is not None:
-                self.F.append(rad3_matrix(n, m, K, lm, nvcol, t3, taps))
+            if rad3 is not None or taps3d is not None:
+                self.F.append(rad3_matrix(n, m, K, lm, nvcol, t3, taps, zero_levels=taps3d is not None))
             else:
                 self.F.append(np.kron(radial_matrix(n, m, taps), IK))
             if wfield is not None and wfield[g] is not None:

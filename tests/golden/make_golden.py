@@ -37,21 +37,25 @@ CASES = {
     # filtering_rad3 (mgbf_proc 4): 3-D variables over (i, j, L), 2-D variables
     "rad3_32x24_gm3_km2": dict(nx=32, ny=24, n_ens=1, lm=5, km3=2, km2=3, gm=3, h=4, hz=2,
                                weights=[1.0, 0.5, 0.25], proc=4),
+    # filtering_lin3 (mgbf_proc 5, mgbf_line): hexad lines of the 3-D variables
+    "lin3_32x24_gm3_km2": dict(nx=32, ny=24, n_ens=1, lm=6, km3=2, km2=2, gm=3, h=4, hz=3,
+                               weights=[1.0, 0.5, 0.25], proc=5, ampl02=0.7, ampl03=0.5),
 }
 
 
 def config(nx, ny, n_ens, lm, gm, h, hz, weights, l_filt_g1=True, lquart=False, proc=6,
-           km3=1, km2=0, ampl02=None):
+           km3=1, km2=0, ampl02=None, ampl03=None):
     ampl01 = (hz + 1) ** 2 / 7.0
     a2 = (h + 1) ** 2 / 8.0 if ampl02 is None else ampl02
-    a3 = (min(h, hz) + 1) ** 2 / 9.0
-    bckg = proc in (1, 2, 3, 4)
+    a3 = (min(h, hz) + 1) ** 2 / 9.0 if ampl03 is None else ampl03
+    bckg = proc in (1, 2, 3, 4, 5)
     c = dict(nx=nx, ny=ny, K=n_ens * (km2 + lm * km3), lm=lm, nvcol=n_ens * km3, hx=h, hy=h, hz=hz,
              p=2, gm=gm, ampl01=ampl01, ampl02=a2, weights=weights,
              l_filt_g1=True if bckg else l_filt_g1, n_ens=n_ens, lquart=lquart,
-             line=proc in (2, 7), line_tables=True, final_vft=proc == 7, rad3=proc == 4, ampl03=a3)
-    cfg = dict(mgbf_proc=proc, lquart=lquart, mgbf_line=proc in (2, 7), km2=km2, km3=km3,
-               l_filt_g1=l_filt_g1, mg_ampl02=a2)
+             line=proc in (2, 7), line_tables=True, final_vft=proc == 7, rad3=proc == 4, ampl03=a3,
+             lin3=proc == 5)
+    cfg = dict(mgbf_proc=proc, lquart=lquart, mgbf_line=proc in (2, 5, 7), km2=km2, km3=km3,
+               l_filt_g1=l_filt_g1, mg_ampl02=a2, mg_ampl03=a3)
     return c, cfg
 
 

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from standalone_mgbf_loc_amd import mgbf as M
@@ -77,7 +78,7 @@ def test_nm0_divisibility_is_checked():
     assert "not divisible" in str(_err(cfg))
 
 
-@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 5), ("mgbf_proc", 9)])
+@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 0), ("mgbf_proc", 9)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)
     setattr(cfg, key, val)
@@ -155,3 +156,19 @@ def test_rad3_procedure_taps_follow_the_oracle(ampl03, p, hz):
 def test_rad3_procedure_needs_more_levels_than_hz():
     cfg = M.Config.for_grid(64, 64, 2, gm=3, h=4, hz=2, mgbf_proc=4, km3=2, l_vertical_filter=False)
     assert "lm must exceed hz" in str(_err(cfg))
+
+
+@pytest.mark.parametrize("ampl03,hz", [(0.5, 3), (0.25, 4), (1.0, 2)])
+def test_lin3_procedure_taps_follow_the_oracle_hexad(ampl03, hz):
+    # the product of the x, y and level lines of the oracle's hextform
+    from oracle import oracle as O
+    cfg = M.Config.for_grid(64, 64, 8, gm=3, h=4, hz=hz, mgbf_proc=5, km3=2, mgbf_line=True)
+    cfg.mg_ampl02 = 0.7
+    cfg.mg_ampl03 = ampl03
+    P = O.OracleParams(nx=64, ny=64, K=16, lm=8, nvcol=2, hx=4, hy=4, hz=hz, p=2, lin3=True,
+                       ampl03=ampl03, ampl02=0.7)
+    lines = O.hexlines(P)
+    assert [g for _, g, _, _ in lines[:3]] == [(1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    assert M.derive(cfg).ntaps3 == int(np.prod([2 * h + 1 for _, _, h, _ in lines[:3]]))
+    cfg.mgbf_line = False
+    assert M.derive(cfg).ntaps3 == 1

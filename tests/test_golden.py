@@ -26,8 +26,8 @@ def _oparams(c):
 
 def test_fixtures_present():
     # radial (ensemble, no generation-1 filter, single field), lquart, line
-    # procedures 7 and 2, and the 3-D procedure 4
-    assert len(CASES) == 7
+    # procedures 7 and 2, and the 3-D procedures 4 and 5
+    assert len(CASES) == 8
 
 
 @pytest.mark.parametrize("name", CASES)

@@ -30,7 +30,7 @@ def relerr(y, y64):
 
 
 def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
-    bckg = cfg.mgbf_proc in (1, 2, 3, 4)
+    bckg = cfg.mgbf_proc in (1, 2, 3, 4, 5)
     return O.OracleParams(
         nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx, hy=cfg.hy,
         hz=cfg.hz, p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
@@ -38,6 +38,7 @@ def oparams(cfg: M.Config, d: M.Dims) -> O.OracleParams:
         l_vertical_filter=bool(cfg.l_vertical_filter), fast=cfg.mgbf_proc in (3, 8),
         lquart=bool(cfg.lquart), line=cfg.mgbf_proc in (2, 7), line_tables=bool(cfg.mgbf_line),
         final_vft=cfg.mgbf_proc == 7, rad3=cfg.mgbf_proc == 4, ampl03=cfg.mg_ampl03,
+        lin3=cfg.mgbf_proc == 5,
         weights=cfg.weights()[: d.gm] + [cfg.mg_weig5] * max(0, d.gm - 5))
 
 
@@ -480,6 +481,46 @@ def test_procedures_with_one_or_two_generations_and_ragged_stack(proc, gm):
         sync()
         ref = O.wawt(oparams(cfg, d), x.astype(np.float64), wfield=[w.astype(np.float64) for w in wf])
         assert relerr(y, ref) <= TOL_OP
+
+
+@pytest.mark.parametrize("case", ["lines", "no_tables", "lquart", "wide"])
+def test_lin3_procedure_matches_oracle(case):
+    # filtering_lin3 (mgbf_proc = 5, mgbf_line): hexad lines along x, y and
+    # the levels (colors 1, 2, 3; zeros beyond the column ends), the 2-D
+    # variables through the triad lines; every generation, no vertical filter
+    kw = dict(gm=3, h=4, hz=3, mgbf_proc=5, km3=3, km2=2, mgbf_line=True, weights=[1.0, 0.5, 0.25])
+    if case == "no_tables":
+        kw["mgbf_line"] = False
+    if case == "lquart":
+        kw["lquart"] = True
+    cfg = M.Config.for_grid(64, 48, 12, **kw)
+    cfg.mg_ampl02 = 0.7
+    cfg.mg_ampl03 = 0.25 if case == "wide" else 0.5
+    if case == "wide":
+        cfg.hz = 4          # level reach 4 at spread**2 2
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        if case != "no_tables":
+            assert d.ntaps3 == int(np.prod([2 * h + 1 for _, _, h, _ in O.hexlines(P)[:3]]))
+        x = field(d.gen_shape(1), 180)
+        v = np.random.default_rng(181).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        f = op.stage(M.STAGE_F, 1, xd)
+        ft = op.stage(M.STAGE_FT, 1, xd)
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        assert relerr(f, O.radial(P, x.astype(np.float64))) <= TOL_STAGE
+        assert relerr(ft, O.radial(P, x.astype(np.float64), adjoint=True)) <= TOL_STAGE
+        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs = op.dot(w, xd)
+        rhs = op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
 
 
 def test_dot_is_fp64_accumulated():

@@ -596,3 +596,70 @@ def test_rad3_wawt_matches_dense_model(lquart):
     Bd = Dm.B()
     assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
     assert np.max(np.abs(B - B.T)) < 1e-13 * np.max(np.abs(B))
+
+
+# --------------------------------------------------------------------------
+# hexad line filters of the 3-D variables (filtering_lin3, procedure 5)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_hexad_resolves_the_aspect_onto_six_lines(seed):
+    # sum_c w_c l_c l_c^T reproduces the tensor, w >= 0, one color is null
+    rng = np.random.default_rng(200 + seed)
+    M = rng.standard_normal((3, 3))
+    A = M @ M.T + (0.05 + 0.5 * seed) * np.eye(3)
+    a = [A[0, 0], A[1, 1], A[2, 2], A[1, 2], A[2, 0], A[0, 1]]
+    L, w, ff = O.hexad(a)
+    assert not ff
+    assert (w >= -1e-13).all()
+    R = sum(w[c] * np.outer(L[c], L[c]) for c in range(7))
+    assert np.max(np.abs(R - A)) < 1e-12 * np.max(np.abs(A))
+    assert sum(1 for c in range(7) if not L[c].any()) == 1
+
+
+def test_hexad_lines_of_the_domain_are_the_axes_in_color_order():
+    # the diagonal factor 1/sqrt(mg_ampl03) I: x, y, z lines with colors 1, 2, 3
+    # and spread**2 1/sqrt(mg_ampl03); colors 4-6 carry idle lines
+    P = _P(lin3=True, ampl03=0.25, hx=4, hy=4, hz=4)
+    lines = O.hexlines(P)
+    assert [(q, g) for q, g, _, _ in lines[:3]] == [(1, (1, 0, 0)), (2, (0, 1, 0)), (3, (0, 0, 1))]
+    for q, g, h, fil in lines[:3]:
+        full = {a: fil[abs(a)] for a in range(-h, h + 1)}
+        assert abs(_spread2(full) - 2.0) < 1e-13
+    for q, g, h, fil in lines[3:]:
+        assert q in (4, 5, 6) and fil[0] == 1.0 and not fil[1:].any()
+
+
+def test_lin3_levels_end_with_zeros_and_the_adjoint_is_the_transpose():
+    # the level halo is mirrored once, then zeroed by the first line's
+    # a = b: the z line (color 3) meets zeros beyond the column ends, and its
+    # adjoint drops what it scatters there (bocoT_3d folds x and y only)
+    P = _P(nx=16, ny=12, K=9, lm=4, nvcol=2, hx=4, hy=4, hz=3, gm=2, lin3=True, ampl03=0.5,
+           ampl02=0.7)
+    x, y = _rand((12, 16, 9), 97), _rand((12, 16, 9), 98)
+    lhs = np.vdot(O.radial(P, x), y)
+    rhs = np.vdot(x, O.radial(P, y, adjoint=True))
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    out = O.radial(P, np.ones((12, 16, 9)))
+    fz = _dibeta_from_spread(P.p, 1 / np.sqrt(0.5))
+    edge = sum(w for c, w in fz.items() if 0 <= c < 4)     # level 0 keeps taps c >= 0
+    assert np.allclose(out[..., 0], edge, atol=1e-14)
+
+
+@pytest.mark.parametrize("lquart", [False, True])
+def test_lin3_wawt_matches_dense_model(lquart):
+    P = _P(nx=16, ny=12, K=5, lm=3, nvcol=1, hx=3, hy=3, hz=2, gm=2, lin3=True, ampl03=1.5,
+           ampl02=Fraction(3, 4), weights=[1.0, 0.6], lquart=lquart)
+    shape = P.gen_shape(0)
+    N = int(np.prod(shape))
+    B = _oracle_matrix(P, lambda x: O.wawt(P, x.reshape(shape)).ravel(), N)
+    f3 = _dibeta_from_spread(P.p, 1 / np.sqrt(1.5))
+    f2 = _dibeta_from_spread(P.p, 1 / np.sqrt(0.75))
+    taps3 = [(a, b, c, f3[a] * f3[b] * f3[c]) for c in f3 for b in f3 for a in f3]
+    taps2 = [(a, b, f2[a] * f2[b]) for b in f2 for a in f2]
+    Dm = D.DenseMGBF(P.nx, P.ny, P.K, P.lm, P.nvcol, P.hx, P.hz, P.p, P.gm, None,
+                     Fraction(4, 7), list(P.wconst()), True, True, None, quad=lquart,
+                     taps2d=taps2, taps3d=taps3)
+    Bd = Dm.B()
+    assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
+    assert np.max(np.abs(B - B.T)) < 1e-13 * np.max(np.abs(B))
