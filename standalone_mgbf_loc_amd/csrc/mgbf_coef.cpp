@@ -706,10 +706,11 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
                    c.mgbf_proc);
         return MGBF_ERR_UNSUPPORTED;
     }
-    if (c.l_loc) {
-        *err = "l_loc=.true. (localization generations) is not built";
-        return MGBF_ERR_UNSUPPORTED;
-    }
+    // l_loc and l_loc_vertical change no result: l_loc only builds the MPI
+    // neighbour tables of init_domain_loc (mg_entrymod.f90:94-96) and the
+    // w*_loc weight arrays (mg_intstate.f90:145-152, 325-330), and the only
+    // readers of those, filtering_rad2_z_loc_g3/g4 (mg_filtering.f90:1510-1830),
+    // are private and never called by filtering_procedure (:100-117).
     // km, km_all (mg_parameter.f90:293-297)
     d.km = c.km2 + c.lm * c.km3;
     const bool bckg = c.mgbf_proc >= 1 && c.mgbf_proc <= 5;

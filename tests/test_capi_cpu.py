@@ -2,6 +2,7 @@
 point include/mgbf.h declares, and derives/validates configurations the way
 init_mg_parameter does (src.mgbf/mg_parameter.f90:179-635)."""
 import ctypes
+import dataclasses
 import os
 import re
 
@@ -78,11 +79,23 @@ def test_nm0_divisibility_is_checked():
     assert "not divisible" in str(_err(cfg))
 
 
-@pytest.mark.parametrize("key,val", [("l_loc", True), ("mgbf_proc", 0), ("mgbf_proc", 9)])
+@pytest.mark.parametrize("key,val", [("mgbf_proc", 0), ("mgbf_proc", 9)])
 def test_options_not_built_are_refused(key, val):
     cfg = M.Config.for_grid(192, 192, 32)
     setattr(cfg, key, val)
     assert _err(cfg).status == -4
+
+
+def test_l_loc_is_accepted_and_changes_no_dimension():
+    """l_loc only sets up the MPI tables and weight arrays of the uncalled
+    filtering_rad2_z_loc_g3/g4 (mg_filtering.f90:1510-1830), so the operator
+    is the one without it."""
+    cfg = M.Config.for_grid(192, 192, 32)
+    base = M.derive(cfg)
+    cfg.l_loc = True
+    cfg.l_loc_vertical = True
+    d = M.derive(cfg)
+    assert dataclasses.astuple(d) == dataclasses.astuple(base)
 
 
 def test_ensemble_vertical_filter_with_2d_variables_is_refused():
