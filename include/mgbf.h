@@ -94,12 +94,22 @@ enum {
     MGBF_OP_ASQRT = 4, /* v = A^{1/2} v   : multigrid -> multigrid */
     MGBF_OP_BSQRT = 5, /* y = W A^{1/2} v : multigrid vector -> field */
     MGBF_OP_BSQRTT = 6, /* v = A^{1/2} W^T x : field -> multigrid vector */
-    MGBF_OP_A2F = 7,   /* anal_to_filt_all2 (mg_transfer.f90:154-176): analysis field
+    MGBF_OP_A2F = 7,   /* anal_to_filt_all2 (mg_transfer.f90:154-183): analysis field
                           (km_a_all x na x ma) -> generation-1 field; the transpose of F2A */
-    MGBF_OP_F2A = 8    /* filt_to_anal_all2 (mg_transfer.f90:179-193): generation-1 field ->
-                          analysis field: 4-point horizontal interpolation (lsqr_direct_offset,
-                          mg_interpolate.f90:634-681) then lm -> lm_a = 2 lm - 1 levels
-                          (l_vertical_direct_spec2, :1006-1044); a copy when the grids agree */
+    MGBF_OP_F2A = 8,   /* filt_to_anal_all2 (mg_transfer.f90:186-203): generation-1 field ->
+                          analysis field: the horizontal interpolation of filt_to_anal
+                          (:406-464; lsqr_direct_offset mg_interpolate.f90:634-681, or
+                          lin_direct_offset :858-903 with l_lin_horizontal, or
+                          quad_direct_offset :750-795 with l_quad_horizontal), then lm -> lm_a
+                          levels (l_vertical_direct_spec2, :1006-1044; odd lm_a <= 2 lm - 1,
+                          km2 = 0); a copy when the grids agree */
+    MGBF_OP_A2F_ALL = 9,  /* anal_to_filt_all (mg_transfer.f90:58-110): the transpose of
+                             F2A_ALL */
+    MGBF_OP_F2A_ALL = 10  /* filt_to_anal_all (mg_transfer.f90:113-151): the horizontal
+                             interpolation of F2A, then lm -> lm_a levels of the 3-D variables
+                             (S2C_ens / C2S_ens, km2 = 0) with l_vertical_direct_spec
+                             (:595-631) under l_lin_vertical, else lwq_vertical_direct_spec
+                             (:511-553, weights of lwq_vertical_coef :301-370) */
 };
 
 /* single stages of mgbf_apply_stage (gen is 1-based) */

@@ -1731,24 +1731,52 @@ void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* con
  * analysis <-> filter transfer
  * ---------------------------------------------------------------------- */
 
-/* lsqr_mg_coef (mg_interpolate.f90:89-191) on one PE tile: analysis points at
-   xa(n) = dxa/2 + dxa (n-1), filter points at xf(i) = dxf/2 + dxf (i-1) over a
-   tile of length nloc_a (mg_parameter.f90:518-541); iref(n) = (first i in
-   1-ib .. im+ib-1 with xa(n) < xf(i)) - 2, ib = 6; the 4 weights blend the
-   quadratic interpolants through points iref..iref+2 and iref+1..iref+3
-   linearly between points iref+1 and iref+2. Returns -1 if no i is found. */
-static int anal_coef(int nloc_a, int nloc_f, int* iref, double* cx)
+/* Stencil width and halo of the horizontal schemes of anal_to_filt /
+   filt_to_anal (mg_transfer.f90:339-464): 0 lsqr (4 points, ibm = 3),
+   1 lin (2 points, ibm = 1), 2 quad (3 points, ibm = 2). */
+static int anal_width(int scheme) { return scheme == 1 ? 2 : scheme == 2 ? 3 : 4; }
+static int anal_halo(int scheme) { return scheme == 1 ? 1 : scheme == 2 ? 2 : 3; }
+
+/* lsqr_mg_coef (mg_interpolate.f90:89-298) on one PE tile: analysis points
+   at xa(n) = dxa/2 + dxa (n-1), filter points at xf(i) = dxf/2 + dxf (i-1)
+   over a tile of length nloc_a (mg_parameter.f90:518-541). With i0 the first
+   i in 1-ib .. im+ib-1 (ib = 6) with xa(n) < xf(i):
+     lsqr: iref = i0-2; the 4 weights blend the quadratic interpolants through
+           points iref..iref+2 and iref+1..iref+3 linearly between points
+           iref+1 and iref+2 (:156-191);
+     lin:  irefL = i0-1; the linear weights Lx0, Lx1 (:269-296);
+     quad: irefq = i0-1; the 3-point Lagrange weights qx0..qx2 (:231-267).
+   Returns -1 if no i0 is found. */
+static int anal_coef(int nloc_a, int nloc_f, int scheme, int* iref, double* cx)
 {
     const double len = (double)nloc_a;
     const double dxa = len / nloc_a, dxf = len / nloc_f;
+    const int w = anal_width(scheme);
     for (int n = 1; n <= nloc_a; ++n) {
         const double x = 0.5 * dxa + dxa * (n - 1);
         int i0 = 0, found = 0;
         for (int i = 1 - 6; i <= nloc_f + 6 - 1; ++i)
-            if (x < 0.5 * dxf + dxf * (i - 1)) { i0 = i - 2; found = 1; break; }
+            if (x < 0.5 * dxf + dxf * (i - 1)) { i0 = i; found = 1; break; }
         if (!found) return -1;
+        double* c = cx + (size_t)w * (n - 1);
+        if (scheme == 1) {
+            const double z0 = 0.5 * dxf + dxf * (i0 - 2), z1 = 0.5 * dxf + dxf * (i0 - 1);
+            iref[n - 1] = i0 - 1;
+            c[0] = (z1 - x) / (z1 - z0);
+            c[1] = (x - z0) / (z1 - z0);
+            continue;
+        }
+        if (scheme == 2) {
+            double z[3];
+            for (int t = 0; t < 3; ++t) z[t] = 0.5 * dxf + dxf * (i0 - 2 + t);
+            iref[n - 1] = i0 - 1;
+            c[0] = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
+            c[1] = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
+            c[2] = (x - z[0]) * (x - z[1]) / ((z[2] - z[0]) * (z[2] - z[1]));
+            continue;
+        }
         double z[4];
-        for (int t = 0; t < 4; ++t) z[t] = 0.5 * dxf + dxf * (i0 + t - 1);
+        for (int t = 0; t < 4; ++t) z[t] = 0.5 * dxf + dxf * (i0 - 3 + t);
         /* left quadratic through z0,z1,z2 and right one through z1,z2,z3 */
         const double l0 = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
         const double l1 = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
@@ -1757,16 +1785,124 @@ static int anal_coef(int nloc_a, int nloc_f, int* iref, double* cx)
         const double r1 = (x - z[1]) * (x - z[3]) / ((z[2] - z[1]) * (z[2] - z[3]));
         const double r2 = (x - z[1]) * (x - z[2]) / ((z[3] - z[1]) * (z[3] - z[2]));
         const double wl = (z[2] - x) / (z[2] - z[1]), wr = (x - z[1]) / (z[2] - z[1]);
-        iref[n - 1] = i0;
-        cx[4 * (n - 1) + 0] = wl * l0;
-        cx[4 * (n - 1) + 1] = wl * l1 + wr * r0;
-        cx[4 * (n - 1) + 2] = wl * l2 + wr * r1;
-        cx[4 * (n - 1) + 3] = wr * r2;
+        iref[n - 1] = i0 - 2;
+        c[0] = wl * l0;
+        c[1] = wl * l1 + wr * r0;
+        c[2] = wl * l2 + wr * r1;
+        c[3] = wr * r2;
     }
     return 0;
 }
 
-#define XH 3   /* transfer halo (ibm = jbm = 3 in anal_to_filt / filt_to_anal) */
+/* lwq_vertical_coef (mg_interpolate.f90:301-370) for nm analysis levels
+   over im filter levels: filter level i at height x(i) = i-1, analysis level
+   n at y(n) = (n-1)(im-1)/(nm-1) (the last one pinned to x(im)); interior n
+   takes kref = floor(y)+1 and the 4-point lsqr-type weights on x(kref-1..
+   kref+2), with the weight of x(0) folded onto x(2) when kref = 1 and that
+   of x(im+1) onto x(im-1) when kref = im-1 (whole-sample mirror). n = 1 and
+   n = nm copy levels 1 and im. */
+void mgbfo_lwq_coef(int nm, int im, double* c, int* kref)
+{
+    const double dy = 1.0 * (im - 1) / (nm - 1);
+    for (int n = 2; n <= nm - 1; ++n) {
+        const double y = (n - 1) * dy;
+        const int i = (int)(y + 1.0);
+        const double x1 = i - 2, x2 = i - 1, x3 = i, x4 = i + 1;
+        const double d1 = y - x1, d2 = y - x2, d3 = y - x3, d4 = y - x4;
+        const double d13 = d1 * d3, d23 = 0.5 * d2 * d3, d24 = d2 * d4;
+        double* cn = c + 4 * (n - 1);
+        kref[n - 1] = i;
+        cn[0] = -d23 * d3;
+        cn[1] = (d13 + 0.5 * d24) * d3;
+        cn[2] = -(0.5 * d13 + d24) * d2;
+        cn[3] = d23 * d2;
+        if (i == 1) { cn[2] += cn[0]; cn[0] = 0.0; }
+        if (i == im - 1) { cn[1] += cn[3]; cn[3] = 0.0; }
+    }
+    kref[0] = 1;
+    c[0] = 0.0; c[1] = 1.0; c[2] = 0.0; c[3] = 0.0;
+    kref[nm - 1] = im;
+    c[4 * (nm - 1) + 0] = 0.0; c[4 * (nm - 1) + 1] = 1.0;
+    c[4 * (nm - 1) + 2] = 0.0; c[4 * (nm - 1) + 3] = 0.0;
+}
+
+/* One column: lm filter levels f -> la analysis levels o. vmap 0:
+   l_vertical_direct_spec(2) (:595-631, :1006-1044), odd levels copy and even
+   ones average; vmap 1: lwq_vertical_direct_spec (:511-553) with kref / c
+   from mgbfo_lwq_coef. */
+static void vert_direct(int vmap, int lm, int la, const int* kref, const double* c, const double* f,
+                        double* o)
+{
+    if (vmap == 0) {
+        int k = 1;
+        for (int n = 1; n <= la; n += 2, ++k) o[n - 1] = f[k - 1];
+        k = 1;
+        for (int n = 2; n <= la - 1; n += 2, ++k) o[n - 1] = 0.5 * (f[k - 1] + f[k]);
+        return;
+    }
+    for (int n = 2; n <= la - 1; ++n) {
+        const int k = kref[n - 1];
+        const double* cn = c + 4 * (n - 1);
+        if (k == 1)
+            o[n - 1] = cn[1] * f[k - 1] + cn[2] * f[k] + cn[3] * f[k + 1];
+        else if (k == lm - 1)
+            o[n - 1] = cn[0] * f[k - 2] + cn[1] * f[k - 1] + cn[2] * f[k];
+        else if (k == lm)
+            o[n - 1] = cn[1] * f[k - 1];
+        else
+            o[n - 1] = cn[0] * f[k - 2] + cn[1] * f[k - 1] + cn[2] * f[k] + cn[3] * f[k + 1];
+    }
+    o[0] = f[0];
+    o[la - 1] = f[lm - 1];
+}
+
+/* The transpose of vert_direct: l_vertical_adjoint_spec(2) (:556-592,
+   :963-1003) or lwq_vertical_adjoint_spec (:461-508). f is cleared first. */
+static void vert_adjoint(int vmap, int lm, int la, const int* kref, const double* c, const double* o,
+                         double* f)
+{
+    for (int k = 0; k < lm; ++k) f[k] = 0.0;
+    if (vmap == 0) {
+        int k = 1;
+        for (int n = 2; n <= la - 1; n += 2, ++k) {
+            f[k - 1] += 0.5 * o[n - 1];
+            f[k] += 0.5 * o[n - 1];
+        }
+        k = 1;
+        for (int n = 1; n <= la; n += 2, ++k) f[k - 1] += o[n - 1];
+        return;
+    }
+    for (int n = 2; n <= la - 1; ++n) {
+        const int k = kref[n - 1];
+        const double* cn = c + 4 * (n - 1);
+        const double w = o[n - 1];
+        if (k == 1) {
+            f[0] += cn[1] * w; f[1] += cn[2] * w; f[2] += cn[3] * w;
+        } else if (k == lm - 1) {
+            f[lm - 3] += cn[0] * w; f[lm - 2] += cn[1] * w; f[lm - 1] += cn[2] * w;
+        } else if (k == lm) {
+            f[k - 1] += cn[1] * w;
+        } else {
+            f[k - 2] += cn[0] * w; f[k - 1] += cn[1] * w; f[k] += cn[2] * w; f[k + 1] += cn[3] * w;
+        }
+    }
+    f[0] += o[0];
+    f[lm - 1] += o[la - 1];
+}
+
+/* The vertical map must stay inside each column and write every analysis
+   level: the linear one needs an odd la with (la+1)/2 <= lm (an even la
+   leaves level la unwritten), the lwq one lm >= 3 (its edge branches touch
+   3 levels). */
+static void anal_check(const mgbfo_params* P, const mgbfo_anal* A, int K2)
+{
+    const int lm = P->lm, la = A->lm_a;
+    if (la < lm) abort();
+    if (la != lm && K2 != 0) abort();   /* the maps run over the 3-D columns only */
+    if (la != lm && A->vmap == 0 && ((la + 1) / 2 > lm || la % 2 == 0)) abort();
+    if (la != lm && A->vmap == 1 && lm < 3) abort();
+    if (A->scheme < 0 || A->scheme > 2 || A->vmap < 0 || A->vmap > 1) abort();
+}
 
 void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double* vf, double* wa)
 {
@@ -1774,18 +1910,21 @@ void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double
     const int im = nx / A->npx, jm = ny / A->npy, nm = A->na / A->npx, mm = A->ma / A->npy;
     const int lm = P->lm, la = A->lm_a, nv = P->nvcol;
     const int K2 = K - nv * lm, Ka = nv * la + K2;
-    /* the reference's spec2 maps run over the nv columns only (km2 = 0) */
-    if (la != lm && K2 != 0) abort();
+    anal_check(P, A, K2);
+    const int w = anal_width(A->scheme), XH = anal_halo(A->scheme);
     int* iref = (int*)malloc(sizeof(int) * nm);
     int* jref = (int*)malloc(sizeof(int) * mm);
-    double* cx = (double*)malloc(sizeof(double) * 4 * nm);
-    double* cy = (double*)malloc(sizeof(double) * 4 * mm);
-    if (anal_coef(nm, im, iref, cx) || anal_coef(mm, jm, jref, cy)) abort();
-    /* boco_2d: the whole-domain field with a half-sample mirror halo of 3 */
+    double* cx = (double*)malloc(sizeof(double) * w * nm);
+    double* cy = (double*)malloc(sizeof(double) * w * mm);
+    if (anal_coef(nm, im, A->scheme, iref, cx) || anal_coef(mm, jm, A->scheme, jref, cy)) abort();
+    int* kref = (int*)malloc(sizeof(int) * la);
+    double* cv = (double*)malloc(sizeof(double) * 4 * la);
+    if (la != lm && A->vmap == 1) mgbfo_lwq_coef(la, lm, cv, kref);
+    /* boco_2d: the whole-domain field with a half-sample mirror halo of XH */
     const int px = nx + 2 * XH, py = ny + 2 * XH;
     double* V = (double*)malloc(sizeof(double) * (size_t)K * px * py);
     mgbfo_pad(K, nx, ny, XH, vf, V);
-    /* lsqr_direct_offset on every PE tile, at K levels of the filter */
+    /* *_direct_offset on every PE tile, at K levels of the filter */
     double* W = (double*)malloc(sizeof(double) * (size_t)K * A->na * A->ma);
     double* VX = (double*)malloc(sizeof(double) * (size_t)K * nm * (jm + 2 * XH));
     for (int tj = 0; tj < A->npy; ++tj)
@@ -1794,26 +1933,26 @@ void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double
                 for (int n = 1; n <= nm; ++n) {
                     double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
                     for (int k = 0; k < K; ++k) vx[k] = 0.0;
-                    for (int t = 0; t < 4; ++t) {
+                    for (int t = 0; t < w; ++t) {
                         const int gi = ti * im + iref[n - 1] + t;   /* 1-based */
                         const int gj = tj * jm + j;
                         const double* v = V + (size_t)K * ((size_t)(gj - 1 + XH) * px + (gi - 1 + XH));
-                        for (int k = 0; k < K; ++k) vx[k] += cx[4 * (n - 1) + t] * v[k];
+                        for (int k = 0; k < K; ++k) vx[k] += cx[w * (n - 1) + t] * v[k];
                     }
                 }
             for (int m = 1; m <= mm; ++m)
                 for (int n = 1; n <= nm; ++n) {
                     const int gn = ti * nm + n, gm = tj * mm + m;
-                    double* w = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
-                    for (int k = 0; k < K; ++k) w[k] = 0.0;
-                    for (int t = 0; t < 4; ++t) {
+                    double* o = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
+                    for (int k = 0; k < K; ++k) o[k] = 0.0;
+                    for (int t = 0; t < w; ++t) {
                         const int j = jref[m - 1] + t;
                         const double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
-                        for (int k = 0; k < K; ++k) w[k] += cy[4 * (m - 1) + t] * vx[k];
+                        for (int k = 0; k < K; ++k) o[k] += cy[w * (m - 1) + t] * vx[k];
                     }
                 }
         }
-    /* l_vertical_direct_spec2 over the nv columns (or a copy when lm_a = lm) */
+    /* the vertical map over the nv columns (a copy when lm_a = lm) */
     const long long npts = (long long)A->na * A->ma;
     for (long long q = 0; q < npts; ++q) {
         const double* f = W + (size_t)K * q;
@@ -1822,16 +1961,10 @@ void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double
             for (int k = 0; k < K; ++k) o[k] = f[k];
             continue;
         }
-        for (int e = 0; e < nv; ++e) {
-            int k = 1;
-            for (int n = 1; n <= la; n += 2, ++k) o[e * la + n - 1] = f[e * lm + k - 1];
-            k = 1;
-            for (int n = 2; n <= la - 1; n += 2, ++k)
-                o[e * la + n - 1] = 0.5 * (f[e * lm + k - 1] + f[e * lm + k]);
-        }
+        for (int e = 0; e < nv; ++e) vert_direct(A->vmap, lm, la, kref, cv, f + e * lm, o + e * la);
         for (int k = 0; k < K2; ++k) o[nv * la + k] = f[nv * lm + k];
     }
-    free(VX); free(W); free(V); free(cy); free(cx); free(jref); free(iref);
+    free(VX); free(W); free(V); free(cv); free(kref); free(cy); free(cx); free(jref); free(iref);
 }
 
 void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double* wa, double* vf)
@@ -1840,35 +1973,30 @@ void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double
     const int im = nx / A->npx, jm = ny / A->npy, nm = A->na / A->npx, mm = A->ma / A->npy;
     const int lm = P->lm, la = A->lm_a, nv = P->nvcol;
     const int K2 = K - nv * lm, Ka = nv * la + K2;
-    /* the reference's spec2 maps run over the nv columns only (km2 = 0) */
-    if (la != lm && K2 != 0) abort();
+    anal_check(P, A, K2);
+    const int w = anal_width(A->scheme), XH = anal_halo(A->scheme);
     int* iref = (int*)malloc(sizeof(int) * nm);
     int* jref = (int*)malloc(sizeof(int) * mm);
-    double* cx = (double*)malloc(sizeof(double) * 4 * nm);
-    double* cy = (double*)malloc(sizeof(double) * 4 * mm);
-    if (anal_coef(nm, im, iref, cx) || anal_coef(mm, jm, jref, cy)) abort();
-    /* l_vertical_adjoint_spec2 (a copy when lm_a = lm) */
+    double* cx = (double*)malloc(sizeof(double) * w * nm);
+    double* cy = (double*)malloc(sizeof(double) * w * mm);
+    if (anal_coef(nm, im, A->scheme, iref, cx) || anal_coef(mm, jm, A->scheme, jref, cy)) abort();
+    int* kref = (int*)malloc(sizeof(int) * la);
+    double* cv = (double*)malloc(sizeof(double) * 4 * la);
+    if (la != lm && A->vmap == 1) mgbfo_lwq_coef(la, lm, cv, kref);
+    /* the adjoint vertical map (a copy when lm_a = lm) */
     const long long npts = (long long)A->na * A->ma;
     double* W = (double*)calloc((size_t)K * npts, sizeof(double));
     for (long long q = 0; q < npts; ++q) {
-        const double* w = wa + (size_t)Ka * q;
+        const double* o = wa + (size_t)Ka * q;
         double* f = W + (size_t)K * q;
         if (la == lm) {
-            for (int k = 0; k < K; ++k) f[k] = w[k];
+            for (int k = 0; k < K; ++k) f[k] = o[k];
             continue;
         }
-        for (int e = 0; e < nv; ++e) {
-            int k = 1;
-            for (int n = 2; n <= la - 1; n += 2, ++k) {
-                f[e * lm + k - 1] += 0.5 * w[e * la + n - 1];
-                f[e * lm + k] += 0.5 * w[e * la + n - 1];
-            }
-            k = 1;
-            for (int n = 1; n <= la; n += 2, ++k) f[e * lm + k - 1] += w[e * la + n - 1];
-        }
-        for (int k = 0; k < K2; ++k) f[nv * lm + k] = w[nv * la + k];
+        for (int e = 0; e < nv; ++e) vert_adjoint(A->vmap, lm, la, kref, cv, o + e * la, f + e * lm);
+        for (int k = 0; k < K2; ++k) f[nv * lm + k] = o[nv * la + k];
     }
-    /* lsqr_adjoint_offset on every PE tile into the padded field */
+    /* *_adjoint_offset on every PE tile into the padded field */
     const int px = nx + 2 * XH, py = ny + 2 * XH;
     double* V = (double*)calloc((size_t)K * px * py, sizeof(double));
     double* VX = (double*)malloc(sizeof(double) * (size_t)K * nm * (jm + 2 * XH));
@@ -1878,25 +2006,25 @@ void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double
             for (int m = 1; m <= mm; ++m)
                 for (int n = 1; n <= nm; ++n) {
                     const int gn = ti * nm + n, gm = tj * mm + m;
-                    const double* w = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
-                    for (int t = 0; t < 4; ++t) {
+                    const double* o = W + (size_t)K * ((size_t)(gm - 1) * A->na + (gn - 1));
+                    for (int t = 0; t < w; ++t) {
                         const int j = jref[m - 1] + t;
                         double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
-                        for (int k = 0; k < K; ++k) vx[k] += w[k] * cy[4 * (m - 1) + t];
+                        for (int k = 0; k < K; ++k) vx[k] += o[k] * cy[w * (m - 1) + t];
                     }
                 }
             for (int n = 1; n <= nm; ++n)
                 for (int j = 1 - XH; j <= jm + XH; ++j) {
                     const double* vx = VX + (size_t)K * ((size_t)(j - 1 + XH) * nm + (n - 1));
-                    for (int t = 0; t < 4; ++t) {
+                    for (int t = 0; t < w; ++t) {
                         const int gi = ti * im + iref[n - 1] + t;
                         const int gj = tj * jm + j;
                         double* v = V + (size_t)K * ((size_t)(gj - 1 + XH) * px + (gi - 1 + XH));
-                        for (int k = 0; k < K; ++k) v[k] += vx[k] * cx[4 * (n - 1) + t];
+                        for (int k = 0; k < K; ++k) v[k] += vx[k] * cx[w * (n - 1) + t];
                     }
                 }
         }
     /* bocoT_2d: fold the halo back with the half-sample mirror */
     mgbfo_fold(K, nx, ny, XH, V, vf);
-    free(VX); free(V); free(W); free(cy); free(cx); free(jref); free(iref);
+    free(VX); free(V); free(W); free(cv); free(kref); free(cy); free(cx); free(jref); free(iref);
 }

@@ -123,23 +123,37 @@ void mgbfo_weight(const mgbfo_params* P, const double* wconst, const double* con
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int mgbfo_num_threads(void);
 
-/* ---- analysis <-> filter transfer (mg_transfer.f90:154-203) ---- */
+/* ---- analysis <-> filter transfer (mg_transfer.f90:58-203, 339-464) ---- */
 /* The analysis grid: na x ma global points (nm0 x mm0) split over an
    npx x npy PE grid like the filter grid (nx = npx*im, ny = npy*jm), and
-   lm_a levels per column (lm_a = lm or 2 lm - 1). */
+   lm_a >= lm levels per column. scheme: the horizontal interpolation of
+   anal_to_filt / filt_to_anal, 0 lsqr (4 points, halo 3), 1 lin
+   (l_lin_horizontal, 2 points, halo 1), 2 quad (l_quad_horizontal, 3 points,
+   halo 2). vmap: the vertical map for lm_a > lm, 0 the linear one
+   (l_vertical_*_spec2 of the _all2 routines; l_vertical_*_spec of the _all
+   routines with l_lin_vertical), 1 lwq_vertical_*_spec (the _all routines
+   without l_lin_vertical). */
 typedef struct mgbfo_anal {
     int na, ma;
     int npx, npy;
     int lm_a;
+    int scheme;
+    int vmap;
 } mgbfo_anal;
-/* filt_to_anal_all2: boco_2d(.,3,3) halo, lsqr_direct_offset per PE tile
-   (mg_interpolate.f90:634-681, coefficients of lsqr_mg_coef :89-191), then
-   l_vertical_direct_spec2 (:1006-1044). vf: K x nx x ny; wa: Ka x na x ma
-   with Ka = nvcol*lm_a + (K - nvcol*lm). */
+/* filt_to_anal: boco_2d halo, the *_direct_offset interpolation per PE tile
+   (mg_interpolate.f90:634-681, 750-795, 858-903, coefficients of
+   lsqr_mg_coef :89-298), then the vertical map (:511-553, 595-631,
+   1006-1044). vf: K x nx x ny; wa: Ka x na x ma with Ka = nvcol*lm_a +
+   (K - nvcol*lm). With lm_a > lm the columns must cover the stack. */
 void mgbfo_filt_to_anal(const mgbfo_params* P, const mgbfo_anal* A, const double* vf, double* wa);
-/* anal_to_filt_all2: l_vertical_adjoint_spec2 (:963-1003), lsqr_adjoint_offset
-   per PE tile (:684-747) into the halo-padded field, then bocoT_2d(.,3,3). */
+/* anal_to_filt: the adjoint vertical map (:461-508, 556-592, 963-1003), the
+   *_adjoint_offset interpolation per PE tile (:684-747, 798-855, 906-960)
+   into the halo-padded field, then bocoT_2d. */
 void mgbfo_anal_to_filt(const mgbfo_params* P, const mgbfo_anal* A, const double* wa, double* vf);
+/* lwq_vertical_coef (mg_interpolate.f90:301-370): nm analysis levels over im
+   filter levels; c holds 4 weights per analysis level, kref the filter
+   level (1-based) of the second weight. */
+void mgbfo_lwq_coef(int nm, int im, double* c, int* kref);
 
 #ifdef __cplusplus
 }

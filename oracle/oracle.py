@@ -78,38 +78,56 @@ def lib():
         L.mgbfo_num_threads.restype = ctypes.c_int
         L.mgbfo_filt_to_anal.argtypes = [P, ctypes.POINTER(_Anal), dp, dp]
         L.mgbfo_anal_to_filt.argtypes = [P, ctypes.POINTER(_Anal), dp, dp]
+        L.mgbfo_lwq_coef.argtypes = [ctypes.c_int, ctypes.c_int, dp, ip]
         _lib = L
     return _lib
 
 
 class _Anal(ctypes.Structure):
     _fields_ = [("na", ctypes.c_int), ("ma", ctypes.c_int), ("npx", ctypes.c_int),
-                ("npy", ctypes.c_int), ("lm_a", ctypes.c_int)]
+                ("npy", ctypes.c_int), ("lm_a", ctypes.c_int), ("scheme", ctypes.c_int),
+                ("vmap", ctypes.c_int)]
+
+# horizontal schemes of anal_to_filt / filt_to_anal and the vertical maps
+LSQR, LIN, QUAD = 0, 1, 2
+VLIN, VLWQ = 0, 1
 
 
 def _anal_ka(P, lm_a: int) -> int:
     return P.nvcol * lm_a + (P.K - P.nvcol * P.lm)
 
 
-def filt_to_anal(P, na: int, ma: int, npx: int, npy: int, lm_a: int, vf: np.ndarray) -> np.ndarray:
-    """filt_to_anal_all2: generation-1 field (ny, nx, K) -> analysis field
-    (ma, na, Ka) over an npx x npy PE grid."""
+def filt_to_anal(P, na: int, ma: int, npx: int, npy: int, lm_a: int, vf: np.ndarray,
+                 scheme: int = LSQR, vmap: int = VLIN) -> np.ndarray:
+    """filt_to_anal: generation-1 field (ny, nx, K) -> analysis field
+    (ma, na, Ka) over an npx x npy PE grid. scheme: LSQR, LIN
+    (l_lin_horizontal) or QUAD (l_quad_horizontal); vmap: VLIN (the _all2
+    routines, or _all with l_lin_vertical) or VLWQ (_all without it)."""
     vf = _f64(vf)
     assert vf.shape == P.gen_shape(0)
-    A = _Anal(na, ma, npx, npy, lm_a)
+    A = _Anal(na, ma, npx, npy, lm_a, scheme, vmap)
     wa = np.empty((ma, na, _anal_ka(P, lm_a)))
     lib().mgbfo_filt_to_anal(ctypes.byref(P.c()), ctypes.byref(A), _dp(vf), _dp(wa))
     return wa
 
 
-def anal_to_filt(P, na: int, ma: int, npx: int, npy: int, lm_a: int, wa: np.ndarray) -> np.ndarray:
-    """anal_to_filt_all2: the transpose of filt_to_anal."""
+def anal_to_filt(P, na: int, ma: int, npx: int, npy: int, lm_a: int, wa: np.ndarray,
+                 scheme: int = LSQR, vmap: int = VLIN) -> np.ndarray:
+    """anal_to_filt: the transpose of filt_to_anal."""
     wa = _f64(wa)
     assert wa.shape == (ma, na, _anal_ka(P, lm_a))
-    A = _Anal(na, ma, npx, npy, lm_a)
+    A = _Anal(na, ma, npx, npy, lm_a, scheme, vmap)
     vf = np.empty(P.gen_shape(0))
     lib().mgbfo_anal_to_filt(ctypes.byref(P.c()), ctypes.byref(A), _dp(wa), _dp(vf))
     return vf
+
+
+def lwq_coef(lm_a: int, lm: int):
+    """lwq_vertical_coef: (weights (lm_a, 4), kref (lm_a,), 1-based)."""
+    c = np.zeros((lm_a, 4))
+    k = np.zeros(lm_a, dtype=np.int32)
+    lib().mgbfo_lwq_coef(lm_a, lm, _dp(c), k.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return c, k
 
 
 def _dp(a: np.ndarray):

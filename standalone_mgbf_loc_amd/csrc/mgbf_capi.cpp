@@ -18,6 +18,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mgbf.h"
@@ -58,6 +59,13 @@ struct DeviceGuard {
 };
 
 }  // namespace
+
+// Window table of a transfer along the stacked k (launch_kwin).
+struct VMap {
+    const int* s = nullptr;
+    const float* w = nullptr;
+    int width = 0;
+};
 
 struct mgbf_ctx {
     mgbf::Setup s;
@@ -116,12 +124,10 @@ struct mgbf_ctx {
     // analysis <-> filter transfer: separable window tables (direct: filter ->
     // analysis, adjoint: its transpose) and the vertical k windows
     mgbf::XferArgs xf_dir{}, xf_adj{};
-    const int* d_kvd_s = nullptr;
-    const float* d_kvd_w = nullptr;
-    int kvd_w = 0;
-    const int* d_kva_s = nullptr;
-    const float* d_kva_w = nullptr;
-    int kva_w = 0;
+    // vertical transfer of the _all2 (kv) and the _all (kw) routines:
+    // direct (d) and adjoint (a) window tables; null when lm_a = lm or the
+    // routine family cannot run
+    VMap kv_d, kv_a, kw_d, kw_a;
     float* ws_a = nullptr;  // analysis grid x K, when the vertical transfer is not the identity
     float* ws_u = nullptr;  // multigrid-sized
     float* ws_t = nullptr;  // multigrid-sized
@@ -593,39 +599,44 @@ int apply_w_rad3(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipSt
     return rad3_upsweep(c, U, y, st);
 }
 
-// filt_to_anal_all2: analysis field = T_v T_h (generation-1 field), the
-// horizontal 4-point interpolation first, then lm -> lm_a levels.
-int apply_f2a(mgbf_ctx* c, const float* in, float* out, hipStream_t st) {
+// filt_to_anal_all2 / filt_to_anal_all: analysis field = T_v T_h
+// (generation-1 field), the horizontal interpolation first, then lm -> lm_a
+// levels with the routine family's vertical map (vd: its direct table).
+int apply_f2a(mgbf_ctx* c, const VMap& vd, const char* refusal, const float* in, float* out,
+              hipStream_t st) {
     const mgbf_dims& d = c->s.dims;
+    if (refusal) return fail(MGBF_ERR_CONFIG, refusal);
     if (c->s.anal_identity) {
         if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
         return MGBF_OK;
     }
     if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
-    float* mid = c->d_kvd_s ? c->ws_a : out;
+    float* mid = vd.s ? c->ws_a : out;
     mgbf::XferArgs a = c->xf_dir;
     a.in = in;
     a.out = mid;
     HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
-    if (c->d_kvd_s)
-        HIP_TRY(mgbf::launch_kwin(mid, out, c->K, d.km_a_all, (long long)d.na * d.ma, c->d_kvd_s,
-                                  c->d_kvd_w, c->kvd_w, st),
+    if (vd.s)
+        HIP_TRY(mgbf::launch_kwin(mid, out, c->K, d.km_a_all, (long long)d.na * d.ma, vd.s, vd.w, vd.width, st),
                 "vertical transfer launch");
     return MGBF_OK;
 }
 
-// anal_to_filt_all2: the transpose of filt_to_anal_all2.
-int apply_a2f(mgbf_ctx* c, const float* in, float* out, hipStream_t st) {
+// anal_to_filt_all2 / anal_to_filt_all: the transpose of the matching
+// filt_to_anal (va: the adjoint vertical table).
+int apply_a2f(mgbf_ctx* c, const VMap& va, const char* refusal, const float* in, float* out,
+              hipStream_t st) {
     const mgbf_dims& d = c->s.dims;
+    if (refusal) return fail(MGBF_ERR_CONFIG, refusal);
     if (c->s.anal_identity) {
         if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
         return MGBF_OK;
     }
     if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
     const float* src = in;
-    if (c->d_kva_s) {
-        HIP_TRY(mgbf::launch_kwin(in, c->ws_a, d.km_a_all, c->K, (long long)d.na * d.ma, c->d_kva_s,
-                                  c->d_kva_w, c->kva_w, st),
+    if (va.s) {
+        HIP_TRY(mgbf::launch_kwin(in, c->ws_a, d.km_a_all, c->K, (long long)d.na * d.ma, va.s, va.w, va.width,
+                                  st),
                 "vertical transfer launch");
         src = c->ws_a;
     }
@@ -842,7 +853,8 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         bytes += carve_size(sizeof(float) * rytab[g].size()) + carve_size(sizeof(int) * rymeta[g].size());
     }
     {
-        const mgbf::AxisTable* tabs[6] = {&s.ax_d, &s.ay_d, &s.ax_a, &s.ay_a, &s.kv_d, &s.kv_a};
+        const mgbf::AxisTable* tabs[8] = {&s.ax_d, &s.ay_d, &s.ax_a, &s.ay_a,
+                                          &s.kv_d, &s.kv_a, &s.kw_d, &s.kw_a};
         for (const auto* t : tabs)
             bytes += carve_size(sizeof(int) * t->start.size()) + carve_size(sizeof(float) * t->w.size());
     }
@@ -856,34 +868,53 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         delete c;
         return hip_fail(e, "hipMalloc(tables)");
     }
+    // Host staging and device addresses are carved side by side. A table
+    // the size count above missed lands in a spill buffer instead of past
+    // the staging array, and init fails before anything reaches the device.
     char* hp = host.data();
+    char* const hend = host.data() + host.size();
     char* dp = static_cast<char*>(c->d_tables);
+    bool overflow = false;
+    std::vector<std::vector<char>> spill;
+    auto take = [&](auto* tag, size_t count, auto** dev) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        const size_t nb = carve_size(sizeof(T) * count);
+        *dev = carve<T>(dp, count);
+        if (hp + nb > hend) {
+            overflow = true;
+            spill.emplace_back(nb + sizeof(T));
+            return reinterpret_cast<T*>(spill.back().data());
+        }
+        return carve<T>(hp, count);
+    };
     auto put_f = [&](const std::vector<double>& v) -> const float* {
-        float* h = carve<float>(hp, v.size());
-        float* d = carve<float>(dp, v.size());
+        float* d = nullptr;
+        float* h = take((float*)nullptr, v.size(), &d);
         for (size_t i = 0; i < v.size(); ++i) h[i] = (float)v[i];
         return d;
     };
     auto put_i = [&](const std::vector<int>& v) -> const int* {
-        int* h = carve<int>(hp, v.size());
-        int* d = carve<int>(dp, v.size());
+        int* d = nullptr;
+        int* h = take((int*)nullptr, v.size(), &d);
         for (size_t i = 0; i < v.size(); ++i) h[i] = v[i];
         return d;
     };
     for (int b = 0; b < 3; ++b) c->d_band[b] = put_f(*bands[b]);
     {
-        _Float16* h = carve<_Float16>(hp, afrag.size());
-        c->d_afrag = carve<_Float16>(dp, afrag.size());
+        _Float16* da = nullptr;
+        _Float16* h = take((_Float16*)nullptr, afrag.size(), &da);
         for (size_t i = 0; i < afrag.size(); ++i) h[i] = afrag[i];
+        c->d_afrag = da;
     }
-    c->d_dot = carve<double>(dp, 1);
-    carve<double>(hp, 1);
+    take((double*)nullptr, 1, &c->d_dot);
     if (c->rad3) {
         auto put_taps = [&](size_t nt, auto tap, const int4** dt, const float** dw) {
-            int4* ht = carve<int4>(hp, nt);
-            *dt = carve<int4>(dp, nt);
-            float* hw = carve<float>(hp, nt);
-            *dw = carve<float>(dp, nt);
+            int4* d4 = nullptr;
+            float* dwp = nullptr;
+            int4* ht = take((int4*)nullptr, nt, &d4);
+            float* hw = take((float*)nullptr, nt, &dwp);
+            *dt = d4;
+            *dw = dwp;
             for (size_t t = 0; t < nt; ++t) tap(t, ht[t], hw[t]);
         };
         put_taps(s.taps3.size(), [&](size_t t, int4& g, float& w) {
@@ -905,11 +936,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->d_pys[g] = put_i(s.py[g].start);
         c->d_pyw[g] = put_f(s.py[g].w);
         if (c->res_ok[g]) {
-            float* hf = carve<float>(hp, rytab[g].size());
-            float* df = carve<float>(dp, rytab[g].size());
+            float* df = nullptr;
+            int* di = nullptr;
+            float* hf = take((float*)nullptr, rytab[g].size(), &df);
             for (size_t i = 0; i < rytab[g].size(); ++i) hf[i] = rytab[g][i];
-            int* hi = carve<int>(hp, rymeta[g].size());
-            int* di = carve<int>(dp, rymeta[g].size());
+            int* hi = take((int*)nullptr, rymeta[g].size(), &di);
             for (size_t i = 0; i < rymeta[g].size(); ++i) hi[i] = rymeta[g][i];
             c->d_rytab[g] = reinterpret_cast<const float4*>(df);
             c->d_rymeta[g] = reinterpret_cast<const int2*>(di);
@@ -923,20 +954,28 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->xf_adj = mgbf::XferArgs{nullptr, nullptr, c->K, d.na, d.ma, d.nx, d.ny,
                                    put_i(s.ax_a.start), put_f(s.ax_a.w), s.ax_a.width,
                                    put_i(s.ay_a.start), put_f(s.ay_a.w), s.ay_a.width};
-        if (s.kv_d.n_out > 0) {
-            c->d_kvd_s = put_i(s.kv_d.start);
-            c->d_kvd_w = put_f(s.kv_d.w);
-            c->kvd_w = s.kv_d.width;
-            c->d_kva_s = put_i(s.kv_a.start);
-            c->d_kva_w = put_f(s.kv_a.w);
-            c->kva_w = s.kv_a.width;
-        }
+        auto put_v = [&](const mgbf::AxisTable& t, VMap& v) {
+            if (t.n_out == 0) return;
+            v.s = put_i(t.start);
+            v.w = put_f(t.w);
+            v.width = t.width;
+        };
+        put_v(s.kv_d, c->kv_d);
+        put_v(s.kv_a, c->kv_a);
+        put_v(s.kw_d, c->kw_d);
+        put_v(s.kw_a, c->kw_a);
+    }
+    if (overflow) {
+        // a table was not counted above: its device address lies past the
+        // allocation, so nothing may use it
+        mgbf_finalize(c);
+        return fail(MGBF_ERR_HIP, "internal: coefficient tables exceed their allocation");
     }
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->ws_t, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->d_kmax, sizeof(float) * (size_t)c->gm * c->K);
-    if (e == hipSuccess && c->d_kvd_s)
+    if (e == hipSuccess && s.dims.km_a_all != c->K)
         e = hipMalloc(&c->ws_a, sizeof(float) * (size_t)c->K * s.dims.na * s.dims.ma);
     if (e != hipSuccess) {
         int r = hip_fail(e, "device allocation");
@@ -998,8 +1037,10 @@ int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
         case MGBF_OP_ASQRT: return apply_weights(c, in, out, true, st);
         case MGBF_OP_BSQRT: return apply_w(c, in, out, true, st);
         case MGBF_OP_BSQRTT: return apply_wt(c, in, out, true, st);
-        case MGBF_OP_A2F: return apply_a2f(c, in, out, st);
-        case MGBF_OP_F2A: return apply_f2a(c, in, out, st);
+        case MGBF_OP_A2F: return apply_a2f(c, c->kv_a, c->s.all2_refusal, in, out, st);
+        case MGBF_OP_F2A: return apply_f2a(c, c->kv_d, c->s.all2_refusal, in, out, st);
+        case MGBF_OP_A2F_ALL: return apply_a2f(c, c->kw_a, c->s.all_refusal, in, out, st);
+        case MGBF_OP_F2A_ALL: return apply_f2a(c, c->kw_d, c->s.all_refusal, in, out, st);
         default: return fail(MGBF_ERR_ARG, "unknown operation");
     }
 }

@@ -142,30 +142,75 @@ int max_span(int rows, int cols, const std::vector<char>& nz, bool transpose) {
     return wmax;
 }
 
-// Vertical transfer over the stacked k (l_vertical_direct_spec2 /
-// l_vertical_adjoint_spec2, mg_interpolate.f90:963-1044): ncol columns of
-// lm filter levels and lm_a = 2 lm - 1 analysis levels; analysis level n
-// (1-based) is filter level (n+1)/2 for odd n and the mean of levels n/2 and
-// n/2+1 for even n. Tables of the direct map (rows: analysis k) and of its
-// transpose (rows: filter k), windows kept inside each column.
-void vertical_transfer_tables(int ncol, int lm, AxisTable& dir, AxisTable& adj) {
-    const int la = 2 * lm - 1;
-    std::vector<double> V((size_t)la * lm, 0.0);   // one column, la x lm
-    std::vector<char> nz((size_t)la * lm, 0);
-    for (int n = 1; n <= la; ++n) {
-        if (n & 1) {
-            V[(size_t)(n - 1) * lm + (n + 1) / 2 - 1] = 1.0;
-            nz[(size_t)(n - 1) * lm + (n + 1) / 2 - 1] = 1;
-        } else {
-            for (int l = n / 2; l <= n / 2 + 1; ++l) {
-                V[(size_t)(n - 1) * lm + l - 1] = 0.5;
-                nz[(size_t)(n - 1) * lm + l - 1] = 1;
+// One column of the vertical transfer, lm filter levels -> la analysis
+// levels, as a dense la x lm matrix (row n, column k, both 0-based).
+// vmap 0, the linear map (l_vertical_direct_spec2 / l_vertical_direct_spec,
+// mg_interpolate.f90:595-631, 1006-1044): odd analysis level n (1-based)
+// copies filter level (n+1)/2, even n averages levels n/2 and n/2+1. vmap 1,
+// lwq_vertical_direct_spec (:511-553) with the weights of lwq_vertical_coef
+// (:301-370): analysis height y = (n-1)(lm-1)/(la-1) over filter heights
+// 0..lm-1, kref = floor(y)+1, the two quadratic interpolants through
+// kref-1..kref+1 and kref..kref+2 (1-based levels) blended linearly between
+// kref and kref+1; at kref = 1 (and lm-1) the weight of the ghost level
+// below (above) the column folds onto its whole-sample mirror image; the end
+// levels copy.
+std::vector<double> vertical_column(int lm, int la, int vmap) {
+    std::vector<double> V((size_t)la * lm, 0.0);
+    auto at = [&](int n, int k) -> double& { return V[(size_t)(n - 1) * lm + (k - 1)]; };
+    if (vmap == 0) {
+        for (int n = 1; n <= la; ++n) {
+            if (n & 1) {
+                at(n, (n + 1) / 2) = 1.0;
+            } else {
+                at(n, n / 2) += 0.5;
+                at(n, n / 2 + 1) += 0.5;
             }
         }
+        return V;
     }
+    const double dy = 1.0 * (lm - 1) / (la - 1);
+    for (int n = 2; n <= la - 1; ++n) {
+        const double y = (n - 1) * dy;
+        const int kr = (int)(y + 1.0);
+        // nodes at heights kr-2 .. kr+1 (levels kr-1 .. kr+2)
+        double z[4], ql[3], qr[3];
+        for (int t = 0; t < 4; ++t) z[t] = kr - 2 + t;
+        auto lag3 = [&](const double* zz, double* l) {
+            l[0] = (y - zz[1]) * (y - zz[2]) / ((zz[0] - zz[1]) * (zz[0] - zz[2]));
+            l[1] = (y - zz[0]) * (y - zz[2]) / ((zz[1] - zz[0]) * (zz[1] - zz[2]));
+            l[2] = (y - zz[0]) * (y - zz[1]) / ((zz[2] - zz[0]) * (zz[2] - zz[1]));
+        };
+        lag3(z, ql);
+        lag3(z + 1, qr);
+        const double bl = z[2] - y, br = y - z[1];
+        double c[4] = {bl * ql[0], bl * ql[1] + br * qr[0], bl * ql[2] + br * qr[1], br * qr[2]};
+        if (kr == 1) {
+            c[2] += c[0];
+            c[0] = 0.0;
+        }
+        if (kr == lm - 1) {
+            c[1] += c[3];
+            c[3] = 0.0;
+        }
+        for (int t = 0; t < 4; ++t)
+            if (c[t] != 0.0) at(n, kr - 1 + t) += c[t];
+    }
+    at(1, 1) = 1.0;
+    at(la, lm) = 1.0;
+    return V;
+}
+
+// Direct (rows: analysis k) and transpose (rows: filter k) window tables
+// of the vertical transfer tiled over ncol columns, windows inside each
+// column.
+bool vertical_transfer_tables(int ncol, int lm, int la, int vmap, AxisTable& dir, AxisTable& adj) {
+    const std::vector<double> V = vertical_column(lm, la, vmap);
+    std::vector<char> nz(V.size());
+    for (size_t e = 0; e < V.size(); ++e) nz[e] = V[e] != 0.0;
     AxisTable d1, a1;
-    pack_rows(la, lm, V, nz, false, 2, d1);
-    pack_rows(lm, la, V, nz, true, 3, a1);
+    if (!pack_rows(la, lm, V, nz, false, max_span(la, lm, nz, false), d1) ||
+        !pack_rows(lm, la, V, nz, true, max_span(lm, la, nz, true), a1))
+        return false;
     auto tile = [&](const AxisTable& one, int nrow, int ncolk, AxisTable& t) {
         t.n_out = ncol * nrow;
         t.width = one.width;
@@ -180,56 +225,76 @@ void vertical_transfer_tables(int ncol, int lm, AxisTable& dir, AxisTable& adj) 
     };
     tile(d1, la, lm, dir);
     tile(a1, lm, la, adj);
+    return true;
 }
 
 }  // namespace
 
-bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, std::vector<double>& T) {
+bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, int scheme, std::vector<double>& T) {
     // Per PE tile the reference places the analysis points at cell centres
     // xa(n) = dxa/2 + dxa (n-1) and the filter points at xf(i) = dxf/2 +
     // dxf (i-1), with the tile length equal to nloc_a (mg_parameter.f90:
-    // 518-541), so dxa = 1 and dxf = nloc_a/nloc_f. Analysis point n takes
-    // the filter points iref..iref+3, iref = (first i with xa(n) < xf(i)) - 2
-    // (searched over 1-ib .. im+ib-1, ib = 6), with the 4-point weights of
-    // lsqr_mg_coef (mg_interpolate.f90:156-191): the quadratic Lagrange
-    // interpolants through (x1,x2,x3) and (x2,x3,x4), blended linearly
-    // between x2 and x3. Filter indices outside the global domain take the
-    // half-sample mirror (boco_2d); inside, tiles join up seamlessly.
+    // 518-541), so dxa = 1 and dxf = nloc_a/nloc_f. With i0 the first i in
+    // 1-ib .. im+ib-1 (ib = 6) with xa(n) < xf(i) (lsqr_mg_coef,
+    // mg_interpolate.f90:129-149), analysis point n takes
+    //   lsqr: filter points i0-2 .. i0+1, the quadratic Lagrange interpolants
+    //         through the first and the last three blended linearly between
+    //         the middle two (:156-191);
+    //   lin:  points i0-1, i0, linear weights (:269-296);
+    //   quad: points i0-1 .. i0+1, quadratic Lagrange weights (:231-267).
+    // Filter indices outside the global domain take the half-sample mirror
+    // (boco_2d); inside, tiles join up seamlessly.
     const int na = npe * nloc_a, nf = npe * nloc_f;
     T.assign((size_t)na * nf, 0.0);
     const double len = 1.0 * nloc_a;
     const double dxa = len / nloc_a, dxf = len / nloc_f;
     const double xa0 = 0.5 * dxa, xf0 = 0.5 * dxf;
     const int ib = 6;
+    // Lagrange basis of the nodes z[0..2] at x
+    auto lag3 = [](double x, const double* z, double* l) {
+        l[0] = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
+        l[1] = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
+        l[2] = (x - z[0]) * (x - z[1]) / ((z[2] - z[0]) * (z[2] - z[1]));
+    };
     for (int n = 1; n <= nloc_a; ++n) {
         const double x = xa0 + dxa * (n - 1);
-        int iref = 0;
-        bool found = false;
-        for (int i = 1 - ib; i <= nloc_f + ib - 1; ++i)
-            if (x < xf0 + dxf * (i - 1)) {
-                iref = i - 2;
-                found = true;
-                break;
-            }
-        if (!found) return false;
-        double xs[4];
-        for (int t = 0; t < 4; ++t) xs[t] = xf0 + dxf * (iref + t - 1);
-        // quadratic Lagrange basis on 3 nodes at x
-        auto lag3 = [&](const double* z, double* l) {
-            l[0] = (x - z[1]) * (x - z[2]) / ((z[0] - z[1]) * (z[0] - z[2]));
-            l[1] = (x - z[0]) * (x - z[2]) / ((z[1] - z[0]) * (z[1] - z[2]));
-            l[2] = (x - z[0]) * (x - z[1]) / ((z[2] - z[0]) * (z[2] - z[1]));
-        };
-        double ql[3], qr[3];
-        lag3(xs, ql);
-        lag3(xs + 1, qr);
-        const double bl = (xs[2] - x) / (xs[2] - xs[1]);   // weight of the left interpolant
-        const double br = (x - xs[1]) / (xs[2] - xs[1]);
-        const double c[4] = {bl * ql[0], bl * ql[1] + br * qr[0], bl * ql[2] + br * qr[1], br * qr[2]};
+        int i0 = 0;
+        for (int i = 1 - ib; i <= nloc_f + ib - 1 && !i0; ++i)
+            if (x < xf0 + dxf * (i - 1)) i0 = i;
+        if (!i0) return false;
+        int first = 0, width = 0;
+        double c[4] = {};
+        if (scheme == 1) {
+            first = i0 - 1;
+            width = 2;
+            const double z0 = xf0 + dxf * (first - 1), z1 = xf0 + dxf * first;
+            c[0] = (z1 - x) / (z1 - z0);
+            c[1] = (x - z0) / (z1 - z0);
+        } else if (scheme == 2) {
+            first = i0 - 1;
+            width = 3;
+            double z[3];
+            for (int t = 0; t < 3; ++t) z[t] = xf0 + dxf * (first + t - 1);
+            lag3(x, z, c);
+        } else {
+            first = i0 - 2;
+            width = 4;
+            double z[4];
+            for (int t = 0; t < 4; ++t) z[t] = xf0 + dxf * (first + t - 1);
+            double ql[3], qr[3];
+            lag3(x, z, ql);
+            lag3(x, z + 1, qr);
+            const double bl = (z[2] - x) / (z[2] - z[1]);   // weight of the left interpolant
+            const double br = (x - z[1]) / (z[2] - z[1]);
+            c[0] = bl * ql[0];
+            c[1] = bl * ql[1] + br * qr[0];
+            c[2] = bl * ql[2] + br * qr[1];
+            c[3] = br * qr[2];
+        }
         for (int p = 0; p < npe; ++p) {
             const int row = p * nloc_a + n - 1;
-            for (int t = 0; t < 4; ++t) {
-                int I = p * nloc_f + iref + t;   // 1-based global filter index
+            for (int t = 0; t < width; ++t) {
+                int I = p * nloc_f + first + t;   // 1-based global filter index
                 if (I < 1) I = 1 - I;
                 if (I > nf) I = 2 * nf + 1 - I;
                 if (I < 1 || I > nf) return false;
@@ -692,6 +757,35 @@ bool fast_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     return true;
 }
 
+const char* transfer_all2_refusal(const mgbf_config& c) {
+    if (c.lm_a == c.lm) return nullptr;
+    // l_vertical_*_spec2 (mg_interpolate.f90:963-1044) run over km3*n_ens
+    // columns of lm_a levels: only the 3-D variables, with every analysis
+    // level written and every filter level inside the column
+    if (c.km2 > 0)
+        return "anal_to_filt_all2 / filt_to_anal_all2 with lm_a > lm need km2 = 0 (the vertical map "
+               "covers the 3-D variables only)";
+    if (c.lm_a % 2 == 0 || c.lm_a > 2 * c.lm - 1)
+        return "anal_to_filt_all2 / filt_to_anal_all2 with lm_a > lm need an odd lm_a <= 2 lm - 1 (the "
+               "linear vertical map writes odd levels from level (n+1)/2)";
+    return nullptr;
+}
+
+const char* transfer_all_refusal(const mgbf_config& c) {
+    // S2C_ens / C2S_ens (mg_transfer.f90:269-336) move the 3-D variables
+    // only: the 2-D ones would keep whatever the target array held
+    if (c.km2 > 0)
+        return "anal_to_filt_all / filt_to_anal_all need km2 = 0 (S2C_ens and C2S_ens carry the 3-D "
+               "variables only)";
+    if (c.lm_a == c.lm) return nullptr;
+    if (c.l_lin_vertical && (c.lm_a % 2 == 0 || c.lm_a > 2 * c.lm - 1))
+        return "anal_to_filt_all / filt_to_anal_all with l_lin_vertical need an odd lm_a <= 2 lm - 1";
+    if (!c.l_lin_vertical && c.lm < 3)
+        return "anal_to_filt_all / filt_to_anal_all with the lwq vertical map need lm >= 3 (its edge "
+               "branches touch three levels)";
+    return nullptr;
+}
+
 int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* err) {
     d = mgbf_dims{};
     if (c.nxPE < 1 || c.nyPE < 1 || c.im_filt < 1 || c.jm_filt < 1 || c.lm < 1 || c.n_ens < 1 ||
@@ -745,25 +839,22 @@ int derive(const mgbf_config& c, mgbf_dims& d, int* nx, int* ny, std::string* er
         *err = "mm0 is not divisible by nym";
         return MGBF_ERR_CONFIG;
     }
-    // analysis <-> filter transfer (mg_transfer.f90:154-203, lsqr_mg_coef
-    // mg_interpolate.f90:89-298, l_vertical_*_spec2 :963-1044)
+    // analysis <-> filter transfer (mg_transfer.f90:58-203, 339-464,
+    // lsqr_mg_coef mg_interpolate.f90:89-298): every horizontal scheme fits
+    // its halo when the filter grid is no finer than the analysis grid
     if (nm < c.im_filt || mm < c.jm_filt) {
         *err = "filter grid finer than the analysis grid (im_filt > nm0/nxPE or jm_filt > mm0/nyPE): "
                "the transfer assumes im <= nm and jm <= mm";
         return MGBF_ERR_UNSUPPORTED;
     }
-    if ((nm != c.im_filt || mm != c.jm_filt) && (c.l_lin_horizontal || c.l_quad_horizontal)) {
-        *err = "l_lin_horizontal / l_quad_horizontal: only the 4-point (lsqr) horizontal transfer is built";
-        return MGBF_ERR_UNSUPPORTED;
+    if (c.lm_a < c.lm) {
+        *err = fmt("lm_a=%d < lm=%d: the analysis columns must hold at least the filter levels", c.lm_a, c.lm);
+        return MGBF_ERR_CONFIG;
     }
-    if (c.lm_a != c.lm && c.lm_a != 2 * c.lm - 1) {
-        *err = fmt("lm_a=%d: the vertical transfer maps lm_a = 2 lm - 1 = %d levels (or lm_a = lm)",
-                   c.lm_a, 2 * c.lm - 1);
-        return MGBF_ERR_UNSUPPORTED;
-    }
-    if (c.lm_a != c.lm && (c.km2 > 0 || c.lm < 2)) {
-        // l_vertical_*_spec2 run over km3*n_ens columns of lm_a levels only
-        *err = "lm_a > lm needs km2 = 0 and lm >= 2 (the vertical transfer covers the 3-D variables)";
+    // the driver calls the _all2 routines with l_new_map and the _all ones
+    // without (RBETA_TEST.f90:48-52, 92-96)
+    if (const char* why = c.l_new_map ? transfer_all2_refusal(c) : transfer_all_refusal(c)) {
+        *err = why;
         return MGBF_ERR_CONFIG;
     }
     // generations (mg_parameter.f90:322-328)
@@ -964,6 +1055,8 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     // analysis <-> filter transfer tables
     const int nm = cfg.nm0 / cfg.nxPE, mm = cfg.mm0 / cfg.nyPE;
     s.anal_identity = nm == cfg.im_filt && mm == cfg.jm_filt && cfg.lm_a == cfg.lm;
+    // l_lin_horizontal wins over l_quad_horizontal (mg_transfer.f90:368-382)
+    const int scheme = cfg.l_lin_horizontal ? 1 : cfg.l_quad_horizontal ? 2 : 0;
     if (!s.anal_identity) {
         const int axes[2][3] = {{cfg.nxPE, nm, cfg.im_filt}, {cfg.nyPE, mm, cfg.jm_filt}};
         AxisTable* dirs[2] = {&s.ax_d, &s.ay_d};
@@ -971,8 +1064,8 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
         for (int a = 0; a < 2; ++a) {
             const int na = axes[a][0] * axes[a][1], nf = axes[a][0] * axes[a][2];
             std::vector<double> T;
-            if (!anal_axis_matrix(axes[a][0], axes[a][1], axes[a][2], T)) {
-                *err = "analysis-filter interpolation: stencil outside the 3-point halo";
+            if (!anal_axis_matrix(axes[a][0], axes[a][1], axes[a][2], scheme, T)) {
+                *err = "analysis-filter interpolation: stencil outside the mirrored domain";
                 return MGBF_ERR_CONFIG;
             }
             std::vector<char> nz(T.size());
@@ -984,8 +1077,19 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
                 return MGBF_ERR_UNSUPPORTED;
             }
         }
-        if (cfg.lm_a != cfg.lm)
-            vertical_transfer_tables(s.dims.km3_all, cfg.lm, s.kv_d, s.kv_a);
+    }
+    s.all2_refusal = transfer_all2_refusal(cfg);
+    s.all_refusal = transfer_all_refusal(cfg);
+    if (cfg.lm_a != cfg.lm) {
+        bool ok = true;
+        if (!s.all2_refusal) ok = vertical_transfer_tables(s.dims.km3_all, cfg.lm, cfg.lm_a, 0, s.kv_d, s.kv_a);
+        if (ok && !s.all_refusal)
+            ok = vertical_transfer_tables(s.dims.km3_all, cfg.lm, cfg.lm_a, cfg.l_lin_vertical ? 0 : 1, s.kw_d,
+                                          s.kw_a);
+        if (!ok) {
+            *err = "vertical transfer tables";
+            return MGBF_ERR_CONFIG;
+        }
     }
     return MGBF_OK;
 }

@@ -52,16 +52,25 @@ struct Setup {
     AxisTable rx[kMaxGen], ry[kMaxGen];   // restriction g -> g+1 (coarse rows)
     AxisTable px[kMaxGen], py[kMaxGen];   // prolongation g+1 -> g (fine rows)
     double wconst[kMaxGen] = {};
-    // analysis <-> filter transfer (anal_to_filt_all2 / filt_to_anal_all2):
-    // direct = filter -> analysis, adjoint = its transpose
+    // analysis <-> filter transfer: direct = filter -> analysis, adjoint =
+    // its transpose; the horizontal tables serve both routine families
     bool anal_identity = true;         // nm = im, mm = jm and lm_a = lm
     AxisTable ax_d, ax_a, ay_d, ay_a;  // horizontal, per axis
-    AxisTable kv_d, kv_a;              // vertical over the stacked k (lm_a = 2 lm - 1)
+    AxisTable kv_d, kv_a;              // vertical of the _all2 routines (lm_a > lm)
+    AxisTable kw_d, kw_a;              // vertical of the _all routines (lm_a > lm)
+    const char* all2_refusal = nullptr;  // why the _all2 routines cannot run, or null
+    const char* all_refusal = nullptr;   // why the _all routines cannot run, or null
 };
 
 // Dense 1-D analysis <- filter interpolation matrix of one axis over the
 // whole domain (na = npe*nloc_a rows, nf = npe*nloc_f columns, row-major).
-bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, std::vector<double>& T);
+// scheme: 0 lsqr (4 points), 1 lin (l_lin_horizontal), 2 quad
+// (l_quad_horizontal).
+bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, int scheme, std::vector<double>& T);
+// Why anal_to_filt_all2 / filt_to_anal_all2 (resp. the _all routines) cannot
+// run with this configuration, or null when they can.
+const char* transfer_all2_refusal(const mgbf_config& cfg);
+const char* transfer_all_refusal(const mgbf_config& cfg);
 
 // init_mg_parameter-style validation and derived sizes. Returns MGBF_OK or
 // an error status with a message in *err.

@@ -184,11 +184,13 @@ end subroutine mg_finalize
 end module mg_entrymod
 
 !=======================================================================
-! Analysis <-> filter grid (src.mgbf/mg_transfer.f90:154-203): WORKA goes
+! Analysis <-> filter grid (src.mgbf/mg_transfer.f90:58-203): WORKA goes
 ! to the device through an FP32 staging array, and the library maps it to
-! the generation-1 field (anal_to_filt_all2: the transpose of the 4-point
-! horizontal and 2:1 vertical interpolation) and back (filt_to_anal_all2).
-! With nm = im, mm = jm and lm_a = lm both maps are copies.
+! the generation-1 field (anal_to_filt_all2 / anal_to_filt_all: the
+! transpose of the horizontal and vertical interpolation) and back
+! (filt_to_anal_all2 / filt_to_anal_all). The _all2 routines map the levels
+! linearly, the _all routines linearly (l_lin_vertical) or with the lwq
+! weights. With nm = im, mm = jm and lm_a = lm every map is a copy.
 !=======================================================================
 module mg_transfer
 use iso_c_binding
@@ -201,26 +203,36 @@ public :: anal_to_filt_all, filt_to_anal_all, anal_to_filt_all2, filt_to_anal_al
 
 contains
 
-subroutine anal_to_filt_all2
+subroutine to_filter(op)
+  integer(c_int), intent(in) :: op
   stage4 = real(WORKA, c_float)
   if (hipMemcpy(d_worka, c_loc(stage4), worka_bytes, hipMemcpyHostToDevice) /= 0) &
     stop 'anal_to_filt: hipMemcpy failed'
-  call mgbf_check(mgbf_apply(mgbf_ctx, MGBF_OP_A2F, d_worka, d_vall, c_null_ptr), 'anal_to_filt')
-end subroutine anal_to_filt_all2
+  call mgbf_check(mgbf_apply(mgbf_ctx, op, d_worka, d_vall, c_null_ptr), 'anal_to_filt')
+end subroutine to_filter
 
-subroutine filt_to_anal_all2
-  call mgbf_check(mgbf_apply(mgbf_ctx, MGBF_OP_F2A, d_vall, d_worka, c_null_ptr), 'filt_to_anal')
+subroutine to_analysis(op)
+  integer(c_int), intent(in) :: op
+  call mgbf_check(mgbf_apply(mgbf_ctx, op, d_vall, d_worka, c_null_ptr), 'filt_to_anal')
   if (hipMemcpy(c_loc(stage4), d_worka, worka_bytes, hipMemcpyDeviceToHost) /= 0) &
     stop 'filt_to_anal: hipMemcpy failed'
   WORKA = real(stage4, r_kind)
+end subroutine to_analysis
+
+subroutine anal_to_filt_all2
+  call to_filter(MGBF_OP_A2F)
+end subroutine anal_to_filt_all2
+
+subroutine filt_to_anal_all2
+  call to_analysis(MGBF_OP_F2A)
 end subroutine filt_to_anal_all2
 
 subroutine anal_to_filt_all
-  call anal_to_filt_all2
+  call to_filter(MGBF_OP_A2F_ALL)
 end subroutine anal_to_filt_all
 
 subroutine filt_to_anal_all
-  call filt_to_anal_all2
+  call to_analysis(MGBF_OP_F2A_ALL)
 end subroutine filt_to_anal_all
 
 end module mg_transfer

@@ -10,7 +10,8 @@ integer(c_int), parameter :: MGBF_OK = 0
 integer(c_int), parameter :: MGBF_OP_WAWT = 0, MGBF_OP_WT = 1, MGBF_OP_W = 2, &
                              MGBF_OP_A = 3, MGBF_OP_ASQRT = 4,              &
                              MGBF_OP_BSQRT = 5, MGBF_OP_BSQRTT = 6,         &
-                             MGBF_OP_A2F = 7, MGBF_OP_F2A = 8
+                             MGBF_OP_A2F = 7, MGBF_OP_F2A = 8, &
+                             MGBF_OP_A2F_ALL = 9, MGBF_OP_F2A_ALL = 10
 integer(c_int), parameter :: MGBF_NTIMERS = 6
 integer(c_int), parameter :: hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2
 

@@ -23,7 +23,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "lib", "libmgbf.so")
 
 MGBF_OK = 0
-OP_WAWT, OP_WT, OP_W, OP_A, OP_ASQRT, OP_BSQRT, OP_BSQRTT, OP_A2F, OP_F2A = range(9)
+(OP_WAWT, OP_WT, OP_W, OP_A, OP_ASQRT, OP_BSQRT, OP_BSQRTT, OP_A2F, OP_F2A, OP_A2F_ALL,
+ OP_F2A_ALL) = range(11)
 STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_R, STAGE_P, STAGE_A = range(7)
 
 EXPORTED_SYMBOLS = (
@@ -322,9 +323,9 @@ class MGBF:
     # --- operators ---
     def apply(self, op: int, x, out=None, stream=None):
         import torch
-        if op in (OP_WAWT, OP_W, OP_BSQRT, OP_A2F):
+        if op in (OP_WAWT, OP_W, OP_BSQRT, OP_A2F, OP_A2F_ALL):
             oshape = self.field_shape()
-        elif op == OP_F2A:
+        elif op in (OP_F2A, OP_F2A_ALL):
             oshape = self.dims.anal_shape()
         else:
             oshape = (self.dims.mg_size,)
@@ -339,20 +340,31 @@ class MGBF:
 
     def anal_to_filt(self, w, out=None, stream=None):
         """Analysis field (mm0, nm0, km_a_all) -> generation-1 field
-        (anal_to_filt_all2, src.mgbf/mg_transfer.f90:154-176)."""
+        (anal_to_filt_all2, src.mgbf/mg_transfer.f90:154-183)."""
         return self.apply(OP_A2F, w, out, stream)
 
     def filt_to_anal(self, v, out=None, stream=None):
         """Generation-1 field -> analysis field (filt_to_anal_all2,
-        src.mgbf/mg_transfer.f90:179-193)."""
+        src.mgbf/mg_transfer.f90:186-203)."""
         return self.apply(OP_F2A, v, out, stream)
 
+    def anal_to_filt_all(self, w, out=None, stream=None):
+        """anal_to_filt_all (src.mgbf/mg_transfer.f90:58-110): the vertical
+        map of l_lin_vertical (linear) or the lwq one; needs km2 = 0."""
+        return self.apply(OP_A2F_ALL, w, out, stream)
+
+    def filt_to_anal_all(self, v, out=None, stream=None):
+        """filt_to_anal_all (src.mgbf/mg_transfer.f90:113-151)."""
+        return self.apply(OP_F2A_ALL, v, out, stream)
+
     def localize_analysis(self, w, stream=None):
-        """The driver's sequence on an analysis field: anal_to_filt_all2,
-        filtering_procedure, filt_to_anal_all2 (RBETA_TEST.f90:34-126)."""
-        v = self.anal_to_filt(w, stream=stream)
+        """The driver's sequence on an analysis field (RBETA_TEST.f90:34-126):
+        anal_to_filt_all2, filtering_procedure, filt_to_anal_all2 with
+        l_new_map, the _all routines without."""
+        new = self.cfg.l_new_map
+        v = self.apply(OP_A2F if new else OP_A2F_ALL, w, stream=stream)
         self.apply(OP_WAWT, v, v, stream)
-        return self.filt_to_anal(v, stream=stream)
+        return self.apply(OP_F2A if new else OP_F2A_ALL, v, stream=stream)
 
     def stage(self, stage: int, gen: int, x, out=None, stream=None):
         import torch

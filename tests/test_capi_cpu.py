@@ -185,3 +185,23 @@ def test_lin3_procedure_taps_follow_the_oracle_hexad(ampl03, hz):
     assert M.derive(cfg).ntaps3 == int(np.prod([2 * h + 1 for _, _, h, _ in lines[:3]]))
     cfg.mgbf_line = False
     assert M.derive(cfg).ntaps3 == 1
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(lm_a=10), "odd lm_a"),                                       # _all2: even lm_a
+    (dict(lm_a=9), "odd lm_a"),                                        # _all2: lm_a > 2 lm - 1
+    (dict(lm_a=3), "lm_a=3 < lm=4"),
+    (dict(l_new_map=False, km2=2, lm_a=4, mgbf_proc=1), "S2C_ens"),    # _all: 3-D variables only
+    (dict(l_new_map=False, l_lin_vertical=True, lm_a=10), "odd lm_a"),
+])
+def test_analysis_transfer_families_refuse_what_they_cannot_map(kw, msg):
+    cfg = M.Config.for_grid(32, 24, 4, gm=3, h=4, hz=2, nm0=48, mm0=36, **kw)
+    assert msg in str(_err(cfg))
+
+
+def test_lwq_map_needs_three_levels_and_takes_any_lm_a_above_lm():
+    cfg = M.Config.for_grid(32, 24, 2, gm=3, h=4, hz=1, nm0=48, mm0=36, lm_a=3, l_new_map=False)
+    assert "lm >= 3" in str(_err(cfg))
+    for lm_a in (4, 10, 64):
+        cfg = M.Config.for_grid(32, 24, 3, gm=3, h=4, hz=2, nm0=48, mm0=36, lm_a=lm_a, l_new_map=False)
+        assert M.derive(cfg).km_a_all == lm_a

@@ -109,6 +109,36 @@ def test_fortran_driver_on_a_finer_analysis_grid_matches_oracle(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_driver_without_the_new_map_matches_oracle(tmp_path):
+    # l_new_map=.false.: the driver calls anal_to_filt_all / filt_to_anal_all
+    # (RBETA_TEST.f90:48-52, 92-96), here with the linear horizontal
+    # interpolation and the lwq vertical map onto 12 analysis levels
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from oracle import oracle as O
+    _need(DRIVER)
+    cfg = M.Config.for_grid(64, 48, 8, n_ens=2, gm=3, h=4, hz=2, weights=[1.0, 0.5, 0.25],
+                            nm0=96, mm0=72, lm_a=12, l_new_map=False, l_lin_horizontal=True)
+    d = M.derive(cfg)
+    write_namelist(cfg, str(tmp_path / "mgbeta.nml"))
+    x = np.random.default_rng(7).standard_normal(d.anal_shape())
+    x.astype("<f8").tofile(tmp_path / "in.bin")
+    out = subprocess.run([DRIVER, "mgbeta.nml", "in.bin", "out.bin"], cwd=tmp_path,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    y = np.fromfile(tmp_path / "out.bin", dtype="<f8").reshape(d.anal_shape())
+    P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
+                       hz=2, p=2, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                       weights=[1.0, 0.5, 0.25])
+    args = (P, 96, 72, cfg.nxPE, cfg.nyPE, 12)
+    kw = dict(scheme=O.LIN, vmap=O.VLWQ)
+    xin = x.astype(np.float32).astype(np.float64)
+    ref = O.filt_to_anal(*args, O.wawt(P, O.anal_to_filt(*args, xin, **kw)), **kw)
+    assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) <= 1e-5
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("proc", [4, 7])
 def test_fortran_driver_runs_the_3d_and_line_procedures(tmp_path, proc):
     # filtering_procedure(4) (filtering_rad3) and (7) (filtering_lin2_ens

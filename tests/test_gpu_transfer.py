@@ -1,8 +1,9 @@
-"""Analysis <-> filter transfer on the GPU (anal_to_filt_all2 /
-filt_to_anal_all2, src.mgbf/mg_transfer.f90:154-203) against the FP64
-oracle: the 4-point horizontal interpolation between a coarser filter grid
-and the analysis grid, the 2:1 vertical map (lm_a = 2 lm - 1), their
-transposes, and the driver's whole sequence on an analysis field."""
+"""Analysis <-> filter transfer on the GPU against the FP64 oracle
+(src.mgbf/mg_transfer.f90:58-464): the 4-point, linear and quadratic
+horizontal interpolations between a coarser filter grid and the analysis
+grid, the linear vertical map of the _all2 routines (odd lm_a <= 2 lm - 1)
+and the linear or lwq map of the _all routines, their transposes, and the
+driver's whole sequence on an analysis field."""
 import numpy as np
 import pytest
 
@@ -25,9 +26,9 @@ def relerr(y, ref):
     return float(np.max(np.abs(y - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
-def setup(nx, ny, lm, n_ens, na, ma, lm_a, gm=3, h=4):
+def setup(nx, ny, lm, n_ens, na, ma, lm_a, gm=3, h=4, **kw):
     cfg = M.Config.for_grid(nx, ny, lm, n_ens=n_ens, gm=gm, h=h, hz=2, weights=[1.0, 0.5, 0.25],
-                            nm0=na, mm0=ma, lm_a=lm_a)
+                            nm0=na, mm0=ma, lm_a=lm_a, **kw)
     op = M.MGBF(cfg)
     d = op.dims
     P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=lm, nvcol=d.km3_all, hx=h, hy=h, hz=2, p=2,
@@ -86,5 +87,71 @@ def test_matching_grids_make_the_transfer_a_copy():
         x = torch.randn(op.field_shape(), device=DEV)
         assert torch.equal(op.filt_to_anal(x), x)
         assert torch.equal(op.anal_to_filt(x), x)
+    finally:
+        op.close()
+
+
+# horizontal scheme flags, and the routine family with its vertical map
+SCHEMES = {O.LSQR: {}, O.LIN: {"l_lin_horizontal": True}, O.QUAD: {"l_quad_horizontal": True}}
+FAMILIES = {"all2": (M.OP_F2A, M.OP_A2F, O.VLIN, {}),
+            "all_lin": (M.OP_F2A_ALL, M.OP_A2F_ALL, O.VLIN, {"l_new_map": False, "l_lin_vertical": True}),
+            "all_lwq": (M.OP_F2A_ALL, M.OP_A2F_ALL, O.VLWQ, {"l_new_map": False})}
+
+
+@pytest.mark.parametrize("scheme", [O.LSQR, O.LIN, O.QUAD])
+@pytest.mark.parametrize("family,grid", [
+    ("all2", (32, 24, 4, 2, 48, 36, 7)), ("all2", (32, 32, 5, 1, 64, 64, 7)),
+    ("all_lin", (32, 24, 4, 2, 48, 36, 7)), ("all_lwq", (32, 24, 4, 2, 48, 36, 7)),
+    ("all_lwq", (32, 32, 5, 1, 64, 64, 12)), ("all_lwq", (48, 32, 3, 3, 72, 48, 3))])
+def test_every_transfer_variant_matches_oracle(scheme, family, grid):
+    nx, ny, lm, n_ens, na, ma, lm_a = grid
+    f2a, a2f, vmap, flags = FAMILIES[family]
+    print(f"transfer scheme={scheme} family={family} grid={grid}", flush=True)
+    cfg, op, P = setup(nx, ny, lm, n_ens, na, ma, lm_a, **SCHEMES[scheme], **flags)
+    try:
+        d = op.dims
+        rng = np.random.default_rng(scheme + 10 * lm_a + na)
+        vf = rng.standard_normal(op.field_shape()).astype(np.float32)
+        wa = rng.standard_normal(d.anal_shape()).astype(np.float32)
+        vd, wd = torch.from_numpy(vf).to(DEV), torch.from_numpy(wa).to(DEV)
+        a = op.apply(f2a, vd)
+        b = op.apply(a2f, wd)
+        torch.cuda.synchronize()
+        args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
+        assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
+        assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
+        lhs, rhs = op.dot(a, wd), op.dot(vd, b)
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs))
+    finally:
+        op.close()
+
+
+def test_driver_sequence_without_the_new_map():
+    # l_new_map=.false.: anal_to_filt_all -> filtering_procedure ->
+    # filt_to_anal_all with the quadratic horizontal and the lwq vertical map
+    cfg, op, P = setup(32, 24, 4, 2, 48, 36, 10, l_new_map=False, l_quad_horizontal=True)
+    try:
+        w = np.random.default_rng(22).standard_normal(op.dims.anal_shape()).astype(np.float32)
+        y = op.localize_analysis(torch.from_numpy(w).to(DEV))
+        torch.cuda.synchronize()
+        args = (P, 48, 36, cfg.nxPE, cfg.nyPE, 10)
+        kw = dict(scheme=O.QUAD, vmap=O.VLWQ)
+        ref = O.filt_to_anal(*args, O.wawt(P, O.anal_to_filt(*args, w.astype(np.float64), **kw)), **kw)
+        assert relerr(y, ref) <= TOL
+    finally:
+        op.close()
+
+
+def test_the_other_routine_family_refuses_what_it_cannot_map():
+    # lm_a = 10 is even: the _all2 routines' linear map would leave the top
+    # analysis level unwritten, so with l_new_map=.false. the context builds
+    # and only the _all routines run
+    cfg, op, P = setup(32, 24, 4, 2, 48, 36, 10, l_new_map=False)
+    try:
+        w = torch.zeros(op.dims.anal_shape(), device=DEV)
+        with pytest.raises(M.MGBFError, match="odd lm_a"):
+            op.anal_to_filt(w)
+        op.anal_to_filt_all(w)
+        torch.cuda.synchronize()
     finally:
         op.close()

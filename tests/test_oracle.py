@@ -305,6 +305,147 @@ def test_transfer_tiles_join_up_like_one_domain():
     assert np.max(np.abs(one - tiled)) < 1e-12
 
 
+def _axis_matrix(scheme, nf, na):
+    # the 1-D interpolation matrix of one axis (the other axis matching, so
+    # exactly the identity), read off unit impulses
+    P = _tparams(nf, 4, 1, 0, K=1)
+    T = np.zeros((na, nf))
+    for i in range(nf):
+        v = np.zeros(P.gen_shape(0))
+        v[:, i, 0] = 1.0
+        a = O.filt_to_anal(P, na, 4, 1, 1, 1, v, scheme=scheme)
+        assert np.all(a[1:] == a[0])
+        T[:, i] = a[0, :, 0]
+    return T
+
+
+def test_linear_and_quadratic_horizontal_weights_at_ratio_two():
+    # analysis points at n - 1/2, filter points at 2 i - 1: the linear
+    # weights alternate (3/4, 1/4) and (1/4, 3/4); the quadratic ones through
+    # the filter points at 1, 3, 5 evaluated at 1.5 are (21, 14, -3)/32. The
+    # first analysis point (at 0.5) sits between the mirrored point at -1 and
+    # the point at 1: weights (1/4, 3/4) on the same value.
+    L = _axis_matrix(O.LIN, 8, 16)
+    assert np.array_equal(L[0], np.eye(8)[0])
+    assert np.array_equal(L[1, :2], [0.75, 0.25]) and np.array_equal(L[2, :2], [0.25, 0.75])
+    assert np.array_equal(L[15], np.eye(8)[7])
+    Q = _axis_matrix(O.QUAD, 8, 16)
+    assert np.allclose(Q[1, :3], np.array([21, 14, -3]) / 32, atol=1e-15)
+    assert np.allclose(Q.sum(axis=1), 1.0, atol=1e-14)
+    # each analysis row has 2 (linear) and at most 3 (quadratic) neighbours
+    assert max(np.count_nonzero(r) for r in L) == 2 and max(np.count_nonzero(r) for r in Q) == 3
+
+
+@pytest.mark.parametrize("scheme,deg", [(O.LIN, 1), (O.QUAD, 2)])
+def test_horizontal_schemes_reproduce_their_polynomials_inside(scheme, deg):
+    # linear interpolation is exact for bilinear fields, the 3-point Lagrange
+    # one for biquadratic fields, away from the mirrored edges; a product
+    # term x^deg y^deg checks that the two axes are applied as a product
+    P = _tparams(32, 24, 1, 0, K=1)
+    xf = (np.arange(32) + 0.5) * 1.5
+    yf = (np.arange(24) + 0.5) * 1.5
+    f = lambda x, y: 1 + 0.3 * x - 0.2 * y + 0.01 * x ** deg * y ** deg + 0.02 * (x ** deg - y ** deg)
+    XF, YF = np.meshgrid(xf, yf)
+    a = O.filt_to_anal(P, 48, 36, 2, 2, 1, f(XF, YF)[:, :, None], scheme=scheme)[:, :, 0]
+    XA, YA = np.meshgrid(np.arange(48) + 0.5, np.arange(36) + 0.5)
+    err = np.abs(a - f(XA, YA))
+    assert np.max(err[3:-3, 3:-3]) < 1e-10 * np.max(np.abs(a))
+    # one degree more is not reproduced
+    g = lambda x, y: x ** (deg + 1)
+    b = O.filt_to_anal(P, 48, 36, 2, 2, 1, g(XF, YF)[:, :, None], scheme=scheme)[:, :, 0]
+    assert np.max(np.abs(b - g(XA, YA))[3:-3, 3:-3]) > 1e-3
+    c = O.filt_to_anal(P, 48, 36, 2, 2, 1, np.full(P.gen_shape(0), 2.5), scheme=scheme)
+    assert np.max(np.abs(c - 2.5)) < 1e-13
+
+
+def _lagrange_blend(y, nodes):
+    # the lwq weights rebuilt from Vandermonde solves: the quadratic
+    # interpolants through nodes 0..2 and 1..3 (the weights that reproduce
+    # 1, y, y^2, in coordinates centred on node 1), blended linearly between
+    # nodes 1 and 2
+    def quad(ns):
+        z = np.asarray(ns) - nodes[1]
+        return np.linalg.solve(np.vander(z, 3, increasing=True).T, (y - nodes[1]) ** np.arange(3))
+    l, r = quad(nodes[:3]), quad(nodes[1:])
+    b = (y - nodes[1]) / (nodes[2] - nodes[1])
+    return np.array([(1 - b) * l[0], (1 - b) * l[1] + b * r[0], (1 - b) * l[2] + b * r[1], b * r[2]])
+
+
+@pytest.mark.parametrize("la,lm", [(65, 33), (13, 5), (10, 4), (7, 3)])
+def test_lwq_vertical_coefficients(la, lm):
+    # lwq_vertical_coef: analysis level n at height (n-1)(lm-1)/(la-1) on
+    # filter heights 0..lm-1; inside, the blend of two quadratic
+    # interpolants; at kref = 1 and lm-1 the weight of the mirrored ghost
+    # level folds onto its image; the end levels copy
+    c, k = O.lwq_coef(la, lm)
+    assert k[0] == 1 and k[-1] == lm
+    assert np.array_equal(c[0], [0, 1, 0, 0]) and np.array_equal(c[-1], [0, 1, 0, 0])
+    for n in range(2, la):
+        y = (n - 1) * (lm - 1) / (la - 1)
+        kr = int(np.floor(y)) + 1
+        assert k[n - 1] == kr
+        ref = _lagrange_blend(y, np.arange(kr - 2, kr + 2, dtype=float))
+        if kr == 1:
+            ref[2] += ref[0]
+            ref[0] = 0.0
+        if kr == lm - 1:
+            ref[1] += ref[3]
+            ref[3] = 0.0
+        assert np.allclose(c[n - 1], ref, atol=1e-13), n
+        assert abs(c[n - 1].sum() - 1.0) < 1e-13
+
+
+def test_lwq_vertical_map_reproduces_quadratics_and_mirrors_at_the_ends():
+    # on matching horizontal grids the transfer is the column map alone:
+    # quadratics in the level height are kept wherever the stencil stays
+    # inside the column, and quadratics even about an end level are kept at
+    # that end (the whole-sample mirror of the fold)
+    lm, la, nv = 9, 25, 2
+    P = _tparams(8, 8, lm, nv)
+    h = np.arange(lm, dtype=float)
+    ya = np.arange(la) * (lm - 1) / (la - 1)
+    kref = np.floor(ya).astype(int) + 1
+    inside = (kref >= 2) & (kref <= lm - 2)
+    for q, ok in [(lambda z: 1 + 0.5 * z - 0.07 * z * z, inside),
+                  (lambda z: 2 - 0.3 * z * z, kref <= lm - 2),
+                  (lambda z: 1 + 0.2 * (z - lm + 1) ** 2, kref >= 2)]:
+        f = np.broadcast_to(np.tile(q(h), nv), P.gen_shape(0)).copy()
+        a = O.filt_to_anal(P, 8, 8, 1, 1, la, f, vmap=O.VLWQ)
+        for e in range(nv):
+            col = a[3, 5, la * e: la * (e + 1)]
+            assert np.max(np.abs(col - q(ya))[ok]) < 1e-12
+    # the end levels copy whatever the column
+    g = np.random.default_rng(15).standard_normal(P.gen_shape(0))
+    a = O.filt_to_anal(P, 8, 8, 1, 1, la, g, vmap=O.VLWQ)
+    assert np.array_equal(a[:, :, 0], g[:, :, 0]) and np.array_equal(a[:, :, la - 1], g[:, :, lm - 1])
+
+
+def test_linear_vertical_map_with_fewer_than_two_to_one_levels():
+    # l_vertical_direct_spec with an odd lm_a < 2 lm - 1: the same
+    # odd-copy / even-average rule; filter levels above (lm_a+1)/2 receive
+    # nothing. (An even lm_a leaves the top analysis level unwritten in the
+    # reference, so it is refused.)
+    P = _tparams(8, 8, 5, 1)
+    f = np.random.default_rng(16).standard_normal(P.gen_shape(0))
+    a = O.filt_to_anal(P, 8, 8, 1, 1, 7, f)
+    assert np.array_equal(a[:, :, 0::2], f[:, :, :4])
+    assert np.allclose(a[:, :, 1::2], 0.5 * (f[:, :, :3] + f[:, :, 1:4]), atol=1e-15)
+    b = O.anal_to_filt(P, 8, 8, 1, 1, 7, a)
+    assert np.all(b[:, :, 4] == 0.0)
+
+
+@pytest.mark.parametrize("scheme", [O.LSQR, O.LIN, O.QUAD])
+@pytest.mark.parametrize("vmap,la", [(O.VLIN, 5), (O.VLWQ, 7), (O.VLIN, 3)])
+def test_every_transfer_variant_has_its_adjoint(scheme, vmap, la):
+    P = _tparams(32, 24, 3, 2)
+    rng = np.random.default_rng(17)
+    vf = rng.standard_normal(P.gen_shape(0))
+    wa = rng.standard_normal((36, 48, 2 * la))
+    a = O.filt_to_anal(P, 48, 36, 2, 2, la, vf, scheme=scheme, vmap=vmap)
+    b = O.anal_to_filt(P, 48, 36, 2, 2, la, wa, scheme=scheme, vmap=vmap)
+    assert abs(np.sum(a * wa) - np.sum(vf * b)) <= 1e-12 * np.sqrt(np.sum(a * a) * np.sum(wa * wa))
+
+
 # ---- the fast (separable) filter of procedures 3 and 8 ----
 
 def test_fast_taps_take_the_aspect_amplitude_as_the_factor():
