@@ -81,6 +81,128 @@ def stream_copy_gbps(torch, nbytes=4 << 30, reps=5):
     return 2 * nbytes / t / 1e9
 
 
+# The other filtering procedures, the quadratic inter-generation transfer
+# and the analysis <-> filter transfer, each on a workload of its own:
+#   ensemble rows (procedures 7, 8 and lquart with 6): the headline workload,
+#     80 members x 768x768x64, 4 generations;
+#   background rows (procedures 1-5): one field of 8 three-dimensional
+#     variables x 64 levels (K = 512) on 1536x1536, 4 generations;
+#   transfer: anal_to_filt_all2 then filt_to_anal_all2 between the
+#     768x768x64 filter grid and a 1152x1152 analysis grid with 127 levels,
+#     16 members.
+ROWS = ("proc1", "proc2", "proc3", "proc4", "proc5", "proc7", "proc8", "lquart", "transfer")
+ROW_NAMES = {"proc1": "filtering_rad2_bckg", "proc2": "filtering_lin2_bckg (mgbf_line)",
+             "proc3": "filtering_fast_bckg", "proc4": "filtering_rad3", "proc5": "filtering_lin3 (mgbf_line)",
+             "proc7": "filtering_lin2_ens (mgbf_line)", "proc8": "filtering_fast_ens",
+             "lquart": "filtering_rad2_ens with the quadratic transfer (lquart)",
+             "transfer": "anal_to_filt_all2 + filt_to_anal_all2 (4-point horizontal, 2:1 vertical)"}
+
+
+def run_row(args, torch, MG):
+    row = args.row
+    weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
+    gm = args.gm
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    if row == "transfer":
+        M, n, lm, na, la = 16, 768, 64, 1152, 127
+        cfg = MG.Config.for_grid(n, n, lm, n_ens=M, gm=gm, h=6, hz=3, p=2, weights=weights,
+                                 nm0=na, mm0=na, lm_a=la)
+        workload = (f"{ROW_NAMES[row]}: {M} members, filter grid {n}x{n}x{lm}, "
+                    f"analysis grid {na}x{na}x{la}")
+    elif row in ("proc7", "proc8", "lquart"):
+        M, n, lm = 80, 768, 64
+        kw = {"proc7": dict(mgbf_proc=7, mgbf_line=True), "proc8": dict(mgbf_proc=8),
+              "lquart": dict(lquart=True)}[row]
+        cfg = MG.Config.for_grid(n, n, lm, n_ens=M, gm=gm, h=6, hz=3, p=2, weights=weights, **kw)
+        workload = f"{ROW_NAMES[row]}: {M} members x {n}x{n}x{lm}, {gm} generations, h=6, hz=3, p=2"
+    else:
+        proc = int(row[4:])
+        M, n, lm = 1, 1536, 64
+        cfg = MG.Config.for_grid(n, n, lm, n_ens=1, gm=gm, h=6, hz=3, p=2, weights=weights,
+                                 mgbf_proc=proc, km3=8, mgbf_line=proc in (2, 5))
+        workload = (f"{ROW_NAMES[row]}: one field of 8 variables x {lm} levels on {n}x{n}, "
+                    f"{gm} generations, h=6, hz=3, p=2")
+    log(f"[bench] row {row}: {workload}")
+    op = MG.MGBF(cfg)
+    d = op.dims
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    if row == "transfer":
+        w = torch.randn(d.anal_shape(), generator=g, device="cuda", dtype=torch.float32)
+        v = torch.empty(op.field_shape(), dtype=torch.float32, device="cuda")
+        w2 = torch.empty_like(w)
+        units = d.na * d.ma * d.km_a_all
+        # algorithmic bytes of the pair: each map reads its input and writes
+        # its output once (the vertical intermediate is not counted)
+        pair_bytes = 2 * 4.0 * (d.na * d.ma * d.km_a_all + d.nx * d.ny * d.km_all)
+
+        def step():
+            op.anal_to_filt(w, out=v)
+            op.filt_to_anal(v, out=w2)
+    else:
+        x = torch.randn(op.field_shape(), generator=g, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        units = d.nx * d.ny * d.km_all
+        if not args.no_tune:
+            op.autotune()
+
+        def step():
+            op.filtering_procedure(x, out=y)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    op.timers(True)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = op.read_timers()
+    op.timers(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = units / (ms_step / 1e3)
+    if row == "transfer":
+        # events on the launch stream around the timed steps: nothing else runs there
+        k_ms = e0.elapsed_time(e1)
+        k_n, k_bytes = 4 * args.steps, pair_bytes * args.steps
+        kernel = "xfer2d_kernel + kwin_kernel (A2F and F2A, two launches each)"
+        avg = k_ms / k_n
+        achieved = k_bytes / (k_ms / 1e3) / 1e9
+    else:
+        k_ms = phases["hfiltT"][0] + phases["hfilt"][0]
+        k_n = phases["hfiltT"][1] + phases["hfilt"][1]
+        k_bytes = phases["hfiltT"][2] + phases["hfilt"][2]
+        kernel = ("ball_kernel (3-D filter on every generation)" if row in ("proc4", "proc5")
+                  else "radial_kernel (F and F^T on every generation)")
+        avg = k_ms / max(k_n, 1)
+        achieved = k_bytes / (k_ms / 1e3) / 1e9
+    phase_out = {k: {"ms": round(v_[0] / args.steps, 4), "launches": v_[1] // args.steps}
+                 for k, v_ in phases.items()}
+    log(f"[bench] {ms_step:.3f} ms/step, {value:.4e} units/s, phases {phase_out}")
+    out = {
+        "metric": METRIC, "row": row, "value": value, "unit": "gridpoint·members/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: N(0,1), seed 7",
+        "config": {"workload": workload},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": kernel,
+                     "launches": k_n,
+                     "avg_launch_ms": round(avg, 4),
+                     "algorithmic_bytes_per_launch": k_bytes / max(k_n, 1)},
+        "cpu_baseline": None,
+        "phases": phase_out,
+    }
+    print(json.dumps(out), flush=True)
+    op.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +219,9 @@ def main():
                     help='pin radial tiles instead of autotuning, JSON as printed under "tiles", '
                          'e.g. \'{"g1k0": [4, 384], ...}\'')
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
+    ap.add_argument("--row", choices=ROWS, default=None,
+                    help="measure one of the other procedures or the analysis transfer instead "
+                         "(one JSON line with the same keys; no plain or CPU leg)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -111,6 +236,10 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+
+    if args.row:
+        run_row(args, torch, MG)
+        return
 
     M, n, lm, gm = args.members, args.n, args.levels, args.gm
     weights = [1.0, 0.5, 0.25, 0.125, 0.0625]

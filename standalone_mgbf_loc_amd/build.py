@@ -22,6 +22,8 @@ INCLUDE = os.path.join(ROOT, "include")
 SOURCES = ["mgbf_coef.cpp", "mgbf_capi.cpp", "mgbf_kernels.hip"]
 RADIAL_TU = "mgbf_radial_inst.hip"
 RADIAL_H = range(1, 11)
+BALL_TU = "mgbf_ball3.hip"
+BALL_H = range(1, 5)
 ARCH = os.environ.get("MGBF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -70,6 +72,9 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     for h in RADIAL_H:
         work.append((os.path.join(CSRC, RADIAL_TU), os.path.join(objdir, f"radial_h{h}.o"),
                      [f"-DMGBF_RADIAL_H={h}"], verbose))
+    for h in BALL_H:
+        work.append((os.path.join(CSRC, BALL_TU), os.path.join(objdir, f"ball3_h{h}.o"),
+                     [f"-DMGBF_BALL_H={h}"], verbose))
     jobs = jobs or min(len(work), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         errs = [e for e in ex.map(_compile, work) if e]

@@ -1,0 +1,244 @@
+// The 3-D filter of procedures 4 and 5 on the 3-D variables, in the
+// streaming form of the radial kernel. One translation unit per horizontal
+// half-width: compiled once for each MGBF_BALL_H in 1..4.
+//
+// The taps are sign-symmetric in every axis, w(a,b,c) = w(|a|,|b|,|c|), so
+// with D_c the plane filter of the taps at level offset +-c and Z_c the
+// level operator that adds the two mirrored shifts by +-c (Z_0 = I),
+//   F = sum_c D_c (x) Z_c,    F^T = sum_c D_c (x) Z_c^T
+// (the plane filters are symmetric under the half-sample mirror). Level and
+// plane operators commute, so every loaded input row is first paired in the
+// level direction, X_c = Z_c x (or Z_c^T x), across the lanes of the wave
+// (a wave holds whole columns: lm divides 64), and then filtered in the
+// plane like the radial kernel does: x-pair sums of X_c, per-row partial
+// sums over the taps of every c, a ring of 2H+1 accumulator rows.
+//
+// Level pairs by DPP lane shifts: with ext the column extended by the
+// whole-sample mirror (or by zeros, ZL), U_d(L) = ext(L+d) and
+// D_d(L) = ext(L-d) follow from U_{d-1} and D_{d-1} by one shift each:
+//   U_d = (L == lm-1) ? D_{d-1}(L-1) : U_{d-1}(L+1)   (0 under ZL)
+//   D_d = (L == 0)    ? U_{d-1}(L+1) : D_{d-1}(L-1)   (0 under ZL)
+// and Z_c x = U_c + D_c. The transpose of the mirrored pair differs from it
+// in four lanes of a column only:
+//   Z_c^T x = (1 + [L = lm-1-c] - [L = 0]) U_c + (1 + [L = c] - [L = lm-1]) D_c
+// (lm >= 2 HZ + 2 keeps the four lanes apart). Under ZL, Z_c^T = Z_c.
+#include "mgbf_device.hpp"
+#include "mgbf_launch.hpp"
+
+#ifndef MGBF_BALL_H
+#error "MGBF_BALL_H must be defined"
+#endif
+
+namespace mgbf {
+
+namespace {
+
+template <int H, int HZ, bool BALL, bool T, bool ZL>
+__global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
+    constexpr int TI = 4;
+    constexpr int NR = 2 * H + 1;
+    constexpr int NC = TI + 2 * H;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
+    if (item >= p.nitems) return;
+    const int K = p.K, n = p.n, m = p.m, lm = p.lm;
+    const int ti = item % p.ntile_i;
+    const int rest = item / p.ntile_i;
+    const int tj = rest % p.ntile_j;
+    const int kc = rest / p.ntile_j;
+    const int k = kc * kWave + lane;
+    const bool kvalid = k < p.K3;
+    const int kk = k < K ? k : K - 1;
+    const int i0 = ti * TI, j0 = tj * p.TJ;
+    const size_t rowstride = (size_t)n * K;
+
+    // this lane's level and the lanes of its column's ends
+    const int L = k % lm;
+    const bool top = L == lm - 1, bot = L == 0;
+    float cu[HZ + 1], cd[HZ + 1];
+#pragma unroll
+    for (int c = 1; c <= HZ; ++c) {
+        cu[c] = 1.f + (L == lm - 1 - c ? 1.f : 0.f) - (bot ? 1.f : 0.f);
+        cd[c] = 1.f + (L == c ? 1.f : 0.f) - (top ? 1.f : 0.f);
+    }
+
+    const int vk = kk * 4;
+    int soff[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - H + c, n) * K * 4;
+
+    float acc[NR][TI];
+#pragma unroll
+    for (int s = 0; s < NR; ++s)
+#pragma unroll
+        for (int t = 0; t < TI; ++t) acc[s][t] = 0.f;
+
+    const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
+    const int nsteps = tj_eff + 2 * H;
+    for (int rb = 0; rb < nsteps; rb += NR) {
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            const int rr = rb + s;
+            if (rr < nsteps) {
+                const int r = mirror_half(j0 - H + rr, m);
+                const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)r * rowstride);
+                // level pairs X[c][col] of the loaded row
+                float X[HZ + 1][NC];
+#pragma unroll
+                for (int col = 0; col < NC; ++col) {
+                    const float v = buf_ld(rs, vk, soff[col]);
+                    X[0][col] = v;
+                    float u = v, d = v;
+#pragma unroll
+                    for (int c = 1; c <= HZ; ++c) {
+                        const float fa = lane_from_above(u);
+                        const float fb = lane_from_below(d);
+                        if constexpr (ZL) {
+                            u = top ? 0.f : fa;
+                            d = bot ? 0.f : fb;
+                        } else {
+                            u = top ? fb : fa;
+                            d = bot ? fa : fb;
+                        }
+                        if constexpr (T && !ZL)
+                            X[c][col] = __builtin_fmaf(cu[c], u, cd[c] * d);
+                        else
+                            X[c][col] = u + d;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TI; ++t) {
+                    if constexpr (BALL) {
+                        // w(a,b,c) = wr[a^2 + b^2 + c^2] on the ball
+                        float Z[H + 1];
+#pragma unroll
+                        for (int b = 0; b <= H; ++b) Z[b] = 0.f;
+#pragma unroll
+                        for (int c = 0; c <= HZ; ++c) {
+                            float P[H + 1];
+                            P[0] = X[c][t + H];
+#pragma unroll
+                            for (int a = 1; a <= H; ++a) P[a] = X[c][t + H + a] + X[c][t + H - a];
+#pragma unroll
+                            for (int b = 0; b <= H; ++b)
+#pragma unroll
+                                for (int a = 0; a <= H; ++a)
+                                    if (a * a + b * b + c * c < (H + 1) * (H + 1))
+                                        Z[b] = __builtin_fmaf(p.W.wr[a * a + b * b + c * c], P[a], Z[b]);
+                        }
+#pragma unroll
+                        for (int b = 0; b <= H; ++b) {
+                            acc[(s + b) % NR][t] += Z[b];
+                            if (b > 0) acc[(s + NR - b) % NR][t] += Z[b];
+                        }
+                    } else {
+                        // the box of line products w(a,b,c) = fx(a) fy(b) fz(c):
+                        // R = sum_c fz(c) sum_a fx(a) P_a(X_c), row r+-b gets fy(b) R
+                        float R = 0.f;
+#pragma unroll
+                        for (int c = 0; c <= HZ; ++c) {
+                            float q = p.W.fx[0] * X[c][t + H];
+#pragma unroll
+                            for (int a = 1; a <= H; ++a)
+                                q = __builtin_fmaf(p.W.fx[a], X[c][t + H + a] + X[c][t + H - a], q);
+                            R = __builtin_fmaf(p.W.fz[c], q, R);
+                        }
+#pragma unroll
+                        for (int b = 0; b <= H; ++b) {
+                            acc[(s + b) % NR][t] = __builtin_fmaf(p.W.fy[b], R, acc[(s + b) % NR][t]);
+                            if (b > 0)
+                                acc[(s + NR - b) % NR][t] =
+                                    __builtin_fmaf(p.W.fy[b], R, acc[(s + NR - b) % NR][t]);
+                        }
+                    }
+                }
+                // output row j = j0 + rr - 2H is complete
+                const int jrel = rr - 2 * H;
+                constexpr int slot_off = NR - H;   // (s - H) mod NR
+                if (jrel >= 0 && jrel < tj_eff) {
+                    const int j = j0 + jrel;
+                    const size_t rowoff = (size_t)j * rowstride + (size_t)i0 * K;
+                    float o[TI];
+                    if (p.wfield) {
+                        const __amdgpu_buffer_rsrc_t rw = buf_rsrc(p.wfield + rowoff);
+#pragma unroll
+                        for (int t = 0; t < TI; ++t) {
+                            const int tc = i0 + t < n ? t : n - 1 - i0;
+                            o[t] = buf_ld(rw, vk, tc * K * 4) * acc[(s + slot_off) % NR][t];
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < TI; ++t) o[t] = p.wconst * acc[(s + slot_off) % NR][t];
+                    }
+                    if (kvalid) {
+                        const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + rowoff);
+#pragma unroll
+                        for (int t = 0; t < TI; ++t)
+                            if (i0 + t < n) buf_st(o[t], ro, vk, t * K * 4);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < TI; ++t) acc[(s + slot_off) % NR][t] = 0.f;
+            }
+        }
+    }
+}
+
+template <int H, int HZ, bool BALL, bool T, bool ZL>
+hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
+    Ball3Params p{};
+    p.in = a.in;
+    p.out = a.out;
+    p.wfield = a.wfield;
+    p.wconst = a.wconst;
+    p.K = a.K;
+    p.n = a.n;
+    p.m = a.m;
+    p.TJ = a.TJ;
+    p.lm = a.lm;
+    p.K3 = a.K3;
+    p.W = a.coef;
+    p.ntile_i = (a.n + 3) / 4;
+    p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
+    const long long nkc = (a.K3 + kWave - 1) / kWave;
+    const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
+    if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
+    p.nitems = (int)nitems;
+    const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
+    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    return hipGetLastError();
+}
+
+template <int H, int HZ>
+hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
+    if (a.zero_levels) {
+        // zeros beyond the column ends: Z_c^T = Z_c, the box of the line products
+        if (a.ball) return hipErrorInvalidValue;
+        return launch_variant<H, HZ, false, false, true>(a, st);
+    }
+    if (!a.ball || HZ != H) return hipErrorInvalidValue;
+    if constexpr (HZ == H)
+        return a.transpose ? launch_variant<H, HZ, true, true, false>(a, st)
+                           : launch_variant<H, HZ, true, false, false>(a, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+#define MGBF_CAT2(x, y) x##y
+#define MGBF_CAT(x, y) MGBF_CAT2(x, y)
+
+hipError_t MGBF_CAT(launch_ball3_H, MGBF_BALL_H)(const Ball3Args& a, hipStream_t st) {
+    constexpr int H = MGBF_BALL_H;
+    switch (a.HZ) {
+        case 0: return launch_hz<H, 0>(a, st);
+        case 1: return launch_hz<H, 1>(a, st);
+        case 2: return launch_hz<H, 2>(a, st);
+        case 3: return launch_hz<H, 3>(a, st);
+        case 4: return launch_hz<H, 4>(a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mgbf

@@ -85,6 +85,11 @@ struct mgbf_ctx {
     // procedure 4 (filtering_rad3): tap lists of the 3-D filter and of the
     // 2-D filter of its 2-D variables
     bool rad3 = false;
+    // the streaming 3-D filter on the 3-D variables (procedures 4 and 5),
+    // when the taps and the column length allow it; else the gather
+    bool ball3 = false;
+    mgbf::Ball3Coef b3{};
+    int b3H = 0, b3HZ = 0, b3ball = 0;
     const int4* d_t3 = nullptr;
     const float* d_w3 = nullptr;
     const int4* d_t2 = nullptr;
@@ -526,8 +531,80 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
 // 2-D variables in the plane), no separate vertical filter:
 //   y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
 // ---------------------------------------------------------------------------
-int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st) {
+// Plans the streaming 3-D filter: procedure 4's ball weights by |g|^2, or
+// procedure 5's line factors; false (the gather then runs) when the taps,
+// the half-widths or the column length are outside what it is built for.
+bool plan_ball3(const mgbf::Setup& s, const mgbf_config& cfg, mgbf::Ball3Coef& co, int& H, int& HZ,
+                int& ball) {
+    co = mgbf::Ball3Coef{};
+    const int lm = cfg.lm;
+    if (s.dims.km3_all < 1 || lm < 1 || 64 % lm != 0) return false;   // whole columns per wavefront
+    if (cfg.mgbf_proc == 4) {
+        H = HZ = 0;
+        for (const auto& t : s.taps3) {
+            H = std::max(H, std::max(std::abs(t.gx), std::abs(t.gy)));
+            HZ = std::max(HZ, std::abs(t.gz));
+        }
+        if (H < 1 || H != HZ || H > mgbf::kBall3MaxH) return false;
+        std::vector<double> wr((H + 1) * (H + 1), 0.0);
+        std::vector<char> seen(wr.size(), 0);
+        for (const auto& t : s.taps3) {
+            const int r2 = t.gx * t.gx + t.gy * t.gy + t.gz * t.gz;
+            if (r2 >= (H + 1) * (H + 1)) return false;
+            if (seen[r2] && std::fabs(wr[r2] - t.w) > 1e-12 * std::fabs(t.w)) return false;
+            wr[r2] = t.w;
+            seen[r2] = 1;
+        }
+        for (size_t r = 0; r < wr.size(); ++r) co.wr[r] = (float)wr[r];
+        ball = 1;
+    } else if (cfg.mgbf_proc == 5) {
+        const std::vector<double>* f = s.hex_f;
+        if (f[0].empty() || f[1].empty() || f[2].empty()) return false;
+        H = std::max(1, (int)std::max(f[0].size(), f[1].size()) - 1);
+        HZ = (int)f[2].size() - 1;
+        if (H > mgbf::kBall3MaxH || HZ > mgbf::kBall3MaxHZ) return false;
+        for (size_t a = 0; a < f[0].size(); ++a) co.fx[a] = (float)f[0][a];
+        for (size_t b = 0; b < f[1].size(); ++b) co.fy[b] = (float)f[1][b];
+        for (size_t z = 0; z < f[2].size(); ++z) co.fz[z] = (float)f[2][z];
+        if (!s.zero_levels) return false;
+        ball = 0;
+    } else {
+        return false;
+    }
+    return lm >= 2 * HZ + 2;
+}
+
+// The 3-D filter of generation g (F, or F^T with transpose). With `scaled`
+// given, the output is also multiplied by the generation's weights when the
+// streaming pass covers the whole stack, and *scaled says whether it was.
+int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st,
+         bool* scaled = nullptr) {
     Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st, 8.0 * (double)c->gsize[g]);
+    const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
+    if (scaled) *scaled = false;
+    if (c->ball3) {
+        const bool fuse = scaled && K3 >= c->K;
+        mgbf::Ball3Args b{};
+        b.in = in;
+        b.out = out;
+        b.wfield = fuse ? c->wfield[g] : nullptr;
+        b.wconst = fuse ? c->wconst[g] : 1.f;
+        if (fuse) *scaled = true;
+        b.K = c->K;
+        b.n = c->nx[g];
+        b.m = c->ny[g];
+        b.lm = c->s.cfg.lm;
+        b.K3 = K3;
+        b.H = c->b3H;
+        b.HZ = c->b3HZ;
+        b.TJ = pick_tj(K3, b.n, b.m, 4);
+        b.ball = c->b3ball;
+        b.transpose = transpose ? 1 : 0;
+        b.zero_levels = c->s.zero_levels ? 1 : 0;
+        b.coef = c->b3;
+        HIP_TRY(mgbf::launch_ball3(b, st), "3-D filter launch");
+        if (K3 >= c->K) return MGBF_OK;
+    }
     mgbf::BallArgs a{};
     a.in = in;
     a.out = out;
@@ -544,6 +621,7 @@ int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStr
     a.K3 = c->s.dims.km3_all * c->s.cfg.lm;
     a.transpose = transpose ? 1 : 0;
     a.zero_levels = c->s.zero_levels ? 1 : 0;
+    a.klo = c->ball3 ? K3 : 0;   // the 2-D variables only, after the streaming pass
     HIP_TRY(mgbf::launch_ball(a, st), "3-D filter launch");
     return MGBF_OK;
 }
@@ -562,10 +640,12 @@ int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     float* T = c->ws_t;
     CALL(restrict_chain(c, x, U, st));
     for (int g = 0; g < c->gm; ++g) {
-        CALL(ball(c, g, g == 0 ? x : U + c->goff[g], T + c->goff[g], true, st));
-        HIP_TRY(mgbf::launch_scale(T + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 0,
-                                   c->gsize[g], st),
-                "scale launch");
+        bool scaled = false;
+        CALL(ball(c, g, g == 0 ? x : U + c->goff[g], T + c->goff[g], true, st, &scaled));
+        if (!scaled)
+            HIP_TRY(mgbf::launch_scale(T + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 0,
+                                       c->gsize[g], st),
+                    "scale launch");
         CALL(ball(c, g, T + c->goff[g], U + c->goff[g], false, st));
     }
     return rad3_upsweep(c, U, y, st);
@@ -741,6 +821,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     c->quad = cfg->lquart != 0;
     c->rad3 = cfg->mgbf_proc == 4 || cfg->mgbf_proc == 5;
+    if (c->rad3) {
+        // MGBF_BALL_GATHER=1 keeps the gather for the 3-D variables too
+        const char* bg = std::getenv("MGBF_BALL_GATHER");
+        c->ball3 = !(bg && bg[0] == '1') && plan_ball3(s, *cfg, c->b3, c->b3H, c->b3HZ, c->b3ball);
+    }
     {
         // MGBF_EXACT_ORDER=1 keeps the reference's order Vf^T ... Vf also for
         // weights that are constant along the stack

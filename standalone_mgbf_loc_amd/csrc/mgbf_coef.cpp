@@ -550,9 +550,12 @@ bool line_hexad(const double a[6], int l7[7][3], double w7[7]) {
 
 }  // namespace
 
-int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err) {
+int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err,
+             std::vector<double>* factors) {
     taps.clear();
     *zero_levels = true;
+    if (factors)
+        for (int e = 0; e < 3; ++e) factors[e].assign(1, 1.0);
     if (!cfg.mgbf_line) {
         // without init_mg_line no color has a line: every call copies
         taps.push_back({0, 0, 0, 1.0});
@@ -620,6 +623,8 @@ int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels,
         *err = "a level line of the first color is not built";
         return MGBF_ERR_UNSUPPORTED;
     }
+    if (factors)
+        for (int e = 0; e < 3; ++e) factors[e] = f[e];
     const int hx = (int)f[0].size() - 1, hy = (int)f[1].size() - 1, hz = (int)f[2].size() - 1;
     for (int c = -hz; c <= hz; ++c)
         for (int b = -hy; b <= hy; ++b)
@@ -951,7 +956,7 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
     }
     if (cfg.mgbf_proc == 4) ball_taps(cfg, s.taps3);   // filtering_rad3 (reach checked above)
     if (cfg.mgbf_proc == 5) {                           // filtering_lin3
-        st = hex_taps(cfg, s.taps3, &s.zero_levels, err);
+        st = hex_taps(cfg, s.taps3, &s.zero_levels, err, s.hex_f);
         if (st != MGBF_OK) return st;
     }
     s.dims.ntaps2 = (int)s.taps2.size();

@@ -42,6 +42,7 @@ struct Setup {
     std::vector<Tap2> taps2;           // reference loop order
     std::vector<Tap3> taps3;           // 3-D filter of procedures 4 (vrbeta3 loop order) and 5
     bool zero_levels = false;          // the 3-D filter reads zeros beyond the column ends (5)
+    std::vector<double> hex_f[3];      // procedure 5: the x, y, level line factors of taps3
     std::vector<int> taps1_gx;         // vertical taps
     std::vector<double> taps1_w;
     int H = 0;                          // max(hx, hy) used by the radial kernels
@@ -90,6 +91,9 @@ bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<dou
 int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err);
 // 3-D taps of the hexad line filters of procedure 5 (mgbf_line): the x, y and
 // level lines as a product; *zero_levels: zeros beyond the column ends.
-int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err);
+// factors (3 vectors, optional): the x, y and level line filters whose
+// product the taps are, indexed by |offset|.
+int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err,
+             std::vector<double>* factors = nullptr);
 
 }  // namespace mgbf

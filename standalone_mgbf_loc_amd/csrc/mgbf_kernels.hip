@@ -402,6 +402,19 @@ hipError_t launch_radial(const RadialArgs& a, hipStream_t st) {
     }
 }
 
+hipError_t launch_ball3(const Ball3Args& a, hipStream_t st) {
+    if (a.HZ < 0 || a.HZ > kBall3MaxHZ || a.lm < 2 * a.HZ + 2 || kWave % a.lm != 0 || a.TJ < 1 ||
+        a.K3 > a.K || a.K3 < 1)
+        return hipErrorInvalidValue;
+    switch (a.H) {
+        case 1: return launch_ball3_H1(a, st);
+        case 2: return launch_ball3_H2(a, st);
+        case 3: return launch_ball3_H3(a, st);
+        case 4: return launch_ball3_H4(a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
     constexpr int TI2 = 4, TJ2 = 16;
     const int nc = a.nf / 2, mc = a.mf / 2;
@@ -533,11 +546,11 @@ template <bool T>
 __global__ void __launch_bounds__(kBlock) ball_kernel(BallArgs a, int nkc, long long nitems) {
     const long long item = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
     if (item >= nitems) return;
-    const int kc = (int)(item % nkc);
+    const int kc = (int)(item % nkc) + a.klo / kWave;
     const long long rest = item / nkc;
     const int i = (int)(rest % a.n), j = (int)(rest / a.n);
     const int k = kc * kWave + (threadIdx.x & (kWave - 1));
-    if (k >= a.K) return;
+    if (k >= a.K || k < a.klo) return;
     const int lm = a.lm;
     float acc = 0.f;
     if (k < a.K3) {
@@ -571,7 +584,8 @@ __global__ void __launch_bounds__(kBlock) ball_kernel(BallArgs a, int nkc, long 
 }
 
 hipError_t launch_ball(const BallArgs& a, hipStream_t st) {
-    const int nkc = (a.K + kWave - 1) / kWave;
+    if (a.klo < 0 || a.klo >= a.K) return hipErrorInvalidValue;
+    const int nkc = (a.K + kWave - 1) / kWave - a.klo / kWave;
     const long long nitems = (long long)nkc * a.n * a.m;
     const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
     if (nblocks >= (1ll << 31)) return hipErrorInvalidValue;

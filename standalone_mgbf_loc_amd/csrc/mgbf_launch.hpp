@@ -157,7 +157,45 @@ struct BallArgs {
     int K, n, m, lm, K3;
     int transpose;
     int zero_levels;     // zeros beyond the column ends instead of the level mirror
+    int klo;             // only k >= klo is written
 };
 hipError_t launch_ball(const BallArgs& a, hipStream_t st);
+
+// The same 3-D filter on the 3-D variables in the streaming form of the
+// radial kernel (mgbf_ball3.hip): sign-symmetric taps with |a|,|b| <= H and
+// |c| <= HZ, a wave holding whole columns (lm divides 64, lm >= 2 HZ + 2).
+// ball: procedure 4's taps, w = wr[a^2+b^2+c^2] inside the lattice ball of
+// radius H+1 (H = HZ, level mirror); otherwise procedure 5's box of line
+// products w = fx[|a|] fy[|b|] fz[|c|] under zero_levels.
+constexpr int kBall3MaxH = 4;
+constexpr int kBall3MaxHZ = 4;
+struct Ball3Coef {
+    float wr[(kBall3MaxH + 1) * (kBall3MaxH + 1)];
+    float fx[kBall3MaxH + 1], fy[kBall3MaxH + 1], fz[kBall3MaxHZ + 1];
+};
+struct Ball3Args {
+    const float* in;
+    float* out;
+    const float* wfield;   // per-point scale of the output, or nullptr
+    float wconst;          // constant scale when wfield is nullptr
+    int K, n, m, lm, K3;   // only k < K3 is written
+    int H, HZ, TJ;
+    int ball, transpose, zero_levels;
+    Ball3Coef coef;
+};
+struct Ball3Params {
+    const float* in;
+    float* out;
+    const float* wfield;
+    float wconst;
+    int K, n, m, TJ, lm, K3;
+    int ntile_i, ntile_j, nitems;
+    Ball3Coef W;
+};
+hipError_t launch_ball3(const Ball3Args& a, hipStream_t st);
+hipError_t launch_ball3_H1(const Ball3Args& a, hipStream_t st);
+hipError_t launch_ball3_H2(const Ball3Args& a, hipStream_t st);
+hipError_t launch_ball3_H3(const Ball3Args& a, hipStream_t st);
+hipError_t launch_ball3_H4(const Ball3Args& a, hipStream_t st);
 
 }  // namespace mgbf

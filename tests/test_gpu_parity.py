@@ -6,6 +6,8 @@ adjoint dot-product mismatch |<Wv,x> - <v,W^T x>| <= 1e-6 * |<Wv,x>|, or
 <= 1e-6 * |Wv| |x| (the Cauchy-Schwarz scale) where random x and v make
 <Wv,x> cancel down to a small fraction of that scale.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -521,6 +523,73 @@ def test_lin3_procedure_matches_oracle(case):
         rhs = op.dot(vd, wt)
         scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
         assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("proc,lm,km3,km2,hz,case", [
+    (4, 16, 3, 0, 2, ""),        # K3 = 48: a partial wavefront of whole columns
+    (4, 16, 4, 5, 3, ""),        # the 2-D variables in the next wavefront (gather)
+    (4, 32, 3, 2, 1, ""),        # a wavefront shared by 3-D and 2-D variables
+    (4, 8, 2, 0, 3, ""),         # lm = 2 hz + 2: the four corrected lanes adjacent
+    (4, 64, 1, 0, 4, ""),        # one column per wavefront, half-width 4
+    (4, 16, 3, 2, 2, "lquart"),
+    (5, 16, 3, 2, 3, ""),
+    (5, 32, 2, 0, 4, "wide"),
+    (5, 16, 3, 2, 3, "no_tables"),
+])
+def test_streaming_3d_filter_matches_oracle(proc, lm, km3, km2, hz, case):
+    # the streaming 3-D filter (columns of lm | 64 levels across the lanes of
+    # a wavefront, level pairs by lane shifts) against the FP64 oracle, and
+    # against the gather it replaces (MGBF_BALL_GATHER=1), with per-point
+    # weights on generation 1
+    kw = dict(gm=3, h=4, hz=hz, mgbf_proc=proc, km3=km3, km2=km2, weights=[1.0, 0.5, 0.25])
+    if proc == 5:
+        kw["mgbf_line"] = case != "no_tables"
+    if case == "lquart":
+        kw["lquart"] = True
+    cfg = M.Config.for_grid(64, 48, lm, **kw)
+    if proc == 5:
+        cfg.mg_ampl02 = 0.7
+        cfg.mg_ampl03 = 0.25 if case == "wide" else 0.5
+    print(f"streaming 3-D filter proc={proc} lm={lm} km3={km3} km2={km2} hz={hz} {case}", flush=True)
+    outs = {}
+    for gather in (False, True):
+        os.environ["MGBF_BALL_GATHER"] = "1" if gather else "0"
+        try:
+            op = M.MGBF(cfg)
+        finally:
+            os.environ.pop("MGBF_BALL_GATHER", None)
+        with op:
+            d = op.dims
+            P = oparams(cfg, d)
+            rng = np.random.default_rng(190 + lm + hz)
+            wf = (0.5 + rng.random(d.gen_shape(1))).astype(np.float32)
+            wd = to_dev(wf)
+            op.set_weights(1, field=wd)
+            x = field(d.gen_shape(1), 191)
+            v = np.random.default_rng(192).standard_normal(d.mg_size).astype(np.float32)
+            xd, vd = to_dev(x), to_dev(v)
+            f = op.stage(M.STAGE_F, 1, xd)
+            ft = op.stage(M.STAGE_FT, 2, to_dev(field(d.gen_shape(2), 193)))
+            y = op.filtering_procedure(xd)
+            wt = op.apply(M.OP_WT, xd)
+            w = op.apply(M.OP_W, vd)
+            sync()
+            wfl = [wf.astype(np.float64), None, None]
+            assert relerr(f, O.radial(P, x.astype(np.float64))) <= TOL_STAGE
+            x2 = field(d.gen_shape(2), 193).astype(np.float64)
+            assert relerr(ft, O.radial(P, x2, adjoint=True)) <= TOL_STAGE
+            assert relerr(y, O.wawt(P, x.astype(np.float64), wfield=wfl)) <= TOL_OP
+            assert relerr(wt, O.wt(P, x.astype(np.float64))) <= TOL_OP
+            assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+            lhs, rhs = op.dot(w, xd), op.dot(vd, wt)
+            scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+            assert abs(lhs - rhs) <= 1e-6 * scale
+            outs[gather] = y.clone()
+    # the two engines sum in different orders: equal to rounding, not bitwise
+    # (without line tables the filter is the identity, exact in both)
+    if case != "no_tables":
+        assert not torch.equal(outs[False], outs[True])
+    assert relerr(outs[False], outs[True].double().cpu().numpy()) <= 1e-6
 
 
 def test_dot_is_fp64_accumulated():
