@@ -129,6 +129,10 @@ struct mgbf_ctx {
     // analysis <-> filter transfer: separable window tables (direct: filter ->
     // analysis, adjoint: its transpose) and the vertical k windows
     mgbf::XferArgs xf_dir{}, xf_adj{};
+    // the same transfers by 8 x 8 output tiles, when their blocks fit
+    bool xt_ok[2] = {false, false};   // [0] direct (filter -> analysis), [1] adjoint
+    bool xfer_gather = false;         // MGBF_XFER_GATHER=1: the per-point gather instead
+    mgbf::XferTileArgs xt[2]{};
     // vertical transfer of the _all2 (kv) and the _all (kw) routines:
     // direct (d) and adjoint (a) window tables; null when lm_a = lm or the
     // routine family cannot run
@@ -531,6 +535,58 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
 // 2-D variables in the plane), no separate vertical filter:
 //   y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
 // ---------------------------------------------------------------------------
+// Dense per-tile blocks of a 1-D window table for the tiled transfer: tile
+// q covers outputs q*TB .. q*TB+TB-1 and reads inputs s0[q] .. s0[q]+S-1
+// with weights W[q][t][c]. S is the widest span over the tiles, rounded up
+// to a built width for the x axis (round_x); false when S exceeds the input,
+// the largest built width, or 64 rows.
+bool tile_blocks(const mgbf::AxisTable& t, int nin, int TB, bool round_x, int& S, std::vector<int>& s0,
+                 std::vector<double>& W) {
+    const int nt = (t.n_out + TB - 1) / TB;
+    if (nt < 1) return false;
+    std::vector<int> lo(nt, nin), hi(nt, -1);
+    for (int r = 0; r < t.n_out; ++r)
+        for (int u = 0; u < t.width; ++u)
+            if (t.w[(size_t)r * t.width + u] != 0.0) {
+                const int col = t.start[r] + u;
+                lo[r / TB] = std::min(lo[r / TB], col);
+                hi[r / TB] = std::max(hi[r / TB], col);
+            }
+    int span = 1;
+    for (int q = 0; q < nt; ++q)
+        if (hi[q] >= lo[q]) span = std::max(span, hi[q] - lo[q] + 1);
+    if (round_x) {
+        const int sizes[6] = {8, 12, 16, 20, 24, 32};
+        int r = 0;
+        for (int z : sizes)
+            if (!r && z >= span) r = z;
+        if (!r) return false;
+        span = r;
+    } else if (span > 64) {
+        return false;
+    }
+    if (span > nin) return false;
+    S = span;
+    s0.assign(nt, 0);
+    W.assign((size_t)nt * TB * S, 0.0);
+    for (int q = 0; q < nt; ++q) {
+        int st = hi[q] >= lo[q] ? lo[q] : 0;
+        if (st + S > nin) st = nin - S;
+        s0[q] = st;
+    }
+    for (int r = 0; r < t.n_out; ++r) {
+        const int q = r / TB, tt = r % TB;
+        for (int u = 0; u < t.width; ++u) {
+            const double w = t.w[(size_t)r * t.width + u];
+            if (w == 0.0) continue;
+            const int col = t.start[r] + u - s0[q];
+            if (col < 0 || col >= S) return false;
+            W[((size_t)q * TB + tt) * S + col] += w;
+        }
+    }
+    return true;
+}
+
 // Plans the streaming 3-D filter: procedure 4's ball weights by |g|^2, or
 // procedure 5's line factors; false (the gather then runs) when the taps,
 // the half-widths or the column length are outside what it is built for.
@@ -692,10 +748,17 @@ int apply_f2a(mgbf_ctx* c, const VMap& vd, const char* refusal, const float* in,
     }
     if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
     float* mid = vd.s ? c->ws_a : out;
-    mgbf::XferArgs a = c->xf_dir;
-    a.in = in;
-    a.out = mid;
-    HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    if (c->xt_ok[0] && !c->xfer_gather) {
+        mgbf::XferTileArgs a = c->xt[0];
+        a.in = in;
+        a.out = mid;
+        HIP_TRY(mgbf::launch_xfer_tile(a, st), "transfer launch");
+    } else {
+        mgbf::XferArgs a = c->xf_dir;
+        a.in = in;
+        a.out = mid;
+        HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    }
     if (vd.s)
         HIP_TRY(mgbf::launch_kwin(mid, out, c->K, d.km_a_all, (long long)d.na * d.ma, vd.s, vd.w, vd.width, st),
                 "vertical transfer launch");
@@ -720,10 +783,17 @@ int apply_a2f(mgbf_ctx* c, const VMap& va, const char* refusal, const float* in,
                 "vertical transfer launch");
         src = c->ws_a;
     }
-    mgbf::XferArgs a = c->xf_adj;
-    a.in = src;
-    a.out = out;
-    HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    if (c->xt_ok[1] && !c->xfer_gather) {
+        mgbf::XferTileArgs a = c->xt[1];
+        a.in = src;
+        a.out = out;
+        HIP_TRY(mgbf::launch_xfer_tile(a, st), "transfer launch");
+    } else {
+        mgbf::XferArgs a = c->xf_adj;
+        a.in = src;
+        a.out = out;
+        HIP_TRY(mgbf::launch_xfer2d(a, st), "transfer launch");
+    }
     return MGBF_OK;
 }
 
@@ -821,6 +891,10 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
     c->quad = cfg->lquart != 0;
     c->rad3 = cfg->mgbf_proc == 4 || cfg->mgbf_proc == 5;
+    {
+        const char* xg = std::getenv("MGBF_XFER_GATHER");
+        c->xfer_gather = xg && xg[0] == '1';
+    }
     if (c->rad3) {
         // MGBF_BALL_GATHER=1 keeps the gather for the 3-D variables too
         const char* bg = std::getenv("MGBF_BALL_GATHER");
@@ -947,6 +1021,29 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         bytes += carve_size(sizeof(int4) * s.taps3.size()) + carve_size(sizeof(float) * s.taps3.size());
         bytes += carve_size(sizeof(int4) * s.taps2.size()) + carve_size(sizeof(float) * s.taps2.size());
     }
+    // tiled transfer blocks: [0] direct (filter -> analysis), [1] adjoint
+    struct Blocks {
+        std::vector<int> x0, y0;
+        std::vector<double> wx, wy;
+        int SX = 0, SY = 0;
+        bool ok = false;
+    } tb[2];
+    if (!s.anal_identity) {
+        const mgbf_dims& d = s.dims;
+        const mgbf::AxisTable* ax[2] = {&s.ax_d, &s.ax_a};
+        const mgbf::AxisTable* ay[2] = {&s.ay_d, &s.ay_a};
+        const int ninx[2] = {d.nx, d.na}, niny[2] = {d.ny, d.ma};
+        for (int e = 0; e < 2; ++e) {
+            Blocks& b = tb[e];
+            // 32-bit row offsets inside an input row
+            b.ok = (long long)ninx[e] * c->K * 4 < (1ll << 31) &&
+                   tile_blocks(*ax[e], ninx[e], 8, true, b.SX, b.x0, b.wx) &&
+                   tile_blocks(*ay[e], niny[e], 8, false, b.SY, b.y0, b.wy);
+            if (b.ok)
+                bytes += carve_size(sizeof(int) * b.x0.size()) + carve_size(sizeof(float) * b.wx.size()) +
+                         carve_size(sizeof(int) * b.y0.size()) + carve_size(sizeof(float) * b.wy.size());
+        }
+    }
     std::vector<char> host(bytes, 0);
     hipError_t e = hipMalloc(&c->d_tables, bytes);
     if (e != hipSuccess) {
@@ -1039,6 +1136,15 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         c->xf_adj = mgbf::XferArgs{nullptr, nullptr, c->K, d.na, d.ma, d.nx, d.ny,
                                    put_i(s.ax_a.start), put_f(s.ax_a.w), s.ax_a.width,
                                    put_i(s.ay_a.start), put_f(s.ay_a.w), s.ay_a.width};
+        for (int e = 0; e < 2; ++e) {
+            if (!tb[e].ok) continue;
+            const bool dir = e == 0;
+            c->xt[e] = mgbf::XferTileArgs{nullptr, nullptr, c->K,
+                                          dir ? d.nx : d.na, dir ? d.ny : d.ma, dir ? d.na : d.nx, dir ? d.ma : d.ny,
+                                          put_i(tb[e].x0), put_f(tb[e].wx), tb[e].SX,
+                                          put_i(tb[e].y0), put_f(tb[e].wy), tb[e].SY};
+            c->xt_ok[e] = true;
+        }
         auto put_v = [&](const mgbf::AxisTable& t, VMap& v) {
             if (t.n_out == 0) return;
             v.s = put_i(t.start);

@@ -522,17 +522,112 @@ __global__ void __launch_bounds__(kBlock) xfer2d_kernel(XferArgs a, long long ni
     a.out[k + (size_t)a.K * ((size_t)a.nout_x * j + i)] = acc;
 }
 
+// One wave per 8 x 8 output points and 64 consecutive k: the input block of
+// the tile streams through registers row by row; each row is interpolated in
+// x for the 8 output columns (dense tile weights, wave-uniform) and added
+// into the 8 output rows with their y weights.
+template <int SX>
+__global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, int ntx, int nty, long long nitems) {
+    constexpr int TB = 8;
+    const long long item = (long long)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
+                           __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int p = (int)(item % ntx);
+    const long long rest = item / ntx;
+    const int q = (int)(rest % nty);
+    const int kc = (int)(rest / nty);
+    const int k = kc * kWave + lane;
+    const bool kvalid = k < a.K;
+    const int vk = (kvalid ? k : a.K - 1) * 4;
+    const int x0 = a.x0[p], y0 = a.y0[q];
+    const float* WX = a.wx + (size_t)p * TB * SX;
+    const float* WY = a.wy + (size_t)q * TB * a.SY;
+    int soff[SX];
+#pragma unroll
+    for (int c = 0; c < SX; ++c) soff[c] = (x0 + c) * a.K * 4;
+    float acc[TB][TB];
+#pragma unroll
+    for (int u = 0; u < TB; ++u)
+#pragma unroll
+        for (int t = 0; t < TB; ++t) acc[u][t] = 0.f;
+    const size_t rowstride = (size_t)a.nin_x * a.K;
+    for (int r = 0; r < a.SY; ++r) {
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)(y0 + r) * rowstride);
+        float v[SX];
+#pragma unroll
+        for (int c = 0; c < SX; ++c) v[c] = buf_ld(rs, vk, soff[c]);
+        float xr[TB];
+#pragma unroll
+        for (int t = 0; t < TB; ++t) {
+            float z = 0.f;
+#pragma unroll
+            for (int c = 0; c < SX; ++c) z = __builtin_fmaf(WX[t * SX + c], v[c], z);
+            xr[t] = z;
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            const float wy = WY[u * a.SY + r];
+#pragma unroll
+            for (int t = 0; t < TB; ++t) acc[u][t] = __builtin_fmaf(wy, xr[t], acc[u][t]);
+        }
+    }
+    if (!kvalid) return;
+#pragma unroll
+    for (int u = 0; u < TB; ++u) {
+        const int j = q * TB + u;
+        if (j < a.nout_y) {
+            const __amdgpu_buffer_rsrc_t ro = buf_rsrc(a.out + (size_t)j * a.nout_x * a.K + (size_t)p * TB * a.K);
+#pragma unroll
+            for (int t = 0; t < TB; ++t)
+                if (p * TB + t < a.nout_x) buf_st(acc[u][t], ro, vk, t * a.K * 4);
+        }
+    }
+}
+
+// One wave per 64 consecutive output k and kWinPoints consecutive points:
+// each lane reads its window (start and weights) once and applies it to
+// every point of the run.
+constexpr int kWinPoints = 8;
+constexpr int kWinMaxWidth = 8;
+
+template <int W>
 __global__ void __launch_bounds__(kBlock)
-kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int kout, long long count,
-            const int* __restrict__ ks, const float* __restrict__ kw, int w) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += stride) {
-        const int ko = (int)(e % kout);
-        const long long p = e / kout;
-        const float* col = in + p * kin + ks[ko];
-        float acc = 0.f;
-        for (int u = 0; u < w; ++u) acc = __builtin_fmaf(kw[(size_t)ko * w + u], col[u], acc);
-        out[e] = acc;
+kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int kout, long long npts,
+            const int* __restrict__ ks, const float* __restrict__ kw, int w, int nkc, long long nitems) {
+    const long long item = (long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (item >= nitems) return;
+    const int kc = (int)(item % nkc);
+    const long long p0 = (item / nkc) * kWinPoints;
+    const int ko = kc * kWave + (threadIdx.x & (kWave - 1));
+    if (ko >= kout) return;
+    const int s0 = ks[ko];
+    if constexpr (W == 0) {   // any width: the weights are read per point
+        for (int q = 0; q < kWinPoints; ++q) {
+            const long long p = p0 + q;
+            if (p < npts) {
+                const float* col = in + p * kin + s0;
+                float acc = 0.f;
+                for (int u = 0; u < w; ++u) acc = __builtin_fmaf(kw[(size_t)ko * w + u], col[u], acc);
+                out[p * kout + ko] = acc;
+            }
+        }
+    } else {
+        float c[W];
+#pragma unroll
+        for (int u = 0; u < W; ++u) c[u] = u < w ? kw[(size_t)ko * w + u] : 0.f;
+#pragma unroll
+        for (int q = 0; q < kWinPoints; ++q) {
+            const long long p = p0 + q;
+            if (p < npts) {
+                const float* col = in + p * kin + s0;
+                float acc = 0.f;
+#pragma unroll
+                for (int u = 0; u < W; ++u)
+                    if (u < w) acc = __builtin_fmaf(c[u], col[u], acc);
+                out[p * kout + ko] = acc;
+            }
+        }
     }
 }
 
@@ -604,11 +699,40 @@ hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st) {
+    const int ntx = (a.nout_x + 7) / 8, nty = (a.nout_y + 7) / 8;
+    const long long nkc = (a.K + kWave - 1) / kWave;
+    const long long nitems = nkc * ntx * nty;
+    const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (nblocks >= (1ll << 31) || a.SY < 1 || a.SY > a.nin_y || a.SX > a.nin_x) return hipErrorInvalidValue;
+    const dim3 g((unsigned)nblocks), b(kBlock);
+    switch (a.SX) {
+        case 8: hipLaunchKernelGGL(xfer_tile_kernel<8>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 12: hipLaunchKernelGGL(xfer_tile_kernel<12>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 16: hipLaunchKernelGGL(xfer_tile_kernel<16>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 20: hipLaunchKernelGGL(xfer_tile_kernel<20>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 24: hipLaunchKernelGGL(xfer_tile_kernel<24>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 32: hipLaunchKernelGGL(xfer_tile_kernel<32>, g, b, 0, st, a, ntx, nty, nitems); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,
                        const float* kw, int w, hipStream_t st) {
-    const long long count = (long long)kout * npts;
-    hipLaunchKernelGGL(kwin_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, in, out, kin, kout,
-                       count, ks, kw, w);
+    if (w < 1) return hipErrorInvalidValue;
+    const int nkc = (kout + kWave - 1) / kWave;
+    const long long nitems = (long long)nkc * ((npts + kWinPoints - 1) / kWinPoints);
+    const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (nblocks >= (1ll << 31)) return hipErrorInvalidValue;
+    const dim3 g((unsigned)nblocks), b(kBlock);
+    if (w <= 4)
+        hipLaunchKernelGGL(kwin_kernel<4>, g, b, 0, st, in, out, kin, kout, npts, ks, kw, w, nkc, nitems);
+    else if (w <= kWinMaxWidth)
+        hipLaunchKernelGGL(kwin_kernel<kWinMaxWidth>, g, b, 0, st, in, out, kin, kout, npts, ks, kw, w, nkc,
+                           nitems);
+    else
+        hipLaunchKernelGGL(kwin_kernel<0>, g, b, 0, st, in, out, kin, kout, npts, ks, kw, w, nkc, nitems);
     return hipGetLastError();
 }
 

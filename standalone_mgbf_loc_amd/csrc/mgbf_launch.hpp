@@ -133,6 +133,23 @@ struct XferArgs {
     int wy;
 };
 hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st);
+// The same separable transfer by output tiles of 8 x 8 points: tile (p, q)
+// reads the input block x0[p] .. x0[p]+SX-1 by y0[q] .. y0[q]+SY-1 and
+// applies the dense tile weights wx[p][t][c] (8 x SX) and wy[q][t][r]
+// (8 x SY), zero outside each output's window. SX in {8, 12, 16, 20, 24, 32}.
+struct XferTileArgs {
+    const float* in;
+    float* out;
+    int K;
+    int nin_x, nin_y, nout_x, nout_y;
+    const int* x0;
+    const float* wx;
+    int SX;
+    const int* y0;
+    const float* wy;
+    int SY;
+};
+hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st);
 // Window transfer along the stacked k of every point:
 // out(ko,p) = sum_u kw[ko][u] in(ks[ko]+u, p)
 hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,

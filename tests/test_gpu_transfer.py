@@ -4,6 +4,8 @@ horizontal interpolations between a coarser filter grid and the analysis
 grid, the linear vertical map of the _all2 routines (odd lm_a <= 2 lm - 1)
 and the linear or lwq map of the _all routines, their transposes, and the
 driver's whole sequence on an analysis field."""
+import os
+
 import numpy as np
 import pytest
 
@@ -102,7 +104,8 @@ FAMILIES = {"all2": (M.OP_F2A, M.OP_A2F, O.VLIN, {}),
 @pytest.mark.parametrize("family,grid", [
     ("all2", (32, 24, 4, 2, 48, 36, 7)), ("all2", (32, 32, 5, 1, 64, 64, 7)),
     ("all_lin", (32, 24, 4, 2, 48, 36, 7)), ("all_lwq", (32, 24, 4, 2, 48, 36, 7)),
-    ("all_lwq", (32, 32, 5, 1, 64, 64, 12)), ("all_lwq", (48, 32, 3, 3, 72, 48, 3))])
+    ("all_lwq", (32, 32, 5, 1, 64, 64, 12)), ("all_lwq", (48, 32, 3, 3, 72, 48, 3)),
+    ("all_lwq", (32, 32, 3, 1, 64, 64, 40))])   # adjoint windows of 40 levels
 def test_every_transfer_variant_matches_oracle(scheme, family, grid):
     nx, ny, lm, n_ens, na, ma, lm_a = grid
     f2a, a2f, vmap, flags = FAMILIES[family]
@@ -155,3 +158,34 @@ def test_the_other_routine_family_refuses_what_it_cannot_map():
         torch.cuda.synchronize()
     finally:
         op.close()
+
+
+@pytest.mark.parametrize("scheme,grid", [(O.LSQR, (32, 24, 4, 2, 48, 36, 7)), (O.LIN, (32, 32, 5, 1, 64, 64, 5)),
+                                         (O.QUAD, (48, 32, 3, 3, 72, 48, 5)), (O.LSQR, (32, 32, 6, 1, 96, 96, 6))])
+def test_tiled_transfer_matches_the_gather(scheme, grid):
+    # the 8 x 8 tile kernel (dense tile blocks) against the per-point gather
+    # (MGBF_XFER_GATHER=1) and the oracle, both directions; ratios 1.5, 2, 3
+    nx, ny, lm, n_ens, na, ma, lm_a = grid
+    out = {}
+    for gather in (False, True):
+        os.environ["MGBF_XFER_GATHER"] = "1" if gather else "0"
+        try:
+            cfg, op, P = setup(nx, ny, lm, n_ens, na, ma, lm_a, **SCHEMES[scheme])
+        finally:
+            os.environ.pop("MGBF_XFER_GATHER", None)
+        try:
+            d = op.dims
+            rng = np.random.default_rng(31 + na)
+            vf = rng.standard_normal(op.field_shape()).astype(np.float32)
+            wa = rng.standard_normal(d.anal_shape()).astype(np.float32)
+            a = op.filt_to_anal(torch.from_numpy(vf).to(DEV))
+            b = op.anal_to_filt(torch.from_numpy(wa).to(DEV))
+            torch.cuda.synchronize()
+            args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
+            assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), scheme=scheme)) <= TOL
+            assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), scheme=scheme)) <= TOL
+            out[gather] = (a.cpu(), b.cpu())
+        finally:
+            op.close()
+    assert relerr(out[False][0], out[True][0].double().numpy()) <= 1e-6
+    assert relerr(out[False][1], out[True][1].double().numpy()) <= 1e-6
