@@ -33,7 +33,7 @@ namespace mgbf {
 
 namespace {
 
-template <int H, int HZ, bool BALL, bool T, bool ZL>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO>
 __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     constexpr int TI = 4;
     constexpr int NR = 2 * H + 1;
@@ -73,6 +73,46 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     for (int s = 0; s < NR; ++s)
 #pragma unroll
         for (int t = 0; t < TI; ++t) acc[s][t] = 0.f;
+
+    // PRO: the prolongation of the next coarser generation (direct1 on the
+    // half-sample mirrored coarse field) added to the output, as the radial
+    // kernel does: a window of 4 x-interpolated coarse rows slides down with
+    // the output rows, the next raw coarse row loaded one step ahead
+    constexpr int NCC = TI / 2 + 4;
+    float XV[4][TI];
+    float cvn[NCC];
+    int csoff[NCC];
+    int cbase = 0;
+    const int nc = n / 2, mc = m / 2;
+    const size_t crow = (size_t)nc * K;
+    auto load_craw = [&](int r, float* dst) {
+        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(r, mc) * crow);
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) dst[c] = buf_ld(rc, vk, csoff[c]);
+    };
+    auto xinterp = [&](const float* cv, float* dst) {
+#pragma unroll
+        for (int t = 0; t < TI; ++t) {
+            if ((t & 1) == 0)
+                dst[t] = kQc[0] * cv[t / 2] + kQc[1] * cv[t / 2 + 1] + kQc[2] * cv[t / 2 + 2] +
+                         kQc[3] * cv[t / 2 + 3];
+            else
+                dst[t] = kPc[0] * cv[(t + 1) / 2] + kPc[1] * cv[(t + 1) / 2 + 1] +
+                         kPc[2] * cv[(t + 1) / 2 + 2] + kPc[3] * cv[(t + 1) / 2 + 3];
+        }
+    };
+    if constexpr (PRO) {
+        const int v0 = i0 / 2;
+#pragma unroll
+        for (int c = 0; c < NCC; ++c) csoff[c] = mirror_half(v0 - 2 + c, nc) * K * 4;
+        cbase = ((j0 + 1) >> 1) - 2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            load_craw(cbase + u, cvn);
+            xinterp(cvn, XV[u]);
+        }
+        load_craw(cbase + 4, cvn);
+    }
 
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
     const int nsteps = tj_eff + 2 * H;
@@ -171,6 +211,24 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
 #pragma unroll
                         for (int t = 0; t < TI; ++t) o[t] = p.wconst * acc[(s + slot_off) % NR][t];
                     }
+                    if constexpr (PRO) {
+                        const int nb = ((j + 1) >> 1) - 2;
+                        if (nb != cbase) {   // the window moves down one coarse row
+#pragma unroll
+                            for (int u = 0; u < 3; ++u)
+#pragma unroll
+                                for (int t = 0; t < TI; ++t) XV[u][t] = XV[u + 1][t];
+                            xinterp(cvn, XV[3]);
+                            load_craw(nb + 4, cvn);
+                            cbase = nb;
+                        }
+                        const bool oj = (j & 1) != 0;
+                        const float y0 = oj ? kPc[0] : kQc[0], y1 = oj ? kPc[1] : kQc[1];
+                        const float y2 = oj ? kPc[2] : kQc[2], y3 = oj ? kPc[3] : kQc[3];
+#pragma unroll
+                        for (int t = 0; t < TI; ++t)
+                            o[t] += y0 * XV[0][t] + y1 * XV[1][t] + y2 * XV[2][t] + y3 * XV[3][t];
+                    }
                     if (kvalid) {
                         const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + rowoff);
 #pragma unroll
@@ -185,11 +243,12 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     }
 }
 
-template <int H, int HZ, bool BALL, bool T, bool ZL>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO>
 hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     Ball3Params p{};
     p.in = a.in;
     p.out = a.out;
+    p.coarse = a.coarse;
     p.wfield = a.wfield;
     p.wconst = a.wconst;
     p.K = a.K;
@@ -206,21 +265,26 @@ hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL, PRO>), dim3(nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
 template <int H, int HZ>
 hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
+    const bool pro = a.coarse != nullptr;
+    if (pro && (a.transpose || (a.n & 1) || (a.m & 1))) return hipErrorInvalidValue;
     if (a.zero_levels) {
         // zeros beyond the column ends: Z_c^T = Z_c, the box of the line products
         if (a.ball) return hipErrorInvalidValue;
-        return launch_variant<H, HZ, false, false, true>(a, st);
+        return pro ? launch_variant<H, HZ, false, false, true, true>(a, st)
+                   : launch_variant<H, HZ, false, false, true, false>(a, st);
     }
     if (!a.ball || HZ != H) return hipErrorInvalidValue;
-    if constexpr (HZ == H)
-        return a.transpose ? launch_variant<H, HZ, true, true, false>(a, st)
-                           : launch_variant<H, HZ, true, false, false>(a, st);
+    if constexpr (HZ == H) {
+        if (a.transpose) return launch_variant<H, HZ, true, true, false, false>(a, st);
+        return pro ? launch_variant<H, HZ, true, false, false, true>(a, st)
+                   : launch_variant<H, HZ, true, false, false, false>(a, st);
+    }
     return hipErrorInvalidValue;
 }
 

@@ -633,16 +633,22 @@ bool plan_ball3(const mgbf::Setup& s, const mgbf_config& cfg, mgbf::Ball3Coef& c
 // The 3-D filter of generation g (F, or F^T with transpose). With `scaled`
 // given, the output is also multiplied by the generation's weights when the
 // streaming pass covers the whole stack, and *scaled says whether it was.
+// coarse (forward only, when ball3_pro(c)): the prolongation of generation
+// g+1 is added to the output, s_g = F in + P s_{g+1}.
 int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st,
-         bool* scaled = nullptr) {
-    Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st, 8.0 * (double)c->gsize[g]);
+         bool* scaled = nullptr, const float* coarse = nullptr) {
+    Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st,
+             8.0 * (double)c->gsize[g] + (coarse ? 4.0 * (double)c->gsize[g + 1] : 0.0));
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     if (scaled) *scaled = false;
+    if (coarse && (transpose || !c->ball3 || K3 < c->K))
+        return fail(MGBF_ERR_ARG, "internal: prolongation fused into a 3-D pass that cannot take it");
     if (c->ball3) {
         const bool fuse = scaled && K3 >= c->K;
         mgbf::Ball3Args b{};
         b.in = in;
         b.out = out;
+        b.coarse = coarse;
         b.wfield = fuse ? c->wfield[g] : nullptr;
         b.wconst = fuse ? c->wconst[g] : 1.f;
         if (fuse) *scaled = true;
@@ -691,10 +697,26 @@ int rad3_upsweep(mgbf_ctx* c, float* U, float* y, hipStream_t st) {
     return MGBF_OK;
 }
 
+// The streaming 3-D pass takes the prolongation of the up-sweep in its
+// epilogue when it covers the whole stack and the transfer is the 4-point one.
+bool ball3_pro(const mgbf_ctx* c) {
+    return c->ball3 && !c->quad && c->s.dims.km3_all * c->s.cfg.lm >= c->K;
+}
+
+// Forward passes from the coarsest generation up, each adding the
+// prolongation of the one below: s_g = F T_g + P s_{g+1} (into U, y at g = 0).
+int rad3_forward_upsweep(mgbf_ctx* c, const float* T, float* U, float* y, hipStream_t st) {
+    for (int g = c->gm - 1; g >= 0; --g)
+        CALL(ball(c, g, T + c->goff[g], g == 0 ? y : U + c->goff[g], false, st, nullptr,
+                  g + 1 < c->gm ? U + c->goff[g + 1] : nullptr));
+    return MGBF_OK;
+}
+
 int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
     CALL(restrict_chain(c, x, U, st));
+    const bool pro = ball3_pro(c);
     for (int g = 0; g < c->gm; ++g) {
         bool scaled = false;
         CALL(ball(c, g, g == 0 ? x : U + c->goff[g], T + c->goff[g], true, st, &scaled));
@@ -702,8 +724,9 @@ int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
             HIP_TRY(mgbf::launch_scale(T + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 0,
                                        c->gsize[g], st),
                     "scale launch");
-        CALL(ball(c, g, T + c->goff[g], U + c->goff[g], false, st));
+        if (!pro) CALL(ball(c, g, T + c->goff[g], U + c->goff[g], false, st));
     }
+    if (pro) return rad3_forward_upsweep(c, T, U, y, st);
     return rad3_upsweep(c, U, y, st);
 }
 
@@ -722,6 +745,16 @@ int apply_wt_rad3(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipS
 int apply_w_rad3(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
+    if (ball3_pro(c)) {
+        if (sqrt_weights) {
+            for (int g = 0; g < c->gm; ++g)
+                HIP_TRY(mgbf::launch_scale(v + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 1,
+                                           c->gsize[g], st),
+                        "scale launch");
+            return rad3_forward_upsweep(c, T, U, y, st);
+        }
+        return rad3_forward_upsweep(c, v, U, y, st);
+    }
     for (int g = 0; g < c->gm; ++g) {
         const float* src = v + c->goff[g];
         if (sqrt_weights) {

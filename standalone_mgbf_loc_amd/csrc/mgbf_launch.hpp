@@ -193,6 +193,7 @@ struct Ball3Coef {
 struct Ball3Args {
     const float* in;
     float* out;
+    const float* coarse;   // next coarser generation to prolong and add (forward only), or nullptr
     const float* wfield;   // per-point scale of the output, or nullptr
     float wconst;          // constant scale when wfield is nullptr
     int K, n, m, lm, K3;   // only k < K3 is written
@@ -203,6 +204,7 @@ struct Ball3Args {
 struct Ball3Params {
     const float* in;
     float* out;
+    const float* coarse;
     const float* wfield;
     float wconst;
     int K, n, m, TJ, lm, K3;
