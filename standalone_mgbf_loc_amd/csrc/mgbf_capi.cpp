@@ -297,7 +297,7 @@ bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int 
 // rows j0-H .. j0+TJ-1+H and owns coarse rows j0/2 .. (j0+TJ)/2-1, whose
 // 8-row windows must lie inside what it loads.
 bool fused_ok(const mgbf_ctx* c, int g, int tj) {
-    if (g + 1 >= c->gm || !c->res_ok[g] || c->s.H < 4 || tj < 2 || (tj & 1)) return false;
+    if (g + 1 >= c->gm || !c->res_ok[g] || c->s.H < 3 || tj < 2 || (tj & 1)) return false;
     const int m = c->ny[g], H = c->s.H;
     const std::vector<int>& st = c->s.ry[g].start;
     for (int j0 = 0; j0 < m; j0 += tj) {
@@ -313,7 +313,8 @@ bool fused_ok(const mgbf_ctx* c, int g, int tj) {
 bool can_fuse_band(const mgbf_ctx* c, int which) {
     const int hb = c->band_hb[which];
     const int lm = c->s.cfg.lm;
-    return c->nfilt > 0 && c->s.exact_disc && c->gm > 1 && (64 % lm) == 0 && (c->K % 64) == 0 &&
+    return c->nfilt > 0 && (c->s.exact_disc || c->s.separable) && c->gm > 1 && (64 % lm) == 0 &&
+           (c->K % 64) == 0 &&
            (hb == 3 || hb == 4 || hb == 6 || hb == 8);
 }
 
@@ -356,6 +357,7 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
         a.wexp = c->wexp;
     }
     a.exact_disc = c->s.exact_disc;
+    a.separable = c->s.separable;
     a.coef = c->coef;
     HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
     return MGBF_OK;
@@ -968,6 +970,12 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     for (int b = 0; b <= mgbf::kMaxHaloDev; ++b)
         for (int a = 0; a <= mgbf::kMaxHaloDev; ++a)
             c->coef.w[b][a] = (b <= mgbf::kMaxHalo && a <= mgbf::kMaxHalo) ? (float)s.wsym[b][a] : 0.f;
+    if (s.separable) {
+        // the separable kernels read the two factors from rows 0 and 1
+        c->coef = mgbf::RadialCoef{};
+        for (int a = 0; a <= s.H; ++a) c->coef.w[0][a] = (float)s.sepx[a];
+        for (int b = 0; b <= s.H; ++b) c->coef.w[1][b] = (float)s.sepy[b];
+    }
 
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || device < 0 || device >= dev_count) {
@@ -1355,7 +1363,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
             if ((kind == 1 || kind == 2) && c->quad) continue;   // never run with lquart
             if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
-            if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 4)) continue;
+            if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 3)) continue;
             const float* coarse = kind == 1 || kind == 2 ? c->ws_u + c->goff[g + 1] : nullptr;
             float* rout = kind == 3 ? c->ws_u + c->goff[g + 1] : nullptr;
             const int band = kind == 2 ? vband : -1;

@@ -1001,6 +1001,20 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err) {
             }
         s.exact_disc = exact;
     }
+    // a product w(a,b) = fx(a) fy(b) (the fast and line procedures' taps)?
+    // The separable kernels then sum the x-pairs once per row and scale by
+    // fy(b) per output row.
+    if (!s.exact_disc && s.wsym[0][0] != 0.0) {
+        bool sep = true;
+        for (int a = 0; a <= s.H; ++a) s.sepx[a] = s.wsym[0][a];
+        for (int b = 0; b <= s.H; ++b) s.sepy[b] = s.wsym[b][0] / s.wsym[0][0];
+        for (int b = 0; b <= s.H && sep; ++b)
+            for (int a = 0; a <= s.H; ++a) {
+                const double prod = s.sepx[a] * s.sepy[b];
+                if (std::fabs(s.wsym[b][a] - prod) > 1e-12 * std::fabs(s.wsym[0][0])) sep = false;
+            }
+        s.separable = sep;
+    }
     // vertical band matrices (sup_vrbeta1_ens / sup_vrbeta1T_ens)
     {
         const int lm = cfg.lm, hz = cfg.hz, W = 2 * hz + 1;

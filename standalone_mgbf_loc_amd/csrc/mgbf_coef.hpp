@@ -48,6 +48,8 @@ struct Setup {
     int H = 0;                          // max(hx, hy) used by the radial kernels
     double wsym[kMaxHalo + 1][kMaxHalo + 1] = {};  // w(|a|,|b|) as [b][a]
     bool exact_disc = false;            // taps = lattice disc a^2+b^2 < (H+1)^2
+    bool separable = false;             // w(a,b) = sepx[a] sepy[b] (not a disc)
+    double sepx[kMaxHalo + 1] = {}, sepy[kMaxHalo + 1] = {};
     std::vector<double> vf, vft;       // lm x (2hz+1) band of Vf and Vf^T
     std::vector<double> vvt;           // lm x (4hz+1) band of Vf Vf^T
     AxisTable rx[kMaxGen], ry[kMaxGen];   // restriction g -> g+1 (coarse rows)

@@ -33,12 +33,13 @@ struct RadialArgs {
     const void* afrag;      // FP16 hi/lo lane fragments of the weight bands
     int wexp;               // power-of-two exponent applied to the weights
     int TJ;               // output rows per wavefront
-    // restriction to the next generation fused into the pass (TI = 4, H >= 4)
+    // restriction to the next generation fused into the pass (TI = 4, H >= 3)
     float* rout;            // coarse output, or nullptr
     const float4* rytab;    // per fine row: weights into the 6 open coarse rows (2 x float4)
     const int2* rymeta;     // per fine row: lowest open coarse row, rows completed
     float* rkmax;           // per-k max |coarse output|, or nullptr
     bool exact_disc;
+    bool separable;       // coef.w[0][a] = fx(a), coef.w[1][b] = fy(b): w(a,b) = fx(a) fy(b)
     RadialCoef coef;
 };
 

@@ -40,11 +40,15 @@ __host__ __device__ constexpr int disc_extent(int b) {
 // The wave owns coarse rows j0/2 .. (j0+TJ)/2 - 1, whose windows lie inside
 // the rows it loads (checked on the host).
 // ---------------------------------------------------------------------------
-template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES, bool SEP = false>
 __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
-    static_assert(!RES || (TI == 4 && H >= 4 && !PRO && VHB == 0), "restriction epilogue shape");
+    static_assert(!RES || (TI == 4 && H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
+    static_assert(!(SEP && DISC), "a separable filter is read as a box");
     constexpr int NR = 2 * H + 1;
-    constexpr int NC = TI + 2 * H;
+    // columns loaded on each side of the tile: H for the filter, 4 for the
+    // restriction's 8-point stencils at the domain edges
+    constexpr int HL = (RES && H < 4) ? 4 : H;
+    constexpr int NC = TI + 2 * HL;
     constexpr int NCC = TI / 2 + 4;   // coarse columns of the tile (TI even)
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -65,7 +69,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     const int vk = kk * 4;
     int soff[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - H + c, n) * K * 4;
+    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - HL + c, n) * K * 4;
 
     float acc[NR][TI];
 #pragma unroll
@@ -163,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[H - 3 + 2 * c + t], a);
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[HL - 3 + 2 * c + t], a);
                                 xr[c] = a;
                             }
                         } else if (xcls == 1) {   // fine columns 0 .. 7
@@ -171,7 +175,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], v[H + t], a);
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], v[HL + t], a);
                                 xr[c] = a;
                             }
                         } else {                  // fine columns n-8 .. n-1
@@ -180,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                                 float a = 0.f;
 #pragma unroll
                                 for (int t = 0; t < 8; ++t)
-                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], v[H - 4 + t], a);
+                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], v[HL - 4 + t], a);
                                 xr[c] = a;
                             }
                         }
@@ -261,9 +265,23 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 #pragma unroll
                 for (int t = 0; t < TI; ++t) {
                     float P[H + 1];
-                    P[0] = v[t + H];
+                    P[0] = v[t + HL];
 #pragma unroll
-                    for (int a = 1; a <= H; ++a) P[a] = v[t + H + a] + v[t + H - a];
+                    for (int a = 1; a <= H; ++a) P[a] = v[t + HL + a] + v[t + HL - a];
+                    if constexpr (SEP) {
+                        // w(a,b) = fx(a) fy(b): one x-sum per row, fy(b) into rows r+-b
+                        float q = p.W.w[0][0] * P[0];
+#pragma unroll
+                        for (int a = 1; a <= H; ++a) q = __builtin_fmaf(p.W.w[0][a], P[a], q);
+#pragma unroll
+                        for (int b = 0; b <= H; ++b) {
+                            acc[(s + b) % NR][t] = __builtin_fmaf(p.W.w[1][b], q, acc[(s + b) % NR][t]);
+                            if (b > 0)
+                                acc[(s + NR - b) % NR][t] =
+                                    __builtin_fmaf(p.W.w[1][b], q, acc[(s + NR - b) % NR][t]);
+                        }
+                        continue;
+                    }
 #pragma unroll
                     for (int b = 0; b <= H; ++b) {
                         const int A = DISC ? disc_extent<H>(b) : H;
@@ -357,7 +375,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     }
 }
 
-template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false, bool SEP = false>
 hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     RadialParams p{};
     p.in = a.in;
@@ -383,7 +401,7 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB, RES>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB, RES, SEP>), dim3(nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
@@ -392,13 +410,26 @@ template <int H, int TI>
 hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     const int vhb = a.band ? a.vhb : 0;
-    if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 4
-        if constexpr (TI == 4 && H >= 4) {
+    if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 3
+        if constexpr (TI == 4 && H >= 3) {
             if (pro || vhb) return hipErrorInvalidValue;
+            if (a.separable) return launch_radial_variant<H, 4, false, false, 0, true, true>(a, st);
             return a.exact_disc ? launch_radial_variant<H, 4, true, false, 0, true>(a, st)
                                 : launch_radial_variant<H, 4, false, false, 0, true>(a, st);
         } else {
             return hipErrorInvalidValue;
+        }
+    }
+    if (a.separable) {
+        if (vhb == 0) return pro ? launch_radial_variant<H, TI, false, true, 0, false, true>(a, st)
+                                 : launch_radial_variant<H, TI, false, false, 0, false, true>(a, st);
+        if (!pro) return hipErrorInvalidValue;
+        switch (vhb) {
+            case 3: return launch_radial_variant<H, TI, false, true, 3, false, true>(a, st);
+            case 4: return launch_radial_variant<H, TI, false, true, 4, false, true>(a, st);
+            case 6: return launch_radial_variant<H, TI, false, true, 6, false, true>(a, st);
+            case 8: return launch_radial_variant<H, TI, false, true, 8, false, true>(a, st);
+            default: return hipErrorInvalidValue;
         }
     }
     if (a.exact_disc) {

@@ -184,7 +184,7 @@ def test_wawt_192x192x32_four_generations_matches_oracle():
 
 
 @pytest.mark.parametrize("n_ens,lm,gm,h,filt_g1", [(3, 16, 3, 6, True), (2, 20, 4, 4, False),
-                                                  (1, 64, 2, 8, True)])
+                                                  (1, 64, 2, 8, True), (2, 24, 4, 3, True)])
 def test_wawt_ensemble_matches_oracle(n_ens, lm, gm, h, filt_g1):
     cfg = M.Config.for_grid(96, 64, lm, n_ens=n_ens, gm=gm, h=h, hz=3,
                             weights=[1.0, 0.7, 0.4, 0.2], l_filt_g1=filt_g1)
