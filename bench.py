@@ -178,14 +178,14 @@ def run_row(args, torch, MG):
         # events on the launch stream around the timed steps: nothing else runs there
         k_ms = e0.elapsed_time(e1)
         k_n, k_bytes = 4 * args.steps, pair_bytes * args.steps
-        kernel = "xfer2d_kernel + kwin_kernel (A2F and F2A, two launches each)"
+        kernel = "xfer_tile_kernel + kwin_kernel (A2F and F2A, two launches each)"
         avg = k_ms / k_n
         achieved = k_bytes / (k_ms / 1e3) / 1e9
     else:
         k_ms = phases["hfiltT"][0] + phases["hfilt"][0]
         k_n = phases["hfiltT"][1] + phases["hfilt"][1]
         k_bytes = phases["hfiltT"][2] + phases["hfilt"][2]
-        kernel = ("ball_kernel (3-D filter on every generation)" if row in ("proc4", "proc5")
+        kernel = ("ball3_kernel (3-D filter, F and F^T on every generation)" if row in ("proc4", "proc5")
                   else "radial_kernel (F and F^T on every generation)")
         avg = k_ms / max(k_n, 1)
         achieved = k_bytes / (k_ms / 1e3) / 1e9
