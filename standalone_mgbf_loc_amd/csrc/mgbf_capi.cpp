@@ -358,6 +358,7 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
     }
     a.exact_disc = c->s.exact_disc;
     a.separable = c->s.separable;
+    a.quad = c->quad;
     a.coef = c->coef;
     HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
     return MGBF_OK;
@@ -460,13 +461,13 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, fl
             bool sqrt_w, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    // the radial kernels fuse the 4-point prolongation only: with the
-    // quadratic one it runs on its own, and the band after it
+    // the radial kernels fuse the 4-point and the quadratic prolongation,
+    // and the band after the 4-point one only
     const bool fuse = band >= 0 && c->fuse >= 2 && !c->quad && can_fuse_band(c, band);
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
-        const float* pro = c->fuse >= 1 && !c->quad ? coarse : nullptr;
+        const float* pro = c->fuse >= 1 ? coarse : nullptr;
         const bool direct = g == 0 && (fuse || band < 0);
         float* dst = direct ? y : U + c->goff[g];
         const int fb = (g == 0 && fuse) ? band : -1;
@@ -1361,8 +1362,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
         const int n = c->nx[g], m = c->ny[g];
         for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
-            if ((kind == 1 || kind == 2) && c->quad) continue;   // never run with lquart
-            if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
+            if (kind == 2 && (c->quad || !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband)))) continue;
             if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 3)) continue;
             const float* coarse = kind == 1 || kind == 2 ? c->ws_u + c->goff[g + 1] : nullptr;
             float* rout = kind == 3 ? c->ws_u + c->goff[g + 1] : nullptr;

@@ -40,6 +40,7 @@ struct RadialArgs {
     float* rkmax;           // per-k max |coarse output|, or nullptr
     bool exact_disc;
     bool separable;       // coef.w[0][a] = fx(a), coef.w[1][b] = fy(b): w(a,b) = fx(a) fy(b)
+    bool quad;            // the prolongation epilogue uses the quadratic transfer (lquart)
     RadialCoef coef;
 };
 

@@ -40,8 +40,9 @@ __host__ __device__ constexpr int disc_extent(int b) {
 // The wave owns coarse rows j0/2 .. (j0+TJ)/2 - 1, whose windows lie inside
 // the rows it loads (checked on the host).
 // ---------------------------------------------------------------------------
-template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES, bool SEP = false>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES, bool SEP = false, bool QP = false>
 __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
+    static_assert(!QP || (PRO && VHB == 0), "the quadratic prolongation epilogue has no band");
     static_assert(!RES || (TI == 4 && H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
     static_assert(!(SEP && DISC), "a separable filter is read as a box");
     constexpr int NR = 2 * H + 1;
@@ -90,7 +91,20 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 #pragma unroll
         for (int c = 0; c < NCC; ++c) dst[c] = buf_ld(rc, vk, csoff[c]);
     };
+    // QP: the quadratic transfer (lquart): fine index i reads coarse
+    // (i>>1)-1 .. (i>>1)+1 with Stencil<true>::C[i&1]; the window then starts
+    // at (j>>1)-1 and its fourth row is not read
+    constexpr bool quadp = QP;
+    using SQ = Stencil<true>;
+    auto wbase = [&](int j) { return quadp ? (j >> 1) - 1 : ((j + 1) >> 1) - 2; };
     auto xinterp = [&](const float* cv, float* dst) {
+        if constexpr (quadp) {
+#pragma unroll
+            for (int t = 0; t < TI; ++t)
+                dst[t] = SQ::C[t & 1][0] * cv[t / 2 + 1] + SQ::C[t & 1][1] * cv[t / 2 + 2] +
+                         SQ::C[t & 1][2] * cv[t / 2 + 3];
+            return;
+        }
 #pragma unroll
         for (int t = 0; t < TI; ++t) {
             if ((t & 1) == 0)
@@ -105,7 +119,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
         const int v0 = i0 / 2;
 #pragma unroll
         for (int c = 0; c < NCC; ++c) csoff[c] = mirror_half(v0 - 2 + c, nc) * K * 4;
-        cbase = ((j0 + 1) >> 1) - 2;
+        cbase = wbase(j0);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             load_craw(cbase + u, cvn);
@@ -315,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                         for (int t = 0; t < TI; ++t) o[t] = p.wconst * acc[(s + slot_off) % NR][t];
                     }
                     if constexpr (PRO) {
-                        const int nb = ((j + 1) >> 1) - 2;
+                        const int nb = wbase(j);
                         if (nb != cbase) {   // the window moves down one coarse row
 #pragma unroll
                             for (int u = 0; u < 3; ++u)
@@ -331,8 +345,18 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             cbase = nb;
                         }
                         const bool oj = (j & 1) != 0;
-                        const float y0 = oj ? kPc[0] : kQc[0], y1 = oj ? kPc[1] : kQc[1];
-                        const float y2 = oj ? kPc[2] : kQc[2], y3 = oj ? kPc[3] : kQc[3];
+                        float y0, y1, y2, y3;
+                        if constexpr (quadp) {
+                            y0 = oj ? SQ::C[1][0] : SQ::C[0][0];
+                            y1 = oj ? SQ::C[1][1] : SQ::C[0][1];
+                            y2 = oj ? SQ::C[1][2] : SQ::C[0][2];
+                            y3 = 0.f;
+                        } else {
+                            y0 = oj ? kPc[0] : kQc[0];
+                            y1 = oj ? kPc[1] : kQc[1];
+                            y2 = oj ? kPc[2] : kQc[2];
+                            y3 = oj ? kPc[3] : kQc[3];
+                        }
 #pragma unroll
                         for (int t = 0; t < TI; ++t)
                             o[t] += y0 * XV[0][t] + y1 * XV[1][t] + y2 * XV[2][t] + y3 * XV[3][t];
@@ -375,7 +399,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     }
 }
 
-template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false, bool SEP = false>
+template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false, bool SEP = false, bool QP = false>
 hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     RadialParams p{};
     p.in = a.in;
@@ -401,7 +425,8 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB, RES, SEP>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((radial_kernel<H, TI, DISC, PRO, VHB, RES, SEP, QP>), dim3(nblocks), dim3(kBlock), 0, st,
+                       p);
     return hipGetLastError();
 }
 
@@ -410,6 +435,12 @@ template <int H, int TI>
 hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     const int vhb = a.band ? a.vhb : 0;
+    if (a.quad && pro) {   // the quadratic prolongation epilogue, without band
+        if (vhb) return hipErrorInvalidValue;
+        if (a.separable) return launch_radial_variant<H, TI, false, true, 0, false, true, true>(a, st);
+        return a.exact_disc ? launch_radial_variant<H, TI, true, true, 0, false, false, true>(a, st)
+                            : launch_radial_variant<H, TI, false, true, 0, false, false, true>(a, st);
+    }
     if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 3
         if constexpr (TI == 4 && H >= 3) {
             if (pro || vhb) return hipErrorInvalidValue;
