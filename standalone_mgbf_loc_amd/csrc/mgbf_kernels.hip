@@ -552,11 +552,22 @@ __global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, int n
 #pragma unroll
         for (int t = 0; t < TB; ++t) acc[u][t] = 0.f;
     const size_t rowstride = (size_t)a.nin_x * a.K;
+    // the next input row is loaded while the current one is combined
+    float vn[SX];
+    {
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)y0 * rowstride);
+#pragma unroll
+        for (int c = 0; c < SX; ++c) vn[c] = buf_ld(rs, vk, soff[c]);
+    }
     for (int r = 0; r < a.SY; ++r) {
-        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)(y0 + r) * rowstride);
         float v[SX];
 #pragma unroll
-        for (int c = 0; c < SX; ++c) v[c] = buf_ld(rs, vk, soff[c]);
+        for (int c = 0; c < SX; ++c) v[c] = vn[c];
+        if (r + 1 < a.SY) {
+            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)(y0 + r + 1) * rowstride);
+#pragma unroll
+            for (int c = 0; c < SX; ++c) vn[c] = buf_ld(rs, vk, soff[c]);
+        }
         float xr[TB];
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
@@ -616,17 +627,21 @@ kwin_kernel(const float* __restrict__ in, float* __restrict__ out, int kin, int 
         float c[W];
 #pragma unroll
         for (int u = 0; u < W; ++u) c[u] = u < w ? kw[(size_t)ko * w + u] : 0.f;
+        // every load of the run is issued before the first store
+        float x[kWinPoints][W];
 #pragma unroll
         for (int q = 0; q < kWinPoints; ++q) {
-            const long long p = p0 + q;
-            if (p < npts) {
-                const float* col = in + p * kin + s0;
-                float acc = 0.f;
+            const long long p = p0 + q < npts ? p0 + q : npts - 1;
+            const float* col = in + p * kin + s0;
 #pragma unroll
-                for (int u = 0; u < W; ++u)
-                    if (u < w) acc = __builtin_fmaf(c[u], col[u], acc);
-                out[p * kout + ko] = acc;
-            }
+            for (int u = 0; u < W; ++u) x[q][u] = u < w ? col[u] : 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < kWinPoints; ++q) {
+            float acc = 0.f;
+#pragma unroll
+            for (int u = 0; u < W; ++u) acc = __builtin_fmaf(c[u], x[q][u], acc);
+            if (p0 + q < npts) out[(p0 + q) * kout + ko] = acc;
         }
     }
 }
