@@ -84,24 +84,35 @@ def test_gradients_match_fp64_central_differences(filt_g1):
             assert abs(gan[idx] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(gan).max()), (idx, gan[idx], fd)
 
 
-@pytest.mark.parametrize("case", ["rad3", "fast", "lquart", "line_bckg"])
+@pytest.mark.parametrize("case", ["rad3", "fast", "lquart", "line_bckg", "rad3_stream", "lin3_stream"])
 def test_weight_gradients_of_the_other_procedures(case):
-    # procedures 4 (3-D radial), 8 (fast), 6 with lquart and 2 (line): the
-    # weight gradients <(W^T g)_g, (W^T z)_g> against FP64 central differences
+    # procedures 4 (3-D radial), 8 (fast), 6 with lquart and 2 (line), and
+    # procedures 4 and 5 on the streaming 3-D filter (columns of 16 levels):
+    # the weight gradients <(W^T g)_g, (W^T z)_g> against FP64 central
+    # differences
     kw = dict(gm=3, h=4, hz=2)
+    lm = 6
     if case == "rad3":
         kw.update(mgbf_proc=4, km3=2, km2=1)
+    elif case == "rad3_stream":
+        kw.update(mgbf_proc=4, km3=2)
+        lm = 16
+    elif case == "lin3_stream":
+        kw.update(mgbf_proc=5, km3=2, mgbf_line=True, hz=3)
+        lm = 16
     elif case == "fast":
         kw.update(mgbf_proc=8, n_ens=2)
     elif case == "lquart":
         kw.update(n_ens=2, lquart=True)
     else:
         kw.update(mgbf_proc=2, km3=2, km2=1, mgbf_line=True)
-    cfg = M.Config.for_grid(48, 48, 6, **kw)
+    cfg = M.Config.for_grid(48, 48, lm, **kw)
     if case == "fast":
         cfg.mg_ampl02 = 1.0
-    if case == "line_bckg":
+    if case in ("line_bckg", "lin3_stream"):
         cfg.mg_ampl02 = 0.7
+    if case == "lin3_stream":
+        cfg.mg_ampl03 = 0.5
     w0 = [0.9, 0.6, 0.3]
     with M.MGBF(cfg) as op:
         d = op.dims
@@ -110,9 +121,10 @@ def test_weight_gradients_of_the_other_procedures(case):
         J, ga, gw = loss_and_grads(op, torch.from_numpy(a).to(DEV), torch.from_numpy(x).to(DEV),
                                    torch.from_numpy(y).to(DEV), ws)
         P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=4, hy=4,
-                           hz=2, p=2, gm=3, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+                           hz=cfg.hz, p=2, gm=3, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
                            fast=case == "fast", lquart=case == "lquart", line=case == "line_bckg",
-                           rad3=case == "rad3", ampl03=cfg.mg_ampl03)
+                           rad3=case in ("rad3", "rad3_stream"), lin3=case == "lin3_stream",
+                           ampl03=cfg.mg_ampl03)
         J64 = oracle_J(P, a, x, y, w0)
         assert abs(J - J64) <= 1e-5 * abs(J64)
         h = 1e-3

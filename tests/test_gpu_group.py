@@ -141,3 +141,36 @@ def test_uneven_batch_full_size_with_amplitude_fields():
     finally:
         for op in ops:
             op.close()
+
+
+def test_group_of_other_procedures_replays_like_eager():
+    # the streaming 3-D filter (procedures 4 and 5), the separable fast and
+    # line filters and the quadratic transfer captured in one graph: every
+    # replay equals the eager calls bit for bit
+    cfgs = [M.Config.for_grid(64, 48, 16, gm=3, h=4, hz=2, mgbf_proc=4, km3=2),
+            M.Config.for_grid(64, 48, 16, gm=3, h=4, hz=3, mgbf_proc=5, km3=2, mgbf_line=True,
+                              mg_ampl02=0.7, mg_ampl03=0.5),
+            M.Config.for_grid(64, 64, 16, n_ens=2, gm=3, h=6, hz=3, mgbf_proc=8, mg_ampl02=0.4),
+            M.Config.for_grid(64, 64, 16, n_ens=2, gm=3, h=6, hz=2, mgbf_proc=7, mgbf_line=True,
+                              mg_ampl02=0.5),
+            M.Config.for_grid(96, 64, 16, n_ens=2, gm=3, h=4, hz=2, lquart=True)]
+    ops = [M.MGBF(c) for c in cfgs]
+    try:
+        xs = [torch.from_numpy(np.random.default_rng(320 + i).standard_normal(op.field_shape())
+                               .astype(np.float32)).to(DEV) for i, op in enumerate(ops)]
+        eager = [op.filtering_procedure(x) for op, x in zip(ops, xs)]
+        torch.cuda.synchronize()
+        outs = [torch.empty_like(x) for x in xs]
+        with M.Group(ops) as grp:
+            for rep in range(3):
+                for o in outs:
+                    o.fill_(float("nan"))
+                grp.apply(M.OP_WAWT, xs, outs)
+                torch.cuda.synchronize()
+                for y, o in zip(eager, outs):
+                    assert torch.equal(y, o), rep
+                ops[rep % len(ops)].filtering_procedure(xs[rep % len(ops)])
+                torch.cuda.synchronize()
+    finally:
+        for op in ops:
+            op.close()
