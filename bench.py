@@ -98,6 +98,55 @@ ROW_NAMES = {"proc1": "filtering_rad2_bckg", "proc2": "filtering_lin2_bckg (mgbf
              "transfer": "anal_to_filt_all2 + filt_to_anal_all2 (4-point horizontal, 2:1 vertical)"}
 
 
+def _oracle_params(O, cfg, d):
+    """The FP64 oracle's description of a configuration."""
+    bckg = cfg.mgbf_proc in (1, 2, 3, 4, 5)
+    return O.OracleParams(
+        nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx, hy=cfg.hy, hz=cfg.hz,
+        p=cfg.p, gm=d.gm, ampl01=cfg.mg_ampl01, ampl02=cfg.mg_ampl02,
+        l_filt_g1=True if bckg else bool(cfg.l_filt_g1), l_vertical_filter=bool(cfg.l_vertical_filter),
+        fast=cfg.mgbf_proc in (3, 8), lquart=bool(cfg.lquart), line=cfg.mgbf_proc in (2, 7),
+        line_tables=bool(cfg.mgbf_line), final_vft=cfg.mgbf_proc == 7, rad3=cfg.mgbf_proc == 4,
+        ampl03=cfg.mg_ampl03, lin3=cfg.mgbf_proc == 5, weights=cfg.weights()[: d.gm])
+
+
+def row_cpu_baseline(MG, cfg, row):
+    """The FP64 oracle on a bounded sample of a row's workload: one member (or
+    one 3-D variable) of the same filter, on 768x768 (384x384 for the 3-D
+    radial procedure), timed on this host's cores."""
+    import numpy as np
+    from oracle import oracle as O
+    n = 384 if row == "proc4" else 768
+    kw = {k: getattr(cfg, k) for k in ("mgbf_proc", "mgbf_line", "lquart", "mg_ampl01", "mg_ampl02",
+                                       "mg_ampl03", "hx", "hy", "hz", "p", "gm_max")}
+    if row == "transfer":
+        cs = MG.Config.for_grid(n, n, cfg.lm, n_ens=1, gm=cfg.gm_max, weights=cfg.weights(),
+                                nm0=cfg.nm0 * n // (cfg.nxPE * cfg.im_filt),
+                                mm0=cfg.mm0 * n // (cfg.nyPE * cfg.jm_filt), lm_a=cfg.lm_a)
+    else:
+        cs = MG.Config.for_grid(n, n, cfg.lm, n_ens=1, gm=cfg.gm_max, weights=cfg.weights(),
+                                km3=1 if cfg.mgbf_proc <= 5 else cfg.km3)
+        for k, v in kw.items():
+            setattr(cs, k, v)
+    d = MG.derive(cs)
+    P = _oracle_params(O, cs, d)
+    x = np.random.default_rng(1).standard_normal(P.gen_shape(0))
+    t0 = time.perf_counter()
+    if row == "transfer":
+        args = (P, d.na, d.ma, cs.nxPE, cs.nyPE, cs.lm_a)
+        w = O.filt_to_anal(*args, x)
+        O.anal_to_filt(*args, w)
+        units = d.na * d.ma * d.km_a_all
+        what = f"one member: {n}x{n}x{cs.lm} <-> {d.na}x{d.ma}x{cs.lm_a}"
+    else:
+        O.wawt(P, x)
+        units = d.nx * d.ny * d.km_all
+        what = f"one {'variable' if cfg.mgbf_proc <= 5 else 'member'} on {n}x{n}x{cs.lm}"
+    tc = time.perf_counter() - t0
+    return {"value": units / tc, "unit": "gridpoint·members/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{what}, FP64 C restatement with OpenMP, {tc:.2f} s"}
+
+
 def run_row(args, torch, MG):
     row = args.row
     weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
@@ -206,8 +255,11 @@ def run_row(args, torch, MG):
         "cpu_baseline": None,
         "phases": phase_out,
     }
-    print(json.dumps(out), flush=True)
     op.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = row_cpu_baseline(MG, cfg, row)
+        log(f"[bench] cpu: {out['cpu_baseline']['sample']}")
+    print(json.dumps(out), flush=True)
 
 
 def main():
