@@ -3,7 +3,8 @@
     python tests/golden/make_golden.py
 
 Each fixture holds seeded inputs, the configuration and the FP64 outputs of
-W A W^T, W^T and the single stages at small sizes. They are regression
+W A W^T, W^T and the single stages at small sizes; the xfer_* fixtures hold
+both directions of the analysis <-> filter transfer. They are regression
 vectors: the oracle itself is pinned by tests/test_oracle.py. The json file
 holds the oracle's parameters, the namelist values the library is built
 from ("cfg", on top of Config.for_grid) and, for the stages, whether the
@@ -59,7 +60,40 @@ def config(nx, ny, n_ens, lm, gm, h, hz, weights, l_filt_g1=True, lquart=False, 
     return c, cfg
 
 
+# analysis <-> filter transfer: filter grid (nx, ny) with n_ens members of lm
+# levels, analysis grid (na, ma) with lm_a levels, over a pe x pe PE grid;
+# horizontal scheme and vertical map as in oracle.filt_to_anal, and the
+# namelist values that select them
+XFER_CASES = {
+    "xfer_lin_all2_32x24_to_48x36": dict(nx=32, ny=24, n_ens=2, lm=4, na=48, ma=36, lm_a=7, pe=4,
+                                         scheme=O.LIN, vmap=O.VLIN, cfg=dict(l_lin_horizontal=True)),
+    "xfer_quad_lwq_32x32_to_64x64": dict(nx=32, ny=32, n_ens=1, lm=5, na=64, ma=64, lm_a=12, pe=4,
+                                         scheme=O.QUAD, vmap=O.VLWQ,
+                                         cfg=dict(l_quad_horizontal=True, l_new_map=False)),
+    "xfer_lsqr_lwq_48x32_to_72x48": dict(nx=48, ny=32, n_ens=3, lm=4, na=72, ma=48, lm_a=10, pe=4,
+                                         scheme=O.LSQR, vmap=O.VLWQ, cfg=dict(l_new_map=False)),
+}
+
+
+def transfer_main():
+    for seed, (name, a) in enumerate(XFER_CASES.items()):
+        K = a["n_ens"] * a["lm"]
+        P = O.OracleParams(nx=a["nx"], ny=a["ny"], K=K, lm=a["lm"], nvcol=a["n_ens"], hx=3, hy=3, hz=1,
+                           p=2, gm=3)
+        rng = np.random.default_rng(2000 + seed)
+        x = rng.standard_normal(P.gen_shape(0)).astype(np.float32).astype(np.float64)
+        w = rng.standard_normal((a["ma"], a["na"], a["n_ens"] * a["lm_a"])).astype(np.float32).astype(np.float64)
+        args = (P, a["na"], a["ma"], a["pe"], a["pe"], a["lm_a"])
+        kw = dict(scheme=a["scheme"], vmap=a["vmap"])
+        out = dict(x=x, w=w, f2a=O.filt_to_anal(*args, x, **kw), a2f=O.anal_to_filt(*args, w, **kw))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        with open(os.path.join(HERE, name + ".json"), "w") as f:
+            json.dump(a, f, indent=1)
+        print(name, {k: v.shape for k, v in out.items()})
+
+
 def main():
+    transfer_main()
     for seed, (name, args) in enumerate(CASES.items()):
         c, cfg = config(**args)
         P = O.OracleParams(**{k: v for k, v in c.items() if k != "n_ens"})

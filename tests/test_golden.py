@@ -11,7 +11,9 @@ import pytest
 from oracle import oracle as O
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+_ALL = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(HERE, "*.npz")))
+CASES = [c for c in _ALL if not c.startswith("xfer_")]
+XFER = [c for c in _ALL if c.startswith("xfer_")]
 
 
 def _load(name):
@@ -26,8 +28,48 @@ def _oparams(c):
 
 def test_fixtures_present():
     # radial (ensemble, no generation-1 filter, single field), lquart, line
-    # procedures 7 and 2, and the 3-D procedures 4 and 5
+    # procedures 7 and 2, and the 3-D procedures 4 and 5; the transfer with
+    # the linear, quadratic and 4-point schemes and both vertical maps
     assert len(CASES) == 8
+    assert len(XFER) == 3
+
+
+def _xfer_params(c):
+    P = O.OracleParams(nx=c["nx"], ny=c["ny"], K=c["n_ens"] * c["lm"], lm=c["lm"], nvcol=c["n_ens"], hx=3,
+                       hy=3, hz=1, p=2, gm=3)
+    return P, (P, c["na"], c["ma"], c["pe"], c["pe"], c["lm_a"]), dict(scheme=c["scheme"], vmap=c["vmap"])
+
+
+@pytest.mark.parametrize("name", XFER)
+def test_oracle_reproduces_transfer_golden(name):
+    c, g = _load(name)
+    _, args, kw = _xfer_params(c)
+    for key, val in (("f2a", O.filt_to_anal(*args, g["x"], **kw)), ("a2f", O.anal_to_filt(*args, g["w"], **kw))):
+        assert np.max(np.abs(val - g[key])) <= 1e-12 * np.max(np.abs(g[key])), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", XFER)
+def test_hip_transfer_matches_golden(name):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from standalone_mgbf_loc_amd import mgbf as M
+    c, g = _load(name)
+    cfg = M.Config.for_grid(c["nx"], c["ny"], c["lm"], n_ens=c["n_ens"], gm=3, h=3, hz=1, nm0=c["na"],
+                            mm0=c["ma"], lm_a=c["lm_a"], **c["cfg"])
+    assert cfg.nxPE == c["pe"] and cfg.nyPE == c["pe"]
+    new_map = cfg.l_new_map
+    with M.MGBF(cfg) as op:
+        x = torch.from_numpy(g["x"].astype(np.float32)).to("cuda:0")
+        w = torch.from_numpy(g["w"].astype(np.float32)).to("cuda:0")
+        got = {"f2a": op.apply(M.OP_F2A if new_map else M.OP_F2A_ALL, x),
+               "a2f": op.apply(M.OP_A2F if new_map else M.OP_A2F_ALL, w)}
+        torch.cuda.synchronize()
+        for key, y in got.items():
+            ref = g[key]
+            err = np.max(np.abs(y.cpu().double().numpy().reshape(ref.shape) - ref)) / np.max(np.abs(ref))
+            assert err <= 1e-5, (key, err)
 
 
 @pytest.mark.parametrize("name", CASES)
