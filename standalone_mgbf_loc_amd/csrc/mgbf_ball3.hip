@@ -33,11 +33,15 @@ namespace mgbf {
 
 namespace {
 
-template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false>
 __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
+    static_assert(!RES || (H >= 3 && !PRO), "restriction epilogue shape");
     constexpr int TI = 4;
     constexpr int NR = 2 * H + 1;
-    constexpr int NC = TI + 2 * H;
+    // columns loaded on each side: H for the filter, 4 for the restriction's
+    // 8-point stencils at the domain edges
+    constexpr int HL = (RES && H < 4) ? 4 : H;
+    constexpr int NC = TI + 2 * HL;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
@@ -66,7 +70,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     const int vk = kk * 4;
     int soff[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - H + c, n) * K * 4;
+    for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - HL + c, n) * K * 4;
 
     float acc[NR][TI];
 #pragma unroll
@@ -116,6 +120,30 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
 
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
     const int nsteps = tj_eff + 2 * H;
+
+    // RES: the restriction of the input to the next generation rides on the
+    // loaded rows, as in the radial kernel: each fine row restricted in x
+    // over the wave's 2 coarse columns, then added into a shift register of
+    // the open coarse rows (4 inside, up to 6 next to an edge; per-row
+    // weights from a table there); a coarse row is stored when it closes
+    constexpr int NRC = RES ? TI / 2 : 1;
+    constexpr int NQ = 6;
+    float racc[NQ][NRC];
+    int xcls = 0;   // 0 interior, 1 left edge, 2 right edge
+    const int Jlo = j0 / 2, Jhi = (j0 + tj_eff) / 2, nc2 = n / 2;
+    if constexpr (RES) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int c = 0; c < NRC; ++c) racc[q][c] = 0.f;
+        xcls = i0 == 0 ? 1 : (i0 + TI >= n ? 2 : 0);
+    }
+    auto store_coarse = [&](int J, int q) {
+        const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
+#pragma unroll
+        for (int c = 0; c < NRC; ++c)
+            if (i0 / 2 + c < nc2) buf_st(racc[q][c], ro, vk, c * K * 4);
+    };
     for (int rb = 0; rb < nsteps; rb += NR) {
 #pragma unroll
         for (int s = 0; s < NR; ++s) {
@@ -147,6 +175,84 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                             X[c][col] = u + d;
                     }
                 }
+                if constexpr (RES) {
+                    const int vr = j0 - H + rr;   // fine row, before mirroring
+                    if (vr >= 0 && vr < m) {
+                        float xr[NRC];
+                        if (xcls == 0) {          // fine columns 2I-3 .. 2I+4
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], X[0][HL - 3 + 2 * c + t], a);
+                                xr[c] = a;
+                            }
+                        } else if (xcls == 1) {   // fine columns 0 .. 7
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], X[0][HL + t], a);
+                                xr[c] = a;
+                            }
+                        } else {                  // fine columns n-8 .. n-1
+#pragma unroll
+                            for (int c = 0; c < NRC; ++c) {
+                                float a = 0.f;
+#pragma unroll
+                                for (int t = 0; t < 8; ++t)
+                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], X[0][HL - 4 + t], a);
+                                xr[c] = a;
+                            }
+                        }
+                        if (vr >= 8 && vr <= m - 9) {
+                            // interior fine row: coarse rows jb .. jb+3 are open;
+                            // an even row reads taps 7,5,3,1 of their windows
+                            // and closes row jb, an odd row reads taps 6,4,2,0
+                            const bool odd = (vr & 1) != 0;
+                            const float yw[4] = {odd ? kW8[6] : kW8[7], odd ? kW8[4] : kW8[5],
+                                                 odd ? kW8[2] : kW8[3], odd ? kW8[0] : kW8[1]};
+#pragma unroll
+                            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c) racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
+                            if (!odd) {
+                                const int J = (vr - 3) >> 1;
+                                if (J >= Jlo && J < Jhi && kvalid) store_coarse(J, 0);
+#pragma unroll
+                                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                                    for (int c = 0; c < NRC; ++c) racc[q][c] = racc[q + 1][c];
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c) racc[3][c] = 0.f;
+                            }
+                        } else {
+                            // rows next to an edge: weights and closing rows from the table
+                            const float4 ya = p.rytab[2 * vr], yb = p.rytab[2 * vr + 1];
+                            const float yw[NQ] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y};
+                            const int2 meta = p.rymeta[vr];
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                                for (int c = 0; c < NRC; ++c) racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
+#pragma unroll
+                            for (int e = 0; e < NQ; ++e) {
+                                const int J = meta.x + e;
+                                if (e < meta.y && J >= Jlo && J < Jhi && kvalid) store_coarse(J, e);
+                            }
+#pragma unroll
+                            for (int sft = 1; sft <= NQ; ++sft) {
+                                if (meta.y == sft) {
+#pragma unroll
+                                    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+                                        for (int c = 0; c < NRC; ++c)
+                                            racc[q][c] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][c] : 0.f;
+                                }
+                            }
+                        }
+                    }
+                }
 #pragma unroll
                 for (int t = 0; t < TI; ++t) {
                     if constexpr (BALL) {
@@ -157,9 +263,9 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
 #pragma unroll
                         for (int c = 0; c <= HZ; ++c) {
                             float P[H + 1];
-                            P[0] = X[c][t + H];
+                            P[0] = X[c][t + HL];
 #pragma unroll
-                            for (int a = 1; a <= H; ++a) P[a] = X[c][t + H + a] + X[c][t + H - a];
+                            for (int a = 1; a <= H; ++a) P[a] = X[c][t + HL + a] + X[c][t + HL - a];
 #pragma unroll
                             for (int b = 0; b <= H; ++b)
 #pragma unroll
@@ -178,10 +284,10 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                         float R = 0.f;
 #pragma unroll
                         for (int c = 0; c <= HZ; ++c) {
-                            float q = p.W.fx[0] * X[c][t + H];
+                            float q = p.W.fx[0] * X[c][t + HL];
 #pragma unroll
                             for (int a = 1; a <= H; ++a)
-                                q = __builtin_fmaf(p.W.fx[a], X[c][t + H + a] + X[c][t + H - a], q);
+                                q = __builtin_fmaf(p.W.fx[a], X[c][t + HL + a] + X[c][t + HL - a], q);
                             R = __builtin_fmaf(p.W.fz[c], q, R);
                         }
 #pragma unroll
@@ -243,12 +349,15 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     }
 }
 
-template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false>
 hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     Ball3Params p{};
     p.in = a.in;
     p.out = a.out;
     p.coarse = a.coarse;
+    p.rout = a.rout;
+    p.rytab = a.rytab;
+    p.rymeta = a.rymeta;
     p.wfield = a.wfield;
     p.wconst = a.wconst;
     p.K = a.K;
@@ -265,7 +374,7 @@ hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL, PRO>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL, PRO, RES>), dim3(nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
@@ -273,7 +382,9 @@ template <int H, int HZ>
 hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     if (pro && (a.transpose || (a.n & 1) || (a.m & 1))) return hipErrorInvalidValue;
+    if (a.rout && ((a.n & 1) || (a.m & 1) || (a.TJ & 1))) return hipErrorInvalidValue;
     if (a.zero_levels) {
+        if (a.rout) return hipErrorInvalidValue;
         // zeros beyond the column ends: Z_c^T = Z_c, the box of the line products
         if (a.ball) return hipErrorInvalidValue;
         return pro ? launch_variant<H, HZ, false, false, true, true>(a, st)
@@ -281,6 +392,14 @@ hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
     }
     if (!a.ball || HZ != H) return hipErrorInvalidValue;
     if constexpr (HZ == H) {
+        if (a.rout) {   // the transposed pass with the fused restriction (H >= 3)
+            if constexpr (H >= 3) {
+                if (!a.transpose || pro) return hipErrorInvalidValue;
+                return launch_variant<H, HZ, true, true, false, false, true>(a, st);
+            } else {
+                return hipErrorInvalidValue;
+            }
+        }
         if (a.transpose) return launch_variant<H, HZ, true, true, false, false>(a, st);
         return pro ? launch_variant<H, HZ, true, false, false, true>(a, st)
                    : launch_variant<H, HZ, true, false, false, false>(a, st);

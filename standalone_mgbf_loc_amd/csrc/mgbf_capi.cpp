@@ -296,9 +296,9 @@ bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int 
 // output rows per wave TJ? The wave of output rows j0 .. j0+TJ-1 loads fine
 // rows j0-H .. j0+TJ-1+H and owns coarse rows j0/2 .. (j0+TJ)/2-1, whose
 // 8-row windows must lie inside what it loads.
-bool fused_ok(const mgbf_ctx* c, int g, int tj) {
-    if (g + 1 >= c->gm || !c->res_ok[g] || c->s.H < 3 || tj < 2 || (tj & 1)) return false;
-    const int m = c->ny[g], H = c->s.H;
+bool fused_ok(const mgbf_ctx* c, int g, int tj, int H) {
+    if (g + 1 >= c->gm || !c->res_ok[g] || H < 3 || tj < 2 || (tj & 1)) return false;
+    const int m = c->ny[g];
     const std::vector<int>& st = c->s.ry[g].start;
     for (int j0 = 0; j0 < m; j0 += tj) {
         const int te = tj < m - j0 ? tj : m - j0;
@@ -439,7 +439,7 @@ int down_sweep(mgbf_ctx* c, const float* u1, float* U, float* T, bool weighted, 
         float* next = last ? nullptr : U + c->goff[g + 1];
         const float* wf = weighted ? c->wfield[g] : nullptr;
         const float wc = weighted ? c->wconst[g] : 1.0f;
-        if (filt && !last && c->fuse >= 3 && fused_ok(c, g, c->tile_tj[g][3])) {
+        if (filt && !last && c->fuse >= 3 && fused_ok(c, g, c->tile_tj[g][3], c->s.H)) {
             CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT, nullptr, -1, next));
             if (c->d_kmax) c->kmax_valid |= 1u << (g + 1);
             continue;
@@ -639,19 +639,24 @@ bool plan_ball3(const mgbf::Setup& s, const mgbf_config& cfg, mgbf::Ball3Coef& c
 // coarse (forward only, when ball3_pro(c)): the prolongation of generation
 // g+1 is added to the output, s_g = F in + P s_{g+1}.
 int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st,
-         bool* scaled = nullptr, const float* coarse = nullptr) {
+         bool* scaled = nullptr, const float* coarse = nullptr, float* rout = nullptr) {
     Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st,
-             8.0 * (double)c->gsize[g] + (coarse ? 4.0 * (double)c->gsize[g + 1] : 0.0));
+             8.0 * (double)c->gsize[g] + (coarse || rout ? 4.0 * (double)c->gsize[g + 1] : 0.0));
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     if (scaled) *scaled = false;
     if (coarse && (transpose || !c->ball3 || K3 < c->K))
         return fail(MGBF_ERR_ARG, "internal: prolongation fused into a 3-D pass that cannot take it");
+    if (rout && (!transpose || !c->ball3 || K3 < c->K))
+        return fail(MGBF_ERR_ARG, "internal: restriction fused into a 3-D pass that cannot take it");
     if (c->ball3) {
         const bool fuse = scaled && K3 >= c->K;
         mgbf::Ball3Args b{};
         b.in = in;
         b.out = out;
         b.coarse = coarse;
+        b.rout = rout;
+        b.rytab = rout ? c->d_rytab[g] : nullptr;
+        b.rymeta = rout ? c->d_rymeta[g] : nullptr;
         b.wfield = fuse ? c->wfield[g] : nullptr;
         b.wconst = fuse ? c->wconst[g] : 1.f;
         if (fuse) *scaled = true;
@@ -715,14 +720,36 @@ int rad3_forward_upsweep(mgbf_ctx* c, const float* T, float* U, float* y, hipStr
     return MGBF_OK;
 }
 
+// The transposed 3-D passes of the down-sweep, T_g = F^T u_g (scaled by A_g
+// when `scaled` is given and the pass can take it), each carrying the
+// restriction u_{g+1} = R u_g where the streaming kernel and its tiles allow,
+// else preceded by a restriction pass of its own.
+int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scaled, hipStream_t st) {
+    const float* ug = g == 0 ? x : U + c->goff[g];
+    float* next = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
+    const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
+    const bool res = next && c->ball3 && c->b3ball && K3 >= c->K && c->fuse >= 3 &&
+                     fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), c->b3H);
+    if (next && !res) CALL(restrict_g(c, g, ug, next, st));
+    return ball(c, g, ug, T + c->goff[g], true, st, scaled, nullptr, res ? next : nullptr);
+}
+
+// The per-k maxima the restriction passes record start from zero on every call.
+int reset_kmax(mgbf_ctx* c, hipStream_t st) {
+    c->kmax_valid = 0;
+    if (c->d_kmax && c->gm > 1)
+        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)(c->gm - 1) * c->K, st), "fill launch");
+    return MGBF_OK;
+}
+
 int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    CALL(restrict_chain(c, x, U, st));
+    CALL(reset_kmax(c, st));
     const bool pro = ball3_pro(c);
     for (int g = 0; g < c->gm; ++g) {
         bool scaled = false;
-        CALL(ball(c, g, g == 0 ? x : U + c->goff[g], T + c->goff[g], true, st, &scaled));
+        CALL(rad3_down(c, x, U, T, g, &scaled, st));
         if (!scaled)
             HIP_TRY(mgbf::launch_scale(T + c->goff[g], T + c->goff[g], c->wfield[g], c->wconst[g], 0,
                                        c->gsize[g], st),
@@ -734,9 +761,10 @@ int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
 }
 
 int apply_wt_rad3(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
-    CALL(restrict_chain(c, x, c->ws_u, st));
+    CALL(reset_kmax(c, st));
     for (int g = 0; g < c->gm; ++g) {
-        CALL(ball(c, g, g == 0 ? x : c->ws_u + c->goff[g], v + c->goff[g], true, st));
+        // the transposed pass writes v_g; v + goff stands in for T
+        CALL(rad3_down(c, x, c->ws_u, v, g, nullptr, st));
         if (sqrt_weights)
             HIP_TRY(mgbf::launch_scale(v + c->goff[g], v + c->goff[g], c->wfield[g], c->wconst[g], 1,
                                        c->gsize[g], st),
@@ -1381,7 +1409,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
                     bool dup = false;
                     for (int q2 = 0; q2 < q; ++q2) dup = dup || tjs[q2] == tj;
                     if (tj > m || dup) continue;
-                    if (kind == 3 && !fused_ok(c, g, tj)) continue;
+                    if (kind == 3 && !fused_ok(c, g, tj, c->s.H)) continue;
                     const long long waves = (long long)((c->K + 63) / 64) * ((n + en - 1) / en) *
                                             ((m + tj - 1) / tj) * (en == 16 ? 4 : 1);
                     if (waves < 2048 && tj != 16) continue;

@@ -196,6 +196,9 @@ struct Ball3Args {
     const float* in;
     float* out;
     const float* coarse;   // next coarser generation to prolong and add (forward only), or nullptr
+    float* rout;           // the input restricted to the next generation (transposed ball pass), or nullptr
+    const float4* rytab;   // its per-fine-row weights and closing rows (as RadialArgs)
+    const int2* rymeta;
     const float* wfield;   // per-point scale of the output, or nullptr
     float wconst;          // constant scale when wfield is nullptr
     int K, n, m, lm, K3;   // only k < K3 is written
@@ -207,6 +210,9 @@ struct Ball3Params {
     const float* in;
     float* out;
     const float* coarse;
+    float* rout;
+    const float4* rytab;
+    const int2* rymeta;
     const float* wfield;
     float wconst;
     int K, n, m, TJ, lm, K3;

@@ -531,6 +531,7 @@ def test_lin3_procedure_matches_oracle(case):
     (4, 32, 3, 2, 1, ""),        # a wavefront shared by 3-D and 2-D variables
     (4, 8, 2, 0, 3, ""),         # lm = 2 hz + 2: the four corrected lanes adjacent
     (4, 64, 1, 0, 4, ""),        # one column per wavefront, half-width 4
+    (4, 16, 4, 0, 3, ""),        # the restriction fused into the transposed pass (H = 3)
     (4, 16, 3, 2, 2, "lquart"),
     (5, 16, 3, 2, 3, ""),
     (5, 32, 2, 0, 4, "wide"),
