@@ -71,6 +71,16 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     int soff[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) soff[c] = mirror_half(i0 - HL + c, n) * K * 4;
+    // with the restriction state the column offsets move to VGPRs (per-lane
+    // offsets vk + soff) where that ends the SGPR spills at the same
+    // occupancy (from the compiler's register counts: disc H = 5, 6 and
+    // separable H = 6; elsewhere the VGPRs would cost a wave per SIMD)
+    constexpr bool VOFF = RES && ((DISC && (H == 5 || H == 6)) || (SEP && H == 6));
+    int voff[VOFF ? NC : 1];
+    if constexpr (VOFF) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) voff[c] = vk + soff[c];
+    }
 
     float acc[NR][TI];
 #pragma unroll
@@ -171,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                 const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)r * rowstride);
                 float v[NC];
 #pragma unroll
-                for (int c = 0; c < NC; ++c) v[c] = buf_ld(rs, vk, soff[c]);
+                for (int c = 0; c < NC; ++c) v[c] = VOFF ? buf_ld(rs, voff[c], 0) : buf_ld(rs, vk, soff[c]);
                 if constexpr (RES) {
                     const int vr = j0 - H + rr;   // fine row, before mirroring
                     if (vr >= 0 && vr < m) {
