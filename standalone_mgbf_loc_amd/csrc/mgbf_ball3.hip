@@ -35,13 +35,17 @@ namespace {
 
 template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false>
 __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
-    static_assert(!RES || (H >= 3 && !PRO), "restriction epilogue shape");
+    static_assert(!RES || !PRO, "restriction epilogue shape");
     constexpr int TI = 4;
     constexpr int NR = 2 * H + 1;
     // columns loaded on each side: H for the filter, 4 for the restriction's
     // 8-point stencils at the domain edges
     constexpr int HL = (RES && H < 4) ? 4 : H;
     constexpr int NC = TI + 2 * HL;
+    // rows loaded on each side: H for the filter, 3 for the restriction's
+    // 8-row windows; the D extra rows feed the restriction only
+    constexpr int HR = (RES && H < 3) ? 3 : H;
+    constexpr int D = HR - H;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int item = (int)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
@@ -119,7 +123,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     }
 
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
-    const int nsteps = tj_eff + 2 * H;
+    const int nsteps = tj_eff + 2 * HR;
 
     // RES: the restriction of the input to the next generation rides on the
     // loaded rows, as in the radial kernel: each fine row restricted in x
@@ -149,13 +153,19 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
         for (int s = 0; s < NR; ++s) {
             const int rr = rb + s;
             if (rr < nsteps) {
-                const int r = mirror_half(j0 - H + rr, m);
+                const int r = mirror_half(j0 - HR + rr, m);
                 const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)r * rowstride);
+                float vraw[NC];
+#pragma unroll
+                for (int col = 0; col < NC; ++col) vraw[col] = buf_ld(rs, vk, soff[col]);
+                // filter row f of the ring (the first D loaded rows feed the restriction only)
+                const int f = rr - D;
+                const bool frow = f >= 0 && f < tj_eff + 2 * H;
                 // level pairs X[c][col] of the loaded row
                 float X[HZ + 1][NC];
 #pragma unroll
                 for (int col = 0; col < NC; ++col) {
-                    const float v = buf_ld(rs, vk, soff[col]);
+                    const float v = vraw[col];
                     X[0][col] = v;
                     float u = v, d = v;
 #pragma unroll
@@ -176,7 +186,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                     }
                 }
                 if constexpr (RES) {
-                    const int vr = j0 - H + rr;   // fine row, before mirroring
+                    const int vr = j0 - HR + rr;   // fine row, before mirroring
                     if (vr >= 0 && vr < m) {
                         float xr[NRC];
                         if (xcls == 0) {          // fine columns 2I-3 .. 2I+4
@@ -184,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], X[0][HL - 3 + 2 * c + t], a);
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], vraw[HL - 3 + 2 * c + t], a);
                                 xr[c] = a;
                             }
                         } else if (xcls == 1) {   // fine columns 0 .. 7
@@ -192,7 +202,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], X[0][HL + t], a);
+                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], vraw[HL + t], a);
                                 xr[c] = a;
                             }
                         } else {                  // fine columns n-8 .. n-1
@@ -201,7 +211,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                                 float a = 0.f;
 #pragma unroll
                                 for (int t = 0; t < 8; ++t)
-                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], X[0][HL - 4 + t], a);
+                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], vraw[HL - 4 + t], a);
                                 xr[c] = a;
                             }
                         }
@@ -255,6 +265,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                 }
 #pragma unroll
                 for (int t = 0; t < TI; ++t) {
+                    if (!frow) break;
                     if constexpr (BALL) {
                         // w(a,b,c) = wr[a^2 + b^2 + c^2] on the ball
                         float Z[H + 1];
@@ -275,8 +286,8 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                         }
 #pragma unroll
                         for (int b = 0; b <= H; ++b) {
-                            acc[(s + b) % NR][t] += Z[b];
-                            if (b > 0) acc[(s + NR - b) % NR][t] += Z[b];
+                            acc[(s + NR - D + b) % NR][t] += Z[b];
+                            if (b > 0) acc[(s + 2 * NR - D - b) % NR][t] += Z[b];
                         }
                     } else {
                         // the box of line products w(a,b,c) = fx(a) fy(b) fz(c):
@@ -292,17 +303,18 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                         }
 #pragma unroll
                         for (int b = 0; b <= H; ++b) {
-                            acc[(s + b) % NR][t] = __builtin_fmaf(p.W.fy[b], R, acc[(s + b) % NR][t]);
+                            acc[(s + NR - D + b) % NR][t] =
+                                __builtin_fmaf(p.W.fy[b], R, acc[(s + NR - D + b) % NR][t]);
                             if (b > 0)
-                                acc[(s + NR - b) % NR][t] =
-                                    __builtin_fmaf(p.W.fy[b], R, acc[(s + NR - b) % NR][t]);
+                                acc[(s + 2 * NR - D - b) % NR][t] =
+                                    __builtin_fmaf(p.W.fy[b], R, acc[(s + 2 * NR - D - b) % NR][t]);
                         }
                     }
                 }
-                // output row j = j0 + rr - 2H is complete
-                const int jrel = rr - 2 * H;
-                constexpr int slot_off = NR - H;   // (s - H) mod NR
-                if (jrel >= 0 && jrel < tj_eff) {
+                // output row j = j0 + f - 2H is complete
+                const int jrel = f - 2 * H;
+                constexpr int slot_off = 2 * NR - D - H;   // (f - H) mod NR
+                if (frow && jrel >= 0 && jrel < tj_eff) {
                     const int j = j0 + jrel;
                     const size_t rowoff = (size_t)j * rowstride + (size_t)i0 * K;
                     float o[TI];
@@ -384,21 +396,17 @@ hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
     if (pro && (a.transpose || (a.n & 1) || (a.m & 1))) return hipErrorInvalidValue;
     if (a.rout && ((a.n & 1) || (a.m & 1) || (a.TJ & 1))) return hipErrorInvalidValue;
     if (a.zero_levels) {
-        if (a.rout) return hipErrorInvalidValue;
         // zeros beyond the column ends: Z_c^T = Z_c, the box of the line products
         if (a.ball) return hipErrorInvalidValue;
+        if (a.rout) return pro ? hipErrorInvalidValue : launch_variant<H, HZ, false, false, true, false, true>(a, st);
         return pro ? launch_variant<H, HZ, false, false, true, true>(a, st)
                    : launch_variant<H, HZ, false, false, true, false>(a, st);
     }
     if (!a.ball || HZ != H) return hipErrorInvalidValue;
     if constexpr (HZ == H) {
-        if (a.rout) {   // the transposed pass with the fused restriction (H >= 3)
-            if constexpr (H >= 3) {
-                if (!a.transpose || pro) return hipErrorInvalidValue;
-                return launch_variant<H, HZ, true, true, false, false, true>(a, st);
-            } else {
-                return hipErrorInvalidValue;
-            }
+        if (a.rout) {   // the transposed pass with the fused restriction
+            if (!a.transpose || pro) return hipErrorInvalidValue;
+            return launch_variant<H, HZ, true, true, false, false, true>(a, st);
         }
         if (a.transpose) return launch_variant<H, HZ, true, true, false, false>(a, st);
         return pro ? launch_variant<H, HZ, true, false, false, true>(a, st)

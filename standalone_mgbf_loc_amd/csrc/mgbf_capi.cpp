@@ -728,8 +728,9 @@ int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scal
     const float* ug = g == 0 ? x : U + c->goff[g];
     float* next = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
-    const bool res = next && c->ball3 && c->b3ball && K3 >= c->K && c->fuse >= 3 &&
-                     fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), c->b3H);
+    // the streaming pass loads at least 3 halo rows when it restricts
+    const bool res = next && c->ball3 && K3 >= c->K && c->fuse >= 3 &&
+                     fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), std::max(c->b3H, 3));
     if (next && !res) CALL(restrict_g(c, g, ug, next, st));
     return ball(c, g, ug, T + c->goff[g], true, st, scaled, nullptr, res ? next : nullptr);
 }
