@@ -224,64 +224,105 @@ int pick_tj(int K, int n, int m, int ti) {
 
 enum { BAND_VF = 0, BAND_VFT = 1, BAND_VVT = 2 };
 
-// Restriction weights of the 2 coarse indices next to an edge over the first
-// 8 fine indices (the host twin of the kernels' compile-time table).
-void edge_restrict_weights(double w[2][8]) {
+// The inter-generation restriction along one axis, as the fused kernels
+// apply it: interior coarse I reads fine 2I+roff .. 2I+roff+nr-1 with r[];
+// the 2 coarse indices next to an edge read the first 8 fine indices with
+// edge[I][] (R = P^T with the coarse indices of P mirrored half-sample into
+// the domain; the far edge is the mirror image). The host twin of the
+// kernels' compile-time stencils: 4-point (direct1) or quadratic (direct2).
+struct ResStencil {
+    int nr, roff;
+    double r[8];
+    double edge[2][8];
+};
+
+ResStencil res_stencil(bool quad) {
+    ResStencil S{};
     const double kp[4] = {-9. / 128., 111. / 128., 29. / 128., -3. / 128.};
     const double kq[4] = {-3. / 128., 29. / 128., 111. / 128., -9. / 128.};
-    for (int I = 0; I < 2; ++I)
-        for (int t = 0; t < 8; ++t) w[I][t] = 0.0;
-    for (int i = 0; i < 8; ++i) {
-        const bool odd = (i & 1) != 0;
-        const int u = i / 2;
-        for (int t = 0; t < 4; ++t) {
-            int I = odd ? u - 1 + t : u - 2 + t;
+    const double ka[3] = {5. / 32., 30. / 32., -3. / 32.};
+    const double kb[3] = {-3. / 32., 30. / 32., 5. / 32.};
+    // fine i reads coarse (i >> 1) + off[i & 1] + t, t < np, with c[i & 1][t]
+    const int np = quad ? 3 : 4;
+    const int off[2] = {quad ? -1 : -2, -1};
+    const double* c[2] = {quad ? ka : kq, quad ? kb : kp};
+    S.nr = 2 * np;
+    S.roff = quad ? -2 : -3;
+    for (int i = 0; i < 2 * np + 8; ++i) {
+        const int par = i & 1, u = i >> 1;
+        for (int t = 0; t < np; ++t) {
+            int I = u + off[par] + t;
+            // interior weights from a coarse index far from the edges
+            if (I == np && i - 2 * np - S.roff >= 0 && i - 2 * np - S.roff < S.nr)
+                S.r[i - 2 * np - S.roff] += c[par][t];
+            if (i >= 8) continue;
             if (I < 0) I = -1 - I;
-            if (I < 2) w[I][i] += odd ? kp[t] : kq[t];
+            if (I < 2) S.edge[I][i] += c[par][t];
         }
     }
+    return S;
 }
 
 // Does the restriction of a generation with nf x mf fine points have the
-// shape the fused radial kernel assumes? Along x: interior coarse columns
-// read fine 2I-3 .. 2I+4 with the 8-point stencil, the 2 columns at each edge
-// read the 8 fine columns next to it with the edge weights; nf % 4 == 0.
-// Along y the per-fine-row table (lowest open coarse row Jb, rows closing at
-// v, weights into rows Jb .. Jb+5, padded to 8) is built here; every fine row
-// must feed only those 6 rows (4 in the interior, 6 next to an edge, where
-// the two edge rows share a window), and windows must close in row order.
-bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int nf, int mf,
+// shape the fused kernels assume? Along x (nf % 4 == 0) every coarse row of
+// the table must equal, over the fine columns the kernel reads, the stencil
+// of its tile class (edge, interior) and have no weight outside them. Along
+// y the per-fine-row table (lowest open coarse row Jb, rows closing at v,
+// weights into rows Jb .. Jb+5, padded to 8) is built here; a window closes
+// at its last nonzero weight, every fine row must feed only those 6 rows
+// (NR/2 in the interior, up to 6 next to an edge, where the edge rows share
+// a window), and windows must open and close in row order.
+bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int nf, int mf, bool quad,
                       std::vector<float>& tab, std::vector<int>& meta) {
     if (nf % 4 || nf < 8 || mf < 8 || rx.width != 8 || ry.width != 8) return false;
-    const double w8[8] = {-3. / 128., -9. / 128., 29. / 128., 111. / 128.,
-                          111. / 128., 29. / 128., -9. / 128., -3. / 128.};
-    double we[2][8];
-    edge_restrict_weights(we);
+    const ResStencil S = res_stencil(quad);
     const int nc = nf / 2, mc = mf / 2;
     auto close = [](double a, double b) { return std::fabs(a - b) <= 1e-12; };
     for (int I = 0; I < nc; ++I) {
+        // the fine columns the kernel reads for I, and its weight on each
+        int lo, len;
+        if (I < 2) lo = 0, len = 8;
+        else if (I >= nc - 2) lo = nf - 8, len = 8;
+        else lo = 2 * I + S.roff, len = S.nr;
+        auto kern = [&](int col) {
+            if (I < 2) return S.edge[I][col];
+            if (I >= nc - 2) return S.edge[nc - 1 - I][nf - 1 - col];
+            return S.r[col - lo];
+        };
         for (int t = 0; t < 8; ++t) {
+            const int col = rx.start[I] + t;
             const double w = rx.w[(size_t)I * 8 + t];
-            bool ok;
-            if (I < 2) ok = rx.start[I] == 0 && close(w, we[I][t]);
-            else if (I >= nc - 2) ok = rx.start[I] == nf - 8 && close(w, we[nc - 1 - I][7 - t]);
-            else ok = rx.start[I] == 2 * I - 3 && close(w, w8[t]);
-            if (!ok) return false;
+            if (col < lo || col >= lo + len) {
+                if (w != 0.0) return false;
+            } else if (!close(w, kern(col))) {
+                return false;
+            }
+        }
+        for (int col = lo; col < lo + len; ++col) {
+            const int t = col - rx.start[I];
+            if ((t < 0 || t >= 8) && kern(col) != 0.0) return false;
         }
     }
+    // window ends: the last fine row with a nonzero weight
+    std::vector<int> end(mc);
+    for (int J = 0; J < mc; ++J) {
+        int e = ry.start[J];
+        for (int t = 0; t < 8; ++t)
+            if (ry.w[(size_t)J * 8 + t] != 0.0) e = ry.start[J] + t;
+        end[J] = e;
+    }
     for (int J = 1; J < mc; ++J)
-        if (ry.start[J] < ry.start[J - 1]) return false;
+        if (ry.start[J] < ry.start[J - 1] || end[J] < end[J - 1]) return false;
     tab.assign((size_t)mf * 8, 0.f);
     meta.assign((size_t)mf * 2, 0);
     for (int v = 0; v < mf; ++v) {
         int jb = 0, ne = 0;
         for (int J = 0; J < mc; ++J) {
-            const int e = ry.start[J] + 7;
-            if (e < v) ++jb;
-            if (e == v) ++ne;
+            if (end[J] < v) ++jb;
+            if (end[J] == v) ++ne;
         }
         for (int J = 0; J < mc; ++J) {
-            if (v < ry.start[J] || v > ry.start[J] + 7) continue;
+            if (v < ry.start[J] || v > end[J]) continue;
             if (J < jb || J > jb + 5) return false;
             tab[(size_t)v * 8 + (J - jb)] = (float)ry.w[(size_t)J * 8 + (v - ry.start[J])];
         }
@@ -729,7 +770,8 @@ int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scal
     float* next = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     // the streaming pass loads at least 3 halo rows when it restricts
-    const bool res = next && c->ball3 && K3 >= c->K && c->fuse >= 3 &&
+    // (the 3-D kernel's restriction carries the 4-point stencil only)
+    const bool res = next && c->ball3 && !c->quad && K3 >= c->K && c->fuse >= 3 &&
                      fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), std::max(c->b3H, 3));
     if (next && !res) CALL(restrict_g(c, g, ug, next, st));
     return ball(c, g, ug, T + c->goff[g], true, st, scaled, nullptr, res ? next : nullptr);
@@ -1077,9 +1119,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         const mgbf::AxisTable* tabs[4] = {&s.rx[g], &s.ry[g], &s.px[g], &s.py[g]};
         for (const auto* t : tabs)
             bytes += carve_size(sizeof(int) * t->n_out) + carve_size(sizeof(float) * t->w.size());
-        // the fused restriction carries the 4-point stencil only
-        c->res_ok[g] = !c->quad &&
-                       build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], rytab[g], rymeta[g]);
+        c->res_ok[g] = build_res_tables(s.rx[g], s.ry[g], s.nx[g], s.ny[g], c->quad, rytab[g], rymeta[g]);
         bytes += carve_size(sizeof(float) * rytab[g].size()) + carve_size(sizeof(int) * rymeta[g].size());
     }
     {

@@ -76,29 +76,6 @@ constexpr float kQc[4] = {-3.f / 128.f, 29.f / 128.f, 111.f / 128.f, -9.f / 128.
 constexpr float kW8[8] = {-3.f / 128.f, -9.f / 128.f, 29.f / 128.f, 111.f / 128.f,
                           111.f / 128.f, 29.f / 128.f, -9.f / 128.f, -3.f / 128.f};
 
-// Restriction weights of the two coarse indices next to a domain edge over
-// the first 8 fine indices: R = P^T where fine 2u reads coarse u-2..u+1 with
-// kQc and fine 2u+1 reads coarse u-1..u+2 with kPc, the coarse indices
-// mirrored (half-sample) into the domain. The far edge is the mirror image:
-// R[nc-1-I][nf-1-t] = R[I][t].
-struct EdgeRW {
-    float w[2][8];
-};
-constexpr EdgeRW make_edge_rw() {
-    EdgeRW e{};
-    for (int i = 0; i < 8; ++i) {
-        const bool odd = (i & 1) != 0;
-        const int u = i / 2;
-        for (int t = 0; t < 4; ++t) {
-            int I = odd ? u - 1 + t : u - 2 + t;
-            if (I < 0) I = -1 - I;
-            if (I < 2) e.w[I][i] += odd ? kPc[t] : kQc[t];
-        }
-    }
-    return e;
-}
-constexpr EdgeRW kEdgeRW = make_edge_rw();
-
 // Inter-generation transfer along one axis, as a compile-time stencil.
 // Fine index i (0-based, u = i >> 1) reads coarse u + OFF[i & 1] + t,
 // t < NP, with weight C[i & 1][t], the coarse indices mirrored (half-sample)
@@ -129,5 +106,29 @@ struct Stencil<true> {
     static constexpr float R[6] = {-3.f / 32.f, 5.f / 32.f, 30.f / 32.f,
                                    30.f / 32.f, 5.f / 32.f, -3.f / 32.f};
 };
+
+// Restriction weights of the two coarse indices next to a domain edge over
+// the first 8 fine indices: R = P^T of the stencil S (fine i reads coarse
+// (i >> 1) + S::OFF[i & 1] + t with weight S::C[i & 1][t]), the coarse
+// indices mirrored (half-sample) into the domain. The far edge is the mirror
+// image: R[nc-1-I][nf-1-t] = R[I][t] (both stencils are symmetric).
+struct EdgeRW {
+    float w[2][8];
+};
+template <bool Q>
+constexpr EdgeRW make_edge_rw() {
+    EdgeRW e{};
+    for (int i = 0; i < 8; ++i) {
+        const int par = i & 1, u = i >> 1;
+        for (int t = 0; t < Stencil<Q>::NP; ++t) {
+            int I = u + Stencil<Q>::OFF[par] + t;
+            if (I < 0) I = -1 - I;
+            if (I < 2) e.w[I][i] += Stencil<Q>::C[par][t];
+        }
+    }
+    return e;
+}
+constexpr EdgeRW kEdgeRW = make_edge_rw<false>();    // 4-point (direct1 / adjoint)
+constexpr EdgeRW kEdgeRWQ = make_edge_rw<true>();    // quadratic (direct2 / adjoint2)
 
 }  // namespace mgbf

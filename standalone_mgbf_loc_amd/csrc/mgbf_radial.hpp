@@ -32,17 +32,19 @@ __host__ __device__ constexpr int disc_extent(int b) {
 // (each wave then holds whole columns: lm divides 64 and K % 64 == 0).
 // RES restricts the input field to the next generation on the way (R = P^T
 // with the folded edges): every loaded fine row is restricted in x over the
-// wave's 2 coarse columns (interior columns 2I-3 .. 2I+4 with kW8, the 2
-// columns at each domain edge with kEdgeRW) and added into a shift register
-// of the coarse rows whose 8-row windows are open (4 in the interior, up to
-// 6 next to an edge), with weights from a per-fine-row table; a coarse row
+// wave's 2 coarse columns (interior columns 2I-3 .. 2I+4 with kW8, or
+// 2I-2 .. 2I+3 with the quadratic weights under QP; the 2 columns at each
+// domain edge with the edge table) and added into a shift register of the
+// coarse rows whose windows are open (4 or 3 in the interior, up to 6 next
+// to an edge, with weights from a per-fine-row table there); a coarse row
 // is stored when its window closes.
+// QP selects the quadratic transfer (direct2 / adjoint2) for PRO or RES.
 // The wave owns coarse rows j0/2 .. (j0+TJ)/2 - 1, whose windows lie inside
 // the rows it loads (checked on the host).
 // ---------------------------------------------------------------------------
 template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES, bool SEP = false, bool QP = false>
 __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
-    static_assert(!QP || (PRO && VHB == 0), "the quadratic prolongation epilogue has no band");
+    static_assert(!QP || ((PRO || RES) && VHB == 0), "the quadratic transfer epilogues have no band");
     static_assert(!RES || (TI == 4 && H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
     static_assert(!(SEP && DISC), "a separable filter is read as a box");
     constexpr int NR = 2 * H + 1;
@@ -185,13 +187,15 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                 if constexpr (RES) {
                     const int vr = j0 - H + rr;   // fine row, before mirroring
                     if (vr >= 0 && vr < m) {
+                        using RS = Stencil<QP>;
                         float xr[NRC];
-                        if (xcls == 0) {          // fine columns 2I-3 .. 2I+4
+                        if (xcls == 0) {          // fine columns 2I+ROFF .. 2I+ROFF+NR-1
 #pragma unroll
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], v[HL - 3 + 2 * c + t], a);
+                                for (int t = 0; t < RS::NR; ++t)
+                                    a = __builtin_fmaf(RS::R[t], v[HL + RS::ROFF + 2 * c + t], a);
                                 xr[c] = a;
                             }
                         } else if (xcls == 1) {   // fine columns 0 .. 7
@@ -199,7 +203,8 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], v[HL + t], a);
+                                for (int t = 0; t < 8; ++t)
+                                    a = __builtin_fmaf(QP ? kEdgeRWQ.w[c][t] : kEdgeRW.w[c][t], v[HL + t], a);
                                 xr[c] = a;
                             }
                         } else {                  // fine columns n-8 .. n-1
@@ -208,25 +213,33 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                                 float a = 0.f;
 #pragma unroll
                                 for (int t = 0; t < 8; ++t)
-                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], v[HL - 4 + t], a);
+                                    a = __builtin_fmaf(QP ? kEdgeRWQ.w[NRC - 1 - c][7 - t] : kEdgeRW.w[NRC - 1 - c][7 - t],
+                                                       v[HL - 4 + t], a);
                                 xr[c] = a;
                             }
                         }
                         if (vr >= 8 && vr <= m - 9) {
-                            // interior fine row: coarse rows jb .. jb+3 are open
-                            // (windows 2J-3 .. 2J+4); an even row reads taps
-                            // 7,5,3,1 of their windows and closes row jb, an
-                            // odd row reads taps 6,4,2,0
-                            const bool odd = (vr & 1) != 0;
-                            const float yw[4] = {odd ? kW8[6] : kW8[7], odd ? kW8[4] : kW8[5],
-                                                 odd ? kW8[2] : kW8[3], odd ? kW8[0] : kW8[1]};
+                            // interior fine row: coarse rows jb .. jb+NO-1 are
+                            // open (windows 2J+ROFF .. 2J+ROFF+NR-1). A row of
+                            // parity CP is the last of row jb's window: it reads
+                            // taps NR-1, NR-3, .. of the open windows and closes
+                            // row jb; a row of the other parity reads taps
+                            // NR-2, NR-4, .. (4-point: even rows close, quadratic:
+                            // odd rows)
+                            constexpr int NO = RS::NR / 2;
+                            constexpr int CP = (RS::ROFF + RS::NR - 1) & 1;
+                            const bool closes = (vr & 1) == CP;
+                            float yw[NO];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q)
+                            for (int q = 0; q < NO; ++q)
+                                yw[q] = closes ? RS::R[RS::NR - 1 - 2 * q] : RS::R[RS::NR - 2 - 2 * q];
+#pragma unroll
+                            for (int q = 0; q < NO; ++q)
 #pragma unroll
                                 for (int c = 0; c < NRC; ++c)
                                     racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
-                            if (!odd) {
-                                const int J = (vr - 3) >> 1;
+                            if (closes) {
+                                const int J = (vr - RS::ROFF - RS::NR + 1) >> 1;
                                 if (J >= Jlo && J < Jhi && kvalid) {
                                     const __amdgpu_buffer_rsrc_t ro =
                                         buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
@@ -238,13 +251,13 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                                         }
                                     }
                                 }
-                                // slots 4 and 5 stay empty between the edges
+                                // slots NO .. 5 stay empty between the edges
 #pragma unroll
-                                for (int q = 0; q < 3; ++q)
+                                for (int q = 0; q + 1 < NO; ++q)
 #pragma unroll
                                     for (int c = 0; c < NRC; ++c) racc[q][c] = racc[q + 1][c];
 #pragma unroll
-                                for (int c = 0; c < NRC; ++c) racc[3][c] = 0.f;
+                                for (int c = 0; c < NRC; ++c) racc[NO - 1][c] = 0.f;
                             }
                         } else {
                             // rows next to an edge: weights and closing rows
@@ -454,6 +467,11 @@ hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 3
         if constexpr (TI == 4 && H >= 3) {
             if (pro || vhb) return hipErrorInvalidValue;
+            if (a.quad) {
+                if (a.separable) return launch_radial_variant<H, 4, false, false, 0, true, true, true>(a, st);
+                return a.exact_disc ? launch_radial_variant<H, 4, true, false, 0, true, false, true>(a, st)
+                                    : launch_radial_variant<H, 4, false, false, 0, true, false, true>(a, st);
+            }
             if (a.separable) return launch_radial_variant<H, 4, false, false, 0, true, true>(a, st);
             return a.exact_disc ? launch_radial_variant<H, 4, true, false, 0, true>(a, st)
                                 : launch_radial_variant<H, 4, false, false, 0, true>(a, st);

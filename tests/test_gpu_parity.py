@@ -370,6 +370,15 @@ def test_lquart_procedures_match_oracle(monkeypatch, case):
         rhs = op.dot(vd, wt)
         scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
         assert abs(lhs - rhs) <= 1e-6 * scale
+        if case == "ens":
+            # the quadratic restriction rides in the first radial pass of
+            # every generation but the last (half-width 6): no restriction
+            # launch of its own
+            op.timers(True)
+            op.filtering_procedure(xd)
+            ph = op.read_timers()
+            op.timers(False)
+            assert ph["upsend"][1] == 0 and ph["hfiltT"][1] == d.gm
 
 
 @pytest.mark.parametrize("case", ["ens", "ens_exact_order", "ens_no_tables", "ens_filt_g1_off",
