@@ -502,9 +502,9 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, fl
             bool sqrt_w, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    // the radial kernels fuse the 4-point and the quadratic prolongation,
-    // and the band after the 4-point one only
-    const bool fuse = band >= 0 && c->fuse >= 2 && !c->quad && can_fuse_band(c, band);
+    // the radial kernels fuse the 4-point or the quadratic prolongation,
+    // and the band after either
+    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band);
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
@@ -1431,7 +1431,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
         const int n = c->nx[g], m = c->ny[g];
         for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
-            if (kind == 2 && (c->quad || !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband)))) continue;
+            if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
             if (kind == 3 && !(c->fuse >= 3 && c->res_ok[g] && c->s.H >= 3)) continue;
             const float* coarse = kind == 1 || kind == 2 ? c->ws_u + c->goff[g + 1] : nullptr;
             float* rout = kind == 3 ? c->ws_u + c->goff[g + 1] : nullptr;

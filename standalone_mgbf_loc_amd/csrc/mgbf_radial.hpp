@@ -44,7 +44,7 @@ __host__ __device__ constexpr int disc_extent(int b) {
 // ---------------------------------------------------------------------------
 template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES, bool SEP = false, bool QP = false>
 __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
-    static_assert(!QP || ((PRO || RES) && VHB == 0), "the quadratic transfer epilogues have no band");
+    static_assert(!QP || PRO || (RES && VHB == 0), "the quadratic transfer rides in PRO or RES");
     static_assert(!RES || (TI == 4 && H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
     static_assert(!(SEP && DISC), "a separable filter is read as a box");
     constexpr int NR = 2 * H + 1;
@@ -458,11 +458,29 @@ template <int H, int TI>
 hipError_t launch_radial_ti(const RadialArgs& a, hipStream_t st) {
     const bool pro = a.coarse != nullptr;
     const int vhb = a.band ? a.vhb : 0;
-    if (a.quad && pro) {   // the quadratic prolongation epilogue, without band
-        if (vhb) return hipErrorInvalidValue;
-        if (a.separable) return launch_radial_variant<H, TI, false, true, 0, false, true, true>(a, st);
-        return a.exact_disc ? launch_radial_variant<H, TI, true, true, 0, false, false, true>(a, st)
-                            : launch_radial_variant<H, TI, false, true, 0, false, false, true>(a, st);
+    if (a.quad && pro) {   // the quadratic prolongation epilogue, with or without band
+        if (a.separable) {
+            switch (vhb) {
+                case 0: return launch_radial_variant<H, TI, false, true, 0, false, true, true>(a, st);
+                case 3: return launch_radial_variant<H, TI, false, true, 3, false, true, true>(a, st);
+                case 4: return launch_radial_variant<H, TI, false, true, 4, false, true, true>(a, st);
+                case 6: return launch_radial_variant<H, TI, false, true, 6, false, true, true>(a, st);
+                case 8: return launch_radial_variant<H, TI, false, true, 8, false, true, true>(a, st);
+                default: return hipErrorInvalidValue;
+            }
+        }
+        if (!a.exact_disc) {
+            if (vhb) return hipErrorInvalidValue;
+            return launch_radial_variant<H, TI, false, true, 0, false, false, true>(a, st);
+        }
+        switch (vhb) {
+            case 0: return launch_radial_variant<H, TI, true, true, 0, false, false, true>(a, st);
+            case 3: return launch_radial_variant<H, TI, true, true, 3, false, false, true>(a, st);
+            case 4: return launch_radial_variant<H, TI, true, true, 4, false, false, true>(a, st);
+            case 6: return launch_radial_variant<H, TI, true, true, 6, false, false, true>(a, st);
+            case 8: return launch_radial_variant<H, TI, true, true, 8, false, false, true>(a, st);
+            default: return hipErrorInvalidValue;
+        }
     }
     if (a.rout) {   // restriction epilogue: plain pass, 4 columns per lane, H >= 3
         if constexpr (TI == 4 && H >= 3) {

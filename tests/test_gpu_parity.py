@@ -337,7 +337,7 @@ def test_w_and_wt_match_oracle_and_pass_the_dot_product_test():
         assert (y1 - y2).abs().max().item() <= 1e-5 * y1.abs().max().item()
 
 
-@pytest.mark.parametrize("case", ["ens", "ens_filt_g1_off", "bckg", "fast", "fuse0"])
+@pytest.mark.parametrize("case", ["ens", "ens_band", "ens_filt_g1_off", "bckg", "fast", "fuse0"])
 def test_lquart_procedures_match_oracle(monkeypatch, case):
     # upsending2_ens / downsending2_ens: the quadratic transfer between
     # generations in every procedure; W and W^T stay adjoint
@@ -346,6 +346,8 @@ def test_lquart_procedures_match_oracle(monkeypatch, case):
     kw = dict(n_ens=2, gm=4, h=6, hz=3, weights=[1.0, 0.6, 0.3, 0.15], lquart=True)
     if case == "ens_filt_g1_off":
         kw["l_filt_g1"] = False
+    if case == "ens_band":
+        kw["n_ens"] = 4   # K = 64: the band rides in the last radial pass
     if case == "bckg":
         kw.update(n_ens=1, km2=3, km3=2, mgbf_proc=1)
     if case == "fast":
@@ -370,15 +372,17 @@ def test_lquart_procedures_match_oracle(monkeypatch, case):
         rhs = op.dot(vd, wt)
         scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
         assert abs(lhs - rhs) <= 1e-6 * scale
-        if case == "ens":
+        if case in ("ens", "ens_band"):
             # the quadratic restriction rides in the first radial pass of
             # every generation but the last (half-width 6): no restriction
-            # launch of its own
+            # launch of its own; with K % 64 == 0 the vertical band rides in
+            # the last pass after the quadratic prolongation
             op.timers(True)
             op.filtering_procedure(xd)
             ph = op.read_timers()
             op.timers(False)
             assert ph["upsend"][1] == 0 and ph["hfiltT"][1] == d.gm
+            assert ph["vfilt"][1] == (0 if case == "ens_band" else 1)
 
 
 @pytest.mark.parametrize("case", ["ens", "ens_exact_order", "ens_no_tables", "ens_filt_g1_off",
