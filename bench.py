@@ -147,6 +147,24 @@ def row_cpu_baseline(MG, cfg, row):
             "sample": f"{what}, FP64 C restatement with OpenMP, {tc:.2f} s"}
 
 
+def pin_or_tune(args, op, log):
+    """Radial tiles: pinned from --tiles, else autotuned unless --no-tune
+    (then the defaults, reported as None)."""
+    if args.tiles:
+        pinned = {(int(k[1:k.index("k")]), int(k[k.index("k") + 1:])): tuple(v)
+                  for k, v in json.loads(args.tiles).items()}
+        op.set_tiles(pinned)
+        tiles = op.tiles()
+        log(f"[bench] pinned tiles: {tiles}")
+        return tiles
+    if args.no_tune:
+        return None
+    t0 = time.perf_counter()
+    tiles = op.autotune()
+    log(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiles}")
+    return tiles
+
+
 def run_row(args, torch, MG):
     row = args.row
     weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
@@ -155,6 +173,7 @@ def run_row(args, torch, MG):
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
 
+    tiles = None
     if row == "transfer":
         M, n, lm, na, la = 16, 768, 64, 1152, 127
         cfg = MG.Config.for_grid(n, n, lm, n_ens=M, gm=gm, h=6, hz=3, p=2, weights=weights,
@@ -202,8 +221,7 @@ def run_row(args, torch, MG):
         x = torch.randn(op.field_shape(), generator=g, device="cuda", dtype=torch.float32)
         y = torch.empty_like(x)
         units = d.nx * d.ny * d.km_all
-        if not args.no_tune:
-            op.autotune()
+        tiles = pin_or_tune(args, op, log)
 
         def step():
             op.filtering_procedure(x, out=y)
@@ -254,6 +272,7 @@ def run_row(args, torch, MG):
                      "algorithmic_bytes_per_launch": k_bytes / max(k_n, 1)},
         "cpu_baseline": None,
         "phases": phase_out,
+        "tiles": {f"g{g}k{k}": list(v) for (g, k), v in tiles.items()} if tiles else None,
     }
     op.close()
     if not args.no_cpu:
@@ -313,17 +332,7 @@ def main():
     y = torch.empty_like(x)
     op = MG.MGBF(cfg)
     assert op.dims.gm == gm
-    tiles = None
-    if args.tiles:
-        pinned = {(int(k[1:k.index("k")]), int(k[k.index("k") + 1:])): tuple(v)
-                  for k, v in json.loads(args.tiles).items()}
-        op.set_tiles(pinned)
-        tiles = op.tiles()
-        log(f"[bench] pinned tiles: {tiles}")
-    elif not args.no_tune:
-        t0 = time.perf_counter()
-        tiles = op.autotune()
-        log(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiles}")
+    tiles = pin_or_tune(args, op, log)
 
     # ---- batched: one application over all members ----
     for _ in range(args.warmup):
