@@ -674,23 +674,35 @@ bool plan_ball3(const mgbf::Setup& s, const mgbf_config& cfg, mgbf::Ball3Coef& c
     return lm >= 2 * HZ + 2;
 }
 
+// Rows per wave of the radial pass over the 2-D variables (k >= K3) of
+// procedures 4 and 5: even, so that a restriction can ride in it.
+int rad2d_tj(const mgbf_ctx* c, int g) {
+    const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
+    int tj = pick_tj(c->K - K3, c->nx[g], c->ny[g], 4);
+    if (tj & 1) tj = tj + 1 <= c->ny[g] ? tj + 1 : tj - 1;
+    return tj;
+}
+
 // The 3-D filter of generation g (F, or F^T with transpose). With `scaled`
 // given, the output is also multiplied by the generation's weights when the
-// streaming pass covers the whole stack, and *scaled says whether it was.
+// streaming pass runs, and *scaled says whether it was.
 // coarse (forward only, when ball3_pro(c)): the prolongation of generation
 // g+1 is added to the output, s_g = F in + P s_{g+1}.
+// rout (transposed only): the restriction of `in` to generation g+1.
+// The 2-D variables after the 3-D ones (k >= K3) take the same epilogues in
+// a radial pass of their own.
 int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st,
          bool* scaled = nullptr, const float* coarse = nullptr, float* rout = nullptr) {
     Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st,
              8.0 * (double)c->gsize[g] + (coarse || rout ? 4.0 * (double)c->gsize[g + 1] : 0.0));
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     if (scaled) *scaled = false;
-    if (coarse && (transpose || !c->ball3 || K3 < c->K))
+    if (coarse && (transpose || !c->ball3))
         return fail(MGBF_ERR_ARG, "internal: prolongation fused into a 3-D pass that cannot take it");
-    if (rout && (!transpose || !c->ball3 || K3 < c->K))
+    if (rout && (!transpose || !c->ball3))
         return fail(MGBF_ERR_ARG, "internal: restriction fused into a 3-D pass that cannot take it");
     if (c->ball3) {
-        const bool fuse = scaled && K3 >= c->K;
+        const bool fuse = scaled != nullptr;
         mgbf::Ball3Args b{};
         b.in = in;
         b.out = out;
@@ -715,6 +727,38 @@ int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStr
         b.coef = c->b3;
         HIP_TRY(mgbf::launch_ball3(b, st), "3-D filter launch");
         if (K3 >= c->K) return MGBF_OK;
+        // the 2-D variables (k >= K3): the procedure's 2-D filter (radial disc
+        // or line product, even in x and y, so F^T = F) streams through the
+        // radial kernel from k = K3 on, with the same weighting, restriction
+        // or prolongation as the 3-D pass
+        mgbf::RadialArgs r{};
+        r.in = in;
+        r.out = out;
+        r.wfield = fuse ? c->wfield[g] : nullptr;
+        r.wconst = fuse ? c->wconst[g] : 1.f;
+        r.coarse = coarse;
+        r.lm = c->s.cfg.lm;
+        r.nfilt = c->nfilt;
+        r.K = c->K;
+        r.klo = K3;
+        r.n = c->nx[g];
+        r.m = c->ny[g];
+        // the restriction needs 3 halo rows: a narrower filter runs at
+        // half-width 3 with zero taps beyond its own (coef is zero there)
+        r.H = rout ? std::max(c->s.H, 3) : c->s.H;
+        r.TI = 4;
+        r.TJ = rad2d_tj(c, g);
+        if (rout) {
+            r.rout = rout;
+            r.rytab = c->d_rytab[g];
+            r.rymeta = c->d_rymeta[g];
+        }
+        r.exact_disc = c->s.exact_disc;
+        r.separable = c->s.separable;
+        r.quad = c->quad;
+        r.coef = c->coef;
+        HIP_TRY(mgbf::launch_radial(r, st), "2-D filter launch");
+        return MGBF_OK;
     }
     mgbf::BallArgs a{};
     a.in = in;
@@ -747,9 +791,10 @@ int rad3_upsweep(mgbf_ctx* c, float* U, float* y, hipStream_t st) {
 }
 
 // The streaming 3-D pass takes the prolongation of the up-sweep in its
-// epilogue when it covers the whole stack and the transfer is the 4-point one.
+// epilogue when the transfer is the 4-point one (the 2-D variables after the
+// 3-D ones take it in their radial pass).
 bool ball3_pro(const mgbf_ctx* c) {
-    return c->ball3 && !c->quad && c->s.dims.km3_all * c->s.cfg.lm >= c->K;
+    return c->ball3 && !c->quad;
 }
 
 // Forward passes from the coarsest generation up, each adding the
@@ -771,8 +816,10 @@ int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scal
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     // the streaming pass loads at least 3 halo rows when it restricts
     // (the 3-D kernel's restriction carries the 4-point stencil only)
-    const bool res = next && c->ball3 && !c->quad && K3 >= c->K && c->fuse >= 3 &&
-                     fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), std::max(c->b3H, 3));
+    // (the 2-D variables after K3 restrict in their radial pass, at H >= 3)
+    const bool res = next && c->ball3 && !c->quad && c->fuse >= 3 &&
+                     fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), std::max(c->b3H, 3)) &&
+                     (K3 >= c->K || fused_ok(c, g, rad2d_tj(c, g), std::max(c->s.H, 3)));
     if (next && !res) CALL(restrict_g(c, g, ug, next, st));
     return ball(c, g, ug, T + c->goff[g], true, st, scaled, nullptr, res ? next : nullptr);
 }

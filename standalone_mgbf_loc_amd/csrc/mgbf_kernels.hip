@@ -387,6 +387,9 @@ unsigned grid_for(long long count) {
 }  // namespace
 
 hipError_t launch_radial(const RadialArgs& a, hipStream_t st) {
+    // a k offset takes no band (whole columns are not in the wave) and no
+    // matrix-core engine
+    if (a.klo < 0 || a.klo >= a.K || (a.klo && (a.mfma || a.band))) return hipErrorInvalidValue;
     switch (a.H) {
         case 1: return launch_radial_H1(a, st);
         case 2: return launch_radial_H2(a, st);

@@ -24,6 +24,8 @@ struct RadialArgs {
     int vhb;              // its half-width
     int lm, nfilt;        // column length and vertically filtered part of the stack
     int K, n, m;
+    int klo;              // first k filtered (k < klo is left alone): 0, or K3 for
+                          // the 2-D variables after the 3-D ones of procedures 4, 5
     int H;                // max(hx, hy)
     int TI;               // output columns per lane (2 or 4)
     // matrix-core engine (first pass of a generation)
@@ -53,6 +55,7 @@ struct RadialParams {
     const float* band;
     float wconst;
     int K, n, m, TJ, lm, nfilt;
+    int klo;                  // first k of the pass
     int ntile_i, ntile_j, nitems;
     // restriction to the next generation fused into the pass (RES)
     float* rout;              // coarse output (n/2 x m/2 x K), or nullptr

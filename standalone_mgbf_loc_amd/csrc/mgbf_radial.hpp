@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     const int rest = item / p.ntile_i;
     const int tj = rest % p.ntile_j;
     const int kc = rest / p.ntile_j;
-    const int k = kc * kWave + lane;
+    const int k = p.klo + kc * kWave + lane;
     const bool kvalid = k < K;
     const int kk = kvalid ? k : K - 1;
     const int i0 = ti * TI, j0 = tj * p.TJ;
@@ -432,6 +432,7 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     p.coarse = a.coarse;
     p.band = a.band;
     p.K = a.K;
+    p.klo = a.klo;
     p.n = a.n;
     p.m = a.m;
     p.TJ = a.TJ;
@@ -444,7 +445,7 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     p.rkmax = a.rkmax;
     p.ntile_i = (a.n + TI - 1) / TI;
     p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
-    const int nkc = (a.K + kWave - 1) / kWave;
+    const int nkc = (a.K - a.klo + kWave - 1) / kWave;
     const long long nitems = (long long)p.ntile_i * p.ntile_j * nkc;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
