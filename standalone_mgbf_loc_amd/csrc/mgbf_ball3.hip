@@ -33,9 +33,10 @@ namespace mgbf {
 
 namespace {
 
-template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false, bool QR = false>
 __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     static_assert(!RES || !PRO, "restriction epilogue shape");
+    static_assert(!QR || RES, "QR selects the quadratic restriction (lquart)");
     constexpr int TI = 4;
     constexpr int NR = 2 * H + 1;
     // columns loaded on each side: H for the filter, 4 for the restriction's
@@ -188,13 +189,15 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                 if constexpr (RES) {
                     const int vr = j0 - HR + rr;   // fine row, before mirroring
                     if (vr >= 0 && vr < m) {
+                        using RS = Stencil<QR>;
                         float xr[NRC];
-                        if (xcls == 0) {          // fine columns 2I-3 .. 2I+4
+                        if (xcls == 0) {          // fine columns 2I+ROFF .. 2I+ROFF+NR-1
 #pragma unroll
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kW8[t], vraw[HL - 3 + 2 * c + t], a);
+                                for (int t = 0; t < RS::NR; ++t)
+                                    a = __builtin_fmaf(RS::R[t], vraw[HL + RS::ROFF + 2 * c + t], a);
                                 xr[c] = a;
                             }
                         } else if (xcls == 1) {   // fine columns 0 .. 7
@@ -202,7 +205,8 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                             for (int c = 0; c < NRC; ++c) {
                                 float a = 0.f;
 #pragma unroll
-                                for (int t = 0; t < 8; ++t) a = __builtin_fmaf(kEdgeRW.w[c][t], vraw[HL + t], a);
+                                for (int t = 0; t < 8; ++t)
+                                    a = __builtin_fmaf(QR ? kEdgeRWQ.w[c][t] : kEdgeRW.w[c][t], vraw[HL + t], a);
                                 xr[c] = a;
                             }
                         } else {                  // fine columns n-8 .. n-1
@@ -211,30 +215,36 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                                 float a = 0.f;
 #pragma unroll
                                 for (int t = 0; t < 8; ++t)
-                                    a = __builtin_fmaf(kEdgeRW.w[NRC - 1 - c][7 - t], vraw[HL - 4 + t], a);
+                                    a = __builtin_fmaf(QR ? kEdgeRWQ.w[NRC - 1 - c][7 - t] : kEdgeRW.w[NRC - 1 - c][7 - t],
+                                                       vraw[HL - 4 + t], a);
                                 xr[c] = a;
                             }
                         }
                         if (vr >= 8 && vr <= m - 9) {
-                            // interior fine row: coarse rows jb .. jb+3 are open;
-                            // an even row reads taps 7,5,3,1 of their windows
-                            // and closes row jb, an odd row reads taps 6,4,2,0
-                            const bool odd = (vr & 1) != 0;
-                            const float yw[4] = {odd ? kW8[6] : kW8[7], odd ? kW8[4] : kW8[5],
-                                                 odd ? kW8[2] : kW8[3], odd ? kW8[0] : kW8[1]};
+                            // interior fine row: coarse rows jb .. jb+NO-1 are
+                            // open; a row of parity CP closes row jb and reads
+                            // taps NR-1, NR-3, .. of their windows, a row of
+                            // the other parity taps NR-2, NR-4, ..
+                            constexpr int NO = RS::NR / 2;
+                            constexpr int CP = (RS::ROFF + RS::NR - 1) & 1;
+                            const bool closes = (vr & 1) == CP;
+                            float yw[NO];
 #pragma unroll
-                            for (int q = 0; q < 4; ++q)
+                            for (int q = 0; q < NO; ++q)
+                                yw[q] = closes ? RS::R[RS::NR - 1 - 2 * q] : RS::R[RS::NR - 2 - 2 * q];
+#pragma unroll
+                            for (int q = 0; q < NO; ++q)
 #pragma unroll
                                 for (int c = 0; c < NRC; ++c) racc[q][c] = __builtin_fmaf(yw[q], xr[c], racc[q][c]);
-                            if (!odd) {
-                                const int J = (vr - 3) >> 1;
+                            if (closes) {
+                                const int J = (vr - RS::ROFF - RS::NR + 1) >> 1;
                                 if (J >= Jlo && J < Jhi && kvalid) store_coarse(J, 0);
 #pragma unroll
-                                for (int q = 0; q < 3; ++q)
+                                for (int q = 0; q + 1 < NO; ++q)
 #pragma unroll
                                     for (int c = 0; c < NRC; ++c) racc[q][c] = racc[q + 1][c];
 #pragma unroll
-                                for (int c = 0; c < NRC; ++c) racc[3][c] = 0.f;
+                                for (int c = 0; c < NRC; ++c) racc[NO - 1][c] = 0.f;
                             }
                         } else {
                             // rows next to an edge: weights and closing rows from the table
@@ -361,7 +371,7 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     }
 }
 
-template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false>
+template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false, bool QR = false>
 hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     Ball3Params p{};
     p.in = a.in;
@@ -386,7 +396,7 @@ hipError_t launch_variant(const Ball3Args& a, hipStream_t st) {
     if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
     p.nitems = (int)nitems;
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
-    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL, PRO, RES>), dim3(nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((ball3_kernel<H, HZ, BALL, T, ZL, PRO, RES, QR>), dim3(nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
@@ -398,7 +408,11 @@ hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
     if (a.zero_levels) {
         // zeros beyond the column ends: Z_c^T = Z_c, the box of the line products
         if (a.ball) return hipErrorInvalidValue;
-        if (a.rout) return pro ? hipErrorInvalidValue : launch_variant<H, HZ, false, false, true, false, true>(a, st);
+        if (a.rout) {
+            if (pro) return hipErrorInvalidValue;
+            return a.quad ? launch_variant<H, HZ, false, false, true, false, true, true>(a, st)
+                          : launch_variant<H, HZ, false, false, true, false, true>(a, st);
+        }
         return pro ? launch_variant<H, HZ, false, false, true, true>(a, st)
                    : launch_variant<H, HZ, false, false, true, false>(a, st);
     }
@@ -406,7 +420,8 @@ hipError_t launch_hz(const Ball3Args& a, hipStream_t st) {
     if constexpr (HZ == H) {
         if (a.rout) {   // the transposed pass with the fused restriction
             if (!a.transpose || pro) return hipErrorInvalidValue;
-            return launch_variant<H, HZ, true, true, false, false, true>(a, st);
+            return a.quad ? launch_variant<H, HZ, true, true, false, false, true, true>(a, st)
+                          : launch_variant<H, HZ, true, true, false, false, true>(a, st);
         }
         if (a.transpose) return launch_variant<H, HZ, true, true, false, false>(a, st);
         return pro ? launch_variant<H, HZ, true, false, false, true>(a, st)

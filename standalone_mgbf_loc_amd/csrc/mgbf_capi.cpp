@@ -336,15 +336,27 @@ bool build_res_tables(const mgbf::AxisTable& rx, const mgbf::AxisTable& ry, int 
 // Can the restriction of generation g ride in its first radial pass with
 // output rows per wave TJ? The wave of output rows j0 .. j0+TJ-1 loads fine
 // rows j0-H .. j0+TJ-1+H and owns coarse rows j0/2 .. (j0+TJ)/2-1, whose
-// 8-row windows must lie inside what it loads.
+// windows (8 rows, 6 for the quadratic transfer) must lie inside what it
+// loads.
 bool fused_ok(const mgbf_ctx* c, int g, int tj, int H) {
     if (g + 1 >= c->gm || !c->res_ok[g] || H < 3 || tj < 2 || (tj & 1)) return false;
     const int m = c->ny[g];
-    const std::vector<int>& st = c->s.ry[g].start;
+    // each coarse row's window: its first and last fine row of nonzero weight
+    const mgbf::AxisTable& ry = c->s.ry[g];
+    auto first = [&](int J) {
+        for (int t = 0; t < ry.width; ++t)
+            if (ry.w[(size_t)J * ry.width + t] != 0.0) return ry.start[J] + t;
+        return ry.start[J];
+    };
+    auto last = [&](int J) {
+        for (int t = ry.width - 1; t >= 0; --t)
+            if (ry.w[(size_t)J * ry.width + t] != 0.0) return ry.start[J] + t;
+        return ry.start[J];
+    };
     for (int j0 = 0; j0 < m; j0 += tj) {
         const int te = tj < m - j0 ? tj : m - j0;
         for (int J = j0 / 2; J < (j0 + te) / 2; ++J)
-            if (st[J] < j0 - H || st[J] + 7 > j0 + te - 1 + H) return false;
+            if (first(J) < j0 - H || last(J) > j0 + te - 1 + H) return false;
     }
     return true;
 }
@@ -724,6 +736,7 @@ int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStr
         b.ball = c->b3ball;
         b.transpose = transpose ? 1 : 0;
         b.zero_levels = c->s.zero_levels ? 1 : 0;
+        b.quad = c->quad ? 1 : 0;
         b.coef = c->b3;
         HIP_TRY(mgbf::launch_ball3(b, st), "3-D filter launch");
         if (K3 >= c->K) return MGBF_OK;
@@ -815,9 +828,8 @@ int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scal
     float* next = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     // the streaming pass loads at least 3 halo rows when it restricts
-    // (the 3-D kernel's restriction carries the 4-point stencil only)
     // (the 2-D variables after K3 restrict in their radial pass, at H >= 3)
-    const bool res = next && c->ball3 && !c->quad && c->fuse >= 3 &&
+    const bool res = next && c->ball3 && c->fuse >= 3 &&
                      fused_ok(c, g, pick_tj(K3, c->nx[g], c->ny[g], 4), std::max(c->b3H, 3)) &&
                      (K3 >= c->K || fused_ok(c, g, rad2d_tj(c, g), std::max(c->s.H, 3)));
     if (next && !res) CALL(restrict_g(c, g, ug, next, st));

@@ -207,6 +207,7 @@ struct Ball3Args {
     int K, n, m, lm, K3;   // only k < K3 is written
     int H, HZ, TJ;
     int ball, transpose, zero_levels;
+    int quad;              // the restriction is the quadratic one (lquart)
     Ball3Coef coef;
 };
 struct Ball3Params {
