@@ -190,8 +190,9 @@ def run_row(args, torch, MG):
         proc = int(row[4:])
         M, n, lm = 1, 1536, 64
         cfg = MG.Config.for_grid(n, n, lm, n_ens=1, gm=gm, h=6, hz=3, p=2, weights=weights,
-                                 mgbf_proc=proc, km3=8, mgbf_line=proc in (2, 5))
-        workload = (f"{ROW_NAMES[row]}: one field of 8 variables x {lm} levels on {n}x{n}, "
+                                 mgbf_proc=proc, km3=8, km2=args.km2, mgbf_line=proc in (2, 5))
+        extra = f" plus {args.km2} 2-D variables" if args.km2 else ""
+        workload = (f"{ROW_NAMES[row]}: one field of 8 variables x {lm} levels{extra} on {n}x{n}, "
                     f"{gm} generations, h=6, hz=3, p=2")
     if row in ("proc2", "proc3", "proc7", "proc8"):
         # the line and fast filters take mg_ampl02 as their factor: 0.4 is
@@ -297,6 +298,8 @@ def main():
                     help='pin radial tiles instead of autotuning, JSON as printed under "tiles", '
                          'e.g. \'{"g1k0": [4, 384], ...}\'')
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
+    ap.add_argument("--km2", type=int, default=0,
+                    help="2-D variables after the 3-D ones in the background rows proc1..proc5")
     ap.add_argument("--row", choices=ROWS, default=None,
                     help="measure one of the other procedures or the analysis transfer instead "
                          "(one JSON line with the same keys; no plain or CPU leg)")
