@@ -308,7 +308,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
-                          "(the operator shards by member with no exchange; see DESIGN.md)"}))
+                          "(the operator would shard by member with no exchange between GPUs)"}))
         sys.exit(2)
 
     import torch

@@ -52,8 +52,17 @@ typedef struct mgbf_config {
     double mg_weig1, mg_weig2, mg_weig3, mg_weig4, mg_weig5; /* generation weights (g>=5 -> weig5) */
     int hx, hy, hz;  /* halo widths = maximum filter reach */
     int p;           /* beta exponent (integer in the reference) */
-    int mgbf_line;   /* line (dibeta) filters: not built */
-    int mgbf_proc;   /* filtering procedure: 1 (rad2_bckg) and 6 (rad2_ens) are built */
+    int mgbf_line;   /* init_mg_line: the triad (procedures 2, 7) and hexad (procedure 5)
+                        line filters; without it every line is the identity. Lines off
+                        the grid axes (a non-diagonal aspect tensor, which def_mg_weights
+                        never builds) are refused with MGBF_ERR_UNSUPPORTED */
+    int mgbf_proc;   /* filtering procedure (filtering_procedure, mg_filtering.f90:67-120),
+                        all built: 1 rad2_bckg, 2 lin2_bckg, 3 fast_bckg, 4 rad3, 5 lin3,
+                        6 rad2_ens, 7 lin2_ens, 8 fast_ens. l_filt_g1 is read by 6-8 only.
+                        Procedure 7 ends with Vf^T as the reference's does, so its W A W^T
+                        is not symmetric; km2 > 0 is refused with 6 (the reference's
+                        ensemble vertical filter strides the stack wrongly there,
+                        mg_filtering.f90:2066-2069) */
     int n_ens;       /* ensemble members, stacked in k */
     int lm_a, lm;    /* analysis / filter vertical levels */
     int km2, km3;    /* 2-D and 3-D variables */

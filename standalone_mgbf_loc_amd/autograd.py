@@ -71,7 +71,8 @@ class _Localize(torch.autograd.Function):
 
 
 def _filters_g1(op: M.MGBF) -> bool:
-    return op.cfg.mgbf_proc in (1, 2, 3, 4) or bool(op.cfg.l_filt_g1)
+    # l_filt_g1 is read by the ensemble procedures (6-8) only
+    return op.cfg.mgbf_proc in (1, 2, 3, 4, 5) or bool(op.cfg.l_filt_g1)
 
 
 def localize(op: M.MGBF, z: torch.Tensor, weights) -> torch.Tensor:

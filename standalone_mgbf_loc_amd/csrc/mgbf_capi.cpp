@@ -73,6 +73,10 @@ struct mgbf_ctx {
     int K = 0, gm = 0;
     int nfilt = 0;          // vertically filtered part of the stack (0: none)
     bool filt_g1 = true;
+    // bumped by every call that changes what the launches read (weights,
+    // tiles, engine choice): a group graph captured at an older version is
+    // captured again
+    unsigned long long version = 0;
     bool quad = false;      // lquart: quadratic inter-generation transfer
     int vf_last = 0;        // vertical band that ends W A W^T in the reference's order
     mgbf::RadialCoef coef{};
@@ -1048,10 +1052,13 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     c->device = device;
     c->K = s.dims.km_all;
     c->gm = s.dims.gm;
-    // the background procedures filter generation 1 whatever l_filt_g1 says
-    c->filt_g1 = cfg->mgbf_proc == 1 || cfg->mgbf_proc == 2 || cfg->mgbf_proc == 3
-                     ? true
-                     : cfg->l_filt_g1 != 0;
+    // l_filt_g1 is read by the ensemble procedures only: the background
+    // procedures (1-3) and the 3-D ones (4, 5: filtering_rad3 and
+    // filtering_lin3 weight through weighting_all -> weighting, which scales
+    // generation 1 unconditionally, mg_filtering.f90:203, 427,
+    // mg_generations.f90:133-155, 973-1009) filter and weight generation 1
+    // whatever it says
+    c->filt_g1 = (cfg->mgbf_proc >= 1 && cfg->mgbf_proc <= 5) ? true : cfg->l_filt_g1 != 0;
     // filtering_lin2_ens ends with sup_vrbeta1T_ens (mg_filtering.f90:1303-1305)
     c->vf_last = cfg->mgbf_proc == 7 ? BAND_VFT : BAND_VF;
     c->nfilt = cfg->l_vertical_filter ? s.dims.km3_all * cfg->lm : 0;
@@ -1373,6 +1380,7 @@ int mgbf_set_weights(mgbf_ctx* c, int gen, const float* field, double value) {
     if (gen < 1 || gen > c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
     c->wfield[gen - 1] = field;
     if (!field) c->wconst[gen - 1] = (float)value;
+    ++c->version;
     return MGBF_OK;
 }
 
@@ -1535,6 +1543,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
         }
     }
     c->timing = timing;
+    ++c->version;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return status;
@@ -1560,6 +1569,7 @@ int mgbf_set_tile(mgbf_ctx* c, int gen, int kind, int ti, int tj) {
     c->tile_ti[gen - 1][kind] = ti == 16 ? 4 : ti;
     c->tile_tj[gen - 1][kind] = tj;
     c->tile_mfma[gen - 1][kind] = ti == 16 ? 1 : 0;
+    ++c->version;
     return MGBF_OK;
 }
 
@@ -1578,6 +1588,7 @@ struct mgbf_group {
     int key_op = -1;
     std::vector<const float*> key_in;
     std::vector<float*> key_out;
+    std::vector<unsigned long long> key_version;   // each context's version at capture
 };
 
 namespace {
@@ -1630,6 +1641,8 @@ int group_capture(mgbf_group* g, int op, const float* const* in, float* const* o
     g->key_op = op;
     g->key_in.assign(in, in + n);
     g->key_out.assign(out, out + n);
+    g->key_version.resize(n);
+    for (int i = 0; i < n; ++i) g->key_version[i] = g->ctx[i]->version;
     return MGBF_OK;
 }
 
@@ -1669,9 +1682,10 @@ int mgbf_group_apply(mgbf_group* g, int op, const float* const* in, float* const
     for (int i = 0; i < n; ++i)
         if (!in[i] || !out[i]) return fail(MGBF_ERR_ARG, "NULL buffer in group apply");
     DeviceGuard guard(g->device);
-    const bool same = g->exec && g->key_op == op &&
-                      std::equal(g->key_in.begin(), g->key_in.end(), in) &&
-                      std::equal(g->key_out.begin(), g->key_out.end(), out);
+    bool same = g->exec && g->key_op == op &&
+                std::equal(g->key_in.begin(), g->key_in.end(), in) &&
+                std::equal(g->key_out.begin(), g->key_out.end(), out);
+    for (int i = 0; i < n && same; ++i) same = g->key_version[i] == g->ctx[i]->version;
     if (!same) {
         int st = group_capture(g, op, in, out);
         if (st != MGBF_OK) return st;

@@ -399,43 +399,6 @@ void line_moments(int p, double* bnorm, double* bsprds) {
     }
 }
 
-// triad (jp_pbfil3.f90:573-624): aspect (A11, A22, A12) -> generators
-// l[c] and weights w[c] >= 0, from the superbase (1,0), (0,1), (-1,-1),
-// flipping the line of the most negative weight until none is negative.
-bool line_triad(const double a[3], int l[3][2], double w[3]) {
-    int L[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
-    int lui[3][3] = {{1, 0, -1}, {0, 1, -1}, {0, 0, 1}};   // lui[c][r]
-    for (int c = 0; c < 3; ++c) w[c] = a[0] * lui[c][0] + a[1] * lui[c][1] + a[2] * lui[c][2];
-    bool done = false;
-    for (int it = 0; it < 200 && !done; ++it) {
-        int k = 0;
-        for (int c = 1; c < 3; ++c)
-            if (w[c] < w[k]) k = c;
-        const double dw = 2.0 * w[k];
-        if (dw >= -1.e-14) {
-            done = true;
-            break;
-        }
-        const int lc = (k + 1) % 3, mc = (lc + 1) % 3;
-        int dl[3];
-        for (int r = 0; r < 3; ++r) dl[r] = 2 * lui[k][r];
-        for (int e = 0; e < 2; ++e) {
-            L[lc][e] = -L[lc][e];
-            L[k][e] = -L[lc][e] - L[mc][e];
-        }
-        for (int c = 0; c < 3; ++c) {
-            const int sg = c == k ? -1 : 1;
-            for (int r = 0; r < 3; ++r) lui[c][r] += dl[r] * sg;
-            w[c] += sg * dw;
-        }
-    }
-    for (int c = 0; c < 3; ++c) {
-        l[c][0] = L[c][0];
-        l[c][1] = L[c][1];
-    }
-    return done;
-}
-
 // hstform (jp_pbfil3.f90:1399-1430): spread**2 -> real half-span by linear
 // interpolation in the table of spreads**2; false when negative or beyond it.
 bool line_hstform(const double* bsprds, double& hs) {
@@ -473,82 +436,47 @@ bool is_identity(const std::vector<double>& fil) {
     return id;
 }
 
-// hexad (jp_pbfil3.f90:798-904): aspect (A11, A22, A33, A23, A31, A12) ->
-// generators l7[c] and weights w7[c] of colors c+1 = 1..7 (one null), from
-// the default hexad, iterating its K-set of three generators with the
-// Fano-plane tables until no weight is negative.
-bool line_hexad(const double a[6], int l7[7][3], double w7[7]) {
-    static const int deflhex[7][3] = {{0, 0, 0}, {1, -1, 0}, {0, 1, -1}, {0, 0, 1},
-                                      {-1, 0, 1}, {0, 1, 0}, {1, 0, 0}};
-    static const int deflui[7][6] = {{0, 0, 0, 0, 0, 0},  {0, 0, 0, 0, 0, -1}, {0, 0, 0, -1, 0, 0},
-                                     {0, 0, 1, 1, 1, 0},  {0, 0, 0, 0, -1, 0}, {0, 1, 0, 1, 0, 1},
-                                     {1, 0, 0, 0, 1, 1}};
-    static const int mm[6][3][3] = {{{1, 1, -1}, {1, 0, 0}, {1, 0, -1}},
-                                    {{-1, 1, 0}, {-1, 1, 1}, {0, 1, 0}},
-                                    {{0, -1, 1}, {1, -1, 0}, {1, 0, 0}},
-                                    {{0, 0, 1}, {0, -1, 1}, {1, -1, 1}},
-                                    {{-1, 0, 1}, {0, 0, 1}, {-1, 1, 0}},
-                                    {{0, 1, 0}, {1, 0, -1}, {0, 1, -1}}};   // [d][column][row]
-    static const int ttriad[6] = {5, 3, 3, 6, 5, 6};
-    static const int sstriad[13] = {-1, -1, 1, -1, 1, 1, 1, -1, -1, 1, -1, 1, 1};   // -6..6
-    static const int line3[3] = {1, 2, 4};
-    static const int jcol[7] = {7, 4, 6, 3, 5, 2, 1};
-    auto mod7 = [](int q) { return ((q % 7) + 7) % 7; };
-    int lhex[7][3], lui[7][6], kset[3][3];
-    double w[7];
-    std::copy(&deflhex[0][0], &deflhex[0][0] + 21, &lhex[0][0]);
-    std::copy(&deflui[0][0], &deflui[0][0] + 42, &lui[0][0]);
-    int hcol = 0, kcol = 0;
-    for (int c = 0; c < 7; ++c) {
-        w[c] = 0.0;
-        for (int r = 0; r < 6; ++r) w[c] += a[r] * lui[c][r];
-    }
-    for (int i = 0; i < 3; ++i)
-        for (int e = 0; e < 3; ++e) kset[i][e] = lhex[mod7(hcol - line3[i])][e];
-    bool done = false;
-    for (int it = 0; it < 200 && !done; ++it) {
-        kcol = 0;
-        for (int c = 1; c < 7; ++c)
-            if (w[c] < w[kcol]) kcol = c;
-        const double dw = w[kcol];
-        if (dw >= -1.e-14) {
-            done = true;
-            break;
-        }
-        const int dcol = mod7(kcol - hcol);
-        hcol = kcol;
-        const int L = (hcol + ttriad[dcol - 1]) % 7;
-        int kn[3][3];
-        for (int j = 0; j < 3; ++j)
-            for (int e = 0; e < 3; ++e) {
-                int acc = 0;
-                for (int i = 0; i < 3; ++i) acc += kset[i][e] * mm[dcol - 1][j][i];
-                kn[j][e] = acc;
-            }
-        std::copy(&kn[0][0], &kn[0][0] + 9, &kset[0][0]);
-        int dl[6];
-        for (int r = 0; r < 6; ++r) dl[r] = lui[hcol][r];
-        for (int c = 0; c < 7; ++c) {
-            const int sg = sstriad[c - L + 6];
-            for (int r = 0; r < 6; ++r) lui[c][r] += dl[r] * sg;
-            w[c] += sg * dw;
-        }
-    }
-    if (!done) return false;
-    for (int i = 0; i < 3; ++i)
-        for (int e = 0; e < 3; ++e) {
-            lhex[mod7(hcol - line3[i])][e] = kset[i][e];
-            lhex[(hcol + line3[i]) % 7][e] = kset[i][e] - kset[(i + 1) % 3][e];
-        }
-    for (int e = 0; e < 3; ++e) lhex[kcol][e] = 0;
-    for (int i = 0; i < 7; ++i) {
-        for (int e = 0; e < 3; ++e) l7[jcol[i] - 1][e] = lhex[i][e];
-        w7[jcol[i] - 1] = w[i];
+}  // namespace
+
+// Line decompositions of the aspect tensor that init_mg_line hands to
+// tritform / hextform (mg_intstate.f90:436-480). A namelist reaches diagonal
+// tensors only (def_mg_weights fixes p_del = p_eps = 0, mg_intstate.f90:
+// 316-319), so both are built in closed form for A = diag(a_1, .., a_d),
+// every a_e > 0, and anything else is refused (false):
+//   A = sum_c w_c l_c l_c^T, w_c >= 0, over the lines of a superbase.
+// The axes with w = a_e already give A; every other line of the superbase
+// carries w = 0 and its filter is the identity. The colors (the order in
+// which the reference applies the lines) are those of the reference's
+// starting superbase, which its flip iterations (jp_pbfil3.f90 triad and
+// hexad) leave alone when no starting weight is negative, as here.
+//
+// 2-D (tritform): colors 1, 2 = the x and y axes with a_1, a_2; color 3 =
+// the idle diagonal (-1, -1). a = (A11, A22, A12).
+bool line_triad(const double a[3], int l[3][2], double w[3]) {
+    if (a[2] != 0.0 || !(a[0] > 0.0) || !(a[1] > 0.0)) return false;
+    const int lines[3][2] = {{1, 0}, {0, 1}, {-1, -1}};
+    const double wt[3] = {a[0], a[1], 0.0};
+    for (int c = 0; c < 3; ++c) {
+        l[c][0] = lines[c][0];
+        l[c][1] = lines[c][1];
+        w[c] = wt[c];
     }
     return true;
 }
 
-}  // namespace
+// 3-D (hextform): colors 1, 2, 3 = the x, y and level axes with a_1, a_2,
+// a_3; colors 4, 5, 6 = the idle face diagonals (1,-1,0), (-1,0,1),
+// (0,1,-1); color 7 = the null generator. a = (A11, A22, A33, A23, A31, A12).
+bool line_hexad(const double a[6], int l7[7][3], double w7[7]) {
+    if (a[3] != 0.0 || a[4] != 0.0 || a[5] != 0.0) return false;
+    if (!(a[0] > 0.0) || !(a[1] > 0.0) || !(a[2] > 0.0)) return false;
+    const int diag[3][3] = {{1, -1, 0}, {-1, 0, 1}, {0, 1, -1}};
+    for (int c = 0; c < 7; ++c) {
+        w7[c] = c < 3 ? a[c] : 0.0;
+        for (int e = 0; e < 3; ++e) l7[c][e] = c < 3 ? (e == c ? 1 : 0) : (c < 6 ? diag[c - 3][e] : 0);
+    }
+    return true;
+}
 
 int hex_taps(const mgbf_config& cfg, std::vector<Tap3>& taps, bool* zero_levels, std::string* err,
              std::vector<double>* factors) {

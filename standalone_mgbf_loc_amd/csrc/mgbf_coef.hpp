@@ -91,6 +91,13 @@ bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<dou
 // 2-D taps of the triad line filters of procedures 2 and 7 (mgbf_line), or
 // the identity without line tables. Returns MGBF_OK or an error status.
 int line_taps(const mgbf_config& cfg, std::vector<Tap2>& taps, std::string* err);
+// Line decompositions A = sum_c w_c l_c l_c^T (w_c >= 0) of a diagonal aspect
+// tensor, by color: the triad of tritform (a = A11, A22, A12; 3 lines) and
+// the hexad of hextform (a = A11, A22, A33, A23, A31, A12; 7 colors, one
+// null). False for a tensor with an off-diagonal entry or a non-positive
+// diagonal one, which no namelist reaches.
+bool line_triad(const double a[3], int l[3][2], double w[3]);
+bool line_hexad(const double a[6], int l7[7][3], double w7[7]);
 // 3-D taps of the hexad line filters of procedure 5 (mgbf_line): the x, y and
 // level lines as a product; *zero_levels: zeros beyond the column ends.
 // factors (3 vectors, optional): the x, y and level line filters whose

@@ -242,6 +242,54 @@ def derive(cfg: Config) -> Dims:
     return Dims(*[getattr(d, f) for f, _ in _CDims._fields_])
 
 
+def op_shapes(dims: "Dims", op: int):
+    """(input shape, output shape) of an operation (MGBF_OP_*); a multigrid
+    vector is flat (mg_size,). The C ABI takes bare pointers, so the wrapper
+    checks every buffer against these before a call."""
+    fld, mg, an = dims.gen_shape(1), (dims.mg_size,), dims.anal_shape()
+    table = {OP_WAWT: (fld, fld), OP_WT: (fld, mg), OP_W: (mg, fld), OP_A: (mg, mg),
+             OP_ASQRT: (mg, mg), OP_BSQRT: (mg, fld), OP_BSQRTT: (fld, mg),
+             OP_A2F: (an, fld), OP_A2F_ALL: (an, fld), OP_F2A: (fld, an), OP_F2A_ALL: (fld, an)}
+    if op not in table:
+        raise ValueError(f"unknown operation {op}")
+    return table[op]
+
+
+def stage_shapes(dims: "Dims", stage: int, gen: int):
+    """(input shape, output shape) of a single stage (MGBF_STAGE_*) on
+    generation gen (1-based)."""
+    if not 1 <= gen <= dims.gm:
+        raise ValueError(f"generation {gen} out of range 1..{dims.gm}")
+    g = dims.gen_shape
+    if stage in (STAGE_R, STAGE_P):
+        if gen >= dims.gm:
+            raise ValueError("no coarser generation")
+        return (g(gen), g(gen + 1)) if stage == STAGE_R else (g(gen + 1), g(gen))
+    if stage in (STAGE_VF, STAGE_VFT) and gen != 1:
+        raise ValueError("the vertical filter acts on generation 1")
+    if stage in (STAGE_VF, STAGE_VFT, STAGE_F, STAGE_FT, STAGE_A):
+        return g(gen), g(gen)
+    raise ValueError(f"unknown stage {stage}")
+
+
+def _numel(shape) -> int:
+    n = 1
+    for s in shape:
+        n *= int(s)
+    return n
+
+
+def check_sizes(what: str, x, out, ishape, oshape):
+    """Raises ValueError unless x (and out, when given) hold exactly the
+    elements of the expected shapes."""
+    if x.numel() != _numel(ishape):
+        raise ValueError(f"{what}: input has {x.numel()} elements, expected {_numel(ishape)} "
+                         f"(shape {tuple(ishape)})")
+    if out is not None and out.numel() != _numel(oshape):
+        raise ValueError(f"{what}: output has {out.numel()} elements, expected {_numel(oshape)} "
+                         f"(shape {tuple(oshape)})")
+
+
 def _ptr(t) -> int:
     import torch
     if not isinstance(t, torch.Tensor):
@@ -320,15 +368,18 @@ class MGBF:
             v = value if value is not None else self.cfg.weights()[min(gen, 5) - 1]
             _check(self._L.mgbf_set_weights(self._h, gen, None, float(v)))
 
+    # --- sizes: the C ABI takes bare pointers, so every call checks here ---
+    def op_shapes(self, op: int):
+        return op_shapes(self.dims, op)
+
+    def stage_shapes(self, stage: int, gen: int):
+        return stage_shapes(self.dims, stage, gen)
+
     # --- operators ---
     def apply(self, op: int, x, out=None, stream=None):
         import torch
-        if op in (OP_WAWT, OP_W, OP_BSQRT, OP_A2F, OP_A2F_ALL):
-            oshape = self.field_shape()
-        elif op in (OP_F2A, OP_F2A_ALL):
-            oshape = self.dims.anal_shape()
-        else:
-            oshape = (self.dims.mg_size,)
+        ishape, oshape = op_shapes(self.dims, op)
+        check_sizes(f"operation {op}", x, out, ishape, oshape)
         if out is None:
             out = torch.empty(oshape, dtype=torch.float32, device=x.device)
         _check(self._L.mgbf_apply(self._h, op, _ptr(x), _ptr(out), _stream(stream)))
@@ -368,12 +419,8 @@ class MGBF:
 
     def stage(self, stage: int, gen: int, x, out=None, stream=None):
         import torch
-        if stage == STAGE_R:
-            oshape = self.dims.gen_shape(gen + 1)
-        elif stage == STAGE_P:
-            oshape = self.dims.gen_shape(gen)
-        else:
-            oshape = self.dims.gen_shape(gen)
+        ishape, oshape = stage_shapes(self.dims, stage, gen)
+        check_sizes(f"stage {stage} on generation {gen}", x, out, ishape, oshape)
         if out is None:
             out = torch.empty(oshape, dtype=torch.float32, device=x.device)
         _check(self._L.mgbf_apply_stage(self._h, stage, gen, _ptr(x), _ptr(out), _stream(stream)))
@@ -439,6 +486,9 @@ class Group:
         n = len(self.ops)
         if len(xs) != n or len(outs) != n:
             raise ValueError("one input and one output per domain")
+        for o, x, y in zip(self.ops, xs, outs):
+            ishape, oshape = op_shapes(o.dims, op)
+            check_sizes(f"operation {op}", x, y, ishape, oshape)
         ins = (ctypes.c_void_p * n)(*[_ptr(x) for x in xs])
         ous = (ctypes.c_void_p * n)(*[_ptr(y) for y in outs])
         _check(self._L.mgbf_group_apply(self._h, op, ins, ous, _stream(stream)))

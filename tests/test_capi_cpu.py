@@ -205,3 +205,36 @@ def test_lwq_map_needs_three_levels_and_takes_any_lm_a_above_lm():
     for lm_a in (4, 10, 64):
         cfg = M.Config.for_grid(32, 24, 3, gm=3, h=4, hz=2, nm0=48, mm0=36, lm_a=lm_a, l_new_map=False)
         assert M.derive(cfg).km_a_all == lm_a
+
+
+def test_wrapper_sizes_every_operation_and_stage():
+    # the C ABI takes bare pointers: the wrapper checks every buffer's
+    # element count against the operation before a call
+    torch = pytest.importorskip("torch")
+    cfg = M.Config.for_grid(64, 48, 8, n_ens=2, gm=3, h=4, hz=2, nm0=96, mm0=72)
+    d = M.derive(cfg)
+    fld, mg, an = d.gen_shape(1), (d.mg_size,), d.anal_shape()
+    assert an == (72, 96, 16) and mg == (16 * (64 * 48 + 32 * 24 + 16 * 12),)
+    expect = {M.OP_WAWT: (fld, fld), M.OP_WT: (fld, mg), M.OP_W: (mg, fld), M.OP_A: (mg, mg),
+              M.OP_ASQRT: (mg, mg), M.OP_BSQRT: (mg, fld), M.OP_BSQRTT: (fld, mg),
+              M.OP_A2F: (an, fld), M.OP_F2A: (fld, an), M.OP_A2F_ALL: (an, fld),
+              M.OP_F2A_ALL: (fld, an)}
+    for op, shapes in expect.items():
+        assert M.op_shapes(d, op) == shapes
+    with pytest.raises(ValueError):
+        M.op_shapes(d, 99)
+    assert M.stage_shapes(d, M.STAGE_R, 2) == (d.gen_shape(2), d.gen_shape(3))
+    assert M.stage_shapes(d, M.STAGE_P, 1) == (d.gen_shape(2), d.gen_shape(1))
+    assert M.stage_shapes(d, M.STAGE_F, 3) == (d.gen_shape(3), d.gen_shape(3))
+    for bad in ((M.STAGE_R, 3), (M.STAGE_P, 3), (M.STAGE_VF, 2), (M.STAGE_F, 0), (M.STAGE_F, 4)):
+        with pytest.raises(ValueError):
+            M.stage_shapes(d, *bad)
+    # a field-sized vector where a multigrid vector belongs, and the reverse
+    x_fld = torch.empty(fld)
+    x_mg = torch.empty(mg)
+    M.check_sizes("W", x_mg, x_fld, *M.op_shapes(d, M.OP_W))
+    with pytest.raises(ValueError, match="input has"):
+        M.check_sizes("W", x_fld, x_fld, *M.op_shapes(d, M.OP_W))
+    with pytest.raises(ValueError, match="output has"):
+        M.check_sizes("WT", x_fld, x_fld, *M.op_shapes(d, M.OP_WT))
+    M.check_sizes("WAWT", x_fld, None, *M.op_shapes(d, M.OP_WAWT))

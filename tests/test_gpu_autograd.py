@@ -33,6 +33,23 @@ def smooth(a, passes=2):
     return a
 
 
+def box3_passes(t, passes):
+    """3x3 box smoothing of a (m, n, K) device tensor with the half-sample
+    mirror (edge replication for a 1-point pad), `passes` times."""
+    F = torch.nn.functional
+    v = t.permute(2, 0, 1).unsqueeze(0)          # (1, K, m, n)
+    for _ in range(passes):
+        v = F.avg_pool2d(F.pad(v, (1, 1, 1, 1), mode="replicate"), 3, stride=1)
+    return v.squeeze(0).permute(1, 2, 0).contiguous()
+
+
+def test_box3_passes_matches_numpy_smoothing():
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal((12, 10, 3))
+    got = box3_passes(torch.from_numpy(a).to(DEV), 2).cpu().numpy()
+    assert np.allclose(got, smooth(a, 2), atol=1e-12)
+
+
 def inputs(shape, seed):
     rng = np.random.default_rng(seed)
     x = rng.standard_normal(shape).astype(np.float32)
@@ -84,13 +101,19 @@ def test_gradients_match_fp64_central_differences(filt_g1):
             assert abs(gan[idx] - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(gan).max()), (idx, gan[idx], fd)
 
 
-@pytest.mark.parametrize("case", ["rad3", "fast", "lquart", "line_bckg", "rad3_stream", "lin3_stream"])
+@pytest.mark.parametrize("case", ["rad3", "fast", "lquart", "line_bckg", "rad3_stream", "lin3_stream",
+                                  "rad3_filt_g1_off", "lin3_filt_g1_off"])
 def test_weight_gradients_of_the_other_procedures(case):
     # procedures 4 (3-D radial), 8 (fast), 6 with lquart and 2 (line), and
     # procedures 4 and 5 on the streaming 3-D filter (columns of 16 levels):
     # the weight gradients <(W^T g)_g, (W^T z)_g> against FP64 central
-    # differences
+    # differences. Procedures 4 and 5 weight generation 1 whatever
+    # l_filt_g1 says (weighting_all, mg_filtering.f90:203, 427), so with
+    # l_filt_g1 = .false. dJ/dw_1 is still the central difference
     kw = dict(gm=3, h=4, hz=2)
+    if case.endswith("_filt_g1_off"):
+        kw["l_filt_g1"] = False
+        case = case[: -len("_filt_g1_off")] + "_stream"
     lm = 6
     if case == "rad3":
         kw.update(mgbf_proc=4, km3=2, km2=1)
@@ -179,14 +202,19 @@ def test_per_point_weight_field_gradient():
 
 
 def test_full_size_gradients_are_homogeneous():
-    # 16 members x 512x512x64, 5 generations, learnable weights [1, .5, .25, .125, .0625]
+    # 16 members x 512x512x64, 5 generations, learnable weights [1, .5, .25, .125, .0625];
+    # homogeneity and central differences of the weights and of the amplitude field
     cfg = M.Config.for_grid(512, 512, 64, n_ens=16, gm=5, h=6, hz=3)
     with M.MGBF(cfg) as op:
         d = op.dims
         gen = torch.Generator(device="cuda").manual_seed(11)
         x = torch.randn(d.gen_shape(1), generator=gen, device=DEV)
         y = torch.randn(d.gen_shape(1), generator=gen, device=DEV)
-        a = 1.0 + 0.2 * torch.randn(d.gen_shape(1), generator=gen, device=DEV)
+        # amplitude field a = 1 + 0.2 N(0,1) smoothed by two 3x3 box passes
+        # with the half-sample mirror (for a 1-point pad, edge replication)
+        noise = torch.randn(d.gen_shape(1), generator=gen, device=DEV)
+        a = 1.0 + 0.2 * box3_passes(noise, 2)
+        del noise
         w0 = [1.0, 0.5, 0.25, 0.125, 0.0625]
         ws = [torch.tensor(v, dtype=torch.float64, device=DEV) for v in w0]
         J, ga, gw = loss_and_grads(op, a, x, y, ws)
@@ -200,4 +228,29 @@ def test_full_size_gradients_are_homogeneous():
         lhs_a = op.dot(a.contiguous(), ga.to(torch.float32).contiguous())
         assert abs(lhs_w - rBz) <= 1e-5 * abs(rBz)
         assert abs(lhs_a - rBz) <= 1e-4 * abs(rBz)
-        del x, y, a, z, Bz, r, ga
+
+        # central differences at full size (h = 1e-3): J is quadratic in
+        # every w_g and in a, so J(+h) - J(-h) = 2h dJ exactly; J is
+        # accumulated in FP64 over the FP32 operator
+        def J_of(wv, av):
+            for g, v in enumerate(wv, start=1):
+                op.set_weights(g, value=v)
+            rr = op.filtering_procedure((av * x).contiguous()) - y
+            return 0.5 * float((rr.double() ** 2).sum())
+
+        # the steps are taken as the library stores them (FP32), so the
+        # difference quotient divides by the step actually taken
+        h = 1e-3
+        f32 = lambda v: float(np.float32(v))
+        for g in range(5):
+            wp, wm = list(w0), list(w0)
+            wp[g], wm[g] = f32(w0[g] + h), f32(w0[g] - h)
+            fd = (J_of(wp, a) - J_of(wm, a)) / (wp[g] - wm[g])
+            assert abs(gw[g].item() - fd) <= 1e-4 * abs(fd), (g, gw[g].item(), fd)
+        # the amplitude gradient along a smooth random direction
+        da = box3_passes(torch.randn(d.gen_shape(1), generator=gen, device=DEV), 2)
+        ap, am = a + h * da, a - h * da
+        fd = (J_of(w0, ap) - J_of(w0, am))
+        got = float((ga.double() * (ap.double() - am.double())).sum())
+        assert abs(got - fd) <= 1e-4 * abs(fd), (got, fd)
+        del x, y, a, z, Bz, r, ga, da, ap, am

@@ -67,6 +67,46 @@ def test_group_graph_matches_eager_and_oracle():
             op.close()
 
 
+def test_group_recaptures_after_weights_or_tiles_change():
+    # the captured kernels read the weights and tiles of the moment of
+    # capture: after set_weights / set_tiles on one member the next replay
+    # must be captured again, and equal the eager calls with the new state
+    cfgs = [_cfg(*dm) for dm in DOMAINS[:3]]
+    ops = [M.MGBF(c) for c in cfgs]
+    try:
+        xs = [torch.from_numpy(np.random.default_rng(340 + i).standard_normal(op.field_shape())
+                               .astype(np.float32)).to(DEV) for i, op in enumerate(ops)]
+        outs = [torch.empty_like(x) for x in xs]
+        with M.Group(ops) as grp:
+            grp.apply(M.OP_WAWT, xs, outs)
+            torch.cuda.synchronize()
+            before = [o.clone() for o in outs]
+            gen = torch.Generator(device="cuda").manual_seed(341)
+            wf = 0.5 + torch.rand(ops[0].dims.gen_shape(2), generator=gen, device=DEV)
+            ops[0].set_weights(2, field=wf)
+            ops[1].set_weights(1, value=0.3)
+            grp.apply(M.OP_WAWT, xs, outs)
+            torch.cuda.synchronize()
+            eager = [op.filtering_procedure(x) for op, x in zip(ops, xs)]
+            torch.cuda.synchronize()
+            for i, (y, o) in enumerate(zip(eager, outs)):
+                assert torch.equal(y, o), i
+            assert not torch.equal(before[0], outs[0]) and not torch.equal(before[1], outs[1])
+            assert torch.equal(before[2], outs[2])
+            # a new tile on one member: captured again, same bits as eager
+            t = ops[2].tiles()
+            t[(1, 0)] = (2, 16)
+            ops[2].set_tiles(t)
+            grp.apply(M.OP_WAWT, xs, outs)
+            torch.cuda.synchronize()
+            y2 = ops[2].filtering_procedure(xs[2])
+            torch.cuda.synchronize()
+            assert torch.equal(y2, outs[2])
+    finally:
+        for op in ops:
+            op.close()
+
+
 def test_group_rejects_a_context_twice():
     op = M.MGBF(_cfg(*DOMAINS[2]))
     try:
@@ -91,10 +131,12 @@ def _uneven_domains(seed=24, count=24):
 
 
 def test_uneven_batch_full_size_with_amplitude_fields():
-    """24 domains of 384^2x32 .. 1024^2x127 in one graph; generation 1 carries a
-    per-point amplitude field 0.5 + U(0,1). Every replay equals the eager calls
-    bit for bit; the first domain of the two smallest sizes is checked against
-    the FP64 oracle; the largest domain is checked for symmetry <x,By> = <Bx,y>."""
+    """24 domains of 384^2x32 .. 1024^2x127 in one graph, each with tiles from
+    the autotuner; every generation carries a per-point amplitude field
+    0.5 + U(0,1) in its a_diff (mg_intstate.f90:161-164). Every replay equals
+    the eager calls bit for bit; the first domain of each of the three
+    smaller sizes is checked against the FP64 oracle; the largest domain is
+    checked for symmetry <x,By> = <Bx,y>."""
     doms = _uneven_domains()
     assert {d[0] for d in doms} == {384, 512, 768, 1024}
     weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
@@ -104,10 +146,14 @@ def test_uneven_batch_full_size_with_amplitude_fields():
         gen = torch.Generator(device="cuda")
         xs, amps = [], []
         for i, op in enumerate(ops):
+            print(f"uneven batch: domain {i} {doms[i]}: inputs and autotune", flush=True)
             gen.manual_seed(500 + i)
             xs.append(torch.randn(op.field_shape(), generator=gen, device=DEV))
-            amps.append(0.5 + torch.rand(op.field_shape(), generator=gen, device=DEV))
-            op.set_weights(1, field=amps[-1])
+            amps.append([0.5 + torch.rand(op.dims.gen_shape(g), generator=gen, device=DEV)
+                         for g in range(1, op.dims.gm + 1)])
+            for g, a in enumerate(amps[-1], start=1):
+                op.set_weights(g, field=a)
+            op.autotune()
         eager = [op.filtering_procedure(x) for op, x in zip(ops, xs)]
         torch.cuda.synchronize()
         outs = [torch.empty_like(x) for x in xs]
@@ -121,14 +167,15 @@ def test_uneven_batch_full_size_with_amplitude_fields():
                     assert torch.equal(y, o), (rep, i, doms[i])
                 ops[rep].filtering_procedure(xs[rep])
                 torch.cuda.synchronize()
-        for size in (384, 512):
+        for size in (384, 512, 768):
             i = next(i for i, d in enumerate(doms) if d[0] == size)
+            print(f"uneven batch: domain {i} {doms[i]} against the FP64 oracle", flush=True)
             cfg, op = cfgs[i], ops[i]
             d = op.dims
             P = O.OracleParams(nx=d.nx, ny=d.ny, K=d.km_all, lm=cfg.lm, nvcol=d.km3_all, hx=cfg.hx,
                                hy=cfg.hy, hz=cfg.hz, p=2, gm=d.gm, ampl01=cfg.mg_ampl01,
                                ampl02=cfg.mg_ampl02, weights=weights[: d.gm])
-            wf = [amps[i].cpu().double().numpy()] + [None] * (d.gm - 1)
+            wf = [a.cpu().double().numpy() for a in amps[i]]
             ref = O.wawt(P, xs[i].cpu().double().numpy(), wfield=wf)
             err = np.max(np.abs(eager[i].cpu().double().numpy() - ref)) / np.max(np.abs(ref))
             assert err <= 1e-5, (doms[i], err)

@@ -465,6 +465,46 @@ def test_rad3_procedure_matches_oracle(case):
         assert abs(lhs - rhs) <= 1e-6 * scale
 
 
+@pytest.mark.parametrize("proc", [4, 5])
+def test_3d_procedures_weight_generation_1_whatever_l_filt_g1_says(proc):
+    # filtering_rad3 / filtering_lin3 weight through weighting_all ->
+    # weighting, which scales generation 1 unconditionally
+    # (mg_filtering.f90:203, 427; mg_generations.f90:133-155, 973-1009):
+    # l_filt_g1 = .false. changes nothing for procedures 4 and 5, neither in
+    # W A W^T nor in its halves nor in the weighting A on its own
+    kw = dict(gm=3, h=4, hz=3, mgbf_proc=proc, km3=2, km2=2, weights=[0.8, 0.5, 0.25])
+    if proc == 5:
+        kw["mgbf_line"] = True
+    cfgs = []
+    for filt_g1 in (True, False):
+        c = M.Config.for_grid(64, 48, 16, l_filt_g1=filt_g1, **kw)
+        if proc == 5:
+            c.mg_ampl02 = 0.7
+            c.mg_ampl03 = 0.5
+        cfgs.append(c)
+    x = field((48, 64, 2 + 2 * 16), 200)
+    v = np.random.default_rng(201).standard_normal(
+        M.derive(cfgs[0]).mg_size).astype(np.float32)
+    res = []
+    for cfg in cfgs:
+        with M.MGBF(cfg) as op:
+            d = op.dims
+            xd, vd = to_dev(x), to_dev(v)
+            y = op.filtering_procedure(xd)
+            a = op.apply(M.OP_A, vd)
+            asq = op.apply(M.OP_ASQRT, vd)
+            y2 = op.apply(M.OP_BSQRT, op.apply(M.OP_BSQRTT, xd))
+            sync()
+            assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+            assert (y - y2).abs().max().item() <= 1e-5 * y.abs().max().item()
+            w = cfg.weights()
+            for g, (ag, sg, vg) in enumerate(zip(op.split_mg(a), op.split_mg(asq), op.split_mg(vd))):
+                assert torch.allclose(ag, vg * w[g], rtol=1e-6, atol=0), g
+                assert torch.allclose(sg, vg * float(np.sqrt(w[g])), rtol=1e-6, atol=0), g
+            res.append((y.clone(), a.clone()))
+    assert torch.equal(res[0][0], res[1][0]) and torch.equal(res[0][1], res[1][1])
+
+
 @pytest.mark.parametrize("proc,gm", [(4, 1), (4, 2), (7, 1), (2, 2), (8, 1)])
 def test_procedures_with_one_or_two_generations_and_ragged_stack(proc, gm):
     # the smallest hierarchies (gm = 1: no transfer at all) and a stack that
@@ -549,13 +589,16 @@ def test_lin3_procedure_matches_oracle(case):
     (5, 16, 3, 2, 3, ""),
     (5, 32, 2, 0, 4, "wide"),
     (5, 16, 3, 2, 3, "no_tables"),
+    (4, 16, 3, 2, 3, "h2"),      # 2-D variables at half-width 2, run at 3 with zero taps
+    (4, 32, 1, 3, 2, "h2"),      # ... beside a 3-D ball of half-width 2
 ])
 def test_streaming_3d_filter_matches_oracle(proc, lm, km3, km2, hz, case):
     # the streaming 3-D filter (columns of lm | 64 levels across the lanes of
     # a wavefront, level pairs by lane shifts) against the FP64 oracle, and
     # against the gather it replaces (MGBF_BALL_GATHER=1), with per-point
     # weights on generation 1
-    kw = dict(gm=3, h=4, hz=hz, mgbf_proc=proc, km3=km3, km2=km2, weights=[1.0, 0.5, 0.25])
+    h = 2 if case == "h2" else 4
+    kw = dict(gm=3, h=h, hz=hz, mgbf_proc=proc, km3=km3, km2=km2, weights=[1.0, 0.5, 0.25])
     if proc == 5:
         kw["mgbf_line"] = case != "no_tables"
     if case == "lquart":
@@ -599,7 +642,7 @@ def test_streaming_3d_filter_matches_oracle(proc, lm, km3, km2, hz, case):
             scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
             assert abs(lhs - rhs) <= 1e-6 * scale
             outs[gather] = y.clone()
-            if not gather and km2 > 0 and case in ("", "lquart"):
+            if not gather and km2 > 0 and case in ("", "lquart", "h2"):
                 # the 2-D variables take the restriction, the weighting and
                 # the prolongation in their own radial pass beside the 3-D one
                 # (procedure 5's line filter at half-width 3, zero taps beyond);
@@ -694,3 +737,58 @@ def test_single_field_768x768x64_five_generations_full_size():
         y = op.filtering_procedure(to_dev(x))
         sync()
         assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+
+
+def test_batched_ensemble_80x768x768x64_full_size():
+    # the benchmark workload: 80 members x 768x768x64 stacked in k, 4
+    # generations, radial half-width 6, vertical half-width 3, p = 2, weights
+    # [1, 1/2, 1/4, 1/8]; member m ~ N(0,1) with seed 100+m. Members 0, 39
+    # and 79 against the FP64 oracle, every member against the plain
+    # per-member context, and the W / W^T dot-product test at full size
+    # (filtering_rad2_ens, mg_filtering.f90:1000-1147)
+    M_, n, lm, gm = 80, 768, 64, 4
+    w = [1.0, 0.5, 0.25, 0.125]
+    cfg = M.Config.for_grid(n, n, lm, n_ens=M_, gm=gm, h=6, hz=3, p=2, weights=w)
+    print("batched ensemble 80 x 768x768x64: generating members", flush=True)
+    x = torch.empty((n, n, M_ * lm), dtype=torch.float32, device=DEV)
+    xv = x.view(n, n, M_, lm)
+    g = torch.Generator(device="cuda")
+    for m in range(M_):
+        g.manual_seed(100 + m)
+        xv[:, :, m, :] = torch.randn((n, n, lm), generator=g, device=DEV, dtype=torch.float32)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.gm == gm and d.km_all == M_ * lm and d.ntaps2 == 145 and d.ntaps1 == 7
+        op.autotune()
+        y = op.filtering_procedure(x)
+        sync()
+        yv = y.view(n, n, M_, lm)
+        cfg1 = M.Config.for_grid(n, n, lm, n_ens=1, gm=gm, h=6, hz=3, p=2, weights=w)
+        P = O.OracleParams(nx=n, ny=n, K=lm, lm=lm, nvcol=1, hx=6, hy=6, hz=3, p=2, gm=gm,
+                           ampl01=cfg1.mg_ampl01, ampl02=cfg1.mg_ampl02, weights=w)
+        for m in (0, 39, 79):
+            print(f"batched ensemble: member {m} against the FP64 oracle", flush=True)
+            ref = O.wawt(P, xv[:, :, m, :].double().cpu().numpy())
+            err = relerr(yv[:, :, m, :], ref)
+            assert err <= TOL_OP, (m, err)
+        print("batched ensemble: every member against the plain context", flush=True)
+        with M.MGBF(cfg1) as op1:
+            worst = 0.0
+            for m in range(M_):
+                ym = op1.filtering_procedure(xv[:, :, m, :].contiguous())
+                ref = yv[:, :, m, :]
+                worst = max(worst, ((ym - ref).abs().max() / ref.abs().max()).item())
+            assert worst <= 1e-6, worst
+        print("batched ensemble: W / W^T dot-product test", flush=True)
+        gv = torch.Generator(device="cuda").manual_seed(7)
+        v = torch.randn((d.mg_size,), generator=gv, device=DEV, dtype=torch.float32)
+        wv = op.apply(M.OP_W, v)
+        lhs = op.dot(wv, x)
+        del wv, y
+        wt = op.apply(M.OP_WT, x)
+        rhs = op.dot(v, wt)
+        scale = float(torch.linalg.vector_norm(v.double())) * float(torch.linalg.vector_norm(wt.double()))
+        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), 1e-3 * scale), (lhs, rhs, scale)
+        del v, wt
+    del x
+    torch.cuda.empty_cache()

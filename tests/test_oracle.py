@@ -804,3 +804,86 @@ def test_lin3_wawt_matches_dense_model(lquart):
     Bd = Dm.B()
     assert np.max(np.abs(B - Bd)) < 1e-13 * np.max(np.abs(Bd))
     assert np.max(np.abs(B - B.T)) < 1e-13 * np.max(np.abs(B))
+
+
+# ---- closed form of the lsqr 4-point weights at ratio 2 ----
+# def_offset_coef's constants (mg_interpolate.f90:76-79): an analysis point a
+# quarter of the way from filter point i to i+1 reads i-1 .. i+2 with
+# p_coef = (-9, 111, 29, -3)/128, three quarters of the way with
+# q_coef = (-3, 29, 111, -9)/128; filter points beyond the edges are the
+# half-sample mirror images.
+P_COEF = np.array([-9, 111, 29, -3]) / 128.0
+Q_COEF = np.array([-3, 29, 111, -9]) / 128.0
+
+
+def _offset_matrix(nf):
+    """The 2:1 interpolation of def_offset_coef on nf filter points, read
+    from the constants alone: analysis n (0-based, at n + 1/2) with filter
+    point i at 2i + 1."""
+    T = np.zeros((2 * nf, nf))
+    mirror = lambda i: -1 - i if i < 0 else (2 * nf - 1 - i if i >= nf else i)
+    for n in range(2 * nf):
+        if n % 2:      # at 2u + 1.5: a quarter past filter point u
+            u, c = (n - 1) // 2, P_COEF
+        else:          # at 2u + 0.5: three quarters past filter point u - 1
+            u, c = n // 2, Q_COEF
+        base = u - 1 if n % 2 else u - 2
+        for t in range(4):
+            T[n, mirror(base + t)] += c[t]
+    return T
+
+
+def _lagrange_row(nodes, x):
+    w = np.ones(len(nodes))
+    for a, xa in enumerate(nodes):
+        for b, xb in enumerate(nodes):
+            if a != b:
+                w[a] *= (x - xb) / (xa - xb)
+    return w
+
+
+def _blend_matrix(nf, shift=0, flip=False):
+    """A 4-point blend of the two 3-point quadratic interpolants through the
+    filter points around each analysis point (weights t and 1 - t), on the
+    mirrored grid; `shift` moves the node window, `flip` swaps the blend
+    weights. Every variant reproduces quadratics (an affine blend of exact
+    quadratic interpolants), so only the closed form tells them apart."""
+    T = np.zeros((2 * nf, nf))
+    mirror = lambda i: -1 - i if i < 0 else (2 * nf - 1 - i if i >= nf else i)
+    for n in range(2 * nf):
+        x = (n + 0.5 - 1.0) / 2.0            # in filter-index units
+        i = int(np.floor(x)) + shift
+        t = x - np.floor(x)
+        nodes = [i - 1, i, i + 1, i + 2]
+        left = np.append(_lagrange_row(nodes[:3], x), 0.0)
+        right = np.insert(_lagrange_row(nodes[1:], x), 0, 0.0)
+        w = (t * left + (1 - t) * right) if flip else ((1 - t) * left + t * right)
+        for k, node in enumerate(nodes):
+            T[n, mirror(node)] += w[k]
+    return T
+
+
+def test_lsqr_weights_at_ratio_two_are_the_offset_constants():
+    # the blend of two quadratic interpolants that lsqr_mg_coef builds
+    # (mg_interpolate.f90:129-191) is, at ratio 2, the 4-point offset
+    # interpolation of def_offset_coef, edge rows folded by the mirror
+    for nf in (8, 12):
+        T = _axis_matrix(O.LSQR, nf, 2 * nf)
+        assert np.max(np.abs(T - _offset_matrix(nf))) < 1e-14
+    # the Lagrange blend written here from the point positions is the same
+    # matrix, so the constants and the blend agree independently of the oracle
+    assert np.max(np.abs(_blend_matrix(12) - _offset_matrix(12))) < 1e-14
+
+
+@pytest.mark.parametrize("shift,flip", [(1, False), (-1, False), (0, True)])
+def test_closed_form_rejects_a_shifted_window_or_a_flipped_blend(shift, flip):
+    # planted mistakes: each variant still reproduces quadratics away from
+    # the edges (what the older pins check), but the closed form fails it
+    nf = 12
+    bad = _blend_matrix(nf, shift=shift, flip=flip)
+    xf = np.arange(nf) * 2.0 + 1.0
+    xa = np.arange(2 * nf) + 0.5
+    f = lambda x: 0.3 + 0.2 * x - 0.01 * x * x
+    inside = slice(6, 2 * nf - 6)
+    assert np.max(np.abs(bad @ f(xf) - f(xa))[inside]) < 1e-12
+    assert np.max(np.abs(bad - _offset_matrix(nf))) > 1e-3
