@@ -41,18 +41,24 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 
 def profiled_traffic(M, n, lm, gm):
     """HBM bytes per radial launch measured by rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE on this workload, calibrated against a copy kernel; recorded in
-    the profile summary as `traffic_bytes_per_launch: <workload> <bytes>`."""
+    WRITE_SIZE on this workload, calibrated against a copy kernel, and the
+    tiles of that profile run; recorded in the profile summary as
+    `traffic_bytes_per_launch: <workload> <bytes>` and
+    `traffic_tiles: <workload> <tiles as JSON>`. (None, None) without it."""
     key = f"{M}x{n}x{n}x{lm}_gm{gm}"
+    nbytes, tiles = None, None
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == key:
-                    return float(parts[2])
-    except OSError:
+                parts = line.split(None, 2)
+                if len(parts) == 3 and parts[1] == key:
+                    if parts[0] == "traffic_bytes_per_launch:":
+                        nbytes = float(parts[2])
+                    elif parts[0] == "traffic_tiles:":
+                        tiles = json.loads(parts[2])
+    except (OSError, ValueError):
         pass
-    return None
+    return nbytes, tiles
 
 
 def make_members(torch, M, n, lm, seed0=100):
@@ -90,12 +96,13 @@ def stream_copy_gbps(torch, nbytes=4 << 30, reps=5):
 #   transfer: anal_to_filt_all2 then filt_to_anal_all2 between the
 #     768x768x64 filter grid and a 1152x1152 analysis grid with 127 levels,
 #     16 members.
-ROWS = ("proc1", "proc2", "proc3", "proc4", "proc5", "proc7", "proc8", "lquart", "transfer")
+ROWS = ("proc1", "proc2", "proc3", "proc4", "proc5", "proc7", "proc8", "lquart", "transfer", "group")
 ROW_NAMES = {"proc1": "filtering_rad2_bckg", "proc2": "filtering_lin2_bckg (mgbf_line)",
              "proc3": "filtering_fast_bckg", "proc4": "filtering_rad3", "proc5": "filtering_lin3 (mgbf_line)",
              "proc7": "filtering_lin2_ens (mgbf_line)", "proc8": "filtering_fast_ens",
              "lquart": "filtering_rad2_ens with the quadratic transfer (lquart)",
-             "transfer": "anal_to_filt_all2 + filt_to_anal_all2 (4-point horizontal, 2:1 vertical)"}
+             "transfer": "anal_to_filt_all2 + filt_to_anal_all2 (4-point horizontal, 2:1 vertical)",
+             "group": "uneven batch of regional domains in one hipGraph"}
 
 
 def _oracle_params(O, cfg, d):
@@ -163,6 +170,96 @@ def pin_or_tune(args, op, log):
     tiles = op.autotune()
     log(f"[bench] autotune {time.perf_counter() - t0:.1f} s: {tiles}")
     return tiles
+
+
+# The uneven batch: the regional domains of the workload drawn as the GPU
+# test draws them (seed 24): sizes from {384^2 x 32, 512^2 x 64, 768^2 x 64,
+# 1024^2 x 127} with 4, 5, 5, 6 generations, radial half-width from {6, 8,
+# 10}; per-domain N(0,1) inputs and per-point amplitude fields 0.5 + U(0,1)
+# on every generation (a_diff_f / a_diff_h); tiles from the autotuner.
+UNEVEN_SIZES = [(384, 32, 4), (512, 64, 5), (768, 64, 5), (1024, 127, 6)]
+
+
+def uneven_domains(seed=24, count=24):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n, lm, gm = UNEVEN_SIZES[rng.integers(len(UNEVEN_SIZES))]
+        out.append((n, lm, gm, int(rng.choice([6, 8, 10]))))
+    return out
+
+
+def run_group(args, torch, MG):
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    weights = [1.0, 0.5, 0.25, 0.125, 0.0625]
+    doms = uneven_domains()
+    log(f"[bench] group: {len(doms)} domains {doms}")
+    ops, xs, outs, amps = [], [], [], []
+    gen = torch.Generator(device="cuda")
+    t0 = time.perf_counter()
+    for i, (n, lm, gm, h) in enumerate(doms):
+        op = MG.MGBF(MG.Config.for_grid(n, n, lm, gm=gm, h=h, hz=3, weights=weights))
+        gen.manual_seed(500 + i)
+        xs.append(torch.randn(op.field_shape(), generator=gen, device="cuda"))
+        outs.append(torch.empty_like(xs[-1]))
+        amps.append([0.5 + torch.rand(op.dims.gen_shape(g), generator=gen, device="cuda")
+                     for g in range(1, op.dims.gm + 1)])
+        for g, a in enumerate(amps[-1], start=1):
+            op.set_weights(g, field=a)
+        if not args.no_tune:
+            op.autotune()
+        ops.append(op)
+    log(f"[bench] group: contexts and autotune {time.perf_counter() - t0:.1f} s")
+    units = sum(op.dims.nx * op.dims.ny * op.dims.km_all for op in ops)
+    # algorithmic bytes: the two-sweep minimum of each domain plus one read
+    # of every generation's weight field
+    abytes = 0.0
+    for op in ops:
+        d = op.dims
+        S = sum(4.0 ** -(g - 1) for g in range(1, d.gm + 1))
+        abytes += d.nx * d.ny * d.km_all * (bytes_per_unit(d.gm) + 4.0 * S)
+    grp = MG.Group(ops)
+    for _ in range(args.warmup):
+        grp.apply(MG.OP_WAWT, xs, outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        grp.apply(MG.OP_WAWT, xs, outs)
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+    # the same applications eagerly, one after the other on one stream
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for op, x, y in zip(ops, xs, outs):
+            op.filtering_procedure(x, out=y)
+    torch.cuda.synchronize()
+    ms_eager = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = units / (ms_graph / 1e3)
+    achieved = abytes / (ms_graph / 1e3) / 1e9
+    log(f"[bench] group: graph {ms_graph:.3f} ms, eager {ms_eager:.3f} ms, {value:.4e} units/s")
+    out = {
+        "metric": METRIC, "row": "group", "value": value, "unit": "gridpoint·members/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_graph, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: per-domain N(0,1) (seed 500+i), amplitude fields 0.5+U(0,1)",
+        "config": {"workload": f"uneven batch: {len(doms)} regional domains (n, levels, generations, "
+                               f"half-width) {doms}, per-point amplitude fields on every generation, "
+                               f"autotuned tiles, one hipGraph"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "kernel": "the whole graph (two-sweep minimum bytes of every domain plus one "
+                               "read of every weight field, over the graph's time)"},
+        "cpu_baseline": None,
+        "eager_ms_per_step": round(ms_eager, 4),
+        "graph_over_eager": round(ms_eager / ms_graph, 3),
+    }
+    grp.close()
+    for op in ops:
+        op.close()
+    print(json.dumps(out), flush=True)
 
 
 def run_row(args, torch, MG):
@@ -318,6 +415,9 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
 
+    if args.row == "group":
+        run_group(args, torch, MG)
+        return
     if args.row:
         run_row(args, torch, MG)
         return
@@ -412,6 +512,18 @@ def main():
                          f"restatement with OpenMP, {tc:.2f} s"}
         log(f"[bench] cpu: {tc:.2f} s, rel err {relerr:.3e}")
 
+    # the traffic figure comes from a profile run of this workload; say which
+    # tiles it ran and whether this run's tiles are the same
+    traffic_bytes, traffic_tiles = profiled_traffic(M, n, lm, gm)
+    run_tiles = {f"g{g}k{k}": list(v) for (g, k), v in tiles.items()} if tiles else None
+    traffic_source = None
+    if traffic_bytes is not None:
+        same = traffic_tiles is not None and run_tiles is not None and \
+            all(traffic_tiles.get(k) == v for k, v in run_tiles.items())
+        traffic_source = (f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated, from "
+                          f"{os.path.relpath(TRAFFIC_PROFILE, os.path.dirname(os.path.abspath(__file__)))} "
+                          f"(a profile run with tiles {json.dumps(traffic_tiles)}); this run's tiles "
+                          f"{'are the same' if same else 'differ'}")
     out = {
         "metric": METRIC,
         "value": value,
@@ -431,7 +543,8 @@ def main():
                    "members": M, "nx": n, "ny": n, "levels": lm, "gm": gm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": profiled_traffic(M, n, lm, gm),
+                     "traffic": traffic_bytes,
+                     "traffic_source": traffic_source,
                      "kernel": "radial_kernel (F and F^T on every generation)",
                      "launches": r_n, "avg_launch_ms": round(r_ms / max(r_n, 1), 4),
                      "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1)},

@@ -147,7 +147,7 @@ int mgbf_derive(const mgbf_config* cfg, mgbf_dims* dims);
 /* mg_initialize (src.mgbf/mg_entrymod.f90:36-219) without MPI: validates,
    builds the FP64 coefficients (def_offset_coef, def_mg_weights ->
    cholaspect/getlinesum, src.mgbf/mg_intstate.f90:296-433), uploads the
-   FP32 tables and allocates the workspace on `device`. */
+   FP32 tables on `device`. */
 int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** ctx);
 
 /* mg_finalize (src.mgbf/mg_entrymod.f90:222-273): frees the context. */
@@ -161,7 +161,23 @@ int mgbf_get_dims(const mgbf_ctx* ctx, mgbf_dims* dims);
    the context uses it, or NULL to use the constant `value`. */
 int mgbf_set_weights(mgbf_ctx* ctx, int gen, const float* field, double value);
 
-/* Applies an operator (MGBF_OP_*). For WAWT, `in` and `out` may alias. */
+/* Per-point aspect tensors of generation gen (1-based): the pasp2(:,:,i,j)
+   that def_mg_weights builds from the p_del and p_eps fields
+   (src.mgbf/mg_intstate.f90:383-390), which the reference's filters read
+   per point (vrbeta2 / vrbeta2T with el(:,:,ix,iy) and ss(ix,iy),
+   src.mgbf/jp_pbfil.f90:888-928, 1011-1050; getlinesum2 :182-218).
+   pasp2 is a host array of n_g*m_g*3 doubles, (A11, A22, A12) of point
+   (i, j) at 3*(i + n_g*j), or NULL to return to the namelist's
+   mg_ampl02 * I. The generation's F then gathers with every point's own
+   ellipse and normalizer, and F^T (no longer F) is the transposed gather of
+   the scatter and fold. Built for the radial procedures 1 and 6
+   (MGBF_ERR_UNSUPPORTED otherwise); a reach beyond the halo is refused with
+   the reference's message. The weights of each point's taps are kept on the
+   device: (2H+1)^2 floats per point of the generation. */
+int mgbf_set_aspect(mgbf_ctx* ctx, int gen, const double* pasp2);
+
+/* Applies an operator (MGBF_OP_*). For WAWT, `in` and `out` may alias. The
+   context's workspaces are allocated by the first call that needs them. */
 int mgbf_apply(mgbf_ctx* ctx, int op, const float* in, float* out, void* stream);
 
 /* Applies one stage (MGBF_STAGE_*) on generation gen; in and out must not alias. */
@@ -218,14 +234,17 @@ int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* byt
 int mgbf_autotune(mgbf_ctx* ctx, void* stream);
 /* The tile in use for generation gen (1-based) and pass kind: *ti is the
    output columns per lane of the vector kernel (2 or 4), or 16 when the pass
-   runs on the matrix cores (16 output columns per block of 4 wavefronts);
-   *tj is the output rows per wavefront or block. */
+   runs on the matrix cores (16 output columns x 64 k per block of 4
+   wavefronts); *tj is the output rows per wavefront or block. */
 int mgbf_get_tile(const mgbf_ctx* ctx, int gen, int kind, int* ti, int* tj);
 /* Pins the tile of generation gen (1-based) and pass kind, in the terms of
-   mgbf_get_tile (ti = 2 or 4, or 16 for the matrix cores, which only the
-   plain pass kind 0 offers, for half-widths up to 8; tj >= 1), so that tiles
-   found by mgbf_autotune in one process can be reused in another without
-   timing again. No counterpart in the reference. */
+   mgbf_get_tile (ti = 2 or 4, or 16 for the matrix cores; tj >= 1, even for
+   kind 3), so that tiles found by mgbf_autotune in one process can be reused
+   in another without timing again. The matrix cores take every pass kind of
+   a generation whose width is a multiple of 16, with K a multiple of 64, a
+   disc or product filter of half-width <= 8 and the 4-point transfer (kind
+   3 also: at least 32 columns, half-width >= 3); else
+   MGBF_ERR_UNSUPPORTED. No counterpart in the reference. */
 int mgbf_set_tile(mgbf_ctx* ctx, int gen, int kind, int ti, int tj);
 
 /* Message of the last failure on this thread ("" if none). */

@@ -12,6 +12,8 @@
  *   - dinvmtf / dl1lmf (2x2 inverse, Cholesky) . jp_pmat.f90:147-183, 867-897
  *   - vrbeta1 / vrbeta1T (1-D gather/scatter) .. jp_pbfil.f90:856-885, 981-1009
  *   - vrbeta2 / vrbeta2T (2-D gather/scatter) .. jp_pbfil.f90:888-928, 1011-1050
+ *     with a per-point aspect field (el, ss per point; pasp2 of
+ *     def_mg_weights, mg_intstate.f90:383-390) ... mgbfo_radial_aspect
  *   - boco_2d (halo fill, half-sample mirror) .. mg_bocos.f90:225-262, 340-377
  *   - bocoT_2d (adjoint fold) .................. mg_bocos.f90:929-957, 1033-1061
  *   - adjoint / direct1 (R^T-side, P) .......... mg_generations.f90:1117-1187, 1190-1250
@@ -1498,6 +1500,125 @@ void mgbfo_radial(const mgbfo_params* P, int K, int n, int m, const double* in, 
     pf_to_interior(&a, out);
     pf_free(&a);
     hfilt_free(&H);
+}
+
+/* ------------------------------------------------------------------------
+ * Per-point aspect tensors: pasp2(:,:,ix,iy) of def_mg_weights
+ * (mg_intstate.f90:383-390) with p_del and p_eps varying over the domain.
+ * cholaspect2 per point (jp_pbfil.f90:90-107), getlinesum2 per point
+ * (:182-218), and vrbeta2 / vrbeta2T with el(:,:,ix,iy) and ss(ix,iy)
+ * (:888-928, 1011-1050), between boco_2d and bocoT_2d.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    double exx, eyy, eyx, ss;
+} pel_t;
+
+/* el of one point, rescaled by rmom2_2, and its line sum; -1 if a reach
+   exceeds the halo (getlinesum2 stops there). a3 = (A11, A22, A12). */
+static int point_el(const mgbfo_params* P, const double a3[3], pel_t* e) {
+    const double a[4] = {a3[0], a3[2], a3[2], a3[1]};
+    double el[4];
+    cholaspect2(a, el);
+    const double rmom2_2 = 1.0 / sqrt((double)(2 * P->p) + 4.0);
+    e->exx = el[0] * rmom2_2;
+    e->eyy = el[3] * rmom2_2;
+    e->eyx = el[1] * rmom2_2;
+    const double y = 1.0 / e->eyy;
+    const int gyl = (int)ceil(-y + EPS), gym = (int)floor(y - EPS);
+    if (gyl < -P->hy || gym > P->hy) return -1;
+    double s = 0.0;
+    for (int gy = gyl; gy <= gym; ++gy) {
+        const double xc = -gy * e->eyx, rrx = (gy * e->eyy) * (gy * e->eyy);
+        const double x = sqrt(1.0 - rrx);
+        const int gxl = (int)ceil((xc - x) / e->exx + EPS), gxm = (int)floor((xc + x) / e->exx - EPS);
+        if (gxl < -P->hx || gxm > P->hx) return -1;
+        for (int gx = gxl; gx <= gxm; ++gx) {
+            const double t = gx * e->exx - xc;
+            s += ipow(1.0 - (rrx + t * t), P->p);
+        }
+    }
+    e->ss = 1.0 / s;
+    return 0;
+}
+
+/* The taps of one point in vrbeta2's loop order: calls f(gx, gy, frow). */
+#define PEL_TAPS(E, BODY)                                                              \
+    do {                                                                               \
+        const double y_ = 1.0 / (E)->eyy;                                              \
+        for (int gy = (int)ceil(-y_ + EPS); gy <= (int)floor(y_ - EPS); ++gy) {        \
+            const double xc_ = -gy * (E)->eyx, rrx_ = (gy * (E)->eyy) * (gy * (E)->eyy); \
+            const double x_ = sqrt(1.0 - rrx_);                                        \
+            const int gxl_ = (int)ceil((xc_ - x_) / (E)->exx + EPS);                   \
+            const int gxm_ = (int)floor((xc_ + x_) / (E)->exx - EPS);                  \
+            for (int gx = gxl_; gx <= gxm_; ++gx) {                                    \
+                const double t_ = gx * (E)->exx - xc_;                                 \
+                const double frow = (E)->ss * ipow(1.0 - (rrx_ + t_ * t_), P->p);      \
+                BODY;                                                                  \
+            }                                                                          \
+        }                                                                              \
+    } while (0)
+
+int mgbfo_radial_aspect(const mgbfo_params* P, int K, int n, int m, const double* asp,
+                        const double* in, double* out, int adjoint) {
+    pel_t* E = malloc(sizeof(pel_t) * (size_t)n * m);
+    for (int q = 0; q < n * m; ++q)
+        if (point_el(P, asp + 3 * (size_t)q, &E[q]) != 0) {
+            free(E);
+            return -1;
+        }
+    pf_t a = pf_alloc(K, n, m, P->hx, P->hy);
+    pf_t b = pf_alloc(K, n, m, P->hx, P->hy);
+    pf_from_interior(&a, in);
+    if (!adjoint) {
+        boco_2d(&a, P->hx, P->hy);
+#pragma omp parallel for schedule(static)
+        for (int iy = 1; iy <= m; ++iy)
+            for (int ix = 1; ix <= n; ++ix) {
+                const pel_t* e = &E[(ix - 1) + (size_t)n * (iy - 1)];
+                double* tb = PCOL(&b, ix, iy);
+                PEL_TAPS(e, {
+                    const double* src = PCOL(&a, ix + gx, iy + gy);
+                    for (int k = 0; k < K; ++k) tb[k] += frow * src[k];
+                });
+            }
+    } else {
+        /* scatter, parallel over k: each k is independent */
+#pragma omp parallel
+        {
+            int nth = 1, tid = 0;
+#ifdef _OPENMP
+            nth = omp_get_num_threads();
+            tid = omp_get_thread_num();
+#endif
+            const int k0 = (int)((long)K * tid / nth), k1 = (int)((long)K * (tid + 1) / nth);
+            for (int iy = 1; iy <= m; ++iy)
+                for (int ix = 1; ix <= n; ++ix) {
+                    const pel_t* e = &E[(ix - 1) + (size_t)n * (iy - 1)];
+                    const double* ta = PCOL(&a, ix, iy);
+                    PEL_TAPS(e, {
+                        double* dst = PCOL(&b, ix + gx, iy + gy);
+                        for (int k = k0; k < k1; ++k) dst[k] += frow * ta[k];
+                    });
+                }
+        }
+        bocoT_2d(&b, P->hx, P->hy);
+    }
+    pf_to_interior(&b, out);
+    pf_free(&a);
+    pf_free(&b);
+    free(E);
+    return 0;
+}
+
+/* The line sums ss(ix,iy) of a per-point aspect field (getlinesum2); -1 if
+   a reach exceeds the halo. */
+int mgbfo_ss2_aspect(const mgbfo_params* P, int n, int m, const double* asp, double* ss) {
+    for (int q = 0; q < n * m; ++q) {
+        pel_t e;
+        if (point_el(P, asp + 3 * (size_t)q, &e) != 0) return -1;
+        ss[q] = e.ss;
+    }
+    return 0;
 }
 
 void mgbfo_vertical(const mgbfo_params* P, int n, int m, const double* in, double* out,

@@ -65,6 +65,13 @@ double mgbfo_ss2(const mgbfo_params* P);
    (adjoint=1: scatter + halo fold), on a generation of size n x m. */
 void mgbfo_radial(const mgbfo_params* P, int K, int n, int m,
                   const double* in, double* out, int adjoint);
+/* F (adjoint=0) or F^T (adjoint=1) on a generation of size n x m whose
+   aspect tensor varies per point: asp holds (A11, A22, A12) of point (i, j)
+   (0-based) at 3*(i + n*j). Returns 0, or -1 if a reach exceeds the halo. */
+int mgbfo_radial_aspect(const mgbfo_params* P, int K, int n, int m, const double* asp,
+                        const double* in, double* out, int adjoint);
+/* The per-point line sums ss = 1/sum(rrc^p) of such a field (getlinesum2). */
+int mgbfo_ss2_aspect(const mgbfo_params* P, int n, int m, const double* asp, double* ss);
 /* Vertical filter Vf / Vf^T on an (K, n, m) field. */
 void mgbfo_vertical(const mgbfo_params* P, int n, int m,
                     const double* in, double* out, int adjoint);

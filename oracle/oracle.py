@@ -55,6 +55,10 @@ def lib():
         L.mgbfo_ss1.argtypes = [P]; L.mgbfo_ss1.restype = ctypes.c_double
         L.mgbfo_ss2.argtypes = [P]; L.mgbfo_ss2.restype = ctypes.c_double
         L.mgbfo_radial.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
+        if hasattr(L, "mgbfo_radial_aspect"):
+            L.mgbfo_radial_aspect.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
+                                              ctypes.c_int]
+            L.mgbfo_ss2_aspect.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_vertical.argtypes = [P, ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int]
         L.mgbfo_restrict.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         L.mgbfo_prolong.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
@@ -305,6 +309,28 @@ def radial(P: OracleParams, x: np.ndarray, adjoint: bool = False) -> np.ndarray:
     y = np.empty_like(x); c = P.c()
     lib().mgbfo_radial(ctypes.byref(c), K, n, m, _dp(x), _dp(y), int(adjoint))
     return y
+
+
+def radial_aspect(P: OracleParams, asp: np.ndarray, x: np.ndarray, adjoint: bool = False) -> np.ndarray:
+    """F (or F^T) of one generation with a per-point aspect tensor field:
+    asp (m, n, 3) = (A11, A22, A12) per point, x (m, n, K)."""
+    x = _f64(x)
+    m, n, K = x.shape
+    a = np.ascontiguousarray(asp, dtype=np.float64)
+    assert a.shape == (m, n, 3)
+    y = np.empty_like(x)
+    if lib().mgbfo_radial_aspect(ctypes.byref(P.c()), K, n, m, _dp(a), _dp(x), _dp(y), int(adjoint)) != 0:
+        raise ValueError("filter reach exceeds the halo (getlinesum2 stops)")
+    return y
+
+
+def ss2_aspect(P: OracleParams, asp: np.ndarray) -> np.ndarray:
+    """getlinesum2's per-point line sums of an aspect field (m, n, 3)."""
+    a = np.ascontiguousarray(asp, dtype=np.float64)
+    ss = np.empty(a.shape[:2])
+    if lib().mgbfo_ss2_aspect(ctypes.byref(P.c()), a.shape[1], a.shape[0], _dp(a), _dp(ss)) != 0:
+        raise ValueError("filter reach exceeds the halo (getlinesum2 stops)")
+    return ss
 
 
 def vertical(P: OracleParams, x: np.ndarray, adjoint: bool = False) -> np.ndarray:

@@ -114,12 +114,16 @@ struct mgbf_ctx {
     const float4* d_rytab[mgbf::kMaxGen] = {};
     const int2* d_rymeta[mgbf::kMaxGen] = {};
     bool res_ok[mgbf::kMaxGen] = {};
-    // matrix-core engine: per-k max |u_g| from the restriction, weight fragments
+    // per-k max |u_g| recorded by the restrictions
     float* d_kmax = nullptr;                 // gm x K
     unsigned kmax_valid = 0;                 // generations whose max is current
-    const void* d_afrag = nullptr;
+    // matrix-core engine: B fragments of the weight bands and, per
+    // generation, of the x restriction (3 tile classes)
+    const void* d_bfrag = nullptr;
+    const void* d_rfrag[mgbf::kMaxGen] = {};
+    bool mx_res_ok[mgbf::kMaxGen] = {};
     int wexp = 0;
-    bool mfma_ok = false;                    // radial half-width fits the MFMA window
+    bool mfma_ok = false;                    // filter, half-width and stack fit the engine
     const int* d_rxs[mgbf::kMaxGen] = {};
     const float* d_rxw[mgbf::kMaxGen] = {};
     const int* d_rys[mgbf::kMaxGen] = {};
@@ -130,6 +134,10 @@ struct mgbf_ctx {
     const float* d_pyw[mgbf::kMaxGen] = {};
     const float* wfield[mgbf::kMaxGen] = {};
     float wconst[mgbf::kMaxGen] = {};
+    // per-point aspect tensors (mgbf_set_aspect): the weights of every
+    // point's taps over the (2H+1)^2 box; F^T is then a filter of its own
+    float* d_aw[mgbf::kMaxGen] = {};
+    bool aniso[mgbf::kMaxGen] = {};
     // analysis <-> filter transfer: separable window tables (direct: filter ->
     // analysis, adjoint: its transpose) and the vertical k windows
     mgbf::XferArgs xf_dir{}, xf_adj{};
@@ -375,13 +383,35 @@ bool can_fuse_band(const mgbf_ctx* c, int which) {
            (hb == 3 || hb == 4 || hb == 6 || hb == 8);
 }
 
+// Can pass `kind` of generation g run on the matrix-core engine? Full
+// 16-column tiles and 64-k chunks, the 4-point transfer in the epilogues,
+// and for the restriction exact x fragments of 3 tile classes.
+bool mx_ok(const mgbf_ctx* c, int g, int kind) {
+    if (!c->mfma_ok || c->nx[g] % 16) return false;
+    if (kind == 0) return true;
+    if (c->quad) return false;
+    if (kind == 3) return c->mx_res_ok[g] && c->nx[g] >= 32 && c->s.H >= 3;
+    return true;
+}
+
 // out = scale * F(in) [+ P coarse] [then vertical band `band`] on generation g
 int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield, float wconst,
            hipStream_t st, int phase = MGBF_T_HFILT, const float* coarse = nullptr,
-           int band = -1, float* rout = nullptr) {
+           int band = -1, float* rout = nullptr, bool transpose = false) {
     Phase ph(c, phase, st,
              4.0 * ((double)c->gsize[g] * (wfield ? 3 : 2) +
                     (coarse || rout ? (double)c->gsize[g + 1] : 0.0)));
+    if (c->aniso[g]) {
+        // per-point aspect tensors: F or F^T on its own, then the weighting;
+        // no epilogue rides in it
+        if (coarse || rout || band >= 0)
+            return fail(MGBF_ERR_ARG, "internal: epilogue fused into a per-point aspect pass");
+        mgbf::AspectArgs aa{in, out, c->d_aw[g], c->K, c->nx[g], c->ny[g], c->s.H};
+        HIP_TRY(mgbf::launch_aspect(aa, transpose, st), "aspect filter launch");
+        if (wfield || wconst != 1.0f)
+            HIP_TRY(mgbf::launch_scale(out, out, wfield, wconst, 0, c->gsize[g], st), "scale launch");
+        return MGBF_OK;
+    }
     mgbf::RadialArgs a{};
     a.in = in;
     a.out = out;
@@ -407,10 +437,10 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
         a.rymeta = c->d_rymeta[g];
         a.rkmax = c->d_kmax ? c->d_kmax + (size_t)(g + 1) * c->K : nullptr;
     }
-    if (kind == 0 && c->tile_mfma[g][0] && c->mfma_ok && ((c->kmax_valid >> g) & 1u)) {
+    if (c->tile_mfma[g][kind] && mx_ok(c, g, kind)) {
         a.mfma = true;
-        a.kmax_in = c->d_kmax + (size_t)g * c->K;
-        a.afrag = c->d_afrag;
+        a.bfrag = c->d_bfrag;
+        a.rfrag = rout ? c->d_rfrag[g] : nullptr;
         a.wexp = c->wexp;
     }
     a.exact_disc = c->s.exact_disc;
@@ -496,13 +526,13 @@ int down_sweep(mgbf_ctx* c, const float* u1, float* U, float* T, bool weighted, 
         float* next = last ? nullptr : U + c->goff[g + 1];
         const float* wf = weighted ? c->wfield[g] : nullptr;
         const float wc = weighted ? c->wconst[g] : 1.0f;
-        if (filt && !last && c->fuse >= 3 && fused_ok(c, g, c->tile_tj[g][3], c->s.H)) {
+        if (filt && !last && c->fuse >= 3 && !c->aniso[g] && fused_ok(c, g, c->tile_tj[g][3], c->s.H)) {
             CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT, nullptr, -1, next));
             if (c->d_kmax) c->kmax_valid |= 1u << (g + 1);
             continue;
         }
         if (!last) CALL(restrict_g(c, g, ug, next, st));
-        if (filt) CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT));
+        if (filt) CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT, nullptr, -1, nullptr, true));
     }
     return MGBF_OK;
 }
@@ -520,11 +550,11 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, fl
     float* T = c->ws_t;
     // the radial kernels fuse the 4-point or the quadratic prolongation,
     // and the band after either
-    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band);
+    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band) && !c->aniso[0];
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
-        const float* pro = c->fuse >= 1 ? coarse : nullptr;
+        const float* pro = c->fuse >= 1 && !c->aniso[g] ? coarse : nullptr;
         const bool direct = g == 0 && (fuse || band < 0);
         float* dst = direct ? y : U + c->goff[g];
         const int fb = (g == 0 && fuse) ? band : -1;
@@ -981,6 +1011,60 @@ int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStr
     return MGBF_OK;
 }
 
+// Floats of each workspace (multigrid U and T, the analysis-grid
+// intermediate, the per-k maxima), each rounded up to 256 bytes.
+struct WsSizes {
+    size_t u = 0, t = 0, a = 0, k = 0;
+    size_t total() const { return u + t + a + k; }
+};
+
+WsSizes ws_sizes(const mgbf_ctx* c) {
+    auto r = [](size_t n) { return (n + 63) & ~size_t(63); };
+    WsSizes w;
+    w.u = r((size_t)c->s.dims.mg_size);
+    w.t = w.u;
+    w.a = c->s.dims.km_a_all != c->K ? r((size_t)c->K * c->s.dims.na * c->s.dims.ma) : 0;
+    w.k = r((size_t)c->gm * c->K);
+    return w;
+}
+
+// Allocates the multigrid workspaces on first use.
+int ensure_workspace(mgbf_ctx* c) {
+    const WsSizes w = ws_sizes(c);
+    if (!c->ws_u) HIP_TRY(hipMalloc(&c->ws_u, sizeof(float) * w.u), "device allocation (workspace)");
+    if (!c->ws_t) HIP_TRY(hipMalloc(&c->ws_t, sizeof(float) * w.t), "device allocation (workspace)");
+    if (w.a && !c->ws_a) HIP_TRY(hipMalloc(&c->ws_a, sizeof(float) * w.a), "device allocation (workspace)");
+    return MGBF_OK;
+}
+
+// mgbf_apply on workspaces already in place.
+int apply_op(mgbf_ctx* c, int op, const float* in, float* out, hipStream_t st) {
+    if (c->rad3) {
+        switch (op) {
+            case MGBF_OP_WAWT: return apply_wawt_rad3(c, in, out, st);
+            case MGBF_OP_WT: return apply_wt_rad3(c, in, out, false, st);
+            case MGBF_OP_W: return apply_w_rad3(c, in, out, false, st);
+            case MGBF_OP_BSQRT: return apply_w_rad3(c, in, out, true, st);
+            case MGBF_OP_BSQRTT: return apply_wt_rad3(c, in, out, true, st);
+            default: break;
+        }
+    }
+    switch (op) {
+        case MGBF_OP_WAWT: return apply_wawt(c, in, out, st);
+        case MGBF_OP_WT: return apply_wt(c, in, out, false, st);
+        case MGBF_OP_W: return apply_w(c, in, out, false, st);
+        case MGBF_OP_A: return apply_weights(c, in, out, false, st);
+        case MGBF_OP_ASQRT: return apply_weights(c, in, out, true, st);
+        case MGBF_OP_BSQRT: return apply_w(c, in, out, true, st);
+        case MGBF_OP_BSQRTT: return apply_wt(c, in, out, true, st);
+        case MGBF_OP_A2F: return apply_a2f(c, c->kv_a, c->s.all2_refusal, in, out, st);
+        case MGBF_OP_F2A: return apply_f2a(c, c->kv_d, c->s.all2_refusal, in, out, st);
+        case MGBF_OP_A2F_ALL: return apply_a2f(c, c->kw_a, c->s.all_refusal, in, out, st);
+        case MGBF_OP_F2A_ALL: return apply_f2a(c, c->kw_d, c->s.all_refusal, in, out, st);
+        default: return fail(MGBF_ERR_ARG, "unknown operation");
+    }
+}
+
 template <typename T>
 T* carve(char*& p, size_t count) {
     T* r = reinterpret_cast<T*>(p);
@@ -1082,10 +1166,11 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         // epilogue of the last radial pass (all levels compute the same)
         const char* fv = std::getenv("MGBF_FUSE");
         if (fv && fv[0] >= '0' && fv[0] <= '3') c->fuse = fv[0] - '0';
-        // MGBF_MFMA=1 puts every eligible first pass on the matrix cores
+        // MGBF_MFMA=1 puts every eligible radial pass on the matrix cores
         const char* mv = std::getenv("MGBF_MFMA");
         if (mv && mv[0] == '1')
-            for (int g = 0; g < mgbf::kMaxGen; ++g) c->tile_mfma[g][0] = 1;
+            for (int g = 0; g < mgbf::kMaxGen; ++g)
+                for (int kind = 0; kind < 4; ++kind) c->tile_mfma[g][kind] = 1;
     }
     long long off = 0;
     for (int g = 0; g < c->gm; ++g) {
@@ -1098,8 +1183,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (int kind = 0; kind < 4; ++kind) {
             c->tile_ti[g][kind] = 4;
             c->tile_tj[g][kind] = pick_tj(c->K, s.nx[g], s.ny[g], 4);
-            if (kind == 0 && c->tile_mfma[g][0])
-                c->tile_tj[g][0] = pick_tj(c->K, s.nx[g], s.ny[g], 16);
+            if (c->tile_mfma[g][kind]) c->tile_tj[g][kind] = pick_tj(c->K, s.nx[g], s.ny[g], 16);
         }
         // the fused restriction needs an even number of rows per wave
         int& tj3 = c->tile_tj[g][3];
@@ -1152,32 +1236,81 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (int t = 0; t < nb; ++t) c->band_int[b].w[t] = (float)(*bands[b])[(size_t)mid * nb + t];
     }
 
-    // FP16 hi/lo lane fragments of the weight bands T_b (b = 0..H) for the
-    // matrix-core engine, A operand of v_mfma_f32_16x16x32_f16: lane l holds
-    // row l%16 (output column) and window columns (l/16)*8 .. +7
-    std::vector<_Float16> afrag;
-    if (s.H <= 8) {
+    // Fragments of the matrix-core engine (B operand of
+    // v_mfma_f32_16x16x32_f16: lane l holds window columns 8(l/16) .. +7 of
+    // output column l%16):
+    //  - the weight bands T_b(c, i) = w(|c - H - i|, b) (b = 0..H), scaled
+    //    by 2^wexp and split into FP16 hi + lo, [b][hi/lo][lane][8];
+    //  - per generation g < gm-1 the x restriction of a 16-column tile onto
+    //    its 8 coarse columns (R = P^T with the folded edges, multiples of
+    //    1/128, exact in FP16) for the tile classes interior, left edge and
+    //    right edge, [class][lane][8]; every tile of the generation must
+    //    equal its class, else the restriction stays on the vector kernel.
+    std::vector<_Float16> bfrag;
+    std::vector<_Float16> rfrag[mgbf::kMaxGen];
+    if (s.H <= 8 && (s.exact_disc || s.separable) && c->K % 64 == 0) {
         double maxw = 0;
         for (int b = 0; b <= s.H; ++b)
             for (int a = 0; a <= s.H; ++a) maxw = std::max(maxw, s.wsym[b][a]);
         c->wexp = maxw > 0 ? 10 - std::ilogb(maxw) : 0;
-        afrag.assign((size_t)(s.H + 1) * 2 * 64 * 8, (_Float16)0.f);
+        bfrag.assign((size_t)(s.H + 1) * 2 * 64 * 8, (_Float16)0.f);
         for (int b = 0; b <= s.H; ++b)
             for (int l = 0; l < 64; ++l)
-                for (int q = 0; q < 8; ++q) {
-                    const int a = (l / 16) * 8 + q - s.H - (l % 16);
+                for (int e = 0; e < 8; ++e) {
+                    const int a = (l / 16) * 8 + e - s.H - (l % 16);
                     const int aa = a < 0 ? -a : a;
                     const float wv = aa <= s.H ? (float)std::ldexp(s.wsym[b][aa], c->wexp) : 0.f;
                     const _Float16 hi = (_Float16)wv;
                     const _Float16 lo = (_Float16)(wv - (float)hi);
-                    afrag[((size_t)(b * 2 + 0) * 64 + l) * 8 + q] = hi;
-                    afrag[((size_t)(b * 2 + 1) * 64 + l) * 8 + q] = lo;
+                    bfrag[((size_t)(b * 2 + 0) * 64 + l) * 8 + e] = hi;
+                    bfrag[((size_t)(b * 2 + 1) * 64 + l) * 8 + e] = lo;
                 }
         c->mfma_ok = true;
+        for (int g = 0; g + 1 < c->gm; ++g) {
+            const mgbf::AxisTable& rt = s.rx[g];
+            const int nf = s.nx[g];
+            if (nf % 16 || nf < 32 || c->quad || s.H < 3) continue;
+            auto tile_frag = [&](int i0, std::vector<_Float16>& f) -> bool {
+                std::vector<double> B(32 * 16, 0.0);
+                for (int I = 0; I < 8; ++I) {
+                    const int Ia = i0 / 2 + I;
+                    for (int t = 0; t < rt.width; ++t) {
+                        const double wv = rt.w[(size_t)Ia * rt.width + t];
+                        if (wv == 0.0) continue;
+                        const int cc = rt.start[Ia] + t - i0 + s.H;
+                        if (cc < 0 || cc >= 32) return false;
+                        B[cc * 16 + I] += wv;
+                    }
+                }
+                f.assign(64 * 8, (_Float16)0.f);
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const double wv = B[((l / 16) * 8 + e) * 16 + (l % 16)];
+                        const _Float16 h = (_Float16)wv;
+                        if ((double)(float)h != wv) return false;   // not exact in FP16
+                        f[l * 8 + e] = h;
+                    }
+                return true;
+            };
+            const int nt = nf / 16;
+            std::vector<_Float16> cls[3], ft;
+            bool ok = tile_frag(0, cls[1]) && tile_frag(nf - 16, cls[2]);
+            if (ok && nt >= 3) ok = tile_frag(16, cls[0]);
+            else cls[0].assign(64 * 8, (_Float16)0.f);
+            for (int t = 1; ok && t + 1 < nt; ++t) {
+                ok = tile_frag(t * 16, ft);
+                for (size_t q = 0; ok && q < ft.size(); ++q) ok = ft[q] == cls[0][q];
+            }
+            if (!ok) continue;
+            rfrag[g].clear();
+            for (int k = 0; k < 3; ++k) rfrag[g].insert(rfrag[g].end(), cls[k].begin(), cls[k].end());
+            c->mx_res_ok[g] = true;
+        }
     }
 
     // one device allocation for all coefficient tables
-    size_t bytes = carve_size(sizeof(double)) + carve_size(sizeof(_Float16) * afrag.size());
+    size_t bytes = carve_size(sizeof(double)) + carve_size(sizeof(_Float16) * bfrag.size());
+    for (int g = 0; g < mgbf::kMaxGen; ++g) bytes += carve_size(sizeof(_Float16) * rfrag[g].size());
     for (int b = 0; b < 3; ++b) bytes += carve_size(sizeof(float) * bands[b]->size());
     std::vector<float> rytab[mgbf::kMaxGen];
     std::vector<int> rymeta[mgbf::kMaxGen];
@@ -1259,12 +1392,15 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         return d;
     };
     for (int b = 0; b < 3; ++b) c->d_band[b] = put_f(*bands[b]);
-    {
-        _Float16* da = nullptr;
-        _Float16* h = take((_Float16*)nullptr, afrag.size(), &da);
-        for (size_t i = 0; i < afrag.size(); ++i) h[i] = afrag[i];
-        c->d_afrag = da;
-    }
+    auto put_h = [&](const std::vector<_Float16>& v) -> const void* {
+        if (v.empty()) return nullptr;
+        _Float16* d = nullptr;
+        _Float16* h = take((_Float16*)nullptr, v.size(), &d);
+        for (size_t i = 0; i < v.size(); ++i) h[i] = v[i];
+        return d;
+    };
+    c->d_bfrag = put_h(bfrag);
+    for (int g = 0; g < mgbf::kMaxGen; ++g) c->d_rfrag[g] = put_h(rfrag[g]);
     take((double*)nullptr, 1, &c->d_dot);
     if (c->rad3) {
         auto put_taps = [&](size_t nt, auto tap, const int4** dt, const float** dw) {
@@ -1340,16 +1476,14 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         return fail(MGBF_ERR_HIP, "internal: coefficient tables exceed their allocation");
     }
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&c->ws_u, sizeof(float) * (size_t)s.dims.mg_size);
-    if (e == hipSuccess) e = hipMalloc(&c->ws_t, sizeof(float) * (size_t)s.dims.mg_size);
     if (e == hipSuccess) e = hipMalloc(&c->d_kmax, sizeof(float) * (size_t)c->gm * c->K);
-    if (e == hipSuccess && s.dims.km_a_all != c->K)
-        e = hipMalloc(&c->ws_a, sizeof(float) * (size_t)c->K * s.dims.na * s.dims.ma);
     if (e != hipSuccess) {
         int r = hip_fail(e, "device allocation");
         mgbf_finalize(c);
         return r;
     }
+    // the multigrid workspaces are allocated by the first call that needs
+    // them (mgbf_apply, mgbf_autotune, mgbf_group_apply)
     *out = c;
     return MGBF_OK;
 }
@@ -1365,6 +1499,8 @@ int mgbf_finalize(mgbf_ctx* c) {
     if (c->d_tables) (void)hipFree(c->d_tables);
     if (c->d_kmax) (void)hipFree(c->d_kmax);
     if (c->ws_a) (void)hipFree(c->ws_a);
+    for (int g = 0; g < mgbf::kMaxGen; ++g)
+        if (c->d_aw[g]) (void)hipFree(c->d_aw[g]);
     delete c;
     return MGBF_OK;
 }
@@ -1384,34 +1520,42 @@ int mgbf_set_weights(mgbf_ctx* c, int gen, const float* field, double value) {
     return MGBF_OK;
 }
 
+int mgbf_set_aspect(mgbf_ctx* c, int gen, const double* pasp2) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    if (gen < 1 || gen > c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
+    const int g = gen - 1;
+    DeviceGuard guard(c->device);
+    if (!pasp2) {
+        if (c->d_aw[g]) {
+            HIP_TRY(hipDeviceSynchronize(), "synchronize before freeing the aspect weights");
+            HIP_TRY(hipFree(c->d_aw[g]), "hipFree(aspect weights)");
+        }
+        c->d_aw[g] = nullptr;
+        c->aniso[g] = false;
+        ++c->version;
+        return MGBF_OK;
+    }
+    const int proc = c->s.cfg.mgbf_proc;
+    if ((proc != 1 && proc != 6) || c->s.separable || !c->s.exact_disc)
+        return fail(MGBF_ERR_UNSUPPORTED,
+                    "per-point aspect tensors are built for the radial procedures 1 and 6");
+    std::vector<float> w;
+    std::string err;
+    const int st = mgbf::aspect_weights(c->s.cfg, c->s.H, c->nx[g], c->ny[g], pasp2, w, &err);
+    if (st != MGBF_OK) return fail(st, err);
+    if (!c->d_aw[g]) HIP_TRY(hipMalloc(&c->d_aw[g], sizeof(float) * w.size()), "hipMalloc(aspect weights)");
+    HIP_TRY(hipMemcpy(c->d_aw[g], w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice),
+            "aspect weights upload");
+    c->aniso[g] = true;
+    ++c->version;
+    return MGBF_OK;
+}
+
 int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
     if (!c || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
     DeviceGuard guard(c->device);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    if (c->rad3) {
-        switch (op) {
-            case MGBF_OP_WAWT: return apply_wawt_rad3(c, in, out, st);
-            case MGBF_OP_WT: return apply_wt_rad3(c, in, out, false, st);
-            case MGBF_OP_W: return apply_w_rad3(c, in, out, false, st);
-            case MGBF_OP_BSQRT: return apply_w_rad3(c, in, out, true, st);
-            case MGBF_OP_BSQRTT: return apply_wt_rad3(c, in, out, true, st);
-            default: break;
-        }
-    }
-    switch (op) {
-        case MGBF_OP_WAWT: return apply_wawt(c, in, out, st);
-        case MGBF_OP_WT: return apply_wt(c, in, out, false, st);
-        case MGBF_OP_W: return apply_w(c, in, out, false, st);
-        case MGBF_OP_A: return apply_weights(c, in, out, false, st);
-        case MGBF_OP_ASQRT: return apply_weights(c, in, out, true, st);
-        case MGBF_OP_BSQRT: return apply_w(c, in, out, true, st);
-        case MGBF_OP_BSQRTT: return apply_wt(c, in, out, true, st);
-        case MGBF_OP_A2F: return apply_a2f(c, c->kv_a, c->s.all2_refusal, in, out, st);
-        case MGBF_OP_F2A: return apply_f2a(c, c->kv_d, c->s.all2_refusal, in, out, st);
-        case MGBF_OP_A2F_ALL: return apply_a2f(c, c->kw_a, c->s.all_refusal, in, out, st);
-        case MGBF_OP_F2A_ALL: return apply_f2a(c, c->kw_d, c->s.all_refusal, in, out, st);
-        default: return fail(MGBF_ERR_ARG, "unknown operation");
-    }
+    CALL(ensure_workspace(c));
+    return apply_op(c, op, in, out, static_cast<hipStream_t>(stream));
 }
 
 int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* out,
@@ -1422,6 +1566,9 @@ int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* ou
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int g = gen - 1;
     if (g < 0 || g >= c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
+    // a single stage reads a caller field whose per-k maxima are not
+    // recorded: the matrix-core engine (which scales by them) stays off
+    c->kmax_valid = 0;
     switch (stage) {
         case MGBF_STAGE_VF:
         case MGBF_STAGE_VFT:
@@ -1430,7 +1577,8 @@ int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* ou
         case MGBF_STAGE_F:
         case MGBF_STAGE_FT:
             if (c->rad3) return ball(c, g, in, out, stage == MGBF_STAGE_FT, st);
-            return radial(c, g, in, out, nullptr, 1.0f, st);
+            return radial(c, g, in, out, nullptr, 1.0f, st, MGBF_T_HFILT, nullptr, -1, nullptr,
+                          stage == MGBF_STAGE_FT);
         case MGBF_STAGE_R:
             if (g + 1 >= c->gm) return fail(MGBF_ERR_ARG, "no coarser generation");
             return restrict_g(c, g, in, out, st);
@@ -1462,6 +1610,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
     if (c->rad3) return MGBF_OK;   // procedure 4 runs no radial kernel: nothing to tune
     DeviceGuard guard(c->device);
+    CALL(ensure_workspace(c));
     hipStream_t st = static_cast<hipStream_t>(stream);
     const size_t mgb = sizeof(float) * (size_t)c->s.dims.mg_size;
     HIP_TRY(hipMemsetAsync(c->ws_u, 0, mgb, st), "autotune memset");
@@ -1496,6 +1645,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     };
     for (int g = 0; g < c->gm && status == MGBF_OK; ++g) {
         const int n = c->nx[g], m = c->ny[g];
+        if (c->aniso[g]) continue;   // the per-point aspect filter has no tiles
         for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
             if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
@@ -1505,13 +1655,13 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             const int band = kind == 2 ? vband : -1;
             float best = 1e30f;
             int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind], bmf = c->tile_mfma[g][kind];
-            const bool try_mfma = kind == 0 && c->mfma_ok && ((c->kmax_valid >> g) & 1u);
+            const bool try_mfma = mx_ok(c, g, kind);
             const int tjs[11] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, m};
             // engines: TI = 2, TI = 4 (vector), 16 (matrix cores)
             const int engines[3] = {2, 4, 16};
             for (int en : engines) {
                 if (en == 16 && !try_mfma) continue;
-                if (kind == 3 && en != 4) continue;
+                if (kind == 3 && en == 2) continue;
                 for (int q = 0; q < 11 && status == MGBF_OK; ++q) {
                     const int tj = tjs[q];
                     bool dup = false;
@@ -1552,7 +1702,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
 int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
     if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 3)
         return fail(MGBF_ERR_ARG, "bad argument");
-    *ti = c->tile_mfma[gen - 1][kind] ? 16 : c->tile_ti[gen - 1][kind];
+    *ti = c->tile_mfma[gen - 1][kind] && mx_ok(c, gen - 1, kind) ? 16 : c->tile_ti[gen - 1][kind];
     *tj = c->tile_tj[gen - 1][kind];
     return MGBF_OK;
 }
@@ -1562,10 +1712,13 @@ int mgbf_set_tile(mgbf_ctx* c, int gen, int kind, int ti, int tj) {
         return fail(MGBF_ERR_ARG, "bad argument");
     if (!(ti == 2 || ti == 4 || ti == 16) || tj < 1)
         return fail(MGBF_ERR_ARG, "tile must be ti in {2, 4, 16} and tj >= 1");
-    if (kind == 3 && (ti != 4 || (tj & 1)))
-        return fail(MGBF_ERR_ARG, "the pass with the restriction takes ti = 4 and an even tj");
-    if (ti == 16 && (kind != 0 || !c->mfma_ok))
-        return fail(MGBF_ERR_UNSUPPORTED, "the matrix-core engine serves pass kind 0 with half-width <= 8");
+    if (kind == 3 && (ti == 2 || (tj & 1)))
+        return fail(MGBF_ERR_ARG, "the pass with the restriction takes ti = 4 or 16 and an even tj");
+    if (ti == 16 && !mx_ok(c, gen - 1, kind))
+        return fail(MGBF_ERR_UNSUPPORTED,
+                    "the matrix-core engine needs half-width <= 8, a disc or product filter, a stack of "
+                    "whole 64-k chunks, 16-column tiles and the 4-point transfer (restriction: tiles "
+                    "of exact FP16 x weights, at least 32 columns)");
     c->tile_ti[gen - 1][kind] = ti == 16 ? 4 : ti;
     c->tile_tj[gen - 1][kind] = tj;
     c->tile_mfma[gen - 1][kind] = ti == 16 ? 1 : 0;
@@ -1604,6 +1757,11 @@ void group_drop_graph(mgbf_group* g) {
 int group_capture(mgbf_group* g, int op, const float* const* in, float* const* out) {
     group_drop_graph(g);
     const int n = (int)g->ctx.size();
+    // workspaces are allocated before the capture begins
+    for (int i = 0; i < n; ++i) {
+        DeviceGuard guard(g->ctx[i]->device);
+        CALL(ensure_workspace(g->ctx[i]));
+    }
     std::vector<hipEvent_t> ev(n + 1, nullptr);
     for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "group event");
     int status = MGBF_OK;
@@ -1617,7 +1775,7 @@ int group_capture(mgbf_group* g, int op, const float* const* in, float* const* o
             mgbf_ctx* c = g->ctx[i];
             const bool timing = c->timing;
             c->timing = false;
-            status = mgbf_apply(c, op, in[i], out[i], bs);
+            status = apply_op(c, op, in[i], out[i], bs);
             c->timing = timing;
             (void)hipEventRecord(ev[i], bs);
             (void)hipStreamWaitEvent(g->cap, ev[i], 0);   // join

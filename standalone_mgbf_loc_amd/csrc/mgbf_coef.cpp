@@ -305,12 +305,15 @@ bool anal_axis_matrix(int npe, int nloc_a, int nloc_f, int scheme, std::vector<d
     return true;
 }
 
-bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
+// The taps of one aspect tensor A = [[a00, a01], [a01, a11]]: cholaspect2
+// (the lower Cholesky factor of A^-1, jp_pbfil.f90:90-107), the lattice
+// offsets vrbeta2 visits (:909-926) and getlinesum2's normalizer
+// (:182-218). 0 on success, 1 if the reach exceeds hy, 2 if it exceeds hx
+// (getlinesum2 stops there).
+int radial_taps_tensor(const mgbf_config& cfg, double a00, double a11, double a01, std::vector<Tap2>& taps) {
     taps.clear();
     const int p = cfg.p;
     const double rmom2_2 = 1.0 / std::sqrt(2.0 * p + 4.0);
-    // aspect tensor pasp2 with p_del = p_eps = 0 (mg_intstate.f90:316-319, 383-390)
-    const double a00 = cfg.mg_ampl02, a11 = cfg.mg_ampl02, a01 = 0.0;
     // inverse, then its lower Cholesky factor
     const double det = a00 * a11 - a01 * a01;
     const double i00 = a11 / det, i11 = a00 / det, i10 = -a01 / det;
@@ -320,7 +323,7 @@ bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     const double exx = b00 * rmom2_2, eyy = b11 * rmom2_2, eyx = b10 * rmom2_2;
     const double y = 1.0 / eyy;
     const int gyl = (int)std::ceil(-y + kEps), gym = (int)std::floor(y - kEps);
-    if (gyl < -cfg.hy || gym > cfg.hy) return false;
+    if (gyl < -cfg.hy || gym > cfg.hy) return 1;
     std::vector<double> rrc;
     for (int gy = gyl; gy <= gym; ++gy) {
         const double yy = gy;
@@ -329,7 +332,7 @@ bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
         const double x = std::sqrt(1.0 - rrx);
         const int gxl = (int)std::ceil((xc - x) / exx + kEps);
         const int gxm = (int)std::floor((xc + x) / exx - kEps);
-        if (gxl < -cfg.hx || gxm > cfg.hx) return false;
+        if (gxl < -cfg.hx || gxm > cfg.hx) return 2;
         for (int gx = gxl; gx <= gxm; ++gx) {
             const double t = gx * exx - xc;
             taps.push_back({gx, gy, 0.0});
@@ -340,7 +343,40 @@ bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
     for (double r : rrc) s += ipow(r, p);
     const double ss = 1.0 / s;
     for (size_t t = 0; t < taps.size(); ++t) taps[t].w = ss * ipow(rrc[t], p);
-    return true;
+    return 0;
+}
+
+bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps) {
+    // aspect tensor pasp2 with p_del = p_eps = 0 (mg_intstate.f90:316-319, 383-390)
+    return radial_taps_tensor(cfg, cfg.mg_ampl02, cfg.mg_ampl02, 0.0, taps) == 0;
+}
+
+int aspect_weights(const mgbf_config& cfg, int H, int n, int m, const double* pasp2, std::vector<float>& w,
+                   std::string* err) {
+    const int NB = 2 * H + 1, nt = NB * NB;
+    w.assign((size_t)n * m * nt, 0.f);
+    std::vector<Tap2> taps;
+    for (size_t q = 0; q < (size_t)n * m; ++q) {
+        const double a00 = pasp2[3 * q], a11 = pasp2[3 * q + 1], a01 = pasp2[3 * q + 2];
+        if (!(a00 > 0.0) || !(a11 > 0.0) || !(a00 * a11 - a01 * a01 > 0.0)) {
+            *err = fmt("aspect tensor at point %d is not positive definite", (int)q);
+            return MGBF_ERR_ARG;
+        }
+        const int r = radial_taps_tensor(cfg, a00, a11, a01, taps);
+        if (r != 0) {
+            *err = r == 1 ? "In getlinesum2; filter reach becomes too large for hy"
+                          : "In getlinesum2; filter reach becomes too large for hx";
+            return MGBF_ERR_CONFIG;
+        }
+        for (const Tap2& t : taps) {
+            if (std::abs(t.gx) > H || std::abs(t.gy) > H) {
+                *err = "aspect taps beyond the radial half-width";
+                return MGBF_ERR_CONFIG;
+            }
+            w[q * nt + (size_t)(t.gy + H) * NB + (t.gx + H)] = (float)t.w;
+        }
+    }
+    return MGBF_OK;
 }
 
 // 1-D beta taps of getlinesum1 / vrbeta1 (jp_pbfil.f90:149-177, 856-885)

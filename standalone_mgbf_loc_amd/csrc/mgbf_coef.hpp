@@ -87,6 +87,14 @@ int build_setup(const mgbf_config& cfg, Setup& s, std::string* err);
 // getlinesum2 + the vrbeta2 tap loop). Returns false if the reach exceeds
 // the halo (the reference stops there).
 bool radial_taps(const mgbf_config& cfg, std::vector<Tap2>& taps);
+// The per-point weights of a generation of n x m points whose aspect tensor
+// varies (pasp2 of def_mg_weights, mg_intstate.f90:383-390: (A11, A22, A12)
+// of point (i, j) at 3*(i + n*j)): per point the (2H+1)^2 offsets of the box
+// |gx|, |gy| <= H at (gy+H)(2H+1) + gx+H, ss (rrc)^p on the taps vrbeta2
+// visits, 0 elsewhere (cholaspect2 + getlinesum2 per point). MGBF_OK, or an
+// error status with the reference's message when a reach exceeds hx or hy.
+int aspect_weights(const mgbf_config& cfg, int H, int n, int m, const double* pasp2, std::vector<float>& w,
+                   std::string* err);
 bool vertical_taps(const mgbf_config& cfg, std::vector<int>& gx, std::vector<double>& w);
 // 2-D taps of the triad line filters of procedures 2 and 7 (mgbf_line), or
 // the identity without line tables. Returns MGBF_OK or an error status.

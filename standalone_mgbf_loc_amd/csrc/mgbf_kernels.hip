@@ -709,6 +709,81 @@ hipError_t launch_ball(const BallArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The radial filter of a generation whose aspect tensor varies per point
+// (vrbeta2 / vrbeta2T with el(:,:,ix,iy) and ss(ix,iy), jp_pbfil.f90:
+// 888-928, 1011-1050). Point p = i + n j carries the weights of its taps
+// over the box |gx|, |gy| <= H, w[p][t] with t = (gy+H)(2H+1) + gx+H
+// (ss included, zero off its ellipse). One wave per point and 64 k:
+//   F   (T = false): out(p) = sum_t w[p][t] in(mirror(p + g_t)), the gather
+//                    of vrbeta2 over the mirrored halo of boco_2d;
+//   F^T (T = true):  the scatter of vrbeta2T followed by the fold of
+//                    bocoT_2d, as a gather: out(q) = sum over the mirror
+//                    preimages e of q (per axis q, -1-q within H of the low
+//                    edge, 2n-1-q within H of the high one) and the taps t
+//                    whose source s = e - g_t lies in the domain, of
+//                    w[s][t] in(s).
+template <bool T>
+__global__ void __launch_bounds__(kBlock) aspect_kernel(AspectArgs a, int nkc, long long nitems) {
+    const long long item = __builtin_amdgcn_readfirstlane(
+        (int)((long long)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)));
+    if (item >= nitems) return;
+    const int kc = (int)(item % nkc);
+    const long long pt = item / nkc;
+    const int i = (int)(pt % a.n), j = (int)(pt / a.n);
+    const int k = kc * kWave + (threadIdx.x & (kWave - 1));
+    const bool kv = k < a.K;
+    const int kk = kv ? k : a.K - 1;
+    const int H = a.H, NB = 2 * H + 1, nt = NB * NB, n = a.n, m = a.m;
+    const size_t K = (size_t)a.K;
+    float acc = 0.f;
+    if (!T) {
+        const float* wp = a.w + (size_t)(i + (size_t)n * j) * nt;
+        for (int gy = -H; gy <= H; ++gy) {
+            const int jj = mirror_half(j + gy, m);
+            for (int gx = -H; gx <= H; ++gx) {
+                const float wv = wp[(gy + H) * NB + gx + H];
+                if (wv != 0.f)
+                    acc = __builtin_fmaf(wv, a.in[kk + K * ((size_t)mirror_half(i + gx, n) + (size_t)n * jj)], acc);
+            }
+        }
+    } else {
+        int ex[3], ey[3], nx = 0, ny = 0;
+        ex[nx++] = i;
+        if (i <= H - 1) ex[nx++] = -1 - i;
+        if (i >= n - H) ex[nx++] = 2 * n - 1 - i;
+        ey[ny++] = j;
+        if (j <= H - 1) ey[ny++] = -1 - j;
+        if (j >= m - H) ey[ny++] = 2 * m - 1 - j;
+        for (int b = 0; b < ny; ++b)
+            for (int gy = -H; gy <= H; ++gy) {
+                const int sy = ey[b] - gy;
+                if (sy < 0 || sy >= m) continue;
+                for (int c = 0; c < nx; ++c)
+                    for (int gx = -H; gx <= H; ++gx) {
+                        const int sx = ex[c] - gx;
+                        if (sx < 0 || sx >= n) continue;
+                        const size_t sp = (size_t)sx + (size_t)n * sy;
+                        const float wv = a.w[sp * nt + (gy + H) * NB + gx + H];
+                        if (wv != 0.f) acc = __builtin_fmaf(wv, a.in[kk + K * sp], acc);
+                    }
+            }
+    }
+    if (kv) a.out[k + K * ((size_t)i + (size_t)n * j)] = acc;
+}
+
+hipError_t launch_aspect(const AspectArgs& a, bool transpose, hipStream_t st) {
+    if (a.H < 1 || a.n < a.H || a.m < a.H || a.K < 1) return hipErrorInvalidValue;
+    const int nkc = (a.K + kWave - 1) / kWave;
+    const long long nitems = (long long)nkc * a.n * a.m;
+    const long long nblocks = (nitems + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (nitems >= (1ll << 31)) return hipErrorInvalidValue;
+    if (transpose)
+        hipLaunchKernelGGL(aspect_kernel<true>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nkc, nitems);
+    else
+        hipLaunchKernelGGL(aspect_kernel<false>, dim3((unsigned)nblocks), dim3(kBlock), 0, st, a, nkc, nitems);
+    return hipGetLastError();
+}
+
 hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st) {
     const long long nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = nkc * a.nout_x * a.nout_y;

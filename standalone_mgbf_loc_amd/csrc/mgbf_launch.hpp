@@ -28,11 +28,10 @@ struct RadialArgs {
                           // the 2-D variables after the 3-D ones of procedures 4, 5
     int H;                // max(hx, hy)
     int TI;               // output columns per lane (2 or 4)
-    // matrix-core engine (first pass of a generation)
+    // matrix-core engine (mgbf_radial_mx.hpp)
     bool mfma;
-    const float* kmax_in;   // per-k max |in| over the field (device, K floats)
-    float* kmax_out;        // per-k max |out| accumulated here, or nullptr
-    const void* afrag;      // FP16 hi/lo lane fragments of the weight bands
+    const void* bfrag;      // FP16 hi/lo B fragments of the Toeplitz weight bands T_b
+    const void* rfrag;      // FP16 B fragments of the x restriction (3 tile classes), or nullptr
     int wexp;               // power-of-two exponent applied to the weights
     int TJ;               // output rows per wavefront
     // restriction to the next generation fused into the pass (TI = 4, H >= 3)
@@ -66,16 +65,20 @@ struct RadialParams {
 };
 
 // kernel argument record of the matrix-core radial kernel
-struct RadialMfmaParams {
+struct RadialMxParams {
     const float* in;
     float* out;
     const float* wfield;
-    const float* kmax_in;
-    float* kmax_out;
-    const void* afrag;
+    const float* coarse;
+    const float* band;
+    const void* bfrag;
+    const void* rfrag;
+    float* rout;
+    const float4* rytab;
+    const int2* rymeta;
     float wconst;
     int wexp;
-    int K, n, m, TJ;
+    int K, n, m, TJ, lm, nfilt;
     int ntile_i, ntile_j;
 };
 
@@ -113,6 +116,17 @@ struct VBandArgs {
     int K, lm, nfilt;
     long long npts;
 };
+
+// The radial filter of a generation with a per-point aspect tensor field:
+// w holds, per point p = i + n j, the weights of the (2H+1)^2 offsets
+// |gx|, |gy| <= H (t = (gy+H)(2H+1) + gx+H), zero off the point's ellipse.
+struct AspectArgs {
+    const float* in;
+    float* out;
+    const float* w;
+    int K, n, m, H;
+};
+hipError_t launch_aspect(const AspectArgs& a, bool transpose, hipStream_t st);
 
 hipError_t launch_radial(const RadialArgs& a, hipStream_t st);
 hipError_t launch_vband(const VBandArgs& a, hipStream_t st);

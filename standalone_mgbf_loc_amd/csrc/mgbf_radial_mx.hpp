@@ -1,0 +1,453 @@
+// Radial 2-D beta filter on the matrix cores (gfx950 v_mfma_f32_16x16x32_f16),
+// with the epilogues of the vector kernel (weighting, prolongation, vertical
+// band, restriction to the next generation).
+//
+// For one input row r the filter's contribution to output row r+-b is a
+// banded Toeplitz product along x: Z_b(k, i) = sum_c v(k, c) T_b(c, i),
+// T_b(c, i) = w(|c - H - i|, b) over a window of 16 + 2H <= 32 columns. A
+// block of 4 wavefronts owns 16 output columns x 64 consecutive k x TJ
+// output rows.
+//
+// Phase A (matrix cores), wave w: k0 + 16w .. +15 is the MFMA's M, the
+// window columns its K, the 16 output columns its N. Lane l loads k =
+// l % 16 at window columns 8(l/16) .. +7 (the A operand layout), splits the
+// values, scaled by a power of two 2^e(k), into FP16 hi + lo (22 bits), and
+// forms Z_b = A_hi T_hi + A_lo T_hi + A_hi T_lo with FP32 accumulation; T_b
+// (hi/lo, scaled by 2^wexp) stays in registers for the whole launch. Z_b is
+// added into a ring of 2H+1 output-row accumulators in the MFMA's C layout
+// (lane l: k = 4(l/16) .. +3 of the tile at output column l % 16). The
+// exponent e(k) starts high and is lowered, with the ring rescaled by the
+// exact power of two, whenever a loaded row would exceed 2^14 after
+// scaling; it is undone exactly at the output (ldexp).
+// With RES the same A operand meets the restriction's x weights (multiples
+// of 1/128, exact in FP16) in two more MFMAs: the input row restricted in
+// x over the block's 8 coarse columns.
+// A finished output row (and the x-restricted row) is unscaled, takes the
+// weighting and the prolongation of the next coarser generation (PRO) in
+// the C layout, and goes to LDS.
+//
+// Phase B (after one barrier per row), wave w owns output columns
+// i0 + 4w .. +3 for all 64 k, lane = k: the vertical band (VHB: the levels
+// k+d read from the LDS rows of the block, which holds whole columns), the
+// y-restriction of the x-restricted rows into the open coarse rows
+// (coarse columns i0/2 + 2w, +1), and stores of 256 contiguous bytes per
+// column. The LDS tiles are double-buffered, so one barrier per row orders
+// both phases.
+#pragma once
+
+#include "mgbf_device.hpp"
+#include "mgbf_launch.hpp"
+
+namespace mgbf {
+
+typedef _Float16 mx_half8 __attribute__((ext_vector_type(8)));
+typedef float mx_float4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMxCols = 16;        // output columns per block
+constexpr int kMxWin = 32;         // window columns (the MFMA's K)
+constexpr int kMxPitch = 20;       // floats per k row of the LDS output tile (80 B)
+constexpr int kMxExInit = 100;     // exponent before the first nonzero row
+constexpr int kMxExMin = -120;
+
+__device__ __forceinline__ mx_float4 mx_ld4(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(mx_float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+template <int H, bool PRO, int VHB, bool RES, int D>
+__global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
+    static_assert(kMxCols + 2 * H <= kMxWin, "window exceeds one MFMA K step");
+    static_assert(!RES || (H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
+    static_assert(D >= 1 && D <= 3, "prefetch depth");
+    constexpr int NR = 2 * H + 1;
+    constexpr int NORow = kWave + 2 * VHB;    // LDS output rows: 64 k plus zero pads
+    __shared__ __align__(16) float lds_o[2][NORow * kMxPitch];
+    __shared__ __align__(16) float lds_r[RES ? 2 : 1][RES ? 8 * kWave : 1];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g4 = lane >> 4, ln = lane & 15;
+    const int K = p.K, n = p.n, m = p.m;
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int ti = tile % p.ntile_i;
+    const int rest = tile / p.ntile_i;
+    const int tj = rest % p.ntile_j;
+    const int kc = rest / p.ntile_j;
+    const int i0 = ti * kMxCols, j0 = tj * p.TJ, k0 = kc * kWave;
+    const size_t rowstride = (size_t)n * K;
+
+    // the zero pads of the band rows (never written by the rows themselves)
+    if constexpr (VHB > 0) {
+        for (int q = threadIdx.x; q < 2 * VHB * kMxPitch; q += kBlock) {
+            const int r = q / kMxPitch, c = q % kMxPitch;
+            const int row = r < VHB ? r : kWave + r;
+            lds_o[0][row * kMxPitch + c] = 0.f;
+            lds_o[1][row * kMxPitch + c] = 0.f;
+        }
+    }
+
+    // ---- phase A roles ----
+    const int kA = k0 + 16 * w + ln;          // A operand row (one k per lane)
+    int vo[8];                                // byte offsets of the window columns
+#pragma unroll
+    for (int e = 0; e < 8; ++e) vo[e] = kA * 4 + mirror_half(i0 - H + 8 * g4 + e, n) * K * 4;
+    const int kC = k0 + 16 * w + 4 * g4;      // first of the 4 k of the C layout
+    const int iC = i0 + ln;                   // output column of the C layout
+
+    mx_half8 bh[H + 1], bl[H + 1];
+    {
+        const mx_half8* bf = reinterpret_cast<const mx_half8*>(p.bfrag);
+#pragma unroll
+        for (int b = 0; b <= H; ++b) {
+            bh[b] = bf[(b * 2 + 0) * kWave + lane];
+            bl[b] = bf[(b * 2 + 1) * kWave + lane];
+        }
+    }
+    mx_half8 rx;
+    if constexpr (RES) {
+        const int cls = i0 == 0 ? 1 : (i0 + kMxCols >= n ? 2 : 0);
+        rx = reinterpret_cast<const mx_half8*>(p.rfrag)[cls * kWave + lane];
+    }
+
+    // scaling state: A lane exponent, and per C row the output exponents
+    int ex = kMxExInit;
+    float sc = __builtin_amdgcn_ldexpf(1.f, ex);
+    float thr = __builtin_amdgcn_ldexpf(1.f, 14 - ex);
+    int eC[4];   // output exponent of the C row: -e(k) - wexp
+#pragma unroll
+    for (int q = 0; q < 4; ++q) eC[q] = -ex - p.wexp;
+
+    mx_float4 acc[NR];
+#pragma unroll
+    for (int s = 0; s < NR; ++s) acc[s] = mx_float4{0.f, 0.f, 0.f, 0.f};
+
+    // prolongation window (C layout): 4 x-interpolated coarse rows of the
+    // lane's 4 k at its fine column
+    const int nc = n / 2, mc = m / 2;
+    const size_t crow = (size_t)nc * K;
+    mx_float4 XV[4];
+    int ccol[4];
+    float cxw[4];
+    int cbase = 0;
+    auto wbase = [](int j) { return ((j + 1) >> 1) - 2; };
+    auto load_cx = [&](int r) -> mx_float4 {
+        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(r, mc) * crow);
+        mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) x += cxw[t] * mx_ld4(rc, ccol[t], 0);
+        return x;
+    };
+    if constexpr (PRO) {
+        const int iu = iC >> 1;
+        const bool odd = iC & 1;
+        const int cb = odd ? iu - 1 : iu - 2;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            ccol[t] = kC * 4 + mirror_half(cb + t, nc) * K * 4;
+            cxw[t] = odd ? kPc[t] : kQc[t];
+        }
+        cbase = wbase(j0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) XV[u] = load_cx(cbase + u);
+    }
+
+    // ---- phase B roles: lane = k, columns i0 + 4w .. +3 ----
+    const int vkB = (k0 + lane) * 4;
+    constexpr int NVB = VHB > 0 ? 2 * VHB + 1 : 1;
+    float vc[NVB];
+    if constexpr (VHB > 0) {
+        const int k = k0 + lane;
+        if (k < p.nfilt) {
+            const float* row = p.band + (k % p.lm) * NVB;
+#pragma unroll
+            for (int d = 0; d < NVB; ++d) vc[d] = row[d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < NVB; ++d) vc[d] = d == VHB ? 1.f : 0.f;
+        }
+    }
+
+    const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
+    const int nsteps = tj_eff + 2 * H;
+
+    // fused restriction state (phase B layout): coarse columns i0/2 + 2w + c
+    constexpr int NQ = 6;
+    float racc[NQ][2];
+    const int Jlo = j0 / 2, Jhi = (j0 + tj_eff) / 2;
+    if constexpr (RES) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) racc[q][0] = racc[q][1] = 0.f;
+    }
+
+    // input rows, loaded D steps ahead
+    float vq[D][8];
+    auto load_row = [&](int rr, float* dst) {
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)mirror_half(j0 - H + rr, m) * rowstride);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) dst[e] = buf_ld(rs, vo[e], 0);
+    };
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (d < nsteps) load_row(d, vq[d]);
+
+    for (int rb = 0; rb < nsteps; rb += NR) {
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
+            const int rr = rb + s;
+            if (rr < nsteps) {
+            const int buf = rr & 1;
+            // ================= phase A =================
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = vq[0][e];
+#pragma unroll
+            for (int d = 0; d + 1 < D; ++d)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) vq[d][e] = vq[d + 1][e];
+            if (rr + D < nsteps) load_row(rr + D, vq[D - 1]);
+
+            float mx = fabsf(v[0]);
+#pragma unroll
+            for (int e = 1; e < 8; ++e) mx = fmaxf(mx, fabsf(v[e]));
+            if (__any(mx > thr)) {
+                // a k of this wave leaves the FP16 range: lower its exponent
+                // so that the row's max scales below 2^14 (the 4 lanes of a
+                // k agree on the max), rescale its ring rows exactly
+                float mk = fmaxf(mx, __shfl_xor(mx, 16));
+                mk = fmaxf(mk, __shfl_xor(mk, 32));
+                int exn = ex;
+                if (mk > thr) {
+                    exn = 14 - __builtin_amdgcn_frexp_expf(mk);
+                    exn = exn < kMxExMin ? kMxExMin : exn;
+                }
+                const int dex = exn - ex;
+                ex = exn;
+                sc = __builtin_amdgcn_ldexpf(1.f, ex);
+                thr = __builtin_amdgcn_ldexpf(1.f, 14 - ex);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dq = __shfl(dex, 4 * g4 + q);
+                    eC[q] -= dq;
+#pragma unroll
+                    for (int sl = 0; sl < NR; ++sl) acc[sl][q] = __builtin_amdgcn_ldexpf(acc[sl][q], dq);
+                }
+            }
+            mx_half8 ah, al;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float xs = v[e] * sc;
+                const _Float16 h = (_Float16)xs;
+                ah[e] = h;
+                al[e] = (_Float16)(xs - (float)h);
+            }
+            const int slot = s;
+            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[0], acc[slot], 0, 0, 0);
+            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[0], acc[slot], 0, 0, 0);
+            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[0], acc[slot], 0, 0, 0);
+#pragma unroll
+            for (int b = 1; b <= H; ++b) {
+                mx_float4 z = mx_float4{0.f, 0.f, 0.f, 0.f};
+                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[b], z, 0, 0, 0);
+                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[b], z, 0, 0, 0);
+                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[b], z, 0, 0, 0);
+                acc[(slot + b) % NR] += z;
+                acc[(slot + NR - b) % NR] += z;
+            }
+            const int vr = j0 - H + rr;   // the loaded fine row, before mirroring
+            if constexpr (RES) {
+                if (vr >= 0 && vr < m) {
+                    mx_float4 cr = mx_float4{0.f, 0.f, 0.f, 0.f};
+                    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rx, cr, 0, 0, 0);
+                    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rx, cr, 0, 0, 0);
+                    if (ln < 8) {
+                        mx_float4 cu;
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) cu[q] = __builtin_amdgcn_ldexpf(cr[q], eC[q] + p.wexp);
+                        *reinterpret_cast<mx_float4*>(&lds_r[buf][ln * kWave + 16 * w + 4 * g4]) = cu;
+                    }
+                }
+            }
+            const int jrel = rr - 2 * H;
+            constexpr int oslot = (NR - H) % NR;   // (s - H) mod NR, shifted by s below
+            const int os = (s + oslot) % NR;
+            const bool orow = jrel >= 0 && jrel < tj_eff;
+            if (orow) {
+                const int j = j0 + jrel;
+                mx_float4 o;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) o[q] = __builtin_amdgcn_ldexpf(acc[os][q], eC[q]);
+                if (p.wfield) {
+                    const __amdgpu_buffer_rsrc_t rw = buf_rsrc(p.wfield + (size_t)j * rowstride);
+                    o *= mx_ld4(rw, kC * 4 + iC * K * 4, 0);
+                } else {
+                    o *= p.wconst;
+                }
+                if constexpr (PRO) {
+                    const int nb = wbase(j);
+                    if (nb != cbase) {   // the window moves down one coarse row
+#pragma unroll
+                        for (int u = 0; u < 3; ++u) XV[u] = XV[u + 1];
+                        XV[3] = load_cx(nb + 3);
+                        cbase = nb;
+                    }
+                    const bool oj = (j & 1) != 0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) o += (oj ? kPc[u] : kQc[u]) * XV[u];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    lds_o[buf][(VHB + 16 * w + 4 * g4 + q) * kMxPitch + ln] = o[q];
+            }
+            acc[os] = mx_float4{0.f, 0.f, 0.f, 0.f};
+            __syncthreads();
+            // ================= phase B =================
+            if constexpr (RES) {
+                if (vr >= 0 && vr < m) {
+                    const float xr[2] = {lds_r[buf][(2 * w) * kWave + lane], lds_r[buf][(2 * w + 1) * kWave + lane]};
+                    using RS = Stencil<false>;
+                    auto store_coarse = [&](int J, const float* val) {
+                        if (J >= Jlo && J < Jhi) {
+                            const __amdgpu_buffer_rsrc_t ro =
+                                buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2 + 2 * w) + (size_t)nc * J));
+                            buf_st(val[0], ro, vkB, 0);
+                            buf_st(val[1], ro, vkB, K * 4);
+                        }
+                    };
+                    if (vr >= 8 && vr <= m - 9) {
+                        constexpr int NO = RS::NR / 2;
+                        constexpr int CP = (RS::ROFF + RS::NR - 1) & 1;
+                        const bool closes = (vr & 1) == CP;
+#pragma unroll
+                        for (int q = 0; q < NO; ++q) {
+                            const float yw = closes ? RS::R[RS::NR - 1 - 2 * q] : RS::R[RS::NR - 2 - 2 * q];
+                            racc[q][0] = __builtin_fmaf(yw, xr[0], racc[q][0]);
+                            racc[q][1] = __builtin_fmaf(yw, xr[1], racc[q][1]);
+                        }
+                        if (closes) {
+                            store_coarse((vr - RS::ROFF - RS::NR + 1) >> 1, racc[0]);
+#pragma unroll
+                            for (int q = 0; q + 1 < NO; ++q) {
+                                racc[q][0] = racc[q + 1][0];
+                                racc[q][1] = racc[q + 1][1];
+                            }
+                            racc[NO - 1][0] = racc[NO - 1][1] = 0.f;
+                        }
+                    } else {
+                        const float4 ya = p.rytab[2 * vr], yb = p.rytab[2 * vr + 1];
+                        const float yw[NQ] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y};
+                        const int2 meta = p.rymeta[vr];
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            racc[q][0] = __builtin_fmaf(yw[q], xr[0], racc[q][0]);
+                            racc[q][1] = __builtin_fmaf(yw[q], xr[1], racc[q][1]);
+                        }
+#pragma unroll
+                        for (int e = 0; e < NQ; ++e)
+                            if (e < meta.y) store_coarse(meta.x + e, racc[e]);
+#pragma unroll
+                        for (int sft = 1; sft <= NQ; ++sft) {
+                            if (meta.y == sft) {
+#pragma unroll
+                                for (int q = 0; q < NQ; ++q) {
+                                    racc[q][0] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][0] : 0.f;
+                                    racc[q][1] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][1] : 0.f;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            if (orow) {
+                const int j = j0 + jrel;
+                float out[4];
+                if constexpr (VHB > 0) {
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) out[t] = 0.f;
+#pragma unroll
+                    for (int d = 0; d < NVB; ++d) {
+                        const float4 r4 =
+                            *reinterpret_cast<const float4*>(&lds_o[buf][(lane + d) * kMxPitch + 4 * w]);
+                        out[0] = __builtin_fmaf(vc[d], r4.x, out[0]);
+                        out[1] = __builtin_fmaf(vc[d], r4.y, out[1]);
+                        out[2] = __builtin_fmaf(vc[d], r4.z, out[2]);
+                        out[3] = __builtin_fmaf(vc[d], r4.w, out[3]);
+                    }
+                } else {
+                    const float4 r4 = *reinterpret_cast<const float4*>(&lds_o[buf][lane * kMxPitch + 4 * w]);
+                    out[0] = r4.x;
+                    out[1] = r4.y;
+                    out[2] = r4.z;
+                    out[3] = r4.w;
+                }
+                const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + (size_t)j * rowstride + (size_t)i0 * K);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) buf_st(out[t], ro, vkB, (4 * w + t) * K * 4);
+            }
+            }   // rr < nsteps
+        }
+    }
+}
+
+template <int H, bool PRO, int VHB, bool RES>
+hipError_t launch_radial_mx_variant(const RadialArgs& a, hipStream_t st) {
+    RadialMxParams p{};
+    p.in = a.in;
+    p.out = a.out;
+    p.wfield = a.wfield;
+    p.wconst = a.wconst;
+    p.coarse = a.coarse;
+    p.band = a.band;
+    p.bfrag = a.bfrag;
+    p.rfrag = a.rfrag;
+    p.wexp = a.wexp;
+    p.K = a.K;
+    p.n = a.n;
+    p.m = a.m;
+    p.TJ = a.TJ;
+    p.lm = a.lm;
+    p.nfilt = a.nfilt;
+    p.rout = a.rout;
+    p.rytab = a.rytab;
+    p.rymeta = a.rymeta;
+    p.ntile_i = a.n / kMxCols;
+    p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
+    const long long nblocks = (long long)p.ntile_i * p.ntile_j * (a.K / kWave);
+    hipLaunchKernelGGL((radial_mx_kernel<H, PRO, VHB, RES, 2>), dim3((unsigned)nblocks), dim3(kBlock), 0, st, p);
+    return hipGetLastError();
+}
+
+// The matrix-core engine for half-width H: the shapes it is built for are
+// checked here (every k-chunk and column tile full, 32-bit row offsets).
+template <int H>
+hipError_t launch_radial_mx_h(const RadialArgs& a, hipStream_t st) {
+    if constexpr (kMxCols + 2 * H > kMxWin) {
+        return hipErrorInvalidValue;
+    } else {
+        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || a.quad || !a.bfrag)
+            return hipErrorInvalidValue;
+        if ((long long)a.n * a.K * 4 >= (1ll << 31)) return hipErrorInvalidValue;
+        const bool pro = a.coarse != nullptr;
+        const int vhb = a.band ? a.vhb : 0;
+        if (a.rout) {
+            if constexpr (H >= 3) {
+                if (pro || vhb || !a.rfrag || a.n < 2 * kMxCols || (a.TJ & 1)) return hipErrorInvalidValue;
+                return launch_radial_mx_variant<H, false, 0, true>(a, st);
+            } else {
+                return hipErrorInvalidValue;
+            }
+        }
+        if (vhb) {
+            if (!pro || a.lm < 1 || kWave % a.lm) return hipErrorInvalidValue;
+            switch (vhb) {
+                case 3: return launch_radial_mx_variant<H, true, 3, false>(a, st);
+                case 4: return launch_radial_mx_variant<H, true, 4, false>(a, st);
+                case 6: return launch_radial_mx_variant<H, true, 6, false>(a, st);
+                case 8: return launch_radial_mx_variant<H, true, 8, false>(a, st);
+                default: return hipErrorInvalidValue;
+            }
+        }
+        return pro ? launch_radial_mx_variant<H, true, 0, false>(a, st)
+                   : launch_radial_mx_variant<H, false, 0, false>(a, st);
+    }
+}
+
+}  // namespace mgbf

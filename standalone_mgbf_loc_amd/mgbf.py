@@ -32,6 +32,7 @@ EXPORTED_SYMBOLS = (
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
     "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
     "mgbf_set_tile", "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
+    "mgbf_set_aspect",
 )
 TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
@@ -177,6 +178,8 @@ def load_library(path: Optional[str] = None):
     L.mgbf_set_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
     L.mgbf_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p]
+    if hasattr(L, "mgbf_set_aspect"):
+        L.mgbf_set_aspect.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.mgbf_apply_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_void_p]
     L.mgbf_dot.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
@@ -367,6 +370,21 @@ class MGBF:
             self._wkeep.pop(gen, None)
             v = value if value is not None else self.cfg.weights()[min(gen, 5) - 1]
             _check(self._L.mgbf_set_weights(self._h, gen, None, float(v)))
+
+    def set_aspect(self, gen: int, pasp2=None):
+        """Per-point aspect tensors of generation gen (1-based): pasp2 is an
+        array (m_g, n_g, 3) of (A11, A22, A12) per point, the tensor
+        def_mg_weights builds from p_del and p_eps (mg_intstate.f90:383-390),
+        or None for the namelist's mg_ampl02 * I."""
+        if pasp2 is None:
+            _check(self._L.mgbf_set_aspect(self._h, gen, None))
+            return
+        import numpy as np
+        ny, nx, _ = self.dims.gen_shape(gen)
+        a = np.ascontiguousarray(pasp2, dtype=np.float64)
+        if a.shape != (ny, nx, 3):
+            raise ValueError(f"aspect field must have shape {(ny, nx, 3)}")
+        _check(self._L.mgbf_set_aspect(self._h, gen, a.ctypes.data_as(ctypes.c_void_p)))
 
     # --- sizes: the C ABI takes bare pointers, so every call checks here ---
     def op_shapes(self, op: int):

@@ -247,9 +247,12 @@ def test_full_size_gradients_are_homogeneous():
             wp[g], wm[g] = f32(w0[g] + h), f32(w0[g] - h)
             fd = (J_of(wp, a) - J_of(wm, a)) / (wp[g] - wm[g])
             assert abs(gw[g].item() - fd) <= 1e-4 * abs(fd), (g, gw[g].item(), fd)
-        # the amplitude gradient along a smooth random direction
+        # the amplitude gradient along a smooth random direction; J is
+        # quadratic in a, so a step of 0.05 has no truncation error and keeps
+        # the difference far above the FP32 noise of J (~1e8) itself
         da = box3_passes(torch.randn(d.gen_shape(1), generator=gen, device=DEV), 2)
-        ap, am = a + h * da, a - h * da
+        ha = 0.05
+        ap, am = a + ha * da, a - ha * da
         fd = (J_of(w0, ap) - J_of(w0, am))
         got = float((ga.double() * (ap.double() - am.double())).sum())
         assert abs(got - fd) <= 1e-4 * abs(fd), (got, fd)

@@ -697,13 +697,16 @@ def test_autotuned_tiles_keep_parity():
             assert torch.equal(y1, y2)
 
 
-@pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 40, 80, 48, 3, 1e-6),
-                                              (8, 130, 64, 64, 2, 1e4), (3, 24, 48, 40, 2, 1.0),
-                                              (6, 128, 192, 96, 4, 1.0)])
+@pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 64, 80, 48, 3, 1e-6),
+                                              (8, 128, 64, 64, 2, 1e4), (3, 192, 64, 48, 2, 1.0),
+                                              (6, 128, 192, 96, 4, 1.0), (5, 64, 128, 64, 3, 1e-30)])
 def test_matrix_core_engine_matches_oracle(monkeypatch, h, K, n, m, gm, scale):
-    # the first radial pass of every restricted generation on v_mfma_f32_16x16x32_f16
+    # every eligible radial pass on v_mfma_f32_16x16x32_f16 (MGBF_MFMA=1):
+    # the plain pass, the restriction, the prolongation and the vertical band
+    # epilogues; input scales from 1e-30 to 1e4 through the per-k exponents
     cfg = M.Config.for_grid(n, m, K, gm=gm, h=h, hz=min(3, K - 1))
     x = (field((m, n, K), 90 + h) * scale).astype(np.float32)
+    print(f"matrix cores h={h} K={K} {n}x{m} gm={gm} scale={scale}", flush=True)
     with M.MGBF(cfg) as op:
         y_vec = op.filtering_procedure(to_dev(x))
         sync()
@@ -711,12 +714,80 @@ def test_matrix_core_engine_matches_oracle(monkeypatch, h, K, n, m, gm, scale):
     with M.MGBF(cfg) as op:
         d = op.dims
         P = oparams(cfg, d)
+        t = op.tiles()
+        assert t[(1, 0)][0] == 16 and t[(1, 3)][0] == 16 and t[(1, 1)][0] == 16
         y = op.filtering_procedure(to_dev(x))
         v = op.apply(M.OP_WT, to_dev(x))
+        vv = np.random.default_rng(91).standard_normal(d.mg_size).astype(np.float32) * scale
+        w = op.apply(M.OP_W, to_dev(vv))
         sync()
-        assert relerr(y, O.wawt(P, x.astype(np.float64))) <= TOL_OP
-        assert relerr(v, O.wt(P, x.astype(np.float64))) <= TOL_OP
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.wawt(P, x64)) <= TOL_OP
+        assert relerr(v, O.wt(P, x64)) <= TOL_OP
+        assert relerr(w, O.w(P, vv.astype(np.float64))) <= TOL_OP
         assert not torch.equal(y, y_vec)   # the matrix-core path ran
+
+
+@pytest.mark.parametrize("case", ["members_apart", "rows_grow", "wfield", "fast", "band64", "lm32"])
+def test_matrix_core_engine_scales_and_epilogues(monkeypatch, case):
+    # members whose magnitudes differ by 1e10 in one stack (each member's k
+    # keep their own exponent; every member is held to the bar on its own),
+    # rows that grow by 2^24 down the field (the exponents are lowered and the
+    # ring rescaled on the way), per-point weights, the separable fast filter,
+    # and the vertical band fused into the last pass with whole 64-level
+    # columns and with two 32-level columns per chunk
+    monkeypatch.setenv("MGBF_MFMA", "1")
+    lm, n_ens, kw = 64, 2, {}
+    if case == "fast":
+        kw = dict(mgbf_proc=8, p=2)
+    if case == "lm32":
+        lm, n_ens = 32, 4
+    cfg = M.Config.for_grid(96, 64, lm, n_ens=n_ens, gm=3, h=6, hz=3, weights=[1.0, 0.5, 0.25], **kw)
+    if case == "fast":
+        cfg.mg_ampl02 = 0.45       # reach sqrt(7)/0.45 = 5.9: 11 taps per axis (half-width 5)
+    print(f"matrix cores: {case}", flush=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 95)
+        if case == "members_apart":
+            x = x.reshape(64, 96, n_ens, lm)
+            x[:, :, 0, :] *= 1e-6
+            x[:, :, 1, :] *= 1e4
+            x = x.reshape(d.gen_shape(1))
+        if case == "rows_grow":
+            x = (x * np.exp2(np.arange(64) * 24.0 / 63)[:, None, None]).astype(np.float32)
+        wf = None
+        if case == "wfield":
+            rng = np.random.default_rng(96)
+            wf = [(0.5 + rng.random(d.gen_shape(g))).astype(np.float32) for g in (1, 2, 3)]
+            keep = [to_dev(a) for a in wf]
+            for g, a in enumerate(keep, start=1):
+                op.set_weights(g, field=a)
+        t = op.tiles()
+        assert all(t[(1, kind)][0] == 16 for kind in (0, 1, 3))
+        y = op.filtering_procedure(to_dev(x))
+        if case in ("band64", "lm32", "members_apart", "rows_grow"):
+            # the band rode in the last radial pass (no vertical launch)
+            op.timers(True)
+            op.filtering_procedure(to_dev(x))
+            ph = op.read_timers()
+            op.timers(False)
+            assert ph["vfilt"][1] == 0 and t[(1, 2)][0] == 16
+        sync()
+        ref = O.wawt(P, x.astype(np.float64),
+                     wfield=[a.astype(np.float64) for a in wf] if wf is not None else None)
+        if case == "members_apart":
+            yv = y.cpu().double().numpy().reshape(64, 96, n_ens, lm)
+            rv = ref.reshape(64, 96, n_ens, lm)
+            for mbr in range(n_ens):
+                assert relerr(yv[:, :, mbr], rv[:, :, mbr]) <= TOL_OP, mbr
+        else:
+            assert relerr(y, ref) <= TOL_OP
+        if case == "rows_grow":
+            # the small rows keep their accuracy relative to their own size
+            yr = y.cpu().double().numpy()[:8]
+            assert np.max(np.abs(yr - ref[:8])) <= 1e-5 * np.max(np.abs(ref[:8]))
 
 
 def box3(a):

@@ -887,3 +887,95 @@ def test_closed_form_rejects_a_shifted_window_or_a_flipped_blend(shift, flip):
     inside = slice(6, 2 * nf - 6)
     assert np.max(np.abs(bad @ f(xf) - f(xa))[inside]) < 1e-12
     assert np.max(np.abs(bad - _offset_matrix(nf))) > 1e-3
+
+
+# ---- per-point aspect tensors (vrbeta2 / vrbeta2T with el, ss per point) ----
+
+def aspect_field(n, m, ampl, dmax=0.5, seed=0):
+    """A rotated, anisotropic aspect tensor varying over the domain, in the
+    reference's parameters: pasp2 = ampl [[1 + p_del, p_eps], [p_eps, 1 - p_del]]
+    with p_del = d cos 2t, p_eps = d sin 2t (mg_intstate.f90:383-390);
+    returns (m, n, 3) = (A11, A22, A12)."""
+    rng = np.random.default_rng(seed)
+    ph = rng.uniform(0, 2 * np.pi, 2)
+    J, I = np.meshgrid(np.arange(m), np.arange(n), indexing="ij")
+    t = np.pi * (I / n + 0.7 * J / m) + ph[0]
+    d = dmax * (0.5 + 0.5 * np.sin(2 * np.pi * I / n + ph[1]) * np.cos(np.pi * J / m))
+    pdel, peps = d * np.cos(2 * t), d * np.sin(2 * t)
+    return np.stack([ampl * (1 + pdel), ampl * (1 - pdel), ampl * peps], axis=-1)
+
+
+def _aparams(n, m, K=3, h=4, p=2):
+    return O.OracleParams(nx=n, ny=m, K=K, lm=K, nvcol=1, hx=h, hy=h, hz=1, p=p, gm=1,
+                          ampl01=4 / 7, ampl02=1.0)
+
+
+def _dense_aspect(asp, n, m, p):
+    """The matrix of F from the definitions alone: output point i takes the
+    lattice offsets g with r2 = g^T A_i^{-1} g / (2p+4) < 1, weights
+    (1 - r2)^p normalized to sum 1, inputs at the half-sample mirror."""
+    mir = lambda q, N: -1 - q if q < 0 else (2 * N - 1 - q if q >= N else q)
+    Fm = np.zeros((n * m, n * m))
+    for j in range(m):
+        for i in range(n):
+            a11, a22, a12 = asp[j, i]
+            Ai = np.linalg.inv(np.array([[a11, a12], [a12, a22]]))
+            taps = []
+            for gy in range(-8, 9):
+                for gx in range(-8, 9):
+                    g = np.array([gx, gy], dtype=float)
+                    r2 = g @ Ai @ g / (2 * p + 4)
+                    if r2 < 1 - 1e-9:
+                        taps.append((gx, gy, (1 - r2) ** p))
+            s = sum(w for _, _, w in taps)
+            for gx, gy, w in taps:
+                Fm[i + n * j, mir(i + gx, n) + n * mir(j + gy, m)] += w / s
+    return Fm
+
+
+def test_aspect_field_uniform_equals_the_domain_aspect():
+    # a field equal everywhere to the domain tensor of p_del / p_eps
+    # reproduces the pinned single-tensor filter (rotated taps included)
+    P = _aparams(20, 16)
+    P.p_del, P.p_eps = 0.3, -0.25
+    asp = np.broadcast_to([1.0 + 0.3, 1.0 - 0.3, -0.25], (16, 20, 3)).copy()
+    x = np.random.default_rng(1).standard_normal((16, 20, 3))
+    for adj in (False, True):
+        assert np.max(np.abs(O.radial_aspect(P, asp, x, adj) - O.radial(P, x, adj))) < 1e-13
+
+
+def test_aspect_field_adjoint_constants_and_mass():
+    P = _aparams(24, 20)
+    asp = aspect_field(24, 20, 1.9, seed=3)
+    rng = np.random.default_rng(4)
+    x, y = rng.standard_normal((2, 20, 24, 3))
+    Fx, Fty = O.radial_aspect(P, asp, x), O.radial_aspect(P, asp, y, adjoint=True)
+    assert abs(np.sum(Fx * y) - np.sum(x * Fty)) < 1e-12 * np.sqrt(np.sum(Fx ** 2) * np.sum(y ** 2))
+    # per-point normalization: F keeps constants everywhere (the mirror
+    # extends a constant), F^T keeps the sum of every k
+    c = np.full((20, 24, 3), 2.5)
+    assert np.max(np.abs(O.radial_aspect(P, asp, c) - 2.5)) < 1e-13
+    assert np.allclose(Fty.sum(axis=(0, 1)), y.sum(axis=(0, 1)), rtol=0, atol=1e-11)
+    # the line sums vary over the domain with the tensor
+    ss = O.ss2_aspect(P, asp)
+    assert ss.max() / ss.min() > 1.1
+
+
+def test_aspect_field_matches_the_dense_model():
+    n, m, p = 14, 12, 2
+    P = _aparams(n, m, K=1, p=p)
+    asp = aspect_field(n, m, 1.9, seed=5)
+    Fm = _dense_aspect(asp, n, m, p)
+    I = np.eye(n * m)
+    Fo = np.stack([O.radial_aspect(P, asp, I[q].reshape(m, n, 1))[:, :, 0].ravel() for q in range(n * m)], 1)
+    Ft = np.stack([O.radial_aspect(P, asp, I[q].reshape(m, n, 1), True)[:, :, 0].ravel() for q in range(n * m)], 1)
+    assert np.max(np.abs(Fo - Fm)) < 1e-12
+    assert np.max(np.abs(Ft - Fm.T)) < 1e-12
+    assert np.max(np.abs(Fm - Fm.T)) > 1e-3      # a varying field is not symmetric
+
+
+def test_aspect_field_reach_beyond_the_halo_is_refused():
+    P = _aparams(12, 12, h=3)
+    asp = aspect_field(12, 12, 3.0, seed=6)      # reach up to sqrt(3*1.5*8) = 6 > 3
+    with pytest.raises(ValueError, match="halo"):
+        O.radial_aspect(P, asp, np.zeros((12, 12, 1)))
