@@ -180,6 +180,21 @@ int mgbf_set_aspect(mgbf_ctx* ctx, int gen, const double* pasp2);
    context's workspaces are allocated by the first call that needs them. */
 int mgbf_apply(mgbf_ctx* ctx, int op, const float* in, float* out, void* stream);
 
+/* mgbf_apply for callers that cannot allocate device memory (code under HIP
+   graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+   counterpart in the reference. The call allocates nothing: instead of the
+   context's workspaces (which mgbf_apply allocates on first use) it uses a
+   static device array of MGBF_NOALLOC_SCRATCH_FLOATS floats (128 MiB), one
+   per device, held from the library's load until the process ends. A
+   context whose workspace (2 multigrid vectors, the per-k maxima and, for
+   the analysis transfers with a vertical map, one analysis-grid field of K
+   levels) exceeds it is refused with MGBF_ERR_ARG. Every call on a device
+   shares that device's array and leaves its contents behind: calls that
+   overlap on one device (any contexts, any streams) race. The timers record
+   nothing during the call. Results equal mgbf_apply's bit for bit. */
+#define MGBF_NOALLOC_SCRATCH_FLOATS (32ll << 20)
+int mgbf_apply_noalloc(mgbf_ctx* ctx, int op, const float* in, float* out, void* stream);
+
 /* Applies one stage (MGBF_STAGE_*) on generation gen; in and out must not alias. */
 int mgbf_apply_stage(mgbf_ctx* ctx, int stage, int gen, const float* in, float* out,
                      void* stream);

@@ -19,7 +19,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libmgbf.so")
 INCLUDE = os.path.join(ROOT, "include")
 
-SOURCES = ["mgbf_coef.cpp", "mgbf_capi.cpp", "mgbf_kernels.hip"]
+SOURCES = ["mgbf_coef.cpp", "mgbf_capi.cpp", "mgbf_kernels.hip", "mgbf_noalloc.hip"]
 RADIAL_TU = "mgbf_radial_inst.hip"
 RADIAL_H = range(1, 11)
 BALL_TU = "mgbf_ball3.hip"

@@ -1482,8 +1482,16 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         mgbf_finalize(c);
         return r;
     }
+    // the static scratch of mgbf_apply_noalloc on this device (its address
+    // is looked up once per device, with the device current)
+    st = mgbf::noalloc_setup(device, &err);
+    if (st != MGBF_OK) {
+        mgbf_finalize(c);
+        return fail(st, err);
+    }
     // the multigrid workspaces are allocated by the first call that needs
-    // them (mgbf_apply, mgbf_autotune, mgbf_group_apply)
+    // them (mgbf_apply, mgbf_autotune, mgbf_group_apply), so that a context
+    // used only through mgbf_apply_noalloc holds nothing beyond its tables
     *out = c;
     return MGBF_OK;
 }
@@ -1557,6 +1565,46 @@ int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
     CALL(ensure_workspace(c));
     return apply_op(c, op, in, out, static_cast<hipStream_t>(stream));
 }
+
+}  // extern "C"
+
+// The workspaces of one call in caller-given device memory: U, T, the
+// analysis-grid intermediate and the per-k maxima as slices of `base`
+// (every one 256-byte aligned). Nothing is allocated and no timer event is
+// recorded; the context's own workspaces are untouched.
+int mgbf::apply_in_scratch(mgbf_ctx* c, int op, const float* in, float* out, void* stream,
+                           float* (*scratch_of)(int device), size_t nfloats) {
+    if (!c || !in || !out) return fail(MGBF_ERR_ARG, "NULL argument");
+    float* const base = scratch_of(c->device);
+    if (!base) return fail(MGBF_ERR_HIP, "the static scratch has no device address on this device");
+    const WsSizes w = ws_sizes(c);
+    if (w.total() > nfloats)
+        return fail(MGBF_ERR_ARG,
+                    "the workspace of this context (" + std::to_string(w.total()) +
+                        " floats) exceeds the static scratch of mgbf_apply_noalloc (" +
+                        std::to_string(nfloats) + " floats)");
+    DeviceGuard guard(c->device);
+    float* const keep_u = c->ws_u;
+    float* const keep_t = c->ws_t;
+    float* const keep_a = c->ws_a;
+    float* const keep_k = c->d_kmax;
+    const bool keep_timing = c->timing;
+    c->ws_u = base;
+    c->ws_t = base + w.u;
+    c->ws_a = w.a ? base + w.u + w.t : nullptr;
+    c->d_kmax = base + w.u + w.t + w.a;
+    c->timing = false;
+    const int status = apply_op(c, op, in, out, static_cast<hipStream_t>(stream));
+    c->ws_u = keep_u;
+    c->ws_t = keep_t;
+    c->ws_a = keep_a;
+    c->d_kmax = keep_k;
+    c->timing = keep_timing;
+    c->kmax_valid = 0;   // the maxima of this call are in the scratch, not in d_kmax
+    return status;
+}
+
+extern "C" {
 
 int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* out,
                      void* stream) {

@@ -3,7 +3,27 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+#include <string>
+
+struct mgbf_ctx;
+
 namespace mgbf {
+
+// Static device scratch of mgbf_apply_noalloc (mgbf_noalloc.hip): 32 Mi
+// floats (128 MiB) per device, from the load of the code object to the end
+// of the process.
+constexpr size_t kNoallocScratchFloats = size_t(32) << 20;
+// Looks up the scratch's device address on `device` (which must be current)
+// once per device; MGBF_OK or an error status with a message in *err.
+int noalloc_setup(int device, std::string* err);
+// The scratch's device address on `device`, or nullptr before its setup.
+float* noalloc_scratch(int device);
+// mgbf_apply with the context's workspaces replaced, for this one call, by
+// slices of the `nfloats` floats at scratch_of(context's device)
+// (mgbf_capi.cpp).
+int apply_in_scratch(mgbf_ctx* c, int op, const float* in, float* out, void* stream,
+                     float* (*scratch_of)(int device), size_t nfloats);
 
 constexpr int kMaxHaloDev = 10;
 constexpr int kRWidthDev = 8;

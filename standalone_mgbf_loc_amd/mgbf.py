@@ -32,8 +32,10 @@ EXPORTED_SYMBOLS = (
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
     "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
     "mgbf_set_tile", "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
-    "mgbf_set_aspect",
+    "mgbf_apply_noalloc", "mgbf_set_aspect",
 )
+# floats of the static device scratch of mgbf_apply_noalloc (MGBF_NOALLOC_SCRATCH_FLOATS)
+NOALLOC_SCRATCH_FLOATS = 32 << 20
 TIMER_NAMES = ("vfiltT", "upsend", "hfiltT", "hfilt", "dnsend", "vfilt")
 
 
@@ -178,6 +180,8 @@ def load_library(path: Optional[str] = None):
     L.mgbf_set_weights.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_double]
     L.mgbf_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p]
+    L.mgbf_apply_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
     if hasattr(L, "mgbf_set_aspect"):
         L.mgbf_set_aspect.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.mgbf_apply_stage.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -401,6 +405,17 @@ class MGBF:
         if out is None:
             out = torch.empty(oshape, dtype=torch.float32, device=x.device)
         _check(self._L.mgbf_apply(self._h, op, _ptr(x), _ptr(out), _stream(stream)))
+        return out
+
+    def apply_noalloc(self, op: int, x, out, stream=None):
+        """apply() through mgbf_apply_noalloc: the workspaces live in the
+        library's static device scratch, so the call allocates nothing (out
+        must be given). Calls that overlap on one device race."""
+        if out is None:
+            raise ValueError("apply_noalloc writes into a caller-given output")
+        ishape, oshape = op_shapes(self.dims, op)
+        check_sizes(f"operation {op}", x, out, ishape, oshape)
+        _check(self._L.mgbf_apply_noalloc(self._h, op, _ptr(x), _ptr(out), _stream(stream)))
         return out
 
     def filtering_procedure(self, x, out=None, stream=None):
