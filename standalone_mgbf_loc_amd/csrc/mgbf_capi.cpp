@@ -437,7 +437,9 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
         a.rymeta = c->d_rymeta[g];
         a.rkmax = c->d_kmax ? c->d_kmax + (size_t)(g + 1) * c->K : nullptr;
     }
-    if (c->tile_mfma[g][kind] && mx_ok(c, g, kind)) {
+    // the matrix-core engine takes constant weights; a weighted pass runs on
+    // the vector kernel at the same TJ
+    if (c->tile_mfma[g][kind] && mx_ok(c, g, kind) && !wfield) {
         a.mfma = true;
         a.bfrag = c->d_bfrag;
         a.rfrag = rout ? c->d_rfrag[g] : nullptr;

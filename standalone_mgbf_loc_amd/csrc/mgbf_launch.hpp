@@ -84,11 +84,11 @@ struct RadialParams {
     RadialCoef W;
 };
 
-// kernel argument record of the matrix-core radial kernel
+// kernel argument record of the matrix-core radial kernel (constant
+// weights only: a pass with a weight field runs on the vector kernel)
 struct RadialMxParams {
     const float* in;
     float* out;
-    const float* wfield;
     const float* coarse;
     const float* band;
     const void* bfrag;

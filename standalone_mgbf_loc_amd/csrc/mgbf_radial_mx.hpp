@@ -8,8 +8,16 @@
 // block of 4 wavefronts owns 16 output columns x 64 consecutive k x TJ
 // output rows.
 //
+// Memory reaches the waves through LDS: each step's input row window (and,
+// with PRO, the coarse row the prolongation window takes in) is fetched by
+// direct 16-byte buffer-to-LDS loads NB - 1 steps ahead into a per-wave
+// slot, and a step waits with an explicit vector-memory count for its own
+// slot only. Loads complete in order, so every load the loop waits for is
+// issued in step order; weight fields are therefore left to the vector
+// kernel (a per-point weight would be one more load per step).
+//
 // Phase A (matrix cores), wave w: k0 + 16w .. +15 is the MFMA's M, the
-// window columns its K, the 16 output columns its N. Lane l loads k =
+// window columns its K, the 16 output columns its N. Lane l reads k =
 // l % 16 at window columns 8(l/16) .. +7 (the A operand layout), splits the
 // values, scaled by a power of two 2^e(k), into FP16 hi + lo (22 bits), and
 // forms Z_b = A_hi T_hi + A_lo T_hi + A_hi T_lo with FP32 accumulation; T_b
@@ -23,8 +31,8 @@
 // of 1/128, exact in FP16) in two more MFMAs: the input row restricted in
 // x over the block's 8 coarse columns.
 // A finished output row (and the x-restricted row) is unscaled, takes the
-// weighting and the prolongation of the next coarser generation (PRO) in
-// the C layout, and goes to LDS.
+// constant weight and the prolongation of the next coarser generation
+// (PRO) in the C layout, and goes to LDS.
 //
 // Phase B (after one barrier per row), wave w owns output columns
 // i0 + 4w .. +3 for all 64 k, lane = k: the vertical band (VHB: the levels
@@ -46,6 +54,9 @@ typedef float mx_float4 __attribute__((ext_vector_type(4)));
 constexpr int kMxCols = 16;        // output columns per block
 constexpr int kMxWin = 32;         // window columns (the MFMA's K)
 constexpr int kMxPitch = 20;       // floats per k row of the LDS output tile (80 B)
+constexpr int kMxRPitch = 68;      // floats per coarse column of the LDS x-restricted tile
+constexpr int kMxSlots = 6;        // LDS slots per wave (5 steps in flight)
+constexpr int kMxCCols = 12;       // coarse columns a wave's prolongation reads
 constexpr int kMxExInit = 100;     // exponent before the first nonzero row
 constexpr int kMxExMin = -120;
 
@@ -53,15 +64,69 @@ __device__ __forceinline__ mx_float4 mx_ld4(__amdgpu_buffer_rsrc_t r, int voff, 
     return __builtin_bit_cast(mx_float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
-template <int H, bool PRO, int VHB, bool RES, int D>
-__global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
+// 16 bytes per lane from the buffer straight into LDS: lane l's bytes land
+// at lds + 4 l floats. Completion is counted by the vector memory counter
+// only, and the caller waits for it explicitly. The LDS address reaches the
+// load as an integer: the compiler then treats the load as writing any LDS
+// (so it keeps LDS accesses in order around it) but does not track it as a
+// store to one variable, which would make it wait for every load in flight
+// before later LDS reads.
+__device__ __forceinline__ void mx_ld_lds16(__amdgpu_buffer_rsrc_t r, float* lds, int voff) {
+    const unsigned a = (unsigned)(size_t)(__attribute__((address_space(3))) float*)lds;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        r, (__attribute__((address_space(3))) void*)(size_t)__builtin_amdgcn_readfirstlane(a), 16, voff, 0, 0, 0);
+}
+
+// Makes a value loaded before the row loop count as consumed there: the
+// empty asm reads (and "rewrites") its registers, so the compiler waits for
+// the load once, here. A load still pending at the loop's back edge is
+// otherwise waited for inside every row step, with a count that also drains
+// the row prefetch issued after it.
+template <class T>
+__device__ __forceinline__ void mx_settle(T& x) {
+    asm volatile("" : "+v"(x));
+}
+
+// Waits until at most n vector memory operations of the wave are
+// outstanding (n rounded down to a multiple of 4, at most 60; n is
+// wave-uniform). Loads, direct-to-LDS loads and stores leave the vector
+// memory counter in the order they were issued.
+__device__ __forceinline__ void mx_wait_vmem(int n) {
+    switch (n < 60 ? n >> 2 : 15) {
+        case 15: asm volatile("s_waitcnt vmcnt(60)" ::: "memory"); break;
+        case 14: asm volatile("s_waitcnt vmcnt(56)" ::: "memory"); break;
+        case 13: asm volatile("s_waitcnt vmcnt(52)" ::: "memory"); break;
+        case 12: asm volatile("s_waitcnt vmcnt(48)" ::: "memory"); break;
+        case 11: asm volatile("s_waitcnt vmcnt(44)" ::: "memory"); break;
+        case 10: asm volatile("s_waitcnt vmcnt(40)" ::: "memory"); break;
+        case 9: asm volatile("s_waitcnt vmcnt(36)" ::: "memory"); break;
+        case 8: asm volatile("s_waitcnt vmcnt(32)" ::: "memory"); break;
+        case 7: asm volatile("s_waitcnt vmcnt(28)" ::: "memory"); break;
+        case 6: asm volatile("s_waitcnt vmcnt(24)" ::: "memory"); break;
+        case 5: asm volatile("s_waitcnt vmcnt(20)" ::: "memory"); break;
+        case 4: asm volatile("s_waitcnt vmcnt(16)" ::: "memory"); break;
+        case 3: asm volatile("s_waitcnt vmcnt(12)" ::: "memory"); break;
+        case 2: asm volatile("s_waitcnt vmcnt(8)" ::: "memory"); break;
+        case 1: asm volatile("s_waitcnt vmcnt(4)" ::: "memory"); break;
+        default: asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); break;
+    }
+}
+
+template <int H, bool PRO, int VHB, bool RES, int NB>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) radial_mx_kernel(RadialMxParams p) {
     static_assert(kMxCols + 2 * H <= kMxWin, "window exceeds one MFMA K step");
     static_assert(!RES || (H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
-    static_assert(D >= 1 && D <= 3, "prefetch depth");
+    // at most 8 (NB - 2) row loads, 4 (NB - 1) stores and NB - 2 weight
+    // loads are counted behind a row, and the wave's counter holds 63
+    static_assert(NB >= 2 && 13 * NB - 22 <= 60, "row slots");
     constexpr int NR = 2 * H + 1;
     constexpr int NORow = kWave + 2 * VHB;    // LDS output rows: 64 k plus zero pads
     __shared__ __align__(16) float lds_o[2][NORow * kMxPitch];
-    __shared__ __align__(16) float lds_r[RES ? 2 : 1][RES ? 8 * kWave : 1];
+    __shared__ __align__(16) float lds_r[RES ? 2 : 1][RES ? 8 * kMxRPitch : 1];
+    // per wave, NB slots: an input row window (32 columns x 16 k) and, with
+    // PRO, the coarse row the step's window takes in (12 columns x 16 k)
+    __shared__ __align__(16) float lds_a[kBlock / kWave][NB][kMxWin * 16];
+    __shared__ __align__(16) float lds_c[PRO ? kBlock / kWave : 1][PRO ? NB : 1][PRO ? kMxCCols * 16 : 4];
 
     const int lane = threadIdx.x & (kWave - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -86,10 +151,18 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
     }
 
     // ---- phase A roles ----
-    const int kA = k0 + 16 * w + ln;          // A operand row (one k per lane)
-    int vo[8];                                // byte offsets of the window columns
+    // The input window of a row reaches LDS by two direct 16-byte loads:
+    // window column c = 8 g + e (g = lane / 16 of the A layout, e its value)
+    // sits at position 4 e + g, 16 k each, so lane l reads value e at
+    // 64 e + l (64 consecutive floats per read, no bank conflict). Load q,
+    // lane l: the 4 k from 4 (l % 4) at the column of position 16 q + l / 4.
+    int vr[2];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) vo[e] = kA * 4 + mirror_half(i0 - H + 8 * g4 + e, n) * K * 4;
+    for (int q = 0; q < 2; ++q) {
+        const int pos = 16 * q + (lane >> 2);
+        const int c = 8 * (pos & 3) + (pos >> 2);
+        vr[q] = (k0 + 16 * w + 4 * (lane & 3)) * 4 + mirror_half(i0 - H + c, n) * K * 4;
+    }
     const int kC = k0 + 16 * w + 4 * g4;      // first of the 4 k of the C layout
     const int iC = i0 + ln;                   // output column of the C layout
 
@@ -101,11 +174,17 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
             bh[b] = bf[(b * 2 + 0) * kWave + lane];
             bl[b] = bf[(b * 2 + 1) * kWave + lane];
         }
+#pragma unroll
+        for (int b = 0; b <= H; ++b) {
+            mx_settle(bh[b]);
+            mx_settle(bl[b]);
+        }
     }
     mx_half8 rx;
     if constexpr (RES) {
         const int cls = i0 == 0 ? 1 : (i0 + kMxCols >= n ? 2 : 0);
         rx = reinterpret_cast<const mx_half8*>(p.rfrag)[cls * kWave + lane];
+        mx_settle(rx);
     }
 
     // scaling state: A lane exponent, and per C row the output exponents
@@ -121,33 +200,37 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
     for (int s = 0; s < NR; ++s) acc[s] = mx_float4{0.f, 0.f, 0.f, 0.f};
 
     // prolongation window (C layout): 4 x-interpolated coarse rows of the
-    // lane's 4 k at its fine column
+    // lane's 4 k at its fine column. The lane's 4 coarse columns cb .. cb+3
+    // are columns cci .. cci+3 of the wave's 12 (i0/2 - 2 .. i0/2 + 9), which
+    // a step's coarse row brings to LDS in one 16-byte load: lane f < 48
+    // takes the 4 k from 4 (f % 4) at column f / 4.
     const int nc = n / 2, mc = m / 2;
     const size_t crow = (size_t)nc * K;
     mx_float4 XV[4];
-    int ccol[4];
     float cxw[4];
-    int cbase = 0;
+    int cci = 0, voc = 0;
     auto wbase = [](int j) { return ((j + 1) >> 1) - 2; };
-    auto load_cx = [&](int r) -> mx_float4 {
-        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(r, mc) * crow);
-        mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int t = 0; t < 4; ++t) x += cxw[t] * mx_ld4(rc, ccol[t], 0);
-        return x;
-    };
     if constexpr (PRO) {
         const int iu = iC >> 1;
         const bool odd = iC & 1;
         const int cb = odd ? iu - 1 : iu - 2;
+        cci = cb - (i0 / 2 - 2);
+        int ccol[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             ccol[t] = kC * 4 + mirror_half(cb + t, nc) * K * 4;
             cxw[t] = odd ? kPc[t] : kQc[t];
         }
-        cbase = wbase(j0);
+        voc = (k0 + 16 * w + 4 * (lane & 3)) * 4 + mirror_half(i0 / 2 - 2 + (lane >> 2), nc) * K * 4;
+        const int cbase = wbase(j0);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) XV[u] = load_cx(cbase + u);
+        for (int u = 0; u < 4; ++u) {
+            const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(cbase + u, mc) * crow);
+            mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x += cxw[t] * mx_ld4(rc, ccol[t], 0);
+            XV[u] = x;
+        }
     }
 
     // ---- phase B roles: lane = k, columns i0 + 4w .. +3 ----
@@ -164,6 +247,8 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
 #pragma unroll
             for (int d = 0; d < NVB; ++d) vc[d] = d == VHB ? 1.f : 0.f;
         }
+#pragma unroll
+        for (int d = 0; d < NVB; ++d) mx_settle(vc[d]);
     }
 
     const int tj_eff = p.TJ < m - j0 ? p.TJ : m - j0;
@@ -178,16 +263,48 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
         for (int q = 0; q < NQ; ++q) racc[q][0] = racc[q][1] = 0.f;
     }
 
-    // input rows, loaded D steps ahead
-    float vq[D][8];
-    auto load_row = [&](int rr, float* dst) {
-        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)mirror_half(j0 - H + rr, m) * rowstride);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) dst[e] = buf_ld(rs, vo[e], 0);
+    // Step rr reads input row rr (j0 - H + rr, mirrored) and has an output
+    // row j0 + rr - 2H when 2H <= rr < 2H + tj_eff. With PRO the window of 4
+    // coarse rows moves down one row at output rows whose base wbase(j)
+    // differs from the previous row's.
+    const int olo = 2 * H, ohi = 2 * H + tj_eff;
+    auto cmove_at = [&](int rr) {
+        const int j = j0 + rr - 2 * H;
+        return rr > olo && rr < ohi && wbase(j) != wbase(j - 1);
     };
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-        if (d < nsteps) load_row(d, vq[d]);
+    // Everything a step reads from memory reaches LDS NB - 1 steps ahead,
+    // the coarse row before the input row, so that the wait for the input
+    // row covers both: the loads complete in order, and waiting for a load
+    // issued later would also wait for every row issued before it.
+    auto issue_step = [&](int rr, int sl) {
+        if constexpr (PRO) {
+            if (cmove_at(rr)) {
+                const int j = j0 + rr - 2 * H;
+                const __amdgpu_buffer_rsrc_t rc =
+                    buf_rsrc(p.coarse + (size_t)mirror_half(wbase(j) + 3, mc) * crow);
+                if (lane < kMxCCols * 4)
+                    mx_ld_lds16(rc, &lds_c[w][sl][0], voc);
+            }
+        }
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)mirror_half(j0 - H + rr, m) * rowstride);
+        mx_ld_lds16(rs, &lds_a[w][sl][0], vr[0]);
+        mx_ld_lds16(rs, &lds_a[w][sl][kWave * 4], vr[1]);
+    };
+    // Vector memory operations issued after input row rr's loads, counted
+    // when step rr waits for it (a lower bound): the later rows' loads and
+    // the 4 output stores of each step with an output row from the step
+    // that issued row rr on.
+    auto outputs_in = [&](int a, int b) {   // steps with an output row in [a, b)
+        const int lo = a > olo ? a : olo, hi = b < ohi ? b : ohi;
+        return hi > lo ? hi - lo : 0;
+    };
+    auto behind = [&](int rr) {
+        const int last = rr + NB - 2 < nsteps - 1 ? rr + NB - 2 : nsteps - 1;
+        const int a = rr - NB + 1 > 0 ? rr - NB + 1 : 0;
+        return 2 * (last - rr) + 4 * outputs_in(a, rr);
+    };
+    for (int d = 0; d < NB - 1 && d < nsteps; ++d) issue_step(d, d);
+    int rslot = 0;   // slot of the step
 
     for (int rb = 0; rb < nsteps; rb += NR) {
 #pragma unroll
@@ -195,15 +312,18 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
             const int rr = rb + s;
             if (rr < nsteps) {
             const int buf = rr & 1;
+            const int jrel = rr - 2 * H;
+            const bool orow = jrel >= 0 && jrel < tj_eff;
+            const int j = j0 + jrel;
+            const int slot = rslot;
             // ================= phase A =================
+            mx_wait_vmem(behind(rr));
             float v[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = vq[0][e];
-#pragma unroll
-            for (int d = 0; d + 1 < D; ++d)
-#pragma unroll
-                for (int e = 0; e < 8; ++e) vq[d][e] = vq[d + 1][e];
-            if (rr + D < nsteps) load_row(rr + D, vq[D - 1]);
+            for (int e = 0; e < 8; ++e) v[e] = lds_a[w][slot][e * kWave + lane];
+            // into the slot of the previous step, read there
+            if (rr + NB - 1 < nsteps) issue_step(rr + NB - 1, slot == 0 ? NB - 1 : slot - 1);
+            rslot = slot == NB - 1 ? 0 : slot + 1;
 
             float mx = fabsf(v[0]);
 #pragma unroll
@@ -239,18 +359,17 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
                 ah[e] = h;
                 al[e] = (_Float16)(xs - (float)h);
             }
-            const int slot = s;
-            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[0], acc[slot], 0, 0, 0);
-            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[0], acc[slot], 0, 0, 0);
-            acc[slot] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[0], acc[slot], 0, 0, 0);
+            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[0], acc[s], 0, 0, 0);
+            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[0], acc[s], 0, 0, 0);
+            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[0], acc[s], 0, 0, 0);
 #pragma unroll
             for (int b = 1; b <= H; ++b) {
                 mx_float4 z = mx_float4{0.f, 0.f, 0.f, 0.f};
                 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[b], z, 0, 0, 0);
                 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[b], z, 0, 0, 0);
                 z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[b], z, 0, 0, 0);
-                acc[(slot + b) % NR] += z;
-                acc[(slot + NR - b) % NR] += z;
+                acc[(s + b) % NR] += z;
+                acc[(s + NR - b) % NR] += z;
             }
             const int vr = j0 - H + rr;   // the loaded fine row, before mirroring
             if constexpr (RES) {
@@ -262,32 +381,26 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
                         mx_float4 cu;
 #pragma unroll
                         for (int q = 0; q < 4; ++q) cu[q] = __builtin_amdgcn_ldexpf(cr[q], eC[q] + p.wexp);
-                        *reinterpret_cast<mx_float4*>(&lds_r[buf][ln * kWave + 16 * w + 4 * g4]) = cu;
+                        *reinterpret_cast<mx_float4*>(&lds_r[buf][ln * kMxRPitch + 16 * w + 4 * g4]) = cu;
                     }
                 }
             }
-            const int jrel = rr - 2 * H;
             constexpr int oslot = (NR - H) % NR;   // (s - H) mod NR, shifted by s below
             const int os = (s + oslot) % NR;
-            const bool orow = jrel >= 0 && jrel < tj_eff;
             if (orow) {
-                const int j = j0 + jrel;
                 mx_float4 o;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) o[q] = __builtin_amdgcn_ldexpf(acc[os][q], eC[q]);
-                if (p.wfield) {
-                    const __amdgpu_buffer_rsrc_t rw = buf_rsrc(p.wfield + (size_t)j * rowstride);
-                    o *= mx_ld4(rw, kC * 4 + iC * K * 4, 0);
-                } else {
-                    o *= p.wconst;
-                }
+                o *= p.wconst;
                 if constexpr (PRO) {
-                    const int nb = wbase(j);
-                    if (nb != cbase) {   // the window moves down one coarse row
+                    if (cmove_at(rr)) {
 #pragma unroll
                         for (int u = 0; u < 3; ++u) XV[u] = XV[u + 1];
-                        XV[3] = load_cx(nb + 3);
-                        cbase = nb;
+                        mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                        for (int t = 0; t < 4; ++t)
+                            x += cxw[t] * *reinterpret_cast<const mx_float4*>(&lds_c[w][slot][(cci + t) * 16 + 4 * g4]);
+                        XV[3] = x;
                     }
                     const bool oj = (j & 1) != 0;
 #pragma unroll
@@ -298,11 +411,14 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
                     lds_o[buf][(VHB + 16 * w + 4 * g4 + q) * kMxPitch + ln] = o[q];
             }
             acc[os] = mx_float4{0.f, 0.f, 0.f, 0.f};
-            __syncthreads();
+            // this wave's LDS writes done, then the block's barrier (no
+            // fence: it would also wait for the input rows in flight)
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
             // ================= phase B =================
             if constexpr (RES) {
                 if (vr >= 0 && vr < m) {
-                    const float xr[2] = {lds_r[buf][(2 * w) * kWave + lane], lds_r[buf][(2 * w + 1) * kWave + lane]};
+                    const float xr[2] = {lds_r[buf][(2 * w) * kMxRPitch + lane],
+                                         lds_r[buf][(2 * w + 1) * kMxRPitch + lane]};
                     using RS = Stencil<false>;
                     auto store_coarse = [&](int J, const float* val) {
                         if (J >= Jlo && J < Jhi) {
@@ -357,7 +473,6 @@ __global__ void __launch_bounds__(kBlock) radial_mx_kernel(RadialMxParams p) {
                 }
             }
             if (orow) {
-                const int j = j0 + jrel;
                 float out[4];
                 if constexpr (VHB > 0) {
 #pragma unroll
@@ -392,7 +507,6 @@ hipError_t launch_radial_mx_variant(const RadialArgs& a, hipStream_t st) {
     RadialMxParams p{};
     p.in = a.in;
     p.out = a.out;
-    p.wfield = a.wfield;
     p.wconst = a.wconst;
     p.coarse = a.coarse;
     p.band = a.band;
@@ -411,7 +525,7 @@ hipError_t launch_radial_mx_variant(const RadialArgs& a, hipStream_t st) {
     p.ntile_i = a.n / kMxCols;
     p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
     const long long nblocks = (long long)p.ntile_i * p.ntile_j * (a.K / kWave);
-    hipLaunchKernelGGL((radial_mx_kernel<H, PRO, VHB, RES, 2>), dim3((unsigned)nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((radial_mx_kernel<H, PRO, VHB, RES, kMxSlots>), dim3((unsigned)nblocks), dim3(kBlock), 0, st, p);
     return hipGetLastError();
 }
 
@@ -422,7 +536,7 @@ hipError_t launch_radial_mx_h(const RadialArgs& a, hipStream_t st) {
     if constexpr (kMxCols + 2 * H > kMxWin) {
         return hipErrorInvalidValue;
     } else {
-        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || a.quad || !a.bfrag)
+        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || a.quad || !a.bfrag || a.wfield)
             return hipErrorInvalidValue;
         if ((long long)a.n * a.K * 4 >= (1ll << 31)) return hipErrorInvalidValue;
         const bool pro = a.coarse != nullptr;
