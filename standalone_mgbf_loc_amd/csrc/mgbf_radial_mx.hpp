@@ -43,6 +43,9 @@
 // both phases.
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "mgbf_device.hpp"
 #include "mgbf_launch.hpp"
 
@@ -65,16 +68,27 @@ __device__ __forceinline__ mx_float4 mx_ld4(__amdgpu_buffer_rsrc_t r, int voff, 
 }
 
 // 16 bytes per lane from the buffer straight into LDS: lane l's bytes land
-// at lds + 4 l floats. Completion is counted by the vector memory counter
-// only, and the caller waits for it explicitly. The LDS address reaches the
-// load as an integer: the compiler then treats the load as writing any LDS
-// (so it keeps LDS accesses in order around it) but does not track it as a
-// store to one variable, which would make it wait for every load in flight
+// at byte lds + 16 l. Completion is counted by the vector memory counter
+// only, and the caller waits for it explicitly. The LDS address arrives as
+// an integer: the compiler then treats the load as writing any LDS (so it
+// keeps LDS accesses in order around it) but does not track it as a store
+// to one variable, which would make it wait for every load in flight
 // before later LDS reads.
-__device__ __forceinline__ void mx_ld_lds16(__amdgpu_buffer_rsrc_t r, float* lds, int voff) {
-    const unsigned a = (unsigned)(size_t)(__attribute__((address_space(3))) float*)lds;
+__device__ __forceinline__ void mx_ld_lds16(__amdgpu_buffer_rsrc_t r, unsigned lds, int voff) {
     __builtin_amdgcn_raw_ptr_buffer_load_lds(
-        r, (__attribute__((address_space(3))) void*)(size_t)__builtin_amdgcn_readfirstlane(a), 16, voff, 0, 0, 0);
+        r, (__attribute__((address_space(3))) void*)(size_t)__builtin_amdgcn_readfirstlane(lds), 16, voff, 0, 0,
+        0);
+}
+
+// byte address of an LDS location
+__device__ __forceinline__ unsigned mx_lds_addr(const float* x) {
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) float*)x;
+}
+
+// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time
+template <class F, int... I>
+__device__ __forceinline__ void mx_static_for(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
 }
 
 // Makes a value loaded before the row loop count as consumed there: the
@@ -268,7 +282,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
     // coarse rows moves down one row at output rows whose base wbase(j)
     // differs from the previous row's.
     const int olo = 2 * H, ohi = 2 * H + tj_eff;
-    auto cmove_at = [&](int rr) {
+    const unsigned lds_a0 = mx_lds_addr(&lds_a[w][0][0]);
+    const unsigned lds_c0 = PRO ? mx_lds_addr(&lds_c[w][0][0]) : 0u;
+    auto cmove_at = [&](int rr) __attribute__((always_inline)) {
         const int j = j0 + rr - 2 * H;
         return rr > olo && rr < ohi && wbase(j) != wbase(j - 1);
     };
@@ -276,29 +292,29 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
     // the coarse row before the input row, so that the wait for the input
     // row covers both: the loads complete in order, and waiting for a load
     // issued later would also wait for every row issued before it.
-    auto issue_step = [&](int rr, int sl) {
+    auto issue_step = [&](int rr, int sl) __attribute__((always_inline)) {
         if constexpr (PRO) {
             if (cmove_at(rr)) {
                 const int j = j0 + rr - 2 * H;
                 const __amdgpu_buffer_rsrc_t rc =
                     buf_rsrc(p.coarse + (size_t)mirror_half(wbase(j) + 3, mc) * crow);
-                if (lane < kMxCCols * 4)
-                    mx_ld_lds16(rc, &lds_c[w][sl][0], voc);
+                if (lane < kMxCCols * 4) mx_ld_lds16(rc, lds_c0 + sl * (kMxCCols * 16 * 4), voc);
             }
         }
         const __amdgpu_buffer_rsrc_t rs = buf_rsrc(p.in + (size_t)mirror_half(j0 - H + rr, m) * rowstride);
-        mx_ld_lds16(rs, &lds_a[w][sl][0], vr[0]);
-        mx_ld_lds16(rs, &lds_a[w][sl][kWave * 4], vr[1]);
+        const unsigned la = lds_a0 + sl * (kMxWin * 16 * 4);
+        mx_ld_lds16(rs, la, vr[0]);
+        mx_ld_lds16(rs, la + kWave * 16, vr[1]);
     };
     // Vector memory operations issued after input row rr's loads, counted
     // when step rr waits for it (a lower bound): the later rows' loads and
     // the 4 output stores of each step with an output row from the step
     // that issued row rr on.
-    auto outputs_in = [&](int a, int b) {   // steps with an output row in [a, b)
+    auto outputs_in = [&](int a, int b) __attribute__((always_inline)) {   // steps with an output row in [a, b)
         const int lo = a > olo ? a : olo, hi = b < ohi ? b : ohi;
         return hi > lo ? hi - lo : 0;
     };
-    auto behind = [&](int rr) {
+    auto behind = [&](int rr) __attribute__((always_inline)) {
         const int last = rr + NB - 2 < nsteps - 1 ? rr + NB - 2 : nsteps - 1;
         const int a = rr - NB + 1 > 0 ? rr - NB + 1 : 0;
         return 2 * (last - rr) + 4 * outputs_in(a, rr);
@@ -306,200 +322,239 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
     for (int d = 0; d < NB - 1 && d < nsteps; ++d) issue_step(d, d);
     int rslot = 0;   // slot of the step
 
-    for (int rb = 0; rb < nsteps; rb += NR) {
+    // One row step: s is the step's place in the ring (rr % NR), FAST a
+    // block of NR steps inside the tile where every step has an output row,
+    // issues the row NB - 1 steps ahead, waits behind the steady count of
+    // later operations and (RES) reads a row away from the domain edges.
+    constexpr int kSteady = 2 * (NB - 2) + 4 * (NB - 1);
+    auto step = [&](const int rr, auto S, auto F) __attribute__((always_inline)) {
+        constexpr int s = decltype(S)::value;
+        constexpr bool FAST = decltype(F)::value;
+        const int buf = rr & 1;
+        const int jrel = rr - 2 * H;
+        const bool orow = FAST || (jrel >= 0 && jrel < tj_eff);
+        const int j = j0 + jrel;
+        const int slot = rslot;
+        // steps are scheduled one by one: interleaving them would hold two
+        // steps' operands at once
+        __builtin_amdgcn_sched_barrier(0);
+        // ================= phase A =================
+        mx_wait_vmem(FAST ? kSteady : behind(rr));
+        float v[8];
 #pragma unroll
-        for (int s = 0; s < NR; ++s) {
-            const int rr = rb + s;
-            if (rr < nsteps) {
-            const int buf = rr & 1;
-            const int jrel = rr - 2 * H;
-            const bool orow = jrel >= 0 && jrel < tj_eff;
-            const int j = j0 + jrel;
-            const int slot = rslot;
-            // ================= phase A =================
-            mx_wait_vmem(behind(rr));
-            float v[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = lds_a[w][slot][e * kWave + lane];
-            // into the slot of the previous step, read there
-            if (rr + NB - 1 < nsteps) issue_step(rr + NB - 1, slot == 0 ? NB - 1 : slot - 1);
-            rslot = slot == NB - 1 ? 0 : slot + 1;
+        for (int e = 0; e < 8; ++e) v[e] = lds_a[w][slot][e * kWave + lane];
+        // into the slot of the previous step, read there
+        if (FAST || rr + NB - 1 < nsteps) issue_step(rr + NB - 1, slot == 0 ? NB - 1 : slot - 1);
+        rslot = slot == NB - 1 ? 0 : slot + 1;
 
-            float mx = fabsf(v[0]);
+        float mx = fabsf(v[0]);
 #pragma unroll
-            for (int e = 1; e < 8; ++e) mx = fmaxf(mx, fabsf(v[e]));
-            if (__any(mx > thr)) {
-                // a k of this wave leaves the FP16 range: lower its exponent
-                // so that the row's max scales below 2^14 (the 4 lanes of a
-                // k agree on the max), rescale its ring rows exactly
-                float mk = fmaxf(mx, __shfl_xor(mx, 16));
-                mk = fmaxf(mk, __shfl_xor(mk, 32));
-                int exn = ex;
-                if (mk > thr) {
-                    exn = 14 - __builtin_amdgcn_frexp_expf(mk);
-                    exn = exn < kMxExMin ? kMxExMin : exn;
+        for (int e = 1; e < 8; ++e) mx = fmaxf(mx, fabsf(v[e]));
+        if (__any(mx > thr)) {
+            // a k of this wave leaves the FP16 range: lower its exponent so
+            // that the row's max scales below 2^14 (the 4 lanes of a k agree
+            // on the max), rescale its ring rows exactly. The ring is
+            // rescaled as whole C tuples (two power-of-two factors, each a
+            // normal float), which keeps the ring in its registers.
+            float mk = fmaxf(mx, __shfl_xor(mx, 16));
+            mk = fmaxf(mk, __shfl_xor(mk, 32));
+            int exn = ex;
+            if (mk > thr) {
+                exn = 14 - __builtin_amdgcn_frexp_expf(mk);
+                exn = exn < kMxExMin ? kMxExMin : exn;
+            }
+            const int dex = exn - ex;
+            ex = exn;
+            sc = __builtin_amdgcn_ldexpf(1.f, ex);
+            thr = __builtin_amdgcn_ldexpf(1.f, 14 - ex);
+            mx_float4 fa, fb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int dq = __shfl(dex, 4 * g4 + q);
+                eC[q] -= dq;
+                fa[q] = __builtin_amdgcn_ldexpf(1.f, dq / 2);
+                fb[q] = __builtin_amdgcn_ldexpf(1.f, dq - dq / 2);
+            }
+#pragma unroll
+            for (int sl = 0; sl < NR; ++sl) acc[sl] = acc[sl] * fa * fb;
+        }
+        mx_half8 ah, al;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float xs = v[e] * sc;
+            const _Float16 h = (_Float16)xs;
+            ah[e] = h;
+            al[e] = (_Float16)(xs - (float)h);
+        }
+        acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[0], acc[s], 0, 0, 0);
+        acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[0], acc[s], 0, 0, 0);
+        acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[0], acc[s], 0, 0, 0);
+#pragma unroll
+        for (int b = 1; b <= H; ++b) {
+            mx_float4 z = mx_float4{0.f, 0.f, 0.f, 0.f};
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[b], z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[b], z, 0, 0, 0);
+            z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[b], z, 0, 0, 0);
+            acc[(s + b) % NR] += z;
+            acc[(s + NR - b) % NR] += z;
+        }
+        const int vr = j0 - H + rr;   // the loaded fine row, before mirroring
+        const bool vin = FAST || (vr >= 0 && vr < m);
+        if constexpr (RES) {
+            if (vin) {
+                mx_float4 cr = mx_float4{0.f, 0.f, 0.f, 0.f};
+                cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rx, cr, 0, 0, 0);
+                cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rx, cr, 0, 0, 0);
+                if (ln < 8) {
+                    mx_float4 cu;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) cu[q] = __builtin_amdgcn_ldexpf(cr[q], eC[q] + p.wexp);
+                    *reinterpret_cast<mx_float4*>(&lds_r[buf][ln * kMxRPitch + 16 * w + 4 * g4]) = cu;
                 }
-                const int dex = exn - ex;
-                ex = exn;
-                sc = __builtin_amdgcn_ldexpf(1.f, ex);
-                thr = __builtin_amdgcn_ldexpf(1.f, 14 - ex);
+            }
+        }
+        constexpr int os = (s + NR - H) % NR;   // the ring row that completes
+        if (orow) {
+            mx_float4 o;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int dq = __shfl(dex, 4 * g4 + q);
-                    eC[q] -= dq;
+            for (int q = 0; q < 4; ++q) o[q] = __builtin_amdgcn_ldexpf(acc[os][q], eC[q]);
+            o *= p.wconst;
+            if constexpr (PRO) {
+                // the window moves at odd output rows past the first
+                if (FAST ? (j & 1) != 0 : cmove_at(rr)) {
 #pragma unroll
-                    for (int sl = 0; sl < NR; ++sl) acc[sl][q] = __builtin_amdgcn_ldexpf(acc[sl][q], dq);
+                    for (int u = 0; u < 3; ++u) XV[u] = XV[u + 1];
+                    mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int t = 0; t < 4; ++t)
+                        x += cxw[t] * *reinterpret_cast<const mx_float4*>(&lds_c[w][slot][(cci + t) * 16 + 4 * g4]);
+                    XV[3] = x;
                 }
-            }
-            mx_half8 ah, al;
+                const bool oj = (j & 1) != 0;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float xs = v[e] * sc;
-                const _Float16 h = (_Float16)xs;
-                ah[e] = h;
-                al[e] = (_Float16)(xs - (float)h);
+                for (int u = 0; u < 4; ++u) o += (oj ? kPc[u] : kQc[u]) * XV[u];
             }
-            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[0], acc[s], 0, 0, 0);
-            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[0], acc[s], 0, 0, 0);
-            acc[s] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[0], acc[s], 0, 0, 0);
 #pragma unroll
-            for (int b = 1; b <= H; ++b) {
-                mx_float4 z = mx_float4{0.f, 0.f, 0.f, 0.f};
-                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[b], z, 0, 0, 0);
-                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[b], z, 0, 0, 0);
-                z = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[b], z, 0, 0, 0);
-                acc[(s + b) % NR] += z;
-                acc[(s + NR - b) % NR] += z;
-            }
-            const int vr = j0 - H + rr;   // the loaded fine row, before mirroring
-            if constexpr (RES) {
-                if (vr >= 0 && vr < m) {
-                    mx_float4 cr = mx_float4{0.f, 0.f, 0.f, 0.f};
-                    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, rx, cr, 0, 0, 0);
-                    cr = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, rx, cr, 0, 0, 0);
-                    if (ln < 8) {
-                        mx_float4 cu;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) cu[q] = __builtin_amdgcn_ldexpf(cr[q], eC[q] + p.wexp);
-                        *reinterpret_cast<mx_float4*>(&lds_r[buf][ln * kMxRPitch + 16 * w + 4 * g4]) = cu;
+            for (int q = 0; q < 4; ++q) lds_o[buf][(VHB + 16 * w + 4 * g4 + q) * kMxPitch + ln] = o[q];
+        }
+        acc[os] = mx_float4{0.f, 0.f, 0.f, 0.f};
+        // this wave's LDS writes done, then the block's barrier (no fence:
+        // it would also wait for the input rows in flight); nothing is
+        // scheduled across it
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        // ================= phase B =================
+        if constexpr (RES) {
+            if (vin) {
+                const float xr[2] = {lds_r[buf][(2 * w) * kMxRPitch + lane],
+                                     lds_r[buf][(2 * w + 1) * kMxRPitch + lane]};
+                using RS = Stencil<false>;
+                auto store_coarse = [&](int J, const float* val) {
+                    if (J >= Jlo && J < Jhi) {
+                        const __amdgpu_buffer_rsrc_t ro =
+                            buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2 + 2 * w) + (size_t)nc * J));
+                        buf_st(val[0], ro, vkB, 0);
+                        buf_st(val[1], ro, vkB, K * 4);
                     }
-                }
-            }
-            constexpr int oslot = (NR - H) % NR;   // (s - H) mod NR, shifted by s below
-            const int os = (s + oslot) % NR;
-            if (orow) {
-                mx_float4 o;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) o[q] = __builtin_amdgcn_ldexpf(acc[os][q], eC[q]);
-                o *= p.wconst;
-                if constexpr (PRO) {
-                    if (cmove_at(rr)) {
-#pragma unroll
-                        for (int u = 0; u < 3; ++u) XV[u] = XV[u + 1];
-                        mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                        for (int t = 0; t < 4; ++t)
-                            x += cxw[t] * *reinterpret_cast<const mx_float4*>(&lds_c[w][slot][(cci + t) * 16 + 4 * g4]);
-                        XV[3] = x;
-                    }
-                    const bool oj = (j & 1) != 0;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) o += (oj ? kPc[u] : kQc[u]) * XV[u];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    lds_o[buf][(VHB + 16 * w + 4 * g4 + q) * kMxPitch + ln] = o[q];
-            }
-            acc[os] = mx_float4{0.f, 0.f, 0.f, 0.f};
-            // this wave's LDS writes done, then the block's barrier (no
-            // fence: it would also wait for the input rows in flight)
-            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-            // ================= phase B =================
-            if constexpr (RES) {
-                if (vr >= 0 && vr < m) {
-                    const float xr[2] = {lds_r[buf][(2 * w) * kMxRPitch + lane],
-                                         lds_r[buf][(2 * w + 1) * kMxRPitch + lane]};
-                    using RS = Stencil<false>;
-                    auto store_coarse = [&](int J, const float* val) {
-                        if (J >= Jlo && J < Jhi) {
-                            const __amdgpu_buffer_rsrc_t ro =
-                                buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2 + 2 * w) + (size_t)nc * J));
-                            buf_st(val[0], ro, vkB, 0);
-                            buf_st(val[1], ro, vkB, K * 4);
-                        }
-                    };
-                    if (vr >= 8 && vr <= m - 9) {
-                        constexpr int NO = RS::NR / 2;
-                        constexpr int CP = (RS::ROFF + RS::NR - 1) & 1;
-                        const bool closes = (vr & 1) == CP;
+                };
+                constexpr int NO = RS::NR / 2;
+                constexpr int CP = (RS::ROFF + RS::NR - 1) & 1;
+                if (FAST || (vr >= 8 && vr <= m - 9)) {
+                    // interior rows: the row enters the NO open coarse rows
+                    // with the stencil's odd or even taps, and with the odd
+                    // ones it closes the oldest
+                    if ((vr & 1) == CP) {
 #pragma unroll
                         for (int q = 0; q < NO; ++q) {
-                            const float yw = closes ? RS::R[RS::NR - 1 - 2 * q] : RS::R[RS::NR - 2 - 2 * q];
-                            racc[q][0] = __builtin_fmaf(yw, xr[0], racc[q][0]);
-                            racc[q][1] = __builtin_fmaf(yw, xr[1], racc[q][1]);
+                            racc[q][0] = __builtin_fmaf(RS::R[RS::NR - 1 - 2 * q], xr[0], racc[q][0]);
+                            racc[q][1] = __builtin_fmaf(RS::R[RS::NR - 1 - 2 * q], xr[1], racc[q][1]);
                         }
-                        if (closes) {
-                            store_coarse((vr - RS::ROFF - RS::NR + 1) >> 1, racc[0]);
+                        store_coarse((vr - RS::ROFF - RS::NR + 1) >> 1, racc[0]);
 #pragma unroll
-                            for (int q = 0; q + 1 < NO; ++q) {
-                                racc[q][0] = racc[q + 1][0];
-                                racc[q][1] = racc[q + 1][1];
-                            }
-                            racc[NO - 1][0] = racc[NO - 1][1] = 0.f;
+                        for (int q = 0; q + 1 < NO; ++q) {
+                            racc[q][0] = racc[q + 1][0];
+                            racc[q][1] = racc[q + 1][1];
                         }
+                        racc[NO - 1][0] = racc[NO - 1][1] = 0.f;
                     } else {
-                        const float4 ya = p.rytab[2 * vr], yb = p.rytab[2 * vr + 1];
-                        const float yw[NQ] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y};
-                        const int2 meta = p.rymeta[vr];
 #pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            racc[q][0] = __builtin_fmaf(yw[q], xr[0], racc[q][0]);
-                            racc[q][1] = __builtin_fmaf(yw[q], xr[1], racc[q][1]);
+                        for (int q = 0; q < NO; ++q) {
+                            racc[q][0] = __builtin_fmaf(RS::R[RS::NR - 2 - 2 * q], xr[0], racc[q][0]);
+                            racc[q][1] = __builtin_fmaf(RS::R[RS::NR - 2 - 2 * q], xr[1], racc[q][1]);
                         }
-#pragma unroll
-                        for (int e = 0; e < NQ; ++e)
-                            if (e < meta.y) store_coarse(meta.x + e, racc[e]);
-#pragma unroll
-                        for (int sft = 1; sft <= NQ; ++sft) {
-                            if (meta.y == sft) {
-#pragma unroll
-                                for (int q = 0; q < NQ; ++q) {
-                                    racc[q][0] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][0] : 0.f;
-                                    racc[q][1] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][1] : 0.f;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
-            if (orow) {
-                float out[4];
-                if constexpr (VHB > 0) {
-#pragma unroll
-                    for (int t = 0; t < 4; ++t) out[t] = 0.f;
-#pragma unroll
-                    for (int d = 0; d < NVB; ++d) {
-                        const float4 r4 =
-                            *reinterpret_cast<const float4*>(&lds_o[buf][(lane + d) * kMxPitch + 4 * w]);
-                        out[0] = __builtin_fmaf(vc[d], r4.x, out[0]);
-                        out[1] = __builtin_fmaf(vc[d], r4.y, out[1]);
-                        out[2] = __builtin_fmaf(vc[d], r4.z, out[2]);
-                        out[3] = __builtin_fmaf(vc[d], r4.w, out[3]);
                     }
                 } else {
-                    const float4 r4 = *reinterpret_cast<const float4*>(&lds_o[buf][lane * kMxPitch + 4 * w]);
-                    out[0] = r4.x;
-                    out[1] = r4.y;
-                    out[2] = r4.z;
-                    out[3] = r4.w;
-                }
-                const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + (size_t)j * rowstride + (size_t)i0 * K);
+                    const float4 ya = p.rytab[2 * vr], yb = p.rytab[2 * vr + 1];
+                    const float yw[NQ] = {ya.x, ya.y, ya.z, ya.w, yb.x, yb.y};
+                    const int2 meta = p.rymeta[vr];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) buf_st(out[t], ro, vkB, (4 * w + t) * K * 4);
+                    for (int q = 0; q < NQ; ++q) {
+                        racc[q][0] = __builtin_fmaf(yw[q], xr[0], racc[q][0]);
+                        racc[q][1] = __builtin_fmaf(yw[q], xr[1], racc[q][1]);
+                    }
+#pragma unroll
+                    for (int e = 0; e < NQ; ++e)
+                        if (e < meta.y) store_coarse(meta.x + e, racc[e]);
+#pragma unroll
+                    for (int sft = 1; sft <= NQ; ++sft) {
+                        if (meta.y == sft) {
+#pragma unroll
+                            for (int q = 0; q < NQ; ++q) {
+                                racc[q][0] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][0] : 0.f;
+                                racc[q][1] = q + sft < NQ ? racc[q + sft < NQ ? q + sft : 0][1] : 0.f;
+                            }
+                        }
+                    }
+                }
             }
-            }   // rr < nsteps
         }
+        if (orow) {
+            float out[4];
+            if constexpr (VHB > 0) {
+#pragma unroll
+                for (int t = 0; t < 4; ++t) out[t] = 0.f;
+#pragma unroll
+                for (int d = 0; d < NVB; ++d) {
+                    const float4 r4 = *reinterpret_cast<const float4*>(&lds_o[buf][(lane + d) * kMxPitch + 4 * w]);
+                    out[0] = __builtin_fmaf(vc[d], r4.x, out[0]);
+                    out[1] = __builtin_fmaf(vc[d], r4.y, out[1]);
+                    out[2] = __builtin_fmaf(vc[d], r4.z, out[2]);
+                    out[3] = __builtin_fmaf(vc[d], r4.w, out[3]);
+                }
+            } else {
+                const float4 r4 = *reinterpret_cast<const float4*>(&lds_o[buf][lane * kMxPitch + 4 * w]);
+                out[0] = r4.x;
+                out[1] = r4.y;
+                out[2] = r4.z;
+                out[3] = r4.w;
+            }
+            const __amdgpu_buffer_rsrc_t ro = buf_rsrc(p.out + (size_t)j * rowstride + (size_t)i0 * K);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) buf_st(out[t], ro, vkB, (4 * w + t) * K * 4);
+        }
+    };
+    // Interior blocks start at a multiple of NR in [fast_lo, fast_hi - NR].
+    int fast_lo = olo + NB - 1, fast_hi = nsteps - NB + 1;
+    if constexpr (RES) {
+        fast_lo = fast_lo > 8 + H - j0 ? fast_lo : 8 + H - j0;
+        fast_hi = fast_hi < m - 8 + H - j0 ? fast_hi : m - 8 + H - j0;
     }
+    using Steps = std::make_integer_sequence<int, NR>;
+    auto general_block = [&](int rb) __attribute__((always_inline)) {
+        mx_static_for(
+            [&](auto S) __attribute__((always_inline)) {
+                if (rb + decltype(S)::value < nsteps) step(rb + decltype(S)::value, S, std::false_type{});
+            },
+            Steps{});
+    };
+    // the interior blocks form one run: general blocks before and after it
+    int rb = 0;
+    for (; rb < nsteps && !(rb >= fast_lo && rb + NR <= fast_hi); rb += NR) general_block(rb);
+    for (; rb + NR <= fast_hi; rb += NR)
+        mx_static_for([&](auto S) __attribute__((always_inline)) { step(rb + decltype(S)::value, S, std::true_type{}); },
+                      Steps{});
+    for (; rb < nsteps; rb += NR) general_block(rb);
 }
 
 template <int H, bool PRO, int VHB, bool RES>
