@@ -154,12 +154,18 @@ def row_cpu_baseline(MG, cfg, row):
             "sample": f"{what}, FP64 C restatement with OpenMP, {tc:.2f} s"}
 
 
-def pin_or_tune(args, op, log):
-    """Radial tiles: pinned from --tiles, else autotuned unless --no-tune
-    (then the defaults, reported as None)."""
-    if args.tiles:
+def pin_or_tune(args, op, log, profile_tiles=None):
+    """Radial tiles: pinned from --tiles; else, unless --tune, the tiles of
+    the traffic profile of this workload (found by the autotuner in that
+    session, so that the profiled traffic is this run's); else autotuned
+    unless --no-tune (then the defaults, reported as None)."""
+    pin = args.tiles
+    if pin is None and profile_tiles is not None and not args.tune and not args.no_tune:
+        pin = json.dumps(profile_tiles)
+        log("[bench] tiles of the traffic profile")
+    if pin:
         pinned = {(int(k[1:k.index("k")]), int(k[k.index("k") + 1:])): tuple(v)
-                  for k, v in json.loads(args.tiles).items()}
+                  for k, v in json.loads(pin).items()}
         op.set_tiles(pinned)
         tiles = op.tiles()
         log(f"[bench] pinned tiles: {tiles}")
@@ -391,6 +397,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default radial tiles")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the radial tiles instead of pinning those of the traffic profile")
     ap.add_argument("--tiles", default=None,
                     help='pin radial tiles instead of autotuning, JSON as printed under "tiles", '
                          'e.g. \'{"g1k0": [4, 384], ...}\'')
@@ -435,7 +443,8 @@ def main():
     y = torch.empty_like(x)
     op = MG.MGBF(cfg)
     assert op.dims.gm == gm
-    tiles = pin_or_tune(args, op, log)
+    traffic_bytes, traffic_tiles = profiled_traffic(M, n, lm, gm)
+    tiles = pin_or_tune(args, op, log, traffic_tiles)
 
     # ---- batched: one application over all members ----
     for _ in range(args.warmup):
@@ -514,7 +523,6 @@ def main():
 
     # the traffic figure comes from a profile run of this workload; say which
     # tiles it ran and whether this run's tiles are the same
-    traffic_bytes, traffic_tiles = profiled_traffic(M, n, lm, gm)
     run_tiles = {f"g{g}k{k}": list(v) for (g, k), v in tiles.items()} if tiles else None
     traffic_source = None
     if traffic_bytes is not None:
@@ -545,7 +553,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic_bytes,
                      "traffic_source": traffic_source,
-                     "kernel": "radial_kernel (F and F^T on every generation)",
+                     "kernel": "radial filter passes (F and F^T on every generation; matrix-core "
+                               "engine radial_mx_kernel where the tiles say 16, radial_kernel elsewhere)",
                      "launches": r_n, "avg_launch_ms": round(r_ms / max(r_n, 1), 4),
                      "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1)},
         "cpu_baseline": cpu,

@@ -114,9 +114,6 @@ struct mgbf_ctx {
     const float4* d_rytab[mgbf::kMaxGen] = {};
     const int2* d_rymeta[mgbf::kMaxGen] = {};
     bool res_ok[mgbf::kMaxGen] = {};
-    // per-k max |u_g| recorded by the restrictions
-    float* d_kmax = nullptr;                 // gm x K
-    unsigned kmax_valid = 0;                 // generations whose max is current
     // matrix-core engine: B fragments of the weight bands and, per
     // generation, of the x restriction (3 tile classes)
     const void* d_bfrag = nullptr;
@@ -435,7 +432,6 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
         a.rout = rout;
         a.rytab = c->d_rytab[g];
         a.rymeta = c->d_rymeta[g];
-        a.rkmax = c->d_kmax ? c->d_kmax + (size_t)(g + 1) * c->K : nullptr;
     }
     // the matrix-core engine takes constant weights; a weighted pass runs on
     // the vector kernel at the same TJ
@@ -455,11 +451,8 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
 
 int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t st) {
     Phase ph(c, MGBF_T_UPSEND, st, 4.0 * (double)(c->gsize[g] + c->gsize[g + 1]));
-    // the restriction reads all of generation g: it records max |u_g| per k
-    float* kmax = c->d_kmax ? c->d_kmax + (size_t)g * c->K : nullptr;
-    mgbf::TransferArgs a{fine, coarse, kmax, c->K, c->nx[g], c->ny[g],
+    mgbf::TransferArgs a{fine, coarse, c->K, c->nx[g], c->ny[g],
                          c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g], c->quad};
-    if (kmax) c->kmax_valid |= 1u << g;
     HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
     return MGBF_OK;
 }
@@ -468,7 +461,7 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
               hipStream_t st) {
     Phase ph(c, MGBF_T_DNSEND, st,
              4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)));
-    mgbf::TransferArgs a{coarse, fine, nullptr, c->K, c->nx[g], c->ny[g],
+    mgbf::TransferArgs a{coarse, fine, c->K, c->nx[g], c->ny[g],
                          c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g], c->quad};
     HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
     return MGBF_OK;
@@ -504,9 +497,6 @@ int vertical(mgbf_ctx* c, const float* in, float* out, int which, hipStream_t st
 
 // Restriction chain u_{g+1} = R u_g from u_1 (in U0 or a caller array).
 int restrict_chain(mgbf_ctx* c, const float* u1, float* U, hipStream_t st) {
-    c->kmax_valid = 0;
-    if (c->d_kmax && c->gm > 1)
-        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)(c->gm - 1) * c->K, st), "fill launch");
     for (int g = 0; g + 1 < c->gm; ++g)
         CALL(restrict_g(c, g, g == 0 ? u1 : U + c->goff[g], U + c->goff[g + 1], st));
     return MGBF_OK;
@@ -518,9 +508,6 @@ int restrict_chain(mgbf_ctx* c, const float* u1, float* U, hipStream_t st) {
 // in the radial pass where the tiles allow (fusion level 3), else it runs on
 // its own first. u_1 is read from u1.
 int down_sweep(mgbf_ctx* c, const float* u1, float* U, float* T, bool weighted, hipStream_t st) {
-    c->kmax_valid = 0;
-    if (c->d_kmax)
-        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)c->gm * c->K, st), "fill launch");
     for (int g = 0; g < c->gm; ++g) {
         const float* ug = g == 0 ? u1 : U + c->goff[g];
         const bool last = g + 1 >= c->gm;
@@ -530,7 +517,6 @@ int down_sweep(mgbf_ctx* c, const float* u1, float* U, float* T, bool weighted, 
         const float wc = weighted ? c->wconst[g] : 1.0f;
         if (filt && !last && c->fuse >= 3 && !c->aniso[g] && fused_ok(c, g, c->tile_tj[g][3], c->s.H)) {
             CALL(radial(c, g, ug, T + c->goff[g], wf, wc, st, MGBF_T_HFILTT, nullptr, -1, next));
-            if (c->d_kmax) c->kmax_valid |= 1u << (g + 1);
             continue;
         }
         if (!last) CALL(restrict_g(c, g, ug, next, st));
@@ -617,7 +603,6 @@ int apply_wt(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream
 // y = W v: the control vector's generations are filtered once and summed up
 // the hierarchy, then Vf.
 int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_t st) {
-    c->kmax_valid = 0;
     return upsweep(c, v, v, v, y, c->nfilt ? BAND_VF : -1, sqrt_weights, st);
 }
 
@@ -872,18 +857,9 @@ int rad3_down(mgbf_ctx* c, const float* x, float* U, float* T, int g, bool* scal
     return ball(c, g, ug, T + c->goff[g], true, st, scaled, nullptr, res ? next : nullptr);
 }
 
-// The per-k maxima the restriction passes record start from zero on every call.
-int reset_kmax(mgbf_ctx* c, hipStream_t st) {
-    c->kmax_valid = 0;
-    if (c->d_kmax && c->gm > 1)
-        HIP_TRY(mgbf::launch_fill(c->d_kmax, 0.f, (long long)(c->gm - 1) * c->K, st), "fill launch");
-    return MGBF_OK;
-}
-
 int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
     float* U = c->ws_u;
     float* T = c->ws_t;
-    CALL(reset_kmax(c, st));
     const bool pro = ball3_pro(c);
     for (int g = 0; g < c->gm; ++g) {
         bool scaled = false;
@@ -899,7 +875,6 @@ int apply_wawt_rad3(mgbf_ctx* c, const float* x, float* y, hipStream_t st) {
 }
 
 int apply_wt_rad3(mgbf_ctx* c, const float* x, float* v, bool sqrt_weights, hipStream_t st) {
-    CALL(reset_kmax(c, st));
     for (int g = 0; g < c->gm; ++g) {
         // the transposed pass writes v_g; v + goff stands in for T
         CALL(rad3_down(c, x, c->ws_u, v, g, nullptr, st));
@@ -1014,10 +989,10 @@ int apply_weights(mgbf_ctx* c, const float* v, float* out, bool use_sqrt, hipStr
 }
 
 // Floats of each workspace (multigrid U and T, the analysis-grid
-// intermediate, the per-k maxima), each rounded up to 256 bytes.
+// intermediate), each rounded up to 256 bytes.
 struct WsSizes {
-    size_t u = 0, t = 0, a = 0, k = 0;
-    size_t total() const { return u + t + a + k; }
+    size_t u = 0, t = 0, a = 0;
+    size_t total() const { return u + t + a; }
 };
 
 WsSizes ws_sizes(const mgbf_ctx* c) {
@@ -1026,7 +1001,6 @@ WsSizes ws_sizes(const mgbf_ctx* c) {
     w.u = r((size_t)c->s.dims.mg_size);
     w.t = w.u;
     w.a = c->s.dims.km_a_all != c->K ? r((size_t)c->K * c->s.dims.na * c->s.dims.ma) : 0;
-    w.k = r((size_t)c->gm * c->K);
     return w;
 }
 
@@ -1478,7 +1452,6 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         return fail(MGBF_ERR_HIP, "internal: coefficient tables exceed their allocation");
     }
     e = hipMemcpy(c->d_tables, host.data(), bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&c->d_kmax, sizeof(float) * (size_t)c->gm * c->K);
     if (e != hipSuccess) {
         int r = hip_fail(e, "device allocation");
         mgbf_finalize(c);
@@ -1507,7 +1480,6 @@ int mgbf_finalize(mgbf_ctx* c) {
     if (c->ws_u) (void)hipFree(c->ws_u);
     if (c->ws_t) (void)hipFree(c->ws_t);
     if (c->d_tables) (void)hipFree(c->d_tables);
-    if (c->d_kmax) (void)hipFree(c->d_kmax);
     if (c->ws_a) (void)hipFree(c->ws_a);
     for (int g = 0; g < mgbf::kMaxGen; ++g)
         if (c->d_aw[g]) (void)hipFree(c->d_aw[g]);
@@ -1570,8 +1542,8 @@ int mgbf_apply(mgbf_ctx* c, int op, const float* in, float* out, void* stream) {
 
 }  // extern "C"
 
-// The workspaces of one call in caller-given device memory: U, T, the
-// analysis-grid intermediate and the per-k maxima as slices of `base`
+// The workspaces of one call in caller-given device memory: U, T and the
+// analysis-grid intermediate as slices of `base`
 // (every one 256-byte aligned). Nothing is allocated and no timer event is
 // recorded; the context's own workspaces are untouched.
 int mgbf::apply_in_scratch(mgbf_ctx* c, int op, const float* in, float* out, void* stream,
@@ -1589,20 +1561,16 @@ int mgbf::apply_in_scratch(mgbf_ctx* c, int op, const float* in, float* out, voi
     float* const keep_u = c->ws_u;
     float* const keep_t = c->ws_t;
     float* const keep_a = c->ws_a;
-    float* const keep_k = c->d_kmax;
     const bool keep_timing = c->timing;
     c->ws_u = base;
     c->ws_t = base + w.u;
     c->ws_a = w.a ? base + w.u + w.t : nullptr;
-    c->d_kmax = base + w.u + w.t + w.a;
     c->timing = false;
     const int status = apply_op(c, op, in, out, static_cast<hipStream_t>(stream));
     c->ws_u = keep_u;
     c->ws_t = keep_t;
     c->ws_a = keep_a;
-    c->d_kmax = keep_k;
     c->timing = keep_timing;
-    c->kmax_valid = 0;   // the maxima of this call are in the scratch, not in d_kmax
     return status;
 }
 
@@ -1616,9 +1584,6 @@ int mgbf_apply_stage(mgbf_ctx* c, int stage, int gen, const float* in, float* ou
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int g = gen - 1;
     if (g < 0 || g >= c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
-    // a single stage reads a caller field whose per-k maxima are not
-    // recorded: the matrix-core engine (which scales by them) stays off
-    c->kmax_valid = 0;
     switch (stage) {
         case MGBF_STAGE_VF:
         case MGBF_STAGE_VFT:
@@ -1675,7 +1640,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     // MGBF_AUTOTUNE_LOG=1 prints every candidate's time (3 passes) to stderr
     const char* logv = std::getenv("MGBF_AUTOTUNE_LOG");
     const bool log = logv && logv[0] == '1';
-    // the restriction chain records the per-k maxima the matrix-core engine needs
+    // the restriction chain fills the coarse generations the candidates read
     if (status == MGBF_OK) status = restrict_chain(c, c->ws_u, c->ws_u, st);
     // kind 3 writes its restriction over the (zero) next generation of U,
     // which the restriction of zeros leaves unchanged

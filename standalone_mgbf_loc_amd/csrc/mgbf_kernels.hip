@@ -198,7 +198,7 @@ __device__ void restrict_edge_tile(const float* __restrict__ fine, float* __rest
                                    int kk, bool kvalid, int K, int nf, int mf, int I0, int J0,
                                    int J1, const int* __restrict__ rxs,
                                    const float* __restrict__ rxw, const int* __restrict__ rys,
-                                   const float* __restrict__ ryw, float& lmax) {
+                                   const float* __restrict__ ryw) {
     constexpr int WR = kRWidthDev;
     const int nc = nf / 2;
     const size_t frow = (size_t)nf * K;
@@ -218,7 +218,6 @@ __device__ void restrict_edge_tile(const float* __restrict__ fine, float* __rest
                     const int ii = xs + tx < nf ? xs + tx : nf - 1;
                     const float wx = xs + tx < nf ? rxw[I * WR + tx] : 0.f;
                     const float fv = row[(size_t)ii * K];
-                    lmax = fmaxf(lmax, fabsf(fv));
                     a = __builtin_fmaf(wx, fv, a);
                 }
                 acc = __builtin_fmaf(wy, a, acc);
@@ -232,8 +231,8 @@ template <int TI2, int TJ2, bool Q>
 __global__ void __launch_bounds__(kBlock)
 restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int K, int nf,
                  int mf, const int* __restrict__ rxs, const float* __restrict__ rxw,
-                 const int* __restrict__ rys, const float* __restrict__ ryw, float* kmax,
-                 int nkc, int ntile_i, int nstrip, int nitems) {
+                 const int* __restrict__ rys, const float* __restrict__ ryw, int nkc, int ntile_i,
+                 int nstrip, int nitems) {
     constexpr int WR = kRWidthDev;
     using S = Stencil<Q>;
     constexpr int NRW = S::NR, E = S::EDGE;
@@ -255,11 +254,8 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
     const int nc = nf / 2, mc = mf / 2;
     const int I0 = ti * TI2, J0 = js * TJ2;
     const int J1 = J0 + TJ2 < mc ? J0 + TJ2 : mc;
-    float lmax = 0.f;   // max |fine| read by this lane (its k)
     if (!(I0 >= E && I0 + TI2 <= nc - E)) {
-        restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw,
-                                lmax);
-        if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));
+        restrict_edge_tile<TI2>(fine, coarse, kk, kvalid, K, nf, mf, I0, J0, J1, rxs, rxw, rys, ryw);
         return;
     }
     const size_t frow = (size_t)nf * K;
@@ -271,10 +267,7 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
         const __amdgpu_buffer_rsrc_t rs = buf_rsrc(base + (size_t)fr * frow);
         float v[NCF];
 #pragma unroll
-        for (int c = 0; c < NCF; ++c) {
-            v[c] = buf_ld(rs, vk, c * K * 4);
-            lmax = fmaxf(lmax, fabsf(v[c]));
-        }
+        for (int c = 0; c < NCF; ++c) v[c] = buf_ld(rs, vk, c * K * 4);
 #pragma unroll
         for (int c = 0; c < TI2; ++c) {
             float a = 0.f;
@@ -329,7 +322,6 @@ restrict3_kernel(const float* __restrict__ fine, float* __restrict__ coarse, int
             for (int c = 0; c < TI2; ++c) buf_st(acc[c], ro, vk, c * K * 4);
         }
     }
-    if (kmax && kvalid) atomicMax(reinterpret_cast<unsigned int*>(kmax) + k, __float_as_uint(lmax));
 }
 
 // out = in * (wfield ? f(wfield) : f(wconst)), f = identity or sqrt
@@ -428,11 +420,11 @@ hipError_t launch_restrict(const TransferArgs& a, hipStream_t st) {
     const unsigned nblocks = (unsigned)((nitems + kWavesPerBlock - 1) / kWavesPerBlock);
     if (a.quad)
         hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2, true>), dim3(nblocks), dim3(kBlock), 0, st,
-                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc,
+                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nkc,
                            ntile_i, nstrip, (int)nitems);
     else
         hipLaunchKernelGGL((restrict3_kernel<TI2, TJ2, false>), dim3(nblocks), dim3(kBlock), 0, st,
-                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, a.kmax, nkc,
+                           a.src, a.dst, a.K, a.nf, a.mf, a.xs, a.xw, a.ys, a.yw, nkc,
                            ntile_i, nstrip, (int)nitems);
     return hipGetLastError();
 }

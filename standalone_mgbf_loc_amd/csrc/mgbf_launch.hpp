@@ -58,7 +58,6 @@ struct RadialArgs {
     float* rout;            // coarse output, or nullptr
     const float4* rytab;    // per fine row: weights into the 6 open coarse rows (2 x float4)
     const int2* rymeta;     // per fine row: lowest open coarse row, rows completed
-    float* rkmax;           // per-k max |coarse output|, or nullptr
     bool exact_disc;
     bool separable;       // coef.w[0][a] = fx(a), coef.w[1][b] = fy(b): w(a,b) = fx(a) fy(b)
     bool quad;            // the prolongation epilogue uses the quadratic transfer (lquart)
@@ -80,7 +79,6 @@ struct RadialParams {
     float* rout;              // coarse output (n/2 x m/2 x K), or nullptr
     const float4* rytab;      // per fine row: weights into the 6 open coarse rows (2 x float4)
     const int2* rymeta;       // per fine row: lowest open coarse row, rows completed
-    float* rkmax;             // per-k max |coarse output| accumulated here, or nullptr
     RadialCoef W;
 };
 
@@ -111,7 +109,6 @@ MGBF_DECLARE_RADIAL(9) MGBF_DECLARE_RADIAL(10)
 struct TransferArgs {
     const float* src;
     float* dst;
-    float* kmax;          // per-k max |src| accumulated here, or nullptr
     int K, nf, mf;        // fine generation size
     const int* xs;        // window starts along x
     const float* xw;      // window weights along x

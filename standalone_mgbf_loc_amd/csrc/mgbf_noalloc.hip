@@ -1,5 +1,5 @@
 // mgbf_apply_noalloc: mgbf_apply whose workspaces (the multigrid U and T,
-// the analysis-grid intermediate of the transfers, the per-k maxima) live in
+// the analysis-grid intermediate of the transfers) live in
 // a static device array instead of memory the context allocates, so that a
 // call allocates nothing. No counterpart in the reference.
 #include <hip/hip_runtime.h>

@@ -163,7 +163,6 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     constexpr int NRC = RES ? TI / 2 : 1;   // coarse columns of the wave
     constexpr int NQ = 6;                    // open coarse rows at most
     float racc[NQ][NRC];
-    float rmax = 0.f;
     int xcls = 0;                            // 0 interior, 1 left edge, 2 right edge
     const int Jlo = j0 / 2, Jhi = (j0 + tj_eff) / 2, nc2 = n / 2;
     if constexpr (RES) {
@@ -245,10 +244,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                                         buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
 #pragma unroll
                                     for (int c = 0; c < NRC; ++c) {
-                                        if (i0 / 2 + c < nc2) {
-                                            buf_st(racc[0][c], ro, vk, c * K * 4);
-                                            rmax = fmaxf(rmax, fabsf(racc[0][c]));
-                                        }
+                                        if (i0 / 2 + c < nc2) buf_st(racc[0][c], ro, vk, c * K * 4);
                                     }
                                 }
                                 // slots NO .. 5 stay empty between the edges
@@ -279,10 +275,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
                                         buf_rsrc(p.rout + (size_t)K * ((size_t)(i0 / 2) + (size_t)nc2 * J));
 #pragma unroll
                                     for (int c = 0; c < NRC; ++c) {
-                                        if (i0 / 2 + c < nc2) {
-                                            buf_st(racc[e][c], ro, vk, c * K * 4);
-                                            rmax = fmaxf(rmax, fabsf(racc[e][c]));
-                                        }
+                                        if (i0 / 2 + c < nc2) buf_st(racc[e][c], ro, vk, c * K * 4);
                                     }
                                 }
                             }
@@ -416,10 +409,6 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
             }
         }
     }
-    if constexpr (RES) {
-        if (p.rkmax && kvalid)
-            atomicMax(reinterpret_cast<unsigned int*>(p.rkmax) + k, __float_as_uint(rmax));
-    }
 }
 
 template <int H, int TI, bool DISC, bool PRO, int VHB, bool RES = false, bool SEP = false, bool QP = false>
@@ -442,7 +431,6 @@ hipError_t launch_radial_variant(const RadialArgs& a, hipStream_t st) {
     p.rout = a.rout;
     p.rytab = a.rytab;
     p.rymeta = a.rymeta;
-    p.rkmax = a.rkmax;
     p.ntile_i = (a.n + TI - 1) / TI;
     p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
     const int nkc = (a.K - a.klo + kWave - 1) / kWave;
