@@ -459,6 +459,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phases = op.read_timers()
+    g1 = op.read_timers_gen(1)
     op.timers(False)
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = units / (ms_step / 1e3)
@@ -556,7 +557,17 @@ def main():
                      "kernel": "radial filter passes (F and F^T on every generation; matrix-core "
                                "engine radial_mx_kernel where the tiles say 16, radial_kernel elsewhere)",
                      "launches": r_n, "avg_launch_ms": round(r_ms / max(r_n, 1), 4),
-                     "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1)},
+                     "algorithmic_bytes_per_launch": r_bytes / max(r_n, 1),
+                     # generation 1's two radial passes on their own (the same
+                     # event timers, launches on generation 1 only)
+                     "gen1_passes": {
+                         name: {"avg_launch_ms": round(g1[ph][0] / max(g1[ph][1], 1), 4),
+                                "algorithmic_bytes_per_launch": g1[ph][2] / max(g1[ph][1], 1),
+                                "achieved": round(g1[ph][2] / (g1[ph][0] / 1e3) / 1e9, 1) if g1[ph][0] else None,
+                                "frac": round(g1[ph][2] / (g1[ph][0] / 1e3) / 1e9 / HBM_PEAK_GBPS, 4)
+                                if g1[ph][0] else None}
+                         for name, ph in (("first (F^T, A, restriction)", "hfiltT"),
+                                          ("second (F, prolongation, vertical band)", "hfilt"))}},
         "cpu_baseline": cpu,
         "max_rel_err_vs_fp64": relerr,
         "plain": plain,

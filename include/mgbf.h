@@ -238,6 +238,10 @@ int mgbf_timers_enable(mgbf_ctx* ctx, int enable);
    milliseconds, number of launches and algorithmic bytes (arrays of
    MGBF_NTIMERS entries; any may be NULL). */
 int mgbf_timers_read(mgbf_ctx* ctx, double* ms, long long* launches, double* bytes);
+/* The same accumulators restricted to the launches that act on generation
+   gen (1..gm): a phase's own time, launches and bytes per generation. No
+   counterpart in the reference (mg_timers sums a phase over generations). */
+int mgbf_timers_read_gen(mgbf_ctx* ctx, int gen, double* ms, long long* launches, double* bytes);
 
 /* Times the radial filter on this context's shapes for every candidate tile
    (output columns per lane x output rows per wavefront), for each generation

@@ -153,6 +153,7 @@ struct mgbf_ctx {
     // per-phase timers
     struct Rec {
         int phase;
+        int gen;   // generation (0-based) of a per-generation launch, or -1
         hipEvent_t a, b;
         double bytes;
     };
@@ -162,6 +163,10 @@ struct mgbf_ctx {
     double t_ms[MGBF_NTIMERS] = {};
     double t_bytes[MGBF_NTIMERS] = {};
     long long t_n[MGBF_NTIMERS] = {};
+    // the same, split by generation
+    double tg_ms[mgbf::kMaxGen][MGBF_NTIMERS] = {};
+    double tg_bytes[mgbf::kMaxGen][MGBF_NTIMERS] = {};
+    long long tg_n[mgbf::kMaxGen][MGBF_NTIMERS] = {};
 };
 
 namespace {
@@ -183,9 +188,10 @@ struct Phase {
     int ph;
     hipStream_t st;
     double bytes;
+    int gen;
     hipEvent_t a = nullptr;
-    Phase(mgbf_ctx* c_, int ph_, hipStream_t st_, double bytes_)
-        : c(c_), ph(ph_), st(st_), bytes(bytes_) {
+    Phase(mgbf_ctx* c_, int ph_, hipStream_t st_, double bytes_, int gen_ = -1)
+        : c(c_), ph(ph_), st(st_), bytes(bytes_), gen(gen_) {
         if (c->timing) {
             a = take_event(c);
             if (a && hipEventRecord(a, st) != hipSuccess) {
@@ -198,7 +204,7 @@ struct Phase {
         if (!a) return;
         hipEvent_t b = take_event(c);
         if (b && hipEventRecord(b, st) == hipSuccess) {
-            c->recs.push_back({ph, a, b, bytes});
+            c->recs.push_back({ph, gen, a, b, bytes});
         } else {
             c->pool.push_back(a);
             if (b) c->pool.push_back(b);
@@ -215,6 +221,11 @@ int drain_timers(mgbf_ctx* c) {
         c->t_ms[r.phase] += ms;
         c->t_bytes[r.phase] += r.bytes;
         c->t_n[r.phase] += 1;
+        if (r.gen >= 0 && r.gen < mgbf::kMaxGen) {
+            c->tg_ms[r.gen][r.phase] += ms;
+            c->tg_bytes[r.gen][r.phase] += r.bytes;
+            c->tg_n[r.gen][r.phase] += 1;
+        }
         c->pool.push_back(r.a);
         c->pool.push_back(r.b);
     }
@@ -397,7 +408,8 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
            int band = -1, float* rout = nullptr, bool transpose = false) {
     Phase ph(c, phase, st,
              4.0 * ((double)c->gsize[g] * (wfield ? 3 : 2) +
-                    (coarse || rout ? (double)c->gsize[g + 1] : 0.0)));
+                    (coarse || rout ? (double)c->gsize[g + 1] : 0.0)),
+             g);
     if (c->aniso[g]) {
         // per-point aspect tensors: F or F^T on its own, then the weighting;
         // no epilogue rides in it
@@ -450,7 +462,7 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
 }
 
 int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t st) {
-    Phase ph(c, MGBF_T_UPSEND, st, 4.0 * (double)(c->gsize[g] + c->gsize[g + 1]));
+    Phase ph(c, MGBF_T_UPSEND, st, 4.0 * (double)(c->gsize[g] + c->gsize[g + 1]), g);
     mgbf::TransferArgs a{fine, coarse, c->K, c->nx[g], c->ny[g],
                          c->d_rxs[g], c->d_rxw[g], c->d_rys[g], c->d_ryw[g], c->quad};
     HIP_TRY(mgbf::launch_restrict(a, st), "restriction launch");
@@ -460,7 +472,7 @@ int restrict_g(mgbf_ctx* c, int g, const float* fine, float* coarse, hipStream_t
 int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, float* fine,
               hipStream_t st) {
     Phase ph(c, MGBF_T_DNSEND, st,
-             4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)));
+             4.0 * (double)(c->gsize[g + 1] + c->gsize[g] * (addend ? 2 : 1)), g);
     mgbf::TransferArgs a{coarse, fine, c->K, c->nx[g], c->ny[g],
                          c->d_pxs[g], c->d_pxw[g], c->d_pys[g], c->d_pyw[g], c->quad};
     HIP_TRY(mgbf::launch_prolong(a, addend, st), "prolongation launch");
@@ -468,7 +480,7 @@ int prolong_g(mgbf_ctx* c, int g, const float* coarse, const float* addend, floa
 }
 
 int vertical(mgbf_ctx* c, const float* in, float* out, int which, hipStream_t st) {
-    Phase ph(c, which == BAND_VFT ? MGBF_T_VFILTT : MGBF_T_VFILT, st, 8.0 * (double)c->gsize[0]);
+    Phase ph(c, which == BAND_VFT ? MGBF_T_VFILTT : MGBF_T_VFILT, st, 8.0 * (double)c->gsize[0], 0);
     if (c->nfilt == 0) {
         if (in != out) HIP_TRY(mgbf::launch_copy(in, out, c->gsize[0], st), "copy launch");
         return MGBF_OK;
@@ -727,7 +739,7 @@ int rad2d_tj(const mgbf_ctx* c, int g) {
 int ball(mgbf_ctx* c, int g, const float* in, float* out, bool transpose, hipStream_t st,
          bool* scaled = nullptr, const float* coarse = nullptr, float* rout = nullptr) {
     Phase ph(c, transpose ? MGBF_T_HFILTT : MGBF_T_HFILT, st,
-             8.0 * (double)c->gsize[g] + (coarse || rout ? 4.0 * (double)c->gsize[g + 1] : 0.0));
+             8.0 * (double)c->gsize[g] + (coarse || rout ? 4.0 * (double)c->gsize[g + 1] : 0.0), g);
     const int K3 = c->s.dims.km3_all * c->s.cfg.lm;
     if (scaled) *scaled = false;
     if (coarse && (transpose || !c->ball3))
@@ -1887,6 +1899,11 @@ int mgbf_timers_enable(mgbf_ctx* c, int enable) {
         c->t_ms[i] = 0;
         c->t_bytes[i] = 0;
         c->t_n[i] = 0;
+        for (int g = 0; g < mgbf::kMaxGen; ++g) {
+            c->tg_ms[g][i] = 0;
+            c->tg_bytes[g][i] = 0;
+            c->tg_n[g][i] = 0;
+        }
     }
     return st;
 }
@@ -1899,6 +1916,19 @@ int mgbf_timers_read(mgbf_ctx* c, double* ms, long long* launches, double* bytes
         if (ms) ms[i] = c->t_ms[i];
         if (launches) launches[i] = c->t_n[i];
         if (bytes) bytes[i] = c->t_bytes[i];
+    }
+    return st;
+}
+
+int mgbf_timers_read_gen(mgbf_ctx* c, int gen, double* ms, long long* launches, double* bytes) {
+    if (!c) return fail(MGBF_ERR_ARG, "ctx is NULL");
+    if (gen < 1 || gen > c->gm) return fail(MGBF_ERR_ARG, "generation out of range");
+    DeviceGuard guard(c->device);
+    int st = drain_timers(c);
+    for (int i = 0; i < MGBF_NTIMERS; ++i) {
+        if (ms) ms[i] = c->tg_ms[gen - 1][i];
+        if (launches) launches[i] = c->tg_n[gen - 1][i];
+        if (bytes) bytes[i] = c->tg_bytes[gen - 1][i];
     }
     return st;
 }

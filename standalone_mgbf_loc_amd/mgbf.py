@@ -32,7 +32,7 @@ EXPORTED_SYMBOLS = (
     "mgbf_set_weights", "mgbf_apply", "mgbf_apply_stage", "mgbf_dot", "mgbf_last_error",
     "mgbf_version", "mgbf_timers_enable", "mgbf_timers_read", "mgbf_autotune", "mgbf_get_tile",
     "mgbf_set_tile", "mgbf_group_create", "mgbf_group_apply", "mgbf_group_destroy",
-    "mgbf_apply_noalloc", "mgbf_set_aspect",
+    "mgbf_apply_noalloc", "mgbf_set_aspect", "mgbf_timers_read_gen",
 )
 # floats of the static device scratch of mgbf_apply_noalloc (MGBF_NOALLOC_SCRATCH_FLOATS)
 NOALLOC_SCRATCH_FLOATS = 32 << 20
@@ -191,6 +191,9 @@ def load_library(path: Optional[str] = None):
     L.mgbf_timers_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
     L.mgbf_timers_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
                                    ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
+    if hasattr(L, "mgbf_timers_read_gen"):
+        L.mgbf_timers_read_gen.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double)]
     L.mgbf_autotune.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.mgbf_get_tile.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -491,6 +494,15 @@ class MGBF:
         cnt = (ctypes.c_longlong * n)()
         by = (ctypes.c_double * n)()
         _check(self._L.mgbf_timers_read(self._h, ms, cnt, by))
+        return {name: (ms[i], cnt[i], by[i]) for i, name in enumerate(TIMER_NAMES)}
+
+    def read_timers_gen(self, gen: int) -> dict:
+        """read_timers() restricted to the launches on generation `gen` (1..gm)."""
+        n = len(TIMER_NAMES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        by = (ctypes.c_double * n)()
+        _check(self._L.mgbf_timers_read_gen(self._h, int(gen), ms, cnt, by))
         return {name: (ms[i], cnt[i], by[i]) for i, name in enumerate(TIMER_NAMES)}
 
     def dot(self, a, b, stream=None) -> float:
