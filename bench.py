@@ -315,7 +315,7 @@ def run_row(args, torch, MG):
         w2 = torch.empty_like(w)
         units = d.na * d.ma * d.km_a_all
         # algorithmic bytes of the pair: each map reads its input and writes
-        # its output once (the vertical intermediate is not counted)
+        # its output once
         pair_bytes = 2 * 4.0 * (d.na * d.ma * d.km_a_all + d.nx * d.ny * d.km_all)
 
         def step():
@@ -348,8 +348,14 @@ def run_row(args, torch, MG):
     if row == "transfer":
         # events on the launch stream around the timed steps: nothing else runs there
         k_ms = e0.elapsed_time(e1)
-        k_n, k_bytes = 4 * args.steps, pair_bytes * args.steps
-        kernel = "xfer_tile_kernel + kwin_kernel (A2F and F2A, two launches each)"
+        if os.environ.get("MGBF_XFER_SPLIT") == "1":
+            per_step = 4
+            kernel = "xfer_tile_kernel + kwin_kernel (A2F and F2A, the vertical map a pass of its own)"
+        else:
+            per_step = 2
+            kernel = ("xfer_tile_kernel with the vertical map fused (A2F: the map first, F2A: the map "
+                      "last; one launch each)")
+        k_n, k_bytes = per_step * args.steps, pair_bytes * args.steps
         avg = k_ms / k_n
         achieved = k_bytes / (k_ms / 1e3) / 1e9
     else:

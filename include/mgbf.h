@@ -186,9 +186,9 @@ int mgbf_apply(mgbf_ctx* ctx, int op, const float* in, float* out, void* stream)
    context's workspaces (which mgbf_apply allocates on first use) it uses a
    static device array of MGBF_NOALLOC_SCRATCH_FLOATS floats (128 MiB), one
    per device, held from the library's load until the process ends. A
-   context whose workspace (2 multigrid vectors and, for the analysis
-   transfers with a vertical map, one analysis-grid field of K levels)
-   exceeds it is refused with MGBF_ERR_ARG. Every call on a device
+   context whose workspace (2 multigrid vectors and, for analysis transfers
+   whose vertical map runs as a pass of its own, one analysis-grid field of
+   K levels) exceeds it is refused with MGBF_ERR_ARG. Every call on a device
    shares that device's array and leaves its contents behind: calls that
    overlap on one device (any contexts, any streams) race. The timers record
    nothing during the call. Results equal mgbf_apply's bit for bit. */

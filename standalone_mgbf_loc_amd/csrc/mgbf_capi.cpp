@@ -65,6 +65,9 @@ struct VMap {
     const int* s = nullptr;
     const float* w = nullptr;
     int width = 0;
+    // the same windows per 64-level chunk of the filter stack, for the tile
+    // kernel that fuses the map (lo is null when they do not fit it)
+    mgbf::XferVArgs fz{};
 };
 
 struct mgbf_ctx {
@@ -141,6 +144,8 @@ struct mgbf_ctx {
     // the same transfers by 8 x 8 output tiles, when their blocks fit
     bool xt_ok[2] = {false, false};   // [0] direct (filter -> analysis), [1] adjoint
     bool xfer_gather = false;         // MGBF_XFER_GATHER=1: the per-point gather instead
+    bool xfer_split = false;          // MGBF_XFER_SPLIT=1: the vertical map as a pass of its own
+    bool need_ws_a = false;           // a vertical map runs as a pass of its own
     mgbf::XferTileArgs xt[2]{};
     // vertical transfer of the _all2 (kv) and the _all (kw) routines:
     // direct (d) and adjoint (a) window tables; null when lm_a = lm or the
@@ -624,6 +629,81 @@ int apply_w(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipStream_
 // 2-D variables in the plane), no separate vertical filter:
 //   y = sum_g P^(g-1) F_g A_g F_g^T R^(g-1) x.
 // ---------------------------------------------------------------------------
+// Chunk tables of a vertical map fused into the tile kernel (XferVArgs)
+// from its window table t over the stacked levels: the direct map
+// (analysis levels from filter levels) or its adjoint. False when a window
+// spans more than 4 levels, the direct windows leave a 64-level chunk of
+// the filter stack or do not advance chunk by chunk, or a chunk's windows
+// cover more than 128 analysis levels.
+struct VFuseHost {
+    std::vector<int> lo, cnt, r0;
+    std::vector<double> w;
+};
+
+bool build_vfuse(const mgbf::AxisTable& t, int K, bool adjoint, VFuseHost& f) {
+    constexpr int kChunk = 64;   // the lanes of a wave
+    const int nkc = (K + kChunk - 1) / kChunk;
+    if (t.n_out < 1 || (adjoint && t.n_out != K)) return false;
+    f.lo.assign(nkc, 0);
+    f.cnt.assign(nkc, 0);
+    f.r0.assign(t.n_out, 0);
+    f.w.assign((size_t)t.n_out * 4, 0.0);
+    // first and last level of output o's nonzero taps (-1: none)
+    auto span = [&](int o, int& first, int& last) {
+        first = last = -1;
+        for (int u = 0; u < t.width; ++u)
+            if (t.w[(size_t)o * t.width + u] != 0.0) {
+                if (first < 0) first = t.start[o] + u;
+                last = t.start[o] + u;
+            }
+    };
+    auto put = [&](int o, int first, int base) {
+        f.r0[o] = first < 0 ? 0 : first - base;
+        for (int u = 0; u < t.width; ++u) {
+            const double wv = t.w[(size_t)o * t.width + u];
+            if (wv != 0.0) f.w[(size_t)o * 4 + (t.start[o] + u - first)] = wv;
+        }
+    };
+    if (!adjoint) {
+        int cur = 0;
+        for (int o = 0; o < t.n_out; ++o) {
+            int first, last;
+            span(o, first, last);
+            if (first >= 0) {
+                const int ch = first / kChunk;
+                if (ch < cur || last / kChunk != ch || last - first >= 4 || last >= K) return false;
+                cur = ch;
+            }
+            if (f.cnt[cur] == 0) f.lo[cur] = o;
+            if (++f.cnt[cur] > 2 * kChunk) return false;
+            put(o, first, kChunk * cur);
+        }
+        return true;
+    }
+    for (int ch = 0; ch < nkc; ++ch) {
+        const int o0 = ch * kChunk, o1 = std::min(o0 + kChunk, K);
+        int lo = -1, hi = -1;
+        for (int o = o0; o < o1; ++o) {
+            int first, last;
+            span(o, first, last);
+            if (first < 0) continue;
+            if (last - first >= 4) return false;
+            lo = lo < 0 ? first : std::min(lo, first);
+            hi = std::max(hi, last);
+        }
+        if (hi < 0) continue;
+        if (hi - lo + 1 > 2 * kChunk) return false;
+        f.lo[ch] = lo;
+        f.cnt[ch] = hi - lo + 1;
+        for (int o = o0; o < o1; ++o) {
+            int first, last;
+            span(o, first, last);
+            put(o, first, lo);
+        }
+    }
+    return true;
+}
+
 // Dense per-tile blocks of a 1-D window table for the tiled transfer: tile
 // q covers outputs q*TB .. q*TB+TB-1 and reads inputs s0[q] .. s0[q]+S-1
 // with weights W[q][t][c]. S is the widest span over the tiles, rounded up
@@ -924,6 +1004,12 @@ int apply_w_rad3(mgbf_ctx* c, const float* v, float* y, bool sqrt_weights, hipSt
     return rad3_upsweep(c, U, y, st);
 }
 
+// The vertical map v rides in the tile kernel of direction e (0 direct,
+// 1 adjoint).
+bool fused_vmap(const mgbf_ctx* c, const VMap& v, int e) {
+    return v.fz.lo && c->xt_ok[e] && !c->xfer_gather && !c->xfer_split;
+}
+
 // filt_to_anal_all2 / filt_to_anal_all: analysis field = T_v T_h
 // (generation-1 field), the horizontal interpolation first, then lm -> lm_a
 // levels with the routine family's vertical map (vd: its direct table).
@@ -936,6 +1022,13 @@ int apply_f2a(mgbf_ctx* c, const VMap& vd, const char* refusal, const float* in,
         return MGBF_OK;
     }
     if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
+    if (vd.s && fused_vmap(c, vd, 0)) {
+        mgbf::XferTileArgs a = c->xt[0];
+        a.in = in;
+        a.out = out;
+        HIP_TRY(mgbf::launch_xfer_tile_v(a, vd.fz, false, st), "transfer launch");
+        return MGBF_OK;
+    }
     float* mid = vd.s ? c->ws_a : out;
     if (c->xt_ok[0] && !c->xfer_gather) {
         mgbf::XferTileArgs a = c->xt[0];
@@ -965,6 +1058,13 @@ int apply_a2f(mgbf_ctx* c, const VMap& va, const char* refusal, const float* in,
         return MGBF_OK;
     }
     if (in == out) return fail(MGBF_ERR_ARG, "the analysis transfer needs distinct in and out");
+    if (va.s && fused_vmap(c, va, 1)) {
+        mgbf::XferTileArgs a = c->xt[1];
+        a.in = in;
+        a.out = out;
+        HIP_TRY(mgbf::launch_xfer_tile_v(a, va.fz, true, st), "transfer launch");
+        return MGBF_OK;
+    }
     const float* src = in;
     if (va.s) {
         HIP_TRY(mgbf::launch_kwin(in, c->ws_a, d.km_a_all, c->K, (long long)d.na * d.ma, va.s, va.w, va.width,
@@ -1012,7 +1112,7 @@ WsSizes ws_sizes(const mgbf_ctx* c) {
     WsSizes w;
     w.u = r((size_t)c->s.dims.mg_size);
     w.t = w.u;
-    w.a = c->s.dims.km_a_all != c->K ? r((size_t)c->K * c->s.dims.na * c->s.dims.ma) : 0;
+    w.a = c->need_ws_a ? r((size_t)c->K * c->s.dims.na * c->s.dims.ma) : 0;
     return w;
 }
 
@@ -1139,6 +1239,8 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     {
         const char* xg = std::getenv("MGBF_XFER_GATHER");
         c->xfer_gather = xg && xg[0] == '1';
+        const char* xs = std::getenv("MGBF_XFER_SPLIT");
+        c->xfer_split = xs && xs[0] == '1';
     }
     if (c->rad3) {
         // MGBF_BALL_GATHER=1 keeps the gather for the 3-D variables too
@@ -1342,6 +1444,20 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
                          carve_size(sizeof(int) * b.y0.size()) + carve_size(sizeof(float) * b.wy.size());
         }
     }
+    // vertical maps fused into the tile kernels: kv_d, kv_a, kw_d, kw_a
+    VFuseHost vfz[4];
+    bool vfz_ok[4] = {};
+    if (!s.anal_identity) {
+        const mgbf::AxisTable* vt[4] = {&s.kv_d, &s.kv_a, &s.kw_d, &s.kw_a};
+        for (int e = 0; e < 4; ++e) {
+            const bool adj = e & 1;
+            if (vt[e]->n_out == 0 || !tb[adj ? 1 : 0].ok) continue;
+            vfz_ok[e] = build_vfuse(*vt[e], c->K, adj, vfz[e]);
+            if (vfz_ok[e])
+                bytes += carve_size(sizeof(int) * vfz[e].lo.size()) + carve_size(sizeof(int) * vfz[e].cnt.size()) +
+                         carve_size(sizeof(int) * vfz[e].r0.size()) + carve_size(sizeof(float) * vfz[e].w.size());
+        }
+    }
     std::vector<char> host(bytes, 0);
     hipError_t e = hipMalloc(&c->d_tables, bytes);
     if (e != hipSuccess) {
@@ -1446,16 +1562,23 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
                                           put_i(tb[e].y0), put_f(tb[e].wy), tb[e].SY};
             c->xt_ok[e] = true;
         }
-        auto put_v = [&](const mgbf::AxisTable& t, VMap& v) {
+        auto put_v = [&](const mgbf::AxisTable& t, VMap& v, int e) {
             if (t.n_out == 0) return;
             v.s = put_i(t.start);
             v.w = put_f(t.w);
             v.width = t.width;
+            if (vfz_ok[e])
+                v.fz = mgbf::XferVArgs{put_i(vfz[e].lo), put_i(vfz[e].cnt), put_i(vfz[e].r0), put_f(vfz[e].w),
+                                       d.km_a_all};
         };
-        put_v(s.kv_d, c->kv_d);
-        put_v(s.kv_a, c->kv_a);
-        put_v(s.kw_d, c->kw_d);
-        put_v(s.kw_a, c->kw_a);
+        put_v(s.kv_d, c->kv_d, 0);
+        put_v(s.kv_a, c->kv_a, 1);
+        put_v(s.kw_d, c->kw_d, 2);
+        put_v(s.kw_a, c->kw_a, 3);
+        // the analysis-grid intermediate, for a vertical map on its own
+        const VMap* vm[4] = {&c->kv_d, &c->kv_a, &c->kw_d, &c->kw_a};
+        for (int e = 0; e < 4; ++e)
+            if (vm[e]->s && !fused_vmap(c, *vm[e], e & 1)) c->need_ws_a = true;
     }
     if (overflow) {
         // a table was not counted above: its device address lies past the

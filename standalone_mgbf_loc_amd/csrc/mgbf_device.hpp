@@ -42,6 +42,16 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base) {
                                              0x00020000);
 }
 
+// Makes a value loaded before a loop count as consumed here: the empty asm
+// reads and "rewrites" its registers, so the compiler waits for the load
+// once, at this point. A load still pending where a loop first uses it is
+// otherwise waited for inside the loop, with a count that also drains the
+// loop's own prefetches.
+template <class T>
+__device__ __forceinline__ void settle(T& x) {
+    asm("" : "+v"(x));
+}
+
 __device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }

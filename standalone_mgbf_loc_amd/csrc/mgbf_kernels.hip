@@ -521,54 +521,143 @@ __global__ void __launch_bounds__(kBlock) xfer2d_kernel(XferArgs a, long long ni
 // the tile streams through registers row by row; each row is interpolated in
 // x for the 8 output columns (dense tile weights, wave-uniform) and added
 // into the 8 output rows with their y weights.
-template <int SX>
-__global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, int ntx, int nty, long long nitems) {
+// MODE 1 (direct transfer with its vertical map): the 64 k are filter
+// levels; each finished output row goes through a per-wave LDS tile, and
+// each lane forms up to two analysis levels of every point from a window
+// of 4 of the chunk's levels. MODE 2 (adjoint transfer): every input point
+// brings the chunk's analysis levels (up to 128, two coalesced loads per
+// lane) through a per-wave LDS tile, and each lane maps its window of 4 to
+// its filter level before the horizontal pass.
+template <int SX, int MODE>
+__global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, XferVArgs v, int ntx, int nty,
+                                                         long long nitems) {
     constexpr int TB = 8;
-    const long long item = (long long)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock +
-                           __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int DP = kWave + 4;        // LDS pitch of an output point's levels (MODE 1)
+    constexpr int AP = 2 * kWave + 4;    // LDS pitch of an input point's levels (MODE 2)
+    constexpr int LDSW = MODE == 1 ? TB * DP : (MODE == 2 ? SX * AP : 1);
+    __shared__ float lds_v[kWavesPerBlock][LDSW];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long long item = (long long)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
     if (item >= nitems) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int p = (int)(item % ntx);
-    const long long rest = item / ntx;
-    const int q = (int)(rest % nty);
-    const int kc = (int)(rest / nty);
+    // With a vertical map the chunks of one tile are neighbours in the
+    // launch: their level runs on the analysis grid (lm_a levels a column)
+    // share cache lines at the chunk boundaries. The adjoint, which reads
+    // the analysis grid, takes the tiles in strips of kStrip columns from
+    // top to bottom: a row of tiles starts as the row above it finishes and
+    // finds the input rows they share still cached.
+    int p, q, kc;
+    if constexpr (MODE == 0) {
+        p = (int)(item % ntx);
+        const long long rest = item / ntx;
+        q = (int)(rest % nty);
+        kc = (int)(rest / nty);
+    } else if constexpr (MODE == 1) {
+        const int nkc = (a.K + kWave - 1) / kWave;
+        kc = (int)(item % nkc);
+        const long long rest = item / nkc;
+        p = (int)(rest % ntx);
+        q = (int)(rest / ntx);
+    } else {
+        constexpr int kStrip = 16;
+        const int nkc = (a.K + kWave - 1) / kWave;
+        kc = (int)(item % nkc);
+        const long long t = item / nkc;
+        const int nfull = ntx / kStrip;
+        const long long fullsz = (long long)kStrip * nty;
+        if (t < nfull * fullsz) {
+            const long long r = t % fullsz;
+            q = (int)(r / kStrip);
+            p = (int)(t / fullsz) * kStrip + (int)(r % kStrip);
+        } else {
+            const long long r = t - nfull * fullsz;
+            const int wl = ntx - nfull * kStrip;
+            q = (int)(r / wl);
+            p = nfull * kStrip + (int)(r % wl);
+        }
+    }
     const int k = kc * kWave + lane;
     const bool kvalid = k < a.K;
     const int vk = (kvalid ? k : a.K - 1) * 4;
     const int x0 = a.x0[p], y0 = a.y0[q];
     const float* WX = a.wx + (size_t)p * TB * SX;
     const float* WY = a.wy + (size_t)q * TB * a.SY;
+    float* const col = lds_v[wv];
+    // levels per input point: the filter stack, or (MODE 2) the analysis one
+    const int kin = MODE == 2 ? v.kv : a.K;
     int soff[SX];
 #pragma unroll
-    for (int c = 0; c < SX; ++c) soff[c] = (x0 + c) * a.K * 4;
+    for (int c = 0; c < SX; ++c) soff[c] = (x0 + c) * kin * 4;
+    // MODE 2: the chunk's analysis levels lo .. lo+cnt-1, loaded as lo + lane
+    // and lo + 64 + lane (clamped into the span, zero beyond it), and the
+    // lane's window r0 .. r0+3 of them
+    int voff[2] = {vk, vk};
+    bool vin[2] = {true, true};
+    int r0 = 0;
+    float vw[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (MODE == 2) {
+        const int lo = v.lo[kc], cnt = v.cnt[kc];
+        const int last = cnt > 0 ? cnt - 1 : 0;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = lane + h * kWave;
+            vin[h] = e < cnt;
+            voff[h] = (lo + (e < last ? e : last)) * 4;
+        }
+        const int kk = kvalid ? k : a.K - 1;
+        r0 = v.r0[kk];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vw[u] = v.w[(size_t)kk * 4 + u];
+        settle(r0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) settle(vw[u]);
+        // the pads past the last level: read by zero taps only
+        for (int e = lane; e < 4 * SX; e += kWave) col[(e >> 2) * AP + 2 * kWave + (e & 3)] = 0.f;
+    }
     float acc[TB][TB];
 #pragma unroll
     for (int u = 0; u < TB; ++u)
 #pragma unroll
         for (int t = 0; t < TB; ++t) acc[u][t] = 0.f;
-    const size_t rowstride = (size_t)a.nin_x * a.K;
+    const size_t rowstride = (size_t)a.nin_x * kin;
+    constexpr int NH = MODE == 2 ? 2 : 1;
     // the next input row is loaded while the current one is combined
-    float vn[SX];
-    {
-        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)y0 * rowstride);
+    float vn[SX][NH];
+    auto load_row = [&](int y) __attribute__((always_inline)) {
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)y * rowstride);
 #pragma unroll
-        for (int c = 0; c < SX; ++c) vn[c] = buf_ld(rs, vk, soff[c]);
-    }
+        for (int c = 0; c < SX; ++c)
+#pragma unroll
+            for (int h = 0; h < NH; ++h) vn[c][h] = buf_ld(rs, voff[h], soff[c]);
+    };
+    load_row(y0);
     for (int r = 0; r < a.SY; ++r) {
-        float v[SX];
+        float vc[SX];
+        if constexpr (MODE == 2) {
+            // the row's points through LDS, then each lane's window
 #pragma unroll
-        for (int c = 0; c < SX; ++c) v[c] = vn[c];
-        if (r + 1 < a.SY) {
-            const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.in + (size_t)(y0 + r + 1) * rowstride);
+            for (int c = 0; c < SX; ++c) {
+                col[c * AP + lane] = vin[0] ? vn[c][0] : 0.f;
+                col[c * AP + kWave + lane] = vin[1] ? vn[c][1] : 0.f;
+            }
+            if (r + 1 < a.SY) load_row(y0 + r + 1);
 #pragma unroll
-            for (int c = 0; c < SX; ++c) vn[c] = buf_ld(rs, vk, soff[c]);
+            for (int c = 0; c < SX; ++c) {
+                const float* cv = col + c * AP + r0;
+                vc[c] = __builtin_fmaf(vw[0], cv[0],
+                                       __builtin_fmaf(vw[1], cv[1], __builtin_fmaf(vw[2], cv[2], vw[3] * cv[3])));
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < SX; ++c) vc[c] = vn[c][0];
+            if (r + 1 < a.SY) load_row(y0 + r + 1);
         }
         float xr[TB];
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
             float z = 0.f;
 #pragma unroll
-            for (int c = 0; c < SX; ++c) z = __builtin_fmaf(WX[t * SX + c], v[c], z);
+            for (int c = 0; c < SX; ++c) z = __builtin_fmaf(WX[t * SX + c], vc[c], z);
             xr[t] = z;
         }
 #pragma unroll
@@ -577,6 +666,54 @@ __global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, int n
 #pragma unroll
             for (int t = 0; t < TB; ++t) acc[u][t] = __builtin_fmaf(wy, xr[t], acc[u][t]);
         }
+    }
+    if constexpr (MODE == 1) {
+        // every lane (also those past K, whose levels no window reads)
+        // writes its level of each point; a lane forms analysis levels
+        // lo + lane and lo + 64 + lane of the chunk
+        const int lo = v.lo[kc], cnt = v.cnt[kc];
+        int o0[2];
+        bool ov[2];
+        float ow[2][4];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int o = s * kWave + lane;
+            ov[s] = o < cnt;
+            // a lane past cnt reads output lo's entry and drops its result
+            const int ko = lo + (ov[s] ? o : 0);
+            o0[s] = v.r0[ko];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) ow[s][u] = v.w[(size_t)ko * 4 + u];
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            settle(o0[s]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) settle(ow[s][u]);
+        }
+        if (lane < 4 * TB) col[(lane >> 2) * DP + kWave + (lane & 3)] = 0.f;
+        const size_t orow = (size_t)a.nout_x * v.kv;
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            const int j = q * TB + u;
+            if (j >= a.nout_y) break;
+#pragma unroll
+            for (int t = 0; t < TB; ++t) col[t * DP + lane] = acc[u][t];
+            const __amdgpu_buffer_rsrc_t ro = buf_rsrc(a.out + (size_t)j * orow);
+#pragma unroll
+            for (int t = 0; t < TB; ++t) {
+                const int i = p * TB + t;
+                if (i >= a.nout_x) break;
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const float* cv = col + t * DP + o0[s];
+                    const float y = __builtin_fmaf(
+                        ow[s][0], cv[0], __builtin_fmaf(ow[s][1], cv[1], __builtin_fmaf(ow[s][2], cv[2], ow[s][3] * cv[3])));
+                    if (ov[s]) buf_st(y, ro, (lo + s * kWave + lane) * 4, i * v.kv * 4);
+                }
+            }
+        }
+        return;
     }
     if (!kvalid) return;
 #pragma unroll
@@ -784,7 +921,8 @@ hipError_t launch_xfer2d(const XferArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st) {
+template <int MODE>
+hipError_t launch_xfer_tile_mode(const XferTileArgs& a, const XferVArgs& v, hipStream_t st) {
     const int ntx = (a.nout_x + 7) / 8, nty = (a.nout_y + 7) / 8;
     const long long nkc = (a.K + kWave - 1) / kWave;
     const long long nitems = nkc * ntx * nty;
@@ -792,15 +930,27 @@ hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st) {
     if (nblocks >= (1ll << 31) || a.SY < 1 || a.SY > a.nin_y || a.SX > a.nin_x) return hipErrorInvalidValue;
     const dim3 g((unsigned)nblocks), b(kBlock);
     switch (a.SX) {
-        case 8: hipLaunchKernelGGL(xfer_tile_kernel<8>, g, b, 0, st, a, ntx, nty, nitems); break;
-        case 12: hipLaunchKernelGGL(xfer_tile_kernel<12>, g, b, 0, st, a, ntx, nty, nitems); break;
-        case 16: hipLaunchKernelGGL(xfer_tile_kernel<16>, g, b, 0, st, a, ntx, nty, nitems); break;
-        case 20: hipLaunchKernelGGL(xfer_tile_kernel<20>, g, b, 0, st, a, ntx, nty, nitems); break;
-        case 24: hipLaunchKernelGGL(xfer_tile_kernel<24>, g, b, 0, st, a, ntx, nty, nitems); break;
-        case 32: hipLaunchKernelGGL(xfer_tile_kernel<32>, g, b, 0, st, a, ntx, nty, nitems); break;
+        case 8: hipLaunchKernelGGL((xfer_tile_kernel<8, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
+        case 12: hipLaunchKernelGGL((xfer_tile_kernel<12, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
+        case 16: hipLaunchKernelGGL((xfer_tile_kernel<16, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
+        case 20: hipLaunchKernelGGL((xfer_tile_kernel<20, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
+        case 24: hipLaunchKernelGGL((xfer_tile_kernel<24, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
+        case 32: hipLaunchKernelGGL((xfer_tile_kernel<32, MODE>), g, b, 0, st, a, v, ntx, nty, nitems); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st) {
+    return launch_xfer_tile_mode<0>(a, XferVArgs{}, st);
+}
+
+hipError_t launch_xfer_tile_v(const XferTileArgs& a, const XferVArgs& v, bool adjoint, hipStream_t st) {
+    if (!v.lo || !v.cnt || !v.r0 || !v.w || v.kv < 1) return hipErrorInvalidValue;
+    // 32-bit offsets inside a row of analysis points
+    const long long npts = adjoint ? a.nin_x : a.nout_x;
+    if (npts * v.kv * 4 >= (1ll << 31)) return hipErrorInvalidValue;
+    return adjoint ? launch_xfer_tile_mode<2>(a, v, st) : launch_xfer_tile_mode<1>(a, v, st);
 }
 
 hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,

@@ -186,6 +186,24 @@ struct XferTileArgs {
     int SY;
 };
 hipError_t launch_xfer_tile(const XferTileArgs& a, hipStream_t st);
+// The vertical map of a transfer, fused into the tile kernel, in windows of
+// at most 4 consecutive levels per 64-level chunk of the filter stack (the
+// wave's lanes). Direct (filter -> analysis, after the horizontal pass):
+// chunk c writes the analysis levels lo[c] .. lo[c]+cnt[c]-1 (cnt <= 128),
+// output level o = sum_u w[4 o + u] h(64 c + r0[o] + u), h the chunk's
+// horizontally interpolated levels. Adjoint (analysis -> filter, before the
+// horizontal pass): chunk c reads the analysis levels lo[c] ..
+// lo[c]+cnt[c]-1 (cnt <= 128) of every input point, and filter level k
+// takes sum_u w[4 k + u] in(lo[c] + r0[k] + u). kv: the stacked levels of
+// the analysis side.
+struct XferVArgs {
+    const int* lo;
+    const int* cnt;
+    const int* r0;
+    const float* w;
+    int kv;
+};
+hipError_t launch_xfer_tile_v(const XferTileArgs& a, const XferVArgs& v, bool adjoint, hipStream_t st);
 // Window transfer along the stacked k of every point:
 // out(ko,p) = sum_u kw[ko][u] in(ks[ko]+u, p)
 hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,

@@ -28,6 +28,15 @@ def relerr(y, ref):
     return float(np.max(np.abs(y - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
+def assert_adjoint(op, a, w, v, b):
+    # <F2A v, w> = <v, A2F w> up to the FP32 rounding of a = F2A v and
+    # b = A2F w: elementwise errors of about 1e-7 relative, independent
+    # across points, move a dot product by about 1e-7 |a * w|_2
+    lhs, rhs = op.dot(a, w), op.dot(v, b)
+    scale = float((a.double() * w.double()).norm() + (v.double() * b.double()).norm())
+    assert abs(lhs - rhs) <= 1e-6 * scale, (lhs, rhs, scale)
+
+
 def setup(nx, ny, lm, n_ens, na, ma, lm_a, gm=3, h=4, **kw):
     cfg = M.Config.for_grid(nx, ny, lm, n_ens=n_ens, gm=gm, h=h, hz=2, weights=[1.0, 0.5, 0.25],
                             nm0=na, mm0=ma, lm_a=lm_a, **kw)
@@ -59,10 +68,7 @@ def test_transfer_matches_oracle_and_is_its_own_transpose(nx, ny, lm, n_ens, na,
         args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
         assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64))) <= TOL
         assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64))) <= TOL
-        wd = torch.from_numpy(wa).to(DEV)
-        lhs = op.dot(a, wd)
-        rhs = op.dot(torch.from_numpy(vf).to(DEV), b)
-        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs))
+        assert_adjoint(op, a, torch.from_numpy(wa).to(DEV), torch.from_numpy(vf).to(DEV), b)
     finally:
         op.close()
 
@@ -123,8 +129,7 @@ def test_every_transfer_variant_matches_oracle(scheme, family, grid):
         args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
         assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
         assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
-        lhs, rhs = op.dot(a, wd), op.dot(vd, b)
-        assert abs(lhs - rhs) <= 1e-6 * max(abs(lhs), abs(rhs))
+        assert_adjoint(op, a, wd, vd, b)
     finally:
         op.close()
 
@@ -185,6 +190,45 @@ def test_tiled_transfer_matches_the_gather(scheme, grid):
             assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), scheme=scheme)) <= TOL
             assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), scheme=scheme)) <= TOL
             out[gather] = (a.cpu(), b.cpu())
+        finally:
+            op.close()
+    assert relerr(out[False][0], out[True][0].double().numpy()) <= 1e-6
+    assert relerr(out[False][1], out[True][1].double().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("lm,n_ens,lm_a,family", [
+    (64, 2, 127, "all2"), (32, 3, 63, "all2"), (64, 2, 127, "all_lin"), (32, 2, 41, "all_lwq"),
+    (48, 2, 95, "all2"), (96, 1, 191, "all2")])
+def test_fused_vertical_map_matches_the_split_passes(lm, n_ens, lm_a, family):
+    # the vertical map inside the tile kernel (windows of <= 4 levels inside
+    # 64-level chunks of the filter stack: lm = 64 and 32, several chunks)
+    # against the map as a pass of its own (MGBF_XFER_SPLIT=1) and the
+    # oracle; lm = 48 and 96 have windows that cross a chunk and keep the
+    # split passes either way
+    f2a, a2f, vmap, flags = FAMILIES[family]
+    nx, ny, na, ma = 32, 24, 48, 36
+    print(f"fused vertical map lm={lm} n_ens={n_ens} lm_a={lm_a} family={family}", flush=True)
+    out = {}
+    for split in (False, True):
+        os.environ["MGBF_XFER_SPLIT"] = "1" if split else "0"
+        try:
+            cfg, op, P = setup(nx, ny, lm, n_ens, na, ma, lm_a, **flags)
+        finally:
+            os.environ.pop("MGBF_XFER_SPLIT", None)
+        try:
+            d = op.dims
+            rng = np.random.default_rng(lm + lm_a)
+            vf = rng.standard_normal(op.field_shape()).astype(np.float32)
+            wa = rng.standard_normal(d.anal_shape()).astype(np.float32)
+            vd, wd = torch.from_numpy(vf).to(DEV), torch.from_numpy(wa).to(DEV)
+            a = op.apply(f2a, vd)
+            b = op.apply(a2f, wd)
+            torch.cuda.synchronize()
+            args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
+            assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), vmap=vmap)) <= TOL
+            assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), vmap=vmap)) <= TOL
+            assert_adjoint(op, a, wd, vd, b)
+            out[split] = (a.cpu(), b.cpu())
         finally:
             op.close()
     assert relerr(out[False][0], out[True][0].double().numpy()) <= 1e-6
