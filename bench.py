@@ -95,14 +95,23 @@ def stream_copy_gbps(torch, nbytes=4 << 30, reps=5):
 #     variables x 64 levels (K = 512) on 1536x1536, 4 generations;
 #   transfer: anal_to_filt_all2 then filt_to_anal_all2 between the
 #     768x768x64 filter grid and a 1152x1152 analysis grid with 127 levels,
-#     16 members.
-ROWS = ("proc1", "proc2", "proc3", "proc4", "proc5", "proc7", "proc8", "lquart", "transfer", "group")
+#     16 members;
+#   group: 24 regional domains of uneven size in one hipGraph;
+#   single: one field 768x768x64, 5 generations, radial half-width 8;
+#   diff: the loss J = 1/2 |W A_w W^T (a x) - y|^2 and its gradients with
+#     respect to the generation weights and the amplitude field, 16 members
+#     x 512x512x64, 5 generations.
+ROWS = ("proc1", "proc2", "proc3", "proc4", "proc5", "proc7", "proc8", "lquart", "transfer", "group", "single",
+        "diff")
 ROW_NAMES = {"proc1": "filtering_rad2_bckg", "proc2": "filtering_lin2_bckg (mgbf_line)",
              "proc3": "filtering_fast_bckg", "proc4": "filtering_rad3", "proc5": "filtering_lin3 (mgbf_line)",
              "proc7": "filtering_lin2_ens (mgbf_line)", "proc8": "filtering_fast_ens",
              "lquart": "filtering_rad2_ens with the quadratic transfer (lquart)",
              "transfer": "anal_to_filt_all2 + filt_to_anal_all2 (4-point horizontal, 2:1 vertical)",
-             "group": "uneven batch of regional domains in one hipGraph"}
+             "group": "uneven batch of regional domains in one hipGraph",
+             "single": "one field (filtering_rad2_ens, one member)",
+             "diff": "differentiation: J = 1/2 |W A_w W^T (a x) - y|^2, its gradients w.r.t. the generation "
+                     "weights w and the amplitude field a"}
 
 
 def _oracle_params(O, cfg, d):
@@ -117,13 +126,24 @@ def _oracle_params(O, cfg, d):
         ampl03=cfg.mg_ampl03, lin3=cfg.mgbf_proc == 5, weights=cfg.weights()[: d.gm])
 
 
+def box3(t, passes):
+    """3x3 box smoothing of a (m, n, K) device field with the half-sample
+    mirror (edge replication for a 1-point pad), `passes` times."""
+    import torch
+    F = torch.nn.functional
+    v = t.permute(2, 0, 1).unsqueeze(0)
+    for _ in range(passes):
+        v = F.avg_pool2d(F.pad(v, (1, 1, 1, 1), mode="replicate"), 3, stride=1)
+    return v.squeeze(0).permute(1, 2, 0).contiguous()
+
+
 def row_cpu_baseline(MG, cfg, row):
     """The FP64 oracle on a bounded sample of a row's workload: one member (or
     one 3-D variable) of the same filter, on 768x768 (384x384 for the 3-D
     radial procedure), timed on this host's cores."""
     import numpy as np
     from oracle import oracle as O
-    n = 384 if row == "proc4" else 768
+    n = {"proc4": 384, "diff": 512}.get(row, 768)
     kw = {k: getattr(cfg, k) for k in ("mgbf_proc", "mgbf_line", "lquart", "mg_ampl01", "mg_ampl02",
                                        "mg_ampl03", "hx", "hy", "hz", "p", "gm_max")}
     if row == "transfer":
@@ -145,6 +165,21 @@ def row_cpu_baseline(MG, cfg, row):
         O.anal_to_filt(*args, w)
         units = d.na * d.ma * d.km_a_all
         what = f"one member: {n}x{n}x{cs.lm} <-> {d.na}x{d.ma}x{cs.lm_a}"
+    elif row == "diff":
+        # the loss and its gradients on one member: B z, B r (dJ/dz), and
+        # W^T z, W^T r for the weight gradients
+        rng = np.random.default_rng(2)
+        a = 1.0 + 0.2 * rng.standard_normal(x.shape)
+        yv = rng.standard_normal(x.shape)
+        z = a * x
+        r = O.wawt(P, z) - yv
+        gz = O.wawt(P, r)
+        vz, vr = O.mg_split(P, O.wt(P, z)), O.mg_split(P, O.wt(P, r))
+        grad_w = [float(np.sum(u * v)) for u, v in zip(vz, vr)]
+        grad_a = x * gz
+        del grad_w, grad_a
+        units = d.nx * d.ny * d.km_all
+        what = f"one member on {n}x{n}x{cs.lm}: loss and gradients"
     else:
         O.wawt(P, x)
         units = d.nx * d.ny * d.km_all
@@ -283,6 +318,18 @@ def run_row(args, torch, MG):
                                  nm0=na, mm0=na, lm_a=la)
         workload = (f"{ROW_NAMES[row]}: {M} members, filter grid {n}x{n}x{lm}, "
                     f"analysis grid {na}x{na}x{la}")
+    elif row == "single":
+        # one field 768x768x64, 5 generations, radial half-width 8, vertical
+        # 4, p = 2, weights 2^-g; seeded N(0,1) smoothed by a 3x3 box
+        M, n, lm = 1, 768, 64
+        cfg = MG.Config.for_grid(n, n, lm, gm=5, h=8, hz=4, p=2, weights=[2.0 ** -g for g in range(5)])
+        workload = f"{ROW_NAMES[row]}: {n}x{n}x{lm}, 5 generations, h=8, hz=4, p=2, weights 2^-g"
+    elif row == "diff":
+        # 16 members x 512x512x64, 5 generations, learnable weights
+        # [1, 1/2, 1/4, 1/8, 1/16], amplitude field a = 1 + 0.2 N(0,1) smoothed
+        M, n, lm = 16, 512, 64
+        cfg = MG.Config.for_grid(n, n, lm, n_ens=M, gm=5, h=6, hz=3, p=2, weights=weights)
+        workload = f"{ROW_NAMES[row]}: {M} members x {n}x{n}x{lm}, 5 generations, h=6, hz=3, p=2"
     elif row in ("proc7", "proc8", "lquart"):
         M, n, lm = 80, 768, 64
         kw = {"proc7": dict(mgbf_proc=7, mgbf_line=True), "proc8": dict(mgbf_proc=8),
@@ -309,6 +356,7 @@ def run_row(args, torch, MG):
     d = op.dims
     g = torch.Generator(device="cuda")
     g.manual_seed(7)
+    data = "synthetic: N(0,1), seed 7"
     if row == "transfer":
         w = torch.randn(d.anal_shape(), generator=g, device="cuda", dtype=torch.float32)
         v = torch.empty(op.field_shape(), dtype=torch.float32, device="cuda")
@@ -321,8 +369,24 @@ def run_row(args, torch, MG):
         def step():
             op.anal_to_filt(w, out=v)
             op.filt_to_anal(v, out=w2)
+    elif row == "diff":
+        from standalone_mgbf_loc_amd.autograd import loss_and_grads
+        x = torch.randn(op.field_shape(), generator=g, device="cuda", dtype=torch.float32)
+        yv = torch.randn(op.field_shape(), generator=g, device="cuda", dtype=torch.float32)
+        a = 1.0 + 0.2 * box3(torch.randn(op.field_shape(), generator=g, device="cuda"), 2)
+        ws = [torch.tensor(v, dtype=torch.float64, device="cuda") for v in weights]
+        units = d.nx * d.ny * d.km_all
+        tiles = pin_or_tune(args, op, log)
+        data = ("synthetic: x, y ~ N(0,1), a = 1 + 0.2 N(0,1) smoothed by two 3x3 box passes, seed 7; "
+                "weights [1, 1/2, 1/4, 1/8, 1/16]")
+
+        def step():
+            loss_and_grads(op, a, x, yv, ws)
     else:
         x = torch.randn(op.field_shape(), generator=g, device="cuda", dtype=torch.float32)
+        if row == "single":
+            x = box3(x, 1)
+            data = "synthetic: N(0,1) smoothed by a 3x3 box, seed 7"
         y = torch.empty_like(x)
         units = d.nx * d.ny * d.km_all
         tiles = pin_or_tune(args, op, log)
@@ -363,7 +427,8 @@ def run_row(args, torch, MG):
         k_n = phases["hfiltT"][1] + phases["hfilt"][1]
         k_bytes = phases["hfiltT"][2] + phases["hfilt"][2]
         kernel = ("ball3_kernel (3-D filter, F and F^T on every generation)" if row in ("proc4", "proc5")
-                  else "radial_kernel (F and F^T on every generation)")
+                  else "radial filter passes (F and F^T on every generation; radial_mx_kernel where "
+                       "the tiles say 16, radial_kernel elsewhere)")
         avg = k_ms / max(k_n, 1)
         achieved = k_bytes / (k_ms / 1e3) / 1e9
     phase_out = {k: {"ms": round(v_[0] / args.steps, 4), "launches": v_[1] // args.steps}
@@ -373,7 +438,7 @@ def run_row(args, torch, MG):
         "metric": METRIC, "row": row, "value": value, "unit": "gridpoint·members/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic: N(0,1), seed 7",
+        "data": data,
         "config": {"workload": workload},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": kernel,

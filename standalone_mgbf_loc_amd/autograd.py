@@ -21,10 +21,22 @@ import torch
 from . import mgbf as M
 
 
-def _set_weights(op: M.MGBF, weights):
-    for g, w in enumerate(weights, start=1):
-        if w.dim() == 0 or w.numel() == 1:
-            op.set_weights(g, value=float(w.detach().reshape(()).item()))
+def _scalar_values(weights):
+    """Host values of the scalar weights (None for fields), read from the
+    device in one transfer."""
+    idx = [i for i, w in enumerate(weights) if w.dim() == 0 or w.numel() == 1]
+    vals = [None] * len(weights)
+    if idx:
+        host = torch.stack([weights[i].detach().reshape(()).double() for i in idx]).cpu().tolist()
+        for i, v in zip(idx, host):
+            vals[i] = v
+    return vals
+
+
+def _set_weights(op: M.MGBF, weights, values):
+    for g, (w, v) in enumerate(zip(weights, values), start=1):
+        if v is not None:
+            op.set_weights(g, value=v)
         else:
             op.set_weights(g, field=w.detach().to(torch.float32).contiguous())
 
@@ -38,10 +50,12 @@ class _Localize(torch.autograd.Function):
                              "not symmetric, and the gradients here assume it is")
         if len(weights) != op.dims.gm:
             raise ValueError(f"need {op.dims.gm} generation weights")
-        _set_weights(op, weights)
+        values = _scalar_values(weights)
+        _set_weights(op, weights, values)
         zc = z.detach().contiguous()
         y = op.apply(M.OP_WAWT, zc)
         ctx.op = op
+        ctx.values = values
         ctx.save_for_backward(zc, *weights)
         return y
 
@@ -49,7 +63,7 @@ class _Localize(torch.autograd.Function):
     def backward(ctx, gy):
         op = ctx.op
         z, *weights = ctx.saved_tensors
-        _set_weights(op, weights)
+        _set_weights(op, weights, ctx.values)
         gy = gy.contiguous()
         gz = op.apply(M.OP_WAWT, gy) if ctx.needs_input_grad[0] else None
         gw = [None] * len(weights)
@@ -81,10 +95,13 @@ def localize(op: M.MGBF, z: torch.Tensor, weights) -> torch.Tensor:
 
 
 def loss_and_grads(op: M.MGBF, a, x, y, weights):
-    """J = 1/2 ||W A_w W^T (a * x) - y||^2 and its gradients w.r.t. a and w."""
+    """J = 1/2 ||W A_w W^T (a * x) - y||^2 and its gradients w.r.t. a and w.
+    J is accumulated in FP64 on the device (mgbf_dot); dJ/dr = r enters the
+    autograd graph of r = W A_w W^T (a * x) - y directly."""
     a = a.detach().requires_grad_(True)
     ws = [w.detach().requires_grad_(True) for w in weights]
     r = localize(op, a * x, ws) - y
-    J = 0.5 * (r.double() ** 2).sum()
-    J.backward()
-    return J.item(), a.grad, [w.grad for w in ws]
+    rd = r.detach().contiguous()
+    J = 0.5 * op.dot(rd, rd)
+    r.backward(rd)
+    return J, a.grad, [w.grad for w in ws]

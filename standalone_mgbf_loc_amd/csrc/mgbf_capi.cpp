@@ -1399,7 +1399,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     }
 
     // one device allocation for all coefficient tables
-    size_t bytes = carve_size(sizeof(double)) + carve_size(sizeof(_Float16) * bfrag.size());
+    size_t bytes = carve_size(sizeof(double) * (1 + mgbf::kDotBlocks)) + carve_size(sizeof(_Float16) * bfrag.size());
     for (int g = 0; g < mgbf::kMaxGen; ++g) bytes += carve_size(sizeof(_Float16) * rfrag[g].size());
     for (int b = 0; b < 3; ++b) bytes += carve_size(sizeof(float) * bands[b]->size());
     std::vector<float> rytab[mgbf::kMaxGen];
@@ -1505,7 +1505,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     };
     c->d_bfrag = put_h(bfrag);
     for (int g = 0; g < mgbf::kMaxGen; ++g) c->d_rfrag[g] = put_h(rfrag[g]);
-    take((double*)nullptr, 1, &c->d_dot);
+    take((double*)nullptr, 1 + mgbf::kDotBlocks, &c->d_dot);
     if (c->rad3) {
         auto put_taps = [&](size_t nt, auto tap, const int4** dt, const float** dw) {
             int4* d4 = nullptr;

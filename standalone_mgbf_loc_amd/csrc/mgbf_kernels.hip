@@ -343,25 +343,54 @@ copy_kernel(const float* __restrict__ in, float* __restrict__ out, long long cou
         out[e] = in[e];
 }
 
-// FP64-accumulated inner product; one double atomic per block into *acc.
-__global__ void __launch_bounds__(kBlock)
-dot_kernel(const float* __restrict__ a, const float* __restrict__ b, long long count,
-           double* acc) {
-    __shared__ double part[kBlock];
-    double s = 0.0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += stride)
-        s += (double)a[e] * (double)b[e];
-    part[threadIdx.x] = s;
+// FP64-accumulated inner product in two launches with a fixed order (so
+// the result does not depend on scheduling): each of kDotBlocks blocks sums
+// its grid-stride share into partial[block], 16-byte loads when both arrays
+// are 16-byte aligned; one block then sums the partials.
+__device__ __forceinline__ double block_sum(double s) {
+    __shared__ double ws[kWavesPerBlock];
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicAdd(acc, part[0]);
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kWavesPerBlock; ++w) t += ws[w];
+    return t;
 }
 
-__global__ void zero_double_kernel(double* p) { *p = 0.0; }
+template <bool VEC>
+__global__ void __launch_bounds__(kBlock)
+dot_partial_kernel(const float* __restrict__ a, const float* __restrict__ b, long long count,
+                   double* __restrict__ partial) {
+    double s = 0.0;
+    const long long tid = (long long)blockIdx.x * kBlock + threadIdx.x;
+    const long long stride = (long long)gridDim.x * kBlock;
+    long long head = 0;
+    if constexpr (VEC) {
+        const long long n4 = count >> 2;
+        const float4* a4 = reinterpret_cast<const float4*>(a);
+        const float4* b4 = reinterpret_cast<const float4*>(b);
+        for (long long e = tid; e < n4; e += stride) {
+            const float4 x = a4[e], y = b4[e];
+            s = __builtin_fma((double)x.x, (double)y.x, s);
+            s = __builtin_fma((double)x.y, (double)y.y, s);
+            s = __builtin_fma((double)x.z, (double)y.z, s);
+            s = __builtin_fma((double)x.w, (double)y.w, s);
+        }
+        head = n4 << 2;
+    }
+    for (long long e = head + tid; e < count; e += stride) s = __builtin_fma((double)a[e], (double)b[e], s);
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kBlock) dot_final_kernel(const double* __restrict__ partial, int n,
+                                                           double* __restrict__ acc) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) s += partial[i];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) *acc = t;
+}
 
 __global__ void __launch_bounds__(kBlock) fill_kernel(float* p, float v, long long count) {
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -978,8 +1007,13 @@ hipError_t launch_fill(float* p, float v, long long count, hipStream_t st) {
 
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st) {
-    hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(1), 0, st, acc);
-    hipLaunchKernelGGL(dot_kernel, dim3(grid_for(count)), dim3(kBlock), 0, st, a, b, count, acc);
+    double* partial = acc + 1;
+    const bool vec = ((reinterpret_cast<size_t>(a) | reinterpret_cast<size_t>(b)) & 15) == 0;
+    if (vec)
+        hipLaunchKernelGGL(dot_partial_kernel<true>, dim3(kDotBlocks), dim3(kBlock), 0, st, a, b, count, partial);
+    else
+        hipLaunchKernelGGL(dot_partial_kernel<false>, dim3(kDotBlocks), dim3(kBlock), 0, st, a, b, count, partial);
+    hipLaunchKernelGGL(dot_final_kernel, dim3(1), dim3(kBlock), 0, st, partial, kDotBlocks, acc);
     return hipGetLastError();
 }
 

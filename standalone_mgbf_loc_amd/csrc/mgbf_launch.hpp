@@ -208,6 +208,9 @@ hipError_t launch_xfer_tile_v(const XferTileArgs& a, const XferVArgs& v, bool ad
 // out(ko,p) = sum_u kw[ko][u] in(ks[ko]+u, p)
 hipError_t launch_kwin(const float* in, float* out, int kin, int kout, long long npts, const int* ks,
                        const float* kw, int w, hipStream_t st);
+// acc[0] = sum_e a[e] b[e] accumulated in FP64; acc holds 1 + kDotBlocks
+// doubles (the partial sums after the first).
+constexpr int kDotBlocks = 1024;
 hipError_t launch_dot(const float* a, const float* b, long long count, double* acc,
                       hipStream_t st);
 
