@@ -130,9 +130,9 @@ template <int H, bool PRO, int VHB, bool RES, int NB>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) radial_mx_kernel(RadialMxParams p) {
     static_assert(kMxCols + 2 * H <= kMxWin, "window exceeds one MFMA K step");
     static_assert(!RES || (H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
-    // at most 8 (NB - 2) row loads, 4 (NB - 1) stores and NB - 2 weight
-    // loads are counted behind a row, and the wave's counter holds 63
-    static_assert(NB >= 2 && 13 * NB - 22 <= 60, "row slots");
+    // at most 3 (NB - 2) row and coarse-row loads and 4 (NB - 1) stores are
+    // behind a row, and the waits count up to 60 of the counter's 63
+    static_assert(NB >= 2 && 3 * (NB - 2) + 4 * (NB - 1) <= 60, "row slots");
     constexpr int NR = 2 * H + 1;
     constexpr int NORow = kWave + 2 * VHB;    // LDS output rows: 64 k plus zero pads
     __shared__ __align__(16) float lds_o[2][NORow * kMxPitch];
