@@ -591,7 +591,10 @@ hipError_t launch_radial_mx_h(const RadialArgs& a, hipStream_t st) {
     if constexpr (kMxCols + 2 * H > kMxWin) {
         return hipErrorInvalidValue;
     } else {
-        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || a.quad || !a.bfrag || a.wfield)
+        // the quadratic transfer (lquart) has no epilogue here; its plain
+        // passes are the same filter
+        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || !a.bfrag || a.wfield ||
+            (a.quad && (a.coarse || a.rout)))
             return hipErrorInvalidValue;
         if ((long long)a.n * a.K * 4 >= (1ll << 31)) return hipErrorInvalidValue;
         const bool pro = a.coarse != nullptr;

@@ -337,16 +337,18 @@ def test_w_and_wt_match_oracle_and_pass_the_dot_product_test():
         assert (y1 - y2).abs().max().item() <= 1e-5 * y1.abs().max().item()
 
 
-@pytest.mark.parametrize("case", ["ens", "ens_band", "ens_filt_g1_off", "bckg", "fast", "fuse0"])
+@pytest.mark.parametrize("case", ["ens", "ens_band", "ens_filt_g1_off", "bckg", "fast", "fuse0", "tuned"])
 def test_lquart_procedures_match_oracle(monkeypatch, case):
     # upsending2_ens / downsending2_ens: the quadratic transfer between
-    # generations in every procedure; W and W^T stay adjoint
+    # generations in every procedure; W and W^T stay adjoint. "tuned":
+    # the autotuner's candidates include the matrix-core engine for the
+    # plain passes (the quadratic transfer epilogues stay on the vector one)
     if case == "fuse0":
         monkeypatch.setenv("MGBF_FUSE", "0")
     kw = dict(n_ens=2, gm=4, h=6, hz=3, weights=[1.0, 0.6, 0.3, 0.15], lquart=True)
     if case == "ens_filt_g1_off":
         kw["l_filt_g1"] = False
-    if case == "ens_band":
+    if case in ("ens_band", "tuned"):
         kw["n_ens"] = 4   # K = 64: the band rides in the last radial pass
     if case == "bckg":
         kw.update(n_ens=1, km2=3, km3=2, mgbf_proc=1)
@@ -358,6 +360,8 @@ def test_lquart_procedures_match_oracle(monkeypatch, case):
     with M.MGBF(cfg) as op:
         d = op.dims
         P = oparams(cfg, d)
+        if case == "tuned":
+            op.autotune()
         x = field(d.gen_shape(1), 140)
         v = np.random.default_rng(141).standard_normal(d.mg_size).astype(np.float32)
         xd, vd = to_dev(x), to_dev(v)
