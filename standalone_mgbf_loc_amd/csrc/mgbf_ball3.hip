@@ -33,6 +33,27 @@ namespace mgbf {
 
 namespace {
 
+// The ball's taps w(a,b,c) = wr[a^2 + b^2 + c^2] group by m = a^2 + c^2:
+// ball_first: (a, c) is the first pair of its m in the loop order (c, then
+// a); ball_m_used: some pair a <= H, c <= HZ has a^2 + c^2 = m.
+template <int H, int HZ>
+__device__ constexpr bool ball_first(int a, int c) {
+    for (int c2 = 0; c2 <= c; ++c2)
+        for (int a2 = 0; a2 <= H; ++a2) {
+            if (c2 == c && a2 >= a) return true;
+            if (a2 * a2 + c2 * c2 == a * a + c * c) return false;
+        }
+    return true;
+}
+
+template <int H, int HZ>
+__device__ constexpr bool ball_m_used(int m) {
+    for (int c = 0; c <= HZ; ++c)
+        for (int a = 0; a <= H; ++a)
+            if (a * a + c * c == m) return true;
+    return false;
+}
+
 template <int H, int HZ, bool BALL, bool T, bool ZL, bool PRO, bool RES = false, bool QR = false>
 __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
     static_assert(!RES || !PRO, "restriction epilogue shape");
@@ -277,22 +298,33 @@ __global__ void __launch_bounds__(kBlock) ball3_kernel(Ball3Params p) {
                 for (int t = 0; t < TI; ++t) {
                     if (!frow) break;
                     if constexpr (BALL) {
-                        // w(a,b,c) = wr[a^2 + b^2 + c^2] on the ball
+                        // w(a,b,c) = wr[a^2 + b^2 + c^2] on the ball: the
+                        // pair sums are first added by m = a^2 + c^2, which
+                        // shares the weight of every b, then Z_b = sum_m
+                        // wr[m + b^2] S_m
+                        constexpr int R2 = (H + 1) * (H + 1);   // a^2 + b^2 + c^2 < R2
+                        constexpr int MM = H * H + HZ * HZ + 1;
+                        float S[MM];
+#pragma unroll
+                        for (int c = 0; c <= HZ; ++c)
+#pragma unroll
+                            for (int a = 0; a <= H; ++a) {
+                                const int mac = a * a + c * c;
+                                if (mac >= R2) continue;
+                                const float P = a == 0 ? X[c][t + HL] : X[c][t + HL + a] + X[c][t + HL - a];
+                                if (ball_first<H, HZ>(a, c))
+                                    S[mac] = P;
+                                else
+                                    S[mac] += P;
+                            }
                         float Z[H + 1];
 #pragma unroll
-                        for (int b = 0; b <= H; ++b) Z[b] = 0.f;
+                        for (int b = 0; b <= H; ++b) {
+                            Z[b] = 0.f;
 #pragma unroll
-                        for (int c = 0; c <= HZ; ++c) {
-                            float P[H + 1];
-                            P[0] = X[c][t + HL];
-#pragma unroll
-                            for (int a = 1; a <= H; ++a) P[a] = X[c][t + HL + a] + X[c][t + HL - a];
-#pragma unroll
-                            for (int b = 0; b <= H; ++b)
-#pragma unroll
-                                for (int a = 0; a <= H; ++a)
-                                    if (a * a + b * b + c * c < (H + 1) * (H + 1))
-                                        Z[b] = __builtin_fmaf(p.W.wr[a * a + b * b + c * c], P[a], Z[b]);
+                            for (int mm = 0; mm < MM; ++mm)
+                                if (ball_m_used<H, HZ>(mm) && mm + b * b < R2)
+                                    Z[b] = __builtin_fmaf(p.W.wr[mm + b * b], S[mm], Z[b]);
                         }
 #pragma unroll
                         for (int b = 0; b <= H; ++b) {
