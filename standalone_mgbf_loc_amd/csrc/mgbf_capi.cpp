@@ -397,12 +397,10 @@ bool can_fuse_band(const mgbf_ctx* c, int which) {
 }
 
 // Can pass `kind` of generation g run on the matrix-core engine? Full
-// 16-column tiles and 64-k chunks, the 4-point transfer in the epilogues,
-// and for the restriction exact x fragments of 3 tile classes.
+// 16-column tiles and 64-k chunks, and for the restriction exact x
+// fragments of 3 tile classes (either transfer stencil).
 bool mx_ok(const mgbf_ctx* c, int g, int kind) {
     if (!c->mfma_ok || c->nx[g] % 16) return false;
-    if (kind == 0) return true;
-    if (c->quad) return false;
     if (kind == 3) return c->mx_res_ok[g] && c->nx[g] >= 32 && c->s.H >= 3;
     return true;
 }
@@ -1333,7 +1331,8 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
     //    by 2^wexp and split into FP16 hi + lo, [b][hi/lo][lane][8];
     //  - per generation g < gm-1 the x restriction of a 16-column tile onto
     //    its 8 coarse columns (R = P^T with the folded edges, multiples of
-    //    1/128, exact in FP16) for the tile classes interior, left edge and
+    //    1/128, or of 1/32 for the quadratic transfer, exact in FP16) for
+    //    the tile classes interior, left edge and
     //    right edge, [class][lane][8]; every tile of the generation must
     //    equal its class, else the restriction stays on the vector kernel.
     std::vector<_Float16> bfrag;
@@ -1359,7 +1358,7 @@ int mgbf_init(const mgbf_config* cfg, int device, mgbf_ctx** out) {
         for (int g = 0; g + 1 < c->gm; ++g) {
             const mgbf::AxisTable& rt = s.rx[g];
             const int nf = s.nx[g];
-            if (nf % 16 || nf < 32 || c->quad || s.H < 3) continue;
+            if (nf % 16 || nf < 32 || s.H < 3) continue;
             auto tile_frag = [&](int i0, std::vector<_Float16>& f) -> bool {
                 std::vector<double> B(32 * 16, 0.0);
                 for (int I = 0; I < 8; ++I) {

@@ -28,11 +28,14 @@
 // exact power of two, whenever a loaded row would exceed 2^14 after
 // scaling; it is undone exactly at the output (ldexp).
 // With RES the same A operand meets the restriction's x weights (multiples
-// of 1/128, exact in FP16) in two more MFMAs: the input row restricted in
+// of 1/128, or 1/32 for the quadratic transfer, exact in FP16) in two more
+// MFMAs: the input row restricted in
 // x over the block's 8 coarse columns.
 // A finished output row (and the x-restricted row) is unscaled, takes the
 // constant weight and the prolongation of the next coarser generation
-// (PRO) in the C layout, and goes to LDS.
+// (PRO) in the C layout, and goes to LDS. QP selects the quadratic
+// transfer between generations (lquart) for the prolongation and the
+// restriction.
 //
 // Phase B (after one barrier per row), wave w owns output columns
 // i0 + 4w .. +3 for all 64 k, lane = k: the vertical band (VHB: the levels
@@ -126,7 +129,7 @@ __device__ __forceinline__ void mx_wait_vmem(int n) {
     }
 }
 
-template <int H, bool PRO, int VHB, bool RES, int NB>
+template <int H, bool PRO, int VHB, bool RES, int NB, bool QP>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) radial_mx_kernel(RadialMxParams p) {
     static_assert(kMxCols + 2 * H <= kMxWin, "window exceeds one MFMA K step");
     static_assert(!RES || (H >= 3 && !PRO && VHB == 0), "restriction epilogue shape");
@@ -213,36 +216,38 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int s = 0; s < NR; ++s) acc[s] = mx_float4{0.f, 0.f, 0.f, 0.f};
 
-    // prolongation window (C layout): 4 x-interpolated coarse rows of the
-    // lane's 4 k at its fine column. The lane's 4 coarse columns cb .. cb+3
-    // are columns cci .. cci+3 of the wave's 12 (i0/2 - 2 .. i0/2 + 9), which
-    // a step's coarse row brings to LDS in one 16-byte load: lane f < 48
-    // takes the 4 k from 4 (f % 4) at column f / 4.
+    // prolongation window (C layout): NP x-interpolated coarse rows of the
+    // lane's 4 k at its fine column (fine i reads coarse (i >> 1) + OFF[i & 1]
+    // .. +NP-1 with the stencil's weights C[i & 1]). The lane's coarse
+    // columns cb .. cb+NP-1 are columns cci .. of the wave's 12 (i0/2 - 2 ..
+    // i0/2 + 9), which a step's coarse row brings to LDS in one 16-byte
+    // load: lane f < 48 takes the 4 k from 4 (f % 4) at column f / 4.
+    using PS = Stencil<QP>;
+    constexpr int NP = PS::NP;
     const int nc = n / 2, mc = m / 2;
     const size_t crow = (size_t)nc * K;
-    mx_float4 XV[4];
-    float cxw[4];
+    mx_float4 XV[NP];
+    float cxw[NP];
     int cci = 0, voc = 0;
-    auto wbase = [](int j) { return ((j + 1) >> 1) - 2; };
+    auto wbase = [](int j) { return (j >> 1) + PS::OFF[j & 1]; };
     if constexpr (PRO) {
-        const int iu = iC >> 1;
-        const bool odd = iC & 1;
-        const int cb = odd ? iu - 1 : iu - 2;
+        const int par = iC & 1;
+        const int cb = (iC >> 1) + PS::OFF[par];
         cci = cb - (i0 / 2 - 2);
-        int ccol[4];
+        int ccol[NP];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < NP; ++t) {
             ccol[t] = kC * 4 + mirror_half(cb + t, nc) * K * 4;
-            cxw[t] = odd ? kPc[t] : kQc[t];
+            cxw[t] = par ? PS::C[1][t] : PS::C[0][t];
         }
         voc = (k0 + 16 * w + 4 * (lane & 3)) * 4 + mirror_half(i0 / 2 - 2 + (lane >> 2), nc) * K * 4;
         const int cbase = wbase(j0);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < NP; ++u) {
             const __amdgpu_buffer_rsrc_t rc = buf_rsrc(p.coarse + (size_t)mirror_half(cbase + u, mc) * crow);
             mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) x += cxw[t] * mx_ld4(rc, ccol[t], 0);
+            for (int t = 0; t < NP; ++t) x += cxw[t] * mx_ld4(rc, ccol[t], 0);
             XV[u] = x;
         }
     }
@@ -297,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
             if (cmove_at(rr)) {
                 const int j = j0 + rr - 2 * H;
                 const __amdgpu_buffer_rsrc_t rc =
-                    buf_rsrc(p.coarse + (size_t)mirror_half(wbase(j) + 3, mc) * crow);
+                    buf_rsrc(p.coarse + (size_t)mirror_half(wbase(j) + NP - 1, mc) * crow);
                 if (lane < kMxCCols * 4) mx_ld_lds16(rc, lds_c0 + sl * (kMxCCols * 16 * 4), voc);
             }
         }
@@ -420,19 +425,21 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
             for (int q = 0; q < 4; ++q) o[q] = __builtin_amdgcn_ldexpf(acc[os][q], eC[q]);
             o *= p.wconst;
             if constexpr (PRO) {
-                // the window moves at odd output rows past the first
-                if (FAST ? (j & 1) != 0 : cmove_at(rr)) {
+                // the window moves past the first output row where its base
+                // does: at odd rows for the 4-point stencil, at even ones
+                // for the quadratic
+                if (FAST ? (j & 1) != (PS::OFF[0] == PS::OFF[1] ? 1 : 0) : cmove_at(rr)) {
 #pragma unroll
-                    for (int u = 0; u < 3; ++u) XV[u] = XV[u + 1];
+                    for (int u = 0; u + 1 < NP; ++u) XV[u] = XV[u + 1];
                     mx_float4 x = mx_float4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int t = 0; t < 4; ++t)
+                    for (int t = 0; t < NP; ++t)
                         x += cxw[t] * *reinterpret_cast<const mx_float4*>(&lds_c[w][slot][(cci + t) * 16 + 4 * g4]);
-                    XV[3] = x;
+                    XV[NP - 1] = x;
                 }
                 const bool oj = (j & 1) != 0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) o += (oj ? kPc[u] : kQc[u]) * XV[u];
+                for (int u = 0; u < NP; ++u) o += (oj ? PS::C[1][u] : PS::C[0][u]) * XV[u];
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) lds_o[buf][(VHB + 16 * w + 4 * g4 + q) * kMxPitch + ln] = o[q];
@@ -449,7 +456,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
             if (vin) {
                 const float xr[2] = {lds_r[buf][(2 * w) * kMxRPitch + lane],
                                      lds_r[buf][(2 * w + 1) * kMxRPitch + lane]};
-                using RS = Stencil<false>;
+                using RS = Stencil<QP>;
                 auto store_coarse = [&](int J, const float* val) {
                     if (J >= Jlo && J < Jhi) {
                         const __amdgpu_buffer_rsrc_t ro =
@@ -557,7 +564,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
     for (; rb < nsteps; rb += NR) general_block(rb);
 }
 
-template <int H, bool PRO, int VHB, bool RES>
+template <int H, bool PRO, int VHB, bool RES, bool QP>
 hipError_t launch_radial_mx_variant(const RadialArgs& a, hipStream_t st) {
     RadialMxParams p{};
     p.in = a.in;
@@ -580,8 +587,36 @@ hipError_t launch_radial_mx_variant(const RadialArgs& a, hipStream_t st) {
     p.ntile_i = a.n / kMxCols;
     p.ntile_j = (a.m + a.TJ - 1) / a.TJ;
     const long long nblocks = (long long)p.ntile_i * p.ntile_j * (a.K / kWave);
-    hipLaunchKernelGGL((radial_mx_kernel<H, PRO, VHB, RES, kMxSlots>), dim3((unsigned)nblocks), dim3(kBlock), 0, st, p);
+    hipLaunchKernelGGL((radial_mx_kernel<H, PRO, VHB, RES, kMxSlots, QP>), dim3((unsigned)nblocks), dim3(kBlock), 0,
+                       st, p);
     return hipGetLastError();
+}
+
+// The pass kinds with a transfer epilogue, for the 4-point (QP = false) or
+// the quadratic transfer.
+template <int H, bool QP>
+hipError_t launch_radial_mx_epi(const RadialArgs& a, hipStream_t st) {
+    const bool pro = a.coarse != nullptr;
+    const int vhb = a.band ? a.vhb : 0;
+    if (a.rout) {
+        if constexpr (H >= 3) {
+            if (pro || vhb || !a.rfrag || a.n < 2 * kMxCols || (a.TJ & 1)) return hipErrorInvalidValue;
+            return launch_radial_mx_variant<H, false, 0, true, QP>(a, st);
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
+    if (vhb) {
+        if (a.lm < 1 || kWave % a.lm) return hipErrorInvalidValue;
+        switch (vhb) {
+            case 3: return launch_radial_mx_variant<H, true, 3, false, QP>(a, st);
+            case 4: return launch_radial_mx_variant<H, true, 4, false, QP>(a, st);
+            case 6: return launch_radial_mx_variant<H, true, 6, false, QP>(a, st);
+            case 8: return launch_radial_mx_variant<H, true, 8, false, QP>(a, st);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    return launch_radial_mx_variant<H, true, 0, false, QP>(a, st);
 }
 
 // The matrix-core engine for half-width H: the shapes it is built for are
@@ -591,34 +626,13 @@ hipError_t launch_radial_mx_h(const RadialArgs& a, hipStream_t st) {
     if constexpr (kMxCols + 2 * H > kMxWin) {
         return hipErrorInvalidValue;
     } else {
-        // the quadratic transfer (lquart) has no epilogue here; its plain
-        // passes are the same filter
-        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || !a.bfrag || a.wfield ||
-            (a.quad && (a.coarse || a.rout)))
+        if (a.K % kWave || a.n % kMxCols || a.TJ < 1 || a.klo != 0 || !a.bfrag || a.wfield)
             return hipErrorInvalidValue;
         if ((long long)a.n * a.K * 4 >= (1ll << 31)) return hipErrorInvalidValue;
         const bool pro = a.coarse != nullptr;
-        const int vhb = a.band ? a.vhb : 0;
-        if (a.rout) {
-            if constexpr (H >= 3) {
-                if (pro || vhb || !a.rfrag || a.n < 2 * kMxCols || (a.TJ & 1)) return hipErrorInvalidValue;
-                return launch_radial_mx_variant<H, false, 0, true>(a, st);
-            } else {
-                return hipErrorInvalidValue;
-            }
-        }
-        if (vhb) {
-            if (!pro || a.lm < 1 || kWave % a.lm) return hipErrorInvalidValue;
-            switch (vhb) {
-                case 3: return launch_radial_mx_variant<H, true, 3, false>(a, st);
-                case 4: return launch_radial_mx_variant<H, true, 4, false>(a, st);
-                case 6: return launch_radial_mx_variant<H, true, 6, false>(a, st);
-                case 8: return launch_radial_mx_variant<H, true, 8, false>(a, st);
-                default: return hipErrorInvalidValue;
-            }
-        }
-        return pro ? launch_radial_mx_variant<H, true, 0, false>(a, st)
-                   : launch_radial_mx_variant<H, false, 0, false>(a, st);
+        if (a.band && !pro) return hipErrorInvalidValue;
+        if (!pro && !a.rout) return launch_radial_mx_variant<H, false, 0, false, false>(a, st);
+        return a.quad ? launch_radial_mx_epi<H, true>(a, st) : launch_radial_mx_epi<H, false>(a, st);
     }
 }
 

@@ -732,19 +732,23 @@ def test_matrix_core_engine_matches_oracle(monkeypatch, h, K, n, m, gm, scale):
         assert not torch.equal(y, y_vec)   # the matrix-core path ran
 
 
-@pytest.mark.parametrize("case", ["members_apart", "rows_grow", "wfield", "fast", "band64", "lm32"])
+@pytest.mark.parametrize("case", ["members_apart", "rows_grow", "wfield", "fast", "band64", "lm32", "lquart",
+                                  "lquart_lm32"])
 def test_matrix_core_engine_scales_and_epilogues(monkeypatch, case):
     # members whose magnitudes differ by 1e10 in one stack (each member's k
     # keep their own exponent; every member is held to the bar on its own),
     # rows that grow by 2^24 down the field (the exponents are lowered and the
     # ring rescaled on the way), per-point weights, the separable fast filter,
-    # and the vertical band fused into the last pass with whole 64-level
-    # columns and with two 32-level columns per chunk
+    # the vertical band fused into the last pass with whole 64-level columns
+    # and with two 32-level columns per chunk, and the quadratic transfer
+    # (lquart) in the restriction, prolongation and band epilogues
     monkeypatch.setenv("MGBF_MFMA", "1")
     lm, n_ens, kw = 64, 2, {}
     if case == "fast":
         kw = dict(mgbf_proc=8, p=2)
-    if case == "lm32":
+    if case.startswith("lquart"):
+        kw = dict(lquart=True)
+    if case in ("lm32", "lquart_lm32"):
         lm, n_ens = 32, 4
     cfg = M.Config.for_grid(96, 64, lm, n_ens=n_ens, gm=3, h=6, hz=3, weights=[1.0, 0.5, 0.25], **kw)
     if case == "fast":
@@ -771,7 +775,7 @@ def test_matrix_core_engine_scales_and_epilogues(monkeypatch, case):
         t = op.tiles()
         assert all(t[(1, kind)][0] == 16 for kind in (0, 1, 3))
         y = op.filtering_procedure(to_dev(x))
-        if case in ("band64", "lm32", "members_apart", "rows_grow"):
+        if case in ("band64", "lm32", "members_apart", "rows_grow", "lquart", "lquart_lm32"):
             # the band rode in the last radial pass (no vertical launch)
             op.timers(True)
             op.filtering_procedure(to_dev(x))
