@@ -7,7 +7,9 @@ installed outside the repository.
 """
 from __future__ import annotations
 
+import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -48,15 +50,34 @@ def flags() -> list:
     ]
 
 
+# the compiler's per-function resource report (-Rpass-analysis=kernel-resource-usage)
+_REMARK = re.compile(r"remark:\s+(Function Name|ScratchSize \[bytes/lane\]|VGPRs|AGPRs|VGPRs Spill|SGPRs Spill|"
+                     r"Occupancy \[waves/SIMD\]|LDS Size \[bytes/block\]):\s+(\S+)")
+RESOURCES = os.path.join(LIBDIR, "kernel_resources.json")
+
+
+def _resources(text: str) -> dict:
+    out, cur = {}, None
+    for m in _REMARK.finditer(text):
+        key, val = m.group(1), m.group(2)
+        if key == "Function Name":
+            cur = val
+            out[cur] = {}
+        elif cur is not None:
+            out[cur][key] = int(val) if val.isdigit() else val
+    return out
+
+
 def _compile(job):
+    """(error text or None, {function: resources})."""
     src, obj, extra, verbose = job
-    cmd = [hipcc()] + flags() + extra + ["-c", src, "-o", obj]
+    cmd = [hipcc()] + flags() + extra + ["-Rpass-analysis=kernel-resource-usage", "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        return " ".join(cmd) + "\n" + res.stdout + res.stderr
-    return None
+        return " ".join(cmd) + "\n" + res.stdout + res.stderr, {}
+    return None, _resources(res.stderr)
 
 
 def build(verbose: bool = False, jobs: int = 0) -> str:
@@ -77,10 +98,23 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
                      [f"-DMGBF_BALL_H={h}"], verbose))
     jobs = jobs or min(len(work), max(1, (os.cpu_count() or 4)))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        errs = [e for e in ex.map(_compile, work) if e]
+        results = list(ex.map(_compile, work))
+    errs = [e for e, _ in results if e]
     if errs:
         sys.stderr.write("\n".join(errs))
         raise RuntimeError("hipcc failed building libmgbf.so")
+    # every kernel keeps its state in registers and LDS: a kernel with
+    # scratch memory (an array the compiler could not keep in registers,
+    # such as a ring indexed in a loop left rolled) runs several times
+    # slower, so it fails the build
+    res = {}
+    for _, r in results:
+        res.update(r)
+    with open(RESOURCES, "w") as f:
+        json.dump(res, f, indent=0, sort_keys=True)
+    scratch = sorted(k for k, v in res.items() if v.get("ScratchSize [bytes/lane]", 0) > 0)
+    if scratch:
+        raise RuntimeError("kernels with scratch memory: " + ", ".join(scratch[:10]))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}"] + [w[1] for w in work] + ["-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -402,6 +402,7 @@ bool can_fuse_band(const mgbf_ctx* c, int which) {
 bool mx_ok(const mgbf_ctx* c, int g, int kind) {
     if (!c->mfma_ok || c->nx[g] % 16) return false;
     if (kind == 3) return c->mx_res_ok[g] && c->nx[g] >= 32 && c->s.H >= 3;
+    if (kind == 2) return c->s.H < mgbf::kMxMaxBandH;
     return true;
 }
 

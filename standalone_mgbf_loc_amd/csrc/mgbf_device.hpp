@@ -1,6 +1,9 @@
 // Device helpers shared by the HIP kernels of the MGBF operator.
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include <hip/hip_runtime.h>
 
 namespace mgbf {
@@ -40,6 +43,13 @@ __device__ __forceinline__ unsigned xcd_block(unsigned bid, unsigned nblocks) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const float* base) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)0xffffffff,
                                              0x00020000);
+}
+
+// f(integral_constant<int, I>) for I in the sequence, unrolled at compile
+// time whatever the size of f's body
+template <class F, int... I>
+__device__ __forceinline__ void static_for(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
 }
 
 // Makes a value loaded before a loop count as consumed here: the empty asm

@@ -82,6 +82,10 @@ struct RadialParams {
     RadialCoef W;
 };
 
+// the matrix-core engine takes the vertical band epilogue below this
+// half-width (at 8 the band's registers would spill beside the ring)
+constexpr int kMxMaxBandH = 8;
+
 // kernel argument record of the matrix-core radial kernel (constant
 // weights only: a pass with a weight field runs on the vector kernel)
 struct RadialMxParams {

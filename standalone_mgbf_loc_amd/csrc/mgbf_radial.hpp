@@ -174,8 +174,11 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
     }
 
     for (int rb = 0; rb < nsteps; rb += NR) {
-#pragma unroll
-        for (int s = 0; s < NR; ++s) {
+        // one step per ring row, unrolled at compile time: the ring index
+        // must be a constant, and a loop the compiler leaves rolled (large
+        // half-widths with the restriction) puts the ring in scratch memory
+        static_for([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = decltype(S)::value;
             const int rr = rb + s;
             if (rr < nsteps) {
                 const int r = mirror_half(j0 - H + rr, m);
@@ -407,7 +410,7 @@ __global__ void __launch_bounds__(kBlock) radial_kernel(RadialParams p) {
 #pragma unroll
                 for (int t = 0; t < TI; ++t) acc[(s + slot_off) % NR][t] = 0.f;
             }
-        }
+        }, std::make_integer_sequence<int, NR>{});
     }
 }
 

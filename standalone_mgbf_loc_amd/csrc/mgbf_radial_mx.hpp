@@ -607,13 +607,19 @@ hipError_t launch_radial_mx_epi(const RadialArgs& a, hipStream_t st) {
         }
     }
     if (vhb) {
-        if (a.lm < 1 || kWave % a.lm) return hipErrorInvalidValue;
-        switch (vhb) {
-            case 3: return launch_radial_mx_variant<H, true, 3, false, QP>(a, st);
-            case 4: return launch_radial_mx_variant<H, true, 4, false, QP>(a, st);
-            case 6: return launch_radial_mx_variant<H, true, 6, false, QP>(a, st);
-            case 8: return launch_radial_mx_variant<H, true, 8, false, QP>(a, st);
-            default: return hipErrorInvalidValue;
+        // the band's coefficients and rows do not fit beside the ring and
+        // the weight fragments of half-width 8 (they would spill)
+        if constexpr (H >= kMxMaxBandH) {
+            return hipErrorInvalidValue;
+        } else {
+            if (a.lm < 1 || kWave % a.lm) return hipErrorInvalidValue;
+            switch (vhb) {
+                case 3: return launch_radial_mx_variant<H, true, 3, false, QP>(a, st);
+                case 4: return launch_radial_mx_variant<H, true, 4, false, QP>(a, st);
+                case 6: return launch_radial_mx_variant<H, true, 6, false, QP>(a, st);
+                case 8: return launch_radial_mx_variant<H, true, 8, false, QP>(a, st);
+                default: return hipErrorInvalidValue;
+            }
         }
     }
     return launch_radial_mx_variant<H, true, 0, false, QP>(a, st);
