@@ -10,7 +10,8 @@ src.mgbf/mg_intstate.f90:333-351). Gradients:
     dJ/dw_g = (W^T g)_g * (W^T z)_g          per-point weight field
 
 with g = dJ/dy. The per-generation states W^T z are recomputed in the
-backward pass, not stored. The adjoint halves are the hand-written kernels of
+backward pass, not stored; W^T g is computed once and also gives
+B_w g = W A_w (W^T g). The adjoint halves are the hand-written kernels of
 the library (W^T: restriction chain + radial filter; W: radial filter +
 prolongation chain + vertical filter).
 """
@@ -65,11 +66,19 @@ class _Localize(torch.autograd.Function):
         z, *weights = ctx.saved_tensors
         _set_weights(op, weights, ctx.values)
         gy = gy.contiguous()
-        gz = op.apply(M.OP_WAWT, gy) if ctx.needs_input_grad[0] else None
+        need_w = any(ctx.needs_input_grad[2:])
+        gz, wg = None, None
+        if need_w:
+            # W^T g serves both: the weight gradients and B g = W A (W^T g)
+            wg = op.apply(M.OP_WT, gy)
+            if ctx.needs_input_grad[0]:
+                gz = op.apply(M.OP_W, op.apply(M.OP_A, wg))
+        elif ctx.needs_input_grad[0]:
+            gz = op.apply(M.OP_WAWT, gy)
         gw = [None] * len(weights)
-        if any(ctx.needs_input_grad[2:]):
+        if need_w:
             # recompute the per-generation states of both vectors
-            vg = op.split_mg(op.apply(M.OP_WT, gy))
+            vg = op.split_mg(wg)
             vz = op.split_mg(op.apply(M.OP_WT, z))
             for g, w in enumerate(weights):
                 if not ctx.needs_input_grad[2 + g]:
