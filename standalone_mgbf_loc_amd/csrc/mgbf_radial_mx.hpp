@@ -229,7 +229,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2, 
     mx_float4 XV[NP];
     float cxw[NP];
     int cci = 0, voc = 0;
-    auto wbase = [](int j) { return (j >> 1) + PS::OFF[j & 1]; };
+    // the window's first coarse row for output row j: (j >> 1) + OFF[j & 1]
+    auto wbase = [](int j) { return QP ? (j >> 1) - 1 : ((j + 1) >> 1) - 2; };
+    static_assert(PS::OFF[0] == (QP ? -1 : -2) && PS::OFF[1] == -1, "prolongation window offsets");
     if constexpr (PRO) {
         const int par = iC & 1;
         const int cb = (iC >> 1) + PS::OFF[par];
