@@ -638,7 +638,9 @@ def main():
                                 "frac": round(g1[ph][2] / (g1[ph][0] / 1e3) / 1e9 / HBM_PEAK_GBPS, 4)
                                 if g1[ph][0] else None}
                          for name, ph in (("first (F^T, A, restriction)", "hfiltT"),
-                                          ("second (F, prolongation, vertical band)", "hfilt"))}},
+                                          ("second (F, prolongation; the vertical band a pass of its own)"
+                                           if tiles and tiles.get((1, 2), (1, 1))[0] == 0 else
+                                           "second (F, prolongation, vertical band)", "hfilt"))}},
         "cpu_baseline": cpu,
         "max_rel_err_vs_fp64": relerr,
         "plain": plain,

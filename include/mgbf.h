@@ -247,22 +247,28 @@ int mgbf_timers_read_gen(mgbf_ctx* ctx, int gen, double* ms, long long* launches
    (output columns per lane x output rows per wavefront), for each generation
    and each pass kind (0 plain, 1 with the prolongation, 2 with the
    prolongation and the vertical band, 3 plain with the restriction to the
-   next generation), and keeps the fastest. Uses the
-   context's workspace; synchronizes `stream`. No counterpart in the
-   reference, whose loops have no tiles. */
+   next generation), and keeps the fastest. The candidate heights are a fixed
+   ladder and the heights that fit the launch into whole rounds of the
+   chip's waves. For generation 1 it also times the vertical band as a pass
+   of its own after a kind-1 pass, and keeps that form when it beats kind 2.
+   Uses the context's workspace; synchronizes `stream`. No counterpart in
+   the reference, whose loops have no tiles. */
 int mgbf_autotune(mgbf_ctx* ctx, void* stream);
 /* The tile in use for generation gen (1-based) and pass kind: *ti is the
    output columns per lane of the vector kernel (2 or 4), or 16 when the pass
    runs on the matrix cores (16 output columns x 64 k per block of 4
-   wavefronts); *tj is the output rows per wavefront or block. */
+   wavefronts), or 0 for generation 1's kind 2 when the vertical band runs as
+   a pass of its own; *tj is the output rows per wavefront or block. */
 int mgbf_get_tile(const mgbf_ctx* ctx, int gen, int kind, int* ti, int* tj);
 /* Pins the tile of generation gen (1-based) and pass kind, in the terms of
-   mgbf_get_tile (ti = 2 or 4, or 16 for the matrix cores; tj >= 1, even for
-   kind 3), so that tiles found by mgbf_autotune in one process can be reused
-   in another without timing again. The matrix cores take every pass kind of
-   a generation whose width is a multiple of 16, with K a multiple of 64, a
-   disc or product filter of half-width <= 8 and the 4-point transfer (kind
-   3 also: at least 32 columns, half-width >= 3); else
+   mgbf_get_tile (ti = 2 or 4, or 16 for the matrix cores, or 0 for
+   generation 1's kind 2: the vertical band as a pass of its own; tj >= 1,
+   even for kind 3), so that tiles found by mgbf_autotune in one process can
+   be reused in another without timing again. The matrix cores take every
+   pass kind of a generation whose width is a multiple of 16, with K a
+   multiple of 64 and a disc or product filter of half-width <= 8, with
+   either transfer stencil (kind 2 also: half-width < 8; kind 3 also: at
+   least 32 columns, half-width >= 3, exact FP16 restriction weights); else
    MGBF_ERR_UNSUPPORTED. No counterpart in the reference. */
 int mgbf_set_tile(mgbf_ctx* ctx, int gen, int kind, int ti, int tj);
 

@@ -107,6 +107,10 @@ struct mgbf_ctx {
     // fusion into the radial passes: 0 none, 1 prolongation, 2 prolongation and
     // band (up-sweep epilogues), 3 also the restriction (down-sweep)
     int fuse = 3;
+    // the generation-1 vertical band as a pass of its own after a kind-1
+    // pass, where that is faster than the fused kind 2 (autotuner), or
+    // pinned by a kind-2 tile with ti = 0
+    bool band_split = false;
     // radial tiles per generation and pass kind (0 plain, 1 +P, 2 +P and band,
     // 3 plain with the restriction to the next generation)
     int tile_ti[mgbf::kMaxGen][4] = {};
@@ -554,7 +558,7 @@ int upsweep(mgbf_ctx* c, const float* in1, const float* inb, const float* u1, fl
     float* T = c->ws_t;
     // the radial kernels fuse the 4-point or the quadratic prolongation,
     // and the band after either
-    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band) && !c->aniso[0];
+    const bool fuse = band >= 0 && c->fuse >= 2 && can_fuse_band(c, band) && !c->aniso[0] && !c->band_split;
     for (int g = c->gm - 1; g >= 0; --g) {
         const float* src = g == 0 ? in1 : inb + c->goff[g];
         const float* coarse = g + 1 < c->gm ? U + c->goff[g + 1] : nullptr;
@@ -1796,6 +1800,7 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     for (int g = 0; g < c->gm && status == MGBF_OK; ++g) {
         const int n = c->nx[g], m = c->ny[g];
         if (c->aniso[g]) continue;   // the per-point aspect filter has no tiles
+        float kind_ms[4] = {-1.f, -1.f, -1.f, -1.f};   // the fastest candidate per kind
         for (int kind = 0; kind < 4 && status == MGBF_OK; ++kind) {
             if (kind >= 1 && g + 1 >= c->gm) continue;
             if (kind == 2 && !(g == 0 && c->nfilt > 0 && can_fuse_band(c, vband))) continue;
@@ -1806,21 +1811,41 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             float best = 1e30f;
             int bti = c->tile_ti[g][kind], btj = c->tile_tj[g][kind], bmf = c->tile_mfma[g][kind];
             const bool try_mfma = mx_ok(c, g, kind);
-            const int tjs[11] = {16, 24, 32, 48, 64, 96, 128, 192, 256, 384, m};
             // engines: TI = 2, TI = 4 (vector), 16 (matrix cores)
             const int engines[3] = {2, 4, 16};
             for (int en : engines) {
                 if (en == 16 && !try_mfma) continue;
                 if (kind == 3 && en == 2) continue;
-                for (int q = 0; q < 11 && status == MGBF_OK; ++q) {
+                // waves of one row of tiles (every column tile and k chunk)
+                const long long strip = (long long)((c->K + 63) / 64) * ((n + en - 1) / en) * (en == 16 ? 4 : 1);
+                // tile heights: a fixed ladder, and for q = 1 .. 16 the
+                // lowest height whose launch stays within q x 1024 waves
+                // (q / occupancy rounds of the chip's 1024 SIMDs). On a
+                // small field the ladder's heights leave the last round of
+                // blocks part empty, or take many rounds of short tiles
+                // that each repeat the 2H rows of the ring's start.
+                int tjs[32];
+                int ntj = 0;
+                auto add_tj = [&](int tj) {
+                    if (tj < 1 || tj > m) return;
+                    for (int q2 = 0; q2 < ntj; ++q2)
+                        if (tjs[q2] == tj) return;
+                    tjs[ntj++] = tj;
+                };
+                for (int tj : {16, 24, 32, 48, 64, 96, 128, 192, 256, 384}) add_tj(tj);
+                add_tj(m);
+                for (long long q = 1; q <= 16; ++q) {
+                    const long long segs = q * 1024 / strip;
+                    if (segs < 1) continue;
+                    int tj = (int)((m + segs - 1) / segs);
+                    if (kind == 3) tj += tj & 1;
+                    add_tj(tj);
+                }
+                for (int q = 0; q < ntj && status == MGBF_OK; ++q) {
                     const int tj = tjs[q];
-                    bool dup = false;
-                    for (int q2 = 0; q2 < q; ++q2) dup = dup || tjs[q2] == tj;
-                    if (tj > m || dup) continue;
                     if (kind == 3 && !fused_ok(c, g, tj, c->s.H)) continue;
-                    const long long waves = (long long)((c->K + 63) / 64) * ((n + en - 1) / en) *
-                                            ((m + tj - 1) / tj) * (en == 16 ? 4 : 1);
-                    if (waves < 2048 && tj != 16) continue;
+                    const long long waves = strip * ((m + tj - 1) / tj);
+                    if (waves < 512 && tj != 16) continue;
                     c->tile_ti[g][kind] = en == 16 ? 4 : en;
                     c->tile_tj[g][kind] = tj;
                     c->tile_mfma[g][kind] = en == 16 ? 1 : 0;
@@ -1840,6 +1865,26 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
             c->tile_ti[g][kind] = bti;
             c->tile_tj[g][kind] = btj;
             c->tile_mfma[g][kind] = bmf;
+            if (best < 1e30f) kind_ms[kind] = best;
+        }
+        // the band fused into the last pass (kind 2), or a kind-1 pass and the
+        // band as a pass of its own: where the band's half-width keeps the
+        // fused pass off the matrix cores, the split form can be faster
+        if (g == 0 && status == MGBF_OK && kind_ms[1] >= 0.f && kind_ms[2] >= 0.f) {
+            float vms = 0.f;
+            status = vertical(c, c->ws_u, c->ws_t, vband, st);
+            if (status == MGBF_OK && hipEventRecord(e0, st) != hipSuccess)
+                status = fail(MGBF_ERR_HIP, "autotune timing");
+            for (int rep = 0; rep < 3 && status == MGBF_OK; ++rep) status = vertical(c, c->ws_u, c->ws_t, vband, st);
+            if (status == MGBF_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&vms, e0, e1) != hipSuccess))
+                status = fail(MGBF_ERR_HIP, "autotune timing");
+            if (status == MGBF_OK) {
+                c->band_split = kind_ms[1] + vms < kind_ms[2];
+                if (log)
+                    std::fprintf(stderr, "[autotune] gen 1 band: fused %.4f ms, split %.4f + %.4f ms\n",
+                                 kind_ms[2] / 3, kind_ms[1] / 3, vms / 3);
+            }
         }
     }
     c->timing = timing;
@@ -1853,6 +1898,7 @@ int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
     if (!c || !ti || !tj || gen < 1 || gen > c->gm || kind < 0 || kind > 3)
         return fail(MGBF_ERR_ARG, "bad argument");
     *ti = c->tile_mfma[gen - 1][kind] && mx_ok(c, gen - 1, kind) ? 16 : c->tile_ti[gen - 1][kind];
+    if (gen == 1 && kind == 2 && c->band_split) *ti = 0;
     *tj = c->tile_tj[gen - 1][kind];
     return MGBF_OK;
 }
@@ -1860,18 +1906,25 @@ int mgbf_get_tile(const mgbf_ctx* c, int gen, int kind, int* ti, int* tj) {
 int mgbf_set_tile(mgbf_ctx* c, int gen, int kind, int ti, int tj) {
     if (!c || gen < 1 || gen > c->gm || kind < 0 || kind > 3)
         return fail(MGBF_ERR_ARG, "bad argument");
+    if (ti == 0 && gen == 1 && kind == 2 && tj >= 1) {
+        // the vertical band as a pass of its own, after the kind-1 pass
+        c->band_split = true;
+        ++c->version;
+        return MGBF_OK;
+    }
     if (!(ti == 2 || ti == 4 || ti == 16) || tj < 1)
-        return fail(MGBF_ERR_ARG, "tile must be ti in {2, 4, 16} and tj >= 1");
+        return fail(MGBF_ERR_ARG, "tile must be ti in {2, 4, 16} (or 0 for generation 1's kind 2) and tj >= 1");
     if (kind == 3 && (ti == 2 || (tj & 1)))
         return fail(MGBF_ERR_ARG, "the pass with the restriction takes ti = 4 or 16 and an even tj");
     if (ti == 16 && !mx_ok(c, gen - 1, kind))
         return fail(MGBF_ERR_UNSUPPORTED,
                     "the matrix-core engine needs half-width <= 8, a disc or product filter, a stack of "
-                    "whole 64-k chunks, 16-column tiles and the 4-point transfer (restriction: tiles "
-                    "of exact FP16 x weights, at least 32 columns)");
+                    "whole 64-k chunks and 16-column tiles (with the vertical band: half-width < 8; with "
+                    "the restriction: half-width >= 3, tiles of exact FP16 x weights, at least 32 columns)");
     c->tile_ti[gen - 1][kind] = ti == 16 ? 4 : ti;
     c->tile_tj[gen - 1][kind] = tj;
     c->tile_mfma[gen - 1][kind] = ti == 16 ? 1 : 0;
+    if (gen == 1 && kind == 2) c->band_split = false;
     ++c->version;
     return MGBF_OK;
 }

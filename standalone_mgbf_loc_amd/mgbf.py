@@ -464,7 +464,9 @@ class MGBF:
 
     def autotune(self, stream=None) -> dict:
         """Times the candidate radial tiles on this context's shapes and keeps
-        the fastest; returns {(gen, kind): (TI, TJ)}."""
+        the fastest; returns {(gen, kind): (TI, TJ)}. TI = 0 for (1, 2): the
+        vertical band runs as a pass of its own after the kind-1 pass, which
+        was faster than the band fused into it."""
         _check(self._L.mgbf_autotune(self._h, _stream(stream)))
         return self.tiles()
 
@@ -479,7 +481,8 @@ class MGBF:
 
     def set_tiles(self, tiles: dict):
         """Pins tiles {(gen, kind): (TI, TJ)} as returned by autotune()/tiles()
-        (TI = 16: the matrix cores)."""
+        (TI = 16: the matrix cores; TI = 0 for (1, 2): the vertical band as a
+        pass of its own)."""
         for (g, kind), (ti, tj) in tiles.items():
             _check(self._L.mgbf_set_tile(self._h, int(g), int(kind), int(ti), int(tj)))
 

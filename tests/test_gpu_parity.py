@@ -688,7 +688,7 @@ def test_autotuned_tiles_keep_parity():
         tiles = op.autotune()
         y1 = op.filtering_procedure(x)
         sync()
-        assert all(ti in (2, 4, 16) and tj >= 1 for ti, tj in tiles.values())
+        assert all((ti in (2, 4, 16) or (key == (1, 2) and ti == 0)) and tj >= 1 for key, (ti, tj) in tiles.items())
         assert (y1 - y0).abs().max().item() <= 1e-6 * y0.abs().max().item() or \
             any(ti == 16 for ti, _ in tiles.values())
         assert relerr(y1, O.wawt(oparams(cfg, d), xh.astype(np.float64))) <= TOL_OP
@@ -699,6 +699,79 @@ def test_autotuned_tiles_keep_parity():
             y2 = op2.filtering_procedure(x)
             sync()
             assert torch.equal(y1, y2)
+
+
+@pytest.mark.parametrize("engine", [2, 4, 16])
+def test_odd_tile_heights_match_oracle(engine):
+    # the autotuner also tries tile heights fitted to the chip's wave count,
+    # odd ones included for the passes without the restriction: tiles then
+    # start at odd rows, where the prolongation's coarse window and the
+    # vertical band must still line up
+    cfg = M.Config.for_grid(96, 80, 32, n_ens=2, gm=3, h=6, hz=3, weights=[1.0, 0.6, 0.3])
+    heights = {(1, 0): 13, (1, 1): 11, (1, 2): 9, (1, 3): 14, (2, 0): 7, (2, 1): 5, (2, 3): 10,
+               (3, 0): 3}
+    print(f"odd tile heights, engine {engine}", flush=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        assert d.km_all == 64
+        P = oparams(cfg, d)
+        tiles = {}
+        for (g, kind), tj in heights.items():
+            ti = engine
+            if kind == 3 and ti == 2:
+                ti = 4
+            if ti == 16 and d.gen_shape(g)[1] % 16:
+                ti = 4
+            tiles[(g, kind)] = (ti, tj)
+        op.set_tiles(tiles)
+        got = op.tiles()
+        for key, val in tiles.items():
+            assert got[key] == val
+        x = field(d.gen_shape(1), 171)
+        v = np.random.default_rng(172).standard_normal(d.mg_size).astype(np.float32)
+        y = op.filtering_procedure(to_dev(x))
+        wt = op.apply(M.OP_WT, to_dev(x))
+        w = op.apply(M.OP_W, to_dev(v))
+        sync()
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.wawt(P, x64)) <= TOL_OP
+        assert relerr(wt, O.wt(P, x64)) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+
+
+@pytest.mark.parametrize("h", [6, 8])
+def test_vertical_band_as_a_pass_of_its_own_matches_fused(h):
+    # a kind-2 tile with TI = 0 (what the autotuner picks where the kind-1
+    # pass and a separate band beat the fused pass) runs the band as a
+    # vertical pass after the last radial pass; the result is the fused one's
+    # to rounding and the oracle's
+    cfg = M.Config.for_grid(96, 64, 32, n_ens=2, gm=3, h=h, hz=4, weights=[1.0, 0.6, 0.3])
+    print(f"band as a pass of its own, h={h}", flush=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = to_dev(field(d.gen_shape(1), 181))
+        op.timers(True)
+        y_fused = op.filtering_procedure(x)
+        fused = op.read_timers()
+        op.timers(False)
+        tj = op.tiles()[(1, 2)][1]
+        op.set_tiles({(1, 2): (0, tj)})
+        assert op.tiles()[(1, 2)] == (0, tj)
+        op.timers(True)
+        y_split = op.filtering_procedure(x)
+        split = op.read_timers()
+        op.timers(False)
+        sync()
+        assert fused["vfilt"][1] == 0 and split["vfilt"][1] == 1
+        assert relerr(y_split, y_fused.double().cpu().numpy()) <= 1e-6
+        assert relerr(y_split, O.wawt(P, x.cpu().numpy().astype(np.float64))) <= TOL_OP
+        # a kind-2 tile of an engine brings the fused pass back
+        op.set_tiles({(1, 2): (4, tj)})
+        assert op.tiles()[(1, 2)] == (4, tj)
+        y_back = op.filtering_procedure(x)
+        sync()
+        assert relerr(y_back, y_fused.double().cpu().numpy()) <= 1e-6
 
 
 @pytest.mark.parametrize("h,K,n,m,gm,scale", [(6, 64, 96, 96, 3, 1.0), (4, 64, 80, 48, 3, 1e-6),
