@@ -453,19 +453,26 @@ int radial(mgbf_ctx* c, int g, const float* in, float* out, const float* wfield,
         a.rytab = c->d_rytab[g];
         a.rymeta = c->d_rymeta[g];
     }
-    // the matrix-core engine takes constant weights; a weighted pass runs on
-    // the vector kernel at the same TJ
-    if (c->tile_mfma[g][kind] && mx_ok(c, g, kind) && !wfield) {
+    // the matrix-core engine takes constant weights: a per-point weight
+    // field follows its pass as a scale pass of its own (the tuner times the
+    // two together against the vector kernel, which reads the field in its
+    // epilogue)
+    const bool mx = c->tile_mfma[g][kind] && mx_ok(c, g, kind);
+    if (mx) {
         a.mfma = true;
         a.bfrag = c->d_bfrag;
         a.rfrag = rout ? c->d_rfrag[g] : nullptr;
         a.wexp = c->wexp;
+        a.wfield = nullptr;
+        a.wconst = wfield ? 1.0f : wconst;
     }
     a.exact_disc = c->s.exact_disc;
     a.separable = c->s.separable;
     a.quad = c->quad;
     a.coef = c->coef;
     HIP_TRY(mgbf::launch_radial(a, st), "radial filter launch");
+    if (mx && wfield)
+        HIP_TRY(mgbf::launch_scale(out, out, wfield, wconst, 0, c->gsize[g], st), "scale launch");
     return MGBF_OK;
 }
 
@@ -1783,14 +1790,16 @@ int mgbf_autotune(mgbf_ctx* c, void* stream) {
     if (status == MGBF_OK) status = restrict_chain(c, c->ws_u, c->ws_u, st);
     // kind 3 writes its restriction over the (zero) next generation of U,
     // which the restriction of zeros leaves unchanged
+    // the pass with the restriction is the down-sweep's weighted one: timed
+    // with the generation's weight field, if it has one
     auto time_pass = [&](int g, const float* coarse, int band, float* rout, float* ms) -> int {
-        int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT,
-                       coarse, band, rout);
+        const float* wf = rout ? c->wfield[g] : nullptr;
+        const float wc = wf ? c->wconst[g] : 1.0f;
+        int r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, wf, wc, st, MGBF_T_HFILT, coarse, band, rout);
         if (r != MGBF_OK) return r;
         if (hipEventRecord(e0, st) != hipSuccess) return fail(MGBF_ERR_HIP, "autotune timing");
         for (int rep = 0; rep < 3 && r == MGBF_OK; ++rep)
-            r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, nullptr, 1.0f, st, MGBF_T_HFILT, coarse,
-                       band, rout);
+            r = radial(c, g, c->ws_u + c->goff[g], c->ws_t, wf, wc, st, MGBF_T_HFILT, coarse, band, rout);
         if (r != MGBF_OK) return r;
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(ms, e0, e1) != hipSuccess)
