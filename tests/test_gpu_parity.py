@@ -897,7 +897,15 @@ def test_single_field_768x768x64_five_generations_full_size():
         x = box3(np.random.default_rng(3).standard_normal(d.gen_shape(1))).astype(np.float32)
         y = op.filtering_procedure(to_dev(x))
         sync()
-        assert relerr(y, O.wawt(oparams(cfg, d), x.astype(np.float64))) <= TOL_OP
+        ref = O.wawt(oparams(cfg, d), x.astype(np.float64))
+        assert relerr(y, ref) <= TOL_OP
+        # with the autotuner's tiles (on this field: matrix-core passes with
+        # tile heights fitted to one round of waves, and the vertical band
+        # as a pass of its own where that was faster)
+        print(f"single field: autotuned tiles {op.autotune()}", flush=True)
+        y2 = op.filtering_procedure(to_dev(x))
+        sync()
+        assert relerr(y2, ref) <= TOL_OP
 
 
 def test_batched_ensemble_80x768x768x64_full_size():
