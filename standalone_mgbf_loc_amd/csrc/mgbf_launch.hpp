@@ -87,7 +87,7 @@ struct RadialParams {
 constexpr int kMxMaxBandH = 8;
 
 // kernel argument record of the matrix-core radial kernel (constant
-// weights only: a pass with a weight field runs on the vector kernel)
+// weights only: a per-point weight field follows as a scale pass)
 struct RadialMxParams {
     const float* in;
     float* out;
