@@ -240,6 +240,11 @@ def run_group(args, torch, MG):
     log(f"[bench] group: {len(doms)} domains {doms}")
     ops, xs, outs, amps = [], [], [], []
     gen = torch.Generator(device="cuda")
+    # --tiles: a list of per-domain tile dicts (the "tiles" of an earlier
+    # group line), pinned instead of autotuning
+    pinned = json.loads(args.tiles) if args.tiles else None
+    if pinned is not None and len(pinned) != len(doms):
+        raise SystemExit(f"--tiles: {len(pinned)} tile sets for {len(doms)} domains")
     t0 = time.perf_counter()
     for i, (n, lm, gm, h) in enumerate(doms):
         op = MG.MGBF(MG.Config.for_grid(n, n, lm, gm=gm, h=h, hz=3, weights=weights))
@@ -250,10 +255,14 @@ def run_group(args, torch, MG):
                      for g in range(1, op.dims.gm + 1)])
         for g, a in enumerate(amps[-1], start=1):
             op.set_weights(g, field=a)
-        if not args.no_tune:
+        if pinned is not None:
+            op.set_tiles({(int(k[1:k.index("k")]), int(k[k.index("k") + 1:])): tuple(v)
+                          for k, v in pinned[i].items()})
+        elif not args.no_tune:
             op.autotune()
         ops.append(op)
-    log(f"[bench] group: contexts and autotune {time.perf_counter() - t0:.1f} s")
+    log(f"[bench] group: contexts and {'pinned tiles' if pinned is not None else 'autotune'} "
+        f"{time.perf_counter() - t0:.1f} s")
     units = sum(op.dims.nx * op.dims.ny * op.dims.km_all for op in ops)
     # algorithmic bytes: the two-sweep minimum of each domain plus one read
     # of every generation's weight field
@@ -288,7 +297,7 @@ def run_group(args, torch, MG):
         "data": "synthetic: per-domain N(0,1) (seed 500+i), amplitude fields 0.5+U(0,1)",
         "config": {"workload": f"uneven batch: {len(doms)} regional domains (n, levels, generations, "
                                f"half-width) {doms}, per-point amplitude fields on every generation, "
-                               f"autotuned tiles, one hipGraph"},
+                               f"{'pinned' if pinned is not None else 'autotuned'} tiles, one hipGraph"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                      "kernel": "the whole graph (two-sweep minimum bytes of every domain plus one "
@@ -296,6 +305,7 @@ def run_group(args, torch, MG):
         "cpu_baseline": None,
         "eager_ms_per_step": round(ms_eager, 4),
         "graph_over_eager": round(ms_eager / ms_graph, 3),
+        "tiles": [{f"g{g}k{k}": list(v) for (g, k), v in op.tiles().items()} for op in ops],
     }
     grp.close()
     for op in ops:
@@ -472,7 +482,7 @@ def main():
                     help="autotune the radial tiles instead of pinning those of the traffic profile")
     ap.add_argument("--tiles", default=None,
                     help='pin radial tiles instead of autotuning, JSON as printed under "tiles", '
-                         'e.g. \'{"g1k0": [4, 384], ...}\'')
+                         'e.g. \'{"g1k0": [4, 384], ...}\' (for --row group a list of such, one per domain)')
     ap.add_argument("--no-cpu", action="store_true", help="skip the FP64 CPU leg (error check and baseline)")
     ap.add_argument("--km2", type=int, default=0,
                     help="2-D variables after the 3-D ones in the background rows proc1..proc5")
