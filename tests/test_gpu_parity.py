@@ -337,6 +337,63 @@ def test_w_and_wt_match_oracle_and_pass_the_dot_product_test():
         assert (y1 - y2).abs().max().item() <= 1e-5 * y1.abs().max().item()
 
 
+def seeded_configuration(seed):
+    """A radial configuration drawn from a seed: background or ensemble
+    procedure, 2-4 generations, radial half-width 3-8, vertical 1-4, p = 1
+    or 2, 8-64 levels, 1-3 members, either transfer stencil, l_filt_g1 on or
+    off, random generation weights; grids of 64-160 points a side (every
+    generation at least as wide as the halo)."""
+    rng = np.random.default_rng(seed)
+    proc = int(rng.choice([1, 6]))
+    gm = int(rng.integers(2, 5))
+    h = int(rng.integers(3, 9))
+    hz = int(rng.integers(1, 5))
+    p = int(rng.choice([1, 2]))
+    lm = int(rng.choice([8, 16, 32, 64]))
+    lm = max(lm, 2 * hz + 2)
+    n_ens = 1 if proc == 1 else int(rng.integers(1, 4))
+    pe = {2: 2, 3: 4, 4: 8}[gm]
+    unit = pe * 2 ** max(0, gm - 1 - int(np.log2(pe)))
+    nx = unit * int(np.ceil(64 / unit)) + unit * int(rng.integers(0, 3))
+    ny = unit * int(np.ceil(64 / unit)) + unit * int(rng.integers(0, 3))
+    weights = [float(w) for w in rng.uniform(0.1, 1.0, 4)]
+    kw = dict(mgbf_proc=proc, lquart=bool(rng.integers(0, 2)))
+    if proc == 6:
+        kw["l_filt_g1"] = bool(rng.integers(0, 2))
+    cfg = M.Config.for_grid(nx, ny, lm, n_ens=n_ens, gm=gm, h=h, hz=hz, p=p, weights=weights, **kw)
+    return cfg, bool(rng.integers(0, 2))
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_seeded_configurations_match_oracle(seed):
+    # combinations the named cases do not pin one by one, each with the
+    # default tiles or the autotuner's: W A W^T, W and W^T against the
+    # oracle, and the W / W^T dot-product test
+    cfg, tune = seeded_configuration(1000 + seed)
+    print(f"seeded configuration {seed}: proc {cfg.mgbf_proc}, {cfg.nm0}x{cfg.mm0}x{cfg.lm} x{cfg.n_ens}, "
+          f"gm {cfg.gm_max}, h {cfg.hx}, hz {cfg.hz}, p {cfg.p}, lquart {cfg.lquart}, "
+          f"l_filt_g1 {getattr(cfg, 'l_filt_g1', True)}, tuned {tune}", flush=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        if tune:
+            op.autotune()
+        x = field(d.gen_shape(1), 2000 + seed)
+        v = np.random.default_rng(3000 + seed).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.wawt(P, x64)) <= TOL_OP
+        assert relerr(wt, O.wt(P, x64)) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs, rhs = op.dot(w, xd), op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
+
+
 @pytest.mark.parametrize("case", ["ens", "ens_band", "ens_filt_g1_off", "bckg", "fast", "fuse0", "tuned"])
 def test_lquart_procedures_match_oracle(monkeypatch, case):
     # upsending2_ens / downsending2_ens: the quadratic transfer between
