@@ -394,6 +394,69 @@ def test_seeded_configurations_match_oracle(seed):
         assert abs(lhs - rhs) <= 1e-6 * scale
 
 
+def seeded_other_procedure(seed):
+    """A line, fast or 3-D configuration drawn from a seed: procedures 2, 3,
+    7, 8 (half-width 6, mg_ampl02 from the values the line and fast factors
+    allow at that halo, triad tables on or off) and 4, 5 (half-width 4,
+    3-D variables with or without 2-D ones), 2-3 generations, either
+    transfer stencil, random generation weights."""
+    rng = np.random.default_rng(seed)
+    proc = int(rng.choice([2, 3, 4, 5, 7, 8]))
+    gm = int(rng.integers(2, 4))
+    nx = 64 + 4 * int(rng.integers(0, 3))
+    ny = 64 + 4 * int(rng.integers(0, 3))
+    weights = [float(w) for w in rng.uniform(0.1, 1.0, 3)]
+    kw = dict(mgbf_proc=proc, lquart=bool(rng.integers(0, 2)), weights=weights)
+    if proc in (4, 5):
+        lm = int(rng.choice([8, 12, 16]))
+        hz = int(rng.choice([2, 3]))
+        kw.update(km3=int(rng.integers(1, 4)), km2=int(rng.integers(0, 4)))
+        if proc == 5:
+            kw["mgbf_line"] = bool(rng.integers(0, 2))
+        cfg = M.Config.for_grid(nx, ny, lm, gm=gm, h=4, hz=hz, **kw)
+        return cfg
+    lm = int(rng.choice([8, 16]))
+    hz = int(rng.choice([2, 3]))
+    if proc in (2, 3):
+        kw.update(km3=int(rng.integers(1, 3)), km2=int(rng.integers(0, 4)))
+        n_ens = 1
+    else:
+        n_ens = int(rng.integers(1, 3))
+        kw["l_filt_g1"] = bool(rng.integers(0, 2))
+    if proc in (2, 7):
+        kw["mgbf_line"] = bool(rng.integers(0, 2))
+    cfg = M.Config.for_grid(nx, ny, lm, n_ens=n_ens, gm=gm, h=6, hz=hz, **kw)
+    cfg.mg_ampl02 = float(rng.choice([0.5, 0.8, 2.0] if proc in (2, 7) else [0.45, 1.0, 2.0]))
+    return cfg
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_seeded_other_procedures_match_oracle(seed):
+    # the line, fast and 3-D procedures on drawn configurations: W A W^T, W
+    # and W^T against the oracle, and the W / W^T dot-product test
+    cfg = seeded_other_procedure(4000 + seed)
+    print(f"seeded procedure {seed}: proc {cfg.mgbf_proc}, {cfg.nm0}x{cfg.mm0}x{cfg.lm} x{cfg.n_ens}, "
+          f"km3 {cfg.km3} km2 {cfg.km2}, gm {cfg.gm_max}, hz {cfg.hz}, ampl02 {cfg.mg_ampl02:.2f}, "
+          f"line {cfg.mgbf_line}, lquart {cfg.lquart}", flush=True)
+    with M.MGBF(cfg) as op:
+        d = op.dims
+        P = oparams(cfg, d)
+        x = field(d.gen_shape(1), 5000 + seed)
+        v = np.random.default_rng(6000 + seed).standard_normal(d.mg_size).astype(np.float32)
+        xd, vd = to_dev(x), to_dev(v)
+        y = op.filtering_procedure(xd)
+        wt = op.apply(M.OP_WT, xd)
+        w = op.apply(M.OP_W, vd)
+        sync()
+        x64 = x.astype(np.float64)
+        assert relerr(y, O.wawt(P, x64)) <= TOL_OP
+        assert relerr(wt, O.wt(P, x64)) <= TOL_OP
+        assert relerr(w, O.w(P, v.astype(np.float64))) <= TOL_OP
+        lhs, rhs = op.dot(w, xd), op.dot(vd, wt)
+        scale = float(torch.linalg.vector_norm(w.double())) * float(torch.linalg.vector_norm(xd.double()))
+        assert abs(lhs - rhs) <= 1e-6 * scale
+
+
 @pytest.mark.parametrize("case", ["ens", "ens_band", "ens_filt_g1_off", "bckg", "fast", "fuse0", "tuned"])
 def test_lquart_procedures_match_oracle(monkeypatch, case):
     # upsending2_ens / downsending2_ens: the quadratic transfer between
