@@ -233,3 +233,51 @@ def test_fused_vertical_map_matches_the_split_passes(lm, n_ens, lm_a, family):
             op.close()
     assert relerr(out[False][0], out[True][0].double().numpy()) <= 1e-6
     assert relerr(out[False][1], out[True][1].double().numpy()) <= 1e-6
+
+
+def seeded_transfer(seed):
+    """A transfer configuration drawn from a seed: filter grid 32-64 points
+    a side, analysis grid 1-2 times finer per axis (ratios 1, 1.25, 1.5, 2),
+    3-8 filter levels and 1-3 members, the routine family with a vertical
+    map it accepts (at least lm analysis levels; the linear map: lm_a odd
+    and at most 2 lm - 1, or equal to lm; lwq: up to 3 lm - 1), and a
+    horizontal scheme."""
+    rng = np.random.default_rng(seed)
+    nx, ny = (int(v) for v in rng.choice([32, 48, 64], 2))
+    na = int(nx * rng.choice([1.0, 1.25, 1.5, 2.0]))
+    ma = int(ny * rng.choice([1.0, 1.25, 1.5, 2.0]))
+    lm = int(rng.integers(3, 9))
+    n_ens = int(rng.integers(1, 4))
+    family = str(rng.choice(list(FAMILIES)))
+    if family == "all_lwq":
+        lm_a = int(rng.integers(lm, 3 * lm))
+    else:
+        lm_a = int(rng.choice([lm] + [v for v in range(lm, 2 * lm, 1) if v % 2 == 1]))
+    scheme = int(rng.choice([O.LSQR, O.LIN, O.QUAD]))
+    return (nx, ny, lm, n_ens, na, ma, lm_a), family, scheme
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_seeded_transfers_match_oracle(seed):
+    # drawn grid ratios, level counts, routine families and schemes: F2A and
+    # A2F against the oracle and against each other (adjointness)
+    grid, family, scheme = seeded_transfer(7000 + seed)
+    nx, ny, lm, n_ens, na, ma, lm_a = grid
+    f2a, a2f, vmap, flags = FAMILIES[family]
+    print(f"seeded transfer {seed}: scheme={scheme} family={family} grid={grid}", flush=True)
+    cfg, op, P = setup(nx, ny, lm, n_ens, na, ma, lm_a, **SCHEMES[scheme], **flags)
+    try:
+        d = op.dims
+        rng = np.random.default_rng(8000 + seed)
+        vf = rng.standard_normal(op.field_shape()).astype(np.float32)
+        wa = rng.standard_normal(d.anal_shape()).astype(np.float32)
+        vd, wd = torch.from_numpy(vf).to(DEV), torch.from_numpy(wa).to(DEV)
+        a = op.apply(f2a, vd)
+        b = op.apply(a2f, wd)
+        torch.cuda.synchronize()
+        args = (P, na, ma, cfg.nxPE, cfg.nyPE, lm_a)
+        assert relerr(a, O.filt_to_anal(*args, vf.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
+        assert relerr(b, O.anal_to_filt(*args, wa.astype(np.float64), scheme=scheme, vmap=vmap)) <= TOL
+        assert_adjoint(op, a, wd, vd, b)
+    finally:
+        op.close()
