@@ -565,6 +565,9 @@ __global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, XferV
     constexpr int AP = 2 * kWave + 4;    // LDS pitch of an input point's levels (MODE 2)
     constexpr int LDSW = MODE == 1 ? TB * DP : (MODE == 2 ? SX * AP : 1);
     __shared__ float lds_v[kWavesPerBlock][LDSW];
+    // the tile's x weights, read by every lane at the same address (held in
+    // scalar registers they spill: TB x SX of them beside the column offsets)
+    __shared__ __align__(16) float lds_wx[kWavesPerBlock][TB * SX];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long long item = (long long)xcd_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wv;
     if (item >= nitems) return;
@@ -610,6 +613,8 @@ __global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, XferV
     const int vk = (kvalid ? k : a.K - 1) * 4;
     const int x0 = a.x0[p], y0 = a.y0[q];
     const float* WX = a.wx + (size_t)p * TB * SX;
+    for (int e = lane; e < TB * SX; e += kWave) lds_wx[wv][e] = WX[e];
+    const float* WXL = lds_wx[wv];
     const float* WY = a.wy + (size_t)q * TB * a.SY;
     float* const col = lds_v[wv];
     // levels per input point: the filter stack, or (MODE 2) the analysis one
@@ -682,11 +687,17 @@ __global__ void __launch_bounds__(kBlock) xfer_tile_kernel(XferTileArgs a, XferV
             if (r + 1 < a.SY) load_row(y0 + r + 1);
         }
         float xr[TB];
+        // the x weights again at every row (an offset the compiler cannot
+        // hoist): held in registers across the loop they would take TB x SX
+        // VGPRs
+        int wo = 0;
+        asm("" : "+s"(wo) : "s"(r));
+        const float* wl = WXL + wo;
 #pragma unroll
         for (int t = 0; t < TB; ++t) {
             float z = 0.f;
 #pragma unroll
-            for (int c = 0; c < SX; ++c) z = __builtin_fmaf(WX[t * SX + c], vc[c], z);
+            for (int c = 0; c < SX; ++c) z = __builtin_fmaf(wl[t * SX + c], vc[c], z);
             xr[t] = z;
         }
 #pragma unroll
